@@ -1,0 +1,112 @@
+/*
+ * stopeps_hip.h - C ABI of the MI355X Born-sampling path of StoPEPS.jl.
+ *
+ * The reference has no FFI: its plugin point is Julia dispatch on
+ * `RunConfig.contraction::Type{<:ContractionMethod}` (src/driver/config.jl:51),
+ * instantiated per chi role by `_boundary(c, role)` (src/driver/particles.jl:105-112)
+ * and consumed by `SamplingCache` / `sample_configuration`
+ * (src/sampling/born.jl:45-163), `lognorm` (src/contraction/boundary_mps.jl:180-186)
+ * and `project_configuration` (born.jl:27-36). A Julia `ccall` layer (or the
+ * Python ctypes layer shipped with this library) binds the functions below
+ * in place of those methods; the per-sample loops of the callers
+ * (src/driver/run.jl:273-294) become one batched call.
+ *
+ * Conventions
+ *  - Sites are indexed row-major, i = x + y*Lx (x fastest), as
+ *    `site_order` (src/states/finite.jl:16).
+ *  - A site tensor is a dense row-major FP64 array over the legs
+ *    (W, N, Pout, Paux, E, S) (src/states/finite.jl:22-26), each leg sorted by
+ *    Z2 parity with its `even` first indices of parity 0. Dense
+ *    (ComplexSpace) tensors pass even = dim on every leg.
+ *  - Host arrays are caller-owned and read/written only during a call; calls
+ *    return after the device work finished.
+ *  - Return value: 0 on success, negative on an API/HIP error (message via
+ *    spp_last_error). Numerical conditions are reported per sample in
+ *    `status` (0 ok, 1 invalid Born conditional, 2 zero conditional mass,
+ *    3 nonfinite environment), so the caller can raise the reference's
+ *    messages (born.jl:137-142, boundary_mps.jl:131,258).
+ *  - One context = one HIP stream; a context is not re-entrant. Separate
+ *    contexts may be used from separate threads.
+ */
+#ifndef STOPEPS_HIP_H
+#define STOPEPS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct spp_ctx spp_ctx;
+typedef struct spp_state spp_state;
+typedef struct spp_cache spp_cache;
+
+/* Lifecycle. No reference counterpart (Julia owns its process state). */
+int spp_create(int device, spp_ctx** out);
+int spp_destroy(spp_ctx* ctx);
+const char* spp_last_error(const spp_ctx* ctx);
+int spp_synchronize(spp_ctx* ctx);
+
+/* Arithmetic of the contraction GEMMs: 0 = FP64 everywhere, 1 = FP32 MFMA
+ * contractions with FP64 eigensolvers, log weights and probabilities.
+ * Takes effect for caches built afterwards. */
+int spp_set_precision(spp_ctx* ctx, int precision);
+
+/* Tuning: "ksub" (subspace size of the truncation eigensolver, <= 64),
+ * "max_subspace_iter", "subspace_tol", "chunk" (samples per device batch,
+ * 0 = sized from free memory). */
+int spp_set_option(spp_ctx* ctx, const char* name, double value);
+
+/* counters[0] = contraction flops issued (2*M*N*K over parity blocks),
+ * counters[1] = contraction launches, counters[2] = subspace iterations of the
+ * last truncation, counters[3] = peak workspace bytes. Returns the count written. */
+int spp_get_counters(spp_ctx* ctx, double* counters, int n);
+int spp_reset_counters(spp_ctx* ctx);
+
+/* Upload a FinitePEPO (kind 0) or FinitePEPS (kind 1)
+ * (`FiniteState`, src/states/finite.jl:27-44). dims/evens: [Lx*Ly][6];
+ * data: Lx*Ly pointers to dense site arrays. */
+int spp_state_upload(spp_ctx* ctx, int Lx, int Ly, int kind, const int* dims, const int* evens,
+                     const double* const* data, double beta, double log_amplitude, spp_state** out);
+int spp_state_free(spp_state* state);
+
+/* `SamplingCache(state, BoundaryMPS(chi); kind=:auxiliary, basis)`
+ * (src/sampling/born.jl:56-73): double layers, aux-projected candidate layers
+ * and all south boundary environments. `basis` is the 2x2 row-major matrix
+ * U[a][k] whose columns are the local basis vectors (NULL: identity, i.e.
+ * `SymmetryBasis()` on Z2Space(0=>1,1=>1); XBasis for dense TFIM). */
+int spp_sampling_cache_build(spp_ctx* ctx, const spp_state* pepo, int chi, const double* basis,
+                             spp_cache** out);
+int spp_sampling_cache_free(spp_cache* cache);
+/* south_max_trunc[Ly+1]: `max_truncation` of every south boundary. */
+int spp_sampling_cache_info(const spp_cache* cache, double* south_max_trunc);
+
+/* Batched `sample_configuration(cache; rng)` (src/sampling/born.jl:86-163),
+ * auxiliary kind, for n samples at once.
+ *  uniforms   [n][Lx*Ly]  the one uniform the reference draws per site
+ *                          (`rand(rng)` in _categorical, selection.jl:71),
+ *                          or NULL in evaluation mode;
+ *  labels_in  [n][Lx*Ly]  0-based basis indices (evaluation mode,
+ *                          `configuration=` keyword) or NULL;
+ *  labels_out [n][Lx*Ly]  drawn 0-based basis indices;
+ *  log_prob   [n]         log probability of the configuration;
+ *  conditionals [n][Lx*Ly] drawn conditional probability per site;
+ *  max_trunc  [n]         max_environment_truncation;
+ *  tensor_log_norm [n]    `lognorm(project_configuration(...)).tensor_log_norm`
+ *                          at the sampling chi (may be NULL);
+ *  log_amplitude   [n]    log_amplitude of the projected, normalised PEPS
+ *                          (`normalize_state!`, finite.jl:89-100; may be NULL);
+ *  status [n], fail_site [n] per-sample numerical condition and site index. */
+int spp_born_sample_batch(spp_ctx* ctx, spp_cache* cache, int n, const double* uniforms, const int* labels_in,
+                          int* labels_out, double* log_prob, double* conditionals, double* max_trunc,
+                          double* tensor_log_norm, double* log_amplitude, int* status, int* fail_site);
+
+/* Plain dense C[M][N] = A[M][K] B[K][N] (row-major, FP64 in/out) through the
+ * library's contraction kernel in the given precision; a self-test of the
+ * matrix-core kernel with no reference counterpart. */
+int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, const double* A, const double* B,
+                      double* C);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STOPEPS_HIP_H */

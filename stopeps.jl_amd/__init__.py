@@ -1,0 +1,18 @@
+"""stopeps.jl_amd - MI355X-native Born-sampling path of StoPEPS.jl.
+
+The package directory name contains a dot, so load it with
+``load_package()`` from ``__graft_entry__`` or the tests (it registers the
+package as ``stopeps_jl_amd``). Compute runs in ``libstopeps_hip.so``
+(hand-written HIP for gfx950, C ABI in include/stopeps_hip.h); this package
+is the host-side mirror of the reference's Julia API for that path.
+"""
+from .states import (Lattice, Site, FiniteState, FinitePEPO, FinitePEPS, identity_pepo, parity_sorted,
+                     LocalBasis, SymmetryBasis, XBasis, resolve_basis)
+from .sampling import (Context, DeviceState, SamplingCache, BornError, sample_configuration,
+                       sample_configurations, sample_uniforms, project_configuration, splitmix64)
+from . import _lib
+
+__all__ = ["Lattice", "Site", "FiniteState", "FinitePEPO", "FinitePEPS", "identity_pepo", "parity_sorted",
+           "LocalBasis", "SymmetryBasis", "XBasis", "resolve_basis", "Context", "DeviceState",
+           "SamplingCache", "BornError", "sample_configuration", "sample_configurations", "sample_uniforms",
+           "project_configuration", "splitmix64"]

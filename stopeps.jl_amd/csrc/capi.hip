@@ -1,0 +1,261 @@
+// C ABI (include/stopeps_hip.h). Every entry point catches exceptions and
+// reports them through spp_last_error.
+#include "../../include/stopeps_hip.h"
+#include "sampler.h"
+#include <cstring>
+#include <memory>
+
+using namespace spp;
+
+struct spp_ctx {
+  Context c;
+};
+
+struct spp_state {
+  StateDev s;
+};
+
+struct spp_cache {
+  int precision = 0;
+  std::unique_ptr<CacheDev<float>> f;
+  std::unique_ptr<CacheDev<double>> d;
+};
+
+template <typename F>
+static int guarded(spp_ctx* ctx, F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->c.last_error = e.what();
+    return -1;
+  } catch (...) {
+    if (ctx) ctx->c.last_error = "unknown error";
+    return -2;
+  }
+}
+
+extern "C" {
+
+int spp_create(int device, spp_ctx** out) {
+  if (!out) return -3;
+  *out = nullptr;
+  spp_ctx* ctx = new spp_ctx();
+  int rc = guarded(ctx, [&] {
+    int n = 0;
+    SPP_CHECK(hipGetDeviceCount(&n));
+    if (n <= 0) throw HipError("no HIP device available");
+    if (device < 0 || device >= n) throw HipError("invalid device index");
+    ctx->c.device = device;
+    SPP_CHECK(hipSetDevice(device));
+    SPP_CHECK(hipStreamCreate(&ctx->c.stream));
+  });
+  if (rc != 0) {
+    static thread_local std::string err;
+    err = ctx->c.last_error;
+    delete ctx;
+    return rc;
+  }
+  *out = ctx;
+  return 0;
+}
+
+int spp_destroy(spp_ctx* ctx) {
+  if (!ctx) return 0;
+  (void)hipSetDevice(ctx->c.device);
+  if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+  return 0;
+}
+
+const char* spp_last_error(const spp_ctx* ctx) { return ctx ? ctx->c.last_error.c_str() : "null context"; }
+
+int spp_synchronize(spp_ctx* ctx) {
+  return guarded(ctx, [&] { SPP_CHECK(hipStreamSynchronize(ctx->c.stream)); });
+}
+
+int spp_set_precision(spp_ctx* ctx, int precision) {
+  return guarded(ctx, [&] {
+    if (precision != 0 && precision != 1) throw HipError("precision must be 0 (FP64) or 1 (FP32)");
+    ctx->c.precision = precision;
+  });
+}
+
+int spp_set_option(spp_ctx* ctx, const char* name, double value) {
+  return guarded(ctx, [&] {
+    std::string n = name ? name : "";
+    if (n == "ksub") {
+      if (value < 2 || value > 64) throw HipError("ksub must lie in [2, 64]");
+      ctx->c.ksub = (int)value;
+    } else if (n == "max_subspace_iter") {
+      if (value < 1) throw HipError("max_subspace_iter must be positive");
+      ctx->c.max_subspace_iter = (int)value;
+    } else if (n == "subspace_tol") {
+      if (!(value > 0)) throw HipError("subspace_tol must be positive");
+      ctx->c.subspace_tol = value;
+    } else if (n == "chunk") {
+      if (value < 0) throw HipError("chunk must be >= 0");
+      ctx->c.chunk = (long long)value;
+    } else {
+      throw HipError("unknown option " + n);
+    }
+  });
+}
+
+int spp_get_counters(spp_ctx* ctx, double* counters, int n) {
+  if (!ctx || !counters) return -3;
+  double v[4] = {ctx->c.counters.gemm_flops, ctx->c.counters.gemm_launches, (double)ctx->c.last_iterations,
+                 (double)ctx->c.arena.peak()};
+  int m = n < 4 ? n : 4;
+  for (int i = 0; i < m; ++i) counters[i] = v[i];
+  return m;
+}
+
+int spp_reset_counters(spp_ctx* ctx) {
+  if (!ctx) return -3;
+  ctx->c.counters = Counters{};
+  return 0;
+}
+
+int spp_state_upload(spp_ctx* ctx, int Lx, int Ly, int kind, const int* dims, const int* evens,
+                     const double* const* data, double beta, double log_amplitude, spp_state** out) {
+  if (out) *out = nullptr;
+  std::unique_ptr<spp_state> st(new spp_state());
+  int rc = guarded(ctx, [&] {
+    if (!out || !dims || !evens || !data) throw HipError("null argument");
+    if (Lx <= 0 || Ly <= 0) throw HipError("lattice dimensions must be positive");
+    if (kind != 0 && kind != 1) throw HipError("kind must be 0 (PEPO) or 1 (PEPS)");
+    SPP_CHECK(hipSetDevice(ctx->c.device));
+    StateDev& s = st->s;
+    s.Lx = Lx;
+    s.Ly = Ly;
+    s.kind = kind;
+    s.beta = beta;
+    s.log_amplitude = log_amplitude;
+    s.sites.resize((size_t)Lx * Ly);
+    for (int i = 0; i < Lx * Ly; ++i) {
+      SiteDev& d = s.sites[i];
+      for (int k = 0; k < 6; ++k) {
+        d.dim[k] = dims[6 * i + k];
+        d.even[k] = evens[6 * i + k];
+        if (d.dim[k] <= 0 || d.even[k] < 0 || d.even[k] > d.dim[k]) throw HipError("invalid leg description");
+      }
+      int x = i % Lx, y = i / Lx;
+      if ((x == 0 && d.dim[0] != 1) || (y == 0 && d.dim[1] != 1) || (x == Lx - 1 && d.dim[4] != 1) ||
+          (y == Ly - 1 && d.dim[5] != 1))
+        throw HipError("nontrivial OBC leg at site " + std::to_string(i));
+      if (x > 0 && (dims[6 * (i - 1) + 4] != d.dim[0] || evens[6 * (i - 1) + 4] != d.even[0]))
+        throw HipError("horizontal space mismatch");
+      if (y > 0 && (dims[6 * (i - Lx) + 5] != d.dim[1] || evens[6 * (i - Lx) + 5] != d.even[1]))
+        throw HipError("vertical space mismatch");
+      long long n = d.size();
+      SPP_CHECK(hipMalloc(&d.d, sizeof(double) * n));
+      SPP_CHECK(hipMemcpyAsync(d.d, data[i], sizeof(double) * n, hipMemcpyHostToDevice, ctx->c.stream));
+    }
+    SPP_CHECK(hipStreamSynchronize(ctx->c.stream));
+  });
+  if (rc == 0) *out = st.release();
+  return rc;
+}
+
+int spp_state_free(spp_state* state) {
+  delete state;
+  return 0;
+}
+
+int spp_sampling_cache_build(spp_ctx* ctx, const spp_state* pepo, int chi, const double* basis,
+                             spp_cache** out) {
+  if (out) *out = nullptr;
+  std::unique_ptr<spp_cache> c(new spp_cache());
+  int rc = guarded(ctx, [&] {
+    if (!pepo || !out) throw HipError("null argument");
+    if (chi < 1) throw HipError("chi must be positive");
+    if (pepo->s.kind != 0) throw HipError("the sampling cache needs a PEPO");
+    SPP_CHECK(hipSetDevice(ctx->c.device));
+    c->precision = ctx->c.precision;
+    if (c->precision == 1) {
+      c->f.reset(new CacheDev<float>());
+      build_cache<float>(ctx->c, pepo->s, chi, basis, *c->f);
+    } else {
+      c->d.reset(new CacheDev<double>());
+      build_cache<double>(ctx->c, pepo->s, chi, basis, *c->d);
+    }
+  });
+  if (rc == 0) *out = c.release();
+  return rc;
+}
+
+int spp_sampling_cache_free(spp_cache* cache) {
+  delete cache;
+  return 0;
+}
+
+int spp_sampling_cache_info(const spp_cache* cache, double* south_max_trunc) {
+  if (!cache || !south_max_trunc) return -3;
+  const std::vector<double>& v = cache->precision == 1 ? cache->f->south_max_trunc : cache->d->south_max_trunc;
+  for (size_t i = 0; i < v.size(); ++i) south_max_trunc[i] = v[i];
+  return 0;
+}
+
+int spp_born_sample_batch(spp_ctx* ctx, spp_cache* cache, int n, const double* uniforms, const int* labels_in,
+                          int* labels_out, double* log_prob, double* conditionals, double* max_trunc,
+                          double* tensor_log_norm, double* log_amplitude, int* status, int* fail_site) {
+  return guarded(ctx, [&] {
+    if (!cache) throw HipError("null cache");
+    if (n < 0) throw HipError("negative sample count");
+    if (!uniforms && !labels_in) throw HipError("an explicit RNG is required");
+    SPP_CHECK(hipSetDevice(ctx->c.device));
+    DrawOutputs o;
+    o.labels = labels_out;
+    o.log_prob = log_prob;
+    o.conditionals = conditionals;
+    o.max_trunc = max_trunc;
+    o.tensor_log_norm = tensor_log_norm;
+    o.log_amplitude = log_amplitude;
+    o.status = status;
+    o.fail_site = fail_site;
+    if (cache->precision == 1) draw_batch<float>(ctx->c, *cache->f, n, uniforms, labels_in, o);
+    else draw_batch<double>(ctx->c, *cache->d, n, uniforms, labels_in, o);
+  });
+}
+
+int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, const double* A, const double* B,
+                      double* C) {
+  return guarded(ctx, [&] {
+    if (M <= 0 || N <= 0 || K <= 0) throw HipError("dimensions must be positive");
+    hipStream_t st = ctx->c.stream;
+    size_t es = precision == 1 ? sizeof(float) : sizeof(double);
+    void *dA, *dB, *dC, *tA = nullptr, *tB = nullptr;
+    SPP_CHECK(hipMalloc(&dA, sizeof(double) * M * K));
+    SPP_CHECK(hipMalloc(&dB, sizeof(double) * K * N));
+    SPP_CHECK(hipMalloc(&dC, sizeof(double) * M * N));
+    SPP_CHECK(hipMemcpyAsync(dA, A, sizeof(double) * M * K, hipMemcpyHostToDevice, st));
+    SPP_CHECK(hipMemcpyAsync(dB, B, sizeof(double) * K * N, hipMemcpyHostToDevice, st));
+    Leg lm = fixed_leg(M, M), ln = fixed_leg(N, N), lk = fixed_leg(K, K);
+    if (precision == 1) {
+      SPP_CHECK(hipMalloc(&tA, es * M * K));
+      SPP_CHECK(hipMalloc(&tB, es * K * N));
+      void* tC;
+      SPP_CHECK(hipMalloc(&tC, es * M * N));
+      SPP_CHECK((launch_convert<float, double>((float*)tA, (const double*)dA, (long long)M * K, st)));
+      SPP_CHECK((launch_convert<float, double>((float*)tB, (const double*)dB, (long long)K * N, st)));
+      contract<float>(ctx->c, dense_ref(tC, 0, "mn", {lm, ln}), dense_ref(tA, 0, "mk", {lm, lk}),
+                      dense_ref(tB, 0, "kn", {lk, ln}), 1);
+      SPP_CHECK((launch_convert<double, float>((double*)dC, (const float*)tC, (long long)M * N, st)));
+      SPP_CHECK(hipStreamSynchronize(st));
+      (void)hipFree(tC);
+    } else {
+      contract<double>(ctx->c, dense_ref(dC, 0, "mn", {lm, ln}), dense_ref(dA, 0, "mk", {lm, lk}),
+                       dense_ref(dB, 0, "kn", {lk, ln}), 1);
+    }
+    SPP_CHECK(hipMemcpyAsync(C, dC, sizeof(double) * M * N, hipMemcpyDeviceToHost, st));
+    SPP_CHECK(hipStreamSynchronize(st));
+    (void)hipFree(dA);
+    (void)hipFree(dB);
+    (void)hipFree(dC);
+    if (tA) (void)hipFree(tA);
+    if (tB) (void)hipFree(tB);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,168 @@
+// Host-side engine of the MI355X Born-sampling path.
+//
+// Device data model: every tensor is dense per sample with parity-sorted legs
+// (even indices first). Double-layer legs pair (ket, bra) in the parity-sorted
+// pair order of split_pair(). Boundary bonds carry a per-sample even count.
+// South boundaries are stored rotated by 180 degrees so that one forward
+// compression routine serves both directions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include <memory>
+#include <stdexcept>
+#include "z2gemm.h"
+#include "kernels.h"
+
+namespace spp {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define SPP_CHECK(expr)                                                                       \
+  do {                                                                                        \
+    hipError_t e__ = (expr);                                                                  \
+    if (e__ != hipSuccess)                                                                    \
+      throw ::spp::HipError(std::string(#expr) + ": " + hipGetErrorString(e__) + " at " + \
+                            __FILE__ + ":" + std::to_string(__LINE__));                       \
+  } while (0)
+
+// Device arena with stack-like release.
+class Arena {
+ public:
+  ~Arena();
+  void reserve(size_t bytes);
+  void* take(size_t bytes);
+  size_t mark() const { return top_; }
+  void release(size_t m) { top_ = m; }
+  size_t capacity() const { return cap_; }
+  size_t peak() const { return peak_; }
+
+ private:
+  char* base_ = nullptr;
+  size_t cap_ = 0, top_ = 0, peak_ = 0;
+};
+
+struct Leg {
+  int dim = 1;
+  int even = 1;              // fixed even count (when split == nullptr)
+  const int* split = nullptr;  // per-sample even counts (device)
+  bool flat = false;         // ungraded leg (subspace columns)
+};
+
+inline Leg fixed_leg(int dim, int even) {
+  Leg l;
+  l.dim = dim;
+  l.even = even;
+  return l;
+}
+inline Leg flat_leg(int dim) {
+  Leg l;
+  l.dim = dim;
+  l.even = dim;
+  l.flat = true;
+  return l;
+}
+
+// A batched operand: dense (strides from dims) or compact per-parity block matrix.
+struct TRef {
+  void* base = nullptr;
+  long long sstride = 0;
+  void* const* ptrs = nullptr;
+  long long pstride = 0;
+  std::string letters;
+  std::vector<long long> stride;
+  std::vector<Leg> leg;
+  bool compact = false;
+  int nrow = 0;       // compact: number of row letters
+  long long ld = 0;   // compact: leading dimension
+};
+
+TRef dense_ref(void* base, long long sstride, const std::string& letters, const std::vector<Leg>& legs);
+TRef compact_ref(void* base, long long sstride, long long pstride, long long ld, const std::string& letters,
+                 int nrow, const std::vector<Leg>& legs);
+TRef with_letters(const TRef& t, const std::string& letters);
+TRef with_ptrs(const TRef& t, void* const* ptrs);
+// Permute a dense TRef's letters without moving data (e.g. read a south tensor in reversed order).
+TRef relabel(const TRef& t, const std::string& new_letters_in_storage_order);
+
+struct Counters {
+  double gemm_flops = 0;  // 2*M*N*K summed over launched (sample, parity) problems (upper bound)
+  double gemm_launches = 0;
+  double kernel_ms = 0;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int precision = 0;        // 0: FP64, 1: FP32 contraction GEMMs (FP64 factorizations)
+  int ksub = 64;            // subspace size for truncation eigenproblems
+  int max_subspace_iter = 40;
+  double subspace_tol = 1e-12;
+  Arena arena;
+  Counters counters;
+  std::string last_error;
+  long long chunk = 0;      // samples per internal chunk (0: automatic)
+  int last_iterations = 0;
+};
+
+// Contraction C = alpha * A.B (+ C): letters shared by A and B are summed.
+template <typename T>
+void contract(Context& ctx, const TRef& C, const TRef& A, const TRef& B, int nsamples, double alpha = 1.0,
+              bool accumulate = false, const int* pmask = nullptr);
+
+FullView full_view(const TRef& t);
+
+// ------------------------------ boundary MPS -------------------------------
+template <typename T>
+struct Boundary {
+  int L = 0;
+  int nb = 0;                     // samples
+  std::vector<int> bdim;          // size L+1
+  std::vector<int> beven;         // fixed even counts (for shared boundaries)
+  std::vector<int*> bsplit;       // per-sample even counts (device), or nullptr
+  std::vector<int> pdim, peven;   // physical leg per site
+  std::vector<T*> t;              // device tensors [nb][bdim[x]][pdim[x]][bdim[x+1]]
+  double* log_scale = nullptr;    // device [nb]
+  double* max_trunc = nullptr;    // device [nb]
+  bool owns = true;
+  Leg bond(int b) const {
+    Leg l;
+    l.dim = bdim[b];
+    l.even = beven[b];
+    l.split = bsplit[b];
+    return l;
+  }
+  Leg phys(int x) const { return fixed_leg(pdim[x], peven[x]); }
+  long long tsize(int x) const { return (long long)bdim[x] * pdim[x] * bdim[x + 1]; }
+  TRef ref(int x, const std::string& letters) const {
+    return dense_ref(t[x], tsize(x), letters, {bond(x), phys(x), bond(x + 1)});
+  }
+  void release();
+};
+
+// Double layer row of a lattice (shared, batch 1 unless per-sample pointer tables are given).
+struct LayerSpec {
+  Leg w, n, e, s;
+};
+
+template <typename T>
+struct RowLayers {
+  int L = 0;
+  std::vector<LayerSpec> spec;            // per site
+  std::vector<void* const*> ptrs;         // per site: per-sample pointer tables (device) or nullptr
+  std::vector<T*> shared;                 // per site: shared layer when ptrs == nullptr
+  bool rotated = false;                   // read as E_rot[w,n,e,s] = E[e,s,w,n]
+  // Reference to site x with the given letters for (W,N,E,S) in the frame of this row.
+  TRef ref(int x, const std::string& letters) const;
+};
+
+template <typename T>
+void trivial_boundary(Context& ctx, Boundary<T>& b, int L, int nb, const std::vector<Leg>& phys);
+
+// Forward (north-type) absorb of a row of layers into env (in place result in out).
+template <typename T>
+void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& row, int chi, Boundary<T>& out);
+
+}  // namespace spp

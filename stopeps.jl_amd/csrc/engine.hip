@@ -1,0 +1,856 @@
+// Engine: contraction builder and the batched row absorb with canonical
+// compression.
+//
+// Compression (restates compress_boundary!, src/contraction/boundary_mps.jl:94-134,
+// result-equivalently): the reference right-orthogonalises the absorbed MPS by
+// LQ and then truncates left to right by SVD. The singular values and left
+// singular vectors at cut x are those of rho_x = Y_x G_{x+1} Y_x^T, where Y_x
+// is the left-truncated tensor at site x and G_{x+1} the exact Gram matrix of
+// the absorbed MPS to the right of the cut (L_{x+1} L_{x+1}^T). So the kernel
+// sequence is: right Grams by four contractions per site, then per cut rho_x,
+// its top-chi eigenvectors over both parity blocks, U_x and R_x = U_x^T Y_x.
+#include "engine.h"
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+
+namespace spp {
+
+// ------------------------------- arena --------------------------------------
+Arena::~Arena() {
+  if (base_) (void)hipFree(base_);
+}
+
+void Arena::reserve(size_t bytes) {
+  if (bytes <= cap_) return;
+  if (top_ != 0) throw HipError("arena resize while in use");
+  if (base_) SPP_CHECK(hipFree(base_));
+  base_ = nullptr;
+  SPP_CHECK(hipMalloc(&base_, bytes));
+  cap_ = bytes;
+}
+
+void* Arena::take(size_t bytes) {
+  size_t aligned = (bytes + 255) & ~size_t(255);
+  if (top_ + aligned > cap_)
+    throw HipError("device workspace exhausted: need " + std::to_string(top_ + aligned) + " bytes, have " +
+                   std::to_string(cap_));
+  void* p = base_ + top_;
+  top_ += aligned;
+  peak_ = std::max(peak_, top_);
+  return p;
+}
+
+// ------------------------------- TRef ---------------------------------------
+TRef dense_ref(void* base, long long sstride, const std::string& letters, const std::vector<Leg>& legs) {
+  if (letters.size() != legs.size()) throw HipError("dense_ref: letters/legs mismatch for " + letters);
+  TRef t;
+  t.base = base;
+  t.sstride = sstride;
+  t.letters = letters;
+  t.leg = legs;
+  t.stride.assign(legs.size(), 1);
+  for (int k = (int)legs.size() - 2; k >= 0; --k) t.stride[k] = t.stride[k + 1] * legs[k + 1].dim;
+  return t;
+}
+
+TRef compact_ref(void* base, long long sstride, long long pstride, long long ld, const std::string& letters,
+                 int nrow, const std::vector<Leg>& legs) {
+  TRef t;
+  t.base = base;
+  t.sstride = sstride;
+  t.pstride = pstride;
+  t.letters = letters;
+  t.leg = legs;
+  t.stride.assign(legs.size(), 0);
+  t.compact = true;
+  t.nrow = nrow;
+  t.ld = ld;
+  return t;
+}
+
+TRef with_letters(const TRef& t, const std::string& letters) {
+  if (letters.size() != t.letters.size()) throw HipError("with_letters: rank mismatch");
+  TRef r = t;
+  r.letters = letters;
+  return r;
+}
+
+TRef with_ptrs(const TRef& t, void* const* ptrs) {
+  TRef r = t;
+  r.ptrs = ptrs;
+  return r;
+}
+
+TRef relabel(const TRef& t, const std::string& new_letters) { return with_letters(t, new_letters); }
+
+static long long group_bound(const std::vector<Leg>& legs) {
+  long long p = 1;
+  bool flat = true, evensplit = false;
+  for (auto& l : legs) {
+    p *= l.dim;
+    if (!l.flat) flat = false;
+    if (!l.flat && l.split == nullptr && 2 * l.even == l.dim) evensplit = true;
+  }
+  if (flat || legs.empty()) return p;
+  return evensplit ? p / 2 : p;
+}
+
+static double group_estimate(const std::vector<Leg>& legs, int q) {
+  // tuple count with per-sample splits replaced by an even split
+  double c[2] = {1.0, 0.0};
+  bool flat = !legs.empty();
+  for (auto& l : legs)
+    if (!l.flat) flat = false;
+  if (flat) {
+    double p = 1;
+    for (auto& l : legs) p *= l.dim;
+    return p;
+  }
+  for (auto& l : legs) {
+    double e = l.split ? 0.5 * l.dim : l.even;
+    double o = l.dim - e;
+    double n0 = c[0] * e + c[1] * o, n1 = c[0] * o + c[1] * e;
+    c[0] = n0;
+    c[1] = n1;
+  }
+  return c[q];
+}
+
+static void fill_group(Group& g, const std::vector<Leg>& legs, const std::string& who) {
+  if ((int)legs.size() > kMaxLegs) throw HipError("too many legs in group " + who);
+  g.n = (int)legs.size();
+  bool anyflat = false, anygraded = false;
+  for (int j = 0; j < g.n; ++j) {
+    g.dim[j] = legs[j].dim;
+    g.even[j] = legs[j].even;
+    g.split[j] = legs[j].split;
+    if (legs[j].flat) anyflat = true;
+    else anygraded = true;
+  }
+  if (anyflat && anygraded) throw HipError("mixed flat/graded group " + who);
+  g.graded = anyflat ? 0 : 1;
+}
+
+static void fill_operand_group(OperandGroup& og, const TRef& t, const std::string& letters, int role_group,
+                               const std::string& who) {
+  if (t.compact) {
+    std::string rows = t.letters.substr(0, t.nrow), cols = t.letters.substr(t.nrow);
+    og.linear = 1;
+    if (letters == rows) og.lstride = t.ld;
+    else if (letters == cols) og.lstride = 1;
+    else
+      throw HipError("compact operand " + t.letters + " cannot serve group '" + letters + "' (" + who + ")");
+    (void)role_group;
+    return;
+  }
+  og.linear = 0;
+  og.lstride = 0;
+  for (size_t j = 0; j < letters.size(); ++j) {
+    auto pos = t.letters.find(letters[j]);
+    if (pos == std::string::npos) throw HipError("letter missing in operand (" + who + ")");
+    og.stride[j] = t.stride[pos];
+  }
+}
+
+static void fill_operand(Operand& op, const TRef& t) {
+  op.base = t.base;
+  op.sstride = t.sstride;
+  op.ptrs = t.ptrs;
+  op.pstride = t.pstride;
+}
+
+static std::vector<Leg> legs_of(const TRef& t, const std::string& letters) {
+  std::vector<Leg> out;
+  for (char c : letters) out.push_back(t.leg[t.letters.find(c)]);
+  return out;
+}
+
+template <typename T>
+void contract(Context& ctx, const TRef& C, const TRef& A, const TRef& B, int nsamples, double alpha,
+              bool accumulate, const int* pmask) {
+  std::string M, N, K;
+  auto has = [](const std::string& s, char c) { return s.find(c) != std::string::npos; };
+  for (char c : C.letters) {
+    bool a = has(A.letters, c), b = has(B.letters, c);
+    if (a && !b) M += c;
+    else if (b && !a) N += c;
+    else throw HipError("contract: output letter " + std::string(1, c) + " must appear in exactly one input (" +
+                        C.letters + "=" + A.letters + "*" + B.letters + ")");
+  }
+  for (char c : A.letters)
+    if (!has(C.letters, c)) {
+      if (!has(B.letters, c)) throw HipError("contract: dangling letter in A");
+      K += c;
+    }
+  for (char c : B.letters)
+    if (!has(C.letters, c) && !has(A.letters, c)) throw HipError("contract: dangling letter in B");
+  // K order follows A, except that compact A or B dictate it
+  if (B.compact && !A.compact) {
+    std::string rows = B.letters.substr(0, B.nrow);
+    if (rows.size() == K.size()) K = rows;
+  }
+  std::string who = C.letters + "=" + A.letters + "*" + B.letters;
+  GemmArgs g{};
+  auto lm = legs_of(A, M), ln = legs_of(B, N), lk = legs_of(A, K);
+  fill_group(g.gm, lm, who);
+  fill_group(g.gn, ln, who);
+  fill_group(g.gk, lk, who);
+  fill_operand(g.a, A);
+  fill_operand(g.b, B);
+  fill_operand(g.c, C);
+  fill_operand_group(g.a.g[0], A, M, 0, who);
+  fill_operand_group(g.a.g[1], A, K, 1, who);
+  fill_operand_group(g.b.g[0], B, K, 0, who);
+  fill_operand_group(g.b.g[1], B, N, 1, who);
+  fill_operand_group(g.c.g[0], C, M, 0, who);
+  fill_operand_group(g.c.g[1], C, N, 1, who);
+  g.nsamples = nsamples;
+  g.sample_map = nullptr;
+  g.pmask = pmask;
+  g.alpha = alpha;
+  g.accumulate = accumulate ? 1 : 0;
+  long long mmax = group_bound(lm), nmax = group_bound(ln);
+  double fl = 0;
+  for (int q = 0; q < 2; ++q) fl += group_estimate(lm, q) * group_estimate(ln, q) * group_estimate(lk, q);
+  ctx.counters.gemm_flops += 2.0 * fl * nsamples;
+  ctx.counters.gemm_launches += 1;
+  SPP_CHECK(launch_z2gemm<T>(g, mmax, nmax, ctx.stream));
+}
+
+template void contract<float>(Context&, const TRef&, const TRef&, const TRef&, int, double, bool, const int*);
+template void contract<double>(Context&, const TRef&, const TRef&, const TRef&, int, double, bool, const int*);
+
+FullView full_view(const TRef& t) {
+  if (t.compact) throw HipError("full_view of a compact operand");
+  FullView v{};
+  fill_group(v.g, t.leg, "full_view");
+  for (size_t j = 0; j < t.leg.size(); ++j) v.stride[j] = t.stride[j];
+  v.base = t.base;
+  v.sstride = t.sstride;
+  v.ptrs = t.ptrs;
+  return v;
+}
+
+// ------------------------------ boundaries ---------------------------------
+template <typename T>
+void Boundary<T>::release() {
+  if (owns) {
+    for (auto p : t)
+      if (p) (void)hipFree(p);
+    for (auto p : bsplit)
+      if (p) (void)hipFree(p);
+    if (log_scale) (void)hipFree(log_scale);
+    if (max_trunc) (void)hipFree(max_trunc);
+  }
+  t.clear();
+  bsplit.clear();
+  log_scale = max_trunc = nullptr;
+}
+template struct Boundary<float>;
+template struct Boundary<double>;
+
+template <typename T>
+TRef RowLayers<T>::ref(int x, const std::string& letters) const {
+  const LayerSpec& sp = spec[x];
+  // spec is given in this row's frame; storage of a rotated row is E[e,s,w,n]
+  TRef r;
+  if (!rotated) {
+    r = dense_ref(shared.empty() ? nullptr : shared[x], 0, letters, {sp.w, sp.n, sp.e, sp.s});
+  } else {
+    std::string st = {letters[2], letters[3], letters[0], letters[1]};
+    r = dense_ref(shared.empty() ? nullptr : shared[x], 0, st, {sp.e, sp.s, sp.w, sp.n});
+  }
+  if (!ptrs.empty() && ptrs[x]) r.ptrs = ptrs[x];
+  return r;
+}
+template struct RowLayers<float>;
+template struct RowLayers<double>;
+
+template <typename T>
+void trivial_boundary(Context& ctx, Boundary<T>& b, int L, int nb, const std::vector<Leg>& phys) {
+  b.L = L;
+  b.nb = nb;
+  b.bdim.assign(L + 1, 1);
+  b.beven.assign(L + 1, 1);
+  b.bsplit.assign(L + 1, nullptr);
+  b.pdim.resize(L);
+  b.peven.resize(L);
+  b.t.assign(L, nullptr);
+  for (int x = 0; x < L; ++x) {
+    if (phys[x].dim != 1) throw HipError("nontrivial OBC leg");
+    b.pdim[x] = 1;
+    b.peven[x] = phys[x].even;
+    SPP_CHECK(hipMalloc(&b.t[x], sizeof(T) * nb));
+    SPP_CHECK(launch_fill<T>(b.t[x], nb, T(1), ctx.stream));
+  }
+  SPP_CHECK(hipMalloc(&b.log_scale, sizeof(double) * nb));
+  SPP_CHECK(hipMalloc(&b.max_trunc, sizeof(double) * nb));
+  SPP_CHECK(launch_fill<double>(b.log_scale, nb, 0.0, ctx.stream));
+  SPP_CHECK(launch_fill<double>(b.max_trunc, nb, 0.0, ctx.stream));
+  b.owns = true;
+}
+template void trivial_boundary<float>(Context&, Boundary<float>&, int, int, const std::vector<Leg>&);
+template void trivial_boundary<double>(Context&, Boundary<double>&, int, int, const std::vector<Leg>&);
+
+// ------------------------- truncation helper kernels ------------------------
+
+// Tuple counts of a group for every (sample, parity) problem.
+__global__ void group_counts_kernel(Group g, int* out, int nprob) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nprob) return;
+  GroupView v;
+  resolve_group(g, p >> 1, v);
+  out[p] = (int)group_count(v, p & 1);
+}
+
+__device__ inline unsigned long long mix64(unsigned long long v) {
+  v += 0x9E3779B97F4A7C15ULL;
+  v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  v = (v ^ (v >> 27)) * 0x94D049BB133111EBULL;
+  return v ^ (v >> 31);
+}
+
+// Deterministic pseudo-random start block (rows >= n zero).
+__global__ void random_block_kernel(double* q, long long pstride, int ld, int ncol, const int* n_of, int maxrows) {
+  const int prob = blockIdx.y;
+  const int n = n_of[prob];
+  double* b = q + (long long)prob * pstride;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)maxrows * ncol;
+       i += (long long)gridDim.x * blockDim.x) {
+    int r = (int)(i / ncol), c = (int)(i % ncol);
+    double v = 0.0;
+    if (r < n) {
+      unsigned long long h = mix64(((unsigned long long)prob << 40) ^ ((unsigned long long)r << 20) ^ (unsigned long long)c);
+      v = ((double)(h >> 11) * 0x1.0p-53) - 0.5;
+    }
+    b[(long long)r * ld + c] = v;
+  }
+}
+
+// Thin Householder QR, explicit Q: Z (n x ncol, ld) -> Q (n x ncol, ld) with
+// orthonormal columns spanning range(Z) (LAPACK geqrf/orgqr steps, one
+// workgroup per problem, panel kept in global memory). Backward stable, so
+// small eigen-directions of the subspace survive (a Gram-matrix
+// orthonormalisation would lose those below sqrt(eps)). Z is overwritten by
+// the reflectors; tau needs ncol entries per problem.
+__global__ __launch_bounds__(256) void householder_q_kernel(double* z, double* q, long long pstride, int ld,
+                                                           int ncol, const int* n_of, double* tau_all) {
+  const int prob = blockIdx.x;
+  const int n = n_of[prob];
+  double* Z = z + (long long)prob * pstride;
+  double* Q = q + (long long)prob * pstride;
+  double* tau = tau_all + (long long)prob * ncol;
+  const int tid = threadIdx.x;
+  __shared__ double red[256];
+  __shared__ double wcol[64];
+  __shared__ double sh_tau, sh_beta, sh_scale;
+  const int kmax = min(ncol, n);
+  // ---- factorisation ----
+  for (int j = 0; j < kmax; ++j) {
+    double acc = 0.0;
+    for (int r = j + 1 + tid; r < n; r += 256) {
+      double v = Z[(long long)r * ld + j];
+      acc += v * v;
+    }
+    red[tid] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (tid < w) red[tid] += red[tid + w];
+      __syncthreads();
+    }
+    if (tid == 0) {
+      double alpha = Z[(long long)j * ld + j];
+      double sig = red[0];
+      if (sig == 0.0) {
+        sh_tau = 0.0;
+        sh_beta = alpha;
+        sh_scale = 0.0;
+      } else {
+        double nrm = sqrt(alpha * alpha + sig);
+        double beta = alpha <= 0.0 ? nrm : -nrm;
+        sh_tau = (beta - alpha) / beta;
+        sh_scale = 1.0 / (alpha - beta);
+        sh_beta = beta;
+      }
+      tau[j] = sh_tau;
+    }
+    __syncthreads();
+    const double tj = sh_tau, sc = sh_scale;
+    for (int r = j + 1 + tid; r < n; r += 256) Z[(long long)r * ld + j] *= sc;
+    __syncthreads();
+    if (tid == 0) Z[(long long)j * ld + j] = sh_beta;
+    // w_c = v^T Z[:, c] for c > j, with v_j = 1
+    const int nc = ncol - j - 1;
+    if (tj != 0.0 && nc > 0) {
+      for (int c0 = 0; c0 < nc; c0 += 64) {
+        const int c = c0 + (tid & 63);
+        const int part = tid >> 6;
+        double a2 = 0.0;
+        if (c < nc) {
+          const int cc = j + 1 + c;
+          if (part == 0) a2 = Z[(long long)j * ld + cc];
+          for (int r = j + 1 + part; r < n; r += 4) a2 += Z[(long long)r * ld + j] * Z[(long long)r * ld + cc];
+        }
+        red[tid] = a2;
+        __syncthreads();
+        if (part == 0 && c < nc) wcol[tid & 63] = red[tid] + red[tid + 64] + red[tid + 128] + red[tid + 192];
+        __syncthreads();
+        const int cw = min(64, nc - c0);
+        for (int idx = tid; idx < (n - j) * cw; idx += 256) {
+          int rr = j + idx / cw, ci = idx % cw;
+          int cc = j + 1 + c0 + ci;
+          double v = rr == j ? 1.0 : Z[(long long)rr * ld + j];
+          Z[(long long)rr * ld + cc] -= tj * v * wcol[ci];
+        }
+        __syncthreads();
+      }
+    }
+  }
+  // ---- explicit thin Q = H_0 ... H_{k-1} [I; 0] ----
+  for (int idx = tid; idx < n * ncol; idx += 256) {
+    int r = idx / ncol, c = idx % ncol;
+    Q[(long long)r * ld + c] = (r == c && c < kmax) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int j = kmax - 1; j >= 0; --j) {
+    const double tj = tau[j];
+    if (tj == 0.0) continue;
+    const int nc = ncol - j;
+    for (int c0 = 0; c0 < nc; c0 += 64) {
+      const int c = c0 + (tid & 63);
+      const int part = tid >> 6;
+      double a2 = 0.0;
+      if (c < nc) {
+        const int cc = j + c;
+        if (part == 0) a2 = Q[(long long)j * ld + cc];
+        for (int r = j + 1 + part; r < n; r += 4) a2 += Z[(long long)r * ld + j] * Q[(long long)r * ld + cc];
+      }
+      red[tid] = a2;
+      __syncthreads();
+      if (part == 0 && c < nc) wcol[tid & 63] = red[tid] + red[tid + 64] + red[tid + 128] + red[tid + 192];
+      __syncthreads();
+      const int cw = min(64, nc - c0);
+      for (int idx = tid; idx < (n - j) * cw; idx += 256) {
+        int rr = j + idx / cw, ci = idx % cw;
+        int cc = j + c0 + ci;
+        double v = rr == j ? 1.0 : Z[(long long)rr * ld + j];
+        Q[(long long)rr * ld + cc] -= tj * v * wcol[ci];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Residuals of Ritz pairs: conv[prob] = max_j<t ||Z_j - theta_j Q_j|| <= tol * |theta_0|.
+__global__ void residual_kernel(const double* q, const double* z, long long pstride, int ld, const double* th,
+                                long long tstride, const int* n_of, int t, double tol, int* conv, double* worst) {
+  const int prob = blockIdx.x;
+  const int n = n_of[prob];
+  const double* qb = q + (long long)prob * pstride;
+  const double* zb = z + (long long)prob * pstride;
+  const double* tb = th + (long long)prob * tstride;
+  __shared__ double part[256];
+  __shared__ double mx;
+  if (threadIdx.x == 0) mx = 0.0;
+  __syncthreads();
+  const int col = threadIdx.x % 64, grp = threadIdx.x / 64;
+  for (int c0 = 0; c0 < t; c0 += 64) {
+    int c = c0 + col;
+    double acc = 0.0;
+    if (c < t)
+      for (int r = grp; r < n; r += 4) {
+        double d = zb[(long long)r * ld + c] - tb[c] * qb[(long long)r * ld + c];
+        acc += d * d;
+      }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < 64 && c0 + k < t; ++k) {
+        double s = sqrt(part[k] + part[k + 64] + part[k + 128] + part[k + 192]);
+        if (s > mx) mx = s;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double scale = n > 0 ? fabs(tb[0]) : 0.0;
+    double rel = scale > 0.0 ? mx / scale : 0.0;
+    conv[prob] = (rel <= tol) ? 1 : 0;
+    worst[prob] = rel;
+  }
+}
+
+// Trace of each parity block of rho (compact, ld).
+__global__ void trace_kernel(const double* rho, long long pstride, int ld, const int* n_of, double* tr) {
+  const int prob = blockIdx.x;
+  const int n = n_of[prob];
+  const double* b = rho + (long long)prob * pstride;
+  __shared__ double part[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += b[(long long)i * ld + i];
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tr[prob] = part[0];
+}
+
+// Select the top chi eigenpairs over both parity blocks (truncrank, ties to
+// parity 0 and lower index), scatter them into U[k,p,c] (dense, zeroed
+// beforehand) with the new bond parity-sorted, and record the new even count,
+// the discarded weight fraction and its running maximum.
+template <typename T>
+__global__ void truncate_select_kernel(Group gkp, const double* w, long long wstride, const double* v,
+                                       long long vstride, int ldv, const int* avail_of, const int* n_of,
+                                       const double* tr, int chi, int cap, T* U, long long ustride,
+                                       long long ustride_k, long long ustride_p, int* new_even, double* maxtrunc,
+                                       double* discarded_out) {
+  const int s = blockIdx.x;
+  __shared__ int m0s, m1s;
+  __shared__ GroupView g;
+  if (threadIdx.x == 0) {
+    resolve_group(gkp, s, g);
+    const double* w0 = w + (long long)(2 * s) * wstride;
+    const double* w1 = w + (long long)(2 * s + 1) * wstride;
+    int a0 = min(avail_of[2 * s], n_of[2 * s]);
+    int a1 = min(avail_of[2 * s + 1], n_of[2 * s + 1]);
+    int want = min(chi, min(cap, a0 + a1));
+    int i0 = 0, i1 = 0;
+    double kept = 0.0;
+    while (i0 + i1 < want) {
+      bool take0;
+      if (i0 >= a0) take0 = false;
+      else if (i1 >= a1) take0 = true;
+      else take0 = (w0[i0] >= w1[i1]);
+      if (take0) kept += fmax(w0[i0++], 0.0);
+      else kept += fmax(w1[i1++], 0.0);
+    }
+    m0s = i0;
+    m1s = i1;
+    double total = tr[2 * s] + tr[2 * s + 1];
+    double disc = total - kept;
+    if (disc < 0.0) disc = 0.0;
+    double frac = total > 0.0 ? disc / total : 0.0;
+    if (frac > maxtrunc[s]) maxtrunc[s] = frac;
+    if (discarded_out) discarded_out[s] = frac;
+    new_even[s] = cap - i1;
+  }
+  __syncthreads();
+  const int m0 = m0s, m1 = m1s;
+  T* Ub = U + (long long)s * ustride;
+  int legs[kMaxLegs];
+  for (int q = 0; q < 2; ++q) {
+    const int n = n_of[2 * s + q];
+    const int m = q == 0 ? m0 : m1;
+    const int cbase = q == 0 ? 0 : cap - m1;
+    const double* vb = v + (long long)(2 * s + q) * vstride;
+    for (long long i = threadIdx.x; i < (long long)n * m; i += blockDim.x) {
+      int r = (int)(i / m), j = (int)(i % m);
+      decode(g, q, r, legs);
+      Ub[legs[0] * ustride_k + legs[1] * ustride_p + cbase + j] = (T)vb[(long long)r * ldv + j];
+    }
+  }
+}
+
+// log_scale[s] += log(norm), factor[s] = 1/norm (0 when the norm vanishes)
+__global__ void norm_finish_kernel(const double* nrm2, double* log_scale, double* factor, int* status, int nb) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nb) return;
+  double n = sqrt(nrm2[s]);
+  if (!isfinite(n)) {
+    if (status) status[s] = 3;  // nonfinite boundary norm
+    factor[s] = 0.0;
+    return;
+  }
+  if (n == 0.0) {
+    log_scale[s] = -INFINITY;
+    factor[s] = 0.0;
+  } else {
+    log_scale[s] += log(n);
+    factor[s] = 1.0 / n;
+  }
+}
+
+// --------------------------- truncation driver ------------------------------
+struct EigResult {
+  double* w;        // [nprob][wld]
+  long long wstride;
+  double* v;        // [nprob] compact rows x cols, ld
+  long long vstride;
+  int ldv;
+  int* avail;       // [nprob]
+};
+
+static void eig_topk(Context& ctx, const double* rho, long long rho_pstride, int rho_ld, const Leg& lk,
+                     const Leg& lp, const int* n_of, int nmax, int nb, int want, EigResult& res) {
+  const int nprob = 2 * nb;
+  hipStream_t st = ctx.stream;
+  if (nmax <= ctx.ksub) {
+    // direct: full Jacobi on each block
+    res.w = (double*)ctx.arena.take(sizeof(double) * nprob * std::max(nmax, 1));
+    res.wstride = std::max(nmax, 1);
+    res.ldv = std::max(nmax, 1);
+    res.vstride = (long long)res.ldv * res.ldv;
+    res.v = (double*)ctx.arena.take(sizeof(double) * nprob * res.vstride);
+    res.avail = const_cast<int*>(n_of);
+    SPP_CHECK(launch_jacobi(rho, rho_pstride, rho_ld, res.w, res.wstride, res.v, res.vstride, res.ldv, n_of,
+                            nullptr, nprob, st));
+    ctx.last_iterations = 0;
+    return;
+  }
+  const int kk = ctx.ksub;
+  const long long bs = (long long)nmax * kk;  // per-problem block size
+  double* Qa = (double*)ctx.arena.take(sizeof(double) * nprob * bs);
+  double* Za = (double*)ctx.arena.take(sizeof(double) * nprob * bs);
+  double* Qb = (double*)ctx.arena.take(sizeof(double) * nprob * bs);
+  double* Zb = (double*)ctx.arena.take(sizeof(double) * nprob * bs);
+  const long long ss = (long long)kk * kk;
+  double* Sm = (double*)ctx.arena.take(sizeof(double) * nprob * ss);
+  double* Vm = (double*)ctx.arena.take(sizeof(double) * nprob * ss);
+  double* sig = (double*)ctx.arena.take(sizeof(double) * nprob * kk);
+  int* kk_of = (int*)ctx.arena.take(sizeof(int) * nprob);
+  int* conv = (int*)ctx.arena.take(sizeof(int) * nprob);
+  double* worst = (double*)ctx.arena.take(sizeof(double) * nprob);
+  SPP_CHECK(launch_fill<int>(kk_of, nprob, kk, st));
+  Leg fl = flat_leg(kk);
+  // operand references (compact per (sample, parity) blocks; sample stride = 2 blocks)
+  auto blockref = [&](double* base, const std::string& letters) {
+    return compact_ref(base, 2 * bs, bs, kk, letters, 2, {lk, lp, fl});
+  };
+  auto smallref = [&](double* base, const std::string& letters) {
+    return compact_ref(base, 2 * ss, ss, kk, letters, 1, {fl, fl});
+  };
+  TRef rhoR = compact_ref(const_cast<double*>(rho), 2 * rho_pstride, rho_pstride, rho_ld, "kpKP", 2,
+                          {lk, lp, lk, lp});
+  auto matmul_rho = [&](double* Z, double* Q) {  // Z = rho Q
+    contract<double>(ctx, blockref(Z, "kpc"), rhoR, blockref(Q, "KPc"), nb);
+  };
+  auto rmul = [&](double* out, double* Z, double* V) {  // out = Z V
+    contract<double>(ctx, blockref(out, "kpb"), blockref(Z, "kpa"), smallref(V, "ab"), nb);
+  };
+  double* tau = (double*)ctx.arena.take(sizeof(double) * nprob * kk);
+  auto orth = [&](double* Q, double* Z, double* tmp) {  // Q <- orth(Z), Householder; Z destroyed
+    (void)tmp;
+    hipLaunchKernelGGL(householder_q_kernel, dim3(nprob), dim3(256), 0, st, Z, Q, bs, kk, kk, n_of, tau);
+    SPP_CHECK(hipGetLastError());
+  };
+  // start block
+  hipLaunchKernelGGL(random_block_kernel, dim3(64, nprob), dim3(256), 0, st, Qb, bs, kk, kk, n_of, nmax);
+  SPP_CHECK(hipGetLastError());
+  matmul_rho(Za, Qb);
+  orth(Qa, Za, Zb);
+  const int t = std::min(want, kk);
+  std::vector<int> hconv(nprob);
+  int it = 0;
+  for (it = 0; it < ctx.max_subspace_iter; ++it) {
+    matmul_rho(Za, Qa);
+    // H = Q^T Z
+    contract<double>(ctx, smallref(Sm, "ab"), compact_ref(Qa, 2 * bs, bs, kk, "kpa", 2, {lk, lp, fl}),
+                     blockref(Za, "kpb"), nb);
+    SPP_CHECK(launch_jacobi(Sm, ss, kk, sig, kk, Vm, ss, kk, kk_of, nullptr, nprob, st));
+    rmul(Qb, Qa, Vm);
+    rmul(Zb, Za, Vm);
+    hipLaunchKernelGGL(residual_kernel, dim3(nprob), dim3(256), 0, st, Qb, Zb, bs, kk, sig, (long long)kk, n_of,
+                       t, ctx.subspace_tol, conv, worst);
+    SPP_CHECK(hipGetLastError());
+    SPP_CHECK(hipMemcpyAsync(hconv.data(), conv, sizeof(int) * nprob, hipMemcpyDeviceToHost, st));
+    SPP_CHECK(hipStreamSynchronize(st));
+    bool all = true;
+    for (int p = 0; p < nprob; ++p) all = all && hconv[p];
+    if (all) break;
+    orth(Qa, Zb, Za);
+  }
+  ctx.last_iterations = it + 1;
+  res.w = sig;
+  res.wstride = kk;
+  res.v = Qb;
+  res.vstride = bs;
+  res.ldv = kk;
+  res.avail = kk_of;
+}
+
+// ------------------------------- absorb -------------------------------------
+template <typename T>
+void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& row, int chi, Boundary<T>& out) {
+  const int L = env.L;
+  const int nb = env.nb;
+  hipStream_t st = ctx.stream;
+  if (row.L != L) throw HipError("absorb: row length mismatch");
+  const size_t mark = ctx.arena.mark();
+  // ---------------- right Gram matrices G[x], x = L..1 -----------------------
+  // G[x] legs (l, w, L, W): bond x of env and the W leg of site x.
+  std::vector<T*> G(L + 1, nullptr);
+  std::vector<std::vector<Leg>> Gl(L + 1);
+  G[L] = (T*)ctx.arena.take(sizeof(T) * nb);
+  SPP_CHECK(launch_fill<T>(G[L], nb, T(1), st));
+  Gl[L] = {env.bond(L), row.spec[L - 1].e, env.bond(L), row.spec[L - 1].e};
+  long long zmax = 0;
+  for (int x = 1; x < L; ++x) {
+    const LayerSpec& sp = row.spec[x];
+    long long a = (long long)env.bdim[x] * sp.n.dim * sp.e.dim * env.bdim[x + 1] * sp.e.dim;
+    long long b = (long long)env.bdim[x] * env.bdim[x + 1] * sp.e.dim * sp.w.dim * sp.s.dim;
+    long long c = (long long)env.bdim[x] * env.bdim[x + 1] * sp.w.dim * sp.w.dim * sp.n.dim;
+    zmax = std::max(zmax, std::max(a, std::max(b, c)));
+  }
+  T* Z1 = zmax ? (T*)ctx.arena.take(sizeof(T) * nb * zmax) : nullptr;
+  T* Z2 = zmax ? (T*)ctx.arena.take(sizeof(T) * nb * zmax) : nullptr;
+  for (int x = L - 1; x >= 1; --x) {
+    const LayerSpec& sp = row.spec[x];
+    Leg bl = env.bond(x), br = env.bond(x + 1);
+    long long gs = (long long)bl.dim * sp.w.dim * bl.dim * sp.w.dim;
+    G[x] = (T*)ctx.arena.take(sizeof(T) * nb * gs);
+    Gl[x] = {bl, sp.w, bl, sp.w};
+    TRef A = env.ref(x, "lnr");
+    TRef Gn = dense_ref(G[x + 1], (long long)Gl[x + 1][0].dim * Gl[x + 1][1].dim * Gl[x + 1][2].dim * Gl[x + 1][3].dim,
+                        "reRF", Gl[x + 1]);
+    TRef z1 = dense_ref(Z1, zmax, "lneRF", {bl, sp.n, sp.e, br, sp.e});
+    contract<T>(ctx, z1, A, Gn, nb);
+    TRef z2 = dense_ref(Z2, zmax, "lRFwp", {bl, br, sp.e, sp.w, sp.s});
+    contract<T>(ctx, z2, z1, row.ref(x, "wnep"), nb);
+    TRef z3 = dense_ref(Z1, zmax, "lRwWN", {bl, br, sp.w, sp.w, sp.n});
+    contract<T>(ctx, z3, z2, row.ref(x, "WNFp"), nb);
+    TRef g = dense_ref(G[x], gs, "lwLW", Gl[x]);
+    contract<T>(ctx, g, z3, env.ref(x, "LNR"), nb);
+  }
+  // ---------------- left-to-right truncation sweep --------------------------
+  out.L = L;
+  out.nb = nb;
+  out.bdim.assign(L + 1, 1);
+  out.beven.assign(L + 1, 1);
+  if (out.bsplit.size() != (size_t)(L + 1)) throw HipError("absorb: output boundary not prepared");
+  out.pdim.resize(L);
+  out.peven.resize(L);
+  for (int x = 0; x < L; ++x) {
+    out.pdim[x] = row.spec[x].s.dim;
+    out.peven[x] = row.spec[x].s.even;
+  }
+  SPP_CHECK(hipMemcpyAsync(out.log_scale, env.log_scale, sizeof(double) * nb, hipMemcpyDeviceToDevice, st));
+  SPP_CHECK(hipMemcpyAsync(out.max_trunc, env.max_trunc, sizeof(double) * nb, hipMemcpyDeviceToDevice, st));
+  // host estimate of LQ ranks kR[x] (columns of L_x), per parity, even-split legs assumed
+  std::vector<std::array<long long, 2>> kR(L + 1);
+  kR[L] = {1, 0};
+  for (int x = L - 1; x >= 1; --x) {
+    const LayerSpec& sp = row.spec[x];
+    auto cnt2 = [](const Leg& a, long long ae, long long ao, int q) -> long long {
+      long long e = a.split ? a.dim : a.even, o = a.split ? a.dim : a.dim - a.even;  // per-sample: bound
+      return q == 0 ? e * ae + o * ao : e * ao + o * ae;
+    };
+    for (int q = 0; q < 2; ++q) {
+      long long rows = cnt2(env.bond(x), sp.w.even, sp.w.dim - sp.w.even, q);
+      long long cols = cnt2(sp.s, kR[x + 1][0], kR[x + 1][1], q);
+      kR[x][q] = std::min(rows, cols);
+    }
+  }
+  T* Rprev = (T*)ctx.arena.take(sizeof(T) * nb);
+  SPP_CHECK(launch_fill<T>(Rprev, nb, T(1), st));
+  std::vector<Leg> Rl = {fixed_leg(1, 1), env.bond(0), row.spec[0].w};
+  Leg kb = fixed_leg(1, 1);
+  double* nrm = (double*)ctx.arena.take(sizeof(double) * nb);
+  double* fac = (double*)ctx.arena.take(sizeof(double) * nb);
+  double* tr = (double*)ctx.arena.take(sizeof(double) * 2 * nb);
+  int* ncount = (int*)ctx.arena.take(sizeof(int) * 2 * nb);
+  for (int x = 0; x < L; ++x) {
+    const LayerSpec& sp = row.spec[x];
+    Leg br = env.bond(x + 1);
+    // R for the next site is allocated below this site's scratch mark
+    const long long rs_cap = (long long)chi * br.dim * sp.e.dim;
+    T* Rn = (x < L - 1) ? (T*)ctx.arena.take(sizeof(T) * nb * rs_cap) : nullptr;
+    const size_t xm = ctx.arena.mark();
+    long long qs = (long long)kb.dim * sp.w.dim * sp.n.dim * br.dim;
+    T* Q = (T*)ctx.arena.take(sizeof(T) * nb * qs);
+    TRef Rp = dense_ref(Rprev, (long long)Rl[0].dim * Rl[1].dim * Rl[2].dim, "klw", Rl);
+    TRef Qr = dense_ref(Q, qs, "kwnr", {kb, sp.w, sp.n, br});
+    contract<T>(ctx, Qr, Rp, env.ref(x, "lnr"), nb);
+    long long ys = (long long)kb.dim * sp.s.dim * br.dim * sp.e.dim;
+    T* Y = (T*)ctx.arena.take(sizeof(T) * nb * ys);
+    TRef Yr = dense_ref(Y, ys, "kpre", {kb, sp.s, br, sp.e});
+    contract<T>(ctx, Yr, Qr, row.ref(x, "wnep"), nb);
+    if (x == L - 1) {
+      // last tensor: normalise and store (k, p, 1)
+      if (br.dim != 1 || sp.e.dim != 1) throw HipError("absorb: last site must close with unit legs");
+      FullView fv = full_view(Yr);
+      SPP_CHECK(launch_dot<T>(fv, fv, nrm, nb, st));
+      hipLaunchKernelGGL(norm_finish_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, nrm, out.log_scale, fac,
+                         nullptr, nb);
+      SPP_CHECK(hipGetLastError());
+      SPP_CHECK(launch_scale<T>(fv, fac, nb, st));
+      out.bdim[x] = kb.dim;
+      out.beven[x] = kb.even;
+      SPP_CHECK(hipMemcpyAsync(out.t[x], Y, sizeof(T) * nb * ys, hipMemcpyDeviceToDevice, st));
+      out.bdim[L] = 1;
+      out.beven[L] = 1;
+      ctx.arena.release(xm);
+      break;
+    }
+    // W = Y G[x+1]; rho = W Y^T (compact per parity block)
+    const std::vector<Leg>& gl = Gl[x + 1];
+    TRef Gr = dense_ref(G[x + 1], (long long)gl[0].dim * gl[1].dim * gl[2].dim * gl[3].dim, "reRF", gl);
+    T* W = (T*)ctx.arena.take(sizeof(T) * nb * ys);
+    TRef Wr = dense_ref(W, ys, "kpRF", {kb, sp.s, br, sp.e});
+    contract<T>(ctx, Wr, Yr, Gr, nb);
+    // rows (k,p) tuple counts per problem and their bound
+    long long nmax = group_bound({kb, sp.s});
+    Group gkp{};
+    fill_group(gkp, {kb, sp.s}, "kp");
+    hipLaunchKernelGGL(group_counts_kernel, dim3((2 * nb + 255) / 256), dim3(256), 0, st, gkp, ncount, 2 * nb);
+    SPP_CHECK(hipGetLastError());
+    const long long rps = nmax * nmax;
+    double* rho = (double*)ctx.arena.take(sizeof(double) * 2 * nb * rps);
+    if (sizeof(T) == sizeof(double)) {
+      contract<T>(ctx, compact_ref(rho, 2 * rps, rps, nmax, "kpKP", 2, {kb, sp.s, kb, sp.s}), Wr,
+                  with_letters(Yr, "KPRF"), nb);
+    } else {
+      T* rho_t = (T*)ctx.arena.take(sizeof(T) * 2 * nb * rps);
+      contract<T>(ctx, compact_ref(rho_t, 2 * rps, rps, nmax, "kpKP", 2, {kb, sp.s, kb, sp.s}), Wr,
+                  with_letters(Yr, "KPRF"), nb);
+      SPP_CHECK((launch_convert<double, T>(rho, rho_t, 2 * nb * rps, st)));
+    }
+    hipLaunchKernelGGL(trace_kernel, dim3(2 * nb), dim3(256), 0, st, rho, rps, (int)nmax, ncount, tr);
+    SPP_CHECK(hipGetLastError());
+    // bond capacity of this cut
+    long long nq[2];
+    for (int q = 0; q < 2; ++q) {
+      long long e = kb.split ? kb.dim : kb.even, o = kb.split ? kb.dim : kb.dim - kb.even;
+      long long pe = sp.s.even, po = sp.s.dim - sp.s.even;
+      nq[q] = std::min(nmax, q == 0 ? e * pe + o * po : e * po + o * pe);
+    }
+    int cap = (int)std::min<long long>(chi, std::min(nq[0], kR[x + 1][0]) + std::min(nq[1], kR[x + 1][1]));
+    if (cap < 1) cap = 1;
+    EigResult er{};
+    eig_topk(ctx, rho, rps, (int)nmax, kb, sp.s, ncount, (int)nmax, nb, cap, er);
+    // U[k,p,c] into the output boundary storage
+    long long us = (long long)kb.dim * sp.s.dim * cap;
+    T* U = out.t[x];
+    SPP_CHECK(launch_fill<T>(U, nb * us, T(0), st));
+    hipLaunchKernelGGL(truncate_select_kernel<T>, dim3(nb), dim3(256), 0, st, gkp, er.w, er.wstride, er.v,
+                       er.vstride, er.ldv, er.avail, ncount, tr, chi, cap, U, us, (long long)sp.s.dim * cap,
+                       (long long)cap, out.bsplit[x + 1], out.max_trunc, nullptr);
+    SPP_CHECK(hipGetLastError());
+    out.bdim[x] = kb.dim;
+    out.beven[x] = kb.even;
+    Leg nbond;
+    nbond.dim = cap;
+    nbond.split = out.bsplit[x + 1];
+    nbond.even = cap;
+    // R = U^T Y -> (c, r, e)
+    long long rs = (long long)cap * br.dim * sp.e.dim;
+    contract<T>(ctx, dense_ref(Rn, rs, "cre", {nbond, br, sp.e}), dense_ref(U, us, "kpc", {kb, sp.s, nbond}),
+                Yr, nb);
+    ctx.arena.release(xm);
+    Rprev = Rn;
+    Rl = {nbond, br, sp.e};
+    kb = nbond;
+  }
+  ctx.arena.release(mark);
+}
+
+template void absorb_forward<float>(Context&, const Boundary<float>&, const RowLayers<float>&, int,
+                                    Boundary<float>&);
+template void absorb_forward<double>(Context&, const Boundary<double>&, const RowLayers<double>&, int,
+                                     Boundary<double>&);
+
+}  // namespace spp

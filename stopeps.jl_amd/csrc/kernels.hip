@@ -1,0 +1,395 @@
+// Auxiliary kernels: neutral-entry reductions and updates, double layers,
+// batched Jacobi eigensolver. All are memory- or latency-bound helpers around
+// the MFMA contraction kernel.
+#include "kernels.h"
+
+namespace spp {
+
+// ---------------------------------------------------------------------------
+// Neutral-entry kernels. One block group per sample; tuples of parity 0 over
+// all legs are enumerated with the shared group decoder.
+
+template <typename T>
+__device__ inline T* fv_base(const FullView& v, int s) {
+  return v.ptrs ? reinterpret_cast<T*>(v.ptrs[s]) : reinterpret_cast<T*>(v.base) + (long long)s * v.sstride;
+}
+
+__device__ inline long long fv_off(const FullView& v, const GroupView& g, const int* legs) {
+  long long o = 0;
+  for (int j = 0; j < g.n; ++j) o += (long long)legs[j] * v.stride[j];
+  return o;
+}
+
+template <typename T>
+__global__ void dot_kernel(FullView x, FullView y, double* out) {
+  const int s = blockIdx.x;
+  __shared__ GroupView g;
+  __shared__ double red[256];
+  if (threadIdx.x == 0) resolve_group(x.g, s, g);
+  __syncthreads();
+  const long long n = group_count(g, 0);
+  const T* xb = fv_base<T>(x, s);
+  const T* yb = fv_base<T>(y, s);
+  double acc = 0.0;
+  int legs[kMaxLegs];
+  for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+    decode(g, 0, i, legs);
+    acc += (double)xb[fv_off(x, g, legs)] * (double)yb[fv_off(y, g, legs)];
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[s] = red[0];
+}
+
+template <typename T>
+hipError_t launch_dot(const FullView& x, const FullView& y, double* out, int nsamples, hipStream_t st) {
+  if (nsamples <= 0) return hipSuccess;
+  hipLaunchKernelGGL(dot_kernel<T>, dim3(nsamples), dim3(256), 0, st, x, y, out);
+  return hipGetLastError();
+}
+
+template <typename T>
+__global__ void scale_kernel(FullView x, const double* f) {
+  const int s = blockIdx.y;
+  const double fs = f[s];
+  if (fs == 0.0 || fs == 1.0) return;
+  __shared__ GroupView g;
+  if (threadIdx.x == 0) resolve_group(x.g, s, g);
+  __syncthreads();
+  const long long n = group_count(g, 0);
+  T* xb = fv_base<T>(x, s);
+  int legs[kMaxLegs];
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    decode(g, 0, i, legs);
+    long long o = fv_off(x, g, legs);
+    xb[o] = (T)((double)xb[o] * fs);
+  }
+}
+
+template <typename T>
+hipError_t launch_scale(const FullView& x, const double* f, int nsamples, hipStream_t st) {
+  if (nsamples <= 0) return hipSuccess;
+  hipLaunchKernelGGL(scale_kernel<T>, dim3(8, nsamples), dim3(256), 0, st, x, f);
+  return hipGetLastError();
+}
+
+template <typename T>
+__global__ void select_copy_kernel(FullView out, FullView a, FullView b, const int* choice, const double* f) {
+  const int s = blockIdx.y;
+  __shared__ GroupView g;
+  if (threadIdx.x == 0) resolve_group(out.g, s, g);
+  __syncthreads();
+  const long long n = group_count(g, 0);
+  const int c = choice ? choice[s] : 0;
+  const FullView& src = c == 0 ? a : b;
+  const T* sb = fv_base<T>(src, s);
+  T* ob = fv_base<T>(out, s);
+  const double fs = f ? (f[s] == 0.0 ? 1.0 : f[s]) : 1.0;
+  int legs[kMaxLegs];
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    decode(g, 0, i, legs);
+    ob[fv_off(out, g, legs)] = (T)((double)sb[fv_off(src, g, legs)] * fs);
+  }
+}
+
+template <typename T>
+hipError_t launch_select_copy(const FullView& out, const FullView& a, const FullView& b, const int* choice,
+                              const double* f, int nsamples, hipStream_t st) {
+  if (nsamples <= 0) return hipSuccess;
+  hipLaunchKernelGGL(select_copy_kernel<T>, dim3(8, nsamples), dim3(256), 0, st, out, a, b, choice, f);
+  return hipGetLastError();
+}
+
+template <typename T>
+__global__ void fill_kernel(T* p, long long n, T v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+template <typename T>
+hipError_t launch_fill(T* p, long long n, T v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(fill_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+template <typename TO, typename TI>
+__global__ void convert_kernel(TO* out, const TI* in, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] = (TO)in[i];
+}
+
+template <typename TO, typename TI>
+hipError_t launch_convert(TO* out, const TI* in, long long n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL((convert_kernel<TO, TI>), dim3((unsigned)blocks), dim3(256), 0, st, out, in, n);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Double layers. Delta index of a (ket i, bra j) pair in parity-sorted pair
+// order: even pairs (ee then oo), then odd pairs (eo then oe), bra fastest.
+
+__device__ inline void split_pair(int idx, int dk, int ek, int db, int eb, int& i, int& j) {
+  const int ok = dk - ek, ob = db - eb;
+  const int ee = ek * eb, oo = ok * ob, eo = ek * ob;
+  if (idx < ee) { i = idx / eb; j = idx % eb; return; }
+  idx -= ee;
+  if (idx < oo) { i = ek + idx / ob; j = eb + idx % ob; return; }
+  idx -= oo;
+  if (idx < eo) { i = idx / ob; j = eb + idx % ob; return; }
+  idx -= eo;
+  i = ek + idx / eb;
+  j = idx % eb;
+}
+
+__device__ inline int par_of(int i, int e) { return i >= e ? 1 : 0; }
+
+template <typename T>
+__global__ void double_layer_kernel(const LayerJob* jobs) {
+  const LayerJob& J = jobs[blockIdx.y];
+  const int DW = J.dk[0] * J.db[0], DN = J.dk[1] * J.db[1], DE = J.dk[4] * J.db[4], DS = J.dk[5] * J.db[5];
+  const long long total = (long long)DW * DN * DE * DS;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  long long r = t;
+  const int is = (int)(r % DS); r /= DS;
+  const int ie = (int)(r % DE); r /= DE;
+  const int in_ = (int)(r % DN); r /= DN;
+  const int iw = (int)r;
+  int wk, wb, nk, nb, ek, eb, sk, sb;
+  split_pair(iw, J.dk[0], J.ek[0], J.db[0], J.eb[0], wk, wb);
+  split_pair(in_, J.dk[1], J.ek[1], J.db[1], J.eb[1], nk, nb);
+  split_pair(ie, J.dk[4], J.ek[4], J.db[4], J.eb[4], ek, eb);
+  split_pair(is, J.dk[5], J.ek[5], J.db[5], J.eb[5], sk, sb);
+  T* out = reinterpret_cast<T*>(J.out);
+  const int pk = par_of(wk, J.ek[0]) + par_of(nk, J.ek[1]) + par_of(ek, J.ek[4]) + par_of(sk, J.ek[5]);
+  const int pb = par_of(wb, J.eb[0]) + par_of(nb, J.eb[1]) + par_of(eb, J.eb[4]) + par_of(sb, J.eb[5]);
+  if (((pk + pb) & 1) != 0) {
+    out[t] = T(0);
+    return;
+  }
+  // strides of the (W,N,P,A,E,S) row-major site arrays
+  long long skt[6], sbt[6];
+  skt[5] = 1;
+  sbt[5] = 1;
+  for (int k = 4; k >= 0; --k) {
+    skt[k] = skt[k + 1] * J.dk[k + 1];
+    sbt[k] = sbt[k + 1] * J.db[k + 1];
+  }
+  const long long bk = wk * skt[0] + nk * skt[1] + ek * skt[4] + sk * skt[5];
+  const long long bb = wb * sbt[0] + nb * sbt[1] + eb * sbt[4] + sb * sbt[5];
+  const int dp = J.dk[2], da_k = J.dk[3], da_b = J.db[3];
+  double acc = 0.0;
+  for (int p = 0; p < dp; ++p) {
+    if (J.use_aux) {
+      double kv = 0.0, bv = 0.0;
+      for (int a = 0; a < da_k; ++a) {
+        double v = 0.0;
+        if (J.use_op) {
+          for (int pp = 0; pp < dp; ++pp) v += J.op[p * 2 + pp] * J.ket[bk + pp * skt[2] + a * skt[3]];
+        } else {
+          v = J.ket[bk + p * skt[2] + a * skt[3]];
+        }
+        kv += J.aux[a] * v;
+      }
+      for (int a = 0; a < da_b; ++a) bv += J.aux[a] * J.bra[bb + p * sbt[2] + a * sbt[3]];
+      acc += kv * bv;
+    } else {
+      for (int a = 0; a < da_k; ++a) {
+        double v = 0.0;
+        if (J.use_op) {
+          for (int pp = 0; pp < dp; ++pp) v += J.op[p * 2 + pp] * J.ket[bk + pp * skt[2] + a * skt[3]];
+        } else {
+          v = J.ket[bk + p * skt[2] + a * skt[3]];
+        }
+        acc += v * J.bra[bb + p * sbt[2] + a * sbt[3]];
+      }
+    }
+  }
+  out[t] = (T)acc;
+}
+
+template <typename T>
+hipError_t launch_double_layers(const LayerJob* jobs_dev, int njobs, long long maxout, hipStream_t st) {
+  if (njobs <= 0 || maxout <= 0) return hipSuccess;
+  dim3 grid((unsigned)((maxout + 255) / 256), (unsigned)njobs);
+  hipLaunchKernelGGL(double_layer_kernel<T>, grid, dim3(256), 0, st, jobs_dev);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Cyclic Jacobi eigensolver with round-robin parallel ordering, n <= 64, FP64.
+
+constexpr int JMAX = 64;
+constexpr int JLD = JMAX + 1;
+
+__global__ __launch_bounds__(256) void jacobi_kernel(const double* a, long long astride, int lda, double* w,
+                                                     long long wstride, double* v, long long vstride, int ldv,
+                                                     const int* n_of, const int* pmask) {
+  const int prob = blockIdx.x;
+  if (pmask && !pmask[prob]) return;
+  const int n = n_of[prob];
+  if (n <= 0) return;
+  __shared__ double A[JMAX * JLD];
+  __shared__ double V[JMAX * JLD];
+  __shared__ double cs[JMAX / 2], sn[JMAX / 2];
+  __shared__ int pp_[JMAX / 2], qq_[JMAX / 2];
+  __shared__ int order[JMAX + 1];
+  __shared__ double red[256];
+  __shared__ int done;
+  const int tid = threadIdx.x;
+  const int m = (n & 1) ? n + 1 : n;  // padded size (dummy index n when odd)
+  const double* ap = a + (long long)prob * astride;
+  for (int i = tid; i < m * m; i += blockDim.x) {
+    int r = i / m, c = i % m;
+    double val = 0.0;
+    if (r < n && c < n) val = 0.5 * (ap[(long long)r * lda + c] + ap[(long long)c * lda + r]);
+    A[r * JLD + c] = val;
+    V[r * JLD + c] = (r == c) ? 1.0 : 0.0;
+  }
+  if (tid <= m) order[tid] = tid;
+  __syncthreads();
+  // Frobenius norm for the stopping rule
+  double loc = 0.0;
+  for (int i = tid; i < m * m; i += blockDim.x) {
+    double x = A[(i / m) * JLD + (i % m)];
+    loc += x * x;
+  }
+  red[tid] = loc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) red[tid] += red[tid + s];
+    __syncthreads();
+  }
+  const double fro2 = red[0];
+  __syncthreads();
+  const int half = m / 2;
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    // off-diagonal mass
+    loc = 0.0;
+    for (int i = tid; i < m * m; i += blockDim.x) {
+      int r = i / m, c = i % m;
+      if (r != c) {
+        double x = A[r * JLD + c];
+        loc += x * x;
+      }
+    }
+    red[tid] = loc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) red[tid] += red[tid + s];
+      __syncthreads();
+    }
+    if (tid == 0) done = (red[0] <= 1e-32 * fro2 || fro2 == 0.0) ? 1 : 0;
+    __syncthreads();
+    if (done) break;
+    for (int round = 0; round < m - 1; ++round) {
+      // round-robin pairing: position 0 fixed, others rotate
+      if (tid < half) {
+        int i1 = tid, i2 = m - 1 - tid;
+        int p = (i1 == 0) ? 0 : 1 + ((i1 - 1 + round) % (m - 1));
+        int q = 1 + ((i2 - 1 + round) % (m - 1));
+        if (p > q) { int t = p; p = q; q = t; }
+        pp_[tid] = p;
+        qq_[tid] = q;
+        double apq = A[p * JLD + q];
+        double app = A[p * JLD + p], aqq = A[q * JLD + q];
+        double c = 1.0, s = 0.0;
+        if (apq != 0.0 && fabs(apq) > 1e-300) {
+          double theta = (aqq - app) / (2.0 * apq);
+          double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+          c = 1.0 / sqrt(t * t + 1.0);
+          s = t * c;
+        }
+        cs[tid] = c;
+        sn[tid] = s;
+      }
+      __syncthreads();
+      // rows: A <- J^T A
+      for (int idx = tid; idx < half * m; idx += blockDim.x) {
+        int k = idx / m, j = idx % m;
+        int p = pp_[k], q = qq_[k];
+        double c = cs[k], s = sn[k];
+        double ap_ = A[p * JLD + j], aq_ = A[q * JLD + j];
+        A[p * JLD + j] = c * ap_ - s * aq_;
+        A[q * JLD + j] = s * ap_ + c * aq_;
+      }
+      __syncthreads();
+      // columns: A <- A J, V <- V J
+      for (int idx = tid; idx < half * m; idx += blockDim.x) {
+        int k = idx / m, j = idx % m;
+        int p = pp_[k], q = qq_[k];
+        double c = cs[k], s = sn[k];
+        double ap_ = A[j * JLD + p], aq_ = A[j * JLD + q];
+        A[j * JLD + p] = c * ap_ - s * aq_;
+        A[j * JLD + q] = s * ap_ + c * aq_;
+        double vp = V[j * JLD + p], vq = V[j * JLD + q];
+        V[j * JLD + p] = c * vp - s * vq;
+        V[j * JLD + q] = s * vp + c * vq;
+      }
+      __syncthreads();
+    }
+  }
+  // sort eigenvalues (descending) over the n real indices
+  if (tid == 0) {
+    int cnt = 0;
+    for (int i = 0; i < m; ++i)
+      if (i < n) order[cnt++] = i;
+    for (int i = 1; i < n; ++i) {
+      int key = order[i];
+      double kv = A[key * JLD + key];
+      int j = i - 1;
+      while (j >= 0 && A[order[j] * JLD + order[j]] < kv) {
+        order[j + 1] = order[j];
+        --j;
+      }
+      order[j + 1] = key;
+    }
+  }
+  __syncthreads();
+  double* wp = w + (long long)prob * wstride;
+  double* vp = v + (long long)prob * vstride;
+  for (int i = tid; i < n; i += blockDim.x) wp[i] = A[order[i] * JLD + order[i]];
+  for (int i = tid; i < n * n; i += blockDim.x) {
+    int r = i / n, c = i % n;
+    vp[(long long)r * ldv + c] = V[r * JLD + order[c]];
+  }
+}
+
+hipError_t launch_jacobi(const double* a, long long astride, int lda, double* w, long long wstride, double* v,
+                         long long vstride, int ldv, const int* n_of, const int* pmask, int nprob,
+                         hipStream_t st) {
+  if (nprob <= 0) return hipSuccess;
+  hipLaunchKernelGGL(jacobi_kernel, dim3(nprob), dim3(256), 0, st, a, astride, lda, w, wstride, v, vstride, ldv,
+                     n_of, pmask);
+  return hipGetLastError();
+}
+
+template hipError_t launch_dot<float>(const FullView&, const FullView&, double*, int, hipStream_t);
+template hipError_t launch_dot<double>(const FullView&, const FullView&, double*, int, hipStream_t);
+template hipError_t launch_scale<float>(const FullView&, const double*, int, hipStream_t);
+template hipError_t launch_scale<double>(const FullView&, const double*, int, hipStream_t);
+template hipError_t launch_select_copy<float>(const FullView&, const FullView&, const FullView&, const int*,
+                                              const double*, int, hipStream_t);
+template hipError_t launch_select_copy<double>(const FullView&, const FullView&, const FullView&, const int*,
+                                               const double*, int, hipStream_t);
+template hipError_t launch_fill<float>(float*, long long, float, hipStream_t);
+template hipError_t launch_fill<double>(double*, long long, double, hipStream_t);
+template hipError_t launch_fill<int>(int*, long long, int, hipStream_t);
+template hipError_t launch_convert<float, double>(float*, const double*, long long, hipStream_t);
+template hipError_t launch_convert<double, float>(double*, const float*, long long, hipStream_t);
+template hipError_t launch_convert<double, double>(double*, const double*, long long, hipStream_t);
+template hipError_t launch_convert<float, float>(float*, const float*, long long, hipStream_t);
+template hipError_t launch_double_layers<float>(const LayerJob*, int, long long, hipStream_t);
+template hipError_t launch_double_layers<double>(const LayerJob*, int, long long, hipStream_t);
+
+}  // namespace spp

@@ -1,0 +1,584 @@
+// Batched perfect Born sampling (restates sample_configuration, auxiliary
+// kind, src/sampling/born.jl:86-163, and SamplingCache, born.jl:45-73).
+//
+// Per row y, for all samples of a chunk at once: right row environments
+// (_right_step, src/contraction/boundary_mps.jl:240-246), then for x = 1..Lx
+// the two candidate left steps (_left_step, boundary_mps.jl:233-239) joined
+// with the right environment (_row_scalar, boundary_mps.jl:274-284), the
+// validity check and categorical draw with the injected uniform
+// (born.jl:136-151, _categorical src/resample/selection.jl:68-80); the left
+// step of the drawn candidate is reused instead of being recomputed. The row
+// of drawn candidates is then absorbed into the per-sample north boundary
+// (born.jl:157-160).
+#include "sampler.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace spp {
+
+StateDev::~StateDev() {
+  for (auto& s : sites)
+    if (s.d) (void)hipFree(s.d);
+}
+
+template <typename T>
+CacheDev<T>::~CacheDev() {
+  for (auto p : layers)
+    if (p) (void)hipFree(p);
+  for (int k = 0; k < 2; ++k)
+    for (auto p : cand[k])
+      if (p) (void)hipFree(p);
+  if (cand_lognorm) (void)hipFree(cand_lognorm);
+  for (auto& b : south) b.release();
+}
+template struct CacheDev<float>;
+template struct CacheDev<double>;
+
+// ---------------------------------------------------------------------------
+// small kernels
+
+// log || sum_a A[..a..] u[a] || of one FP64 site, for both basis vectors.
+__global__ void proj_lognorm_kernel(const double* a, int n_before, int naux, int n_after, double u00, double u01,
+                                    double u10, double u11, double* out) {
+  __shared__ double part[2][256];
+  double acc0 = 0.0, acc1 = 0.0;
+  const long long total = (long long)n_before * n_after;
+  for (long long i = threadIdx.x; i < total; i += blockDim.x) {
+    long long b = i / n_after, c = i % n_after;
+    double v0 = 0.0, v1 = 0.0;
+    for (int k = 0; k < naux; ++k) {
+      double x = a[(b * naux + k) * n_after + c];
+      if (k == 0) { v0 += u00 * x; v1 += u01 * x; }
+      else { v0 += u10 * x; v1 += u11 * x; }
+    }
+    acc0 += v0 * v0;
+    acc1 += v1 * v1;
+  }
+  part[0][threadIdx.x] = acc0;
+  part[1][threadIdx.x] = acc1;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      part[0][threadIdx.x] += part[0][threadIdx.x + w];
+      part[1][threadIdx.x] += part[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = 0.5 * log(part[0][0]);
+    out[1] = 0.5 * log(part[1][0]);
+  }
+}
+
+// _safe_scale: factor = 1/norm (0 keeps the tensor), nonfinite norm -> status 3
+__global__ void safe_scale_kernel(const double* nrm2, double* factor, int* status, int* fail_site, int site,
+                                  int nb) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nb) return;
+  double n = sqrt(nrm2[s]);
+  if (!isfinite(n)) {
+    if (status[s] == 0) {
+      status[s] = 3;
+      fail_site[s] = site;
+    }
+    factor[s] = 0.0;
+    return;
+  }
+  factor[s] = n == 0.0 ? 0.0 : 1.0 / n;
+}
+
+// Born conditional and categorical draw for one site, all samples.
+__global__ void born_select_kernel(const double* raw0, const double* raw1, const double* uniforms, const int* labels_in,
+                                   long long sample0, int nsite, int site, int* choice, int* labels, double* logp,
+                                   double* cond, int* status, int* fail_site, int* done, int nb) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nb) return;
+  const long long gs = sample0 + s;
+  double raw[2] = {raw0[s], raw1[s]};
+  int k = 0;
+  if (status[s] == 0 && !done[s]) {
+    double scale = fmax(fabs(raw[0]), fabs(raw[1]));
+    const double eps = 2.220446049250313e-16;
+    bool ok = isfinite(raw[0]) && isfinite(raw[1]) && raw[0] >= -64.0 * eps * scale && raw[1] >= -64.0 * eps * scale;
+    if (!ok) {
+      status[s] = 1;
+      fail_site[s] = site;
+    } else {
+      double v0 = fmax(raw[0], 0.0), v1 = fmax(raw[1], 0.0);
+      double total = v0 + v1;
+      if (!(total > 0.0)) {
+        status[s] = 2;
+        fail_site[s] = site;
+      } else {
+        double p[2] = {v0 / total, v1 / total};
+        if (labels_in) {
+          k = labels_in[gs * nsite + site];
+        } else {
+          // _categorical(log.(p), u): z = logsumexp, cumulative exp(lm - z)
+          double lm[2] = {log(p[0]), log(p[1])};
+          double pivot = fmax(lm[0], lm[1]);
+          double z = pivot + log(exp(lm[0] - pivot) + exp(lm[1] - pivot));
+          double u = uniforms[gs * nsite + site];
+          double cum = 0.0;
+          int last = -1;
+          k = -1;
+          for (int j = 0; j < 2; ++j) {
+            if (lm[j] == -INFINITY) continue;
+            last = j;
+            cum += exp(lm[j] - z);
+            if (u < cum) { k = j; break; }
+          }
+          if (k < 0) k = last;
+        }
+        const double pk = p[k];
+        cond[(long long)s * nsite + site] = pk;
+        if (pk == 0.0) {
+          logp[s] = -INFINITY;
+          done[s] = 1;
+        } else {
+          logp[s] += log(pk);
+        }
+      }
+    }
+  } else if (labels_in) {
+    k = labels_in[gs * nsite + site];
+  }
+  choice[s] = k;
+  labels[(long long)s * nsite + site] = k;
+}
+
+__global__ void make_ptrs_kernel(const int* choice, void* c0, void* c1, void** ptrs, int nb) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nb) return;
+  ptrs[s] = choice[s] == 0 ? c0 : c1;
+}
+
+__global__ void gather_lognorm_kernel(const double* cand_lognorm, const int* labels, int nsite, double* out, int nb) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nb) return;
+  double acc = 0.0;
+  for (int i = 0; i < nsite; ++i) acc += cand_lognorm[2 * i + labels[(long long)s * nsite + i]];
+  out[s] = acc;
+}
+
+// mantissa of the closed boundary when every bond has dimension one
+template <typename T>
+__global__ void close_unit_kernel(T* const* tensors, const long long* sstrides, int L, double* out, int nb) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nb) return;
+  double v = 1.0;
+  for (int x = 0; x < L; ++x) v *= (double)tensors[x][(long long)s * sstrides[x]];
+  out[s] = v;
+}
+
+// ---------------------------------------------------------------------------
+
+template <typename T>
+static void prepare_boundary(Boundary<T>& b, int L, int nb, long long cap_elems) {
+  b.L = L;
+  b.nb = nb;
+  b.bdim.assign(L + 1, 1);
+  b.beven.assign(L + 1, 1);
+  b.bsplit.assign(L + 1, nullptr);
+  b.pdim.assign(L, 1);
+  b.peven.assign(L, 1);
+  b.t.assign(L, nullptr);
+  for (int x = 0; x < L; ++x) SPP_CHECK(hipMalloc(&b.t[x], sizeof(T) * nb * cap_elems));
+  for (int k = 1; k < L; ++k) SPP_CHECK(hipMalloc(&b.bsplit[k], sizeof(int) * nb));
+  SPP_CHECK(hipMalloc(&b.log_scale, sizeof(double) * nb));
+  SPP_CHECK(hipMalloc(&b.max_trunc, sizeof(double) * nb));
+  b.owns = true;
+}
+
+template <typename T>
+static void set_trivial(Context& ctx, Boundary<T>& b, const std::vector<Leg>& phys) {
+  for (int x = 0; x <= b.L; ++x) {
+    b.bdim[x] = 1;
+    b.beven[x] = 1;
+  }
+  for (int k = 1; k < b.L; ++k) SPP_CHECK(launch_fill<int>(b.bsplit[k], b.nb, 1, ctx.stream));
+  for (int x = 0; x < b.L; ++x) {
+    if (phys[x].dim != 1) throw HipError("nontrivial OBC leg");
+    b.pdim[x] = 1;
+    b.peven[x] = phys[x].even;
+    SPP_CHECK(launch_fill<T>(b.t[x], b.nb, T(1), ctx.stream));
+  }
+  SPP_CHECK(launch_fill<double>(b.log_scale, b.nb, 0.0, ctx.stream));
+  SPP_CHECK(launch_fill<double>(b.max_trunc, b.nb, 0.0, ctx.stream));
+}
+
+// Replace per-sample splits of a batch-1 boundary by fixed even counts.
+template <typename T>
+static void freeze_splits(Context& ctx, Boundary<T>& b) {
+  for (int k = 0; k <= b.L; ++k) {
+    if (!b.bsplit[k]) continue;
+    int v = 0;
+    SPP_CHECK(hipMemcpyAsync(&v, b.bsplit[k], sizeof(int), hipMemcpyDeviceToHost, ctx.stream));
+    SPP_CHECK(hipStreamSynchronize(ctx.stream));
+    SPP_CHECK(hipFree(b.bsplit[k]));
+    b.bsplit[k] = nullptr;
+    b.beven[k] = v;
+  }
+}
+
+template <typename T>
+static size_t draw_bytes_per_sample(const CacheDev<T>& c, const Context& ctx) {
+  long long D = 1, chi = c.chi;
+  for (auto& sp : c.spec) D = std::max<long long>(D, std::max(std::max(sp.w.dim, sp.n.dim), std::max(sp.e.dim, sp.s.dim)));
+  const long long Lx = c.Lx;
+  const double ts = sizeof(T);
+  double rowenv = ts * ((Lx + 4) * chi * D * chi + 3.0 * chi * D * D * chi) + 64 * 8;
+  double z = ts * 2.0 * chi * chi * D * D * D;
+  double g = ts * (Lx + 1) * (double)(chi * D) * (chi * D);
+  double sweep = ts * 5.0 * chi * chi * D * D + (8.0 + ts) * 2.0 * (chi * D / 2.0 + 1) * (chi * D / 2.0 + 1) +
+                 8.0 * 2.0 * (4.0 * (chi * D / 2.0 + 1) * ctx.ksub + 3.0 * ctx.ksub * ctx.ksub) +
+                 ts * Lx * chi * chi * D;
+  double absorb = z + g + sweep;
+  return (size_t)(1.15 * std::max(rowenv, absorb)) + 4096;
+}
+
+template <typename T>
+void build_cache(Context& ctx, const StateDev& pepo, int chi, const double* basis, CacheDev<T>& c) {
+  hipStream_t st = ctx.stream;
+  const int Lx = pepo.Lx, Ly = pepo.Ly, ns = Lx * Ly;
+  c.Lx = Lx;
+  c.Ly = Ly;
+  c.chi = chi;
+  c.precision = sizeof(T) == 8 ? 0 : 1;
+  if (basis) std::memcpy(c.basis, basis, sizeof(double) * 4);
+  c.pepo_log_amplitude = pepo.log_amplitude;
+  c.pepo_beta = pepo.beta;
+  c.layers.assign(ns, nullptr);
+  c.cand[0].assign(ns, nullptr);
+  c.cand[1].assign(ns, nullptr);
+  c.spec.resize(ns);
+  std::vector<LayerJob> jobs;
+  long long maxout = 0;
+  for (int i = 0; i < ns; ++i) {
+    const SiteDev& s = pepo.sites[i];
+    if (s.dim[3] != 2 || s.dim[2] != 2) throw HipError("sampling cache needs a PEPO with d = 2 physical and aux legs");
+    LayerSpec sp{pair_leg(s.dim[0], s.even[0]), pair_leg(s.dim[1], s.even[1]), pair_leg(s.dim[4], s.even[4]),
+                 pair_leg(s.dim[5], s.even[5])};
+    c.spec[i] = sp;
+    long long n = (long long)sp.w.dim * sp.n.dim * sp.e.dim * sp.s.dim;
+    maxout = std::max(maxout, n);
+    SPP_CHECK(hipMalloc(&c.layers[i], sizeof(T) * n));
+    SPP_CHECK(hipMalloc(&c.cand[0][i], sizeof(T) * n));
+    SPP_CHECK(hipMalloc(&c.cand[1][i], sizeof(T) * n));
+    for (int v = 0; v < 3; ++v) {
+      LayerJob J{};
+      J.ket = s.d;
+      J.bra = s.d;
+      for (int k = 0; k < 6; ++k) {
+        J.dk[k] = J.db[k] = s.dim[k];
+        J.ek[k] = J.eb[k] = s.even[k];
+      }
+      J.use_op = 0;
+      if (v == 0) {
+        J.out = c.layers[i];
+        J.use_aux = 0;
+      } else {
+        J.out = c.cand[v - 1][i];
+        J.use_aux = 1;
+        J.aux[0] = c.basis[0 * 2 + (v - 1)];
+        J.aux[1] = c.basis[1 * 2 + (v - 1)];
+      }
+      jobs.push_back(J);
+    }
+  }
+  LayerJob* djobs = nullptr;
+  SPP_CHECK(hipMalloc(&djobs, sizeof(LayerJob) * jobs.size()));
+  SPP_CHECK(hipMemcpyAsync(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), hipMemcpyHostToDevice, st));
+  SPP_CHECK(launch_double_layers<T>(djobs, (int)jobs.size(), maxout, st));
+  SPP_CHECK(hipMalloc(&c.cand_lognorm, sizeof(double) * 2 * ns));
+  for (int i = 0; i < ns; ++i) {
+    const SiteDev& s = pepo.sites[i];
+    int nbef = s.dim[0] * s.dim[1] * s.dim[2], naft = s.dim[4] * s.dim[5];
+    hipLaunchKernelGGL(proj_lognorm_kernel, dim3(1), dim3(256), 0, st, s.d, nbef, s.dim[3], naft, c.basis[0],
+                       c.basis[1], c.basis[2], c.basis[3], c.cand_lognorm + 2 * i);
+    SPP_CHECK(hipGetLastError());
+  }
+  // south boundaries, rotated frame (batch 1)
+  if (ctx.arena.mark() == 0) ctx.arena.reserve(std::max(ctx.arena.capacity(), draw_bytes_per_sample(c, ctx)));
+  c.south.resize(Ly + 1);
+  std::vector<Leg> phys(Lx);
+  for (int xr = 0; xr < Lx; ++xr) phys[xr] = c.spec[(Lx - 1 - xr) + (Ly - 1) * Lx].s;
+  prepare_boundary(c.south[Ly], Lx, 1, 1);
+  set_trivial(ctx, c.south[Ly], phys);
+  freeze_splits(ctx, c.south[Ly]);
+  for (int y = Ly - 1; y >= 0; --y) {
+    RowLayers<T> rr;
+    rr.L = Lx;
+    rr.rotated = true;
+    rr.shared.resize(Lx);
+    rr.spec.resize(Lx);
+    long long capmax = 1;
+    for (int xr = 0; xr < Lx; ++xr) {
+      int x = Lx - 1 - xr;
+      const LayerSpec& sp = c.spec[x + y * Lx];
+      rr.shared[xr] = c.layers[x + y * Lx];
+      rr.spec[xr] = LayerSpec{sp.e, sp.s, sp.w, sp.n};
+      capmax = std::max<long long>(capmax, (long long)chi * sp.n.dim * chi);
+    }
+    prepare_boundary(c.south[y], Lx, 1, capmax);
+    absorb_forward<T>(ctx, c.south[y + 1], rr, chi, c.south[y]);
+    freeze_splits(ctx, c.south[y]);
+  }
+  c.south_max_trunc.assign(Ly + 1, 0.0);
+  for (int y = 0; y <= Ly; ++y)
+    SPP_CHECK(hipMemcpyAsync(&c.south_max_trunc[y], c.south[y].max_trunc, sizeof(double), hipMemcpyDeviceToHost, st));
+  SPP_CHECK(hipStreamSynchronize(st));
+  SPP_CHECK(hipFree(djobs));
+}
+
+template void build_cache<float>(Context&, const StateDev&, int, const double*, CacheDev<float>&);
+template void build_cache<double>(Context&, const StateDev&, int, const double*, CacheDev<double>&);
+
+// Reference to the south tensor below original site x with letters (b, s, d).
+template <typename T>
+static TRef bottom_ref(const Boundary<T>& sb, int Lx, int x, const std::string& bsd) {
+  int xr = Lx - 1 - x;
+  std::string stl = {bsd[2], bsd[1], bsd[0]};
+  return dense_ref(sb.t[xr], 0, stl, {sb.bond(xr), sb.phys(xr), sb.bond(xr + 1)});
+}
+template <typename T>
+static Leg south_bond(const Boundary<T>& sb, int Lx, int b) {
+  return sb.bond(Lx - b);
+}
+
+template <typename T>
+void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, const int* labels_in,
+                DrawOutputs& out) {
+  hipStream_t st = ctx.stream;
+  const int Lx = c.Lx, Ly = c.Ly, ns = Lx * Ly, chi = c.chi;
+  if (n <= 0) return;
+  // chunk size from the workspace budget
+  size_t per = draw_bytes_per_sample(c, ctx);
+  size_t freeb = 0, totalb = 0;
+  SPP_CHECK(hipMemGetInfo(&freeb, &totalb));
+  size_t budget = ctx.arena.capacity() > 0 ? std::max(ctx.arena.capacity(), (size_t)(0.6 * freeb))
+                                            : (size_t)(0.6 * freeb);
+  long long chunk = std::max<long long>(1, (long long)(budget / per));
+  if (ctx.chunk > 0) chunk = std::min<long long>(chunk, ctx.chunk);
+  chunk = std::min<long long>(chunk, n);
+  chunk = std::min<long long>(chunk, 16384);
+  ctx.arena.reserve(per * chunk);
+  const int nb = (int)chunk;
+  // persistent per-chunk state
+  long long capmax = 1;
+  for (auto& sp : c.spec) capmax = std::max<long long>(capmax, (long long)chi * sp.s.dim * chi);
+  Boundary<T> NB[2];
+  prepare_boundary(NB[0], Lx, nb, capmax);
+  prepare_boundary(NB[1], Lx, nb, capmax);
+  int *choice = nullptr, *labels = nullptr, *status = nullptr, *fail = nullptr, *done = nullptr;
+  double *logp = nullptr, *cond = nullptr, *raw0 = nullptr, *raw1 = nullptr, *nrm = nullptr, *fac = nullptr;
+  double *lnamp = nullptr, *mant = nullptr;
+  void** ptrs = nullptr;
+  int* dlabels_in = nullptr;
+  double* duni = nullptr;
+  long long* dsst = nullptr;
+  T** dtens = nullptr;
+  SPP_CHECK(hipMalloc(&choice, sizeof(int) * nb));
+  SPP_CHECK(hipMalloc(&labels, sizeof(int) * nb * ns));
+  SPP_CHECK(hipMalloc(&status, sizeof(int) * nb));
+  SPP_CHECK(hipMalloc(&fail, sizeof(int) * nb));
+  SPP_CHECK(hipMalloc(&done, sizeof(int) * nb));
+  SPP_CHECK(hipMalloc(&logp, sizeof(double) * nb));
+  SPP_CHECK(hipMalloc(&cond, sizeof(double) * nb * ns));
+  SPP_CHECK(hipMalloc(&raw0, sizeof(double) * nb));
+  SPP_CHECK(hipMalloc(&raw1, sizeof(double) * nb));
+  SPP_CHECK(hipMalloc(&nrm, sizeof(double) * nb));
+  SPP_CHECK(hipMalloc(&fac, sizeof(double) * nb));
+  SPP_CHECK(hipMalloc(&lnamp, sizeof(double) * nb));
+  SPP_CHECK(hipMalloc(&mant, sizeof(double) * nb));
+  SPP_CHECK(hipMalloc(&ptrs, sizeof(void*) * nb * Lx));
+  SPP_CHECK(hipMalloc(&dsst, sizeof(long long) * Lx));
+  SPP_CHECK(hipMalloc(&dtens, sizeof(T*) * Lx));
+  if (labels_in) SPP_CHECK(hipMalloc(&dlabels_in, sizeof(int) * (long long)n * ns));
+  if (uniforms) SPP_CHECK(hipMalloc(&duni, sizeof(double) * (long long)n * ns));
+  if (labels_in)
+    SPP_CHECK(hipMemcpyAsync(dlabels_in, labels_in, sizeof(int) * (long long)n * ns, hipMemcpyHostToDevice, st));
+  if (uniforms)
+    SPP_CHECK(hipMemcpyAsync(duni, uniforms, sizeof(double) * (long long)n * ns, hipMemcpyHostToDevice, st));
+  const int tb = 256, gb = (nb + tb - 1) / tb;
+  std::vector<int> h_status(nb), h_fail(nb), h_labels((size_t)nb * ns);
+  std::vector<double> h_logp(nb), h_cond((size_t)nb * ns), h_mt(nb), h_mant(nb), h_ls(nb), h_amp(nb);
+  double south_mt = 0.0;
+  for (double v : c.south_max_trunc) south_mt = std::max(south_mt, v);
+  try {
+    for (long long s0 = 0; s0 < n; s0 += nb) {
+      const int cur = (int)std::min<long long>(nb, n - s0);
+      NB[0].nb = cur;
+      NB[1].nb = cur;
+      // per-chunk initialisation
+      std::vector<Leg> phys0(Lx);
+      for (int x = 0; x < Lx; ++x) phys0[x] = c.spec[x].n;
+      set_trivial(ctx, NB[0], phys0);
+      int which = 0;
+      SPP_CHECK(launch_fill<int>(status, nb, 0, st));
+      SPP_CHECK(launch_fill<int>(fail, nb, -1, st));
+      SPP_CHECK(launch_fill<int>(done, nb, 0, st));
+      SPP_CHECK(launch_fill<double>(logp, nb, 0.0, st));
+      SPP_CHECK(launch_fill<double>(cond, (long long)nb * ns, 0.0, st));
+      SPP_CHECK(launch_fill<int>(choice, nb, 0, st));
+      for (int y = 0; y < Ly; ++y) {
+        Boundary<T>& Nc = NB[which];
+        const Boundary<T>& sb = c.south[y + 1];
+        const size_t mark = ctx.arena.mark();
+        // ---------------- right environments ----------------
+        std::vector<T*> R(Lx + 1);
+        std::vector<std::vector<Leg>> Rl(Lx + 1);
+        std::vector<long long> Rs(Lx + 1);
+        for (int x = 0; x <= Lx; ++x) {
+          Leg wl = x < Lx ? c.spec[x + y * Lx].w : c.spec[(Lx - 1) + y * Lx].e;
+          Rl[x] = {Nc.bond(x), wl, south_bond(sb, Lx, x)};
+          Rs[x] = (long long)Rl[x][0].dim * Rl[x][1].dim * Rl[x][2].dim;
+          R[x] = (T*)ctx.arena.take(sizeof(T) * cur * Rs[x]);
+        }
+        SPP_CHECK(launch_fill<T>(R[Lx], cur, T(1), st));
+        for (int x = Lx - 1; x >= 0; --x) {
+          const LayerSpec& sp = c.spec[x + y * Lx];
+          const size_t xm = ctx.arena.mark();
+          long long xs = (long long)Nc.bdim[x] * sp.n.dim * sp.e.dim * Rl[x + 1][2].dim;
+          T* X = (T*)ctx.arena.take(sizeof(T) * cur * xs);
+          TRef Xr = dense_ref(X, xs, "aned", {Nc.bond(x), sp.n, sp.e, Rl[x + 1][2]});
+          contract<T>(ctx, Xr, Nc.ref(x, "anc"), dense_ref(R[x + 1], Rs[x + 1], "ced", Rl[x + 1]), cur);
+          long long ys = (long long)Nc.bdim[x] * Rl[x + 1][2].dim * sp.w.dim * sp.s.dim;
+          T* Y = (T*)ctx.arena.take(sizeof(T) * cur * ys);
+          TRef Yr = dense_ref(Y, ys, "adws", {Nc.bond(x), Rl[x + 1][2], sp.w, sp.s});
+          contract<T>(ctx, Yr, Xr, dense_ref(c.layers[x + y * Lx], 0, "wnes", {sp.w, sp.n, sp.e, sp.s}), cur);
+          TRef Rr = dense_ref(R[x], Rs[x], "awb", Rl[x]);
+          contract<T>(ctx, Rr, Yr, bottom_ref(sb, Lx, x, "bsd"), cur);
+          FullView fv = full_view(Rr);
+          SPP_CHECK(launch_dot<T>(fv, fv, nrm, cur, st));
+          hipLaunchKernelGGL(safe_scale_kernel, dim3(gb), dim3(tb), 0, st, nrm, fac, status, fail, x + y * Lx, cur);
+          SPP_CHECK(hipGetLastError());
+          SPP_CHECK(launch_scale<T>(fv, fac, cur, st));
+          ctx.arena.release(xm);
+        }
+        // ---------------- left sweep with draws ----------------
+        long long lmax = 1;
+        for (int x = 0; x <= Lx; ++x) lmax = std::max(lmax, Rs[x]);
+        T* Lb = (T*)ctx.arena.take(sizeof(T) * cur * lmax);
+        std::vector<Leg> Ll = {Nc.bond(0), c.spec[y * Lx].w, south_bond(sb, Lx, 0)};
+        long long Ls = 1;
+        SPP_CHECK(launch_fill<T>(Lb, cur, T(1), st));
+        for (int x = 0; x < Lx; ++x) {
+          const int site = x + y * Lx;
+          const LayerSpec& sp = c.spec[site];
+          const size_t xm = ctx.arena.mark();
+          Leg sbl = south_bond(sb, Lx, x), sbr = south_bond(sb, Lx, x + 1);
+          long long xs = (long long)sp.w.dim * sbl.dim * sp.n.dim * Nc.bdim[x + 1];
+          T* X = (T*)ctx.arena.take(sizeof(T) * cur * xs);
+          TRef Xr = dense_ref(X, xs, "wbnc", {sp.w, sbl, sp.n, Nc.bond(x + 1)});
+          contract<T>(ctx, Xr, dense_ref(Lb, Ls, "awb", Ll), Nc.ref(x, "anc"), cur);
+          long long ys = (long long)sbl.dim * Nc.bdim[x + 1] * sp.e.dim * sp.s.dim;
+          long long os = (long long)Nc.bdim[x + 1] * sp.e.dim * sbr.dim;
+          T* Yk = (T*)ctx.arena.take(sizeof(T) * cur * ys);
+          T* O[2];
+          O[0] = (T*)ctx.arena.take(sizeof(T) * cur * os);
+          O[1] = (T*)ctx.arena.take(sizeof(T) * cur * os);
+          std::vector<Leg> Ol = {Nc.bond(x + 1), sp.e, sbr};
+          double* rawk[2] = {raw0, raw1};
+          for (int k = 0; k < 2; ++k) {
+            TRef Yr = dense_ref(Yk, ys, "bces", {sbl, Nc.bond(x + 1), sp.e, sp.s});
+            contract<T>(ctx, Yr, Xr, dense_ref(c.cand[k][site], 0, "wnes", {sp.w, sp.n, sp.e, sp.s}), cur);
+            TRef Or = dense_ref(O[k], os, "ced", Ol);
+            contract<T>(ctx, Or, Yr, bottom_ref(sb, Lx, x, "bsd"), cur);
+            SPP_CHECK(launch_dot<T>(full_view(Or), full_view(dense_ref(R[x + 1], Rs[x + 1], "ced", Rl[x + 1])),
+                                    rawk[k], cur, st));
+          }
+          hipLaunchKernelGGL(born_select_kernel, dim3(gb), dim3(tb), 0, st, raw0, raw1, duni, dlabels_in, s0, ns,
+                             site, choice, labels, logp, cond, status, fail, done, cur);
+          SPP_CHECK(hipGetLastError());
+          hipLaunchKernelGGL(make_ptrs_kernel, dim3(gb), dim3(tb), 0, st, choice, (void*)c.cand[0][site],
+                             (void*)c.cand[1][site], ptrs + (long long)x * cur, cur);
+          SPP_CHECK(hipGetLastError());
+          if (x < Lx - 1) {
+            // left <- safe_scale(left_step of the drawn candidate)
+            TRef Lnew = dense_ref(Lb, os, "ced", Ol);
+            SPP_CHECK(launch_select_copy<T>(full_view(Lnew), full_view(dense_ref(O[0], os, "ced", Ol)),
+                                            full_view(dense_ref(O[1], os, "ced", Ol)), choice, nullptr, cur, st));
+            FullView fv = full_view(Lnew);
+            SPP_CHECK(launch_dot<T>(fv, fv, nrm, cur, st));
+            hipLaunchKernelGGL(safe_scale_kernel, dim3(gb), dim3(tb), 0, st, nrm, fac, status, fail, site, cur);
+            SPP_CHECK(hipGetLastError());
+            SPP_CHECK(launch_scale<T>(fv, fac, cur, st));
+            Ll = Ol;
+            Ls = os;
+          }
+          ctx.arena.release(xm);
+        }
+        ctx.arena.release(mark);
+        // ---------------- absorb the drawn row ----------------
+        RowLayers<T> row;
+        row.L = Lx;
+        row.spec.assign(c.spec.begin() + y * Lx, c.spec.begin() + (y + 1) * Lx);
+        row.ptrs.resize(Lx);
+        for (int x = 0; x < Lx; ++x) row.ptrs[x] = ptrs + (long long)x * cur;
+        absorb_forward<T>(ctx, Nc, row, chi, NB[1 - which]);
+        which = 1 - which;
+      }
+      // lognorm of the normalised projected sample from the final north boundary
+      Boundary<T>& Nf = NB[which];
+      for (int x = 0; x <= Lx; ++x)
+        if (Nf.bdim[x] != 1) throw HipError("final boundary must have unit bonds");
+      std::vector<long long> sst(Lx);
+      for (int x = 0; x < Lx; ++x) sst[x] = Nf.tsize(x);
+      SPP_CHECK(hipMemcpyAsync(dsst, sst.data(), sizeof(long long) * Lx, hipMemcpyHostToDevice, st));
+      SPP_CHECK(hipMemcpyAsync(dtens, Nf.t.data(), sizeof(T*) * Lx, hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(close_unit_kernel<T>, dim3(gb), dim3(tb), 0, st, dtens, dsst, Lx, mant, cur);
+      SPP_CHECK(hipGetLastError());
+      hipLaunchKernelGGL(gather_lognorm_kernel, dim3(gb), dim3(tb), 0, st, c.cand_lognorm, labels, ns, lnamp, cur);
+      SPP_CHECK(hipGetLastError());
+      SPP_CHECK(hipMemcpyAsync(h_status.data(), status, sizeof(int) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_fail.data(), fail, sizeof(int) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_labels.data(), labels, sizeof(int) * cur * ns, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_logp.data(), logp, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_cond.data(), cond, sizeof(double) * cur * ns, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_mt.data(), Nf.max_trunc, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_mant.data(), mant, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_ls.data(), Nf.log_scale, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_amp.data(), lnamp, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipStreamSynchronize(st));
+      for (int s = 0; s < cur; ++s) {
+        long long g = s0 + s;
+        if (out.labels) std::memcpy(out.labels + g * ns, h_labels.data() + (size_t)s * ns, sizeof(int) * ns);
+        if (out.conditionals)
+          std::memcpy(out.conditionals + g * ns, h_cond.data() + (size_t)s * ns, sizeof(double) * ns);
+        if (out.log_prob) out.log_prob[g] = h_logp[s];
+        if (out.max_trunc) out.max_trunc[g] = std::max(south_mt, h_mt[s]);
+        if (out.status) out.status[g] = h_status[s];
+        if (out.fail_site) out.fail_site[g] = h_fail[s];
+        // normalised projected PEPS: log_amplitude = pepo + sum log||A_k||,
+        // tensor_log_norm = log(mantissa) + log_scale - 2 sum log||A_k||
+        if (out.log_amplitude) out.log_amplitude[g] = c.pepo_log_amplitude + h_amp[s];
+        if (out.tensor_log_norm) {
+          double m = h_mant[s];
+          out.tensor_log_norm[g] = (m > 0.0 && std::isfinite(m)) ? std::log(m) + h_ls[s] - 2.0 * h_amp[s] : NAN;
+        }
+      }
+    }
+  } catch (...) {
+    NB[0].release();
+    NB[1].release();
+    for (void* p : {(void*)choice, (void*)labels, (void*)status, (void*)fail, (void*)done, (void*)logp, (void*)cond,
+                    (void*)raw0, (void*)raw1, (void*)nrm, (void*)fac, (void*)lnamp, (void*)mant, (void*)ptrs,
+                    (void*)dlabels_in, (void*)duni, (void*)dsst, (void*)dtens})
+      if (p) (void)hipFree(p);
+    ctx.arena.release(0);
+    throw;
+  }
+  NB[0].release();
+  NB[1].release();
+  for (void* p : {(void*)choice, (void*)labels, (void*)status, (void*)fail, (void*)done, (void*)logp, (void*)cond,
+                  (void*)raw0, (void*)raw1, (void*)nrm, (void*)fac, (void*)lnamp, (void*)mant, (void*)ptrs,
+                  (void*)dlabels_in, (void*)duni, (void*)dsst, (void*)dtens})
+    if (p) SPP_CHECK(hipFree(p));
+}
+
+template void draw_batch<float>(Context&, CacheDev<float>&, int, const double*, const int*, DrawOutputs&);
+template void draw_batch<double>(Context&, CacheDev<double>&, int, const double*, const int*, DrawOutputs&);
+
+}  // namespace spp

@@ -1,0 +1,271 @@
+"""Born sampling API on the MI355X library, mirroring StoPEPS.jl
+(src/sampling/born.jl, src/contraction/boundary_mps.jl).
+
+``SamplingCache`` and ``sample_configuration`` keep the reference's names,
+arguments and error messages; ``sample_configurations`` is the batched form
+that replaces the per-sample loop of ``_initial_particles``
+(src/driver/run.jl:273-294). All contraction work runs in the HIP library;
+this module only marshals arrays.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .states import parity_sorted, resolve_basis, FiniteState, Site
+
+MASK64 = (1 << 64) - 1
+PRECISIONS = {"fp64": 0, "fp32": 1}
+
+
+class BornError(ValueError):
+    """Numerical failure of a Born draw (reference messages, born.jl:137-142)."""
+
+
+class Context:
+    """One device context (one HIP stream) of the library."""
+
+    def __init__(self, device=0, precision="fp64", **options):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self._lib.spp_create(int(device), ctypes.byref(h))
+        if rc != 0:
+            raise _lib.HIPLibraryError(f"spp_create failed ({rc}): no usable HIP device")
+        self.handle = h
+        self.set_precision(precision)
+        for k, v in options.items():
+            self.set_option(k, v)
+
+    def set_precision(self, precision):
+        self.precision = precision
+        _lib.check(self.handle, self._lib.spp_set_precision(self.handle, PRECISIONS[precision]))
+
+    def set_option(self, name, value):
+        _lib.check(self.handle, self._lib.spp_set_option(self.handle, name.encode(), float(value)))
+
+    def counters(self):
+        out = np.zeros(4)
+        self._lib.spp_get_counters(self.handle, _lib.dptr(out), 4)
+        return {"gemm_flops": out[0], "gemm_launches": out[1], "subspace_iterations": out[2],
+                "workspace_peak_bytes": out[3]}
+
+    def reset_counters(self):
+        self._lib.spp_reset_counters(self.handle)
+
+    def synchronize(self):
+        _lib.check(self.handle, self._lib.spp_synchronize(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.spp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def gemm(self, A, B, precision="fp64"):
+        """Self-test: C = A @ B through the library's matrix-core kernel."""
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        M, K = A.shape
+        K2, N = B.shape
+        if K != K2:
+            raise ValueError("shape mismatch")
+        C = np.zeros((M, N))
+        _lib.check(self.handle, self._lib.spp_selftest_gemm(self.handle, PRECISIONS[precision], M, N, K,
+                                                             _lib.dptr(A), _lib.dptr(B), _lib.dptr(C)))
+        return C
+
+
+class DeviceState:
+    """A ``FiniteState`` uploaded to the device (``spp_state_upload``)."""
+
+    def __init__(self, ctx, state):
+        self.ctx = ctx
+        self.state = state
+        L = state.lattice
+        dims, evens, arrays = [], [], []
+        for (x, y) in L.site_order():
+            a, ev = parity_sorted(state[(x, y)])
+            arrays.append(a)
+            dims += list(a.shape)
+            evens += ev
+        self._arrays = arrays
+        dims = np.asarray(dims, dtype=np.int32)
+        evens = np.asarray(evens, dtype=np.int32)
+        ptrs = (_lib.c_double_p * len(arrays))(*[_lib.dptr(a) for a in arrays])
+        h = ctypes.c_void_p()
+        kind = 0 if state.kind == "pepo" else 1
+        _lib.check(ctx.handle, ctx._lib.spp_state_upload(ctx.handle, L.Lx, L.Ly, kind, _lib.iptr(dims),
+                                                         _lib.iptr(evens), ptrs, state.beta,
+                                                         state.log_amplitude, ctypes.byref(h)))
+        self.handle = h
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self.ctx._lib.spp_state_free(self.handle)
+        except Exception:
+            pass
+
+
+class SamplingCache:
+    """``SamplingCache(state, BoundaryMPS(chi); kind=:auxiliary, basis=:auto)``
+    (src/sampling/born.jl:45-73). The state must be a PEPO."""
+
+    def __init__(self, ctx, state, chi, basis=None, kind="auxiliary"):
+        if kind != "auxiliary":
+            raise NotImplementedError("physical-kind sampling (resampling proposals) is not part of this library")
+        if state.kind != "pepo":
+            raise ValueError("the auxiliary-kind sampling cache needs a PEPO")
+        self.ctx = ctx
+        self.state = state
+        self.chi = int(chi)
+        self.basis = resolve_basis(state, basis)
+        self.lattice = state.lattice
+        self._dev = DeviceState(ctx, state)
+        U = np.ascontiguousarray(self.basis.U, dtype=np.float64)
+        h = ctypes.c_void_p()
+        _lib.check(ctx.handle, ctx._lib.spp_sampling_cache_build(ctx.handle, self._dev.handle, self.chi,
+                                                                 _lib.dptr(U), ctypes.byref(h)))
+        self.handle = h
+        smt = np.zeros(self.lattice.Ly + 1)
+        ctx._lib.spp_sampling_cache_info(self.handle, _lib.dptr(smt))
+        self.south_max_truncation = smt
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self.ctx._lib.spp_sampling_cache_free(self.handle)
+        except Exception:
+            pass
+
+
+def splitmix64(v):
+    """``_splitmix64`` (src/sampling/born.jl:1-6)."""
+    v = (v + 0x9E3779B97F4A7C15) & MASK64
+    v = ((v ^ (v >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+    v = ((v ^ (v >> 27)) * 0x94D049BB133111EB) & MASK64
+    return v ^ (v >> 31)
+
+
+def sample_uniforms(seed, index, nsites):
+    """Per-site uniforms of sample ``index`` (1-based) for ``seed``: the
+    documented counter-based stream that stands in for ``sample_rng(seed, index)``
+    (born.jl:13-22) where Julia's MersenneTwister is unavailable:
+    u_k = (splitmix64(splitmix64(splitmix64(seed) ^ index) ^ k) >> 11) * 2^-53."""
+    if seed < 0 or index < 1:
+        raise ValueError("invalid random stream identity")
+    base = splitmix64(splitmix64(seed & MASK64) ^ (index & MASK64))
+    return np.array([(splitmix64(base ^ k) >> 11) * 2.0 ** -53 for k in range(nsites)])
+
+
+def _raise_status(cache, status, fail):
+    bad = np.nonzero(status)[0]
+    if len(bad) == 0:
+        return
+    i = int(bad[0])
+    site = int(fail[i])
+    x, y = site % cache.lattice.Lx + 1, site // cache.lattice.Lx + 1
+    if status[i] == 1:
+        raise BornError(f"invalid Born conditional at ({x},{y}); increase chi_sampling")
+    if status[i] == 2:
+        raise BornError(f"Born conditional has zero total mass at ({x},{y})")
+    raise BornError("nonfinite row environment")
+
+
+def sample_configurations(cache, uniforms=None, configurations=None, check=True):
+    """Batched ``sample_configuration`` over n samples.
+
+    ``uniforms``: array [n, Lx*Ly] (row-major sites, x fastest). ``configurations``:
+    evaluation mode, array [n, Lx, Ly] of basis labels. Returns a dict of arrays:
+    configuration [n, Lx, Ly] (labels), log_probability [n], conditionals
+    [n, Lx*Ly], max_environment_truncation [n], tensor_log_norm [n] and
+    log_amplitude [n] of the projected normalised PEPS (at the sampling chi),
+    status [n], fail_site [n].
+    """
+    L = cache.lattice
+    ns = L.Lx * L.Ly
+    if uniforms is None and configurations is None:
+        raise ValueError("an explicit RNG is required")
+    labels_in = None
+    if configurations is not None:
+        cfg = np.asarray(configurations)
+        n = cfg.shape[0]
+        if cfg.shape[1:] != (L.Lx, L.Ly):
+            raise ValueError("configuration lattice")
+        idx = np.vectorize(cache.basis.index)(cfg)
+        labels_in = np.ascontiguousarray(np.transpose(idx, (0, 2, 1)).reshape(n, ns), dtype=np.int32)
+        uni = None
+    else:
+        uni = np.ascontiguousarray(uniforms, dtype=np.float64)
+        n = uni.shape[0]
+        if uni.shape != (n, ns):
+            raise ValueError("uniforms must have shape [n, Lx*Ly]")
+    labels = np.zeros((n, ns), dtype=np.int32)
+    logp = np.zeros(n)
+    cond = np.zeros((n, ns))
+    mt = np.zeros(n)
+    tln = np.zeros(n)
+    lamp = np.zeros(n)
+    status = np.zeros(n, dtype=np.int32)
+    fail = np.zeros(n, dtype=np.int32)
+    ctx = cache.ctx
+    _lib.check(ctx.handle, ctx._lib.spp_born_sample_batch(
+        ctx.handle, cache.handle, n, _lib.dptr(uni), _lib.iptr(labels_in), _lib.iptr(labels), _lib.dptr(logp),
+        _lib.dptr(cond), _lib.dptr(mt), _lib.dptr(tln), _lib.dptr(lamp), _lib.iptr(status), _lib.iptr(fail)))
+    if check:
+        _raise_status(cache, status, fail)
+    lab = np.asarray(cache.basis.labels)[labels].reshape(n, L.Ly, L.Lx).transpose(0, 2, 1)
+    return {"configuration": lab, "log_probability": logp, "conditionals": cond,
+            "max_environment_truncation": mt, "tensor_log_norm": tln, "log_amplitude": lamp,
+            "status": status, "fail_site": fail}
+
+
+def sample_configuration(cache, uniforms=None, configuration=None):
+    """``sample_configuration(cache; rng | configuration)`` (born.jl:86-163) for
+    one sample; ``uniforms`` replaces ``rng`` (one value per site, row-major)."""
+    if configuration is not None:
+        r = sample_configurations(cache, configurations=np.asarray(configuration)[None])
+    else:
+        if uniforms is None:
+            raise ValueError("an explicit RNG is required")
+        r = sample_configurations(cache, uniforms=np.asarray(uniforms)[None])
+    lp = float(r["log_probability"][0])
+    cond = list(r["conditionals"][0])
+    if lp == -np.inf:
+        k = int(np.argmin(np.asarray(cond) > 0))
+        cond = cond[:k + 1]
+    return {"configuration": r["configuration"][0], "log_probability": lp, "conditionals": cond,
+            "max_environment_truncation": float(r["max_environment_truncation"][0]), "repairs": []}
+
+
+def project_configuration(pepo, configuration, basis=None):
+    """``project_configuration`` (born.jl:27-36): aux projection on the host
+    (a slice per site) followed by ``normalize_state!`` (finite.jl:89-100)."""
+    basis = resolve_basis(pepo, basis)
+    L = pepo.lattice
+    sites = [[None] * L.Ly for _ in range(L.Lx)]
+    for x in range(L.Lx):
+        for y in range(L.Ly):
+            A = pepo[(x, y)]
+            k = basis.index(configuration[x][y])
+            a = np.tensordot(A.data, basis.U[:, k], axes=([3], [0]))[:, :, :, None, :, :]
+            a = np.moveaxis(a, 3, 3)
+            charge = basis.charges[k] if np.any(A.parity[3]) else 0
+            par = list(A.parity)
+            par[3] = np.array([charge])
+            sites[x][y] = Site(np.ascontiguousarray(a), par)
+    st = FiniteState(L, sites, "peps", beta=pepo.beta, log_amplitude=pepo.log_amplitude)
+    records = []
+    for (x, y) in L.site_order():
+        v = float(np.linalg.norm(st[(x, y)].data))
+        if not (np.isfinite(v) and v > 0):
+            raise ValueError(f"zero or nonfinite tensor at {(x + 1, y + 1)}")
+        st.sites[x][y] = Site(st[(x, y)].data / v, st[(x, y)].parity)
+        st.log_amplitude += np.log(v)
+        records.append({"site": (x + 1, y + 1), "log_normalization": float(np.log(v))})
+    return st, records

@@ -1,0 +1,140 @@
+"""Host-side state model mirroring StoPEPS.jl's L1 layer (src/states/finite.jl,
+src/models/tfim.jl, src/sampling/basis.jl).
+
+A site tensor is a dense ``numpy`` array over the legs (W, N, P, Aux, E, S)
+with one Z2 parity vector per leg (all zeros for ``ComplexSpace``). The device
+layout wants every leg sorted by parity; :func:`parity_sorted` produces it.
+"""
+import numpy as np
+
+
+class Lattice:
+    """``Lattice(Lx, Ly)`` (src/states/finite.jl:7-16)."""
+
+    def __init__(self, Lx, Ly):
+        if int(Lx) <= 0 or int(Ly) <= 0:
+            raise ValueError("lattice dimensions must be positive")
+        self.Lx, self.Ly = int(Lx), int(Ly)
+
+    def __len__(self):
+        return self.Lx * self.Ly
+
+    def site_order(self):
+        """Row-major order, x fastest (src/states/finite.jl:16)."""
+        return [(x, y) for y in range(self.Ly) for x in range(self.Lx)]
+
+
+class Site:
+    """One site tensor: ``data`` over (W,N,P,Aux,E,S) and per-leg parity vectors."""
+
+    __slots__ = ("data", "parity")
+
+    def __init__(self, data, parity):
+        self.data = np.asarray(data, dtype=np.float64)
+        self.parity = [np.asarray(p, dtype=np.int64) % 2 for p in parity]
+        if self.data.ndim != 6 or len(self.parity) != 6:
+            raise ValueError("expected six local legs")
+        for p, n in zip(self.parity, self.data.shape):
+            if len(p) != n:
+                raise ValueError("parity vector length mismatch")
+
+
+class FiniteState:
+    """``FiniteState{K}`` (src/states/finite.jl:27-44); kind "pepo" or "peps"."""
+
+    def __init__(self, lattice, sites, kind, beta=0.0, log_amplitude=0.0):
+        if kind not in ("pepo", "peps"):
+            raise ValueError("kind must be 'pepo' or 'peps'")
+        self.lattice = lattice
+        self.sites = sites  # sites[x][y] -> Site
+        self.kind = kind
+        self.beta = float(beta)
+        self.log_amplitude = float(log_amplitude)
+        validate_state(self)
+
+    def __getitem__(self, i):
+        return self.sites[i[0]][i[1]]
+
+
+FinitePEPO = lambda lattice, sites, **kw: FiniteState(lattice, sites, "pepo", **kw)  # noqa: E731
+FinitePEPS = lambda lattice, sites, **kw: FiniteState(lattice, sites, "peps", **kw)  # noqa: E731
+
+
+def validate_state(s):
+    """``validate_state`` (src/states/finite.jl:53-73)."""
+    l = s.lattice
+    for (x, y) in l.site_order():
+        A = s[(x, y)]
+        if not np.all(np.isfinite(A.data)):
+            raise ValueError(f"nonfinite tensor at {(x + 1, y + 1)}")
+        if s.kind == "peps" and A.data.shape[3] != 1:
+            raise ValueError("PEPS auxiliary leg must contain one irreducible carrier")
+        for edge, axis in ((x == 0, 0), (y == 0, 1), (x == l.Lx - 1, 4), (y == l.Ly - 1, 5)):
+            if edge and A.data.shape[axis] != 1:
+                raise ValueError(f"nontrivial OBC leg at {(x + 1, y + 1)}")
+        if x < l.Lx - 1 and not np.array_equal(A.parity[4], s[(x + 1, y)].parity[0]):
+            raise ValueError("horizontal space mismatch")
+        if y < l.Ly - 1 and not np.array_equal(A.parity[5], s[(x, y + 1)].parity[1]):
+            raise ValueError("vertical space mismatch")
+    return s
+
+
+def identity_pepo(lattice, symmetric=True):
+    """``identity_pepo`` (src/states/finite.jl:80-86)."""
+    pp = np.array([0, 1]) if symmetric else np.array([0, 0])
+    u = np.array([0])
+    sites = []
+    for x in range(lattice.Lx):
+        col = []
+        for y in range(lattice.Ly):
+            a = np.zeros((1, 1, 2, 2, 1, 1))
+            a[0, 0, :, :, 0, 0] = np.eye(2)
+            col.append(Site(a, [u, u, pp, pp, u, u]))
+        sites.append(col)
+    return FiniteState(lattice, sites, "pepo")
+
+
+def parity_sorted(site):
+    """Dense array with every leg sorted by parity (stable), and even counts."""
+    a = site.data
+    evens = []
+    for k, p in enumerate(site.parity):
+        order = np.argsort(p, kind="stable")
+        a = np.take(a, order, axis=k)
+        evens.append(int(np.sum(p == 0)))
+    return np.ascontiguousarray(a), evens
+
+
+class LocalBasis:
+    """Local sampling basis (``LocalSamplingBasis``, src/sampling/basis.jl:29-63):
+    columns of ``U`` are basis kets; ``labels`` their configuration labels."""
+
+    def __init__(self, U, labels, charges):
+        self.U = np.asarray(U, dtype=np.float64)
+        self.labels = list(labels)
+        self.charges = list(charges)
+
+    def index(self, label):
+        """``_basis_index`` (basis.jl:88-92), 0-based."""
+        try:
+            return self.labels.index(label)
+        except ValueError:
+            raise ValueError("configuration contains an unknown basis label")
+
+
+def SymmetryBasis():
+    """Z2 parity sectors: U = I, labels 1, 2 for charges 0, 1 (basis.jl:15, 51-57)."""
+    return LocalBasis(np.eye(2), [1, 2], [0, 1])
+
+
+def XBasis():
+    """``XBasis()`` (basis.jl:19)."""
+    return LocalBasis(np.array([[1.0, 1.0], [-1.0, 1.0]]) / np.sqrt(2.0), [-1, 1], [0, 0])
+
+
+def resolve_basis(state, basis=None):
+    """``_resolve_basis(:auto)`` (basis.jl:64-80): Z2 -> SymmetryBasis, dense -> XBasis."""
+    if basis is not None:
+        return basis
+    symmetric = bool(np.any(state[(0, 0)].parity[2]))
+    return SymmetryBasis() if symmetric else XBasis()

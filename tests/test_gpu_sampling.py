@@ -1,0 +1,112 @@
+"""GPU parity of the batched Born draw (HIP library, through the C ABI) against
+the FP64 oracle on the same seeded inputs and injected uniforms."""
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import to_product_state, oracle_cfg
+from oracle.model import Lattice, embed_identity_pepo, random_state
+from oracle import born as ob
+from oracle import boundary as obm
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+@pytest.mark.parametrize("shape", [(7, 5, 3), (130, 70, 37), (256, 300, 64)])
+def test_gemm_selftest(spp, gpu_ctx, precision, tol, shape):
+    say("gemm selftest", precision, shape)
+    M, N, K = shape
+    rng = np.random.default_rng(M * 7 + N)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    C = gpu_ctx.gemm(A, B, precision)
+    ref = A @ B
+    err = np.max(np.abs(C - ref)) / np.max(np.abs(ref))
+    assert err < tol, err
+
+
+def _compare_draws(spp, ctx, pepo, chi, nsamp, seed, tol_logp, tol_cond, precision="fp64", check_norm=True):
+    L = pepo.lattice
+    ns = L.Lx * L.Ly
+    ctx.set_precision(precision)
+    state = to_product_state(spp, pepo)
+    cache = spp.SamplingCache(ctx, state, chi)
+    uni = np.stack([spp.sample_uniforms(seed, i, ns) for i in range(1, nsamp + 1)])
+    got = spp.sample_configurations(cache, uniforms=uni)
+    ocache = ob.SamplingCache(pepo, chi)
+    np.testing.assert_allclose(cache.south_max_truncation, [e.max_truncation for e in ocache.south],
+                               atol=1e-10)
+    mism = 0
+    for i in range(nsamp):
+        ref = ob.sample_configuration(ocache, uniforms=uni[i])
+        cfg = oracle_cfg(ref, L)
+        if not np.array_equal(cfg, got["configuration"][i]):
+            mism += 1
+            continue
+        assert abs(ref["log_probability"] - got["log_probability"][i]) <= tol_logp * max(1, abs(ref["log_probability"]))
+        np.testing.assert_allclose(got["conditionals"][i], ref["conditionals"], rtol=tol_cond, atol=tol_cond)
+        assert abs(ref["max_environment_truncation"] - got["max_environment_truncation"][i]) < max(1e-9, 1e-3 * ref["max_environment_truncation"])
+        if check_norm:
+            ps, _ = ob.project_configuration(pepo, ref["configuration"])
+            ln = obm.lognorm(ps, chi)
+            assert abs(ln["tensor_log_norm"] - got["tensor_log_norm"][i]) < max(tol_logp, 1e-10) * 10
+            assert abs(ps.log_amplitude - got["log_amplitude"][i]) < 1e-10
+    return mism
+
+
+def test_draw_exact_chi_fp64(spp, gpu_ctx):
+    say("draw exact chi fp64")
+    L = Lattice(3, 3)
+    pepo = embed_identity_pepo(L, (1, 1), 0.5, seed=11)
+    assert _compare_draws(spp, gpu_ctx, pepo, 16, 6, 1, 1e-11, 1e-11) == 0
+
+
+def test_draw_truncated_fp64(spp, gpu_ctx):
+    say("draw truncated fp64")
+    L = Lattice(4, 4)
+    pepo = embed_identity_pepo(L, (1, 1), 0.6, seed=12)
+    assert _compare_draws(spp, gpu_ctx, pepo, 6, 8, 2, 1e-9, 1e-9) == 0
+
+
+def test_draw_truncated_fp64_d3(spp, gpu_ctx):
+    say("draw truncated fp64 D=(2,1) uneven")
+    L = Lattice(3, 4)
+    pepo = embed_identity_pepo(L, (2, 1), 0.4, seed=13)
+    assert _compare_draws(spp, gpu_ctx, pepo, 10, 6, 3, 1e-9, 1e-9) == 0
+
+
+def test_draw_dense_xbasis_fp64(spp, gpu_ctx):
+    say("draw dense X basis fp64")
+    L = Lattice(3, 3)
+    pepo = embed_identity_pepo(L, 2, 0.5, seed=14, symmetric=False)
+    assert _compare_draws(spp, gpu_ctx, pepo, 8, 6, 4, 1e-10, 1e-10) == 0
+
+
+def test_draw_fp32(spp, gpu_ctx):
+    say("draw truncated fp32")
+    L = Lattice(4, 4)
+    pepo = embed_identity_pepo(L, (1, 1), 0.6, seed=12)
+    mism = _compare_draws(spp, gpu_ctx, pepo, 6, 8, 2, 2e-5, 2e-5, precision="fp32")
+    gpu_ctx.set_precision("fp64")
+    assert mism <= 1
+
+
+def test_evaluation_mode_sums_to_one(spp, gpu_ctx):
+    say("evaluation mode sum p = 1")
+    gpu_ctx.set_precision("fp64")
+    L = Lattice(2, 3)
+    pepo = random_state(L, (1, 1), "pepo", seed=21)
+    cache = spp.SamplingCache(gpu_ctx, to_product_state(spp, pepo), 16)
+    cfgs = []
+    for bits in itertools.product([1, 2], repeat=6):
+        cfgs.append([[bits[x + 2 * y] for y in range(3)] for x in range(2)])
+    r = spp.sample_configurations(cache, configurations=np.array(cfgs))
+    assert abs(np.sum(np.exp(r["log_probability"])) - 1.0) < 1e-12
