@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Born samples/s of the batched sampler on one MI355X.
+
+Workload (one "step"): one batched `sample_configuration` of B samples on the
+beta0 sampling cache of an 8x8 open-lattice Z2 PEPO with bond space
+Z2Space(0=>3,1=>3) (D=6), physical/auxiliary d=2, boundary chi=32, row-major
+draws with one injected uniform per site. The headline run computes the
+contraction GEMMs in FP32 on MFMA (FP64 eigensolvers, probabilities and
+log weights); the plain path is FP64 everywhere at batch 1. The cache build
+is timed separately and amortised over the whole ensemble.
+
+Prints one JSON line (rank 0).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("Born samples/s, 8×8 TFIM D=6 χ=32: batched MFMA vs plain FP64, "
+          "ΔE/site≤1e-6")
+PEAK = {"fp32": 157.3, "fp64": 78.6}  # MI355X dense matrix TFLOP/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(L=8, D=(3, 3), chi=32, seed=5, threads=None):
+    """Time the FP64 oracle on one row of a Born draw at the same shapes and
+    extrapolate to a draw (8 rows). One row = right row environments, the two
+    candidate left steps and the drawn left step per site, and the compressed
+    north absorb (src/sampling/born.jl:105-161). Boundaries are seeded random
+    tensors of the production shapes (cost depends on shapes only)."""
+    from threadpoolctl import threadpool_limits
+    from oracle.model import Lattice, random_state
+    from oracle.z2 import Z2, symmetrize
+    from oracle import boundary as bm
+    threads = threads or min(16, os.cpu_count() or 1)
+    rng = np.random.default_rng(seed)
+    lat = Lattice(L, L)
+    pepo = random_state(lat, D, "pepo", seed)
+    y = 2
+    rowl = [bm.double_layer(pepo[(x, y)]) for x in range(L)]
+    cands = [rowl[x] for x in range(L)]
+
+    def bnd(phys_p):
+        half = np.array([0] * (chi // 2) + [1] * (chi - chi // 2))
+        out = []
+        for x in range(L):
+            lp = np.array([0]) if x == 0 else half
+            rp = np.array([0]) if x == L - 1 else half
+            t = Z2(rng.standard_normal((len(lp), len(phys_p[x]), len(rp))), [lp, phys_p[x], rp])
+            out.append(symmetrize(t))
+        return out
+    north = bnd([E.p[1] for E in rowl])
+    south = bnd([E.p[3] for E in rowl])
+    with threadpool_limits(threads):
+        t0 = time.perf_counter()
+        _, right = bm.row_environments(north, rowl, south, side="right")
+        left = bm.row_edge(north, rowl, south, "left")
+        for x in range(L):
+            steps = [bm.left_step(left, north[x], E, south[x]) for E in (cands[x], cands[x])]
+            _ = [bm.join_scalars(s, right[x + 1]) for s in steps]
+            left = bm.safe_scale(bm.left_step(left, north[x], cands[x], south[x]))
+        t_env = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        bm.absorb(bm.Boundary(north), rowl, chi, "north")
+        t_abs = time.perf_counter() - t1
+    t_row = t_env + t_abs
+    return {"value": 1.0 / (L * t_row), "unit": "Born samples/s", "cores": threads, "kind": "port",
+            "sample": (f"FP64 NumPy oracle (CPU restatement, not the reference), one row of an {L}x{L} D=6 chi={chi} "
+                       f"draw timed ({t_env:.1f} s environments + {t_abs:.1f} s compressed absorb), x{L} rows "
+                       f"per draw; OpenBLAS {threads} threads")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=256, help="Born samples per step")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--L", type=int, default=8)
+    ap.add_argument("--chi", type=int, default=32)
+    ap.add_argument("--eps", type=float, default=1.0, help="perturbation size of the synthetic PEPO")
+    ap.add_argument("--plain-draws", type=int, default=3, help="batch-1 FP64 draws for the plain path")
+    ap.add_argument("--check-samples", type=int, default=8, help="FP64 re-evaluation of drawn samples")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-only", action="store_true", help="timed loop only (for rocprofv3)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the sampler shards samples across GPUs by design, "
+              "but multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__ as ge
+    spp = ge.load_package()
+    L = args.L
+    lat = spp.Lattice(L, L)
+    pepo = spp.synthetic_thermal_pepo(lat, (3, 3), args.eps, seed=1)
+    ns = L * L
+    B = args.batch
+
+    ctx = spp.Context(0, args.precision)
+    t0 = time.perf_counter()
+    cache = spp.SamplingCache(ctx, pepo, args.chi)
+    ctx.synchronize()
+    t_cache = time.perf_counter() - t0
+    log(f"cache build {t_cache:.2f} s")
+
+    def uniforms(first, n):
+        return np.stack([spp.sample_uniforms(1, i, ns) for i in range(first, first + n)])
+
+    index = 1
+    for _ in range(args.warmup):
+        spp.sample_configurations(cache, uniforms=uniforms(index, B))
+        index += B
+    ctx.synchronize()
+    ctx.reset_counters()
+    ctx.set_option("profile", 1)
+    draws = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        draws.append(spp.sample_configurations(cache, uniforms=uniforms(index, B)))
+        index += B
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    cnt = ctx.counters()
+    ctx.set_option("profile", 0)
+    value = B * args.steps / elapsed
+    log(f"timed {args.steps} x {B} draws in {elapsed:.2f} s -> {value:.3f} samples/s")
+    gemm_tf = cnt["gemm_flops_profiled"] / (cnt["gemm_ms"] * 1e-3) / 1e12 if cnt["gemm_ms"] > 0 else 0.0
+    avg_ms = cnt["gemm_ms"] / max(cnt["gemm_launches"], 1)
+    result = {
+        "metric": METRIC, "value": value, "unit": "Born samples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
+        "config": {"workload": f"batched Born draws, {L}x{L} Z2 PEPO D=6 (3+3), d=2, chi={args.chi}, "
+                               f"{B} samples per step", "lattice": [L, L], "D": 6, "chi": args.chi,
+                   "samples_per_step": B, "cache_build_s": t_cache,
+                   "input": f"synthetic PEPO: identity embedded in D=6 bonds + Gaussian perturbation eps={args.eps}, seed 1"},
+        "roofline": {"bound": "mfma", "kernel": f"z2gemm_kernel<{'float' if args.precision == 'fp32' else 'double'}>",
+                     "achieved": gemm_tf, "peak": PEAK[args.precision], "unit": "TFLOP/s",
+                     "frac": gemm_tf / PEAK[args.precision], "traffic": None,
+                     "avg_launch_ms": avg_ms, "launches": cnt["gemm_launches"],
+                     "gemm_share_of_step": cnt["gemm_ms"] / (1e3 * elapsed),
+                     "flops_per_draw": cnt["gemm_flops"] / (B * args.steps)},
+    }
+    if args.profile_only:
+        print(json.dumps(result), flush=True)
+        return
+
+    # plain FP64 path: the same kernels, FP64, one sample at a time
+    pctx = spp.Context(0, "fp64")
+    pcache = spp.SamplingCache(pctx, pepo, args.chi)
+    spp.sample_configurations(pcache, uniforms=uniforms(10 ** 6, 1))  # warm-up
+    pctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.plain_draws):
+        spp.sample_configurations(pcache, uniforms=uniforms(index + i, 1))
+    pctx.synchronize()
+    plain = args.plain_draws / (time.perf_counter() - t0)
+    result["plain_fp64"] = {"value": plain, "unit": "Born samples/s", "batch": 1,
+                            "speedup_of_value": value / plain}
+    # precision check: FP64 evaluation of FP32-drawn configurations
+    k = min(args.check_samples, B)
+    d0 = draws[0]
+    ev = spp.sample_configurations(pcache, configurations=d0["configuration"][:k])
+    dlogp = np.max(np.abs(ev["log_probability"] - d0["log_probability"][:k]))
+    dtln = np.max(np.abs(ev["tensor_log_norm"] - d0["tensor_log_norm"][:k]))
+    result["precision_check"] = {"samples": int(k), "max_abs_dlog_probability": float(dlogp),
+                                 "max_abs_dtensor_log_norm": float(dtln),
+                                 "max_environment_truncation": float(np.max(d0["max_environment_truncation"]))}
+    if not args.no_cpu:
+        try:
+            result["cpu_baseline"] = cpu_baseline(L=L, chi=args.chi)
+        except Exception as exc:  # the baseline must not hide the GPU result
+            result["cpu_baseline"] = {"value": None, "error": repr(exc)}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -52,12 +52,15 @@ int spp_set_precision(spp_ctx* ctx, int precision);
 
 /* Tuning: "ksub" (subspace size of the truncation eigensolver, <= 64),
  * "max_subspace_iter", "subspace_tol", "chunk" (samples per device batch,
- * 0 = sized from free memory). */
+ * 0 = sized from free memory), "profile" (1: time each contraction launch
+ * with HIP events on the library stream). */
 int spp_set_option(spp_ctx* ctx, const char* name, double value);
 
 /* counters[0] = contraction flops issued (2*M*N*K over parity blocks),
  * counters[1] = contraction launches, counters[2] = subspace iterations of the
- * last truncation, counters[3] = peak workspace bytes. Returns the count written. */
+ * last truncation, counters[3] = peak workspace bytes, counters[4] = summed
+ * contraction kernel milliseconds and counters[5] = the flops of the timed
+ * launches (profiling mode). Returns the count written. */
 int spp_get_counters(spp_ctx* ctx, double* counters, int n);
 int spp_reset_counters(spp_ctx* ctx);
 
@@ -99,11 +102,13 @@ int spp_born_sample_batch(spp_ctx* ctx, spp_cache* cache, int n, const double* u
                           int* labels_out, double* log_prob, double* conditionals, double* max_trunc,
                           double* tensor_log_norm, double* log_amplitude, int* status, int* fail_site);
 
-/* Plain dense C[M][N] = A[M][K] B[K][N] (row-major, FP64 in/out) through the
- * library's contraction kernel in the given precision; a self-test of the
- * matrix-core kernel with no reference counterpart. */
-int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, const double* A, const double* B,
-                      double* C);
+/* Self-test of the matrix-core contraction kernel (no reference counterpart):
+ * C[M][N] = A[M][K] B[K][N] (row-major, FP64 in/out) with each index a Z2
+ * leg whose first evenX entries are even. Only parity-matching (neutral)
+ * entries of C are computed; the others are returned as NaN. Pass
+ * evenX = X for a plain dense product. */
+int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, int evenM, int evenN, int evenK,
+                      const double* A, const double* B, double* C);
 
 #ifdef __cplusplus
 }

@@ -3,6 +3,7 @@
 #include "../../include/stopeps_hip.h"
 #include "sampler.h"
 #include <cstring>
+#include <limits>
 #include <memory>
 
 using namespace spp;
@@ -93,6 +94,9 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value) {
     } else if (n == "subspace_tol") {
       if (!(value > 0)) throw HipError("subspace_tol must be positive");
       ctx->c.subspace_tol = value;
+    } else if (n == "profile") {
+      ctx->c.flush_timer();
+      ctx->c.profile = value != 0 ? 1 : 0;
     } else if (n == "chunk") {
       if (value < 0) throw HipError("chunk must be >= 0");
       ctx->c.chunk = (long long)value;
@@ -104,15 +108,17 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value) {
 
 int spp_get_counters(spp_ctx* ctx, double* counters, int n) {
   if (!ctx || !counters) return -3;
-  double v[4] = {ctx->c.counters.gemm_flops, ctx->c.counters.gemm_launches, (double)ctx->c.last_iterations,
-                 (double)ctx->c.arena.peak()};
-  int m = n < 4 ? n : 4;
+  if (guarded(ctx, [&] { ctx->c.flush_timer(); }) != 0) return -1;
+  double v[6] = {ctx->c.counters.gemm_flops, ctx->c.counters.gemm_launches, (double)ctx->c.last_iterations,
+                 (double)ctx->c.arena.peak(), ctx->c.counters.gemm_ms, ctx->c.counters.gemm_flops_profiled};
+  int m = n < 6 ? n : 6;
   for (int i = 0; i < m; ++i) counters[i] = v[i];
   return m;
 }
 
 int spp_reset_counters(spp_ctx* ctx) {
   if (!ctx) return -3;
+  if (guarded(ctx, [&] { ctx->c.flush_timer(); }) != 0) return -1;
   ctx->c.counters = Counters{};
   return 0;
 }
@@ -219,10 +225,12 @@ int spp_born_sample_batch(spp_ctx* ctx, spp_cache* cache, int n, const double* u
   });
 }
 
-int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, const double* A, const double* B,
-                      double* C) {
+int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, int evenM, int evenN, int evenK,
+                      const double* A, const double* B, double* C) {
   return guarded(ctx, [&] {
     if (M <= 0 || N <= 0 || K <= 0) throw HipError("dimensions must be positive");
+    if (evenM < 0 || evenM > M || evenN < 0 || evenN > N || evenK < 0 || evenK > K)
+      throw HipError("even counts must lie in [0, dim]");
     hipStream_t st = ctx->c.stream;
     size_t es = precision == 1 ? sizeof(float) : sizeof(double);
     void *dA, *dB, *dC, *tA = nullptr, *tB = nullptr;
@@ -231,12 +239,14 @@ int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, const do
     SPP_CHECK(hipMalloc(&dC, sizeof(double) * M * N));
     SPP_CHECK(hipMemcpyAsync(dA, A, sizeof(double) * M * K, hipMemcpyHostToDevice, st));
     SPP_CHECK(hipMemcpyAsync(dB, B, sizeof(double) * K * N, hipMemcpyHostToDevice, st));
-    Leg lm = fixed_leg(M, M), ln = fixed_leg(N, N), lk = fixed_leg(K, K);
+    Leg lm = fixed_leg(M, evenM), ln = fixed_leg(N, evenN), lk = fixed_leg(K, evenK);
+    const double qnan = std::numeric_limits<double>::quiet_NaN();
     if (precision == 1) {
       SPP_CHECK(hipMalloc(&tA, es * M * K));
       SPP_CHECK(hipMalloc(&tB, es * K * N));
       void* tC;
       SPP_CHECK(hipMalloc(&tC, es * M * N));
+      SPP_CHECK(launch_fill<float>((float*)tC, (long long)M * N, (float)qnan, st));
       SPP_CHECK((launch_convert<float, double>((float*)tA, (const double*)dA, (long long)M * K, st)));
       SPP_CHECK((launch_convert<float, double>((float*)tB, (const double*)dB, (long long)K * N, st)));
       contract<float>(ctx->c, dense_ref(tC, 0, "mn", {lm, ln}), dense_ref(tA, 0, "mk", {lm, lk}),
@@ -245,6 +255,7 @@ int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, const do
       SPP_CHECK(hipStreamSynchronize(st));
       (void)hipFree(tC);
     } else {
+      SPP_CHECK(launch_fill<double>((double*)dC, (long long)M * N, qnan, st));
       contract<double>(ctx->c, dense_ref(dC, 0, "mn", {lm, ln}), dense_ref(dA, 0, "mk", {lm, lk}),
                        dense_ref(dB, 0, "kn", {lk, ln}), 1);
     }

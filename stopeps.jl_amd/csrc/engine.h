@@ -88,9 +88,16 @@ TRef with_ptrs(const TRef& t, void* const* ptrs);
 TRef relabel(const TRef& t, const std::string& new_letters_in_storage_order);
 
 struct Counters {
-  double gemm_flops = 0;  // 2*M*N*K summed over launched (sample, parity) problems (upper bound)
+  double gemm_flops = 0;  // 2*M*N*K summed over launched (sample, parity) problems
   double gemm_launches = 0;
-  double kernel_ms = 0;
+  double gemm_ms = 0;     // summed contraction kernel time (profiling mode)
+  double gemm_flops_profiled = 0;
+};
+
+// Event pairs recorded around contraction launches when profiling is on.
+struct LaunchTimer {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  std::vector<double> pending_flops;
 };
 
 struct Context {
@@ -105,6 +112,9 @@ struct Context {
   std::string last_error;
   long long chunk = 0;      // samples per internal chunk (0: automatic)
   int last_iterations = 0;
+  int profile = 0;          // 1: time every contraction launch with HIP events
+  LaunchTimer timer;
+  void flush_timer();       // accumulate finished event pairs into counters
 };
 
 // Contraction C = alpha * A.B (+ C): letters shared by A and B are summed.

@@ -216,7 +216,34 @@ void contract(Context& ctx, const TRef& C, const TRef& A, const TRef& B, int nsa
   for (int q = 0; q < 2; ++q) fl += group_estimate(lm, q) * group_estimate(ln, q) * group_estimate(lk, q);
   ctx.counters.gemm_flops += 2.0 * fl * nsamples;
   ctx.counters.gemm_launches += 1;
+  if (ctx.profile) {
+    hipEvent_t a, b;
+    SPP_CHECK(hipEventCreate(&a));
+    SPP_CHECK(hipEventCreate(&b));
+    SPP_CHECK(hipEventRecord(a, ctx.stream));
+    SPP_CHECK(launch_z2gemm<T>(g, mmax, nmax, ctx.stream));
+    SPP_CHECK(hipEventRecord(b, ctx.stream));
+    ctx.timer.pending.emplace_back(a, b);
+    ctx.timer.pending_flops.push_back(2.0 * fl * nsamples);
+    if (ctx.timer.pending.size() >= 4096) ctx.flush_timer();
+    return;
+  }
   SPP_CHECK(launch_z2gemm<T>(g, mmax, nmax, ctx.stream));
+}
+
+void Context::flush_timer() {
+  if (timer.pending.empty()) return;
+  SPP_CHECK(hipStreamSynchronize(stream));
+  for (size_t i = 0; i < timer.pending.size(); ++i) {
+    float ms = 0.f;
+    SPP_CHECK(hipEventElapsedTime(&ms, timer.pending[i].first, timer.pending[i].second));
+    counters.gemm_ms += ms;
+    counters.gemm_flops_profiled += timer.pending_flops[i];
+    (void)hipEventDestroy(timer.pending[i].first);
+    (void)hipEventDestroy(timer.pending[i].second);
+  }
+  timer.pending.clear();
+  timer.pending_flops.clear();
 }
 
 template void contract<float>(Context&, const TRef&, const TRef&, const TRef&, int, double, bool, const int*);
@@ -479,6 +506,23 @@ __global__ void residual_kernel(const double* q, const double* z, long long pstr
     double rel = scale > 0.0 ? mx / scale : 0.0;
     conv[prob] = (rel <= tol) ? 1 : 0;
     worst[prob] = rel;
+  }
+}
+
+// rho <- (rho + rho^T) / 2 on each compact n x n block (rho built in FP32 is
+// symmetric only to rounding; the eigensolver needs an exactly symmetric matrix).
+__global__ void symmetrize_kernel(double* rho, long long pstride, int ld, const int* n_of) {
+  const int prob = blockIdx.y;
+  const int n = n_of[prob];
+  double* b = rho + (long long)prob * pstride;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n * n;
+       i += (long long)gridDim.x * blockDim.x) {
+    int r = (int)(i / n), c = (int)(i % n);
+    if (r < c) {
+      double v = 0.5 * (b[(long long)r * ld + c] + b[(long long)c * ld + r]);
+      b[(long long)r * ld + c] = v;
+      b[(long long)c * ld + r] = v;
+    }
   }
 }
 
@@ -808,6 +852,8 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
       contract<T>(ctx, compact_ref(rho_t, 2 * rps, rps, nmax, "kpKP", 2, {kb, sp.s, kb, sp.s}), Wr,
                   with_letters(Yr, "KPRF"), nb);
       SPP_CHECK((launch_convert<double, T>(rho, rho_t, 2 * nb * rps, st)));
+      hipLaunchKernelGGL(symmetrize_kernel, dim3(64, 2 * nb), dim3(256), 0, st, rho, rps, (int)nmax, ncount);
+      SPP_CHECK(hipGetLastError());
     }
     hipLaunchKernelGGL(trace_kernel, dim3(2 * nb), dim3(256), 0, st, rho, rps, (int)nmax, ncount, tr);
     SPP_CHECK(hipGetLastError());

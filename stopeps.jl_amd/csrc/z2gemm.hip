@@ -144,6 +144,11 @@ __global__ __launch_bounds__(kThreads) void z2gemm_kernel(GemmArgs args) {
     }
   }
 
+  // The offset tables above are read by every thread in the epilogue. When the
+  // K block of this parity is empty (cK == 0) the K loop below, whose barriers
+  // would otherwise order those writes, does not run: barrier here.
+  __syncthreads();
+
   const T* Ab = operand_base<T>(args.a, s, q);
   const T* Bb = operand_base<T>(args.b, s, q);
   T* Cb = operand_base<T>(args.c, s, q);

@@ -44,10 +44,12 @@ class Context:
         _lib.check(self.handle, self._lib.spp_set_option(self.handle, name.encode(), float(value)))
 
     def counters(self):
-        out = np.zeros(4)
-        self._lib.spp_get_counters(self.handle, _lib.dptr(out), 4)
+        out = np.zeros(6)
+        rc = self._lib.spp_get_counters(self.handle, _lib.dptr(out), 6)
+        if rc < 0:
+            _lib.check(self.handle, rc)
         return {"gemm_flops": out[0], "gemm_launches": out[1], "subspace_iterations": out[2],
-                "workspace_peak_bytes": out[3]}
+                "workspace_peak_bytes": out[3], "gemm_ms": out[4], "gemm_flops_profiled": out[5]}
 
     def reset_counters(self):
         self._lib.spp_reset_counters(self.handle)
@@ -66,8 +68,10 @@ class Context:
         except Exception:
             pass
 
-    def gemm(self, A, B, precision="fp64"):
-        """Self-test: C = A @ B through the library's matrix-core kernel."""
+    def gemm(self, A, B, precision="fp64", evens=None):
+        """Self-test: C = A @ B through the library's matrix-core kernel. With
+        ``evens = (eM, eN, eK)`` the indices are Z2 legs (first e entries even)
+        and only parity-matching entries of C are computed (others NaN)."""
         A = np.ascontiguousarray(A, dtype=np.float64)
         B = np.ascontiguousarray(B, dtype=np.float64)
         M, K = A.shape
@@ -75,7 +79,8 @@ class Context:
         if K != K2:
             raise ValueError("shape mismatch")
         C = np.zeros((M, N))
-        _lib.check(self.handle, self._lib.spp_selftest_gemm(self.handle, PRECISIONS[precision], M, N, K,
+        eM, eN, eK = evens if evens is not None else (M, N, K)
+        _lib.check(self.handle, self._lib.spp_selftest_gemm(self.handle, PRECISIONS[precision], M, N, K, eM, eN, eK,
                                                              _lib.dptr(A), _lib.dptr(B), _lib.dptr(C)))
         return C
 

@@ -94,6 +94,39 @@ def identity_pepo(lattice, symmetric=True):
     return FiniteState(lattice, sites, "pepo")
 
 
+def synthetic_thermal_pepo(lattice, D=(3, 3), eps=0.3, seed=1):
+    """Synthetic Z2 PEPO at production shapes: the beta = 0 identity PEPO
+    embedded in bond spaces Z2Space(0=>D[0], 1=>D[1]) plus a seeded Gaussian
+    perturbation of relative size ``eps`` in every allowed block, each site
+    scaled to unit Frobenius norm. Used as the timing input of the cfg
+    "8x8, D=6, chi=32" workload (no beta0 checkpoint ships with the reference)."""
+    rng = np.random.default_rng(seed)
+    bp = np.array([0] * D[0] + [1] * D[1])
+    pp = np.array([0, 1])
+    u = np.array([0])
+    sites = []
+    for x in range(lattice.Lx):
+        col = []
+        for y in range(lattice.Ly):
+            legs = [u if x == 0 else bp, u if y == 0 else bp, pp, pp,
+                    u if x == lattice.Lx - 1 else bp, u if y == lattice.Ly - 1 else bp]
+            shape = [len(p) for p in legs]
+            a = rng.standard_normal(shape)
+            par = np.zeros(shape, dtype=np.int64)
+            for k, p in enumerate(legs):
+                sh = [1] * 6
+                sh[k] = len(p)
+                par = par + p.reshape(sh)
+            a[par % 2 != 0] = 0.0
+            a /= np.linalg.norm(a)
+            a *= eps
+            a[0, 0, :, :, 0, 0] += np.eye(2) / np.sqrt(2.0)
+            a /= np.linalg.norm(a)
+            col.append(Site(a, legs))
+        sites.append(col)
+    return FiniteState(lattice, sites, "pepo")
+
+
 def parity_sorted(site):
     """Dense array with every leg sorted by parity (stable), and even counts."""
     a = site.data

@@ -110,3 +110,27 @@ def test_evaluation_mode_sums_to_one(spp, gpu_ctx):
         cfgs.append([[bits[x + 2 * y] for y in range(3)] for x in range(2)])
     r = spp.sample_configurations(cache, configurations=np.array(cfgs))
     assert abs(np.sum(np.exp(r["log_probability"])) - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_z2_gemm_empty_k_block(spp, gpu_ctx, precision):
+    """Regression: a parity block whose K range is empty (all K indices even)
+    must write exact zeros into its (odd M, odd N) output block."""
+    say("z2 gemm with empty odd K block", precision)
+    M, N, K = 150, 140, 20
+    eM, eN, eK = 70, 60, K
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    pm = np.arange(M) >= eM
+    pn = np.arange(N) >= eN
+    pk = np.arange(K) >= eK
+    A[pm[:, None] != pk[None, :]] = 0.0
+    B[pk[:, None] != pn[None, :]] = 0.0
+    C = gpu_ctx.gemm(A, B, precision, evens=(eM, eN, eK))
+    neutral = pm[:, None] == pn[None, :]
+    ref = A @ B
+    assert np.all(np.isnan(C[~neutral]))
+    assert np.all(C[eM:, eN:] == 0.0)
+    tol = 1e-12 if precision == "fp64" else 2e-5
+    assert np.max(np.abs(C[neutral] - ref[neutral])) <= tol * np.max(np.abs(ref))
