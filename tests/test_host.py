@@ -1,0 +1,123 @@
+"""CPU tests of the host layer and the C ABI surface (no GPU needed)."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, to_product_state
+from oracle.model import Lattice, random_state, embed_identity_pepo
+from oracle import born as ob
+from oracle import boundary as obm
+
+HEADER = os.path.join(ROOT, "include", "stopeps_hip.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(spp_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(spp):
+    lib = spp._lib.load()
+    syms = declared_symbols()
+    assert len(syms) >= 15
+    for name in syms:
+        assert hasattr(lib, name), name
+    assert set(syms) == set(spp._lib.EXPORTED)
+
+
+def test_library_is_built_for_gfx950():
+    so = os.path.join(ROOT, "stopeps.jl_amd", "libstopeps_hip.so")
+    assert os.path.exists(so)
+    data = open(so, "rb").read()
+    assert b"gfx950" in data
+
+
+def test_context_without_gpu_fails_loudly(spp):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(spp._lib.HIPLibraryError):
+        spp.Context(0, "fp64")
+
+
+def test_parity_sorted_roundtrip(spp):
+    rng = np.random.default_rng(0)
+    par = [np.array([1, 0, 1]), np.array([0]), np.array([0, 1]), np.array([1, 0]), np.array([0, 0, 1]), np.array([0])]
+    a = rng.standard_normal([len(p) for p in par])
+    s = spp.Site(a, par)
+    b, ev = spp.parity_sorted(s)
+    assert ev == [1, 1, 1, 1, 2, 1]
+    assert b[0, 0, 0, 0, 0, 0] == a[1, 0, 0, 1, 0, 0]
+    assert b[1, 0, 1, 1, 2, 0] == a[0, 0, 1, 0, 2, 0]
+    assert b[2, 0, 0, 0, 1, 0] == a[2, 0, 0, 1, 1, 0]
+
+
+def test_state_validation_errors(spp):
+    L = spp.Lattice(2, 1)
+    u = np.array([0])
+    bad = spp.Site(np.zeros((1, 1, 2, 2, 2, 1)), [u, u, [0, 1], [0, 1], [0, 1], u])
+    ok = spp.Site(np.zeros((1, 1, 2, 2, 1, 1)), [u, u, [0, 1], [0, 1], u, u])
+    with pytest.raises(ValueError, match="horizontal space mismatch"):
+        spp.FiniteState(L, [[bad], [ok]], "pepo")
+    with pytest.raises(ValueError, match="nontrivial OBC leg"):
+        spp.FiniteState(L, [[ok], [bad]], "pepo")
+    with pytest.raises(ValueError):
+        spp.Lattice(0, 3)
+
+
+def test_uniform_stream_matches_oracle(spp):
+    for i in (1, 2, 1000):
+        np.testing.assert_array_equal(spp.sample_uniforms(20260924, i, 64), ob.counter_uniforms(20260924, i, 64))
+    u = spp.sample_uniforms(1, 1, 10000)
+    assert 0.0 <= u.min() and u.max() < 1.0 and abs(u.mean() - 0.5) < 0.02
+    with pytest.raises(ValueError):
+        spp.sample_uniforms(1, 0, 4)
+
+
+def test_project_configuration_matches_oracle(spp):
+    L = Lattice(3, 2)
+    pepo = embed_identity_pepo(L, (1, 1), 0.5, seed=2)
+    cfg = [[1, 2], [2, 2], [1, 1]]
+    ps_o, rec_o = ob.project_configuration(pepo, cfg)
+    ps_p, rec_p = spp.project_configuration(to_product_state(spp, pepo), cfg)
+    assert abs(ps_o.log_amplitude - ps_p.log_amplitude) < 1e-13
+    for (x, y) in L.site_order():
+        np.testing.assert_allclose(ps_p[(x, y)].data, ps_o[(x, y)].a, atol=1e-15)
+        for k in range(6):
+            np.testing.assert_array_equal(ps_p[(x, y)].parity[k], ps_o[(x, y)].p[k])
+    assert [r["log_normalization"] for r in rec_p] == pytest.approx([r["log_normalization"] for r in rec_o])
+
+
+def test_bases(spp):
+    b = spp.SymmetryBasis()
+    assert b.labels == [1, 2] and b.charges == [0, 1] and b.index(2) == 1
+    x = spp.XBasis()
+    assert np.allclose(x.U.T @ x.U, np.eye(2))
+    with pytest.raises(ValueError, match="unknown basis label"):
+        b.index(3)
+
+
+def test_synthetic_pepo_is_symmetric_and_normalised(spp):
+    L = spp.Lattice(3, 3)
+    st = spp.synthetic_thermal_pepo(L, (3, 3), 0.5, seed=4)
+    for (x, y) in L.site_order():
+        s = st[(x, y)]
+        assert abs(np.linalg.norm(s.data) - 1) < 1e-14
+        par = np.zeros(s.data.shape, dtype=int)
+        for k, p in enumerate(s.parity):
+            sh = [1] * 6
+            sh[k] = len(p)
+            par = par + np.asarray(p).reshape(sh)
+        assert np.all(s.data[par % 2 == 1] == 0)
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
