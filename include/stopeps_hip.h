@@ -3,13 +3,13 @@
  *
  * The reference has no FFI: its plugin point is Julia dispatch on
  * `RunConfig.contraction::Type{<:ContractionMethod}` (src/driver/config.jl:51),
- * instantiated per chi role by `_boundary(c, role)` (src/driver/particles.jl:105-112)
+ * instantiated per chi role by `_boundary(c, role)` (src/driver/particles.jl:27-34)
  * and consumed by `SamplingCache` / `sample_configuration`
  * (src/sampling/born.jl:45-163), `lognorm` (src/contraction/boundary_mps.jl:180-186)
  * and `project_configuration` (born.jl:27-36). A Julia `ccall` layer (or the
  * Python ctypes layer shipped with this library) binds the functions below
  * in place of those methods; the per-sample loops of the callers
- * (src/driver/run.jl:273-294) become one batched call.
+ * (src/driver/run.jl:45-66) become one batched call.
  *
  * Conventions
  *  - Sites are indexed row-major, i = x + y*Lx (x fastest), as

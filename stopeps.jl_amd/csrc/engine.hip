@@ -470,6 +470,161 @@ __global__ __launch_bounds__(256) void householder_q_kernel(double* z, double* q
   }
 }
 
+// Register-resident variant of the thin Householder QR above for n <= NSLAB*RMAX
+// rows and ncol <= 64 columns: 64*NSLAB threads, thread (slab, c) = (t / 64, t % 64)
+// holds column c of rows slab + NSLAB i. Column reductions go through LDS
+// partials; the reflector of step j is broadcast through LDS. Same arithmetic
+// as householder_q_kernel.
+template <int NSLAB, int RMAX>
+__global__ __launch_bounds__(64 * NSLAB) void householder_q_reg_kernel(double* z, double* q, long long pstride, int ld,
+                                                                 int ncol, const int* n_of) {
+  const int prob = blockIdx.x;
+  const int n = n_of[prob];
+  double* Z = z + (long long)prob * pstride;
+  double* Q = q + (long long)prob * pstride;
+  const int tid = threadIdx.x;
+  const int c = tid & 63, slab = tid >> 6;
+  __shared__ double vsh[NSLAB * RMAX];
+  __shared__ double part[NSLAB][64];
+  __shared__ double wsh[64];
+  __shared__ double tau[64];
+  __shared__ double sh_scale, sh_beta;
+  double zr[RMAX];
+#pragma unroll
+  for (int i = 0; i < RMAX; ++i) {
+    const int r = slab + NSLAB * i;
+    zr[i] = (r < n && c < ncol) ? Z[(long long)r * ld + c] : 0.0;
+  }
+  const int kmax = min(ncol, n);
+  for (int j = 0; j < kmax; ++j) {
+    // squared norm of column j below the diagonal
+    if (c == j) {
+      double acc = 0.0;
+#pragma unroll
+      for (int i = 0; i < RMAX; ++i) {
+        const int r = slab + NSLAB * i;
+        if (r > j && r < n) acc += zr[i] * zr[i];
+      }
+      part[slab][0] = acc;
+#pragma unroll
+      for (int i = 0; i < RMAX; ++i)
+        if (slab + NSLAB * i == j) wsh[0] = zr[i];  // alpha = Z[j][j] (static register index)
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double sig = 0.0;
+      for (int k = 0; k < NSLAB; ++k) sig += part[k][0];
+      const double alpha = wsh[0];
+      if (sig == 0.0) {
+        tau[j] = 0.0;
+        sh_beta = alpha;
+        sh_scale = 0.0;
+      } else {
+        const double nrm = sqrt(alpha * alpha + sig);
+        const double beta = alpha <= 0.0 ? nrm : -nrm;
+        tau[j] = (beta - alpha) / beta;
+        sh_scale = 1.0 / (alpha - beta);
+        sh_beta = beta;
+      }
+    }
+    __syncthreads();
+    const double tj = tau[j];
+    if (c == j) {
+#pragma unroll
+      for (int i = 0; i < RMAX; ++i) {
+        const int r = slab + NSLAB * i;
+        if (r > j && r < n) {
+          zr[i] *= sh_scale;
+          vsh[r] = zr[i];
+        } else if (r == j) {
+          vsh[r] = 1.0;
+          zr[i] = sh_beta;
+        } else if (r < n) {
+          vsh[r] = 0.0;
+        }
+      }
+    }
+    __syncthreads();
+    if (tj != 0.0) {
+      // w_c = v^T Z[:, c] for c > j
+      double acc = 0.0;
+      if (c > j && c < ncol) {
+#pragma unroll
+        for (int i = 0; i < RMAX; ++i) {
+          const int r = slab + NSLAB * i;
+          if (r >= j && r < n) acc += vsh[r] * zr[i];
+        }
+      }
+      part[slab][c] = acc;
+      __syncthreads();
+      if (slab == 0) {
+        double w = 0.0;
+        for (int k = 0; k < NSLAB; ++k) w += part[k][c];
+        wsh[c] = w;
+      }
+      __syncthreads();
+      if (c > j && c < ncol) {
+        const double f = tj * wsh[c];
+#pragma unroll
+        for (int i = 0; i < RMAX; ++i) {
+          const int r = slab + NSLAB * i;
+          if (r >= j && r < n) zr[i] -= f * vsh[r];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // keep the reflectors (below the diagonal) in Z for the Q formation
+#pragma unroll
+  for (int i = 0; i < RMAX; ++i) {
+    const int r = slab + NSLAB * i;
+    if (r < n && c < ncol) Z[(long long)r * ld + c] = zr[i];
+  }
+  // Q = H_0 ... H_{k-1} [I; 0], panel in the same registers
+#pragma unroll
+  for (int i = 0; i < RMAX; ++i) {
+    const int r = slab + NSLAB * i;
+    zr[i] = (r == c && c < kmax) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int j = kmax - 1; j >= 0; --j) {
+    const double tj = tau[j];
+    if (tj == 0.0) continue;
+    for (int r = tid; r < n; r += 64 * NSLAB) vsh[r] = r < j ? 0.0 : (r == j ? 1.0 : Z[(long long)r * ld + j]);
+    __syncthreads();
+    double acc = 0.0;
+    if (c >= j && c < ncol) {
+#pragma unroll
+      for (int i = 0; i < RMAX; ++i) {
+        const int r = slab + NSLAB * i;
+        if (r >= j && r < n) acc += vsh[r] * zr[i];
+      }
+    }
+    part[slab][c] = acc;
+    __syncthreads();
+    if (slab == 0) {
+      double w = 0.0;
+      for (int k = 0; k < NSLAB; ++k) w += part[k][c];
+      wsh[c] = w;
+    }
+    __syncthreads();
+    if (c >= j && c < ncol) {
+      const double f = tj * wsh[c];
+#pragma unroll
+      for (int i = 0; i < RMAX; ++i) {
+        const int r = slab + NSLAB * i;
+        if (r >= j && r < n) zr[i] -= f * vsh[r];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < RMAX; ++i) {
+    const int r = slab + NSLAB * i;
+    if (r < n && c < ncol) Q[(long long)r * ld + c] = zr[i];
+  }
+}
+
 // Residuals of Ritz pairs: conv[prob] = max_j<t ||Z_j - theta_j Q_j|| <= tol * |theta_0|.
 __global__ void residual_kernel(const double* q, const double* z, long long pstride, int ld, const double* th,
                                 long long tstride, const int* n_of, int t, double tol, int* conv, double* worst) {
@@ -679,7 +834,11 @@ static void eig_topk(Context& ctx, const double* rho, long long rho_pstride, int
   double* tau = (double*)ctx.arena.take(sizeof(double) * nprob * kk);
   auto orth = [&](double* Q, double* Z, double* tmp) {  // Q <- orth(Z), Householder; Z destroyed
     (void)tmp;
-    hipLaunchKernelGGL(householder_q_kernel, dim3(nprob), dim3(256), 0, st, Z, Q, bs, kk, kk, n_of, tau);
+    if (kk <= 64 && nmax <= 8 * 72) {
+      hipLaunchKernelGGL((householder_q_reg_kernel<8, 72>), dim3(nprob), dim3(512), 0, st, Z, Q, bs, kk, kk, n_of);
+    } else {
+      hipLaunchKernelGGL(householder_q_kernel, dim3(nprob), dim3(256), 0, st, Z, Q, bs, kk, kk, n_of, tau);
+    }
     SPP_CHECK(hipGetLastError());
   };
   // start block

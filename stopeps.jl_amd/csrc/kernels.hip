@@ -273,25 +273,14 @@ __global__ __launch_bounds__(256) void jacobi_kernel(const double* a, long long 
   const double fro2 = red[0];
   __syncthreads();
   const int half = m / 2;
-  for (int sweep = 0; sweep < 30; ++sweep) {
-    // off-diagonal mass
-    loc = 0.0;
-    for (int i = tid; i < m * m; i += blockDim.x) {
-      int r = i / m, c = i % m;
-      if (r != c) {
-        double x = A[r * JLD + c];
-        loc += x * x;
-      }
-    }
-    red[tid] = loc;
+  __shared__ int rotated;
+  (void)fro2;
+  // Cyclic sweeps; a pair is rotated only when |a_pq| > eps * sqrt(|a_pp a_qq|)
+  // (the relative threshold of one-sided/two-sided Jacobi for accurate small
+  // eigenvalues); stop after a sweep without rotations.
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    if (tid == 0) rotated = 0;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (tid < s) red[tid] += red[tid + s];
-      __syncthreads();
-    }
-    if (tid == 0) done = (red[0] <= 1e-32 * fro2 || fro2 == 0.0) ? 1 : 0;
-    __syncthreads();
-    if (done) break;
     for (int round = 0; round < m - 1; ++round) {
       // round-robin pairing: position 0 fixed, others rotate
       if (tid < half) {
@@ -304,11 +293,12 @@ __global__ __launch_bounds__(256) void jacobi_kernel(const double* a, long long 
         double apq = A[p * JLD + q];
         double app = A[p * JLD + p], aqq = A[q * JLD + q];
         double c = 1.0, s = 0.0;
-        if (apq != 0.0 && fabs(apq) > 1e-300) {
+        if (fabs(apq) > 2.220446049250313e-16 * sqrt(fabs(app * aqq)) && fabs(apq) > 1e-300) {
           double theta = (aqq - app) / (2.0 * apq);
           double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
           c = 1.0 / sqrt(t * t + 1.0);
           s = t * c;
+          rotated = 1;
         }
         cs[tid] = c;
         sn[tid] = s;
@@ -317,8 +307,9 @@ __global__ __launch_bounds__(256) void jacobi_kernel(const double* a, long long 
       // rows: A <- J^T A
       for (int idx = tid; idx < half * m; idx += blockDim.x) {
         int k = idx / m, j = idx % m;
-        int p = pp_[k], q = qq_[k];
         double c = cs[k], s = sn[k];
+        if (s == 0.0) continue;
+        int p = pp_[k], q = qq_[k];
         double ap_ = A[p * JLD + j], aq_ = A[q * JLD + j];
         A[p * JLD + j] = c * ap_ - s * aq_;
         A[q * JLD + j] = s * ap_ + c * aq_;
@@ -327,8 +318,9 @@ __global__ __launch_bounds__(256) void jacobi_kernel(const double* a, long long 
       // columns: A <- A J, V <- V J
       for (int idx = tid; idx < half * m; idx += blockDim.x) {
         int k = idx / m, j = idx % m;
-        int p = pp_[k], q = qq_[k];
         double c = cs[k], s = sn[k];
+        if (s == 0.0) continue;
+        int p = pp_[k], q = qq_[k];
         double ap_ = A[j * JLD + p], aq_ = A[j * JLD + q];
         A[j * JLD + p] = c * ap_ - s * aq_;
         A[j * JLD + q] = s * ap_ + c * aq_;
@@ -338,6 +330,8 @@ __global__ __launch_bounds__(256) void jacobi_kernel(const double* a, long long 
       }
       __syncthreads();
     }
+    if (!rotated) break;
+    __syncthreads();
   }
   // sort eigenvalues (descending) over the n real indices
   if (tid == 0) {

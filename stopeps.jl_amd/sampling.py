@@ -4,7 +4,7 @@
 ``SamplingCache`` and ``sample_configuration`` keep the reference's names,
 arguments and error messages; ``sample_configurations`` is the batched form
 that replaces the per-sample loop of ``_initial_particles``
-(src/driver/run.jl:273-294). All contraction work runs in the HIP library;
+(src/driver/run.jl:45-66). All contraction work runs in the HIP library;
 this module only marshals arrays.
 """
 import ctypes
