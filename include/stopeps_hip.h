@@ -102,6 +102,29 @@ int spp_born_sample_batch(spp_ctx* ctx, spp_cache* cache, int n, const double* u
                           int* labels_out, double* log_prob, double* conditionals, double* max_trunc,
                           double* tensor_log_norm, double* log_amplitude, int* status, int* fail_site);
 
+/* Batched `measure_energy(state, model, BoundaryMPS(chi))`
+ * (src/measurement/energy.jl:31-78) of the projected samples
+ * `project_configuration(pepo, labels)` (src/sampling/born.jl:27-36), as
+ * `_observe` runs it for each new particle (src/driver/particles.jl:35-52).
+ * `pepo` is the state `cache` was built from; `chi` is the measurement bond.
+ *  labels [n][Lx*Ly]   0-based basis indices of the samples;
+ *  X[2][2]             onsite operator (row-major), weighted by -h;
+ *  F[2][2], G[2][2]    rank-one split of the two-site operator,
+ *                      pair[p,q,u,v] = F[p][u] G[q][v] (`_operator_factors`,
+ *                      energy.jl:8-12), weighted by -J; factor_odd = 1 when
+ *                      the factor leg carries Z2 charge 1 (Z2 TFIM);
+ *  energy_total, energy_per_site, log_norm, tensor_log_norm, max_trunc [n]
+ *                      as in the reference's NamedTuple;
+ *  status [n]          0 ok, 3 nonfinite row environment
+ *                      (boundary_mps.jl:258), 4 zero local norm environment
+ *                      and 5 nonreal/nonfinite observable (energy.jl:1-6),
+ *                      6 norm not finite and positive (boundary_mps.jl:188);
+ *  fail_site [n]       row-major site of the failing term, or -1. */
+int spp_measure_energy_batch(spp_ctx* ctx, const spp_cache* cache, const spp_state* pepo, int chi, int n,
+                             const int* labels, const double* X, const double* F, const double* G, int factor_odd,
+                             double J, double h, double* energy_total, double* energy_per_site, double* log_norm,
+                             double* tensor_log_norm, double* max_trunc, int* status, int* fail_site);
+
 /* Self-test of the matrix-core contraction kernel (no reference counterpart):
  * C[M][N] = A[M][K] B[K][N] (row-major, FP64 in/out) with each index a Z2
  * leg whose first evenX entries are even. Only parity-matching (neutral)

@@ -31,6 +31,10 @@ _SIGNATURES = {
     "spp_born_sample_batch": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int, c_double_p, c_int_p, c_int_p,
                                              c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
                                              c_int_p, c_int_p]),
+    "spp_measure_energy_batch": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, c_int_p,
+                                                c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_double,
+                                                ctypes.c_double, c_double_p, c_double_p, c_double_p, c_double_p,
+                                                c_double_p, c_int_p, c_int_p]),
     "spp_selftest_gemm": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, c_double_p, c_double_p,
                                          c_double_p]),

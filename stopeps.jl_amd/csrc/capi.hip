@@ -1,7 +1,7 @@
 // C ABI (include/stopeps_hip.h). Every entry point catches exceptions and
 // reports them through spp_last_error.
 #include "../../include/stopeps_hip.h"
-#include "sampler.h"
+#include "energy.h"
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -222,6 +222,37 @@ int spp_born_sample_batch(spp_ctx* ctx, spp_cache* cache, int n, const double* u
     o.fail_site = fail_site;
     if (cache->precision == 1) draw_batch<float>(ctx->c, *cache->f, n, uniforms, labels_in, o);
     else draw_batch<double>(ctx->c, *cache->d, n, uniforms, labels_in, o);
+  });
+}
+
+int spp_measure_energy_batch(spp_ctx* ctx, const spp_cache* cache, const spp_state* pepo, int chi, int n,
+                             const int* labels, const double* X, const double* F, const double* G, int factor_odd,
+                             double J, double h, double* energy_total, double* energy_per_site, double* log_norm,
+                             double* tensor_log_norm, double* max_trunc, int* status, int* fail_site) {
+  return guarded(ctx, [&] {
+    if (!cache || !pepo) throw HipError("null cache or state");
+    if (!X || !F || !G) throw HipError("null operator");
+    if (n < 0) throw HipError("negative sample count");
+    if (n > 0 && !labels) throw HipError("labels are required");
+    if (factor_odd != 0 && factor_odd != 1) throw HipError("factor_odd must be 0 or 1");
+    SPP_CHECK(hipSetDevice(ctx->c.device));
+    EnergyOps ops;
+    std::memcpy(ops.X, X, sizeof(ops.X));
+    std::memcpy(ops.F, F, sizeof(ops.F));
+    std::memcpy(ops.G, G, sizeof(ops.G));
+    ops.odd = factor_odd;
+    ops.J = J;
+    ops.h = h;
+    EnergyOutputs o;
+    o.energy_total = energy_total;
+    o.energy_per_site = energy_per_site;
+    o.log_norm = log_norm;
+    o.tensor_log_norm = tensor_log_norm;
+    o.max_trunc = max_trunc;
+    o.status = status;
+    o.fail_site = fail_site;
+    if (cache->precision == 1) measure_energy_batch<float>(ctx->c, pepo->s, *cache->f, chi, n, labels, ops, o);
+    else measure_energy_batch<double>(ctx->c, pepo->s, *cache->d, chi, n, labels, ops, o);
   });
 }
 

@@ -1,0 +1,548 @@
+// Batched energy measurement (measure_energy, src/measurement/energy.jl:31-78).
+//
+// All samples of a chunk are projected PEPS that share the PEPO and differ in
+// the auxiliary label of every site, so every double layer a sample needs is
+// one of two per (site, kind) and is addressed through per-sample pointer
+// tables. Kinds: the plain layer (the sampling cache's candidate), the onsite
+// impurity (operator X on the ket) and the four halves of the two-site term
+// (_operator_pair, energy.jl:13-23): F on the ket with the factor leg fused
+// into E (horizontal, left site) or S (vertical, upper site), G with the
+// factor leg fused into W or N.
+//
+// Per chunk: the south boundaries of every row (boundary_environments,
+// boundary_mps.jl:193-209, in the rotated frame), then a north sweep that at
+// row y evaluates the row terms against north[y] and south[y+1] (energy.jl:
+// 44-60) and the vertical terms with two-row slab messages against north[y]
+// and south[y+2] (energy.jl:61-72), and absorbs row y.
+//
+// Each left step from the normalised message left[x] is shared by the norm
+// (den), the onsite numerator and the horizontal numerator; the horizontal
+// denominator of bond (x, x+1) equals den[x+1] / f[x], f[x] being the factor
+// that normalised the plain step into left[x+1]. The ratios are therefore the
+// reference's num/den up to rounding.
+#include "energy.h"
+#include "boundary_ops.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace spp {
+namespace {
+
+enum Kind { kPlain = 0, kOnsite, kHorzC, kHorzD, kVertC, kVertD, kKinds };
+
+// tables[(kind * nsite + site) * ld + s] = bases[(kind * nsite + site) * 2 + labels[s][site]]
+__global__ void layer_tables_kernel(const int* labels, int nsite, void* const* bases, int nkinds, void** tables,
+                                    int nb, int ld) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long total = (long long)nkinds * nsite * nb;
+  if (t >= total) return;
+  const int s = (int)(t % nb);
+  const long long ks = t / nb;
+  const int site = (int)(ks % nsite);
+  const int lab = labels[(long long)s * nsite + site];
+  tables[ks * ld + s] = bases[ks * 2 + lab];
+}
+
+// _real_ratio (energy.jl:1-6) for real scalars: acc += num * f / den, where f
+// rescales a numerator taken from an unnormalised message (0 means 1).
+__global__ void ratio_kernel(const double* num, const double* den, const double* fac, double* acc, int* status,
+                             int* fail_site, int site, int nb) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nb || status[s] != 0) return;
+  const double d = den[s];
+  if (d == 0.0) {
+    status[s] = 4;
+    fail_site[s] = site;
+    return;
+  }
+  double f = fac ? fac[s] : 1.0;
+  if (f == 0.0) f = 1.0;
+  const double z = num[s] * f / d;
+  if (!isfinite(z)) {
+    status[s] = 5;
+    fail_site[s] = site;
+    return;
+  }
+  acc[s] += z;
+}
+
+// Pair leg of a double layer whose ket leg carries an extra odd unit charge.
+inline Leg flipped_pair_leg(int dim, int even) {
+  const int odd = dim - even;
+  return fixed_leg(dim * dim, 2 * odd * even);
+}
+
+// Physical-operator entry allowed by the Z2 grading of a leg with `even` of 2 indices.
+bool op_respects_grading(const double* op, int even, int odd_op) {
+  if (even == 2) return true;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      const int pa = a >= even, pb = b >= even;
+      if (((pa ^ pb) != odd_op) && op[a * 2 + b] != 0.0) return false;
+    }
+  return true;
+}
+
+template <typename T>
+struct EnergyRun {
+  Context& ctx;
+  const CacheDev<T>& c;
+  int Lx, Ly, ns, chi, nb = 0, cur = 0;
+  long long D = 1;                          // largest double-layer leg
+  std::vector<LayerSpec> spec[kKinds];      // [kind][site]
+  void** tables = nullptr;                  // [kind][site][nb]
+  int *status = nullptr, *fail = nullptr;
+  double *xsum = nullptr, *zzsum = nullptr, *den = nullptr, *num = nullptr, *numh = nullptr;
+  double *fac = nullptr, *facp = nullptr, *nrm = nullptr;
+
+  EnergyRun(Context& cx, const CacheDev<T>& cc, int chi_)
+      : ctx(cx), c(cc), Lx(cc.Lx), Ly(cc.Ly), ns(cc.Lx * cc.Ly), chi(chi_) {}
+
+  void* const* table(int kind, int site) const { return tables + ((long long)kind * ns + site) * nb; }
+  TRef layer(int kind, int site, const std::string& letters) const {
+    const LayerSpec& sp = spec[kind][site];
+    TRef r = dense_ref(nullptr, 0, letters, {sp.w, sp.n, sp.e, sp.s});
+    r.ptrs = table(kind, site);
+    return r;
+  }
+  T* take(long long elems) { return (T*)ctx.arena.take(sizeof(T) * (size_t)cur * (size_t)std::max(1LL, elems)); }
+  void dot(const TRef& a, const TRef& b, double* out) {
+    SPP_CHECK(launch_dot<T>(full_view(a), full_view(b), out, cur, ctx.stream));
+  }
+  // _safe_scale in place; the factor stays in `fac`
+  void normalize(const TRef& t, int site) {
+    FullView fv = full_view(t);
+    SPP_CHECK(launch_dot<T>(fv, fv, nrm, cur, ctx.stream));
+    SPP_CHECK(launch_safe_factor(nrm, fac, status, fail, site, cur, ctx.stream));
+    SPP_CHECK(launch_scale<T>(fv, fac, cur, ctx.stream));
+  }
+  void ratio(const double* nu, const double* de, const double* f, double* acc, int site) {
+    hipLaunchKernelGGL(ratio_kernel, dim3((cur + 255) / 256), dim3(256), 0, ctx.stream, nu, de, f, acc, status, fail,
+                       site, cur);
+    SPP_CHECK(hipGetLastError());
+  }
+  static long long count(const std::vector<Leg>& l) {
+    long long n = 1;
+    for (auto& g : l) n *= g.dim;
+    return n;
+  }
+
+  // ---------------- one-row terms (energy.jl:44-60) ----------------
+  void row_terms(const Boundary<T>& N, const Boundary<T>& S, int y) {
+    const size_t mark = ctx.arena.mark();
+    const std::vector<LayerSpec>& P = spec[kPlain];
+    const long long small = (long long)chi * D * chi, big = (long long)chi * D * D * chi;
+    std::vector<T*> R(Lx + 1);
+    std::vector<std::vector<Leg>> Rl(Lx + 1);
+    for (int x = 0; x <= Lx; ++x) {
+      Leg wl = x < Lx ? P[x + y * Lx].w : P[(Lx - 1) + y * Lx].e;
+      Rl[x] = {N.bond(x), wl, south_bond(S, Lx, x)};
+      R[x] = take(count(Rl[x]));
+    }
+    T* BX = take(big);
+    T* BY = take(big);
+    SPP_CHECK(launch_fill<T>(R[Lx], cur, T(1), ctx.stream));
+    auto Rref = [&](int x) { return dense_ref(R[x], count(Rl[x]), "ced", Rl[x]); };
+    // right messages (_right_step, boundary_mps.jl:240-246)
+    for (int x = Lx - 1; x >= 0; --x) {
+      const int site = x + y * Lx;
+      const LayerSpec& sp = P[site];
+      std::vector<Leg> xl = {N.bond(x), N.phys(x), Rl[x + 1][1], Rl[x + 1][2]};
+      TRef Xr = dense_ref(BX, count(xl), "aned", xl);
+      contract<T>(ctx, Xr, N.ref(x, "anc"), Rref(x + 1), cur);
+      std::vector<Leg> yl = {N.bond(x), Rl[x + 1][2], sp.w, sp.s};
+      TRef Yr = dense_ref(BY, count(yl), "adws", yl);
+      contract<T>(ctx, Yr, Xr, layer(kPlain, site, "wnes"), cur);
+      TRef Rr = dense_ref(R[x], count(Rl[x]), "awb", Rl[x]);
+      contract<T>(ctx, Rr, Yr, bottom_ref(S, Lx, x, "bsd", true), cur);
+      normalize(Rr, site);
+    }
+    // left sweep (_left_step, boundary_mps.jl:233-239)
+    T* Lb = take(small);
+    T* O0 = take(small);
+    T* Ot = take(small);
+    T* OC = take(small);
+    SPP_CHECK(launch_fill<T>(Lb, cur, T(1), ctx.stream));
+    std::vector<Leg> Ll = {N.bond(0), P[y * Lx].w, south_bond(S, Lx, 0)};
+    // X = L.T over the north bond
+    auto make_x = [&](const TRef& L, int x) {
+      std::vector<Leg> xl = {L.leg[1], L.leg[2], N.phys(x), N.bond(x + 1)};
+      TRef Xr = dense_ref(BX, count(xl), "wbnc", xl);
+      contract<T>(ctx, Xr, L, N.ref(x, "anc"), cur);
+      return Xr;
+    };
+    // (X.E).B for the layer of the given kind at column x
+    auto finish = [&](const TRef& Xr, int kind, int x, T* outbuf) {
+      const LayerSpec& q = spec[kind][x + y * Lx];
+      std::vector<Leg> yl = {Xr.leg[1], N.bond(x + 1), q.e, q.s};
+      TRef Yr = dense_ref(BY, count(yl), "bces", yl);
+      contract<T>(ctx, Yr, Xr, layer(kind, x + y * Lx, "wnes"), cur);
+      std::vector<Leg> ol = {N.bond(x + 1), q.e, south_bond(S, Lx, x + 1)};
+      TRef Or = dense_ref(outbuf, count(ol), "ced", ol);
+      contract<T>(ctx, Or, Yr, bottom_ref(S, Lx, x, "bsd", true), cur);
+      return Or;
+    };
+    for (int x = 0; x < Lx; ++x) {
+      const int site = x + y * Lx;
+      TRef Lr = dense_ref(Lb, count(Ll), "awb", Ll);
+      TRef Xr = make_x(Lr, x);
+      TRef O0r = finish(Xr, kPlain, x, O0);
+      dot(O0r, Rref(x + 1), den);
+      TRef Otr = finish(Xr, kOnsite, x, Ot);
+      dot(Otr, Rref(x + 1), num);
+      ratio(num, den, nullptr, xsum, site);
+      if (x > 0) ratio(numh, den, facp, zzsum, site - 1);
+      if (x < Lx - 1) {
+        TRef OCr = finish(Xr, kHorzC, x, OC);
+        TRef XD = make_x(with_letters(OCr, "awb"), x + 1);
+        TRef ODr = finish(XD, kHorzD, x + 1, Ot);
+        dot(ODr, Rref(x + 2), numh);
+        normalize(O0r, site);
+        std::swap(fac, facp);
+        std::swap(Lb, O0);
+        Ll = O0r.leg;
+      }
+    }
+    ctx.arena.release(mark);
+  }
+
+  // ---------------- two-row (vertical) terms (energy.jl:61-72) ----------------
+  void slab_terms(const Boundary<T>& N, const Boundary<T>& S, int y) {
+    const size_t mark = ctx.arena.mark();
+    const std::vector<LayerSpec>& P = spec[kPlain];
+    const long long mid = (long long)chi * D * D * chi, big = mid * D;
+    auto s1 = [&](int x) { return x + y * Lx; };
+    auto s2 = [&](int x) { return x + (y + 1) * Lx; };
+    std::vector<T*> R(Lx + 1);
+    std::vector<std::vector<Leg>> Rl(Lx + 1);
+    for (int x = 0; x <= Lx; ++x) {
+      Leg w1 = x < Lx ? P[s1(x)].w : P[s1(Lx - 1)].e;
+      Leg w2 = x < Lx ? P[s2(x)].w : P[s2(Lx - 1)].e;
+      Rl[x] = {N.bond(x), w1, w2, south_bond(S, Lx, x)};
+      R[x] = take(count(Rl[x]));
+    }
+    T* BX = take(big);
+    T* BY = take(big);
+    SPP_CHECK(launch_fill<T>(R[Lx], cur, T(1), ctx.stream));
+    auto Rref = [&](int x) { return dense_ref(R[x], count(Rl[x]), "cefd", Rl[x]); };
+    // right slab messages (SlabColumn _right_step, boundary_mps.jl:224-231)
+    for (int x = Lx - 1; x >= 0; --x) {
+      const LayerSpec& u = P[s1(x)];
+      const LayerSpec& l = P[s2(x)];
+      const Leg d = Rl[x + 1][3];
+      std::vector<Leg> xl = {N.bond(x), u.n, Rl[x + 1][1], Rl[x + 1][2], d};
+      TRef Xr = dense_ref(BX, count(xl), "anefd", xl);
+      contract<T>(ctx, Xr, N.ref(x, "anc"), Rref(x + 1), cur);
+      std::vector<Leg> yl = {N.bond(x), u.w, u.s, Rl[x + 1][2], d};
+      TRef Yr = dense_ref(BY, count(yl), "awkfd", yl);
+      contract<T>(ctx, Yr, Xr, layer(kPlain, s1(x), "wnek"), cur);
+      std::vector<Leg> zl = {N.bond(x), u.w, l.w, l.s, d};
+      TRef Zr = dense_ref(BX, count(zl), "awvsd", zl);
+      contract<T>(ctx, Zr, Yr, layer(kPlain, s2(x), "vkfs"), cur);
+      TRef Rr = dense_ref(R[x], count(Rl[x]), "awvb", Rl[x]);
+      contract<T>(ctx, Rr, Zr, bottom_ref(S, Lx, x, "bsd", true), cur);
+      normalize(Rr, s1(x));
+    }
+    // left slab sweep (SlabColumn _left_step, boundary_mps.jl:216-223)
+    T* Lb = take(mid);
+    T* O0 = take(mid);
+    T* Oi = take(mid);
+    SPP_CHECK(launch_fill<T>(Lb, cur, T(1), ctx.stream));
+    std::vector<Leg> Ll = {N.bond(0), P[s1(0)].w, P[s2(0)].w, south_bond(S, Lx, 0)};
+    auto step = [&](const TRef& L, int x, int ku, int kl, T* outbuf) {
+      const LayerSpec& u = spec[ku][s1(x)];
+      const LayerSpec& l = spec[kl][s2(x)];
+      const Leg b = L.leg[3], c = N.bond(x + 1);
+      std::vector<Leg> xl = {L.leg[1], L.leg[2], b, N.phys(x), c};
+      TRef Xr = dense_ref(BX, count(xl), "wvbnc", xl);
+      contract<T>(ctx, Xr, L, N.ref(x, "anc"), cur);
+      std::vector<Leg> yl = {L.leg[2], b, c, u.e, u.s};
+      TRef Yr = dense_ref(BY, count(yl), "vbcek", yl);
+      contract<T>(ctx, Yr, Xr, layer(ku, s1(x), "wnek"), cur);
+      std::vector<Leg> zl = {b, c, u.e, l.e, l.s};
+      TRef Zr = dense_ref(BX, count(zl), "bcefs", zl);
+      contract<T>(ctx, Zr, Yr, layer(kl, s2(x), "vkfs"), cur);
+      std::vector<Leg> ol = {c, u.e, l.e, south_bond(S, Lx, x + 1)};
+      TRef Or = dense_ref(outbuf, count(ol), "cefd", ol);
+      contract<T>(ctx, Or, Zr, bottom_ref(S, Lx, x, "bsd", true), cur);
+      return Or;
+    };
+    for (int x = 0; x < Lx; ++x) {
+      TRef Lr = dense_ref(Lb, count(Ll), "awvb", Ll);
+      TRef O0r = step(Lr, x, kPlain, kPlain, O0);
+      dot(O0r, Rref(x + 1), den);
+      TRef Oir = step(Lr, x, kVertC, kVertD, Oi);
+      dot(Oir, Rref(x + 1), num);
+      ratio(num, den, nullptr, zzsum, s1(x));
+      if (x < Lx - 1) {
+        normalize(O0r, s1(x));
+        std::swap(Lb, O0);
+        Ll = O0r.leg;
+      }
+    }
+    ctx.arena.release(mark);
+  }
+};
+
+}  // namespace
+
+template <typename T>
+void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>& c, int chi, int n,
+                          const int* labels_h, const EnergyOps& ops, EnergyOutputs& out) {
+  hipStream_t st = ctx.stream;
+  const int Lx = c.Lx, Ly = c.Ly, ns = Lx * Ly;
+  if (pepo.Lx != Lx || pepo.Ly != Ly || (int)pepo.sites.size() != ns)
+    throw HipError("the state does not match the sampling cache");
+  if (chi < 1) throw HipError("chi must be positive");
+  if (n <= 0) return;
+  if (!labels_h) throw HipError("labels are required");
+  for (long long i = 0; i < (long long)n * ns; ++i)
+    if (labels_h[i] != 0 && labels_h[i] != 1) throw HipError("labels must be 0 or 1");
+  const int pe = pepo.sites[0].even[2];
+  for (const SiteDev& sd : pepo.sites)
+    if (sd.dim[2] != 2 || sd.even[2] != pe) throw HipError("energy needs one d = 2 physical space on every site");
+  if (pe == 2 && ops.odd) throw HipError("an odd factor leg needs a Z2-graded physical space");
+  if (!op_respects_grading(ops.X, pe, 0) || !op_respects_grading(ops.F, pe, ops.odd) ||
+      !op_respects_grading(ops.G, pe, ops.odd))
+    throw HipError("operator does not respect the Z2 grading of the physical space");
+
+  EnergyRun<T> run(ctx, c, chi);
+  // ---------------- impurity layers ----------------
+  std::vector<void*> owned;
+  std::vector<void*> base((size_t)kKinds * ns * 2, nullptr);
+  std::vector<LayerJob> jobs;
+  long long maxout = 1;
+  for (int k = 0; k < kKinds; ++k) run.spec[k].resize(ns);
+  for (int i = 0; i < ns; ++i) {
+    const SiteDev& sd = pepo.sites[i];
+    const int x = i % Lx, y = i / Lx;
+    const LayerSpec& p = c.spec[i];
+    auto flip = [&](int k) { return ops.odd ? flipped_pair_leg(sd.dim[k], sd.even[k]) : pair_leg(sd.dim[k], sd.even[k]); };
+    run.spec[kPlain][i] = p;
+    run.spec[kOnsite][i] = p;
+    run.spec[kHorzC][i] = LayerSpec{p.w, p.n, flip(4), p.s};
+    run.spec[kHorzD][i] = LayerSpec{flip(0), p.n, p.e, p.s};
+    run.spec[kVertC][i] = LayerSpec{p.w, p.n, p.e, flip(5)};
+    run.spec[kVertD][i] = LayerSpec{p.w, flip(1), p.e, p.s};
+    for (int k = 0; k < 2; ++k) base[((size_t)kPlain * ns + i) * 2 + k] = c.cand[k][i];
+    for (int kind = kOnsite; kind < kKinds; ++kind) {
+      const bool used = kind == kOnsite || (kind == kHorzC && x < Lx - 1) || (kind == kHorzD && x > 0) ||
+                        (kind == kVertC && y < Ly - 1) || (kind == kVertD && y > 0);
+      for (int k = 0; k < 2; ++k) {
+        void*& slot = base[((size_t)kind * ns + i) * 2 + k];
+        if (!used) {
+          slot = c.cand[k][i];  // never read
+          continue;
+        }
+        const LayerSpec& q = run.spec[kind][i];
+        const long long cnt = (long long)q.w.dim * q.n.dim * q.e.dim * q.s.dim;
+        maxout = std::max(maxout, cnt);
+        void* buf = nullptr;
+        SPP_CHECK(hipMalloc(&buf, sizeof(T) * cnt));
+        owned.push_back(buf);
+        slot = buf;
+        LayerJob J{};
+        J.ket = sd.d;
+        J.bra = sd.d;
+        for (int l = 0; l < 6; ++l) {
+          J.dk[l] = J.db[l] = sd.dim[l];
+          J.ek[l] = J.eb[l] = sd.even[l];
+        }
+        J.out = buf;
+        J.use_aux = 1;
+        J.aux[0] = c.basis[0 * 2 + k];
+        J.aux[1] = c.basis[1 * 2 + k];
+        J.use_op = 1;
+        const double* op = kind == kOnsite ? ops.X : (kind == kHorzC || kind == kVertC) ? ops.F : ops.G;
+        std::memcpy(J.op, op, sizeof(double) * 4);
+        const int fl = kind == kHorzC ? 4 : kind == kHorzD ? 0 : kind == kVertC ? 5 : kind == kVertD ? 1 : -1;
+        if (fl >= 0) J.flipk[fl] = ops.odd;
+        jobs.push_back(J);
+      }
+    }
+  }
+  for (int k = 0; k < kKinds; ++k)
+    for (const LayerSpec& q : run.spec[k])
+      run.D = std::max<long long>(run.D, std::max(std::max(q.w.dim, q.n.dim), std::max(q.e.dim, q.s.dim)));
+
+  std::vector<Boundary<T>> south(Ly + 1);
+  Boundary<T> NB[2];
+  LayerJob* djobs = nullptr;
+  void** dbase = nullptr;
+  int* dlabels = nullptr;
+  double *mant = nullptr, *lnamp = nullptr;
+  long long* dsst = nullptr;
+  T** dtens = nullptr;
+  std::vector<void*> scratch;
+  auto dev = [&](size_t bytes) {
+    void* p = nullptr;
+    SPP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+    scratch.push_back(p);
+    return p;
+  };
+  auto cleanup = [&]() {
+    for (auto& b : south) b.release();
+    NB[0].release();
+    NB[1].release();
+    for (void* p : scratch) (void)hipFree(p);
+    scratch.clear();
+    for (void* p : owned) (void)hipFree(p);
+    owned.clear();
+  };
+  try {
+    if (!jobs.empty()) {
+      djobs = (LayerJob*)dev(sizeof(LayerJob) * jobs.size());
+      SPP_CHECK(hipMemcpyAsync(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), hipMemcpyHostToDevice, st));
+      SPP_CHECK(launch_double_layers<T>(djobs, (int)jobs.size(), maxout, st));
+    }
+    // ---------------- chunk size from the memory budget ----------------
+    const long long D = run.D;
+    const double ts = sizeof(T);
+    const double small = (double)chi * D * chi, mid = small * D, big = mid * D;
+    const double row_b = ts * ((Lx + 5) * small + 2.0 * mid);
+    const double slab_b = ts * ((Lx + 4) * mid + 2.0 * big);
+    const double absorb_b = absorb_bytes_per_sample(Lx, chi, D, sizeof(T), ctx.ksub);
+    const size_t per_arena = (size_t)(1.15 * std::max(std::max(row_b, slab_b), absorb_b)) + 8192;
+    long long pmax = 1;
+    for (const LayerSpec& q : c.spec) pmax = std::max<long long>(pmax, std::max(q.n.dim, q.s.dim));
+    const long long capS = (long long)chi * pmax * chi;
+    const size_t per_persist = (size_t)(ts * capS * Lx * (Ly + 3)) + sizeof(int) * (Lx + 1) * (Ly + 3) +
+                               sizeof(void*) * kKinds * ns + 128;
+    SPP_CHECK(hipStreamSynchronize(st));
+    size_t freeb = 0, totalb = 0;
+    SPP_CHECK(hipMemGetInfo(&freeb, &totalb));
+    const double avail = 0.6 * (double)freeb + (double)ctx.arena.capacity();
+    long long chunk = std::max<long long>(1, (long long)(avail / (double)(per_arena + per_persist)));
+    if (ctx.chunk > 0) chunk = std::min<long long>(chunk, ctx.chunk);
+    chunk = std::min<long long>(chunk, n);
+    chunk = std::min<long long>(chunk, 16384);
+    const int nb = (int)chunk;
+    ctx.arena.reserve(per_arena * (size_t)nb);
+    run.nb = nb;
+    for (int y = 0; y <= Ly; ++y) prepare_boundary(south[y], Lx, nb, capS);
+    prepare_boundary(NB[0], Lx, nb, capS);
+    prepare_boundary(NB[1], Lx, nb, capS);
+    run.tables = (void**)dev(sizeof(void*) * (size_t)kKinds * ns * nb);
+    dbase = (void**)dev(sizeof(void*) * base.size());
+    dlabels = (int*)dev(sizeof(int) * (size_t)n * ns);
+    run.status = (int*)dev(sizeof(int) * nb);
+    run.fail = (int*)dev(sizeof(int) * nb);
+    run.xsum = (double*)dev(sizeof(double) * nb);
+    run.zzsum = (double*)dev(sizeof(double) * nb);
+    run.den = (double*)dev(sizeof(double) * nb);
+    run.num = (double*)dev(sizeof(double) * nb);
+    run.numh = (double*)dev(sizeof(double) * nb);
+    run.fac = (double*)dev(sizeof(double) * nb);
+    run.facp = (double*)dev(sizeof(double) * nb);
+    run.nrm = (double*)dev(sizeof(double) * nb);
+    mant = (double*)dev(sizeof(double) * nb);
+    lnamp = (double*)dev(sizeof(double) * nb);
+    dsst = (long long*)dev(sizeof(long long) * Lx);
+    dtens = (T**)dev(sizeof(T*) * Lx);
+    SPP_CHECK(hipMemcpyAsync(dbase, base.data(), sizeof(void*) * base.size(), hipMemcpyHostToDevice, st));
+    SPP_CHECK(hipMemcpyAsync(dlabels, labels_h, sizeof(int) * (size_t)n * ns, hipMemcpyHostToDevice, st));
+    std::vector<int> h_status(nb), h_fail(nb);
+    std::vector<double> h_x(nb), h_zz(nb), h_mant(nb), h_ls(nb), h_mtn(nb), h_mts(nb), h_amp(nb);
+    for (long long s0 = 0; s0 < n; s0 += nb) {
+      const int cur = (int)std::min<long long>(nb, n - s0);
+      run.cur = cur;
+      const int* lab = dlabels + s0 * ns;
+      const long long tot = (long long)kKinds * ns * cur;
+      hipLaunchKernelGGL(layer_tables_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, lab, ns,
+                         (void* const*)dbase, (int)kKinds, run.tables, cur, nb);
+      SPP_CHECK(hipGetLastError());
+      SPP_CHECK(launch_fill<int>(run.status, nb, 0, st));
+      SPP_CHECK(launch_fill<int>(run.fail, nb, -1, st));
+      SPP_CHECK(launch_fill<double>(run.xsum, nb, 0.0, st));
+      SPP_CHECK(launch_fill<double>(run.zzsum, nb, 0.0, st));
+      // south boundaries of the projected samples, rotated frame
+      for (auto& b : south) b.nb = cur;
+      std::vector<Leg> physS(Lx);
+      for (int xr = 0; xr < Lx; ++xr) physS[xr] = c.spec[(Lx - 1 - xr) + (Ly - 1) * Lx].s;
+      set_trivial(ctx, south[Ly], physS);
+      for (int y = Ly - 1; y >= 0; --y) {
+        RowLayers<T> rr;
+        rr.L = Lx;
+        rr.rotated = true;
+        rr.spec.resize(Lx);
+        rr.ptrs.resize(Lx);
+        for (int xr = 0; xr < Lx; ++xr) {
+          const int x = Lx - 1 - xr;
+          const LayerSpec& sp = c.spec[x + y * Lx];
+          rr.spec[xr] = LayerSpec{sp.e, sp.s, sp.w, sp.n};
+          rr.ptrs[xr] = run.table(kPlain, x + y * Lx);
+        }
+        absorb_forward<T>(ctx, south[y + 1], rr, chi, south[y]);
+      }
+      // north sweep with the row and slab terms
+      NB[0].nb = cur;
+      NB[1].nb = cur;
+      std::vector<Leg> physN(Lx);
+      for (int x = 0; x < Lx; ++x) physN[x] = c.spec[x].n;
+      set_trivial(ctx, NB[0], physN);
+      int which = 0;
+      for (int y = 0; y < Ly; ++y) {
+        Boundary<T>& Nc = NB[which];
+        run.row_terms(Nc, south[y + 1], y);
+        if (y < Ly - 1) run.slab_terms(Nc, south[y + 2], y);
+        RowLayers<T> row;
+        row.L = Lx;
+        row.spec.assign(c.spec.begin() + y * Lx, c.spec.begin() + (y + 1) * Lx);
+        row.ptrs.resize(Lx);
+        for (int x = 0; x < Lx; ++x) row.ptrs[x] = run.table(kPlain, x + y * Lx);
+        absorb_forward<T>(ctx, Nc, row, chi, NB[1 - which]);
+        which = 1 - which;
+      }
+      // log norm (lognorm, boundary_mps.jl:180-191)
+      Boundary<T>& Nf = NB[which];
+      for (int x = 0; x <= Lx; ++x)
+        if (Nf.bdim[x] != 1) throw HipError("final boundary must have unit bonds");
+      std::vector<long long> sst(Lx);
+      for (int x = 0; x < Lx; ++x) sst[x] = Nf.tsize(x);
+      SPP_CHECK(hipMemcpyAsync(dsst, sst.data(), sizeof(long long) * Lx, hipMemcpyHostToDevice, st));
+      SPP_CHECK(hipMemcpyAsync(dtens, Nf.t.data(), sizeof(T*) * Lx, hipMemcpyHostToDevice, st));
+      SPP_CHECK(launch_close_unit<T>(dtens, dsst, Lx, mant, cur, st));
+      SPP_CHECK(launch_gather_lognorm(c.cand_lognorm, lab, ns, lnamp, cur, st));
+      SPP_CHECK(hipMemcpyAsync(h_status.data(), run.status, sizeof(int) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_fail.data(), run.fail, sizeof(int) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_x.data(), run.xsum, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_zz.data(), run.zzsum, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_mant.data(), mant, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_ls.data(), Nf.log_scale, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_mtn.data(), Nf.max_trunc, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_mts.data(), south[0].max_trunc, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_amp.data(), lnamp, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipStreamSynchronize(st));
+      for (int s = 0; s < cur; ++s) {
+        const long long g = s0 + s;
+        int stt = h_status[s];
+        const double m = h_mant[s];
+        if (stt == 0 && !(m > 0.0 && std::isfinite(m))) stt = 6;
+        const double tln = std::log(m) + h_ls[s] - 2.0 * h_amp[s];
+        const double e = stt == 0 ? -ops.J * h_zz[s] - ops.h * h_x[s] : NAN;
+        if (out.energy_total) out.energy_total[g] = e;
+        if (out.energy_per_site) out.energy_per_site[g] = e / ns;
+        if (out.tensor_log_norm) out.tensor_log_norm[g] = stt == 0 ? tln : NAN;
+        if (out.log_norm) out.log_norm[g] = stt == 0 ? tln + 2.0 * (c.pepo_log_amplitude + h_amp[s]) : NAN;
+        if (out.max_trunc) out.max_trunc[g] = std::max(h_mtn[s], h_mts[s]);
+        if (out.status) out.status[g] = stt;
+        if (out.fail_site) out.fail_site[g] = stt == 0 ? -1 : h_fail[s];
+      }
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(st);
+    ctx.arena.release(0);
+    cleanup();
+    throw;
+  }
+  SPP_CHECK(hipStreamSynchronize(st));
+  cleanup();
+}
+
+template void measure_energy_batch<float>(Context&, const StateDev&, const CacheDev<float>&, int, int, const int*,
+                                          const EnergyOps&, EnergyOutputs&);
+template void measure_energy_batch<double>(Context&, const StateDev&, const CacheDev<double>&, int, int, const int*,
+                                           const EnergyOps&, EnergyOutputs&);
+
+}  // namespace spp

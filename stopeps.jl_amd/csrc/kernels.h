@@ -32,6 +32,20 @@ hipError_t launch_select_copy(const FullView& out, const FullView& a, const Full
 template <typename T>
 hipError_t launch_fill(T* p, long long n, T v, hipStream_t st);
 
+// factor[s] = 1/sqrt(nrm2[s]) (0 when the norm is 0, which leaves the tensor
+// untouched); a nonfinite norm sets status[s] = 3 and fail_site[s] = site.
+hipError_t launch_safe_factor(const double* nrm2, double* factor, int* status, int* fail_site, int site, int nb,
+                              hipStream_t st);
+
+// out[s] = sum_i cand_lognorm[2 i + labels[s][i]].
+hipError_t launch_gather_lognorm(const double* cand_lognorm, const int* labels, int nsite, double* out, int nb,
+                                 hipStream_t st);
+
+// out[s] = prod_x tensors[x][s * sstrides[x]]: a closed boundary with unit bonds.
+template <typename T>
+hipError_t launch_close_unit(T* const* tensors, const long long* sstrides, int L, double* out, int nb,
+                             hipStream_t st);
+
 // Convert between precisions.
 template <typename TO, typename TI>
 hipError_t launch_convert(TO* out, const TI* in, long long n, hipStream_t st);
@@ -39,7 +53,9 @@ hipError_t launch_convert(TO* out, const TI* in, long long n, hipStream_t st);
 // Double layer of site tensors (W,N,P,A,E,S) -> E[W,N,E,S] with Delta legs in
 // the parity-sorted pair order; optional physical operator O[p',p] (2x2) and
 // auxiliary projector u[a] (projects the aux leg onto u, the same vector on
-// ket and bra). ``ket``/``bra`` point to FP64 dense sites with leg dims and
+// ket and bra). A ket leg with flipk set is the fusion of a virtual leg with a
+// unit leg of odd parity: its parity-sorted order lists the original odd
+// indices first. ``ket``/``bra`` point to FP64 dense sites with leg dims and
 // even counts given per site. One launch builds ``nsites`` layers.
 struct LayerJob {
   const double* ket;
@@ -51,6 +67,7 @@ struct LayerJob {
   int use_op;
   double aux[2];      // aux projector weights; if use_aux == 0: trace over aux
   int use_aux;
+  int flipk[6];       // ket virtual leg carries an extra odd unit charge: its parity order is reversed
 };
 template <typename T>
 hipError_t launch_double_layers(const LayerJob* jobs_dev, int njobs, long long maxout, hipStream_t st);

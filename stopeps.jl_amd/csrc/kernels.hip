@@ -119,6 +119,68 @@ hipError_t launch_fill(T* p, long long n, T v, hipStream_t st) {
   return hipGetLastError();
 }
 
+// _safe_scale (boundary_mps.jl:257-260): factor = 1/norm (0 keeps the tensor),
+// nonfinite norm -> status 3 at the given site.
+__global__ void safe_scale_kernel(const double* nrm2, double* factor, int* status, int* fail_site, int site,
+                                  int nb) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nb) return;
+  double n = sqrt(nrm2[s]);
+  if (!isfinite(n)) {
+    if (status[s] == 0) {
+      status[s] = 3;
+      fail_site[s] = site;
+    }
+    factor[s] = 0.0;
+    return;
+  }
+  factor[s] = n == 0.0 ? 0.0 : 1.0 / n;
+}
+
+hipError_t launch_safe_factor(const double* nrm2, double* factor, int* status, int* fail_site, int site, int nb,
+                              hipStream_t st) {
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(safe_scale_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, nrm2, factor, status, fail_site,
+                     site, nb);
+  return hipGetLastError();
+}
+
+// out[s] = sum_i cand_lognorm[2 i + labels[s][i]] (log norms of the chosen projected sites).
+__global__ void gather_lognorm_kernel(const double* cand_lognorm, const int* labels, int nsite, double* out, int nb) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nb) return;
+  double acc = 0.0;
+  for (int i = 0; i < nsite; ++i) acc += cand_lognorm[2 * i + labels[(long long)s * nsite + i]];
+  out[s] = acc;
+}
+
+// Mantissa of a closed boundary whose bonds all have dimension one
+// (_close_boundary, boundary_mps.jl:153-160).
+template <typename T>
+__global__ void close_unit_kernel(T* const* tensors, const long long* sstrides, int L, double* out, int nb) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nb) return;
+  double v = 1.0;
+  for (int x = 0; x < L; ++x) v *= (double)tensors[x][(long long)s * sstrides[x]];
+  out[s] = v;
+}
+
+hipError_t launch_gather_lognorm(const double* cand_lognorm, const int* labels, int nsite, double* out, int nb,
+                                 hipStream_t st) {
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_lognorm_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, cand_lognorm, labels, nsite,
+                     out, nb);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_close_unit(T* const* tensors, const long long* sstrides, int L, double* out, int nb,
+                             hipStream_t st) {
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(close_unit_kernel<T>, dim3((nb + 255) / 256), dim3(256), 0, st, tensors, sstrides, L, out, nb);
+  return hipGetLastError();
+}
+
 template <typename TO, typename TI>
 __global__ void convert_kernel(TO* out, const TI* in, long long n) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -165,18 +227,27 @@ __global__ void double_layer_kernel(const LayerJob* jobs) {
   const int ie = (int)(r % DE); r /= DE;
   const int in_ = (int)(r % DN); r /= DN;
   const int iw = (int)r;
+  // logical (parity-sorted) ket even counts; flipped legs list the odd indices first
+  int lek[6];
+  for (int k = 0; k < 6; ++k) lek[k] = J.flipk[k] ? J.dk[k] - J.ek[k] : J.ek[k];
   int wk, wb, nk, nb, ek, eb, sk, sb;
-  split_pair(iw, J.dk[0], J.ek[0], J.db[0], J.eb[0], wk, wb);
-  split_pair(in_, J.dk[1], J.ek[1], J.db[1], J.eb[1], nk, nb);
-  split_pair(ie, J.dk[4], J.ek[4], J.db[4], J.eb[4], ek, eb);
-  split_pair(is, J.dk[5], J.ek[5], J.db[5], J.eb[5], sk, sb);
+  split_pair(iw, J.dk[0], lek[0], J.db[0], J.eb[0], wk, wb);
+  split_pair(in_, J.dk[1], lek[1], J.db[1], J.eb[1], nk, nb);
+  split_pair(ie, J.dk[4], lek[4], J.db[4], J.eb[4], ek, eb);
+  split_pair(is, J.dk[5], lek[5], J.db[5], J.eb[5], sk, sb);
   T* out = reinterpret_cast<T*>(J.out);
-  const int pk = par_of(wk, J.ek[0]) + par_of(nk, J.ek[1]) + par_of(ek, J.ek[4]) + par_of(sk, J.ek[5]);
+  const int pk = par_of(wk, lek[0]) + par_of(nk, lek[1]) + par_of(ek, lek[4]) + par_of(sk, lek[5]);
   const int pb = par_of(wb, J.eb[0]) + par_of(nb, J.eb[1]) + par_of(eb, J.eb[4]) + par_of(sb, J.eb[5]);
   if (((pk + pb) & 1) != 0) {
     out[t] = T(0);
     return;
   }
+  // logical -> storage ket index of flipped legs
+  auto unflip = [&](int i, int k) { return J.flipk[k] ? (i < lek[k] ? J.ek[k] + i : i - lek[k]) : i; };
+  wk = unflip(wk, 0);
+  nk = unflip(nk, 1);
+  ek = unflip(ek, 4);
+  sk = unflip(sk, 5);
   // strides of the (W,N,P,A,E,S) row-major site arrays
   long long skt[6], sbt[6];
   skt[5] = 1;
@@ -383,6 +454,8 @@ template hipError_t launch_convert<float, double>(float*, const double*, long lo
 template hipError_t launch_convert<double, float>(double*, const float*, long long, hipStream_t);
 template hipError_t launch_convert<double, double>(double*, const double*, long long, hipStream_t);
 template hipError_t launch_convert<float, float>(float*, const float*, long long, hipStream_t);
+template hipError_t launch_close_unit<float>(float* const*, const long long*, int, double*, int, hipStream_t);
+template hipError_t launch_close_unit<double>(double* const*, const long long*, int, double*, int, hipStream_t);
 template hipError_t launch_double_layers<float>(const LayerJob*, int, long long, hipStream_t);
 template hipError_t launch_double_layers<double>(const LayerJob*, int, long long, hipStream_t);
 

@@ -11,6 +11,7 @@
 // of drawn candidates is then absorbed into the per-sample north boundary
 // (born.jl:157-160).
 #include "sampler.h"
+#include "boundary_ops.h"
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -69,23 +70,6 @@ __global__ void proj_lognorm_kernel(const double* a, int n_before, int naux, int
     out[0] = 0.5 * log(part[0][0]);
     out[1] = 0.5 * log(part[1][0]);
   }
-}
-
-// _safe_scale: factor = 1/norm (0 keeps the tensor), nonfinite norm -> status 3
-__global__ void safe_scale_kernel(const double* nrm2, double* factor, int* status, int* fail_site, int site,
-                                  int nb) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nb) return;
-  double n = sqrt(nrm2[s]);
-  if (!isfinite(n)) {
-    if (status[s] == 0) {
-      status[s] = 3;
-      fail_site[s] = site;
-    }
-    factor[s] = 0.0;
-    return;
-  }
-  factor[s] = n == 0.0 ? 0.0 : 1.0 / n;
 }
 
 // Born conditional and categorical draw for one site, all samples.
@@ -154,73 +138,7 @@ __global__ void make_ptrs_kernel(const int* choice, void* c0, void* c1, void** p
   ptrs[s] = choice[s] == 0 ? c0 : c1;
 }
 
-__global__ void gather_lognorm_kernel(const double* cand_lognorm, const int* labels, int nsite, double* out, int nb) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nb) return;
-  double acc = 0.0;
-  for (int i = 0; i < nsite; ++i) acc += cand_lognorm[2 * i + labels[(long long)s * nsite + i]];
-  out[s] = acc;
-}
-
-// mantissa of the closed boundary when every bond has dimension one
-template <typename T>
-__global__ void close_unit_kernel(T* const* tensors, const long long* sstrides, int L, double* out, int nb) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nb) return;
-  double v = 1.0;
-  for (int x = 0; x < L; ++x) v *= (double)tensors[x][(long long)s * sstrides[x]];
-  out[s] = v;
-}
-
 // ---------------------------------------------------------------------------
-
-template <typename T>
-static void prepare_boundary(Boundary<T>& b, int L, int nb, long long cap_elems) {
-  b.L = L;
-  b.nb = nb;
-  b.bdim.assign(L + 1, 1);
-  b.beven.assign(L + 1, 1);
-  b.bsplit.assign(L + 1, nullptr);
-  b.pdim.assign(L, 1);
-  b.peven.assign(L, 1);
-  b.t.assign(L, nullptr);
-  for (int x = 0; x < L; ++x) SPP_CHECK(hipMalloc(&b.t[x], sizeof(T) * nb * cap_elems));
-  for (int k = 1; k < L; ++k) SPP_CHECK(hipMalloc(&b.bsplit[k], sizeof(int) * nb));
-  SPP_CHECK(hipMalloc(&b.log_scale, sizeof(double) * nb));
-  SPP_CHECK(hipMalloc(&b.max_trunc, sizeof(double) * nb));
-  b.owns = true;
-}
-
-template <typename T>
-static void set_trivial(Context& ctx, Boundary<T>& b, const std::vector<Leg>& phys) {
-  for (int x = 0; x <= b.L; ++x) {
-    b.bdim[x] = 1;
-    b.beven[x] = 1;
-  }
-  for (int k = 1; k < b.L; ++k) SPP_CHECK(launch_fill<int>(b.bsplit[k], b.nb, 1, ctx.stream));
-  for (int x = 0; x < b.L; ++x) {
-    if (phys[x].dim != 1) throw HipError("nontrivial OBC leg");
-    b.pdim[x] = 1;
-    b.peven[x] = phys[x].even;
-    SPP_CHECK(launch_fill<T>(b.t[x], b.nb, T(1), ctx.stream));
-  }
-  SPP_CHECK(launch_fill<double>(b.log_scale, b.nb, 0.0, ctx.stream));
-  SPP_CHECK(launch_fill<double>(b.max_trunc, b.nb, 0.0, ctx.stream));
-}
-
-// Replace per-sample splits of a batch-1 boundary by fixed even counts.
-template <typename T>
-static void freeze_splits(Context& ctx, Boundary<T>& b) {
-  for (int k = 0; k <= b.L; ++k) {
-    if (!b.bsplit[k]) continue;
-    int v = 0;
-    SPP_CHECK(hipMemcpyAsync(&v, b.bsplit[k], sizeof(int), hipMemcpyDeviceToHost, ctx.stream));
-    SPP_CHECK(hipStreamSynchronize(ctx.stream));
-    SPP_CHECK(hipFree(b.bsplit[k]));
-    b.bsplit[k] = nullptr;
-    b.beven[k] = v;
-  }
-}
 
 template <typename T>
 static size_t draw_bytes_per_sample(const CacheDev<T>& c, const Context& ctx) {
@@ -229,13 +147,7 @@ static size_t draw_bytes_per_sample(const CacheDev<T>& c, const Context& ctx) {
   const long long Lx = c.Lx;
   const double ts = sizeof(T);
   double rowenv = ts * ((Lx + 4) * chi * D * chi + 3.0 * chi * D * D * chi) + 64 * 8;
-  double z = ts * 2.0 * chi * chi * D * D * D;
-  double g = ts * (Lx + 1) * (double)(chi * D) * (chi * D);
-  double sweep = ts * 5.0 * chi * chi * D * D + (8.0 + ts) * 2.0 * (chi * D / 2.0 + 1) * (chi * D / 2.0 + 1) +
-                 8.0 * 2.0 * (4.0 * (chi * D / 2.0 + 1) * ctx.ksub + 3.0 * ctx.ksub * ctx.ksub) +
-                 ts * Lx * chi * chi * D;
-  double absorb = z + g + sweep;
-  return (size_t)(1.15 * std::max(rowenv, absorb)) + 4096;
+  return (size_t)(1.15 * std::max(rowenv, absorb_bytes_per_sample(Lx, chi, D, sizeof(T), ctx.ksub))) + 4096;
 }
 
 template <typename T>
@@ -334,18 +246,6 @@ void build_cache(Context& ctx, const StateDev& pepo, int chi, const double* basi
 
 template void build_cache<float>(Context&, const StateDev&, int, const double*, CacheDev<float>&);
 template void build_cache<double>(Context&, const StateDev&, int, const double*, CacheDev<double>&);
-
-// Reference to the south tensor below original site x with letters (b, s, d).
-template <typename T>
-static TRef bottom_ref(const Boundary<T>& sb, int Lx, int x, const std::string& bsd) {
-  int xr = Lx - 1 - x;
-  std::string stl = {bsd[2], bsd[1], bsd[0]};
-  return dense_ref(sb.t[xr], 0, stl, {sb.bond(xr), sb.phys(xr), sb.bond(xr + 1)});
-}
-template <typename T>
-static Leg south_bond(const Boundary<T>& sb, int Lx, int b) {
-  return sb.bond(Lx - b);
-}
 
 template <typename T>
 void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, const int* labels_in,
@@ -452,8 +352,7 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
           contract<T>(ctx, Rr, Yr, bottom_ref(sb, Lx, x, "bsd"), cur);
           FullView fv = full_view(Rr);
           SPP_CHECK(launch_dot<T>(fv, fv, nrm, cur, st));
-          hipLaunchKernelGGL(safe_scale_kernel, dim3(gb), dim3(tb), 0, st, nrm, fac, status, fail, x + y * Lx, cur);
-          SPP_CHECK(hipGetLastError());
+          SPP_CHECK(launch_safe_factor(nrm, fac, status, fail, x + y * Lx, cur, st));
           SPP_CHECK(launch_scale<T>(fv, fac, cur, st));
           ctx.arena.release(xm);
         }
@@ -502,8 +401,7 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
                                             full_view(dense_ref(O[1], os, "ced", Ol)), choice, nullptr, cur, st));
             FullView fv = full_view(Lnew);
             SPP_CHECK(launch_dot<T>(fv, fv, nrm, cur, st));
-            hipLaunchKernelGGL(safe_scale_kernel, dim3(gb), dim3(tb), 0, st, nrm, fac, status, fail, site, cur);
-            SPP_CHECK(hipGetLastError());
+            SPP_CHECK(launch_safe_factor(nrm, fac, status, fail, site, cur, st));
             SPP_CHECK(launch_scale<T>(fv, fac, cur, st));
             Ll = Ol;
             Ls = os;
@@ -528,10 +426,8 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
       for (int x = 0; x < Lx; ++x) sst[x] = Nf.tsize(x);
       SPP_CHECK(hipMemcpyAsync(dsst, sst.data(), sizeof(long long) * Lx, hipMemcpyHostToDevice, st));
       SPP_CHECK(hipMemcpyAsync(dtens, Nf.t.data(), sizeof(T*) * Lx, hipMemcpyHostToDevice, st));
-      hipLaunchKernelGGL(close_unit_kernel<T>, dim3(gb), dim3(tb), 0, st, dtens, dsst, Lx, mant, cur);
-      SPP_CHECK(hipGetLastError());
-      hipLaunchKernelGGL(gather_lognorm_kernel, dim3(gb), dim3(tb), 0, st, c.cand_lognorm, labels, ns, lnamp, cur);
-      SPP_CHECK(hipGetLastError());
+      SPP_CHECK(launch_close_unit<T>(dtens, dsst, Lx, mant, cur, st));
+      SPP_CHECK(launch_gather_lognorm(c.cand_lognorm, labels, ns, lnamp, cur, st));
       SPP_CHECK(hipMemcpyAsync(h_status.data(), status, sizeof(int) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_fail.data(), fail, sizeof(int) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_labels.data(), labels, sizeof(int) * cur * ns, hipMemcpyDeviceToHost, st));

@@ -121,3 +121,22 @@ def test_bench_refuses_multi_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_operator_factors_split_the_pair_operator(spp):
+    """``_operator_factors``: F (x) G reproduces ZZ; the Z2 factor leg is odd
+    (z flips the X-basis parity), the dense one even; rank > 1 is refused."""
+    for sym, even, odd in ((True, 1, 1), (False, 2, 0)):
+        X, ZZ = spp.spin_operators(spp.TFIM(1.0, 0.5, symmetric=sym))
+        F, G, o = spp.operator_factors(ZZ, even)
+        assert o == odd
+        np.testing.assert_allclose(np.einsum("pu,qv->pquv", F, G), ZZ, atol=1e-15)
+    swap = np.zeros((2, 2, 2, 2))
+    for p in range(2):
+        for q in range(2):
+            swap[p, q, q, p] = 1.0
+    with pytest.raises(NotImplementedError):
+        spp.operator_factors(swap, 2)
+    odd_onsite = np.einsum("pu,qv->pquv", np.array([[0.0, 1.0], [1.0, 0.0]]), np.eye(2))
+    with pytest.raises(ValueError):
+        spp.operator_factors(odd_onsite, 1)
