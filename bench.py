@@ -84,15 +84,20 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=256, help="Born samples per step")
+    ap.add_argument("--batch", type=int, default=128, help="Born samples per step")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--L", type=int, default=8)
     ap.add_argument("--chi", type=int, default=32)
     ap.add_argument("--eps", type=float, default=1.0, help="perturbation size of the synthetic PEPO")
     ap.add_argument("--plain-draws", type=int, default=3, help="batch-1 FP64 draws for the plain path")
-    ap.add_argument("--check-samples", type=int, default=8, help="FP64 re-evaluation of drawn samples")
+    ap.add_argument("--check-samples", type=int, default=8,
+                    help="drawn samples re-evaluated in FP64 (log p) and measured in both precisions (energy)")
+    ap.add_argument("--J", type=float, default=1.0)
+    ap.add_argument("--h", type=float, default=2.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-only", action="store_true", help="timed loop only (for rocprofv3)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the timed loop, copy 256 MiB device to device (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the sampler shards samples across GPUs by design, "
@@ -135,8 +140,9 @@ def main():
     ctx.set_option("profile", 0)
     value = B * args.steps / elapsed
     log(f"timed {args.steps} x {B} draws in {elapsed:.2f} s -> {value:.3f} samples/s")
-    gemm_tf = cnt["gemm_flops_profiled"] / (cnt["gemm_ms"] * 1e-3) / 1e12 if cnt["gemm_ms"] > 0 else 0.0
-    avg_ms = cnt["gemm_ms"] / max(cnt["gemm_launches"], 1)
+    kc = cnt[args.precision]  # launches of the contraction kernel of this precision
+    gemm_tf = kc["flops"] / (kc["ms"] * 1e-3) / 1e12 if kc["ms"] > 0 else 0.0
+    avg_ms = kc["ms"] / max(kc["launches"], 1)
     result = {
         "metric": METRIC, "value": value, "unit": "Born samples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
@@ -148,11 +154,15 @@ def main():
         "roofline": {"bound": "mfma", "kernel": f"z2gemm_kernel<{'float' if args.precision == 'fp32' else 'double'}>",
                      "achieved": gemm_tf, "peak": PEAK[args.precision], "unit": "TFLOP/s",
                      "frac": gemm_tf / PEAK[args.precision], "traffic": None,
-                     "avg_launch_ms": avg_ms, "launches": cnt["gemm_launches"],
-                     "gemm_share_of_step": cnt["gemm_ms"] / (1e3 * elapsed),
+                     "avg_launch_ms": avg_ms, "launches": kc["launches"],
+                     "flops_per_launch": kc["flops"] / max(kc["launches"], 1),
+                     "kernel_share_of_step": kc["ms"] / (1e3 * elapsed),
+                     "all_contractions_share_of_step": cnt["gemm_ms"] / (1e3 * elapsed),
                      "flops_per_draw": cnt["gemm_flops"] / (B * args.steps)},
     }
     if args.profile_only:
+        if args.copy_calibration:
+            result["copy_calibration_bytes"] = ctx.copy_calibration(64 << 20)
         print(json.dumps(result), flush=True)
         return
 
@@ -174,9 +184,31 @@ def main():
     ev = spp.sample_configurations(pcache, configurations=d0["configuration"][:k])
     dlogp = np.max(np.abs(ev["log_probability"] - d0["log_probability"][:k]))
     dtln = np.max(np.abs(ev["tensor_log_norm"] - d0["tensor_log_norm"][:k]))
-    result["precision_check"] = {"samples": int(k), "max_abs_dlog_probability": float(dlogp),
-                                 "max_abs_dtensor_log_norm": float(dtln),
-                                 "max_environment_truncation": float(np.max(d0["max_environment_truncation"]))}
+    # energy of the same samples (TFIM J, h at the sampling chi), batched
+    # FP32-MFMA contractions vs the FP64 path; the ensemble estimate weighs
+    # each sample by exp(log_norm - log q) (particles.jl:43-52, ensemble.jl:17)
+    model = spp.TFIM(args.J, args.h, symmetric=True)
+    cfg = d0["configuration"][:k]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    e_lo = spp.measure_energies(cache, cfg, model)
+    ctx.synchronize()
+    t_energy = time.perf_counter() - t0
+    e_hi = spp.measure_energies(pcache, cfg, model)
+
+    def ensemble(e, lw):
+        w = np.exp(lw - np.max(lw))
+        return float(np.sum(w * e) / np.sum(w))
+    E_lo = ensemble(e_lo["energy_per_site"], e_lo["log_norm"] - d0["log_probability"][:k])
+    E_hi = ensemble(e_hi["energy_per_site"], e_hi["log_norm"] - ev["log_probability"])
+    result["precision_check"] = {
+        "samples": int(k), "max_abs_dlog_probability": float(dlogp), "max_abs_dtensor_log_norm": float(dtln),
+        "max_environment_truncation": float(np.max(d0["max_environment_truncation"])),
+        "max_abs_dE_per_site": float(np.max(np.abs(e_lo["energy_per_site"] - e_hi["energy_per_site"]))),
+        "ensemble_E_per_site": E_hi, "ensemble_dE_per_site": abs(E_lo - E_hi),
+        "model": f"TFIM J={args.J} h={args.h} (Z2)"}
+    result["energy_measurement"] = {"value": k / t_energy, "unit": "energy measurements/s",
+                                    "dtype": args.precision, "samples": int(k), "chi": args.chi}
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_baseline(L=L, chi=args.chi)

@@ -60,7 +60,9 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value);
  * counters[1] = contraction launches, counters[2] = subspace iterations of the
  * last truncation, counters[3] = peak workspace bytes, counters[4] = summed
  * contraction kernel milliseconds and counters[5] = the flops of the timed
- * launches (profiling mode). Returns the count written. */
+ * launches (profiling mode); counters[6..8] = launches, milliseconds and flops
+ * of the timed FP64-kernel launches, counters[9..11] the same for the FP32
+ * kernel. Returns the count written. */
 int spp_get_counters(spp_ctx* ctx, double* counters, int n);
 int spp_reset_counters(spp_ctx* ctx);
 
@@ -129,9 +131,16 @@ int spp_measure_energy_batch(spp_ctx* ctx, const spp_cache* cache, const spp_sta
  * C[M][N] = A[M][K] B[K][N] (row-major, FP64 in/out) with each index a Z2
  * leg whose first evenX entries are even. Only parity-matching (neutral)
  * entries of C are computed; the others are returned as NaN. Pass
- * evenX = X for a plain dense product. */
+ * evenX = X for a plain dense product. layout bit 0: A is given as its
+ * transpose [K][M]; bit 1: B is given as its transpose [N][K]. */
 int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, int evenM, int evenN, int evenK,
-                      const double* A, const double* B, double* C);
+                      int layout, const double* A, const double* B, double* C);
+
+/* Calibration copy (no reference counterpart): one launch of the library's
+ * plain copy kernel moving n FP32 values between two device buffers it
+ * allocates, so that HBM traffic counters can be scaled by a known
+ * 8*n bytes (4*n read, 4*n written). */
+int spp_selftest_copy(spp_ctx* ctx, long long n);
 
 #ifdef __cplusplus
 }

@@ -36,8 +36,9 @@ _SIGNATURES = {
                                                 ctypes.c_double, c_double_p, c_double_p, c_double_p, c_double_p,
                                                 c_double_p, c_int_p, c_int_p]),
     "spp_selftest_gemm": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, c_double_p, c_double_p,
-                                         c_double_p]),
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_double_p,
+                                         c_double_p, c_double_p]),
+    "spp_selftest_copy": (ctypes.c_int, [c_void_p, ctypes.c_longlong]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

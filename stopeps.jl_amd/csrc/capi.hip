@@ -109,9 +109,12 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value) {
 int spp_get_counters(spp_ctx* ctx, double* counters, int n) {
   if (!ctx || !counters) return -3;
   if (guarded(ctx, [&] { ctx->c.flush_timer(); }) != 0) return -1;
-  double v[6] = {ctx->c.counters.gemm_flops, ctx->c.counters.gemm_launches, (double)ctx->c.last_iterations,
-                 (double)ctx->c.arena.peak(), ctx->c.counters.gemm_ms, ctx->c.counters.gemm_flops_profiled};
-  int m = n < 6 ? n : 6;
+  const Counters& k = ctx->c.counters;
+  double v[12] = {k.gemm_flops,        k.gemm_launches,    (double)ctx->c.last_iterations,
+                  (double)ctx->c.arena.peak(), k.gemm_ms, k.gemm_flops_profiled,
+                  k.prec_launches[0],  k.prec_ms[0],       k.prec_flops[0],
+                  k.prec_launches[1],  k.prec_ms[1],       k.prec_flops[1]};
+  int m = n < 12 ? n : 12;
   for (int i = 0; i < m; ++i) counters[i] = v[i];
   return m;
 }
@@ -257,7 +260,7 @@ int spp_measure_energy_batch(spp_ctx* ctx, const spp_cache* cache, const spp_sta
 }
 
 int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, int evenM, int evenN, int evenK,
-                      const double* A, const double* B, double* C) {
+                      int layout, const double* A, const double* B, double* C) {
   return guarded(ctx, [&] {
     if (M <= 0 || N <= 0 || K <= 0) throw HipError("dimensions must be positive");
     if (evenM < 0 || evenM > M || evenN < 0 || evenN > N || evenK < 0 || evenK > K)
@@ -271,6 +274,15 @@ int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, int even
     SPP_CHECK(hipMemcpyAsync(dA, A, sizeof(double) * M * K, hipMemcpyHostToDevice, st));
     SPP_CHECK(hipMemcpyAsync(dB, B, sizeof(double) * K * N, hipMemcpyHostToDevice, st));
     Leg lm = fixed_leg(M, evenM), ln = fixed_leg(N, evenN), lk = fixed_leg(K, evenK);
+    if (layout < 0 || layout > 3) throw HipError("layout must lie in [0, 3]");
+    // storage order of the operands: A as [M][K] or [K][M], B as [K][N] or [N][K]
+    const bool at = (layout & 1) != 0, bt = (layout & 2) != 0;
+    auto aref = [&](void* p) {
+      return at ? dense_ref(p, 0, "km", {lk, lm}) : dense_ref(p, 0, "mk", {lm, lk});
+    };
+    auto bref = [&](void* p) {
+      return bt ? dense_ref(p, 0, "nk", {ln, lk}) : dense_ref(p, 0, "kn", {lk, ln});
+    };
     const double qnan = std::numeric_limits<double>::quiet_NaN();
     if (precision == 1) {
       SPP_CHECK(hipMalloc(&tA, es * M * K));
@@ -280,15 +292,13 @@ int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, int even
       SPP_CHECK(launch_fill<float>((float*)tC, (long long)M * N, (float)qnan, st));
       SPP_CHECK((launch_convert<float, double>((float*)tA, (const double*)dA, (long long)M * K, st)));
       SPP_CHECK((launch_convert<float, double>((float*)tB, (const double*)dB, (long long)K * N, st)));
-      contract<float>(ctx->c, dense_ref(tC, 0, "mn", {lm, ln}), dense_ref(tA, 0, "mk", {lm, lk}),
-                      dense_ref(tB, 0, "kn", {lk, ln}), 1);
+      contract<float>(ctx->c, dense_ref(tC, 0, "mn", {lm, ln}), aref(tA), bref(tB), 1);
       SPP_CHECK((launch_convert<double, float>((double*)dC, (const float*)tC, (long long)M * N, st)));
       SPP_CHECK(hipStreamSynchronize(st));
       (void)hipFree(tC);
     } else {
       SPP_CHECK(launch_fill<double>((double*)dC, (long long)M * N, qnan, st));
-      contract<double>(ctx->c, dense_ref(dC, 0, "mn", {lm, ln}), dense_ref(dA, 0, "mk", {lm, lk}),
-                       dense_ref(dB, 0, "kn", {lk, ln}), 1);
+      contract<double>(ctx->c, dense_ref(dC, 0, "mn", {lm, ln}), aref(dA), bref(dB), 1);
     }
     SPP_CHECK(hipMemcpyAsync(C, dC, sizeof(double) * M * N, hipMemcpyDeviceToHost, st));
     SPP_CHECK(hipStreamSynchronize(st));
@@ -297,6 +307,22 @@ int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, int even
     (void)hipFree(dC);
     if (tA) (void)hipFree(tA);
     if (tB) (void)hipFree(tB);
+  });
+}
+
+int spp_selftest_copy(spp_ctx* ctx, long long n) {
+  return guarded(ctx, [&] {
+    if (n <= 0) throw HipError("n must be positive");
+    hipStream_t st = ctx->c.stream;
+    float *a = nullptr, *b = nullptr;
+    SPP_CHECK(hipMalloc(&a, sizeof(float) * n));
+    SPP_CHECK(hipMalloc(&b, sizeof(float) * n));
+    SPP_CHECK(launch_fill<float>(a, n, 1.0f, st));
+    SPP_CHECK(hipStreamSynchronize(st));
+    SPP_CHECK((launch_convert<float, float>(b, a, n, st)));
+    SPP_CHECK(hipStreamSynchronize(st));
+    (void)hipFree(a);
+    (void)hipFree(b);
   });
 }
 

@@ -92,12 +92,17 @@ struct Counters {
   double gemm_launches = 0;
   double gemm_ms = 0;     // summed contraction kernel time (profiling mode)
   double gemm_flops_profiled = 0;
+  // profiled launches split by element type: [0] FP64 kernel, [1] FP32 kernel
+  double prec_launches[2] = {0, 0};
+  double prec_ms[2] = {0, 0};
+  double prec_flops[2] = {0, 0};
 };
 
 // Event pairs recorded around contraction launches when profiling is on.
 struct LaunchTimer {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
   std::vector<double> pending_flops;
+  std::vector<int> pending_prec;  // 0: FP64 kernel, 1: FP32 kernel
 };
 
 struct Context {

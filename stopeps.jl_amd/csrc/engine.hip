@@ -206,12 +206,34 @@ void contract(Context& ctx, const TRef& C, const TRef& A, const TRef& B, int nsa
   fill_operand_group(g.b.g[1], B, N, 1, who);
   fill_operand_group(g.c.g[0], C, M, 0, who);
   fill_operand_group(g.c.g[1], C, N, 1, who);
+  // lanes follow the unit-stride index of each operand
+  auto last_stride = [](const OperandGroup& og, int nlegs) -> long long {
+    if (og.linear) return og.lstride;
+    return nlegs > 0 ? og.stride[nlegs - 1] : (1LL << 62);
+  };
+  // extents from a (sample, parity) base: the kernel addresses operands with
+  // 32-bit offsets and reads past-the-end K indices as zeros
+  auto extent = [](const TRef& t) -> unsigned long long {
+    if (t.compact) return (unsigned long long)t.pstride;
+    unsigned long long e = 1;
+    for (size_t j = 0; j < t.leg.size(); ++j) e += (unsigned long long)(t.leg[j].dim - 1) * t.stride[j];
+    return e;
+  };
+  const unsigned long long ea = extent(A) * sizeof(T), eb = extent(B) * sizeof(T), ec = extent(C);
+  if (ea >= (1ULL << 31) || eb >= (1ULL << 31) || ec >= (1ULL << 31))
+    throw HipError("contract: an operand exceeds 2^31 bytes per sample (" + who + ")");
+  g.a_bytes = (unsigned)ea;
+  g.b_bytes = (unsigned)eb;
+  g.a_mfast = last_stride(g.a.g[0], (int)M.size()) < last_stride(g.a.g[1], (int)K.size()) ? 1 : 0;
+  g.b_kfast = last_stride(g.b.g[0], (int)K.size()) < last_stride(g.b.g[1], (int)N.size()) ? 1 : 0;
   g.nsamples = nsamples;
   g.sample_map = nullptr;
   g.pmask = pmask;
   g.alpha = alpha;
   g.accumulate = accumulate ? 1 : 0;
   long long mmax = group_bound(lm), nmax = group_bound(ln);
+  if (mmax >= (1LL << 31) || nmax >= (1LL << 31) || group_bound(lk) >= (1LL << 31))
+    throw HipError("contract: a leg group exceeds 2^31 tuples (" + who + ")");
   double fl = 0;
   for (int q = 0; q < 2; ++q) fl += group_estimate(lm, q) * group_estimate(ln, q) * group_estimate(lk, q);
   ctx.counters.gemm_flops += 2.0 * fl * nsamples;
@@ -225,6 +247,7 @@ void contract(Context& ctx, const TRef& C, const TRef& A, const TRef& B, int nsa
     SPP_CHECK(hipEventRecord(b, ctx.stream));
     ctx.timer.pending.emplace_back(a, b);
     ctx.timer.pending_flops.push_back(2.0 * fl * nsamples);
+    ctx.timer.pending_prec.push_back(sizeof(T) == 4 ? 1 : 0);
     if (ctx.timer.pending.size() >= 4096) ctx.flush_timer();
     return;
   }
@@ -239,11 +262,16 @@ void Context::flush_timer() {
     SPP_CHECK(hipEventElapsedTime(&ms, timer.pending[i].first, timer.pending[i].second));
     counters.gemm_ms += ms;
     counters.gemm_flops_profiled += timer.pending_flops[i];
+    const int q = timer.pending_prec[i];
+    counters.prec_launches[q] += 1;
+    counters.prec_ms[q] += ms;
+    counters.prec_flops[q] += timer.pending_flops[i];
     (void)hipEventDestroy(timer.pending[i].first);
     (void)hipEventDestroy(timer.pending[i].second);
   }
   timer.pending.clear();
   timer.pending_flops.clear();
+  timer.pending_prec.clear();
 }
 
 template void contract<float>(Context&, const TRef&, const TRef&, const TRef&, int, double, bool, const int*);

@@ -150,6 +150,74 @@ static size_t draw_bytes_per_sample(const CacheDev<T>& c, const Context& ctx) {
   return (size_t)(1.15 * std::max(rowenv, absorb_bytes_per_sample(Lx, chi, D, sizeof(T), ctx.ksub))) + 4096;
 }
 
+static long long layer_count(const LayerSpec& sp) {
+  return (long long)sp.w.dim * sp.n.dim * sp.e.dim * sp.s.dim;
+}
+
+static long long max_leg_dim(const std::vector<LayerSpec>& spec) {
+  long long D = 1;
+  for (auto& sp : spec) D = std::max<long long>(D, std::max(std::max(sp.w.dim, sp.n.dim), std::max(sp.e.dim, sp.s.dim)));
+  return D;
+}
+
+// South boundaries of the unprojected PEPO (batch 1) in the rotated frame:
+// south[y] holds rows y..Ly-1 (boundary_environments, boundary_mps.jl:193-209).
+template <typename TS>
+static void build_south(Context& ctx, const std::vector<LayerSpec>& spec, const std::vector<TS*>& layers, int Lx,
+                        int Ly, int chi, std::vector<Boundary<TS>>& south) {
+  south.resize(Ly + 1);
+  std::vector<Leg> phys(Lx);
+  for (int xr = 0; xr < Lx; ++xr) phys[xr] = spec[(Lx - 1 - xr) + (Ly - 1) * Lx].s;
+  prepare_boundary(south[Ly], Lx, 1, 1);
+  set_trivial(ctx, south[Ly], phys);
+  freeze_splits(ctx, south[Ly]);
+  for (int y = Ly - 1; y >= 0; --y) {
+    RowLayers<TS> rr;
+    rr.L = Lx;
+    rr.rotated = true;
+    rr.shared.resize(Lx);
+    rr.spec.resize(Lx);
+    long long capmax = 1;
+    for (int xr = 0; xr < Lx; ++xr) {
+      const int x = Lx - 1 - xr;
+      const LayerSpec& sp = spec[x + y * Lx];
+      rr.shared[xr] = layers[x + y * Lx];
+      rr.spec[xr] = LayerSpec{sp.e, sp.s, sp.w, sp.n};
+      capmax = std::max<long long>(capmax, (long long)chi * sp.n.dim * chi);
+    }
+    prepare_boundary(south[y], Lx, 1, capmax);
+    absorb_forward<TS>(ctx, south[y + 1], rr, chi, south[y]);
+    freeze_splits(ctx, south[y]);
+  }
+}
+
+// Copy of a batch-1 boundary with frozen splits in another element type.
+template <typename TO, typename TI>
+static void convert_boundary(Context& ctx, const Boundary<TI>& in, Boundary<TO>& out) {
+  hipStream_t st = ctx.stream;
+  out.release();
+  out.L = in.L;
+  out.nb = in.nb;
+  out.bdim = in.bdim;
+  out.beven = in.beven;
+  out.bsplit.assign(in.L + 1, nullptr);
+  for (auto p : in.bsplit)
+    if (p) throw HipError("convert_boundary: per-sample splits must be frozen");
+  out.pdim = in.pdim;
+  out.peven = in.peven;
+  out.t.assign(in.L, nullptr);
+  out.owns = true;
+  for (int x = 0; x < in.L; ++x) {
+    const long long n = in.tsize(x) * in.nb;
+    SPP_CHECK(hipMalloc(&out.t[x], sizeof(TO) * n));
+    SPP_CHECK((launch_convert<TO, TI>(out.t[x], in.t[x], n, st)));
+  }
+  SPP_CHECK(hipMalloc(&out.log_scale, sizeof(double) * in.nb));
+  SPP_CHECK(hipMalloc(&out.max_trunc, sizeof(double) * in.nb));
+  SPP_CHECK(hipMemcpyAsync(out.log_scale, in.log_scale, sizeof(double) * in.nb, hipMemcpyDeviceToDevice, st));
+  SPP_CHECK(hipMemcpyAsync(out.max_trunc, in.max_trunc, sizeof(double) * in.nb, hipMemcpyDeviceToDevice, st));
+}
+
 template <typename T>
 void build_cache(Context& ctx, const StateDev& pepo, int chi, const double* basis, CacheDev<T>& c) {
   hipStream_t st = ctx.stream;
@@ -211,31 +279,46 @@ void build_cache(Context& ctx, const StateDev& pepo, int chi, const double* basi
                        c.basis[1], c.basis[2], c.basis[3], c.cand_lognorm + 2 * i);
     SPP_CHECK(hipGetLastError());
   }
-  // south boundaries, rotated frame (batch 1)
-  if (ctx.arena.mark() == 0) ctx.arena.reserve(std::max(ctx.arena.capacity(), draw_bytes_per_sample(c, ctx)));
-  c.south.resize(Ly + 1);
-  std::vector<Leg> phys(Lx);
-  for (int xr = 0; xr < Lx; ++xr) phys[xr] = c.spec[(Lx - 1 - xr) + (Ly - 1) * Lx].s;
-  prepare_boundary(c.south[Ly], Lx, 1, 1);
-  set_trivial(ctx, c.south[Ly], phys);
-  freeze_splits(ctx, c.south[Ly]);
-  for (int y = Ly - 1; y >= 0; --y) {
-    RowLayers<T> rr;
-    rr.L = Lx;
-    rr.rotated = true;
-    rr.shared.resize(Lx);
-    rr.spec.resize(Lx);
-    long long capmax = 1;
-    for (int xr = 0; xr < Lx; ++xr) {
-      int x = Lx - 1 - xr;
-      const LayerSpec& sp = c.spec[x + y * Lx];
-      rr.shared[xr] = c.layers[x + y * Lx];
-      rr.spec[xr] = LayerSpec{sp.e, sp.s, sp.w, sp.n};
-      capmax = std::max<long long>(capmax, (long long)chi * sp.n.dim * chi);
+  // South boundaries, rotated frame (batch 1). They are computed in FP64 for
+  // every precision: an FP32 cache stores FP64-contracted boundaries.
+  const double need = absorb_bytes_per_sample(Lx, chi, max_leg_dim(c.spec), sizeof(double), ctx.ksub);
+  if (ctx.arena.mark() == 0)
+    ctx.arena.reserve(std::max(ctx.arena.capacity(), std::max(draw_bytes_per_sample(c, ctx), (size_t)(1.15 * need) + 4096)));
+  if constexpr (sizeof(T) == sizeof(double)) {
+    build_south<double>(ctx, c.spec, c.layers, Lx, Ly, chi, c.south);
+  } else {
+    std::vector<double*> l64(ns, nullptr);
+    std::vector<Boundary<double>> s64;
+    try {
+      std::vector<LayerJob> jobs64;
+      for (int i = 0; i < ns; ++i) {
+        const SiteDev& s = pepo.sites[i];
+        SPP_CHECK(hipMalloc(&l64[i], sizeof(double) * layer_count(c.spec[i])));
+        LayerJob J{};
+        J.ket = s.d;
+        J.bra = s.d;
+        for (int k = 0; k < 6; ++k) {
+          J.dk[k] = J.db[k] = s.dim[k];
+          J.ek[k] = J.eb[k] = s.even[k];
+        }
+        J.out = l64[i];
+        jobs64.push_back(J);
+      }
+      SPP_CHECK(hipMemcpyAsync(djobs, jobs64.data(), sizeof(LayerJob) * ns, hipMemcpyHostToDevice, st));
+      SPP_CHECK(launch_double_layers<double>(djobs, ns, maxout, st));
+      build_south<double>(ctx, c.spec, l64, Lx, Ly, chi, s64);
+      c.south.resize(Ly + 1);
+      for (int y = 0; y <= Ly; ++y) convert_boundary<T, double>(ctx, s64[y], c.south[y]);
+      SPP_CHECK(hipStreamSynchronize(st));
+    } catch (...) {
+      for (auto& b : s64) b.release();
+      for (double* p : l64)
+        if (p) (void)hipFree(p);
+      throw;
     }
-    prepare_boundary(c.south[y], Lx, 1, capmax);
-    absorb_forward<T>(ctx, c.south[y + 1], rr, chi, c.south[y]);
-    freeze_splits(ctx, c.south[y]);
+    for (auto& b : s64) b.release();
+    for (double* p : l64)
+      if (p) SPP_CHECK(hipFree(p));
   }
   c.south_max_trunc.assign(Ly + 1, 0.0);
   for (int y = 0; y <= Ly; ++y)

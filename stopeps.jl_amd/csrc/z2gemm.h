@@ -48,6 +48,9 @@ struct GemmArgs {
   double alpha;
   int accumulate;                // C = alpha*AB + (accumulate ? C : 0)
   int conj_b;                    // reserved (real arithmetic only)
+  int a_mfast;                   // A tiles are read with consecutive lanes along M (else along K)
+  int b_kfast;                   // B tiles are read with consecutive lanes along K (else along N)
+  unsigned a_bytes, b_bytes;     // extent of A / B from a (sample, parity) base, bytes (< 2^31)
 };
 
 // Host-side launcher. T is float or double. Grid bounds mmax/nmax are upper
@@ -119,6 +122,36 @@ __device__ inline void decode(const GroupView& g, int q, long long idx, int* leg
     long long pr = idx / cl;
     int lr = (int)(idx - pr * cl);
     legs[j] = lr + (lastpar ? g.ev[j] : 0);
+    idx = pr;
+    q = qp;
+  }
+}
+
+// decode() for tuple indices below 2^31 (32-bit division).
+__device__ inline void decode32(const GroupView& g, int q, int idx, int* legs) {
+  if (!g.graded) {
+    for (int j = g.n - 1; j >= 0; --j) {
+      legs[j] = idx % g.dim[j];
+      idx /= g.dim[j];
+    }
+    return;
+  }
+  for (int j = g.n - 1; j >= 0; --j) {
+    const int c_q = (q == 0) ? g.ev[j] : g.dim[j] - g.ev[j];
+    const int part0 = (int)g.pc[j][0] * c_q;
+    int lastpar, qp, cl;
+    if (idx < part0) {
+      qp = 0;
+      lastpar = q;
+      cl = c_q;
+    } else {
+      idx -= part0;
+      qp = 1;
+      lastpar = q ^ 1;
+      cl = (lastpar == 0) ? g.ev[j] : g.dim[j] - g.ev[j];
+    }
+    const int pr = idx / cl;
+    legs[j] = (idx - pr * cl) + (lastpar ? g.ev[j] : 0);
     idx = pr;
     q = qp;
   }

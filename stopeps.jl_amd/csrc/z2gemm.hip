@@ -4,7 +4,7 @@
 // 2x2 grid, each wave owning a 64x64 sub-tile. FP32 uses v_mfma_f32_32x32x2f32
 // (2x2 tiles per wave), FP64 uses v_mfma_f64_16x16x4f64 (4x4 tiles per wave).
 // Operand tiles are gathered from global memory through the decoded row/column
-// offsets (consecutive lanes on consecutive k of an A row / n of a B row), held
+// offsets, with consecutive lanes along each operand's unit-stride index, held
 // in registers one tile ahead of the MFMA work, and staged k-major in padded
 // LDS so that each MFMA operand read is a contiguous run of lanes.
 #include "z2gemm.h"
@@ -108,15 +108,31 @@ __device__ inline T* operand_base(const Operand& op, int s, int q) {
   return p + (long long)q * op.pstride;
 }
 
-template <typename T>
+// Raw buffer loads: offsets are bytes from the (sample, parity) base; an
+// offset at or beyond the descriptor's extent reads as zero.
+constexpr int kBufferFlags = 0x00020000;
+constexpr unsigned kInvalidK = 0x80000000u;  // K index past the end: a zero operand
+__device__ inline float buffer_load(__amdgpu_buffer_rsrc_t r, unsigned off, float) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+__device__ inline double buffer_load(__amdgpu_buffer_rsrc_t r, unsigned off, double) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+
+constexpr int KLDS = 2048;  // K offsets held in LDS per pass over K
+
+template <typename T, int AMF, int BKF>
 __global__ __launch_bounds__(kThreads) void z2gemm_kernel(GemmArgs args) {
   constexpr int BK = TileCfg<T>::BK;
   constexpr int AL = BM * BK / kThreads;  // A elements per thread per tile
   constexpr int BL = BN * BK / kThreads;  // B elements per thread per tile
   __shared__ T As[BK * (BM + PAD)];
   __shared__ T Bs[BK * (BN + PAD)];
-  __shared__ long long offAm[BM], offCm[BM], offBn[BN], offCn[BN];
-  __shared__ long long offAk[2][BK], offBk[2][BK];
+  // byte offsets of A rows / B columns (invalid: 0, any readable element) and
+  // element offsets of C rows / columns (invalid: -1)
+  __shared__ unsigned offAm[BM], offBn[BN];
+  __shared__ int offCm[BM], offCn[BN];
+  __shared__ unsigned offAk[KLDS], offBk[KLDS];  // byte offsets along K (invalid: kInvalidK)
   __shared__ GroupView gvM, gvN, gvK;
 
   const int tid = threadIdx.x;
@@ -130,56 +146,41 @@ __global__ __launch_bounds__(kThreads) void z2gemm_kernel(GemmArgs args) {
   if (tid == 1) resolve_group(args.gn, s, gvN);
   if (tid == 2) resolve_group(args.gk, s, gvK);
   __syncthreads();
-  const long long cM = group_count(gvM, q);
-  const long long cN = group_count(gvN, q);
-  const long long cK = group_count(gvK, q);
-  const long long m0 = (long long)blockIdx.y * BM;
-  const long long n0 = (long long)blockIdx.x * BN;
+  const int cM = (int)group_count(gvM, q);
+  const int cN = (int)group_count(gvN, q);
+  const int cK = (int)group_count(gvK, q);
+  const int m0 = blockIdx.y * BM;
+  const int n0 = blockIdx.x * BN;
   if (m0 >= cM || n0 >= cN) return;
 
   int legs[kMaxLegs];
   if (tid < BM) {
-    long long idx = m0 + tid;
+    const int idx = m0 + tid;
     if (idx < cM) {
-      decode(gvM, q, idx, legs);
-      offAm[tid] = operand_offset(args.a.g[0], gvM, idx, legs);
-      offCm[tid] = operand_offset(args.c.g[0], gvM, idx, legs);
+      decode32(gvM, q, idx, legs);
+      offAm[tid] = (unsigned)operand_offset(args.a.g[0], gvM, idx, legs) * (unsigned)sizeof(T);
+      offCm[tid] = (int)operand_offset(args.c.g[0], gvM, idx, legs);
     } else {
-      offAm[tid] = -1;
+      offAm[tid] = 0;
       offCm[tid] = -1;
     }
   } else {
-    int t = tid - BM;
-    long long idx = n0 + t;
+    const int t = tid - BM;
+    const int idx = n0 + t;
     if (idx < cN) {
-      decode(gvN, q, idx, legs);
-      offBn[t] = operand_offset(args.b.g[1], gvN, idx, legs);
-      offCn[t] = operand_offset(args.c.g[1], gvN, idx, legs);
+      decode32(gvN, q, idx, legs);
+      offBn[t] = (unsigned)operand_offset(args.b.g[1], gvN, idx, legs) * (unsigned)sizeof(T);
+      offCn[t] = (int)operand_offset(args.c.g[1], gvN, idx, legs);
     } else {
-      offBn[t] = -1;
+      offBn[t] = 0;
       offCn[t] = -1;
     }
   }
-  auto decode_k = [&](long long k0, int buf) {
-    if (tid < BK) {
-      long long idx = k0 + tid;
-      if (idx < cK) {
-        decode(gvK, q, idx, legs);
-        offAk[buf][tid] = operand_offset(args.a.g[1], gvK, idx, legs);
-        offBk[buf][tid] = operand_offset(args.b.g[0], gvK, idx, legs);
-      } else {
-        offAk[buf][tid] = -1;
-        offBk[buf][tid] = -1;
-      }
-    }
-  };
-  if (cK > 0) decode_k(0, 0);
-  // Orders the offset tables before their first use (also when cK == 0: the
-  // epilogue reads offCm/offCn and no K-loop barrier runs).
-  __syncthreads();
 
-  const T* Ab = operand_base<T>(args.a, s, q);
-  const T* Bb = operand_base<T>(args.b, s, q);
+  const __amdgpu_buffer_rsrc_t ra_rsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void*)operand_base<T>(args.a, s, q), 0, (int)args.a_bytes, kBufferFlags);
+  const __amdgpu_buffer_rsrc_t rb_rsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void*)operand_base<T>(args.b, s, q), 0, (int)args.b_bytes, kBufferFlags);
   T* Cb = operand_base<T>(args.c, s, q);
 
   const int lane = tid & 63;
@@ -189,39 +190,56 @@ __global__ __launch_bounds__(kThreads) void z2gemm_kernel(GemmArgs args) {
   WaveAcc<T> acc;
   acc.zero();
 
-  // Tile loads: consecutive lanes read consecutive k of one A row and
-  // consecutive n of one B row, so each wave touches few cache lines.
-  const int a_k = tid % BK, a_r0 = tid / BK;            // rows a_r0 + j * (kThreads / BK)
-  const int b_n = tid % BN, b_k0 = tid / BN;            // k = b_k0 + j * (kThreads / BN)
-  constexpr int AR = kThreads / BK, BR = kThreads / BN;
+  // Tile loads. Consecutive lanes follow the operand's unit-stride direction
+  // (AMF / BKF, chosen on the host): along K (k = tid % BK, rows tid / BK +
+  // j AR) or along the row index (row = tid % 128, k = tid / 128 + j KR), so
+  // that a wave's load touches few cache lines in either layout.
+  constexpr int AR = kThreads / BK;    // row step of a K-fast assignment
+  constexpr int KR = kThreads / BM;    // k step of a row-fast assignment (BM == BN)
+  const int a_r = AMF ? tid % BM : tid / BK, a_k = AMF ? tid / BM : tid % BK;
+  const int b_r = BKF ? tid / BK : tid % BN, b_k = BKF ? tid % BK : tid / BN;
+  constexpr int a_dr = AMF ? 0 : AR, a_dk = AMF ? KR : 0;
+  constexpr int b_dr = BKF ? AR : 0, b_dk = BKF ? 0 : KR;
   T ra[AL], rb[BL];
-  auto load_tile = [&](int buf) {
-    const long long ok = offAk[buf][a_k];
+  auto load_tile = [&](int kt) {  // kt: first k of the tile within the LDS offset table
 #pragma unroll
-    for (int j = 0; j < AL; ++j) {
-      const long long om = offAm[a_r0 + j * AR];
-      ra[j] = (om >= 0 && ok >= 0) ? Ab[om + ok] : T(0);
-    }
-    const long long on = offBn[b_n];
+    for (int j = 0; j < AL; ++j)
+      ra[j] = buffer_load(ra_rsrc, offAm[a_r + j * a_dr] + offAk[kt + a_k + j * a_dk], T(0));
 #pragma unroll
-    for (int j = 0; j < BL; ++j) {
-      const long long okb = offBk[buf][b_k0 + j * BR];
-      rb[j] = (on >= 0 && okb >= 0) ? Bb[okb + on] : T(0);
-    }
+    for (int j = 0; j < BL; ++j)
+      rb[j] = buffer_load(rb_rsrc, offBn[b_r + j * b_dr] + offBk[kt + b_k + j * b_dk], T(0));
   };
-  const long long ntiles = (cK + BK - 1) / BK;
-  if (ntiles > 0) load_tile(0);
-  for (long long t = 0; t < ntiles; ++t) {
-#pragma unroll
-    for (int j = 0; j < AL; ++j) As[a_k * (BM + PAD) + a_r0 + j * AR] = ra[j];
-#pragma unroll
-    for (int j = 0; j < BL; ++j) Bs[(b_k0 + j * BR) * (BN + PAD) + b_n] = rb[j];
-    if (t + 1 < ntiles) decode_k((t + 1) * BK, (int)((t + 1) & 1));
+
+  for (int kc = 0; kc < cK; kc += KLDS) {
+    const int klen = min(KLDS, cK - kc);
+    const int nt = (klen + BK - 1) / BK;
+    // offsets of this pass's K range (the previous pass's tiles are done:
+    // every K loop iteration ends with a barrier)
+    for (int k = tid; k < nt * BK; k += kThreads) {
+      if (k < klen) {
+        decode32(gvK, q, kc + k, legs);
+        offAk[k] = (unsigned)operand_offset(args.a.g[1], gvK, kc + k, legs) * (unsigned)sizeof(T);
+        offBk[k] = (unsigned)operand_offset(args.b.g[0], gvK, kc + k, legs) * (unsigned)sizeof(T);
+      } else {
+        offAk[k] = kInvalidK;
+        offBk[k] = kInvalidK;
+      }
+    }
     __syncthreads();
-    if (t + 1 < ntiles) load_tile((int)((t + 1) & 1));
-    acc.step(As, Bs, wm, wn, lane);
-    __syncthreads();
+    load_tile(0);
+    for (int t = 0; t < nt; ++t) {
+#pragma unroll
+      for (int j = 0; j < AL; ++j) As[(a_k + j * a_dk) * (BM + PAD) + a_r + j * a_dr] = ra[j];
+#pragma unroll
+      for (int j = 0; j < BL; ++j) Bs[(b_k + j * b_dk) * (BN + PAD) + b_r + j * b_dr] = rb[j];
+      __syncthreads();
+      if (t + 1 < nt) load_tile((t + 1) * BK);
+      acc.step(As, Bs, wm, wn, lane);
+      __syncthreads();
+    }
   }
+  // Orders the C offset tables before the epilogue when cK == 0 (no K loop).
+  if (cK == 0) __syncthreads();
 
   const T alpha = (T)args.alpha;
 #pragma unroll
@@ -232,12 +250,13 @@ __global__ __launch_bounds__(kThreads) void z2gemm_kernel(GemmArgs args) {
       for (int v = 0; v < WaveAcc<T>::NV; ++v) {
         int i, j;
         WaveAcc<T>::coord(ta, tb, v, lane, i, j);
-        const long long om = offCm[wm + i];
-        const long long on = offCn[wn + j];
+        const int om = offCm[wm + i];
+        const int on = offCn[wn + j];
         if (om >= 0 && on >= 0) {
           T val = alpha * acc.get(ta, tb, v);
-          if (args.accumulate) val += Cb[om + on];
-          Cb[om + on] = val;
+          T* c = Cb + (unsigned)(om + on);
+          if (args.accumulate) val += *c;
+          *c = val;
         }
       }
 }
@@ -250,7 +269,11 @@ hipError_t launch_z2gemm(const GemmArgs& args, long long mmax, long long nmax, h
   long long gz = 2LL * args.nsamples;
   if (gy > 65535 || gz > 65535 || gx > 2147483647LL) return hipErrorInvalidConfiguration;
   dim3 grid((unsigned)gx, (unsigned)gy, (unsigned)gz);
-  hipLaunchKernelGGL(z2gemm_kernel<T>, grid, dim3(kThreads), 0, stream, args);
+  const int mode = (args.a_mfast ? 1 : 0) + (args.b_kfast ? 2 : 0);
+  if (mode == 0) hipLaunchKernelGGL((z2gemm_kernel<T, 0, 0>), grid, dim3(kThreads), 0, stream, args);
+  else if (mode == 1) hipLaunchKernelGGL((z2gemm_kernel<T, 1, 0>), grid, dim3(kThreads), 0, stream, args);
+  else if (mode == 2) hipLaunchKernelGGL((z2gemm_kernel<T, 0, 1>), grid, dim3(kThreads), 0, stream, args);
+  else hipLaunchKernelGGL((z2gemm_kernel<T, 1, 1>), grid, dim3(kThreads), 0, stream, args);
   return hipGetLastError();
 }
 

@@ -44,12 +44,14 @@ class Context:
         _lib.check(self.handle, self._lib.spp_set_option(self.handle, name.encode(), float(value)))
 
     def counters(self):
-        out = np.zeros(6)
-        rc = self._lib.spp_get_counters(self.handle, _lib.dptr(out), 6)
+        out = np.zeros(12)
+        rc = self._lib.spp_get_counters(self.handle, _lib.dptr(out), 12)
         if rc < 0:
             _lib.check(self.handle, rc)
         return {"gemm_flops": out[0], "gemm_launches": out[1], "subspace_iterations": out[2],
-                "workspace_peak_bytes": out[3], "gemm_ms": out[4], "gemm_flops_profiled": out[5]}
+                "workspace_peak_bytes": out[3], "gemm_ms": out[4], "gemm_flops_profiled": out[5],
+                "fp64": {"launches": out[6], "ms": out[7], "flops": out[8]},
+                "fp32": {"launches": out[9], "ms": out[10], "flops": out[11]}}
 
     def reset_counters(self):
         self._lib.spp_reset_counters(self.handle)
@@ -68,21 +70,31 @@ class Context:
         except Exception:
             pass
 
-    def gemm(self, A, B, precision="fp64", evens=None):
+    def gemm(self, A, B, precision="fp64", evens=None, layout=0):
         """Self-test: C = A @ B through the library's matrix-core kernel. With
         ``evens = (eM, eN, eK)`` the indices are Z2 legs (first e entries even)
-        and only parity-matching entries of C are computed (others NaN)."""
-        A = np.ascontiguousarray(A, dtype=np.float64)
-        B = np.ascontiguousarray(B, dtype=np.float64)
+        and only parity-matching entries of C are computed (others NaN).
+        ``layout`` bit 0 / bit 1 hand A / B to the device transposed in memory."""
+        A = np.asarray(A, dtype=np.float64)
+        B = np.asarray(B, dtype=np.float64)
         M, K = A.shape
         K2, N = B.shape
         if K != K2:
             raise ValueError("shape mismatch")
+        As = np.ascontiguousarray(A.T if layout & 1 else A)
+        Bs = np.ascontiguousarray(B.T if layout & 2 else B)
         C = np.zeros((M, N))
         eM, eN, eK = evens if evens is not None else (M, N, K)
         _lib.check(self.handle, self._lib.spp_selftest_gemm(self.handle, PRECISIONS[precision], M, N, K, eM, eN, eK,
-                                                             _lib.dptr(A), _lib.dptr(B), _lib.dptr(C)))
+                                                             int(layout), _lib.dptr(As), _lib.dptr(Bs),
+                                                             _lib.dptr(C)))
         return C
+
+    def copy_calibration(self, n):
+        """One launch of the library's plain copy kernel over n FP32 values
+        (8 n bytes of HBM traffic), for scaling hardware traffic counters."""
+        _lib.check(self.handle, self._lib.spp_selftest_copy(self.handle, int(n)))
+        return 8 * int(n)
 
 
 class DeviceState:

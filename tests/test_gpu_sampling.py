@@ -21,13 +21,16 @@ def say(*a):
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
 @pytest.mark.parametrize("shape", [(7, 5, 3), (130, 70, 37), (256, 300, 64)])
-def test_gemm_selftest(spp, gpu_ctx, precision, tol, shape):
-    say("gemm selftest", precision, shape)
+@pytest.mark.parametrize("layout", [0, 1, 2, 3])
+def test_gemm_selftest(spp, gpu_ctx, precision, tol, shape, layout):
+    """Dense products in all four operand memory orders (the kernel reads
+    each operand along its unit-stride index)."""
+    say("gemm selftest", precision, shape, layout)
     M, N, K = shape
     rng = np.random.default_rng(M * 7 + N)
     A = rng.standard_normal((M, K))
     B = rng.standard_normal((K, N))
-    C = gpu_ctx.gemm(A, B, precision)
+    C = gpu_ctx.gemm(A, B, precision, layout=layout)
     ref = A @ B
     err = np.max(np.abs(C - ref)) / np.max(np.abs(ref))
     assert err < tol, err
@@ -113,7 +116,8 @@ def test_evaluation_mode_sums_to_one(spp, gpu_ctx):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_z2_gemm_empty_k_block(spp, gpu_ctx, precision):
+@pytest.mark.parametrize("layout", [0, 3])
+def test_z2_gemm_empty_k_block(spp, gpu_ctx, precision, layout):
     """Regression: a parity block whose K range is empty (all K indices even)
     must write exact zeros into its (odd M, odd N) output block."""
     say("z2 gemm with empty odd K block", precision)
@@ -127,7 +131,7 @@ def test_z2_gemm_empty_k_block(spp, gpu_ctx, precision):
     pk = np.arange(K) >= eK
     A[pm[:, None] != pk[None, :]] = 0.0
     B[pk[:, None] != pn[None, :]] = 0.0
-    C = gpu_ctx.gemm(A, B, precision, evens=(eM, eN, eK))
+    C = gpu_ctx.gemm(A, B, precision, evens=(eM, eN, eK), layout=layout)
     neutral = pm[:, None] == pn[None, :]
     ref = A @ B
     assert np.all(np.isnan(C[~neutral]))
