@@ -1,12 +1,18 @@
 // Batched Z2 block-sparse GEMM on CDNA4 matrix cores (see z2gemm.h).
 //
-// Block tile 128x128, K tile 32 (FP32) / 16 (FP64), 256 threads = 4 waves in a
-// 2x2 grid, each wave owning a 64x64 sub-tile. FP32 uses v_mfma_f32_32x32x2f32
-// (2x2 tiles per wave), FP64 uses v_mfma_f64_16x16x4f64 (4x4 tiles per wave).
-// Operand tiles are gathered from global memory through the decoded row/column
-// offsets, with consecutive lanes along each operand's unit-stride index, held
-// in registers one tile ahead of the MFMA work, and staged k-major in padded
-// LDS so that each MFMA operand read is a contiguous run of lanes.
+// 256 threads = 4 waves per block; the block tile BM x BN is split into a
+// WGM x (4 / WGM) grid of wave tiles. FP32 uses v_mfma_f32_32x32x2f32, FP64
+// v_mfma_f64_16x16x4f64, TA x TB MFMA tiles per wave. K advances in tiles of
+// 32 (FP32) / 16 (FP64). Operand tiles are gathered from global memory through
+// decoded row/column offsets (the K offsets of up to 2048 indices decoded once
+// per block into LDS), with consecutive lanes along each operand's unit-stride
+// index, held in registers one tile ahead of the MFMA work, and staged k-major
+// in padded LDS so that each MFMA operand read is a contiguous run of lanes.
+//
+// Tile shapes (chosen per launch by the padded area of the problem):
+//   128 x 128, 2 x 2 waves           general
+//   FP32 256 x 96 / FP64 128 x 96    N a few multiples of 96 (e.g. 648 = 6.75 x 96)
+//   256 x 32, 4 x 1 waves            N <= 32
 #include "z2gemm.h"
 
 namespace spp {
@@ -14,82 +20,110 @@ namespace spp {
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
-constexpr int BM = 128;
-constexpr int BN = 128;
 constexpr int kThreads = 256;
-constexpr int PAD = 1;  // LDS row padding: k-major stores from 16/32 distinct k hit distinct banks
+constexpr int PAD = 1;  // LDS row padding: k-major stores from distinct k hit distinct banks
 
 template <typename T>
-struct TileCfg;
+struct KTile;
 template <>
-struct TileCfg<float> {
+struct KTile<float> {
   static constexpr int BK = 32;
+  static constexpr int MT = 32;  // MFMA tile edge
 };
 template <>
-struct TileCfg<double> {
+struct KTile<double> {
   static constexpr int BK = 16;
+  static constexpr int MT = 16;
 };
 
-template <typename T>
+template <typename T, int TA, int TB>
 struct WaveAcc;
 
-template <>
-struct WaveAcc<float> {
-  static constexpr int BK = TileCfg<float>::BK;
-  floatx16 acc[2][2];
+template <int TA, int TB>
+struct WaveAcc<float, TA, TB> {
+  static constexpr int BK = KTile<float>::BK;
+  static constexpr int NV = 16;
+  floatx16 acc[TA][TB];
   __device__ void zero() {
-    for (int a = 0; a < 2; ++a)
-      for (int b = 0; b < 2; ++b)
+#pragma unroll
+    for (int a = 0; a < TA; ++a)
+#pragma unroll
+      for (int b = 0; b < TB; ++b)
+#pragma unroll
         for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
   }
-  // As/Bs are k-major [BK][BM+PAD] / [BK][BN+PAD]; wm/wn are the wave's origin.
+  // As/Bs are k-major [BK][LDA] / [BK][LDB]; wm/wn are the wave's origin.
+  // The operands of k step kk + 2 are read from LDS before the MFMAs of step
+  // kk are issued, so the LDS latency overlaps the matrix work.
+  template <int LDA, int LDB>
   __device__ void step(const float* As, const float* Bs, int wm, int wn, int lane) {
     const int li = lane & 31, lk = lane >> 5;
+    float a[2][TA], b[2][TB];
+#pragma unroll
+    for (int t = 0; t < TA; ++t) a[0][t] = As[lk * LDA + wm + 32 * t + li];
+#pragma unroll
+    for (int t = 0; t < TB; ++t) b[0][t] = Bs[lk * LDB + wn + 32 * t + li];
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 2) {
-      const float* ar = As + (kk + lk) * (BM + PAD) + wm + li;
-      const float* br = Bs + (kk + lk) * (BN + PAD) + wn + li;
-      float a0 = ar[0], a1 = ar[32], b0 = br[0], b1 = br[32];
-      acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+      const int c = (kk >> 1) & 1;
+      if (kk + 2 < BK) {
+#pragma unroll
+        for (int t = 0; t < TA; ++t) a[c ^ 1][t] = As[(kk + 2 + lk) * LDA + wm + 32 * t + li];
+#pragma unroll
+        for (int t = 0; t < TB; ++t) b[c ^ 1][t] = Bs[(kk + 2 + lk) * LDB + wn + 32 * t + li];
+      }
+      __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of this step's MFMAs
+#pragma unroll
+      for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb)
+          acc[ta][tb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[c][ta], b[c][tb], acc[ta][tb], 0, 0, 0);
     }
   }
-  // Row/column (within the wave's 64x64 tile) of accumulator element (ta,tb,v):
   // v_mfma_f32_32x32x2f32 lane l, element v -> row (v%4) + 4(l/32) + 8(v/4), column l%32.
   static __device__ void coord(int ta, int tb, int v, int lane, int& i, int& j) {
     i = ta * 32 + (v & 3) + 4 * (lane >> 5) + 8 * (v >> 2);
     j = tb * 32 + (lane & 31);
   }
   __device__ float get(int ta, int tb, int v) const { return acc[ta][tb][v]; }
-  static constexpr int TA = 2, TB = 2, NV = 16;
 };
 
-template <>
-struct WaveAcc<double> {
-  static constexpr int BK = TileCfg<double>::BK;
-  doublex4 acc[4][4];
+template <int TA, int TB>
+struct WaveAcc<double, TA, TB> {
+  static constexpr int BK = KTile<double>::BK;
+  static constexpr int NV = 4;
+  doublex4 acc[TA][TB];
   __device__ void zero() {
-    for (int a = 0; a < 4; ++a)
-      for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int a = 0; a < TA; ++a)
+#pragma unroll
+      for (int b = 0; b < TB; ++b)
+#pragma unroll
         for (int v = 0; v < 4; ++v) acc[a][b][v] = 0.0;
   }
+  template <int LDA, int LDB>
   __device__ void step(const double* As, const double* Bs, int wm, int wn, int lane) {
     const int li = lane & 15, lk = lane >> 4;
+    double a[2][TA], b[2][TB];
+#pragma unroll
+    for (int t = 0; t < TA; ++t) a[0][t] = As[lk * LDA + wm + 16 * t + li];
+#pragma unroll
+    for (int t = 0; t < TB; ++t) b[0][t] = Bs[lk * LDB + wn + 16 * t + li];
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      double a[4], b[4];
+      const int c = (kk >> 2) & 1;
+      if (kk + 4 < BK) {
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        a[t] = As[(kk + lk) * (BM + PAD) + wm + 16 * t + li];
-        b[t] = Bs[(kk + lk) * (BN + PAD) + wn + 16 * t + li];
+        for (int t = 0; t < TA; ++t) a[c ^ 1][t] = As[(kk + 4 + lk) * LDA + wm + 16 * t + li];
+#pragma unroll
+        for (int t = 0; t < TB; ++t) b[c ^ 1][t] = Bs[(kk + 4 + lk) * LDB + wn + 16 * t + li];
       }
+      __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of this step's MFMAs
 #pragma unroll
-      for (int ta = 0; ta < 4; ++ta)
+      for (int ta = 0; ta < TA; ++ta)
 #pragma unroll
-        for (int tb = 0; tb < 4; ++tb)
-          acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta], b[tb], acc[ta][tb], 0, 0, 0);
+        for (int tb = 0; tb < TB; ++tb)
+          acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c][ta], b[c][tb], acc[ta][tb], 0, 0, 0);
     }
   }
   // v_mfma_f64_16x16x4f64 output layout on gfx950 (checked with a one-wave
@@ -99,7 +133,6 @@ struct WaveAcc<double> {
     j = tb * 16 + (lane & 15);
   }
   __device__ double get(int ta, int tb, int v) const { return acc[ta][tb][v]; }
-  static constexpr int TA = 4, TB = 4, NV = 4;
 };
 
 template <typename T>
@@ -121,13 +154,60 @@ __device__ inline double buffer_load(__amdgpu_buffer_rsrc_t r, unsigned off, dou
 
 constexpr int KLDS = 2048;  // K offsets held in LDS per pass over K
 
-template <typename T, int AMF, int BKF>
+// Thread -> (row, k) assignment of an R x BK operand tile, j = 0 .. R*BK/256-1.
+// Row-fast: runs of RUN consecutive rows per k (RUN = min(R, 256)); K-fast:
+// consecutive k of one row. Both are affine in j with compile-time steps.
+template <int R, int BK, int ROWFAST>
+struct TileMap {
+  static constexpr int RUN = R < kThreads ? (R % 32 == 0 && R % 64 != 0 ? 32 : R) : kThreads;
+  static constexpr int GROUPS = R / RUN;        // row runs per k (row-fast)
+  static constexpr int KSTEP = kThreads / RUN;  // k rows covered by one j group (row-fast)
+  static constexpr int N = R * BK / kThreads;   // elements per thread
+  static constexpr int RSTEP = kThreads / BK;   // rows covered per j (K-fast)
+  static_assert(kThreads % RUN == 0 && R % RUN == 0 && BK % KSTEP == 0 && R % RSTEP == 0, "tile map");
+  __device__ static int row(int tid, int j) {
+    return ROWFAST ? tid % RUN + RUN * (j % GROUPS) : tid / BK + RSTEP * j;
+  }
+  __device__ static int k(int tid, int j) { return ROWFAST ? tid / RUN + KSTEP * (j / GROUPS) : tid % BK; }
+};
+
+// Tile order. Workgroups are dealt to the 8 XCDs round-robin by linear id,
+// and each XCD has its own L2. The linear id is remapped so that each XCD
+// receives a contiguous range of tiles in the order [sample][M tile][parity]
+// [N tile]: the N tiles of one A row panel run together on one XCD, and so do
+// the panels of both parity problems of a sample, whose rows interleave in
+// memory (a parity-sorted last leg puts both parities in each cache line).
+// The map is a bijection of [0, total).
+constexpr int kXcds = 8;
+__device__ inline void tile_order(int& bn, int& bm, int& prob) {
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned total = gx * gy * gridDim.z;
+  const unsigned bid = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned per = (total + kXcds - 1) / kXcds;
+  const unsigned tall = total % kXcds == 0 ? kXcds : total % kXcds;  // XCDs that receive `per` tiles
+  const unsigned xcd = bid % kXcds, local = bid / kXcds;
+  unsigned t = xcd < tall ? xcd * per + local : tall * per + (xcd - tall) * (per - 1) + local;
+  bn = (int)(t % gx);
+  t /= gx;
+  const int q = (int)(t & 1);
+  t >>= 1;
+  bm = (int)(t % gy);
+  prob = 2 * (int)(t / gy) + q;
+}
+
+template <typename T, int BM, int BN, int WGM, int AMF, int BKF>
 __global__ __launch_bounds__(kThreads) void z2gemm_kernel(GemmArgs args) {
-  constexpr int BK = TileCfg<T>::BK;
-  constexpr int AL = BM * BK / kThreads;  // A elements per thread per tile
-  constexpr int BL = BN * BK / kThreads;  // B elements per thread per tile
-  __shared__ T As[BK * (BM + PAD)];
-  __shared__ T Bs[BK * (BN + PAD)];
+  constexpr int BK = KTile<T>::BK;
+  constexpr int MT = KTile<T>::MT;
+  constexpr int WGN = 4 / WGM;
+  constexpr int WTM = BM / WGM, WTN = BN / WGN;  // wave tile
+  constexpr int TA = WTM / MT, TB = WTN / MT;
+  static_assert(TA * MT == WTM && TB * MT == WTN, "wave tile must be a multiple of the MFMA tile");
+  constexpr int LDA = BM + PAD, LDB = BN + PAD;
+  using MA = TileMap<BM, BK, AMF>;
+  using MB = TileMap<BN, BK, 1 - BKF>;
+  __shared__ T As[BK * LDA];
+  __shared__ T Bs[BK * LDB];
   // byte offsets of A rows / B columns (invalid: 0, any readable element) and
   // element offsets of C rows / columns (invalid: -1)
   __shared__ unsigned offAm[BM], offBn[BN];
@@ -136,7 +216,12 @@ __global__ __launch_bounds__(kThreads) void z2gemm_kernel(GemmArgs args) {
   __shared__ GroupView gvM, gvN, gvK;
 
   const int tid = threadIdx.x;
-  const int prob = blockIdx.z;
+  int bx, by, prob;
+  tile_order(bx, by, prob);
+  // wave-uniform: keeps the operand bases and buffer descriptors in SGPRs
+  bx = __builtin_amdgcn_readfirstlane(bx);
+  by = __builtin_amdgcn_readfirstlane(by);
+  prob = __builtin_amdgcn_readfirstlane(prob);
   const int sp = prob >> 1;
   const int q = prob & 1;
   if (args.pmask && !args.pmask[prob]) return;
@@ -149,31 +234,33 @@ __global__ __launch_bounds__(kThreads) void z2gemm_kernel(GemmArgs args) {
   const int cM = (int)group_count(gvM, q);
   const int cN = (int)group_count(gvN, q);
   const int cK = (int)group_count(gvK, q);
-  const int m0 = blockIdx.y * BM;
-  const int n0 = blockIdx.x * BN;
+  const int m0 = by * BM;
+  const int n0 = bx * BN;
   if (m0 >= cM || n0 >= cN) return;
 
   int legs[kMaxLegs];
-  if (tid < BM) {
-    const int idx = m0 + tid;
-    if (idx < cM) {
-      decode32(gvM, q, idx, legs);
-      offAm[tid] = (unsigned)operand_offset(args.a.g[0], gvM, idx, legs) * (unsigned)sizeof(T);
-      offCm[tid] = (int)operand_offset(args.c.g[0], gvM, idx, legs);
+  for (int r = tid; r < BM + BN; r += kThreads) {
+    if (r < BM) {
+      const int idx = m0 + r;
+      if (idx < cM) {
+        decode32(gvM, q, idx, legs);
+        offAm[r] = (unsigned)operand_offset(args.a.g[0], gvM, idx, legs) * (unsigned)sizeof(T);
+        offCm[r] = (int)operand_offset(args.c.g[0], gvM, idx, legs);
+      } else {
+        offAm[r] = 0;
+        offCm[r] = -1;
+      }
     } else {
-      offAm[tid] = 0;
-      offCm[tid] = -1;
-    }
-  } else {
-    const int t = tid - BM;
-    const int idx = n0 + t;
-    if (idx < cN) {
-      decode32(gvN, q, idx, legs);
-      offBn[t] = (unsigned)operand_offset(args.b.g[1], gvN, idx, legs) * (unsigned)sizeof(T);
-      offCn[t] = (int)operand_offset(args.c.g[1], gvN, idx, legs);
-    } else {
-      offBn[t] = 0;
-      offCn[t] = -1;
+      const int t = r - BM;
+      const int idx = n0 + t;
+      if (idx < cN) {
+        decode32(gvN, q, idx, legs);
+        offBn[t] = (unsigned)operand_offset(args.b.g[1], gvN, idx, legs) * (unsigned)sizeof(T);
+        offCn[t] = (int)operand_offset(args.c.g[1], gvN, idx, legs);
+      } else {
+        offBn[t] = 0;
+        offCn[t] = -1;
+      }
     }
   }
 
@@ -185,29 +272,19 @@ __global__ __launch_bounds__(kThreads) void z2gemm_kernel(GemmArgs args) {
 
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = (wave >> 1) * 64;
-  const int wn = (wave & 1) * 64;
-  WaveAcc<T> acc;
+  const int wm = (wave / WGN) * WTM;
+  const int wn = (wave % WGN) * WTN;
+  WaveAcc<T, TA, TB> acc;
   acc.zero();
 
-  // Tile loads. Consecutive lanes follow the operand's unit-stride direction
-  // (AMF / BKF, chosen on the host): along K (k = tid % BK, rows tid / BK +
-  // j AR) or along the row index (row = tid % 128, k = tid / 128 + j KR), so
-  // that a wave's load touches few cache lines in either layout.
-  constexpr int AR = kThreads / BK;    // row step of a K-fast assignment
-  constexpr int KR = kThreads / BM;    // k step of a row-fast assignment (BM == BN)
-  const int a_r = AMF ? tid % BM : tid / BK, a_k = AMF ? tid / BM : tid % BK;
-  const int b_r = BKF ? tid / BK : tid % BN, b_k = BKF ? tid % BK : tid / BN;
-  constexpr int a_dr = AMF ? 0 : AR, a_dk = AMF ? KR : 0;
-  constexpr int b_dr = BKF ? AR : 0, b_dk = BKF ? 0 : KR;
-  T ra[AL], rb[BL];
+  T ra[MA::N], rb[MB::N];
   auto load_tile = [&](int kt) {  // kt: first k of the tile within the LDS offset table
 #pragma unroll
-    for (int j = 0; j < AL; ++j)
-      ra[j] = buffer_load(ra_rsrc, offAm[a_r + j * a_dr] + offAk[kt + a_k + j * a_dk], T(0));
+    for (int j = 0; j < MA::N; ++j)
+      ra[j] = buffer_load(ra_rsrc, offAm[MA::row(tid, j)] + offAk[kt + MA::k(tid, j)], T(0));
 #pragma unroll
-    for (int j = 0; j < BL; ++j)
-      rb[j] = buffer_load(rb_rsrc, offBn[b_r + j * b_dr] + offBk[kt + b_k + j * b_dk], T(0));
+    for (int j = 0; j < MB::N; ++j)
+      rb[j] = buffer_load(rb_rsrc, offBn[MB::row(tid, j)] + offBk[kt + MB::k(tid, j)], T(0));
   };
 
   for (int kc = 0; kc < cK; kc += KLDS) {
@@ -229,12 +306,12 @@ __global__ __launch_bounds__(kThreads) void z2gemm_kernel(GemmArgs args) {
     load_tile(0);
     for (int t = 0; t < nt; ++t) {
 #pragma unroll
-      for (int j = 0; j < AL; ++j) As[(a_k + j * a_dk) * (BM + PAD) + a_r + j * a_dr] = ra[j];
+      for (int j = 0; j < MA::N; ++j) As[MA::k(tid, j) * LDA + MA::row(tid, j)] = ra[j];
 #pragma unroll
-      for (int j = 0; j < BL; ++j) Bs[(b_k + j * b_dk) * (BN + PAD) + b_r + j * b_dr] = rb[j];
+      for (int j = 0; j < MB::N; ++j) Bs[MB::k(tid, j) * LDB + MB::row(tid, j)] = rb[j];
       __syncthreads();
       if (t + 1 < nt) load_tile((t + 1) * BK);
-      acc.step(As, Bs, wm, wn, lane);
+      acc.template step<LDA, LDB>(As, Bs, wm, wn, lane);
       __syncthreads();
     }
   }
@@ -243,13 +320,13 @@ __global__ __launch_bounds__(kThreads) void z2gemm_kernel(GemmArgs args) {
 
   const T alpha = (T)args.alpha;
 #pragma unroll
-  for (int ta = 0; ta < WaveAcc<T>::TA; ++ta)
+  for (int ta = 0; ta < TA; ++ta)
 #pragma unroll
-    for (int tb = 0; tb < WaveAcc<T>::TB; ++tb)
+    for (int tb = 0; tb < TB; ++tb)
 #pragma unroll
-      for (int v = 0; v < WaveAcc<T>::NV; ++v) {
+      for (int v = 0; v < WaveAcc<T, TA, TB>::NV; ++v) {
         int i, j;
-        WaveAcc<T>::coord(ta, tb, v, lane, i, j);
+        WaveAcc<T, TA, TB>::coord(ta, tb, v, lane, i, j);
         const int om = offCm[wm + i];
         const int on = offCn[wn + j];
         if (om >= 0 && on >= 0) {
@@ -261,20 +338,44 @@ __global__ __launch_bounds__(kThreads) void z2gemm_kernel(GemmArgs args) {
       }
 }
 
+// Block tile shapes per element type.
+template <typename T>
+struct Shapes;
+template <>
+struct Shapes<float> {
+  static constexpr int M1 = 256, N1 = 96, W1 = 4;  // wave tile 64 x 96
+};
+template <>
+struct Shapes<double> {
+  static constexpr int M1 = 128, N1 = 96, W1 = 2;  // wave tile 64 x 48
+};
+
+template <typename T, int BM, int BN, int WGM>
+static hipError_t launch_shape(const GemmArgs& args, long long mmax, long long nmax, hipStream_t stream) {
+  const long long gy = (mmax + BM - 1) / BM;
+  const long long gx = (nmax + BN - 1) / BN;
+  const long long gz = 2LL * args.nsamples;
+  if (gy > 65535 || gz > 65535 || gx > 2147483647LL || gx * gy * gz >= (1LL << 32))
+    return hipErrorInvalidConfiguration;
+  dim3 grid((unsigned)gx, (unsigned)gy, (unsigned)gz);
+  const int mode = (args.a_mfast ? 1 : 0) + (args.b_kfast ? 2 : 0);
+  if (mode == 0) hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, 0, 0>), grid, dim3(kThreads), 0, stream, args);
+  else if (mode == 1) hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, 1, 0>), grid, dim3(kThreads), 0, stream, args);
+  else if (mode == 2) hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, 0, 1>), grid, dim3(kThreads), 0, stream, args);
+  else hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, 1, 1>), grid, dim3(kThreads), 0, stream, args);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_z2gemm(const GemmArgs& args, long long mmax, long long nmax, hipStream_t stream) {
   if (mmax <= 0 || nmax <= 0 || args.nsamples <= 0) return hipSuccess;
-  long long gy = (mmax + BM - 1) / BM;
-  long long gx = (nmax + BN - 1) / BN;
-  long long gz = 2LL * args.nsamples;
-  if (gy > 65535 || gz > 65535 || gx > 2147483647LL) return hipErrorInvalidConfiguration;
-  dim3 grid((unsigned)gx, (unsigned)gy, (unsigned)gz);
-  const int mode = (args.a_mfast ? 1 : 0) + (args.b_kfast ? 2 : 0);
-  if (mode == 0) hipLaunchKernelGGL((z2gemm_kernel<T, 0, 0>), grid, dim3(kThreads), 0, stream, args);
-  else if (mode == 1) hipLaunchKernelGGL((z2gemm_kernel<T, 1, 0>), grid, dim3(kThreads), 0, stream, args);
-  else if (mode == 2) hipLaunchKernelGGL((z2gemm_kernel<T, 0, 1>), grid, dim3(kThreads), 0, stream, args);
-  else hipLaunchKernelGGL((z2gemm_kernel<T, 1, 1>), grid, dim3(kThreads), 0, stream, args);
-  return hipGetLastError();
+  if (nmax <= 32) return launch_shape<T, 256, 32, 4>(args, mmax, nmax, stream);
+  auto padded = [&](long long bm, long long bn) {
+    return ((mmax + bm - 1) / bm) * bm * (((nmax + bn - 1) / bn) * bn);
+  };
+  constexpr int M1 = Shapes<T>::M1, N1 = Shapes<T>::N1, W1 = Shapes<T>::W1;
+  if (padded(M1, N1) < padded(128, 128)) return launch_shape<T, M1, N1, W1>(args, mmax, nmax, stream);
+  return launch_shape<T, 128, 128, 2>(args, mmax, nmax, stream);
 }
 
 template hipError_t launch_z2gemm<float>(const GemmArgs&, long long, long long, hipStream_t);
