@@ -391,8 +391,10 @@ __global__ void random_block_kernel(double* q, long long pstride, int ld, int nc
 // orthonormalisation would lose those below sqrt(eps)). Z is overwritten by
 // the reflectors; tau needs ncol entries per problem.
 __global__ __launch_bounds__(256) void householder_q_kernel(double* z, double* q, long long pstride, int ld,
-                                                           int ncol, const int* n_of, double* tau_all) {
+                                                           int ncol, const int* n_of, double* tau_all,
+                                                           const int* pmask) {
   const int prob = blockIdx.x;
+  if (pmask && !pmask[prob]) return;
   const int n = n_of[prob];
   double* Z = z + (long long)prob * pstride;
   double* Q = q + (long long)prob * pstride;
@@ -505,8 +507,9 @@ __global__ __launch_bounds__(256) void householder_q_kernel(double* z, double* q
 // as householder_q_kernel.
 template <int NSLAB, int RMAX>
 __global__ __launch_bounds__(64 * NSLAB) void householder_q_reg_kernel(double* z, double* q, long long pstride, int ld,
-                                                                 int ncol, const int* n_of) {
+                                                                 int ncol, const int* n_of, const int* pmask) {
   const int prob = blockIdx.x;
+  if (pmask && !pmask[prob]) return;
   const int n = n_of[prob];
   double* Z = z + (long long)prob * pstride;
   double* Q = q + (long long)prob * pstride;
@@ -654,9 +657,13 @@ __global__ __launch_bounds__(64 * NSLAB) void householder_q_reg_kernel(double* z
 }
 
 // Residuals of Ritz pairs: conv[prob] = max_j<t ||Z_j - theta_j Q_j|| <= tol * |theta_0|.
+// Residual test of the Ritz pairs; a problem that converges leaves `active`
+// and keeps its Ritz pairs (later iterations skip it).
 __global__ void residual_kernel(const double* q, const double* z, long long pstride, int ld, const double* th,
-                                long long tstride, const int* n_of, int t, double tol, int* conv, double* worst) {
+                                long long tstride, const int* n_of, int t, double tol, int* conv, double* worst,
+                                int* active) {
   const int prob = blockIdx.x;
+  if (!active[prob]) return;
   const int n = n_of[prob];
   const double* qb = q + (long long)prob * pstride;
   const double* zb = z + (long long)prob * pstride;
@@ -689,6 +696,7 @@ __global__ void residual_kernel(const double* q, const double* z, long long pstr
     double rel = scale > 0.0 ? mx / scale : 0.0;
     conv[prob] = (rel <= tol) ? 1 : 0;
     worst[prob] = rel;
+    active[prob] = (rel <= tol) ? 0 : 1;
   }
 }
 
@@ -853,19 +861,23 @@ static void eig_topk(Context& ctx, const double* rho, long long rho_pstride, int
   };
   TRef rhoR = compact_ref(const_cast<double*>(rho), 2 * rho_pstride, rho_pstride, rho_ld, "kpKP", 2,
                           {lk, lp, lk, lp});
+  // problems still iterating (sample * 2 + parity); converged ones keep their Ritz pairs
+  int* active = (int*)ctx.arena.take(sizeof(int) * nprob);
+  SPP_CHECK(launch_fill<int>(active, nprob, 1, st));
   auto matmul_rho = [&](double* Z, double* Q) {  // Z = rho Q
-    contract<double>(ctx, blockref(Z, "kpc"), rhoR, blockref(Q, "KPc"), nb);
+    contract<double>(ctx, blockref(Z, "kpc"), rhoR, blockref(Q, "KPc"), nb, 1.0, false, active);
   };
   auto rmul = [&](double* out, double* Z, double* V) {  // out = Z V
-    contract<double>(ctx, blockref(out, "kpb"), blockref(Z, "kpa"), smallref(V, "ab"), nb);
+    contract<double>(ctx, blockref(out, "kpb"), blockref(Z, "kpa"), smallref(V, "ab"), nb, 1.0, false, active);
   };
   double* tau = (double*)ctx.arena.take(sizeof(double) * nprob * kk);
   auto orth = [&](double* Q, double* Z, double* tmp) {  // Q <- orth(Z), Householder; Z destroyed
     (void)tmp;
     if (kk <= 64 && nmax <= 8 * 72) {
-      hipLaunchKernelGGL((householder_q_reg_kernel<8, 72>), dim3(nprob), dim3(512), 0, st, Z, Q, bs, kk, kk, n_of);
+      hipLaunchKernelGGL((householder_q_reg_kernel<8, 72>), dim3(nprob), dim3(512), 0, st, Z, Q, bs, kk, kk, n_of,
+                         active);
     } else {
-      hipLaunchKernelGGL(householder_q_kernel, dim3(nprob), dim3(256), 0, st, Z, Q, bs, kk, kk, n_of, tau);
+      hipLaunchKernelGGL(householder_q_kernel, dim3(nprob), dim3(256), 0, st, Z, Q, bs, kk, kk, n_of, tau, active);
     }
     SPP_CHECK(hipGetLastError());
   };
@@ -881,12 +893,12 @@ static void eig_topk(Context& ctx, const double* rho, long long rho_pstride, int
     matmul_rho(Za, Qa);
     // H = Q^T Z
     contract<double>(ctx, smallref(Sm, "ab"), compact_ref(Qa, 2 * bs, bs, kk, "kpa", 2, {lk, lp, fl}),
-                     blockref(Za, "kpb"), nb);
-    SPP_CHECK(launch_jacobi(Sm, ss, kk, sig, kk, Vm, ss, kk, kk_of, nullptr, nprob, st));
+                     blockref(Za, "kpb"), nb, 1.0, false, active);
+    SPP_CHECK(launch_jacobi(Sm, ss, kk, sig, kk, Vm, ss, kk, kk_of, active, nprob, st));
     rmul(Qb, Qa, Vm);
     rmul(Zb, Za, Vm);
     hipLaunchKernelGGL(residual_kernel, dim3(nprob), dim3(256), 0, st, Qb, Zb, bs, kk, sig, (long long)kk, n_of,
-                       t, ctx.subspace_tol, conv, worst);
+                       t, ctx.subspace_tol, conv, worst, active);
     SPP_CHECK(hipGetLastError());
     SPP_CHECK(hipMemcpyAsync(hconv.data(), conv, sizeof(int) * nprob, hipMemcpyDeviceToHost, st));
     SPP_CHECK(hipStreamSynchronize(st));

@@ -299,13 +299,25 @@ hipError_t launch_double_layers(const LayerJob* jobs_dev, int njobs, long long m
 
 // ---------------------------------------------------------------------------
 // Cyclic Jacobi eigensolver with round-robin parallel ordering, n <= 64, FP64.
+//
+// Each round rotates m/2 disjoint index pairs at once. With G the block
+// diagonal rotation of the round, A <- G A G^T changes a 2x2 block
+// A[{p1,q1},{p2,q2}] only through the rotations of its own two pairs, so one
+// pass over all (pair, pair) blocks applies the whole round with a single
+// barrier. A pair is rotated when |a_pq| exceeds both eps sqrt(|a_pp a_qq|)
+// (relative accuracy of the small eigenvalues) and 1e-17 ||A||_F (below that
+// the rotation changes no eigenvalue in FP64); the sweeps stop after one
+// without rotations.
 
 constexpr int JMAX = 64;
 constexpr int JLD = JMAX + 1;
+constexpr int JT = 256;                            // threads
+constexpr int JB = (JMAX / 2) * (JMAX / 2) / JT;   // 2x2 blocks per thread
+constexpr int JV = JMAX * (JMAX / 2) / JT;         // V row updates per thread
 
-__global__ __launch_bounds__(256) void jacobi_kernel(const double* a, long long astride, int lda, double* w,
-                                                     long long wstride, double* v, long long vstride, int ldv,
-                                                     const int* n_of, const int* pmask) {
+__global__ __launch_bounds__(JT) void jacobi_kernel(const double* a, long long astride, int lda, double* w,
+                                                    long long wstride, double* v, long long vstride, int ldv,
+                                                    const int* n_of, const int* pmask) {
   const int prob = blockIdx.x;
   if (pmask && !pmask[prob]) return;
   const int n = n_of[prob];
@@ -315,94 +327,125 @@ __global__ __launch_bounds__(256) void jacobi_kernel(const double* a, long long 
   __shared__ double cs[JMAX / 2], sn[JMAX / 2];
   __shared__ int pp_[JMAX / 2], qq_[JMAX / 2];
   __shared__ int order[JMAX + 1];
-  __shared__ double red[256];
-  __shared__ int done;
+  __shared__ double red[JT];
+  __shared__ int rotated;
   const int tid = threadIdx.x;
   const int m = (n & 1) ? n + 1 : n;  // padded size (dummy index n when odd)
+  const int half = m / 2;
   const double* ap = a + (long long)prob * astride;
-  for (int i = tid; i < m * m; i += blockDim.x) {
-    int r = i / m, c = i % m;
+  for (int i = tid; i < m * m; i += JT) {
+    const int r = i / m, c = i - (i / m) * m;
     double val = 0.0;
     if (r < n && c < n) val = 0.5 * (ap[(long long)r * lda + c] + ap[(long long)c * lda + r]);
     A[r * JLD + c] = val;
     V[r * JLD + c] = (r == c) ? 1.0 : 0.0;
   }
   if (tid <= m) order[tid] = tid;
+  // this thread's (pair, pair) blocks and (row, pair) V updates
+  int bk1[JB], bk2[JB], vj[JV], vk[JV];
+#pragma unroll
+  for (int i = 0; i < JB; ++i) {
+    const int b = tid + JT * i;
+    bk1[i] = half > 0 ? b / half : 0;
+    bk2[i] = half > 0 ? b - bk1[i] * half : 0;
+  }
+#pragma unroll
+  for (int i = 0; i < JV; ++i) {
+    const int e = tid + JT * i;
+    vk[i] = half > 0 ? e / m : 0;
+    vj[i] = e - vk[i] * m;
+  }
   __syncthreads();
-  // Frobenius norm for the stopping rule
   double loc = 0.0;
-  for (int i = tid; i < m * m; i += blockDim.x) {
-    double x = A[(i / m) * JLD + (i % m)];
+  for (int i = tid; i < m * m; i += JT) {
+    const double x = A[(i / m) * JLD + (i - (i / m) * m)];
     loc += x * x;
   }
   red[tid] = loc;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
+  for (int s = JT / 2; s > 0; s >>= 1) {
     if (tid < s) red[tid] += red[tid + s];
     __syncthreads();
   }
-  const double fro2 = red[0];
-  __syncthreads();
-  const int half = m / 2;
-  __shared__ int rotated;
-  (void)fro2;
-  // Cyclic sweeps; a pair is rotated only when |a_pq| > eps * sqrt(|a_pp a_qq|)
-  // (the relative threshold of one-sided/two-sided Jacobi for accurate small
-  // eigenvalues); stop after a sweep without rotations.
+  const double floor_abs = 1e-17 * sqrt(red[0]);
   for (int sweep = 0; sweep < 40; ++sweep) {
     if (tid == 0) rotated = 0;
     __syncthreads();
     for (int round = 0; round < m - 1; ++round) {
-      // round-robin pairing: position 0 fixed, others rotate
       if (tid < half) {
-        int i1 = tid, i2 = m - 1 - tid;
-        int p = (i1 == 0) ? 0 : 1 + ((i1 - 1 + round) % (m - 1));
-        int q = 1 + ((i2 - 1 + round) % (m - 1));
-        if (p > q) { int t = p; p = q; q = t; }
+        // round-robin pairing: position 0 fixed, the others rotate
+        const int i1 = tid, i2 = m - 1 - tid;
+        int p = 0;
+        if (i1 != 0) {
+          p = i1 - 1 + round;
+          if (p >= m - 1) p -= m - 1;
+          p += 1;
+        }
+        int q = i2 - 1 + round;
+        if (q >= m - 1) q -= m - 1;
+        q += 1;
+        if (p > q) {
+          const int t = p;
+          p = q;
+          q = t;
+        }
         pp_[tid] = p;
         qq_[tid] = q;
-        double apq = A[p * JLD + q];
-        double app = A[p * JLD + p], aqq = A[q * JLD + q];
-        double c = 1.0, s = 0.0;
-        if (fabs(apq) > 2.220446049250313e-16 * sqrt(fabs(app * aqq)) && fabs(apq) > 1e-300) {
-          double theta = (aqq - app) / (2.0 * apq);
-          double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double apq = A[p * JLD + q];
+        const double app = A[p * JLD + p], aqq = A[q * JLD + q];
+        double c = 1.0, sv = 0.0;
+        if (fabs(apq) > 2.220446049250313e-16 * sqrt(fabs(app * aqq)) && fabs(apq) > floor_abs) {
+          const double theta = (aqq - app) / (2.0 * apq);
+          const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
           c = 1.0 / sqrt(t * t + 1.0);
-          s = t * c;
+          sv = t * c;
           rotated = 1;
         }
         cs[tid] = c;
-        sn[tid] = s;
+        sn[tid] = sv;
       }
       __syncthreads();
-      // rows: A <- J^T A
-      for (int idx = tid; idx < half * m; idx += blockDim.x) {
-        int k = idx / m, j = idx % m;
-        double c = cs[k], s = sn[k];
-        if (s == 0.0) continue;
-        int p = pp_[k], q = qq_[k];
-        double ap_ = A[p * JLD + j], aq_ = A[q * JLD + j];
-        A[p * JLD + j] = c * ap_ - s * aq_;
-        A[q * JLD + j] = s * ap_ + c * aq_;
+      // A <- G A G^T block by block; G rows: p' = c p - s q, q' = s p + c q
+#pragma unroll
+      for (int i = 0; i < JB; ++i) {
+        const int k1 = bk1[i], k2 = bk2[i];
+        if (k1 >= half) continue;
+        const double s1 = sn[k1], s2 = sn[k2];
+        if (s1 == 0.0 && s2 == 0.0) continue;
+        const double c1 = cs[k1], c2 = cs[k2];
+        const int p1 = pp_[k1], q1 = qq_[k1], p2 = pp_[k2], q2 = qq_[k2];
+        const double a00 = A[p1 * JLD + p2], a01 = A[p1 * JLD + q2];
+        const double a10 = A[q1 * JLD + p2], a11 = A[q1 * JLD + q2];
+        // left: rows (p1, q1)
+        const double l00 = c1 * a00 - s1 * a10, l01 = c1 * a01 - s1 * a11;
+        const double l10 = s1 * a00 + c1 * a10, l11 = s1 * a01 + c1 * a11;
+        // right: columns (p2, q2)
+        double r00 = c2 * l00 - s2 * l01, r01 = s2 * l00 + c2 * l01;
+        double r10 = c2 * l10 - s2 * l11, r11 = s2 * l10 + c2 * l11;
+        if (k1 == k2) {  // the rotated pair's own block: exact zero off the diagonal
+          r01 = 0.0;
+          r10 = 0.0;
+        }
+        A[p1 * JLD + p2] = r00;
+        A[p1 * JLD + q2] = r01;
+        A[q1 * JLD + p2] = r10;
+        A[q1 * JLD + q2] = r11;
       }
-      __syncthreads();
-      // columns: A <- A J, V <- V J
-      for (int idx = tid; idx < half * m; idx += blockDim.x) {
-        int k = idx / m, j = idx % m;
-        double c = cs[k], s = sn[k];
-        if (s == 0.0) continue;
-        int p = pp_[k], q = qq_[k];
-        double ap_ = A[j * JLD + p], aq_ = A[j * JLD + q];
-        A[j * JLD + p] = c * ap_ - s * aq_;
-        A[j * JLD + q] = s * ap_ + c * aq_;
-        double vp = V[j * JLD + p], vq = V[j * JLD + q];
-        V[j * JLD + p] = c * vp - s * vq;
-        V[j * JLD + q] = s * vp + c * vq;
+#pragma unroll
+      for (int i = 0; i < JV; ++i) {
+        const int k = vk[i], j = vj[i];
+        if (k >= half) continue;
+        const double sv = sn[k];
+        if (sv == 0.0) continue;
+        const double c = cs[k];
+        const int p = pp_[k], q = qq_[k];
+        const double vp = V[j * JLD + p], vq = V[j * JLD + q];
+        V[j * JLD + p] = c * vp - sv * vq;
+        V[j * JLD + q] = sv * vp + c * vq;
       }
       __syncthreads();
     }
     if (!rotated) break;
-    __syncthreads();
   }
   // sort eigenvalues (descending) over the n real indices
   if (tid == 0) {
@@ -410,8 +453,8 @@ __global__ __launch_bounds__(256) void jacobi_kernel(const double* a, long long 
     for (int i = 0; i < m; ++i)
       if (i < n) order[cnt++] = i;
     for (int i = 1; i < n; ++i) {
-      int key = order[i];
-      double kv = A[key * JLD + key];
+      const int key = order[i];
+      const double kv = A[key * JLD + key];
       int j = i - 1;
       while (j >= 0 && A[order[j] * JLD + order[j]] < kv) {
         order[j + 1] = order[j];
@@ -423,9 +466,9 @@ __global__ __launch_bounds__(256) void jacobi_kernel(const double* a, long long 
   __syncthreads();
   double* wp = w + (long long)prob * wstride;
   double* vp = v + (long long)prob * vstride;
-  for (int i = tid; i < n; i += blockDim.x) wp[i] = A[order[i] * JLD + order[i]];
-  for (int i = tid; i < n * n; i += blockDim.x) {
-    int r = i / n, c = i % n;
+  for (int i = tid; i < n; i += JT) wp[i] = A[order[i] * JLD + order[i]];
+  for (int i = tid; i < n * n; i += JT) {
+    const int r = i / n, c = i - (i / n) * n;
     vp[(long long)r * ldv + c] = V[r * JLD + order[c]];
   }
 }
@@ -434,7 +477,7 @@ hipError_t launch_jacobi(const double* a, long long astride, int lda, double* w,
                          long long vstride, int ldv, const int* n_of, const int* pmask, int nprob,
                          hipStream_t st) {
   if (nprob <= 0) return hipSuccess;
-  hipLaunchKernelGGL(jacobi_kernel, dim3(nprob), dim3(256), 0, st, a, astride, lda, w, wstride, v, vstride, ldv,
+  hipLaunchKernelGGL(jacobi_kernel, dim3(nprob), dim3(JT), 0, st, a, astride, lda, w, wstride, v, vstride, ldv,
                      n_of, pmask);
   return hipGetLastError();
 }

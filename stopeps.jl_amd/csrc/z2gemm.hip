@@ -3,13 +3,14 @@
 // 256 threads = 4 waves per block; the block tile BM x BN is split into a
 // WGM x (4 / WGM) grid of wave tiles. FP32 uses v_mfma_f32_32x32x2f32, FP64
 // v_mfma_f64_16x16x4f64, TA x TB MFMA tiles per wave. K advances in tiles of
-// 32 (FP32) / 16 (FP64). Operand tiles are gathered from global memory through
-// decoded row/column offsets (the K offsets of up to 2048 indices decoded once
-// per block into LDS), with consecutive lanes along each operand's unit-stride
+// BK. Operand tiles are gathered from global memory through decoded
+// row/column offsets (the K offsets of up to 1024 indices decoded once per
+// block into LDS), with consecutive lanes along each operand's unit-stride
 // index, held in registers one tile ahead of the MFMA work, and staged k-major
 // in padded LDS so that each MFMA operand read is a contiguous run of lanes.
 //
-// Tile shapes (chosen per launch by the padded area of the problem):
+// Tile shapes (chosen per launch by the padded area of the problem; see
+// Shapes below for the K tile and occupancy target of each):
 //   128 x 128, 2 x 2 waves           general
 //   FP32 256 x 96 / FP64 128 x 96    N a few multiples of 96 (e.g. 648 = 6.75 x 96)
 //   256 x 32, 4 x 1 waves            N <= 32
@@ -27,21 +28,18 @@ template <typename T>
 struct KTile;
 template <>
 struct KTile<float> {
-  static constexpr int BK = 32;
   static constexpr int MT = 32;  // MFMA tile edge
 };
 template <>
 struct KTile<double> {
-  static constexpr int BK = 16;
   static constexpr int MT = 16;
 };
 
-template <typename T, int TA, int TB>
+template <typename T, int TA, int TB, int BK>
 struct WaveAcc;
 
-template <int TA, int TB>
-struct WaveAcc<float, TA, TB> {
-  static constexpr int BK = KTile<float>::BK;
+template <int TA, int TB, int BK>
+struct WaveAcc<float, TA, TB, BK> {
   static constexpr int NV = 16;
   floatx16 acc[TA][TB];
   __device__ void zero() {
@@ -88,9 +86,8 @@ struct WaveAcc<float, TA, TB> {
   __device__ float get(int ta, int tb, int v) const { return acc[ta][tb][v]; }
 };
 
-template <int TA, int TB>
-struct WaveAcc<double, TA, TB> {
-  static constexpr int BK = KTile<double>::BK;
+template <int TA, int TB, int BK>
+struct WaveAcc<double, TA, TB, BK> {
   static constexpr int NV = 4;
   doublex4 acc[TA][TB];
   __device__ void zero() {
@@ -152,7 +149,7 @@ __device__ inline double buffer_load(__amdgpu_buffer_rsrc_t r, unsigned off, dou
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
 }
 
-constexpr int KLDS = 2048;  // K offsets held in LDS per pass over K
+constexpr int KLDS = 1024;  // K offsets held in LDS per pass over K
 
 // Thread -> (row, k) assignment of an R x BK operand tile, j = 0 .. R*BK/256-1.
 // Row-fast: runs of RUN consecutive rows per k (RUN = min(R, 256)); K-fast:
@@ -195,9 +192,9 @@ __device__ inline void tile_order(int& bn, int& bm, int& prob) {
   prob = 2 * (int)(t / gy) + q;
 }
 
-template <typename T, int BM, int BN, int WGM, int AMF, int BKF>
-__global__ __launch_bounds__(kThreads) void z2gemm_kernel(GemmArgs args) {
-  constexpr int BK = KTile<T>::BK;
+// MINW: waves per SIMD the register allocation must allow (occupancy target).
+template <typename T, int BM, int BN, int WGM, int BK, int MINW, int AMF, int BKF>
+__global__ __launch_bounds__(kThreads, MINW) void z2gemm_kernel(GemmArgs args) {
   constexpr int MT = KTile<T>::MT;
   constexpr int WGN = 4 / WGM;
   constexpr int WTM = BM / WGM, WTN = BN / WGN;  // wave tile
@@ -274,7 +271,7 @@ __global__ __launch_bounds__(kThreads) void z2gemm_kernel(GemmArgs args) {
   const int wave = tid >> 6;
   const int wm = (wave / WGN) * WTM;
   const int wn = (wave % WGN) * WTN;
-  WaveAcc<T, TA, TB> acc;
+  WaveAcc<T, TA, TB, BK> acc;
   acc.zero();
 
   T ra[MA::N], rb[MB::N];
@@ -324,9 +321,9 @@ __global__ __launch_bounds__(kThreads) void z2gemm_kernel(GemmArgs args) {
 #pragma unroll
     for (int tb = 0; tb < TB; ++tb)
 #pragma unroll
-      for (int v = 0; v < WaveAcc<T, TA, TB>::NV; ++v) {
+      for (int v = 0; v < WaveAcc<T, TA, TB, BK>::NV; ++v) {
         int i, j;
-        WaveAcc<T, TA, TB>::coord(ta, tb, v, lane, i, j);
+        WaveAcc<T, TA, TB, BK>::coord(ta, tb, v, lane, i, j);
         const int om = offCm[wm + i];
         const int on = offCn[wn + j];
         if (om >= 0 && on >= 0) {
@@ -338,19 +335,23 @@ __global__ __launch_bounds__(kThreads) void z2gemm_kernel(GemmArgs args) {
       }
 }
 
-// Block tile shapes per element type.
+// Block tile shapes per element type: (BM, BN, wave rows, K tile, waves per SIMD).
 template <typename T>
 struct Shapes;
 template <>
 struct Shapes<float> {
-  static constexpr int M1 = 256, N1 = 96, W1 = 4;  // wave tile 64 x 96
+  static constexpr int M0 = 128, N0 = 128, W0 = 2, K0 = 16, O0 = 3;  // wave tile 64 x 64
+  static constexpr int M1 = 256, N1 = 96, W1 = 4, K1 = 16, O1 = 3;   // wave tile 64 x 96
+  static constexpr int M2 = 256, N2 = 32, W2 = 4, K2 = 32, O2 = 2;   // wave tile 64 x 32
 };
 template <>
 struct Shapes<double> {
-  static constexpr int M1 = 128, N1 = 96, W1 = 2;  // wave tile 64 x 48
+  static constexpr int M0 = 128, N0 = 128, W0 = 2, K0 = 16, O0 = 1;  // wave tile 64 x 64
+  static constexpr int M1 = 128, N1 = 96, W1 = 2, K1 = 8, O1 = 2;    // wave tile 64 x 48
+  static constexpr int M2 = 256, N2 = 32, W2 = 4, K2 = 16, O2 = 2;   // wave tile 64 x 32
 };
 
-template <typename T, int BM, int BN, int WGM>
+template <typename T, int BM, int BN, int WGM, int BK, int MINW>
 static hipError_t launch_shape(const GemmArgs& args, long long mmax, long long nmax, hipStream_t stream) {
   const long long gy = (mmax + BM - 1) / BM;
   const long long gx = (nmax + BN - 1) / BN;
@@ -359,23 +360,28 @@ static hipError_t launch_shape(const GemmArgs& args, long long mmax, long long n
     return hipErrorInvalidConfiguration;
   dim3 grid((unsigned)gx, (unsigned)gy, (unsigned)gz);
   const int mode = (args.a_mfast ? 1 : 0) + (args.b_kfast ? 2 : 0);
-  if (mode == 0) hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, 0, 0>), grid, dim3(kThreads), 0, stream, args);
-  else if (mode == 1) hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, 1, 0>), grid, dim3(kThreads), 0, stream, args);
-  else if (mode == 2) hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, 0, 1>), grid, dim3(kThreads), 0, stream, args);
-  else hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, 1, 1>), grid, dim3(kThreads), 0, stream, args);
+  if (mode == 0)
+    hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, BK, MINW, 0, 0>), grid, dim3(kThreads), 0, stream, args);
+  else if (mode == 1)
+    hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, BK, MINW, 1, 0>), grid, dim3(kThreads), 0, stream, args);
+  else if (mode == 2)
+    hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, BK, MINW, 0, 1>), grid, dim3(kThreads), 0, stream, args);
+  else
+    hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, BK, MINW, 1, 1>), grid, dim3(kThreads), 0, stream, args);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_z2gemm(const GemmArgs& args, long long mmax, long long nmax, hipStream_t stream) {
+  using S = Shapes<T>;
   if (mmax <= 0 || nmax <= 0 || args.nsamples <= 0) return hipSuccess;
-  if (nmax <= 32) return launch_shape<T, 256, 32, 4>(args, mmax, nmax, stream);
+  if (nmax <= S::N2) return launch_shape<T, S::M2, S::N2, S::W2, S::K2, S::O2>(args, mmax, nmax, stream);
   auto padded = [&](long long bm, long long bn) {
     return ((mmax + bm - 1) / bm) * bm * (((nmax + bn - 1) / bn) * bn);
   };
-  constexpr int M1 = Shapes<T>::M1, N1 = Shapes<T>::N1, W1 = Shapes<T>::W1;
-  if (padded(M1, N1) < padded(128, 128)) return launch_shape<T, M1, N1, W1>(args, mmax, nmax, stream);
-  return launch_shape<T, 128, 128, 2>(args, mmax, nmax, stream);
+  if (padded(S::M1, S::N1) < padded(S::M0, S::N0))
+    return launch_shape<T, S::M1, S::N1, S::W1, S::K1, S::O1>(args, mmax, nmax, stream);
+  return launch_shape<T, S::M0, S::N0, S::W0, S::K0, S::O0>(args, mmax, nmax, stream);
 }
 
 template hipError_t launch_z2gemm<float>(const GemmArgs&, long long, long long, hipStream_t);
