@@ -148,11 +148,11 @@ struct EnergyRun {
     for (int x = Lx - 1; x >= 0; --x) {
       const int site = x + y * Lx;
       const LayerSpec& sp = P[site];
-      std::vector<Leg> xl = {N.bond(x), N.phys(x), Rl[x + 1][1], Rl[x + 1][2]};
-      TRef Xr = dense_ref(BX, count(xl), "aned", xl);
+      std::vector<Leg> xl = {N.bond(x), Rl[x + 1][2], N.phys(x), Rl[x + 1][1]};
+      TRef Xr = folded_ref(BX, count(xl), "adne", xl);
       contract<T>(ctx, Xr, N.ref(x, "anc"), Rref(x + 1), cur);
       std::vector<Leg> yl = {N.bond(x), Rl[x + 1][2], sp.w, sp.s};
-      TRef Yr = dense_ref(BY, count(yl), "adws", yl);
+      TRef Yr = folded_ref(BY, count(yl), "adws", yl);
       contract<T>(ctx, Yr, Xr, layer(kPlain, site, "wnes"), cur);
       TRef Rr = dense_ref(R[x], count(Rl[x]), "awb", Rl[x]);
       contract<T>(ctx, Rr, Yr, bottom_ref(S, Lx, x, "bsd", true), cur);
@@ -167,8 +167,8 @@ struct EnergyRun {
     std::vector<Leg> Ll = {N.bond(0), P[y * Lx].w, south_bond(S, Lx, 0)};
     // X = L.T over the north bond
     auto make_x = [&](const TRef& L, int x) {
-      std::vector<Leg> xl = {L.leg[1], L.leg[2], N.phys(x), N.bond(x + 1)};
-      TRef Xr = dense_ref(BX, count(xl), "wbnc", xl);
+      std::vector<Leg> xl = {L.leg[1], L.leg[2], N.bond(x + 1), N.phys(x)};
+      TRef Xr = folded_ref(BX, count(xl), "wbcn", xl);
       contract<T>(ctx, Xr, L, N.ref(x, "anc"), cur);
       return Xr;
     };
@@ -176,7 +176,7 @@ struct EnergyRun {
     auto finish = [&](const TRef& Xr, int kind, int x, T* outbuf) {
       const LayerSpec& q = spec[kind][x + y * Lx];
       std::vector<Leg> yl = {Xr.leg[1], N.bond(x + 1), q.e, q.s};
-      TRef Yr = dense_ref(BY, count(yl), "bces", yl);
+      TRef Yr = folded_ref(BY, count(yl), "bces", yl);
       contract<T>(ctx, Yr, Xr, layer(kind, x + y * Lx, "wnes"), cur);
       std::vector<Leg> ol = {N.bond(x + 1), q.e, south_bond(S, Lx, x + 1)};
       TRef Or = dense_ref(outbuf, count(ol), "ced", ol);
@@ -231,14 +231,14 @@ struct EnergyRun {
       const LayerSpec& u = P[s1(x)];
       const LayerSpec& l = P[s2(x)];
       const Leg d = Rl[x + 1][3];
-      std::vector<Leg> xl = {N.bond(x), u.n, Rl[x + 1][1], Rl[x + 1][2], d};
-      TRef Xr = dense_ref(BX, count(xl), "anefd", xl);
+      std::vector<Leg> xl = {N.bond(x), u.n, d, Rl[x + 1][1], Rl[x + 1][2]};
+      TRef Xr = folded_ref(BX, count(xl), "andef", xl);
       contract<T>(ctx, Xr, N.ref(x, "anc"), Rref(x + 1), cur);
-      std::vector<Leg> yl = {N.bond(x), u.w, u.s, Rl[x + 1][2], d};
-      TRef Yr = dense_ref(BY, count(yl), "awkfd", yl);
+      std::vector<Leg> yl = {N.bond(x), u.w, d, u.s, Rl[x + 1][2]};
+      TRef Yr = folded_ref(BY, count(yl), "awdkf", yl);
       contract<T>(ctx, Yr, Xr, layer(kPlain, s1(x), "wnek"), cur);
-      std::vector<Leg> zl = {N.bond(x), u.w, l.w, l.s, d};
-      TRef Zr = dense_ref(BX, count(zl), "awvsd", zl);
+      std::vector<Leg> zl = {N.bond(x), u.w, d, l.w, l.s};
+      TRef Zr = folded_ref(BX, count(zl), "awdvs", zl);
       contract<T>(ctx, Zr, Yr, layer(kPlain, s2(x), "vkfs"), cur);
       TRef Rr = dense_ref(R[x], count(Rl[x]), "awvb", Rl[x]);
       contract<T>(ctx, Rr, Zr, bottom_ref(S, Lx, x, "bsd", true), cur);
@@ -254,14 +254,14 @@ struct EnergyRun {
       const LayerSpec& u = spec[ku][s1(x)];
       const LayerSpec& l = spec[kl][s2(x)];
       const Leg b = L.leg[3], c = N.bond(x + 1);
-      std::vector<Leg> xl = {L.leg[1], L.leg[2], b, N.phys(x), c};
-      TRef Xr = dense_ref(BX, count(xl), "wvbnc", xl);
+      std::vector<Leg> xl = {L.leg[1], L.leg[2], b, c, N.phys(x)};
+      TRef Xr = folded_ref(BX, count(xl), "wvbcn", xl);
       contract<T>(ctx, Xr, L, N.ref(x, "anc"), cur);
       std::vector<Leg> yl = {L.leg[2], b, c, u.e, u.s};
-      TRef Yr = dense_ref(BY, count(yl), "vbcek", yl);
+      TRef Yr = folded_ref(BY, count(yl), "vbcek", yl);
       contract<T>(ctx, Yr, Xr, layer(ku, s1(x), "wnek"), cur);
       std::vector<Leg> zl = {b, c, u.e, l.e, l.s};
-      TRef Zr = dense_ref(BX, count(zl), "bcefs", zl);
+      TRef Zr = folded_ref(BX, count(zl), "bcefs", zl);
       contract<T>(ctx, Zr, Yr, layer(kl, s2(x), "vkfs"), cur);
       std::vector<Leg> ol = {c, u.e, l.e, south_bond(S, Lx, x + 1)};
       TRef Or = dense_ref(outbuf, count(ol), "cefd", ol);

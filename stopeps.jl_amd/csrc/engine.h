@@ -77,9 +77,18 @@ struct TRef {
   bool compact = false;
   int nrow = 0;       // compact: number of row letters
   long long ld = 0;   // compact: leading dimension
+  // Folded dense storage: the last leg (fixed even count) keeps one parity
+  // half, max(even, odd) slots; an entry's position in that half follows from
+  // its index, and its parity from the neutrality of the stored entries.
+  bool fold = false;
 };
 
 TRef dense_ref(void* base, long long sstride, const std::string& letters, const std::vector<Leg>& legs);
+// Dense tensor with its last leg folded (neutral entries only, about half the
+// storage of dense_ref). Contractions only: no FullView kernels on it.
+TRef folded_ref(void* base, long long sstride, const std::string& letters, const std::vector<Leg>& legs);
+// Elements per sample of folded_ref(..., legs).
+long long folded_count(const std::vector<Leg>& legs);
 TRef compact_ref(void* base, long long sstride, long long pstride, long long ld, const std::string& letters,
                  int nrow, const std::vector<Leg>& legs);
 TRef with_letters(const TRef& t, const std::string& letters);

@@ -55,6 +55,33 @@ TRef dense_ref(void* base, long long sstride, const std::string& letters, const 
   return t;
 }
 
+static int fold_dim(const Leg& l) { return std::max(l.even, l.dim - l.even); }
+
+TRef folded_ref(void* base, long long sstride, const std::string& letters, const std::vector<Leg>& legs) {
+  if (letters.size() != legs.size() || legs.empty()) throw HipError("folded_ref: letters/legs mismatch for " + letters);
+  const Leg& last = legs.back();
+  if (last.split || last.flat) throw HipError("folded_ref: the last leg needs a fixed even count (" + letters + ")");
+  TRef t;
+  t.base = base;
+  t.sstride = sstride;
+  t.letters = letters;
+  t.leg = legs;
+  t.fold = true;
+  t.stride.assign(legs.size(), 1);
+  long long d = fold_dim(last);
+  for (int k = (int)legs.size() - 2; k >= 0; --k) {
+    t.stride[k] = t.stride[k + 1] * d;
+    d = legs[k].dim;
+  }
+  return t;
+}
+
+long long folded_count(const std::vector<Leg>& legs) {
+  long long n = 1;
+  for (size_t k = 0; k + 1 < legs.size(); ++k) n *= legs[k].dim;
+  return legs.empty() ? 1 : n * fold_dim(legs.back());
+}
+
 TRef compact_ref(void* base, long long sstride, long long pstride, long long ld, const std::string& letters,
                  int nrow, const std::vector<Leg>& legs) {
   TRef t;
@@ -135,6 +162,8 @@ static void fill_group(Group& g, const std::vector<Leg>& legs, const std::string
 
 static void fill_operand_group(OperandGroup& og, const TRef& t, const std::string& letters, int role_group,
                                const std::string& who) {
+  og.fold_j = -1;
+  og.fold_even = 0;
   if (t.compact) {
     std::string rows = t.letters.substr(0, t.nrow), cols = t.letters.substr(t.nrow);
     og.linear = 1;
@@ -151,6 +180,10 @@ static void fill_operand_group(OperandGroup& og, const TRef& t, const std::strin
     auto pos = t.letters.find(letters[j]);
     if (pos == std::string::npos) throw HipError("letter missing in operand (" + who + ")");
     og.stride[j] = t.stride[pos];
+    if (t.fold && pos + 1 == t.letters.size()) {
+      og.fold_j = (int)j;
+      og.fold_even = t.leg[pos].even;
+    }
   }
 }
 
@@ -216,7 +249,10 @@ void contract(Context& ctx, const TRef& C, const TRef& A, const TRef& B, int nsa
   auto extent = [](const TRef& t) -> unsigned long long {
     if (t.compact) return (unsigned long long)t.pstride;
     unsigned long long e = 1;
-    for (size_t j = 0; j < t.leg.size(); ++j) e += (unsigned long long)(t.leg[j].dim - 1) * t.stride[j];
+    for (size_t j = 0; j < t.leg.size(); ++j) {
+      const int d = (t.fold && j + 1 == t.leg.size()) ? fold_dim(t.leg[j]) : t.leg[j].dim;
+      e += (unsigned long long)(d - 1) * t.stride[j];
+    }
     return e;
   };
   const unsigned long long ea = extent(A) * sizeof(T), eb = extent(B) * sizeof(T), ec = extent(C);
@@ -279,6 +315,7 @@ template void contract<double>(Context&, const TRef&, const TRef&, const TRef&, 
 
 FullView full_view(const TRef& t) {
   if (t.compact) throw HipError("full_view of a compact operand");
+  if (t.fold) throw HipError("full_view of a folded operand");
   FullView v{};
   fill_group(v.g, t.leg, "full_view");
   for (size_t j = 0; j < t.leg.size(); ++j) v.stride[j] = t.stride[j];
@@ -950,11 +987,11 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
     TRef A = env.ref(x, "lnr");
     TRef Gn = dense_ref(G[x + 1], (long long)Gl[x + 1][0].dim * Gl[x + 1][1].dim * Gl[x + 1][2].dim * Gl[x + 1][3].dim,
                         "reRF", Gl[x + 1]);
-    TRef z1 = dense_ref(Z1, zmax, "lneRF", {bl, sp.n, sp.e, br, sp.e});
+    TRef z1 = folded_ref(Z1, zmax, "lneRF", {bl, sp.n, sp.e, br, sp.e});
     contract<T>(ctx, z1, A, Gn, nb);
-    TRef z2 = dense_ref(Z2, zmax, "lRFwp", {bl, br, sp.e, sp.w, sp.s});
+    TRef z2 = folded_ref(Z2, zmax, "lRFwp", {bl, br, sp.e, sp.w, sp.s});
     contract<T>(ctx, z2, z1, row.ref(x, "wnep"), nb);
-    TRef z3 = dense_ref(Z1, zmax, "lRwWN", {bl, br, sp.w, sp.w, sp.n});
+    TRef z3 = folded_ref(Z1, zmax, "lRwWN", {bl, br, sp.w, sp.w, sp.n});
     contract<T>(ctx, z3, z2, row.ref(x, "WNFp"), nb);
     TRef g = dense_ref(G[x], gs, "lwLW", Gl[x]);
     contract<T>(ctx, g, z3, env.ref(x, "LNR"), nb);

@@ -425,11 +425,11 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
           const size_t xm = ctx.arena.mark();
           long long xs = (long long)Nc.bdim[x] * sp.n.dim * sp.e.dim * Rl[x + 1][2].dim;
           T* X = (T*)ctx.arena.take(sizeof(T) * cur * xs);
-          TRef Xr = dense_ref(X, xs, "aned", {Nc.bond(x), sp.n, sp.e, Rl[x + 1][2]});
+          TRef Xr = folded_ref(X, xs, "adne", {Nc.bond(x), Rl[x + 1][2], sp.n, sp.e});
           contract<T>(ctx, Xr, Nc.ref(x, "anc"), dense_ref(R[x + 1], Rs[x + 1], "ced", Rl[x + 1]), cur);
           long long ys = (long long)Nc.bdim[x] * Rl[x + 1][2].dim * sp.w.dim * sp.s.dim;
           T* Y = (T*)ctx.arena.take(sizeof(T) * cur * ys);
-          TRef Yr = dense_ref(Y, ys, "adws", {Nc.bond(x), Rl[x + 1][2], sp.w, sp.s});
+          TRef Yr = folded_ref(Y, ys, "adws", {Nc.bond(x), Rl[x + 1][2], sp.w, sp.s});
           contract<T>(ctx, Yr, Xr, dense_ref(c.layers[x + y * Lx], 0, "wnes", {sp.w, sp.n, sp.e, sp.s}), cur);
           TRef Rr = dense_ref(R[x], Rs[x], "awb", Rl[x]);
           contract<T>(ctx, Rr, Yr, bottom_ref(sb, Lx, x, "bsd"), cur);
@@ -453,7 +453,7 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
           Leg sbl = south_bond(sb, Lx, x), sbr = south_bond(sb, Lx, x + 1);
           long long xs = (long long)sp.w.dim * sbl.dim * sp.n.dim * Nc.bdim[x + 1];
           T* X = (T*)ctx.arena.take(sizeof(T) * cur * xs);
-          TRef Xr = dense_ref(X, xs, "wbnc", {sp.w, sbl, sp.n, Nc.bond(x + 1)});
+          TRef Xr = folded_ref(X, xs, "wbcn", {sp.w, sbl, Nc.bond(x + 1), sp.n});
           contract<T>(ctx, Xr, dense_ref(Lb, Ls, "awb", Ll), Nc.ref(x, "anc"), cur);
           long long ys = (long long)sbl.dim * Nc.bdim[x + 1] * sp.e.dim * sp.s.dim;
           long long os = (long long)Nc.bdim[x + 1] * sp.e.dim * sbr.dim;
@@ -464,7 +464,7 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
           std::vector<Leg> Ol = {Nc.bond(x + 1), sp.e, sbr};
           double* rawk[2] = {raw0, raw1};
           for (int k = 0; k < 2; ++k) {
-            TRef Yr = dense_ref(Yk, ys, "bces", {sbl, Nc.bond(x + 1), sp.e, sp.s});
+            TRef Yr = folded_ref(Yk, ys, "bces", {sbl, Nc.bond(x + 1), sp.e, sp.s});
             contract<T>(ctx, Yr, Xr, dense_ref(c.cand[k][site], 0, "wnes", {sp.w, sp.n, sp.e, sp.s}), cur);
             TRef Or = dense_ref(O[k], os, "ced", Ol);
             contract<T>(ctx, Or, Yr, bottom_ref(sb, Lx, x, "bsd"), cur);

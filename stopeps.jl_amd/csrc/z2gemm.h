@@ -29,6 +29,8 @@ struct OperandGroup {
   int linear;                    // 1: offset = idx * lstride; 0: sum of leg index * leg stride
   long long lstride;
   long long stride[kMaxLegs];
+  int fold_j;                    // leg of this group stored folded (its index within its parity half), or -1
+  int fold_even;                 // even count of that leg
 };
 
 struct Operand {
@@ -161,7 +163,11 @@ __device__ inline long long operand_offset(const OperandGroup& og, const GroupVi
                                            const int* legs) {
   if (og.linear) return idx * og.lstride;
   long long off = 0;
-  for (int j = 0; j < g.n; ++j) off += (long long)legs[j] * og.stride[j];
+  for (int j = 0; j < g.n; ++j) {
+    int l = legs[j];
+    if (j == og.fold_j && l >= og.fold_even) l -= og.fold_even;
+    off += (long long)l * og.stride[j];
+  }
   return off;
 }
 
