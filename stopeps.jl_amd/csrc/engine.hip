@@ -404,6 +404,127 @@ __device__ inline unsigned long long mix64(unsigned long long v) {
   return v ^ (v >> 31);
 }
 
+// One pass of (shifted) Cholesky QR on the Gram matrices S = Z^T Z of a batch
+// of n x n problems (n <= 64): R = D * inv(L)^T where D scales the columns of
+// Z to unit norm (scale != 0; identity otherwise) and L L^T = D S D + shift I.
+// Then Z R has orthonormal columns to the accuracy the conditioning allows.
+// A problem that is inactive or already marked bad is skipped; a dead column
+// (norm below 1e-150 of the largest, or not finite) or a non-positive pivot
+// marks it bad (bad[p] = 1, ok[p] = 0) for a Householder fallback.
+constexpr int CQ = 64;
+__global__ __launch_bounds__(CQ) void cholqr_kernel(const double* S, long long sstride, int lds, double* R,
+                                                     long long rstride, int ldr, int n, int scale, double shift,
+                                                     const int* active, int* ok, int* bad) {
+  const int prob = blockIdx.x;
+  const int i = threadIdx.x;
+  if (!active[prob] || bad[prob]) {
+    if (i == 0) ok[prob] = 0;
+    return;
+  }
+  __shared__ double A[CQ * (CQ + 1)];
+  __shared__ double X[CQ * (CQ + 1)];
+  __shared__ double d[CQ];
+  __shared__ double smax;
+  __shared__ int fail;
+  const double* sp = S + (long long)prob * sstride;
+  if (i == 0) {
+    fail = 0;
+    double m = 0.0;
+    for (int k = 0; k < n; ++k) m = fmax(m, sp[(long long)k * lds + k]);
+    smax = m;
+  }
+  __syncthreads();
+  if (i < n) {
+    const double sii = sp[(long long)i * lds + i];
+    double di = 1.0;
+    if (scale) {
+      if (!(sii > 1e-150 * smax) || !isfinite(sii)) {
+        fail = 1;
+        di = 0.0;
+      } else {
+        di = 1.0 / sqrt(sii);
+      }
+    }
+    d[i] = di;
+  }
+  __syncthreads();
+  if (i < n)
+    for (int j = 0; j < n; ++j) A[i * (CQ + 1) + j] = d[i] * d[j] * sp[(long long)i * lds + j] + (i == j ? shift : 0.0);
+  __syncthreads();
+  // right-looking Cholesky, thread i owns row i of the lower triangle
+  for (int k = 0; k < n; ++k) {
+    if (i == k) {
+      const double p = A[k * (CQ + 1) + k];
+      if (!(p > 0.0) || !isfinite(p)) fail = 1;
+      A[k * (CQ + 1) + k] = p > 0.0 ? sqrt(p) : 1.0;
+    }
+    __syncthreads();
+    if (i > k && i < n) A[i * (CQ + 1) + k] /= A[k * (CQ + 1) + k];
+    __syncthreads();
+    if (i > k && i < n) {
+      const double lik = A[i * (CQ + 1) + k];
+      for (int m = k + 1; m <= i; ++m) A[i * (CQ + 1) + m] -= lik * A[m * (CQ + 1) + k];
+    }
+    __syncthreads();
+  }
+  // X = inv(L), column i by thread i (forward substitution)
+  if (i < n) {
+    for (int r = 0; r < n; ++r) {
+      double v = (r == i) ? 1.0 : 0.0;
+      if (r >= i) {
+        for (int m = i; m < r; ++m) v -= A[r * (CQ + 1) + m] * X[m * (CQ + 1) + i];
+        v /= A[r * (CQ + 1) + r];
+      } else {
+        v = 0.0;
+      }
+      X[r * (CQ + 1) + i] = v;
+    }
+  }
+  __syncthreads();
+  double* rp = R + (long long)prob * rstride;
+  if (i < n)
+    for (int b = 0; b < n; ++b) rp[(long long)i * ldr + b] = (b >= i) ? d[i] * X[b * (CQ + 1) + i] : 0.0;
+  if (i == 0) {
+    ok[prob] = fail ? 0 : 1;
+    if (fail) bad[prob] = 1;
+  }
+}
+
+// Second orthonormalization pass for a nearly orthonormal block Q (Gram
+// S = Q^T Q = I + E): R = (3 I - S) / 2, so that Q R has Gram I + O(E^2).
+// A problem with max |E| above 1e-7 (or nonfinite) is marked bad.
+__global__ void lowdin_kernel(const double* S, long long sstride, int lds, double* R, long long rstride, int ldr,
+                              int n, const int* active, int* ok, int* bad) {
+  const int prob = blockIdx.x;
+  if (!active[prob] || bad[prob]) {
+    if (threadIdx.x == 0) ok[prob] = 0;
+    return;
+  }
+  __shared__ int fail;
+  if (threadIdx.x == 0) fail = 0;
+  __syncthreads();
+  const double* sp = S + (long long)prob * sstride;
+  double* rp = R + (long long)prob * rstride;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e - (e / n) * n;
+    const double sij = sp[(long long)i * lds + j];
+    const double eij = sij - (i == j ? 1.0 : 0.0);
+    if (!(fabs(eij) <= 1e-7)) fail = 1;
+    rp[(long long)i * ldr + j] = (i == j ? 1.0 : 0.0) - 0.5 * eij;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ok[prob] = fail ? 0 : 1;
+    if (fail) bad[prob] = 1;
+  }
+}
+
+// out[p] = a[p] && b[p]
+__global__ void and_mask_kernel(const int* a, const int* b, int* out, int n) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) out[p] = (a[p] && b[p]) ? 1 : 0;
+}
+
 // Deterministic pseudo-random start block (rows >= n zero).
 __global__ void random_block_kernel(double* q, long long pstride, int ld, int ncol, const int* n_of, int maxrows) {
   const int prob = blockIdx.y;
@@ -918,6 +1039,48 @@ static void eig_topk(Context& ctx, const double* rho, long long rho_pstride, int
     }
     SPP_CHECK(hipGetLastError());
   };
+  // Q <- orth(Z): a column-scaled, shifted Cholesky QR pass followed by a
+  // first-order Lowdin pass (Gram and apply on the matrix cores), Householder
+  // for the problems they mark bad; Z is kept, tmp is overwritten. Used once
+  // the block is nearly orthogonal up to column scaling (Z = rho Q with Q the
+  // previous Ritz vectors).
+  int* okm = (int*)ctx.arena.take(sizeof(int) * nprob);
+  int* badm = (int*)ctx.arena.take(sizeof(int) * nprob);
+  double* Rm = (double*)ctx.arena.take(sizeof(double) * nprob * ss);
+  const bool cholqr = kk <= CQ;
+  std::vector<int> hbad(nprob);
+  auto orth_chol = [&](double* Q, double* Z, double* tmp) {
+    SPP_CHECK(launch_fill<int>(badm, nprob, 0, st));
+    const double shift = 11.0 * ((double)nmax * kk + (double)kk * (kk + 1)) * 1.1102230246251565e-16;
+    // pass 1: column-scaled, shifted Cholesky QR of Z into tmp
+    contract<double>(ctx, smallref(Sm, "ab"), compact_ref(Z, 2 * bs, bs, kk, "kpa", 2, {lk, lp, fl}),
+                     blockref(Z, "kpb"), nb, 1.0, false, active);
+    hipLaunchKernelGGL(cholqr_kernel, dim3(nprob), dim3(CQ), 0, st, Sm, ss, kk, Rm, ss, kk, kk, 1, shift, active,
+                       okm, badm);
+    SPP_CHECK(hipGetLastError());
+    contract<double>(ctx, blockref(tmp, "kpb"), blockref(Z, "kpa"), smallref(Rm, "ab"), nb, 1.0, false, okm);
+    // pass 2: first-order Lowdin correction of the nearly orthonormal tmp into Q
+    contract<double>(ctx, smallref(Sm, "ab"), compact_ref(tmp, 2 * bs, bs, kk, "kpa", 2, {lk, lp, fl}),
+                     blockref(tmp, "kpb"), nb, 1.0, false, okm);
+    hipLaunchKernelGGL(lowdin_kernel, dim3(nprob), dim3(256), 0, st, Sm, ss, kk, Rm, ss, kk, kk, active, okm, badm);
+    SPP_CHECK(hipGetLastError());
+    contract<double>(ctx, blockref(Q, "kpb"), blockref(tmp, "kpa"), smallref(Rm, "ab"), nb, 1.0, false, okm);
+    // Householder for the problems marked bad (rare: dead columns, failed pivots)
+    SPP_CHECK(hipMemcpyAsync(hbad.data(), badm, sizeof(int) * nprob, hipMemcpyDeviceToHost, st));
+    SPP_CHECK(hipStreamSynchronize(st));
+    bool any = false;
+    for (int p = 0; p < nprob; ++p) any = any || hbad[p];
+    if (!any) return;
+    hipLaunchKernelGGL(and_mask_kernel, dim3((nprob + 255) / 256), dim3(256), 0, st, active, badm, okm, nprob);
+    SPP_CHECK(hipGetLastError());
+    if (kk <= 64 && nmax <= 8 * 72) {
+      hipLaunchKernelGGL((householder_q_reg_kernel<8, 72>), dim3(nprob), dim3(512), 0, st, Z, Q, bs, kk, kk, n_of,
+                         okm);
+    } else {
+      hipLaunchKernelGGL(householder_q_kernel, dim3(nprob), dim3(256), 0, st, Z, Q, bs, kk, kk, n_of, tau, okm);
+    }
+    SPP_CHECK(hipGetLastError());
+  };
   // start block
   hipLaunchKernelGGL(random_block_kernel, dim3(64, nprob), dim3(256), 0, st, Qb, bs, kk, kk, n_of, nmax);
   SPP_CHECK(hipGetLastError());
@@ -942,7 +1105,8 @@ static void eig_topk(Context& ctx, const double* rho, long long rho_pstride, int
     bool all = true;
     for (int p = 0; p < nprob; ++p) all = all && hconv[p];
     if (all) break;
-    orth(Qa, Zb, Za);
+    if (cholqr) orth_chol(Qa, Zb, Za);
+    else orth(Qa, Zb, Za);
   }
   ctx.last_iterations = it + 1;
   res.w = sig;
