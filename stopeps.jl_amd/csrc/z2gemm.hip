@@ -192,8 +192,9 @@ __device__ inline void tile_order(int& bn, int& bm, int& prob) {
   prob = 2 * (int)(t / gy) + q;
 }
 
-// MINW: waves per SIMD the register allocation must allow (occupancy target).
-template <typename T, int BM, int BN, int WGM, int BK, int MINW, int AMF, int BKF>
+// MINW: waves per SIMD the register allocation must allow (occupancy target);
+// DB: double-buffered LDS tiles.
+template <typename T, int BM, int BN, int WGM, int BK, int MINW, int DB, int AMF, int BKF>
 __global__ __launch_bounds__(kThreads, MINW) void z2gemm_kernel(GemmArgs args) {
   constexpr int MT = KTile<T>::MT;
   constexpr int WGN = 4 / WGM;
@@ -203,8 +204,9 @@ __global__ __launch_bounds__(kThreads, MINW) void z2gemm_kernel(GemmArgs args) {
   constexpr int LDA = BM + PAD, LDB = BN + PAD;
   using MA = TileMap<BM, BK, AMF>;
   using MB = TileMap<BN, BK, 1 - BKF>;
-  __shared__ T As[BK * LDA];
-  __shared__ T Bs[BK * LDB];
+  constexpr int NBUF = DB ? 2 : 1;  // LDS tile buffers
+  __shared__ T As[NBUF][BK * LDA];
+  __shared__ T Bs[NBUF][BK * LDB];
   // byte offsets of A rows / B columns (invalid: 0, any readable element) and
   // element offsets of C rows / columns (invalid: -1)
   __shared__ unsigned offAm[BM], offBn[BN];
@@ -301,15 +303,32 @@ __global__ __launch_bounds__(kThreads, MINW) void z2gemm_kernel(GemmArgs args) {
     }
     __syncthreads();
     load_tile(0);
-    for (int t = 0; t < nt; ++t) {
+    auto store_tile = [&](int b) {
 #pragma unroll
-      for (int j = 0; j < MA::N; ++j) As[MA::k(tid, j) * LDA + MA::row(tid, j)] = ra[j];
+      for (int j = 0; j < MA::N; ++j) As[b][MA::k(tid, j) * LDA + MA::row(tid, j)] = ra[j];
 #pragma unroll
-      for (int j = 0; j < MB::N; ++j) Bs[MB::k(tid, j) * LDB + MB::row(tid, j)] = rb[j];
+      for (int j = 0; j < MB::N; ++j) Bs[b][MB::k(tid, j) * LDB + MB::row(tid, j)] = rb[j];
+    };
+    if constexpr (DB) {
+      // two LDS buffers: tile t + 1 is stored while tile t is multiplied, one
+      // barrier per tile (buffer (t + 1) & 1 was last read before the
+      // previous barrier)
+      store_tile(0);
       __syncthreads();
-      if (t + 1 < nt) load_tile((t + 1) * BK);
-      acc.template step<LDA, LDB>(As, Bs, wm, wn, lane);
-      __syncthreads();
+      for (int t = 0; t < nt; ++t) {
+        if (t + 1 < nt) load_tile((t + 1) * BK);
+        acc.template step<LDA, LDB>(As[t & 1], Bs[t & 1], wm, wn, lane);
+        if (t + 1 < nt) store_tile((t + 1) & 1);
+        __syncthreads();
+      }
+    } else {
+      for (int t = 0; t < nt; ++t) {
+        store_tile(0);
+        __syncthreads();
+        if (t + 1 < nt) load_tile((t + 1) * BK);
+        acc.template step<LDA, LDB>(As[0], Bs[0], wm, wn, lane);
+        __syncthreads();
+      }
     }
   }
   // Orders the C offset tables before the epilogue when cK == 0 (no K loop).
@@ -335,23 +354,24 @@ __global__ __launch_bounds__(kThreads, MINW) void z2gemm_kernel(GemmArgs args) {
       }
 }
 
-// Block tile shapes per element type: (BM, BN, wave rows, K tile, waves per SIMD).
+// Block tile shapes per element type: (BM, BN, wave rows, K tile, waves per
+// SIMD, double-buffered LDS).
 template <typename T>
 struct Shapes;
 template <>
 struct Shapes<float> {
-  static constexpr int M0 = 128, N0 = 128, W0 = 2, K0 = 16, O0 = 3;  // wave tile 64 x 64
-  static constexpr int M1 = 256, N1 = 96, W1 = 4, K1 = 16, O1 = 3;   // wave tile 64 x 96
-  static constexpr int M2 = 256, N2 = 32, W2 = 4, K2 = 32, O2 = 2;   // wave tile 64 x 32
+  static constexpr int M0 = 128, N0 = 128, W0 = 2, K0 = 16, O0 = 3, D0 = 0;  // wave tile 64 x 64
+  static constexpr int M1 = 256, N1 = 96, W1 = 4, K1 = 8, O1 = 3, D1 = 0;    // wave tile 64 x 96
+  static constexpr int M2 = 256, N2 = 32, W2 = 4, K2 = 32, O2 = 2, D2 = 0;   // wave tile 64 x 32
 };
 template <>
 struct Shapes<double> {
-  static constexpr int M0 = 128, N0 = 128, W0 = 2, K0 = 16, O0 = 1;  // wave tile 64 x 64
-  static constexpr int M1 = 128, N1 = 96, W1 = 2, K1 = 8, O1 = 2;    // wave tile 64 x 48
-  static constexpr int M2 = 256, N2 = 32, W2 = 4, K2 = 16, O2 = 2;   // wave tile 64 x 32
+  static constexpr int M0 = 128, N0 = 128, W0 = 2, K0 = 16, O0 = 1, D0 = 0;  // wave tile 64 x 64
+  static constexpr int M1 = 128, N1 = 96, W1 = 2, K1 = 8, O1 = 2, D1 = 0;    // wave tile 64 x 48
+  static constexpr int M2 = 256, N2 = 32, W2 = 4, K2 = 16, O2 = 2, D2 = 0;   // wave tile 64 x 32
 };
 
-template <typename T, int BM, int BN, int WGM, int BK, int MINW>
+template <typename T, int BM, int BN, int WGM, int BK, int MINW, int DB>
 static hipError_t launch_shape(const GemmArgs& args, long long mmax, long long nmax, hipStream_t stream) {
   const long long gy = (mmax + BM - 1) / BM;
   const long long gx = (nmax + BN - 1) / BN;
@@ -361,13 +381,13 @@ static hipError_t launch_shape(const GemmArgs& args, long long mmax, long long n
   dim3 grid((unsigned)gx, (unsigned)gy, (unsigned)gz);
   const int mode = (args.a_mfast ? 1 : 0) + (args.b_kfast ? 2 : 0);
   if (mode == 0)
-    hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, BK, MINW, 0, 0>), grid, dim3(kThreads), 0, stream, args);
+    hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, BK, MINW, DB, 0, 0>), grid, dim3(kThreads), 0, stream, args);
   else if (mode == 1)
-    hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, BK, MINW, 1, 0>), grid, dim3(kThreads), 0, stream, args);
+    hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, BK, MINW, DB, 1, 0>), grid, dim3(kThreads), 0, stream, args);
   else if (mode == 2)
-    hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, BK, MINW, 0, 1>), grid, dim3(kThreads), 0, stream, args);
+    hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, BK, MINW, DB, 0, 1>), grid, dim3(kThreads), 0, stream, args);
   else
-    hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, BK, MINW, 1, 1>), grid, dim3(kThreads), 0, stream, args);
+    hipLaunchKernelGGL((z2gemm_kernel<T, BM, BN, WGM, BK, MINW, DB, 1, 1>), grid, dim3(kThreads), 0, stream, args);
   return hipGetLastError();
 }
 
@@ -375,13 +395,13 @@ template <typename T>
 hipError_t launch_z2gemm(const GemmArgs& args, long long mmax, long long nmax, hipStream_t stream) {
   using S = Shapes<T>;
   if (mmax <= 0 || nmax <= 0 || args.nsamples <= 0) return hipSuccess;
-  if (nmax <= S::N2) return launch_shape<T, S::M2, S::N2, S::W2, S::K2, S::O2>(args, mmax, nmax, stream);
+  if (nmax <= S::N2) return launch_shape<T, S::M2, S::N2, S::W2, S::K2, S::O2, S::D2>(args, mmax, nmax, stream);
   auto padded = [&](long long bm, long long bn) {
     return ((mmax + bm - 1) / bm) * bm * (((nmax + bn - 1) / bn) * bn);
   };
   if (padded(S::M1, S::N1) < padded(S::M0, S::N0))
-    return launch_shape<T, S::M1, S::N1, S::W1, S::K1, S::O1>(args, mmax, nmax, stream);
-  return launch_shape<T, S::M0, S::N0, S::W0, S::K0, S::O0>(args, mmax, nmax, stream);
+    return launch_shape<T, S::M1, S::N1, S::W1, S::K1, S::O1, S::D1>(args, mmax, nmax, stream);
+  return launch_shape<T, S::M0, S::N0, S::W0, S::K0, S::O0, S::D0>(args, mmax, nmax, stream);
 }
 
 template hipError_t launch_z2gemm<float>(const GemmArgs&, long long, long long, hipStream_t);
