@@ -79,6 +79,21 @@ def cpu_baseline(L=8, D=(3, 3), chi=32, seed=5, threads=None):
                        f"per draw; OpenBLAS {threads} threads")}
 
 
+def measured_traffic(precision, batch, L, chi):
+    """HBM bytes per launch of the dominant kernel for this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent or for
+    another workload)."""
+    path = os.path.join(ROOT, "profiles", f"z2gemm_{precision}_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    if (t.get("precision"), t.get("batch"), t.get("L"), t.get("chi")) != (precision, batch, L, chi):
+        return None, None
+    return t["traffic_bytes_per_launch"], t.get("source")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +155,7 @@ def main():
     ctx.set_option("profile", 0)
     value = B * args.steps / elapsed
     log(f"timed {args.steps} x {B} draws in {elapsed:.2f} s -> {value:.3f} samples/s")
+    traffic, traffic_src = measured_traffic(args.precision, B, L, args.chi)
     kc = cnt[args.precision]  # launches of the contraction kernel of this precision
     gemm_tf = kc["flops"] / (kc["ms"] * 1e-3) / 1e12 if kc["ms"] > 0 else 0.0
     avg_ms = kc["ms"] / max(kc["launches"], 1)
@@ -153,7 +169,9 @@ def main():
                    "input": f"synthetic PEPO: identity embedded in D=6 bonds + Gaussian perturbation eps={args.eps}, seed 1"},
         "roofline": {"bound": "mfma", "kernel": f"z2gemm_kernel<{'float' if args.precision == 'fp32' else 'double'}>",
                      "achieved": gemm_tf, "peak": PEAK[args.precision], "unit": "TFLOP/s",
-                     "frac": gemm_tf / PEAK[args.precision], "traffic": None,
+                     "frac": gemm_tf / PEAK[args.precision], "traffic": traffic,
+                     "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
+                     "traffic_source": traffic_src,
                      "avg_launch_ms": avg_ms, "launches": kc["launches"],
                      "flops_per_launch": kc["flops"] / max(kc["launches"], 1),
                      "kernel_share_of_step": kc["ms"] / (1e3 * elapsed),

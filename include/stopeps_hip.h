@@ -127,6 +127,17 @@ int spp_measure_energy_batch(spp_ctx* ctx, const spp_cache* cache, const spp_sta
                              double J, double h, double* energy_total, double* energy_per_site, double* log_norm,
                              double* tensor_log_norm, double* max_trunc, int* status, int* fail_site);
 
+/* Batched `lognorm(state, BoundaryMPS(chi))` (src/contraction/boundary_mps.jl:162-191)
+ * of n uploaded states (PEPS or PEPO, all on the same lattice with the same
+ * leg spaces), e.g. the particles of `_observe` with energy = false
+ * (src/driver/particles.jl:35-40). Contractions in the context's precision.
+ *  log_norm [n]         log <psi|psi> + 2 log_amplitude;
+ *  tensor_log_norm [n]  log <psi|psi> of the stored tensors;
+ *  max_trunc [n]        max_environment_truncation of the north absorbs;
+ *  status [n]           0 ok, 6 norm not finite and positive (boundary_mps.jl:188). */
+int spp_lognorm_batch(spp_ctx* ctx, int n, const spp_state* const* states, int chi, double* log_norm,
+                      double* tensor_log_norm, double* max_trunc, int* status);
+
 /* Self-test of the matrix-core contraction kernel (no reference counterpart):
  * C[M][N] = A[M][K] B[K][N] (row-major, FP64 in/out) with each index a Z2
  * leg whose first evenX entries are even. Only parity-matching (neutral)

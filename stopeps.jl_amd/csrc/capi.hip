@@ -2,6 +2,7 @@
 // reports them through spp_last_error.
 #include "../../include/stopeps_hip.h"
 #include "energy.h"
+#include "lognorm.h"
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -256,6 +257,27 @@ int spp_measure_energy_batch(spp_ctx* ctx, const spp_cache* cache, const spp_sta
     o.fail_site = fail_site;
     if (cache->precision == 1) measure_energy_batch<float>(ctx->c, pepo->s, *cache->f, chi, n, labels, ops, o);
     else measure_energy_batch<double>(ctx->c, pepo->s, *cache->d, chi, n, labels, ops, o);
+  });
+}
+
+int spp_lognorm_batch(spp_ctx* ctx, int n, const spp_state* const* states, int chi, double* log_norm,
+                      double* tensor_log_norm, double* max_trunc, int* status) {
+  return guarded(ctx, [&] {
+    if (n < 0) throw HipError("negative state count");
+    if (n > 0 && !states) throw HipError("null state list");
+    SPP_CHECK(hipSetDevice(ctx->c.device));
+    std::vector<const StateDev*> v(n);
+    for (int i = 0; i < n; ++i) {
+      if (!states[i]) throw HipError("null state");
+      v[i] = &states[i]->s;
+    }
+    LognormOutputs o;
+    o.log_norm = log_norm;
+    o.tensor_log_norm = tensor_log_norm;
+    o.max_trunc = max_trunc;
+    o.status = status;
+    if (ctx->c.precision == 1) lognorm_batch<float>(ctx->c, v, chi, o);
+    else lognorm_batch<double>(ctx->c, v, chi, o);
   });
 }
 
