@@ -310,3 +310,33 @@ def test_temperature_nodes_and_step_schedule():
     assert len(steps) == 15 and abs(steps[-1] - 2.0) == 0
     assert np.allclose(np.diff([0.5] + steps), 0.1)
     assert oens.step_schedule(0.0, 0.25, 0.1) == [0.2, 0.25]
+
+
+@pytest.mark.parametrize("kind,shape,D,sym", [("peps", (2, 2), (1, 1), True), ("pepo", (3, 2), (1, 1), True),
+                                              ("peps", (2, 3), 2, False)])
+def test_lognorm_gradient_matches_finite_differences(kind, shape, D, sym):
+    """Environment gradient of log<psi|psi> (exact chi) against central finite
+    differences h = 1e-6 of the exact lognorm, on neutral entries of every site."""
+    from oracle import adjoint as oa
+    from oracle.z2 import Z2, total_parity_grid
+    L = Lattice(*shape)
+    state = random_state(L, D, kind, seed=71, symmetric=sym)
+    chi = 64
+    grads, _ = oa.lognorm_gradient(state, chi)
+    rng = np.random.default_rng(3)
+    h = 1e-6
+    for (x, y) in L.site_order():
+        A = state[(x, y)]
+        allowed = np.argwhere(total_parity_grid(A) == 0)
+        for idx in allowed[rng.choice(len(allowed), size=min(3, len(allowed)), replace=False)]:
+            idx = tuple(idx)
+            vals = []
+            for sgn in (1, -1):
+                a = A.a.copy()
+                a[idx] += sgn * h
+                state.sites[x][y] = Z2(a, A.p)
+                vals.append(bm.lognorm(state, chi)["log_norm"])
+            state.sites[x][y] = A
+            fd = (vals[0] - vals[1]) / (2 * h)
+            g = grads[(x, y)].a[idx]
+            assert abs(fd - g) <= 1e-6 * max(1.0, abs(g)), ((x, y), idx, fd, g)
