@@ -138,6 +138,26 @@ int spp_measure_energy_batch(spp_ctx* ctx, const spp_cache* cache, const spp_sta
 int spp_lognorm_batch(spp_ctx* ctx, int n, const spp_state* const* states, int chi, double* log_norm,
                       double* tensor_log_norm, double* max_trunc, int* status);
 
+/* Gradient of log <psi|psi> with respect to every site tensor of n uploaded
+ * states (same lattice and leg spaces), the adjoint of `lognorm`
+ * (src/contraction/boundary_mps.jl:162-191). The reference has no adjoint; the
+ * object it replaces is the single-site norm matrix `_site_hole`
+ * (src/sampling/born.jl:75-83) built from `boundary_environments`
+ * (boundary_mps.jl:193-209) and `row_environments` (boundary_mps.jl:261-273):
+ *   grad_i = (N_i A_i + N_i^T A_i) / (A_i N_i A_i).
+ * This is the environment gradient (truncations held fixed); it equals the
+ * derivative of lognorm when chi reaches the exact boundary bond.
+ *  grads [n][sum_i size_i]  FP64, site i (row-major, x fastest) dense
+ *                           (W,N,P,A,E,S) in the parity-sorted leg order of
+ *                           the upload (even indices first on every leg);
+ *  log_norm, tensor_log_norm, max_trunc [n] as in spp_lognorm_batch, with
+ *                           max_trunc over the north and the south absorbs;
+ *  status [n]               0 ok, 3 nonfinite row environment
+ *                           (boundary_mps.jl:258), 6 norm or local norm not
+ *                           finite and nonzero (boundary_mps.jl:188). */
+int spp_lognorm_grad_batch(spp_ctx* ctx, int n, const spp_state* const* states, int chi, double* grads,
+                           double* log_norm, double* tensor_log_norm, double* max_trunc, int* status);
+
 /* Self-test of the matrix-core contraction kernel (no reference counterpart):
  * C[M][N] = A[M][K] B[K][N] (row-major, FP64 in/out) with each index a Z2
  * leg whose first evenX entries are even. Only parity-matching (neutral)

@@ -12,11 +12,12 @@ from .states import (Lattice, Site, FiniteState, FinitePEPO, FinitePEPS, identit
 from .sampling import (Context, DeviceState, SamplingCache, BornError, sample_configuration,
                        sample_configurations, sample_uniforms, project_configuration, splitmix64)
 from .measurement import TFIM, spin_operators, operator_factors, EnergyError, measure_energies, measure_energy
-from .contraction import NormError, lognorm, lognorms
+from .contraction import NormError, lognorm, lognorms, lognorm_gradient, lognorm_gradients
 from . import _lib
 
 __all__ = ["synthetic_thermal_pepo", "Lattice", "Site", "FiniteState", "FinitePEPO", "FinitePEPS", "identity_pepo", "parity_sorted",
            "LocalBasis", "SymmetryBasis", "XBasis", "resolve_basis", "Context", "DeviceState",
            "SamplingCache", "BornError", "sample_configuration", "sample_configurations", "sample_uniforms",
            "project_configuration", "splitmix64", "TFIM", "spin_operators", "operator_factors", "EnergyError",
-           "measure_energies", "measure_energy", "NormError", "lognorm", "lognorms"]
+           "measure_energies", "measure_energy", "NormError", "lognorm", "lognorms",
+           "lognorm_gradient", "lognorm_gradients"]

@@ -3,6 +3,7 @@
 #include "../../include/stopeps_hip.h"
 #include "energy.h"
 #include "lognorm.h"
+#include "adjoint.h"
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -278,6 +279,28 @@ int spp_lognorm_batch(spp_ctx* ctx, int n, const spp_state* const* states, int c
     o.status = status;
     if (ctx->c.precision == 1) lognorm_batch<float>(ctx->c, v, chi, o);
     else lognorm_batch<double>(ctx->c, v, chi, o);
+  });
+}
+
+int spp_lognorm_grad_batch(spp_ctx* ctx, int n, const spp_state* const* states, int chi, double* grads,
+                           double* log_norm, double* tensor_log_norm, double* max_trunc, int* status) {
+  return guarded(ctx, [&] {
+    if (n < 0) throw HipError("negative state count");
+    if (n > 0 && !states) throw HipError("null state list");
+    SPP_CHECK(hipSetDevice(ctx->c.device));
+    std::vector<const StateDev*> v(n);
+    for (int i = 0; i < n; ++i) {
+      if (!states[i]) throw HipError("null state");
+      v[i] = &states[i]->s;
+    }
+    GradOutputs o;
+    o.grads = grads;
+    o.log_norm = log_norm;
+    o.tensor_log_norm = tensor_log_norm;
+    o.max_trunc = max_trunc;
+    o.status = status;
+    if (ctx->c.precision == 1) lognorm_grad_batch<float>(ctx->c, v, chi, o);
+    else lognorm_grad_batch<double>(ctx->c, v, chi, o);
   });
 }
 
