@@ -7,7 +7,7 @@ package as ``stopeps_jl_amd``). Compute runs in ``libstopeps_hip.so``
 is the host-side mirror of the reference's Julia API for that path.
 """
 from .states import (Lattice, Site, FiniteState, FinitePEPO, FinitePEPS, identity_pepo, parity_sorted,
-                     synthetic_thermal_pepo,
+                     synthetic_thermal_pepo, random_peps,
                      LocalBasis, SymmetryBasis, XBasis, resolve_basis)
 from .sampling import (Context, DeviceState, SamplingCache, BornError, sample_configuration,
                        sample_configurations, sample_uniforms, project_configuration, splitmix64)
@@ -15,7 +15,7 @@ from .measurement import TFIM, spin_operators, operator_factors, EnergyError, me
 from .contraction import NormError, lognorm, lognorms, lognorm_gradient, lognorm_gradients
 from . import _lib
 
-__all__ = ["synthetic_thermal_pepo", "Lattice", "Site", "FiniteState", "FinitePEPO", "FinitePEPS", "identity_pepo", "parity_sorted",
+__all__ = ["synthetic_thermal_pepo", "random_peps", "Lattice", "Site", "FiniteState", "FinitePEPO", "FinitePEPS", "identity_pepo", "parity_sorted",
            "LocalBasis", "SymmetryBasis", "XBasis", "resolve_basis", "Context", "DeviceState",
            "SamplingCache", "BornError", "sample_configuration", "sample_configurations", "sample_uniforms",
            "project_configuration", "splitmix64", "TFIM", "spin_operators", "operator_factors", "EnergyError",

@@ -127,6 +127,38 @@ def synthetic_thermal_pepo(lattice, D=(3, 3), eps=0.3, seed=1):
     return FiniteState(lattice, sites, "pepo")
 
 
+def random_peps(lattice, D=(3, 3), seed=7, kind="peps"):
+    """Seeded random Z2 state: i.i.d. N(0,1) entries in every allowed block of
+    bond spaces Z2Space(0=>D[0], 1=>D[1]), physical Z2Space(0=>1, 1=>1), each
+    site scaled to unit Frobenius norm; ``kind`` "peps" has an auxiliary leg of
+    dimension 1, "pepo" one equal to the physical leg. The input of the
+    log-norm gradient workload (8x8 PEPS, D=6, chi=36, seed 7). Sites are drawn
+    column by column (x outer, y inner), one full-shape Gaussian array each."""
+    rng = np.random.default_rng(seed)
+    bp = np.array([0] * D[0] + [1] * D[1])
+    pp = np.array([0, 1])
+    ap = pp if kind == "pepo" else np.array([0])
+    u = np.array([0])
+    sites = []
+    for x in range(lattice.Lx):
+        col = []
+        for y in range(lattice.Ly):
+            legs = [u if x == 0 else bp, u if y == 0 else bp, pp, ap,
+                    u if x == lattice.Lx - 1 else bp, u if y == lattice.Ly - 1 else bp]
+            shape = [len(p) for p in legs]
+            a = rng.standard_normal(shape)
+            par = np.zeros(shape, dtype=np.int64)
+            for k, p in enumerate(legs):
+                sh = [1] * 6
+                sh[k] = len(p)
+                par = par + p.reshape(sh)
+            a[par % 2 != 0] = 0.0
+            a /= np.linalg.norm(a)
+            col.append(Site(a, legs))
+        sites.append(col)
+    return FiniteState(lattice, sites, kind)
+
+
 def parity_sorted(site):
     """Dense array with every leg sorted by parity (stable), and even counts."""
     a = site.data

@@ -140,3 +140,27 @@ def test_operator_factors_split_the_pair_operator(spp):
     odd_onsite = np.einsum("pu,qv->pquv", np.array([[0.0, 1.0], [1.0, 0.0]]), np.eye(2))
     with pytest.raises(ValueError):
         spp.operator_factors(odd_onsite, 1)
+
+
+@pytest.mark.parametrize("kind", ["peps", "pepo"])
+def test_random_peps_matches_oracle_generator(spp, kind):
+    """The package's seeded input generator draws the oracle's random_state."""
+    from oracle.model import Lattice as OL, random_state
+    s = spp.random_peps(spp.Lattice(3, 2), (2, 1), seed=7, kind=kind)
+    o = random_state(OL(3, 2), (2, 1), kind, seed=7)
+    for x in range(3):
+        for y in range(2):
+            np.testing.assert_allclose(s[(x, y)].data, o[(x, y)].a, rtol=1e-14, atol=1e-16)
+            for k in range(6):
+                np.testing.assert_array_equal(s[(x, y)].parity[k], o[(x, y)].p[k])
+
+
+def test_gradient_unsort_inverts_parity_sort(spp):
+    from importlib import import_module
+    contraction = import_module(spp.__name__ + ".contraction")
+    rng = np.random.default_rng(0)
+    par = [rng.integers(0, 2, size=n) for n in (3, 4, 2, 1, 5, 2)]
+    a = rng.standard_normal([len(p) for p in par])
+    site = spp.Site(a, par)
+    srt, _ = spp.parity_sorted(site)
+    np.testing.assert_array_equal(contraction._unsort(srt, site.parity), a)
