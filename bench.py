@@ -94,6 +94,39 @@ def measured_traffic(precision, batch, L, chi):
     return t["traffic_bytes_per_launch"], t.get("source")
 
 
+def adjoint_timing(spp, reps=2, batch=8, chi=36):
+    """The log-norm gradient workload: 8x8 Z2 PEPS, D=6 (3+3), d=2, i.i.d.
+    N(0,1) blocks scaled to unit norm (seed 7), chi=36, FP64 contractions: all
+    64 site gradients of one state (and, as a throughput figure, of a batch of
+    states with seeds 7, 8, ...)."""
+    lat = spp.Lattice(8, 8)
+    states = [spp.random_peps(lat, (3, 3), seed=7 + k) for k in range(batch)]
+    ctx = spp.Context(0, "fp64")
+    dev = [spp.DeviceState(ctx, s) for s in states]
+    spp.lognorm_gradients(ctx, dev[:1], chi)  # warm-up
+    ctx.synchronize()
+    ctx.reset_counters()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g, info = spp.lognorm_gradients(ctx, dev[:1], chi)
+    ctx.synchronize()
+    t1 = (time.perf_counter() - t0) / reps
+    cnt = ctx.counters()
+    # log<psi|psi> is homogeneous of degree 2 in each site: <A_i, grad_i> = 2
+    euler = max(abs(float(np.sum(states[0][xy].data * g[0][xy])) - 2.0) for xy in lat.site_order())
+    t0 = time.perf_counter()
+    spp.lognorm_gradients(ctx, dev, chi)
+    ctx.synchronize()
+    tb = time.perf_counter() - t0
+    return {"value": 1.0 / t1, "unit": "gradients/s (64 site gradients of one state)", "dtype": "fp64",
+            "workload": f"gradient of log<psi|psi>, 8x8 Z2 PEPS D=6 (3+3), d=2, seed 7, chi={chi}",
+            "s_per_gradient": t1, "log_norm": float(info["log_norm"][0]),
+            "max_environment_truncation": float(info["max_environment_truncation"][0]),
+            "max_euler_residual": euler,
+            "eigensolver_iterations_per_solve": cnt["eig_iterations"] / max(cnt["eig_calls"], 1),
+            "batch": batch, "batch_s_per_state": tb / batch}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +144,8 @@ def main():
     ap.add_argument("--h", type=float, default=2.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-only", action="store_true", help="timed loop only (for rocprofv3)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="time only the log-norm gradient workload (for rocprofv3)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed loop, copy 256 MiB device to device (HBM counter calibration)")
     args = ap.parse_args()
@@ -121,6 +156,9 @@ def main():
 
     import __graft_entry__ as ge
     spp = ge.load_package()
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_timing(spp)}), flush=True)
+        return
     L = args.L
     lat = spp.Lattice(L, L)
     pepo = spp.synthetic_thermal_pepo(lat, (3, 3), args.eps, seed=1)
@@ -227,6 +265,7 @@ def main():
         "model": f"TFIM J={args.J} h={args.h} (Z2)"}
     result["energy_measurement"] = {"value": k / t_energy, "unit": "energy measurements/s",
                                     "dtype": args.precision, "samples": int(k), "chi": args.chi}
+    result["adjoint"] = adjoint_timing(spp)
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_baseline(L=L, chi=args.chi)

@@ -51,7 +51,9 @@ int spp_synchronize(spp_ctx* ctx);
 int spp_set_precision(spp_ctx* ctx, int precision);
 
 /* Tuning: "ksub" (subspace size of the truncation eigensolver, <= 64),
- * "max_subspace_iter", "subspace_tol", "chunk" (samples per device batch,
+ * "max_subspace_iter", "subspace_tol", "cheb_degree" (largest Chebyshev
+ * filter degree of a subspace step, 1 = plain power steps; default 8),
+ * "chunk" (samples per device batch,
  * 0 = sized from free memory), "profile" (1: time each contraction launch
  * with HIP events on the library stream). */
 int spp_set_option(spp_ctx* ctx, const char* name, double value);
@@ -62,7 +64,9 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value);
  * contraction kernel milliseconds and counters[5] = the flops of the timed
  * launches (profiling mode); counters[6..8] = launches, milliseconds and flops
  * of the timed FP64-kernel launches, counters[9..11] the same for the FP32
- * kernel. Returns the count written. */
+ * kernel; counters[12..14] = iterative truncation eigensolves, their summed
+ * subspace iterations and their Chebyshev filter products. Returns the count
+ * written. */
 int spp_get_counters(spp_ctx* ctx, double* counters, int n);
 int spp_reset_counters(spp_ctx* ctx);
 

@@ -96,6 +96,9 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value) {
     } else if (n == "subspace_tol") {
       if (!(value > 0)) throw HipError("subspace_tol must be positive");
       ctx->c.subspace_tol = value;
+    } else if (n == "cheb_degree") {
+      if (value < 1 || value > 32) throw HipError("cheb_degree must lie in [1, 32]");
+      ctx->c.cheb_degree = (int)value;
     } else if (n == "profile") {
       ctx->c.flush_timer();
       ctx->c.profile = value != 0 ? 1 : 0;
@@ -112,11 +115,12 @@ int spp_get_counters(spp_ctx* ctx, double* counters, int n) {
   if (!ctx || !counters) return -3;
   if (guarded(ctx, [&] { ctx->c.flush_timer(); }) != 0) return -1;
   const Counters& k = ctx->c.counters;
-  double v[12] = {k.gemm_flops,        k.gemm_launches,    (double)ctx->c.last_iterations,
+  double v[15] = {k.gemm_flops,        k.gemm_launches,    (double)ctx->c.last_iterations,
                   (double)ctx->c.arena.peak(), k.gemm_ms, k.gemm_flops_profiled,
                   k.prec_launches[0],  k.prec_ms[0],       k.prec_flops[0],
-                  k.prec_launches[1],  k.prec_ms[1],       k.prec_flops[1]};
-  int m = n < 12 ? n : 12;
+                  k.prec_launches[1],  k.prec_ms[1],       k.prec_flops[1],
+                  k.eig_calls,         k.eig_iterations,   k.eig_filter_gemms};
+  int m = n < 15 ? n : 15;
   for (int i = 0; i < m; ++i) counters[i] = v[i];
   return m;
 }

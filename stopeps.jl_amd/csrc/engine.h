@@ -105,6 +105,8 @@ struct Counters {
   double prec_launches[2] = {0, 0};
   double prec_ms[2] = {0, 0};
   double prec_flops[2] = {0, 0};
+  // truncation eigensolver: iterative calls, subspace iterations, filter products
+  double eig_calls = 0, eig_iterations = 0, eig_filter_gemms = 0;
 };
 
 // Event pairs recorded around contraction launches when profiling is on.
@@ -121,6 +123,7 @@ struct Context {
   int ksub = 64;            // subspace size for truncation eigenproblems
   int max_subspace_iter = 40;
   double subspace_tol = 1e-12;
+  int cheb_degree = 8;      // largest Chebyshev filter degree per subspace step (1: plain power step)
   Arena arena;
   Counters counters;
   std::string last_error;

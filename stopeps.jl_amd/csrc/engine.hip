@@ -817,9 +817,15 @@ __global__ __launch_bounds__(64 * NSLAB) void householder_q_reg_kernel(double* z
 // Residuals of Ritz pairs: conv[prob] = max_j<t ||Z_j - theta_j Q_j|| <= tol * |theta_0|.
 // Residual test of the Ritz pairs; a problem that converges leaves `active`
 // and keeps its Ritz pairs (later iterations skip it).
+// log T_k(x) of the Chebyshev polynomial of the first kind, x >= 1
+__device__ inline double log_cheb(int k, double x) {
+  const double a = k * acosh(x);
+  return a + log1p(exp(-2.0 * a)) - 0.69314718055994531;
+}
+
 __global__ void residual_kernel(const double* q, const double* z, long long pstride, int ld, const double* th,
                                 long long tstride, const int* n_of, int t, double tol, int* conv, double* worst,
-                                int* active) {
+                                int* active, int kk, int maxdeg, int* deg) {
   const int prob = blockIdx.x;
   if (!active[prob]) return;
   const int n = n_of[prob];
@@ -855,7 +861,54 @@ __global__ void residual_kernel(const double* q, const double* z, long long pstr
     conv[prob] = (rel <= tol) ? 1 : 0;
     worst[prob] = rel;
     active[prob] = (rel <= tol) ? 0 : 1;
+    // Chebyshev filter degree for the next step: damp [0, a] with a the
+    // smallest Ritz value, the largest degree m <= maxdeg with
+    // T_m(x_1) / T_m(x_t) <= 1e8 (x = 2 lambda / a - 1), so that the filtered
+    // columns stay within eight decades of one another; 1 = plain power step.
+    int m = 1;
+    if (n > kk && t > 0 && t < kk) {
+      const double a = tb[kk - 1], l1 = tb[0], lt = tb[t - 1];
+      if (a > 1e-300 && lt > a * (1.0 + 1e-3) && l1 >= lt) {
+        const double x1 = 2.0 * l1 / a - 1.0, xt = 2.0 * lt / a - 1.0;
+        for (int k = 2; k <= maxdeg; ++k) {
+          if (log_cheb(k, x1) - log_cheb(k, xt) > 18.420680743952367) break;
+          m = k;
+        }
+      }
+    }
+    deg[prob] = m;
   }
+}
+
+// One step of the Chebyshev three-term recurrence on [0, a], a = th[kk-1] of
+// each problem (c = e = a / 2):
+//   j == 0: out = deg >= 2 ? (z - c y) / e : z                  (Y_1, or the plain step)
+//   j >= 1: out = j < deg ? 2 (z - c y) / e - yprev : y         (Y_{j+1}, or carry Y_j)
+// z = rho y where needed. out may alias yprev.
+__global__ void cheb_step_kernel(const double* z, const double* y, const double* yprev, double* out,
+                                 long long pstride, long long count, const double* th, long long tstride, int kk,
+                                 const int* deg, const int* active, int j) {
+  const int prob = blockIdx.y;
+  if (!active[prob]) return;
+  const int m = deg[prob];
+  const double a = th[(long long)prob * tstride + kk - 1];
+  const double c = 0.5 * a, inv_e = 2.0 / a;
+  const double* zb = z + (long long)prob * pstride;
+  const double* yb = y + (long long)prob * pstride;
+  const double* pb = yprev ? yprev + (long long)prob * pstride : nullptr;
+  double* ob = out + (long long)prob * pstride;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (long long)gridDim.x * blockDim.x) {
+    double v;
+    if (j == 0) v = m >= 2 ? (zb[i] - c * yb[i]) * inv_e : zb[i];
+    else v = j < m ? 2.0 * (zb[i] - c * yb[i]) * inv_e - pb[i] : yb[i];
+    ob[i] = v;
+  }
+}
+
+// out[p] = active[p] && deg[p] > j
+__global__ void degree_mask_kernel(const int* active, const int* deg, int j, int* out, int n) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) out[p] = (active[p] && deg[p] > j) ? 1 : 0;
 }
 
 // rho <- (rho + rho^T) / 2 on each compact n x n block (rho built in FP32 is
@@ -1087,7 +1140,9 @@ static void eig_topk(Context& ctx, const double* rho, long long rho_pstride, int
   matmul_rho(Za, Qb);
   orth(Qa, Za, Zb);
   const int t = std::min(want, kk);
-  std::vector<int> hconv(nprob);
+  std::vector<int> hconv(nprob), hdeg(nprob);
+  int* degm = (int*)ctx.arena.take(sizeof(int) * nprob);
+  int* stepm = (int*)ctx.arena.take(sizeof(int) * nprob);
   int it = 0;
   for (it = 0; it < ctx.max_subspace_iter; ++it) {
     matmul_rho(Za, Qa);
@@ -1098,17 +1153,49 @@ static void eig_topk(Context& ctx, const double* rho, long long rho_pstride, int
     rmul(Qb, Qa, Vm);
     rmul(Zb, Za, Vm);
     hipLaunchKernelGGL(residual_kernel, dim3(nprob), dim3(256), 0, st, Qb, Zb, bs, kk, sig, (long long)kk, n_of,
-                       t, ctx.subspace_tol, conv, worst, active);
+                       t, ctx.subspace_tol, conv, worst, active, kk, cholqr ? ctx.cheb_degree : 1, degm);
     SPP_CHECK(hipGetLastError());
     SPP_CHECK(hipMemcpyAsync(hconv.data(), conv, sizeof(int) * nprob, hipMemcpyDeviceToHost, st));
+    SPP_CHECK(hipMemcpyAsync(hdeg.data(), degm, sizeof(int) * nprob, hipMemcpyDeviceToHost, st));
     SPP_CHECK(hipStreamSynchronize(st));
     bool all = true;
-    for (int p = 0; p < nprob; ++p) all = all && hconv[p];
+    int mdeg = 1;
+    for (int p = 0; p < nprob; ++p) {
+      all = all && hconv[p];
+      if (!hconv[p]) mdeg = std::max(mdeg, hdeg[p]);
+    }
     if (all) break;
-    if (cholqr) orth_chol(Qa, Zb, Za);
-    else orth(Qa, Zb, Za);
+    if (!cholqr) {
+      orth(Qa, Zb, Za);
+      continue;
+    }
+    if (mdeg <= 1) {
+      orth_chol(Qa, Zb, Za);
+      continue;
+    }
+    // Chebyshev-filtered step: Y_0 = Ritz vectors (Qb), rho Y_0 = Zb
+    const long long cnt = bs;
+    const dim3 cg((unsigned)std::min<long long>((cnt + 255) / 256, 64), (unsigned)nprob);
+    hipLaunchKernelGGL(cheb_step_kernel, cg, dim3(256), 0, st, Zb, Qb, (const double*)nullptr, Za, bs, cnt, sig,
+                       (long long)kk, kk, degm, active, 0);
+    SPP_CHECK(hipGetLastError());
+    double *yprev = Qb, *ycur = Za, *zfree = Zb;
+    for (int j = 1; j < mdeg; ++j) {
+      hipLaunchKernelGGL(degree_mask_kernel, dim3((nprob + 255) / 256), dim3(256), 0, st, active, degm, j, stepm,
+                         nprob);
+      SPP_CHECK(hipGetLastError());
+      contract<double>(ctx, blockref(zfree, "kpc"), rhoR, blockref(ycur, "KPc"), nb, 1.0, false, stepm);
+      ctx.counters.eig_filter_gemms += 1;
+      hipLaunchKernelGGL(cheb_step_kernel, cg, dim3(256), 0, st, zfree, ycur, yprev, yprev, bs, cnt, sig,
+                         (long long)kk, kk, degm, active, j);
+      SPP_CHECK(hipGetLastError());
+      std::swap(yprev, ycur);
+    }
+    orth_chol(Qa, ycur, yprev);
   }
   ctx.last_iterations = it + 1;
+  ctx.counters.eig_calls += 1;
+  ctx.counters.eig_iterations += it + 1;
   res.w = sig;
   res.wstride = kk;
   res.v = Qb;
