@@ -128,3 +128,21 @@ def test_gradient_empty_batch(spp, gpu_ctx):
     say("lognorm gradient empty batch")
     grads, info = spp.lognorm_gradients(gpu_ctx, [], 4)
     assert grads == [] and info["log_norm"].shape == (0,)
+
+
+def test_gradient_8x8_d6_chi36_against_oracle_fixture(spp, gpu_ctx):
+    """The gradient workload itself (8x8 Z2 PEPS, D=6 as (3,3), seed 7, chi=36,
+    ~68% discarded weight per cut): six site gradients against the oracle's
+    explicit environments, stored by tests/golden/make_adjoint_8x8.py."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "adjoint_8x8_d6_chi36.npz")
+    ref = np.load(path)
+    say("lognorm gradient 8x8 D=6 chi=36 vs oracle fixture")
+    gpu_ctx.set_precision("fp64")
+    state = spp.random_peps(spp.Lattice(8, 8), (3, 3), seed=7)
+    g, ln = spp.lognorm_gradient(gpu_ctx, state, 36)
+    assert abs(ln - float(ref["log_norm"])) <= 1e-10 * abs(float(ref["log_norm"]))
+    for x, y in ref["sites"]:
+        r = ref[f"g_{x}_{y}"]
+        err = np.max(np.abs(g[(int(x), int(y))] - r))
+        assert err <= 1e-8 * np.max(np.abs(r)), ((x, y), err)
