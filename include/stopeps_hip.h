@@ -162,6 +162,23 @@ int spp_lognorm_batch(spp_ctx* ctx, int n, const spp_state* const* states, int c
 int spp_lognorm_grad_batch(spp_ctx* ctx, int n, const spp_state* const* states, int chi, double* grads,
                            double* log_norm, double* tensor_log_norm, double* max_trunc, int* status);
 
+/* Host run of the per-bond full-update solve that the device FU kernel runs
+ * (selftest; no GPU needed): `_initialize_pair` + `_optimize_pair` +
+ * `_fu_balance_bond` (src/evolution/full_update.jl:31-117, 342-482) for one
+ * bond. dims = {na, even_a, nb, even_b, d, even_d, D, even_D} (reduced core,
+ * physical and bond dimensions, parity-sorted); options = {max_iterations,
+ * error_tolerance, diagonal_shift, linear_max_iterations, linear_tolerance};
+ * metric N[rb][sb][rk][sk]; target[r][p][q][s]; outputs the new cores
+ * left[r][p][k], right[k][q][s] in original coordinates (bond zero-padded to D;
+ * sector budget = the bond's even/odd counts) and record = {initial_cost,
+ * final_cost, residual, target_norm, iterations, limit_reached, stop (0
+ * disabled, 1 iteration limit, 2 error tolerance, 3 residual fallback),
+ * status (0 ok, 1 nonfinite metric, 2 zero metric, 3 zero target, 4 PSD repair
+ * needed, 5 nonfinite cost, 6 singular gauge), kept_even, kept_odd,
+ * floored eigenvalues, 0}. Returns 0, or -3 on bad arguments. */
+int spp_selftest_fu_bond(const int* dims, const double* options, const double* metric, const double* target,
+                         double* left, double* right, double* record);
+
 /* Self-test of the matrix-core contraction kernel (no reference counterpart):
  * C[M][N] = A[M][K] B[K][N] (row-major, FP64 in/out) with each index a Z2
  * leg whose first evenX entries are even. Only parity-matching (neutral)

@@ -42,6 +42,8 @@ _SIGNATURES = {
     "spp_selftest_gemm": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_double_p,
                                          c_double_p, c_double_p]),
+    "spp_selftest_fu_bond": (ctypes.c_int, [c_int_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                                            c_double_p, c_double_p]),
     "spp_selftest_copy": (ctypes.c_int, [c_void_p, ctypes.c_longlong]),
 }
 

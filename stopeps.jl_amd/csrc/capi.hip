@@ -4,6 +4,7 @@
 #include "energy.h"
 #include "lognorm.h"
 #include "adjoint.h"
+#include "fu.h"
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -306,6 +307,26 @@ int spp_lognorm_grad_batch(spp_ctx* ctx, int n, const spp_state* const* states, 
     if (ctx->c.precision == 1) lognorm_grad_batch<float>(ctx->c, v, chi, o);
     else lognorm_grad_batch<double>(ctx->c, v, chi, o);
   });
+}
+
+int spp_selftest_fu_bond(const int* dims, const double* options, const double* metric, const double* target,
+                         double* left, double* right, double* record) {
+  if (!dims || !options || !metric || !target || !left || !right || !record) return -3;
+  try {
+    fu::Dims g{dims[0], dims[1], dims[2], dims[3], dims[4], dims[5], dims[6], dims[7]};
+    fu::Opts o{(int)options[0], options[1], options[2], (int)options[3], options[4]};
+    if (g.na <= 0 || g.nb <= 0 || g.d <= 0 || g.D <= 0 || g.ea < 0 || g.ea > g.na || g.eb < 0 || g.eb > g.nb ||
+        g.ed < 0 || g.ed > g.d || g.eD < 0 || g.eD > g.D)
+      return -3;
+    fu::Result r = fu_bond_host(g, o, metric, target, left, right);
+    const double v[12] = {r.initial_cost, r.final_cost, r.residual, r.target_norm, (double)r.iterations,
+                          (double)r.limit_reached, (double)r.stop, (double)r.status, (double)r.kept_even,
+                          (double)r.kept_odd, (double)r.floored, 0.0};
+    for (int i = 0; i < 12; ++i) record[i] = v[i];
+    return 0;
+  } catch (...) {
+    return -1;
+  }
 }
 
 int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, int evenM, int evenN, int evenK,

@@ -136,6 +136,8 @@ def test_gradient_8x8_d6_chi36_against_oracle_fixture(spp, gpu_ctx):
     explicit environments, stored by tests/golden/make_adjoint_8x8.py."""
     import os
     path = os.path.join(os.path.dirname(__file__), "golden", "adjoint_8x8_d6_chi36.npz")
+    if not os.path.exists(path):
+        pytest.skip("fixture not generated (python tests/golden/make_adjoint_8x8.py)")
     ref = np.load(path)
     say("lognorm gradient 8x8 D=6 chi=36 vs oracle fixture")
     gpu_ctx.set_precision("fp64")
