@@ -77,6 +77,41 @@ int spp_state_upload(spp_ctx* ctx, int Lx, int Ly, int kind, const int* dims, co
                      const double* const* data, double beta, double log_amplitude, spp_state** out);
 int spp_state_free(spp_state* state);
 
+/* Live sectors of every virtual bond (no reference counterpart: the device
+ * keeps each leg at its uploaded capacity, and these counts are the bond space
+ * the reference would hold after truncation). live: [Lx*Ly][4 legs W,N,E,S][2]
+ * = (even, odd) counts in use; entries beyond them must be zero. Set to the
+ * full capacity at upload. */
+int spp_state_set_live(spp_ctx* ctx, spp_state* state, const int* live);
+
+/* Copy a state back: data[i] receives site i (row-major, x fastest) as dense
+ * (W,N,P,A,E,S) in the parity-sorted leg order of the upload; beta,
+ * log_amplitude and live (as in spp_state_set_live) may be null. */
+int spp_state_download(spp_ctx* ctx, const spp_state* state, double* const* data, double* beta,
+                       double* log_amplitude, int* live);
+
+/* `evolution_step!(state, model, delta, D, FullUpdate(...), BoundaryMPS(chi);
+ * order, sector_space)` (src/evolution/full_update.jl:563-597) for n states on
+ * the same lattice with the same leg spaces, updated in place. The bond
+ * capacity is the uploaded bond space and doubles as the sector budget
+ * (`sector_space`), D = its dimension. options = {max_iterations,
+ * error_tolerance, diagonal_shift, linear_max_iterations, linear_tolerance}
+ * (`FullUpdate`, full_update.jl:11-17). gates_h [Ly*(Lx-1)][d^4] and gates_v
+ * [(Ly-1)*Lx][d^4] are `trotter_gate(model, lattice, i, j, delta_gate)`
+ * (src/models/tfim.jl:48) as G[p][q][u][v] in the parity-sorted physical
+ * basis, bond (x,y)-(x+1,y) at y*(Lx-1)+x and (x,y)-(x,y+1) at y*Lx+x, with
+ * delta_gate = delta/2 for order 2. Outputs (may be null): records
+ * [n][bond updates in sweep order][8] = {initial_cost, final_cost, residual,
+ * iterations, iteration_limit_reached, stop (0 disabled, 1 iteration limit,
+ * 2 error tolerance, 3 residual fallback), kept_even, kept_odd};
+ * normalizations [n][Lx*Ly] (`normalize_state!` log norms, added to each
+ * state's log_amplitude); max_env_trunc [n]. The PSD repair of the bond metric
+ * (full_update.jl:177-330) is not available: a bond that needs it fails the
+ * call with the reference's message. */
+int spp_fu_evolution_step(spp_ctx* ctx, int n, spp_state* const* states, int chi, int order, double delta,
+                          const double* options, const double* gates_h, const double* gates_v, double* records,
+                          double* normalizations, double* max_env_trunc);
+
 /* `SamplingCache(state, BoundaryMPS(chi); kind=:auxiliary, basis)`
  * (src/sampling/born.jl:56-73): double layers, aux-projected candidate layers
  * and all south boundary environments. `basis` is the 2x2 row-major matrix
@@ -178,6 +213,20 @@ int spp_lognorm_grad_batch(spp_ctx* ctx, int n, const spp_state* const* states, 
  * floored eigenvalues, 0}. Returns 0, or -3 on bad arguments. */
 int spp_selftest_fu_bond(const int* dims, const double* options, const double* metric, const double* target,
                          double* left, double* right, double* record);
+
+/* Host run of the reduced-pair stage that the device FU kernel runs
+ * (selftest; no GPU needed; `_reduced_pair`, src/evolution/full_update.jl:21-29):
+ * QR of both sites of a bond over their live sectors. Sites dense
+ * (W,N,P,A,E,S), parity-sorted; live_* [4 legs W,N,E,S][2]; gate[p][q][u][v].
+ * Outputs: qa, qb pseudo-site Q tensors (the site's slots with Aux in P, a unit
+ * A slot and r in the bond leg's slot; sized from geometry), ca[r][p][k],
+ * cb[r][p][k] (k the bond), target[ra][p][q][rb], geometry = {rcap_a,
+ * even_a, rcap_b, even_b, live r counts a even, a odd, b even, b odd}. Buffers
+ * must hold the capacity sizes. Returns 0. */
+int spp_selftest_fu_pair(int horizontal, const int* dims_a, const int* evens_a, const int* dims_b,
+                         const int* evens_b, const double* site_a, const double* site_b, const int* live_a,
+                         const int* live_b, const double* gate, double* qa, double* qb, double* ca, double* cb,
+                         double* target, int* geometry);
 
 /* Self-test of the matrix-core contraction kernel (no reference counterpart):
  * C[M][N] = A[M][K] B[K][N] (row-major, FP64 in/out) with each index a Z2

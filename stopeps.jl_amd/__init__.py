@@ -7,12 +7,13 @@ package as ``stopeps_jl_amd``). Compute runs in ``libstopeps_hip.so``
 is the host-side mirror of the reference's Julia API for that path.
 """
 from .states import (Lattice, Site, FiniteState, FinitePEPO, FinitePEPS, identity_pepo, parity_sorted,
-                     synthetic_thermal_pepo, random_peps,
+                     synthetic_thermal_pepo, random_peps, pad_bonds,
                      LocalBasis, SymmetryBasis, XBasis, resolve_basis)
 from .sampling import (Context, DeviceState, SamplingCache, BornError, sample_configuration,
                        sample_configurations, sample_uniforms, project_configuration, splitmix64)
 from .measurement import TFIM, spin_operators, operator_factors, EnergyError, measure_energies, measure_energy
 from .contraction import NormError, lognorm, lognorms, lognorm_gradient, lognorm_gradients
+from .evolution import FullUpdate, trotter_gate, evolution_step, evolution_steps
 from . import _lib
 
 __all__ = ["synthetic_thermal_pepo", "random_peps", "Lattice", "Site", "FiniteState", "FinitePEPO", "FinitePEPS", "identity_pepo", "parity_sorted",
@@ -20,4 +21,5 @@ __all__ = ["synthetic_thermal_pepo", "random_peps", "Lattice", "Site", "FiniteSt
            "SamplingCache", "BornError", "sample_configuration", "sample_configurations", "sample_uniforms",
            "project_configuration", "splitmix64", "TFIM", "spin_operators", "operator_factors", "EnergyError",
            "measure_energies", "measure_energy", "NormError", "lognorm", "lognorms",
-           "lognorm_gradient", "lognorm_gradients"]
+           "lognorm_gradient", "lognorm_gradients", "pad_bonds", "FullUpdate", "trotter_gate",
+           "evolution_step", "evolution_steps"]

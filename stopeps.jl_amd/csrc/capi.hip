@@ -168,10 +168,91 @@ int spp_state_upload(spp_ctx* ctx, int Lx, int Ly, int kind, const int* dims, co
       SPP_CHECK(hipMalloc(&d.d, sizeof(double) * n));
       SPP_CHECK(hipMemcpyAsync(d.d, data[i], sizeof(double) * n, hipMemcpyHostToDevice, ctx->c.stream));
     }
+    s.live.assign((size_t)Lx * Ly * 8, 0);
+    for (int i = 0; i < Lx * Ly; ++i) {
+      const int legs[4] = {0, 1, 4, 5};
+      for (int v = 0; v < 4; ++v) {
+        s.live[(size_t)i * 8 + 2 * v] = s.sites[i].even[legs[v]];
+        s.live[(size_t)i * 8 + 2 * v + 1] = s.sites[i].dim[legs[v]] - s.sites[i].even[legs[v]];
+      }
+    }
     SPP_CHECK(hipStreamSynchronize(ctx->c.stream));
   });
   if (rc == 0) *out = st.release();
   return rc;
+}
+
+int spp_state_set_live(spp_ctx* ctx, spp_state* state, const int* live) {
+  return guarded(ctx, [&] {
+    if (!state || !live) throw HipError("null argument");
+    StateDev& s = state->s;
+    const int ns = s.Lx * s.Ly;
+    const int legs[4] = {0, 1, 4, 5};
+    for (int i = 0; i < ns; ++i)
+      for (int v = 0; v < 4; ++v) {
+        const int e = live[i * 8 + 2 * v], o = live[i * 8 + 2 * v + 1];
+        const SiteDev& d = s.sites[i];
+        if (e < 0 || o < 0 || e > d.even[legs[v]] || o > d.dim[legs[v]] - d.even[legs[v]])
+          throw HipError("live counts exceed the leg capacity");
+      }
+    for (int i = 0; i < ns; ++i) {
+      const int x = i % s.Lx, y = i / s.Lx;
+      if (x + 1 < s.Lx && (live[i * 8 + 4] != live[(i + 1) * 8 + 0] || live[i * 8 + 5] != live[(i + 1) * 8 + 1]))
+        throw HipError("live counts differ across a horizontal bond");
+      if (y + 1 < s.Ly &&
+          (live[i * 8 + 6] != live[(i + s.Lx) * 8 + 2] || live[i * 8 + 7] != live[(i + s.Lx) * 8 + 3]))
+        throw HipError("live counts differ across a vertical bond");
+    }
+    s.live.assign(live, live + (size_t)ns * 8);
+  });
+}
+
+int spp_state_download(spp_ctx* ctx, const spp_state* state, double* const* data, double* beta,
+                       double* log_amplitude, int* live) {
+  return guarded(ctx, [&] {
+    if (!state || !data) throw HipError("null argument");
+    SPP_CHECK(hipSetDevice(ctx->c.device));
+    const StateDev& s = state->s;
+    for (size_t i = 0; i < s.sites.size(); ++i) {
+      if (!data[i]) throw HipError("null site buffer");
+      SPP_CHECK(hipMemcpyAsync(data[i], s.sites[i].d, sizeof(double) * s.sites[i].size(), hipMemcpyDeviceToHost,
+                               ctx->c.stream));
+    }
+    SPP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    if (beta) *beta = s.beta;
+    if (log_amplitude) *log_amplitude = s.log_amplitude;
+    if (live) std::memcpy(live, s.live.data(), sizeof(int) * s.live.size());
+  });
+}
+
+int spp_fu_evolution_step(spp_ctx* ctx, int n, spp_state* const* states, int chi, int order, double delta,
+                          const double* options, const double* gates_h, const double* gates_v, double* records,
+                          double* normalizations, double* max_env_trunc) {
+  return guarded(ctx, [&] {
+    if (n < 0) throw HipError("negative state count");
+    if (n > 0 && !states) throw HipError("null state list");
+    if (!options) throw HipError("null options");
+    if (order != 1 && order != 2) throw HipError("order must be 1 or 2");
+    SPP_CHECK(hipSetDevice(ctx->c.device));
+    std::vector<StateDev*> v(n);
+    for (int i = 0; i < n; ++i) {
+      if (!states[i]) throw HipError("null state");
+      v[i] = &states[i]->s;
+    }
+    FuParams P;
+    P.opts = fu::Opts{(int)options[0], options[1], options[2], (int)options[3], options[4]};
+    if (P.opts.max_iterations < 0 || P.opts.linear_max_iterations < 0) throw HipError("negative iteration limit");
+    P.order = order;
+    P.delta = delta;
+    P.gates_h = gates_h;
+    P.gates_v = gates_v;
+    FuOutputs o;
+    o.records = records;
+    o.normalizations = normalizations;
+    o.max_env_trunc = max_env_trunc;
+    if (ctx->c.precision == 1) fu_evolution_step<float>(ctx->c, v, chi, P, o);
+    else fu_evolution_step<double>(ctx->c, v, chi, P, o);
+  });
 }
 
 int spp_state_free(spp_state* state) {
@@ -324,6 +405,21 @@ int spp_selftest_fu_bond(const int* dims, const double* options, const double* m
                           (double)r.kept_odd, (double)r.floored, 0.0};
     for (int i = 0; i < 12; ++i) record[i] = v[i];
     return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
+int spp_selftest_fu_pair(int horizontal, const int* dims_a, const int* evens_a, const int* dims_b,
+                         const int* evens_b, const double* site_a, const double* site_b, const int* live_a,
+                         const int* live_b, const double* gate, double* qa, double* qb, double* ca, double* cb,
+                         double* target, int* geometry) {
+  if (!dims_a || !evens_a || !dims_b || !evens_b || !site_a || !site_b || !live_a || !live_b || !gate || !qa || !qb ||
+      !ca || !cb || !target || !geometry)
+    return -3;
+  try {
+    return fu_pair_host(horizontal, dims_a, evens_a, dims_b, evens_b, site_a, site_b, live_a, live_b, gate, qa, qb,
+                        ca, cb, target, geometry);
   } catch (...) {
     return -1;
   }

@@ -1,12 +1,28 @@
 // Full-update evolution of a batch of states (src/evolution/full_update.jl):
-// the per-bond solve of fu_core.h on the device, and its host selftest.
+// `evolution_step!` with FullUpdate and BoundaryMPS(chi) on the device.
+//
+// Every sample keeps the bond spaces it was uploaded with (the capacity); the
+// sectors a sample actually uses on each bond (its "live" even/odd counts, the
+// reference's truncated bond space) are tracked per sample, and every QR,
+// SVD and ALS step works in the live dimensions only, so that padded slots
+// stay exactly zero. Per bond update (all samples of a chunk at once):
+//   fu_pair_kernel     QR of both sites into Q (exterior legs | r) and cores,
+//                      the gated target (_reduced_pair, full_update.jl:21-29)
+//   double layers      of the Q's (reduced layers, full_update.jl:142-148)
+//   contractions       bond metric N (_reduced_metric, full_update.jl:150-172)
+//   fu_core_kernel     gauge + SVD start + ALS + balance (fu_core.h)
+//   fu_restore_kernel  new sites Q * core (_restore_pair, full_update.jl:128-133)
+// then the sites' double layers and the rolling row message; rows are
+// re-absorbed into the incremental boundary (_sweep!, full_update.jl:510-561).
 #include "fu.h"
-#include "engine.h"
+#include "boundary_ops.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <vector>
 
 namespace spp {
 
-// Host execution of the per-bond solve (selftest of the code the device runs).
 fu::Result fu_bond_host(const fu::Dims& g, const fu::Opts& o, const double* N4, const double* target, double* A,
                         double* B) {
   const long long cap = fu::workspace_doubles(g);
@@ -15,5 +31,1118 @@ fu::Result fu_bond_host(const fu::Dims& g, const fu::Opts& o, const double* N4, 
   fu::View n{const_cast<double*>(N4), 1}, t{const_cast<double*>(target), 1}, a{A, 1}, b{B, 1};
   return fu::fu_bond(g, o, n, t, a, b, w);
 }
+
+namespace {
+
+// (ket i, bra j) pair index in the parity-sorted pair order of the double layers
+__host__ __device__ inline int pair_index(int i, int j, int dim, int e) {
+  const int o = dim - e;
+  const bool ie = i < e, je = j < e;
+  if (ie && je) return i * e + j;
+  if (!ie && !je) return e * e + (i - e) * o + (j - e);
+  if (ie) return e * e + o * o + i * o + (j - e);
+  return e * e + o * o + e * o + (i - e) * e + j;
+}
+
+FU_HD inline bool live_slot(int i, int ecap, int le, int lo) {
+  return i < ecap ? i < le : (i - ecap) < lo;
+}
+
+// Site-side description of one reduced pair factor.
+// rows: the three exterior virtual legs (site leg indices) and Aux; core: the
+// bond leg; pseudo-site slots of the Q tensor (W,N,P,A,E,S): the core leg's
+// slot carries r, P carries Aux, A has dimension 1.
+struct FuSide {
+  int leg[6], even[6];   // site dims / even counts (W,N,P,A,E,S)
+  int rowv[3];           // exterior virtual legs
+  int core;              // bond leg (1 of W,N,E,S)
+  int rcap, recap;       // r capacity and its even count
+  long long rows;        // capacity rows = prod of exterior dims * Aux
+};
+
+struct PairJob {
+  const double* site[2];   // per sample: A, B
+  const int* live[2];      // per sample: live counts [4 legs][2] of A and B
+};
+
+FU_HD inline int vidx(int leg) { return leg == 0 ? 0 : leg == 1 ? 1 : leg == 4 ? 2 : 3; }
+
+// Householder QR of both sites of every sample (thread per sample) into
+// pseudo-site Q tensors and cores, then the gated target. Work arrays are
+// interleaved by sample (stride nbtot).
+FU_HD inline void fu_pair_sample(const PairJob& J, int s, const FuSide& sa, const FuSide& sb, const double* gate,
+                                  int d, int ed, double* qa, long long qa_stride, double* qb, long long qb_stride,
+                                  double* ca, double* cb, double* target, int* rlive, double* work,
+                                  long long work_per, int nbtot) {
+  fu::Work w{work + s, nbtot, 0, work_per};
+  const FuSide* sides[2] = {&sa, &sb};
+  double* qout[2] = {qa + (long long)s * qa_stride, qb + (long long)s * qb_stride};
+  double* cout[2] = {ca, cb};
+  for (int side = 0; side < 2; ++side) {
+    const FuSide& F = *sides[side];
+    const double* A = J.site[side];
+    const int* lv = J.live[side];
+    // strides of the (W,N,P,A,E,S) site array
+    long long st[6];
+    st[5] = 1;
+    for (int k = 4; k >= 0; --k) st[k] = st[k + 1] * F.leg[k + 1];
+    // pseudo-site dims / strides
+    int pd[6];
+    for (int k = 0; k < 6; ++k) pd[k] = F.leg[k];
+    pd[2] = F.leg[3];
+    pd[3] = 1;
+    pd[F.core] = F.rcap;
+    long long ps[6];
+    ps[5] = 1;
+    for (int k = 4; k >= 0; --k) ps[k] = ps[k + 1] * pd[k + 1];
+    double* Q = qout[side];
+    const long long qsize = ps[0] * pd[0];
+    for (long long i = 0; i < qsize; ++i) Q[i] = 0.0;
+    const int ncore = F.leg[F.core];
+    // cores C[r][p][k] (interleaved)
+    fu::View C{cout[side] + s, nbtot};
+    for (long long i = 0; i < (long long)F.rcap * d * ncore; ++i) C[i] = 0.0;
+    const int dv0 = F.leg[F.rowv[0]], dv1 = F.leg[F.rowv[1]], dv2 = F.leg[F.rowv[2]], dax = F.leg[3];
+    int rl[2] = {0, 0};
+    for (int q = 0; q < 2; ++q) {
+      const long long mark = w.top;
+      // live rows / columns of parity q
+      int mq = 0, nq = 0;
+      for (int i0 = 0; i0 < dv0; ++i0)
+        for (int i1 = 0; i1 < dv1; ++i1)
+          for (int i2 = 0; i2 < dv2; ++i2)
+            for (int ax = 0; ax < dax; ++ax) {
+              const int l0 = F.rowv[0], l1 = F.rowv[1], l2 = F.rowv[2];
+              if (!live_slot(i0, F.even[l0], lv[2 * vidx(l0)], lv[2 * vidx(l0) + 1]) ||
+                  !live_slot(i1, F.even[l1], lv[2 * vidx(l1)], lv[2 * vidx(l1) + 1]) ||
+                  !live_slot(i2, F.even[l2], lv[2 * vidx(l2)], lv[2 * vidx(l2) + 1]))
+                continue;
+              const int pr = fu::par(i0, F.even[l0]) ^ fu::par(i1, F.even[l1]) ^ fu::par(i2, F.even[l2]) ^
+                             fu::par(ax, F.even[3]);
+              mq += pr == q;
+            }
+      for (int p = 0; p < d; ++p)
+        for (int k = 0; k < ncore; ++k) {
+          if (!live_slot(k, F.even[F.core], lv[2 * vidx(F.core)], lv[2 * vidx(F.core) + 1])) continue;
+          nq += (fu::par(p, ed) ^ fu::par(k, F.even[F.core])) == q;
+        }
+      if (mq == 0 || nq == 0) continue;
+      fu::View M = w.take((long long)mq * nq);
+      fu::View rowid = w.take(mq), colid = w.take(nq);
+      {
+        int bi = 0;
+        for (int i0 = 0; i0 < dv0; ++i0)
+          for (int i1 = 0; i1 < dv1; ++i1)
+            for (int i2 = 0; i2 < dv2; ++i2)
+              for (int ax = 0; ax < dax; ++ax) {
+                const int l0 = F.rowv[0], l1 = F.rowv[1], l2 = F.rowv[2];
+                if (!live_slot(i0, F.even[l0], lv[2 * vidx(l0)], lv[2 * vidx(l0) + 1]) ||
+                    !live_slot(i1, F.even[l1], lv[2 * vidx(l1)], lv[2 * vidx(l1) + 1]) ||
+                    !live_slot(i2, F.even[l2], lv[2 * vidx(l2)], lv[2 * vidx(l2) + 1]))
+                  continue;
+                const int pr = fu::par(i0, F.even[l0]) ^ fu::par(i1, F.even[l1]) ^ fu::par(i2, F.even[l2]) ^
+                               fu::par(ax, F.even[3]);
+                if (pr != q) continue;
+                rowid[bi] = (double)(((i0 * dv1 + i1) * dv2 + i2) * dax + ax);
+                ++bi;
+              }
+        int bj = 0;
+        for (int p = 0; p < d; ++p)
+          for (int k = 0; k < ncore; ++k) {
+            if (!live_slot(k, F.even[F.core], lv[2 * vidx(F.core)], lv[2 * vidx(F.core) + 1])) continue;
+            if ((fu::par(p, ed) ^ fu::par(k, F.even[F.core])) != q) continue;
+            colid[bj] = (double)(p * ncore + k);
+            ++bj;
+          }
+      }
+      for (int bi = 0; bi < mq; ++bi) {
+        int rid = (int)rowid[bi];
+        const int ax = rid % dax;
+        rid /= dax;
+        const int i2 = rid % dv2;
+        rid /= dv2;
+        const int i1 = rid % dv1;
+        const int i0 = rid / dv1;
+        long long base = (long long)ax * st[3] + (long long)i0 * st[F.rowv[0]] + (long long)i1 * st[F.rowv[1]] +
+                         (long long)i2 * st[F.rowv[2]];
+        for (int bj = 0; bj < nq; ++bj) {
+          const int cid = (int)colid[bj];
+          const int p = cid / ncore, k = cid % ncore;
+          M[(long long)bi * nq + bj] = A[base + (long long)p * st[2] + (long long)k * st[F.core]];
+        }
+      }
+      const int kq = mq < nq ? mq : nq;
+      fu::View Qb = w.take((long long)mq * kq), Rb = w.take((long long)kq * nq);
+      fu::View vs = w.take((long long)mq * kq), tau = w.take(kq);
+      fu::householder_qr(M, mq, nq, Qb, Rb, vs, tau);
+      const int off = q == 0 ? 0 : F.recap;
+      rl[q] = kq;
+      for (int bi = 0; bi < mq; ++bi) {
+        int rid = (int)rowid[bi];
+        const int ax = rid % dax;
+        rid /= dax;
+        const int i2 = rid % dv2;
+        rid /= dv2;
+        const int i1 = rid % dv1;
+        const int i0 = rid / dv1;
+        for (int c = 0; c < kq; ++c) {
+          int idx[6] = {0, 0, 0, 0, 0, 0};
+          idx[F.rowv[0]] = i0;
+          idx[F.rowv[1]] = i1;
+          idx[F.rowv[2]] = i2;
+          idx[2] = ax;
+          idx[F.core] = off + c;
+          long long o = 0;
+          for (int k = 0; k < 6; ++k) o += (long long)idx[k] * ps[k];
+          Q[o] = Qb[(long long)bi * kq + c];
+        }
+      }
+      for (int c = 0; c < kq; ++c)
+        for (int bj = 0; bj < nq; ++bj) {
+          const int cid = (int)colid[bj];
+          const int p = cid / ncore, k = cid % ncore;
+          C[((long long)(off + c) * d + p) * ncore + k] = Rb[(long long)c * nq + bj];
+        }
+      w.top = mark;
+    }
+    rlive[(long long)s * 4 + side * 2] = rl[0];
+    rlive[(long long)s * 4 + side * 2 + 1] = rl[1];
+  }
+  // target[r,p,q,s] = sum_{u,k,v} Ca[r,u,k] Cb[s,v,k] gate[p,q,u,v]
+  const int ra = sa.rcap, rb = sb.rcap, nk = sa.leg[sa.core];
+  fu::View Ca{ca + s, nbtot}, Cb{cb + s, nbtot}, Tg{target + s, nbtot};
+  for (int r = 0; r < ra; ++r)
+    for (int p = 0; p < d; ++p)
+      for (int q = 0; q < d; ++q)
+        for (int t = 0; t < rb; ++t) {
+          double acc = 0.0;
+          for (int u = 0; u < d; ++u)
+            for (int v = 0; v < d; ++v) {
+              const double gv = gate[((p * d + q) * d + u) * d + v];
+              if (gv == 0.0) continue;
+              double ab = 0.0;
+              for (int k = 0; k < nk; ++k) ab += Ca[((long long)r * d + u) * nk + k] * Cb[((long long)t * d + v) * nk + k];
+              acc += gv * ab;
+            }
+          Tg[(((long long)r * d + p) * d + q) * rb + t] = acc;
+        }
+}
+
+__global__ void fu_pair_kernel(const PairJob* jobs, int nb, FuSide sa, FuSide sb, const double* gate, int d,
+                               int ed, double* qa, long long qa_stride, double* qb, long long qb_stride,
+                               double* ca, double* cb, double* target, int* rlive, double* work,
+                               long long work_per, int nbtot) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nb) return;
+  fu_pair_sample(jobs[s], s, sa, sb, gate, d, ed, qa, qa_stride, qb, qb_stride, ca, cb, target, rlive, work,
+                 work_per, nbtot);
+}
+
+// Bond solve of every sample (thread per sample): compact the metric and the
+// target to the live r dimensions, solve, expand the cores.
+template <typename T>
+__global__ void fu_core_kernel(const T* nf, long long nf_stride, const double* target, const int* rlive, int ra,
+                               int era, int rb, int erb, int d, int ed, int D, int eD, fu::Opts o, double* aout,
+                               double* bout, double* rec, int* bond_live, double* work, long long work_per,
+                               int nb, int nbtot) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nb) return;
+  const int la[2] = {rlive[(long long)s * 4], rlive[(long long)s * 4 + 1]};
+  const int lb[2] = {rlive[(long long)s * 4 + 2], rlive[(long long)s * 4 + 3]};
+  fu::Dims g{la[0] + la[1], la[0], lb[0] + lb[1], lb[0], d, ed, D, eD};
+  fu::Work w{work + s, nbtot, 0, work_per};
+  const long long na = g.na, nbk = g.nb;
+  fu::View N4 = w.take(na * nbk * na * nbk), tg = w.take(na * d * d * nbk);
+  fu::View A = w.take(na * d * D), B = w.take((long long)D * d * nbk);
+  auto slot_a = [&](int i) { return i < la[0] ? i : era + (i - la[0]); };
+  auto slot_b = [&](int i) { return i < lb[0] ? i : erb + (i - lb[0]); };
+  const T* N = nf + (long long)s * nf_stride;
+  const int nbr = rb * rb;
+  // N[rb,sb,rk,sk] = Nf[pair(rk, rb)][pair(sk, sb)]. The contraction kernel
+  // writes only the parity-allowed blocks of its output (pair parities equal);
+  // the others hold whatever the buffer held before and are zero by symmetry.
+  for (int i = 0; i < g.na; ++i)
+    for (int j = 0; j < g.nb; ++j)
+      for (int k = 0; k < g.na; ++k)
+        for (int l = 0; l < g.nb; ++l) {
+          const int pR = fu::par(i, la[0]) ^ fu::par(k, la[0]);
+          const int pS = fu::par(j, lb[0]) ^ fu::par(l, lb[0]);
+          double v = 0.0;
+          if (pR == pS) {
+            const int R = pair_index(slot_a(k), slot_a(i), ra, era);
+            const int S = pair_index(slot_b(l), slot_b(j), rb, erb);
+            v = (double)N[(long long)R * nbr + S];
+          }
+          N4[(((long long)i * g.nb + j) * g.na + k) * g.nb + l] = v;
+        }
+  fu::View Tg{const_cast<double*>(target) + s, nbtot};
+  for (int r = 0; r < g.na; ++r)
+    for (int p = 0; p < d; ++p)
+      for (int q = 0; q < d; ++q)
+        for (int t = 0; t < g.nb; ++t)
+          tg[(((long long)r * d + p) * d + q) * g.nb + t] = Tg[(((long long)slot_a(r) * d + p) * d + q) * rb + slot_b(t)];
+  fu::Work inner{work + s, nbtot, w.top, work_per - w.top};
+  fu::Result res = fu::fu_bond(g, o, N4, tg, A, B, inner);
+  fu::View Ao{aout + s, nbtot}, Bo{bout + s, nbtot};
+  for (long long i = 0; i < (long long)ra * d * D; ++i) Ao[i] = 0.0;
+  for (long long i = 0; i < (long long)D * d * rb; ++i) Bo[i] = 0.0;
+  if (res.status == 0) {
+    for (int r = 0; r < g.na; ++r)
+      for (long long j = 0; j < (long long)d * D; ++j) Ao[(long long)slot_a(r) * d * D + j] = A[(long long)r * d * D + j];
+    for (long long kq = 0; kq < (long long)D * d; ++kq)
+      for (int t = 0; t < g.nb; ++t) Bo[kq * rb + slot_b(t)] = B[kq * g.nb + t];
+  }
+  double* rr = rec + (long long)s * 12;
+  rr[0] = res.initial_cost;
+  rr[1] = res.final_cost;
+  rr[2] = res.residual;
+  rr[3] = res.target_norm;
+  rr[4] = res.iterations;
+  rr[5] = res.limit_reached;
+  rr[6] = res.stop;
+  rr[7] = res.status;
+  rr[8] = res.kept_even;
+  rr[9] = res.kept_odd;
+  rr[10] = res.floored;
+  rr[11] = 0.0;
+  bond_live[(long long)s * 2] = res.kept_even;
+  bond_live[(long long)s * 2 + 1] = res.kept_odd;
+}
+
+// New site = Q * core, written into the state's site array (thread per entry).
+struct RestoreJob {
+  double* site[2];
+};
+
+__global__ void fu_restore_kernel(const RestoreJob* jobs, int nb, FuSide sa, FuSide sb, int d, const double* qa,
+                                  long long qa_stride, const double* qb, long long qb_stride, const double* aout,
+                                  const double* bout, int D, int nbtot) {
+  const int s = blockIdx.y;
+  const int side = blockIdx.z;
+  if (s >= nb) return;
+  const FuSide& F = side == 0 ? sa : sb;
+  long long total = 1;
+  for (int k = 0; k < 6; ++k) total *= F.leg[k];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int idx[6];
+  long long r = t;
+  for (int k = 5; k >= 0; --k) {
+    idx[k] = (int)(r % F.leg[k]);
+    r /= F.leg[k];
+  }
+  int pd[6];
+  for (int k = 0; k < 6; ++k) pd[k] = F.leg[k];
+  pd[2] = F.leg[3];
+  pd[3] = 1;
+  pd[F.core] = F.rcap;
+  long long ps[6];
+  ps[5] = 1;
+  for (int k = 4; k >= 0; --k) ps[k] = ps[k + 1] * pd[k + 1];
+  int qi[6] = {idx[0], idx[1], idx[3], 0, idx[4], idx[5]};
+  qi[F.core] = 0;
+  long long qbase = 0;
+  for (int k = 0; k < 6; ++k) qbase += (long long)qi[k] * ps[k];
+  const double* Q = side == 0 ? qa + (long long)s * qa_stride : qb + (long long)s * qb_stride;
+  const int p = idx[2], k = idx[F.core];
+  double acc = 0.0;
+  if (side == 0) {
+    fu::View Ao{const_cast<double*>(aout) + s, nbtot};
+    for (int c = 0; c < F.rcap; ++c) acc += Q[qbase + (long long)c * ps[F.core]] * Ao[((long long)c * d + p) * D + k];
+  } else {
+    fu::View Bo{const_cast<double*>(bout) + s, nbtot};
+    for (int c = 0; c < F.rcap; ++c) acc += Q[qbase + (long long)c * ps[F.core]] * Bo[((long long)k * d + p) * F.rcap + c];
+  }
+  jobs[s].site[side][t] = acc;
+}
+
+// live[(s*ns + site)*8 + v*2 + q] <- bond_live[s*2 + q] for the bond legs of A and B
+__global__ void fu_set_live_kernel(int* live, int ns, int site_a, int va, int site_b, int vb, const int* bond_live,
+                                   int nb) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nb) return;
+  for (int q = 0; q < 2; ++q) {
+    live[((long long)s * ns + site_a) * 8 + va * 2 + q] = bond_live[(long long)s * 2 + q];
+    live[((long long)s * ns + site_b) * 8 + vb * 2 + q] = bond_live[(long long)s * 2 + q];
+  }
+}
+
+// sum of squares of each sample's site array (block per sample)
+__global__ void site_sumsq_kernel(double* const* p, long long n, double* out) {
+  const double* a = p[blockIdx.x];
+  __shared__ double part[256];
+  double acc = 0.0;
+  for (long long i = threadIdx.x; i < n; i += blockDim.x) acc += a[i] * a[i];
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = part[0];
+}
+
+__global__ void site_scale_kernel(double* const* p, long long n, const double* f) {
+  double* a = p[blockIdx.x];
+  const double v = f[blockIdx.x];
+  for (long long i = (long long)blockIdx.y * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.y * blockDim.x)
+    a[i] *= v;
+}
+
+long long count_legs(const std::vector<Leg>& l) {
+  long long n = 1;
+  for (auto& g : l) n *= g.dim;
+  return n;
+}
+
+FuSide make_side(const SiteDev& sd, int rowv0, int rowv1, int rowv2, int core) {
+  FuSide F{};
+  for (int k = 0; k < 6; ++k) {
+    F.leg[k] = sd.dim[k];
+    F.even[k] = sd.even[k];
+  }
+  F.rowv[0] = rowv0;
+  F.rowv[1] = rowv1;
+  F.rowv[2] = rowv2;
+  F.core = core;
+  // r capacity per parity: min(rows of parity q, columns of parity q)
+  long long rq[2] = {0, 0}, cq[2] = {0, 0};
+  for (int a = 0; a < sd.dim[rowv0]; ++a)
+    for (int b = 0; b < sd.dim[rowv1]; ++b)
+      for (int c = 0; c < sd.dim[rowv2]; ++c)
+        for (int x = 0; x < sd.dim[3]; ++x)
+          ++rq[fu::par(a, sd.even[rowv0]) ^ fu::par(b, sd.even[rowv1]) ^ fu::par(c, sd.even[rowv2]) ^
+               fu::par(x, sd.even[3])];
+  for (int p = 0; p < sd.dim[2]; ++p)
+    for (int k = 0; k < sd.dim[core]; ++k) ++cq[fu::par(p, sd.even[2]) ^ fu::par(k, sd.even[core])];
+  const long long k0 = std::min(rq[0], cq[0]), k1 = std::min(rq[1], cq[1]);
+  F.recap = (int)k0;
+  F.rcap = (int)(k0 + k1);
+  F.rows = rq[0] + rq[1];
+  return F;
+}
+
+}  // namespace
+
+// Host run of the reduced-pair stage for one sample (selftest).
+int fu_pair_host(int horizontal, const int* dims_a, const int* evens_a, const int* dims_b, const int* evens_b,
+                 const double* site_a, const double* site_b, const int* live_a, const int* live_b, const double* gate,
+                 double* qa, double* qb, double* ca, double* cb, double* target, int* geometry) {
+  SiteDev A, B;
+  for (int k = 0; k < 6; ++k) {
+    A.dim[k] = dims_a[k];
+    A.even[k] = evens_a[k];
+    B.dim[k] = dims_b[k];
+    B.even[k] = evens_b[k];
+  }
+  FuSide sa = horizontal ? make_side(A, 0, 1, 5, 4) : make_side(A, 0, 1, 4, 5);
+  FuSide sb = horizontal ? make_side(B, 1, 4, 5, 0) : make_side(B, 0, 4, 5, 1);
+  A.d = nullptr;
+  B.d = nullptr;
+  const int d = A.dim[2], ed = A.even[2];
+  long long work = 64;
+  for (const FuSide* F : {&sa, &sb}) {
+    const long long cols = (long long)d * F->leg[F->core];
+    work = std::max(work, 3 * F->rows * cols + 2 * F->rows + 2 * cols + 64);
+  }
+  std::vector<double> ws((size_t)work);
+  PairJob J{};
+  J.site[0] = site_a;
+  J.site[1] = site_b;
+  J.live[0] = live_a;
+  J.live[1] = live_b;
+  int rl[4] = {0, 0, 0, 0};
+  fu_pair_sample(J, 0, sa, sb, gate, d, ed, qa, 0, qb, 0, ca, cb, target, rl, ws.data(), work, 1);
+  geometry[0] = sa.rcap;
+  geometry[1] = sa.recap;
+  geometry[2] = sb.rcap;
+  geometry[3] = sb.recap;
+  for (int k = 0; k < 4; ++k) geometry[4 + k] = rl[k];
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// the evolution step
+// ---------------------------------------------------------------------------
+
+template <typename T>
+void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int chi, const FuParams& P,
+                       FuOutputs& out) {
+  hipStream_t st = ctx.stream;
+  const int n = (int)states.size();
+  if (n == 0) return;
+  if (chi < 1) throw HipError("chi must be positive");
+  const StateDev& s0 = *states[0];
+  const int Lx = s0.Lx, Ly = s0.Ly, ns = Lx * Ly;
+  for (const StateDev* sd : states) {
+    if (!sd || sd->Lx != Lx || sd->Ly != Ly || (int)sd->sites.size() != ns)
+      throw HipError("full update: states must share the lattice");
+    for (int i = 0; i < ns; ++i)
+      for (int k = 0; k < 6; ++k)
+        if (sd->sites[i].dim[k] != s0.sites[i].dim[k] || sd->sites[i].even[k] != s0.sites[i].even[k])
+          throw HipError("full update: states must share every leg space");
+    if ((int)sd->live.size() != ns * 8) throw HipError("full update: state without bond bookkeeping");
+  }
+  const int d = s0.sites[0].dim[2], ed = s0.sites[0].even[2];
+  for (int i = 0; i < ns; ++i)
+    if (s0.sites[i].dim[2] != d || s0.sites[i].even[2] != ed) throw HipError("full update: physical legs differ");
+  if (ns == 1) throw HipError("full update: a single-site lattice has no bonds");
+  const int nh = Ly * (Lx - 1), nv = (Ly - 1) * Lx;
+  if (!P.gates_h || !P.gates_v) throw HipError("full update: gates are required");
+  // leg specs of the double layers (fixed capacities)
+  std::vector<LayerSpec> spec(ns);
+  std::vector<long long> lsize(ns);
+  long long D = 1, maxout = 1, layer_elems = 0;
+  for (int i = 0; i < ns; ++i) {
+    const SiteDev& s = s0.sites[i];
+    spec[i] = LayerSpec{pair_leg(s.dim[0], s.even[0]), pair_leg(s.dim[1], s.even[1]), pair_leg(s.dim[4], s.even[4]),
+                        pair_leg(s.dim[5], s.even[5])};
+    lsize[i] = (long long)spec[i].w.dim * spec[i].n.dim * spec[i].e.dim * spec[i].s.dim;
+    layer_elems += lsize[i];
+    maxout = std::max(maxout, lsize[i]);
+    D = std::max<long long>(D, std::max(std::max(spec[i].w.dim, spec[i].n.dim), std::max(spec[i].e.dim, spec[i].s.dim)));
+  }
+  long long pmax = 1;
+  for (const LayerSpec& q : spec) pmax = std::max<long long>(pmax, std::max(q.n.dim, q.s.dim));
+  const long long cap = (long long)chi * pmax * chi;
+  // the sides of every bond: capacities of r, Q sizes, reduced layer sizes
+  struct BondGeo {
+    int ia, ib;
+    FuSide sa, sb;
+    int Dcap, eDcap;
+    long long qa_size, qb_size, la_size, lb_size;
+    LayerSpec la, lb;
+  };
+  auto geo = [&](int ia, int ib, bool horizontal) {
+    BondGeo g{};
+    g.ia = ia;
+    g.ib = ib;
+    const SiteDev& A = s0.sites[ia];
+    const SiteDev& B = s0.sites[ib];
+    if (horizontal) {
+      g.sa = make_side(A, 0, 1, 5, 4);
+      g.sb = make_side(B, 1, 4, 5, 0);
+      g.Dcap = A.dim[4];
+      g.eDcap = A.even[4];
+    } else {
+      g.sa = make_side(A, 0, 1, 4, 5);
+      g.sb = make_side(B, 0, 4, 5, 1);
+      g.Dcap = A.dim[5];
+      g.eDcap = A.even[5];
+    }
+    auto qsize = [&](const FuSide& F) {
+      long long v = 1;
+      for (int k = 0; k < 6; ++k) v *= (k == F.core ? F.rcap : k == 3 ? 1 : k == 2 ? F.leg[3] : F.leg[k]);
+      return v;
+    };
+    g.qa_size = qsize(g.sa);
+    g.qb_size = qsize(g.sb);
+    auto lspec = [&](const FuSide& F) {
+      int dm[4] = {F.leg[0], F.leg[1], F.leg[4], F.leg[5]}, ev[4] = {F.even[0], F.even[1], F.even[4], F.even[5]};
+      const int ci = F.core == 0 ? 0 : F.core == 1 ? 1 : F.core == 4 ? 2 : 3;
+      dm[ci] = F.rcap;
+      ev[ci] = F.recap;
+      return LayerSpec{pair_leg(dm[0], ev[0]), pair_leg(dm[1], ev[1]), pair_leg(dm[2], ev[2]), pair_leg(dm[3], ev[3])};
+    };
+    g.la = lspec(g.sa);
+    g.lb = lspec(g.sb);
+    g.la_size = (long long)g.la.w.dim * g.la.n.dim * g.la.e.dim * g.la.s.dim;
+    g.lb_size = (long long)g.lb.w.dim * g.lb.n.dim * g.lb.e.dim * g.lb.s.dim;
+    return g;
+  };
+  std::vector<BondGeo> hb(nh), vb(nv);
+  for (int y = 0; y < Ly; ++y)
+    for (int x = 0; x + 1 < Lx; ++x) hb[y * (Lx - 1) + x] = geo(x + y * Lx, x + 1 + y * Lx, true);
+  for (int y = 0; y + 1 < Ly; ++y)
+    for (int x = 0; x < Lx; ++x) vb[y * Lx + x] = geo(x + y * Lx, x + (y + 1) * Lx, false);
+  long long qmax = 1, lredmax = 1, rmax = 1, coremax = 1, cmax = 1;
+  for (const auto* set : {&hb, &vb})
+    for (const BondGeo& g : *set) {
+      qmax = std::max(qmax, std::max(g.qa_size, g.qb_size));
+      lredmax = std::max(lredmax, std::max(g.la_size, g.lb_size));
+      rmax = std::max<long long>(rmax, std::max(g.sa.rcap, g.sb.rcap));
+      fu::Dims cg{g.sa.rcap, g.sa.recap, g.sb.rcap, g.sb.recap, d, ed, g.Dcap, g.eDcap};
+      coremax = std::max(coremax, fu::workspace_doubles(cg) + 2 * (long long)cg.na * cg.nb * cg.na * cg.nb +
+                                      4 * (long long)cg.na * d * d * cg.nb + 4 * (long long)cg.na * d * cg.D +
+                                      4 * (long long)cg.D * d * cg.nb);
+      cmax = std::max<long long>(cmax, (long long)std::max(g.sa.rcap, g.sb.rcap) * d * g.Dcap);
+    }
+  long long pairwork = 64;
+  for (const auto* set : {&hb, &vb})
+    for (const BondGeo& g : *set)
+      for (const FuSide* F : {&g.sa, &g.sb}) {
+        const long long cols = (long long)d * F->leg[F->core];
+        pairwork = std::max(pairwork, 3 * F->rows * cols + 2 * F->rows + 2 * cols + 64);
+      }
+  const long long tsz = rmax * d * d * rmax;
+  const long long nfs = rmax * rmax * rmax * rmax;
+  // ---- chunk size from memory ----
+  const double ts = sizeof(T);
+  const double small = (double)chi * D * chi, mid = small * D;
+  const double slab = mid * (double)rmax * rmax * D / std::max<long long>(D, 1);
+  const double row_b = ts * ((Lx + 6) * mid + 2.0 * small + 4.0 * mid * D + 4.0 * chi * chi * D * rmax * rmax +
+                             slab * 2.0);
+  const size_t per_arena =
+      (size_t)(1.2 * std::max(row_b, absorb_bytes_per_sample(Lx, chi, D, sizeof(T), ctx.ksub))) + 16384;
+  const size_t per_persist = (size_t)(ts * (layer_elems + 2.0 * (Ly + 1) * Lx * cap + 2.0 * lredmax + nfs)) +
+                             sizeof(double) * (2 * qmax + 3 * tsz + 4 * cmax + pairwork + coremax + 64) +
+                             sizeof(int) * (ns * 8 + 16) + sizeof(LayerJob) * ns + 1024;
+  SPP_CHECK(hipStreamSynchronize(st));
+  size_t freeb = 0, totalb = 0;
+  SPP_CHECK(hipMemGetInfo(&freeb, &totalb));
+  const double avail = 0.6 * (double)freeb + (double)ctx.arena.capacity();
+  long long chunk = std::max<long long>(1, (long long)(avail / (double)(per_arena + per_persist)));
+  if (ctx.chunk > 0) chunk = std::min<long long>(chunk, ctx.chunk);
+  chunk = std::min<long long>(chunk, n);
+  chunk = std::min<long long>(chunk, std::max(1, 65535 / ns));
+  const int nb = (int)chunk;
+  ctx.arena.reserve(per_arena * (size_t)nb);
+
+  std::vector<Boundary<T>> north(Ly + 1), south(Ly + 1);
+  std::vector<void*> scratch;
+  auto dev = [&](size_t bytes) {
+    void* p = nullptr;
+    SPP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+    scratch.push_back(p);
+    return p;
+  };
+  auto cleanup = [&]() {
+    for (auto& b : north) b.release();
+    for (auto& b : south) b.release();
+    for (void* p : scratch) (void)hipFree(p);
+    scratch.clear();
+  };
+  const int nrec_bonds = P.order == 2 ? 2 * (nh + nv) : nh + nv;
+  try {
+    T* layers = (T*)dev(sizeof(T) * (size_t)layer_elems * nb);
+    std::vector<T*> site_base(ns);
+    {
+      long long off = 0;
+      for (int i = 0; i < ns; ++i) {
+        site_base[i] = layers + off * nb;
+        off += lsize[i];
+      }
+    }
+    void** ptrs = (void**)dev(sizeof(void*) * (size_t)ns * nb);
+    {
+      std::vector<void*> hptr((size_t)ns * nb);
+      for (int i = 0; i < ns; ++i)
+        for (int s = 0; s < nb; ++s) hptr[(size_t)i * nb + s] = site_base[i] + (long long)s * lsize[i];
+      SPP_CHECK(hipMemcpyAsync(ptrs, hptr.data(), sizeof(void*) * hptr.size(), hipMemcpyHostToDevice, st));
+    }
+    auto table = [&](int site) { return (void* const*)(ptrs + (size_t)site * nb); };
+    LayerJob* djobs = (LayerJob*)dev(sizeof(LayerJob) * (size_t)std::max(ns, 2) * nb);
+    for (int y = 0; y <= Ly; ++y) {
+      prepare_boundary(north[y], Lx, nb, cap);
+      prepare_boundary(south[y], Lx, nb, cap);
+    }
+    double* qa = (double*)dev(sizeof(double) * (size_t)qmax * nb);
+    double* qbuf = (double*)dev(sizeof(double) * (size_t)qmax * nb);
+    T* lred_a = (T*)dev(sizeof(T) * (size_t)lredmax * nb);
+    T* lred_b = (T*)dev(sizeof(T) * (size_t)lredmax * nb);
+    T* nfused = (T*)dev(sizeof(T) * (size_t)nfs * nb);
+    double* ca = (double*)dev(sizeof(double) * (size_t)cmax * nb);
+    double* cb = (double*)dev(sizeof(double) * (size_t)cmax * nb);
+    double* tgt = (double*)dev(sizeof(double) * (size_t)tsz * nb);
+    double* aout = (double*)dev(sizeof(double) * (size_t)cmax * nb);
+    double* bout = (double*)dev(sizeof(double) * (size_t)cmax * nb);
+    double* pwork = (double*)dev(sizeof(double) * (size_t)pairwork * nb);
+    double* cwork = (double*)dev(sizeof(double) * (size_t)coremax * nb);
+    double* drec = (double*)dev(sizeof(double) * 12 * (size_t)nb);
+    int* rlive = (int*)dev(sizeof(int) * 4 * (size_t)nb);
+    int* blive = (int*)dev(sizeof(int) * 2 * (size_t)nb);
+    int* live = (int*)dev(sizeof(int) * (size_t)ns * 8 * nb);
+    double* gate_dev = (double*)dev(sizeof(double) * (size_t)(nh + nv) * d * d * d * d);
+    SPP_CHECK(hipMemcpyAsync(gate_dev, P.gates_h, sizeof(double) * (size_t)nh * d * d * d * d, hipMemcpyHostToDevice,
+                             st));
+    SPP_CHECK(hipMemcpyAsync(gate_dev + (size_t)nh * d * d * d * d, P.gates_v, sizeof(double) * (size_t)nv * d * d * d * d,
+                             hipMemcpyHostToDevice, st));
+    PairJob* pjobs = (PairJob*)dev(sizeof(PairJob) * (size_t)nb);
+    RestoreJob* rjobs = (RestoreJob*)dev(sizeof(RestoreJob) * (size_t)nb);
+    int* status = (int*)dev(sizeof(int) * nb);
+    int* fail = (int*)dev(sizeof(int) * nb);
+    double* nrm = (double*)dev(sizeof(double) * nb);
+    double* fac = (double*)dev(sizeof(double) * nb);
+    std::vector<double> hrec((size_t)12 * nb);
+    std::vector<int> hstatus(nb);
+    std::vector<double> hmt(nb);
+    for (long long c0 = 0; c0 < n; c0 += nb) {
+      const int cur = (int)std::min<long long>(nb, n - c0);
+      // per-sample bookkeeping
+      {
+        std::vector<int> hl((size_t)ns * 8 * nb, 0);
+        for (int s = 0; s < cur; ++s)
+          std::memcpy(hl.data() + (size_t)s * ns * 8, states[c0 + s]->live.data(), sizeof(int) * ns * 8);
+        SPP_CHECK(hipMemcpyAsync(live, hl.data(), sizeof(int) * hl.size(), hipMemcpyHostToDevice, st));
+      }
+      SPP_CHECK(launch_fill<int>(status, nb, 0, st));
+      SPP_CHECK(launch_fill<int>(fail, nb, -1, st));
+      std::vector<double> env_err(cur, 0.0);
+      auto relayer = [&](const std::vector<int>& sites) {
+        std::vector<LayerJob> jobs;
+        for (int i : sites)
+          for (int s = 0; s < cur; ++s) {
+            const SiteDev& sd = states[c0 + s]->sites[i];
+            LayerJob J{};
+            J.ket = sd.d;
+            J.bra = sd.d;
+            for (int k = 0; k < 6; ++k) {
+              J.dk[k] = J.db[k] = sd.dim[k];
+              J.ek[k] = J.eb[k] = sd.even[k];
+            }
+            J.out = site_base[i] + (long long)s * lsize[i];
+            jobs.push_back(J);
+          }
+        // the job table may still be read by queued kernels: wait before rewriting it
+        SPP_CHECK(hipStreamSynchronize(st));
+        SPP_CHECK(hipMemcpyAsync(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), hipMemcpyHostToDevice, st));
+        SPP_CHECK(launch_double_layers<T>(djobs, (int)jobs.size(), maxout, st));
+      };
+      {
+        std::vector<int> all(ns);
+        for (int i = 0; i < ns; ++i) all[i] = i;
+        relayer(all);
+      }
+      const int nsweep = P.order == 2 ? 4 : 2;
+      const bool sweep_h[4] = {true, false, false, true};
+      const bool sweep_rev[4] = {false, false, true, true};
+      int rec_bond = 0;
+      for (int sw = 0; sw < nsweep; ++sw) {
+        const bool horizontal = sweep_h[sw], reverse = sweep_rev[sw];
+        if (horizontal ? nh == 0 : nv == 0) continue;
+        for (auto& b : north) b.nb = cur;
+        for (auto& b : south) b.nb = cur;
+        // boundary environments of the far side (boundary_environments, side = reverse ? north : south)
+        std::vector<Leg> physN(Lx), physS(Lx);
+        for (int x = 0; x < Lx; ++x) physN[x] = spec[x].n;
+        for (int xr = 0; xr < Lx; ++xr) physS[xr] = spec[(Lx - 1 - xr) + (Ly - 1) * Lx].s;
+        auto absorb_row = [&](int y, bool into_south) {
+          if (into_south) {
+            RowLayers<T> rr;
+            rr.L = Lx;
+            rr.rotated = true;
+            rr.spec.resize(Lx);
+            rr.ptrs.resize(Lx);
+            for (int xr = 0; xr < Lx; ++xr) {
+              const int x = Lx - 1 - xr;
+              const LayerSpec& sp = spec[x + y * Lx];
+              rr.spec[xr] = LayerSpec{sp.e, sp.s, sp.w, sp.n};
+              rr.ptrs[xr] = table(x + y * Lx);
+            }
+            absorb_forward<T>(ctx, south[y + 1], rr, chi, south[y]);
+          } else {
+            RowLayers<T> row;
+            row.L = Lx;
+            row.spec.assign(spec.begin() + y * Lx, spec.begin() + (y + 1) * Lx);
+            row.ptrs.resize(Lx);
+            for (int x = 0; x < Lx; ++x) row.ptrs[x] = table(x + y * Lx);
+            absorb_forward<T>(ctx, north[y], row, chi, north[y + 1]);
+          }
+        };
+        set_trivial(ctx, north[0], physN);
+        set_trivial(ctx, south[Ly], physS);
+        if (reverse)
+          for (int y = 0; y < Ly; ++y) absorb_row(y, false);
+        else
+          for (int y = Ly - 1; y >= 0; --y) absorb_row(y, true);
+        const int ymax = horizontal ? Ly : Ly - 1;
+        const int xmax = horizontal ? Lx - 1 : Lx;
+        for (int yi = 0; yi < ymax; ++yi) {
+          const int y = reverse ? ymax - 1 - yi : yi;
+          const int lastrow = y + (horizontal ? 0 : 1);
+          const Boundary<T>& N = north[y];
+          const Boundary<T>& S = south[lastrow + 1];
+          const size_t mark = ctx.arena.mark();
+          auto take = [&](long long elems) {
+            return (T*)ctx.arena.take(sizeof(T) * (size_t)cur * (size_t)std::max(1LL, elems));
+          };
+          auto normalize = [&](const TRef& tr, int site) {
+            FullView fv = full_view(tr);
+            SPP_CHECK(launch_dot<T>(fv, fv, nrm, cur, st));
+            SPP_CHECK(launch_safe_factor(nrm, fac, status, fail, site, cur, st));
+            SPP_CHECK(launch_scale<T>(fv, fac, cur, st));
+          };
+          auto lay = [&](int site) {
+            const LayerSpec& sp = spec[site];
+            TRef r = dense_ref(nullptr, 0, "wnes", {sp.w, sp.n, sp.e, sp.s});
+            r.ptrs = table(site);
+            return r;
+          };
+          auto lay_l = [&](int site, const char* letters) { return with_letters(lay(site), letters); };
+          auto s1 = [&](int x) { return x + y * Lx; };
+          auto s2 = [&](int x) { return x + (y + 1) * Lx; };
+          // message storage: messages[x] for x = 0..Lx (the precomputed side)
+          std::vector<T*> Mb(Lx + 1);
+          std::vector<std::vector<Leg>> Ml(Lx + 1);
+          for (int x = 0; x <= Lx; ++x) {
+            if (horizontal) {
+              Leg wl = x < Lx ? spec[s1(x)].w : spec[s1(Lx - 1)].e;
+              Ml[x] = {N.bond(x), wl, south_bond(S, Lx, x)};
+            } else {
+              Leg w1 = x < Lx ? spec[s1(x)].w : spec[s1(Lx - 1)].e;
+              Leg w2 = x < Lx ? spec[s2(x)].w : spec[s2(Lx - 1)].e;
+              Ml[x] = {N.bond(x), w1, w2, south_bond(S, Lx, x)};
+            }
+            Mb[x] = take(count_legs(Ml[x]));
+          }
+          const long long bigm = std::max((long long)chi * D * D * chi * (horizontal ? 1 : D), 1LL);
+          T* BX = take(bigm);
+          T* BY = take(bigm);
+          T* Roll = take(horizontal ? (long long)chi * D * chi : (long long)chi * D * D * chi);
+          T* Rnew = take(horizontal ? (long long)chi * D * chi : (long long)chi * D * D * chi);
+          const char* ml = horizontal ? "awb" : "awvb";
+          // one right step (messages covering columns >= x) from Rin (covering > x)
+          auto right_step = [&](int x, const TRef& Rin, T* outbuf, std::vector<Leg>& ol) -> TRef {
+            if (horizontal) {
+              const LayerSpec& sp = spec[s1(x)];
+              std::vector<Leg> xl = {N.bond(x), Rin.leg[2], N.phys(x), Rin.leg[1]};
+              TRef Xr = folded_ref(BX, count_legs(xl), "adne", xl);
+              contract<T>(ctx, Xr, N.ref(x, "anc"), with_letters(Rin, "ced"), cur);
+              std::vector<Leg> yl = {N.bond(x), Rin.leg[2], sp.w, sp.s};
+              TRef Yr = folded_ref(BY, count_legs(yl), "adws", yl);
+              contract<T>(ctx, Yr, Xr, lay(s1(x)), cur);
+              ol = {N.bond(x), sp.w, south_bond(S, Lx, x)};
+              TRef Or = dense_ref(outbuf, count_legs(ol), "awb", ol);
+              contract<T>(ctx, Or, Yr, bottom_ref(S, Lx, x, "bsd", true), cur);
+              return Or;
+            }
+            const LayerSpec& u = spec[s1(x)];
+            const LayerSpec& l = spec[s2(x)];
+            const Leg dd = Rin.leg[3];
+            std::vector<Leg> xl = {N.bond(x), u.n, dd, Rin.leg[1], Rin.leg[2]};
+            TRef Xr = folded_ref(BX, count_legs(xl), "andef", xl);
+            contract<T>(ctx, Xr, N.ref(x, "anc"), with_letters(Rin, "cefd"), cur);
+            std::vector<Leg> yl = {N.bond(x), u.w, dd, u.s, Rin.leg[2]};
+            TRef Yr = folded_ref(BY, count_legs(yl), "awdkf", yl);
+            contract<T>(ctx, Yr, Xr, lay_l(s1(x), "wnek"), cur);
+            std::vector<Leg> zl = {N.bond(x), u.w, dd, l.w, l.s};
+            TRef Zr = folded_ref(BX, count_legs(zl), "awdvs", zl);
+            contract<T>(ctx, Zr, Yr, lay_l(s2(x), "vkfs"), cur);
+            ol = {N.bond(x), u.w, l.w, south_bond(S, Lx, x)};
+            TRef Or = dense_ref(outbuf, count_legs(ol), "awvb", ol);
+            contract<T>(ctx, Or, Zr, bottom_ref(S, Lx, x, "bsd", true), cur);
+            return Or;
+          };
+          // one left step (message covering columns <= x) from Lin (covering < x)
+          auto left_step = [&](const TRef& Lin, int x, T* outbuf, std::vector<Leg>& ol) -> TRef {
+            if (horizontal) {
+              const LayerSpec& sp = spec[s1(x)];
+              std::vector<Leg> xl = {Lin.leg[1], Lin.leg[2], N.bond(x + 1), N.phys(x)};
+              TRef Xr = folded_ref(BX, count_legs(xl), "wbcn", xl);
+              contract<T>(ctx, Xr, with_letters(Lin, "awb"), N.ref(x, "anc"), cur);
+              std::vector<Leg> yl = {Lin.leg[2], N.bond(x + 1), sp.e, sp.s};
+              TRef Yr = folded_ref(BY, count_legs(yl), "bces", yl);
+              contract<T>(ctx, Yr, Xr, lay(s1(x)), cur);
+              ol = {N.bond(x + 1), sp.e, south_bond(S, Lx, x + 1)};
+              TRef Or = dense_ref(outbuf, count_legs(ol), "ced", ol);
+              contract<T>(ctx, Or, Yr, bottom_ref(S, Lx, x, "bsd", true), cur);
+              return with_letters(Or, "awb");
+            }
+            const LayerSpec& u = spec[s1(x)];
+            const LayerSpec& l = spec[s2(x)];
+            const Leg b = Lin.leg[3], c = N.bond(x + 1);
+            std::vector<Leg> xl = {Lin.leg[1], Lin.leg[2], b, c, N.phys(x)};
+            TRef Xr = folded_ref(BX, count_legs(xl), "wvbcn", xl);
+            contract<T>(ctx, Xr, with_letters(Lin, "awvb"), N.ref(x, "anc"), cur);
+            std::vector<Leg> yl = {Lin.leg[2], b, c, u.e, u.s};
+            TRef Yr = folded_ref(BY, count_legs(yl), "vbcek", yl);
+            contract<T>(ctx, Yr, Xr, lay_l(s1(x), "wnek"), cur);
+            std::vector<Leg> zl = {b, c, u.e, l.e, l.s};
+            TRef Zr = folded_ref(BX, count_legs(zl), "bcefs", zl);
+            contract<T>(ctx, Zr, Yr, lay_l(s2(x), "vkfs"), cur);
+            ol = {c, u.e, l.e, south_bond(S, Lx, x + 1)};
+            TRef Or = dense_ref(outbuf, count_legs(ol), "cefd", ol);
+            contract<T>(ctx, Or, Zr, bottom_ref(S, Lx, x, "bsd", true), cur);
+            return with_letters(Or, "awvb");
+          };
+          auto Mref = [&](int x) { return dense_ref(Mb[x], count_legs(Ml[x]), ml, Ml[x]); };
+          // precomputed messages: right (forward) or left (reverse)
+          if (!reverse) {
+            SPP_CHECK(launch_fill<T>(Mb[Lx], cur, T(1), st));
+            for (int x = Lx - 1; x >= 0; --x) {
+              std::vector<Leg> ol;
+              TRef Or = right_step(x, Mref(x + 1), Mb[x], ol);
+              normalize(Or, s1(x));
+            }
+          } else {
+            SPP_CHECK(launch_fill<T>(Mb[0], cur, T(1), st));
+            for (int x = 0; x < Lx; ++x) {
+              std::vector<Leg> ol;
+              TRef Or = left_step(Mref(x), x, Mb[x + 1], ol);
+              normalize(Or, s1(x));
+            }
+          }
+          // rolling message starts at the trivial edge
+          std::vector<Leg> rl = reverse ? Ml[Lx] : Ml[0];
+          SPP_CHECK(launch_fill<T>(Roll, cur, T(1), st));
+          for (int xi = 0; xi < xmax; ++xi) {
+            const int x = reverse ? xmax - 1 - xi : xi;
+            const int ia = s1(x), ib = horizontal ? s1(x + 1) : s2(x);
+            const int finalx = horizontal ? x + 1 : x;
+            const BondGeo& G = horizontal ? hb[y * (Lx - 1) + x] : vb[y * Lx + x];
+            const double* gate = gate_dev + (size_t)(horizontal ? y * (Lx - 1) + x : nh + y * Lx + x) * d * d * d * d;
+            TRef leftm = reverse ? Mref(x) : dense_ref(Roll, count_legs(rl), ml, rl);
+            TRef rightm = reverse ? dense_ref(Roll, count_legs(rl), ml, rl) : Mref(finalx + 1);
+            // --- reduced pair ---
+            {
+              std::vector<PairJob> pj(cur);
+              std::vector<RestoreJob> rj(cur);
+              for (int s = 0; s < cur; ++s) {
+                pj[s].site[0] = states[c0 + s]->sites[ia].d;
+                pj[s].site[1] = states[c0 + s]->sites[ib].d;
+                rj[s].site[0] = states[c0 + s]->sites[ia].d;
+                rj[s].site[1] = states[c0 + s]->sites[ib].d;
+              }
+              // live pointers resolved on the device side: per-sample offsets into `live`
+              std::vector<PairJob> pjd(cur);
+              for (int s = 0; s < cur; ++s) {
+                pjd[s] = pj[s];
+                pjd[s].live[0] = live + ((size_t)s * ns + ia) * 8;
+                pjd[s].live[1] = live + ((size_t)s * ns + ib) * 8;
+              }
+              // the previous bond's restore kernel reads rjobs: wait before rewriting the tables
+              SPP_CHECK(hipStreamSynchronize(st));
+              SPP_CHECK(hipMemcpyAsync(pjobs, pjd.data(), sizeof(PairJob) * cur, hipMemcpyHostToDevice, st));
+              SPP_CHECK(hipMemcpyAsync(rjobs, rj.data(), sizeof(RestoreJob) * cur, hipMemcpyHostToDevice, st));
+            }
+            hipLaunchKernelGGL(fu_pair_kernel, dim3((cur + 63) / 64), dim3(64), 0, st, pjobs, cur, G.sa, G.sb, gate, d,
+                               ed, qa, qmax, qbuf, qmax, ca, cb, tgt, rlive, pwork, pairwork, nb);
+            SPP_CHECK(hipGetLastError());
+            // --- reduced layers of the Q pseudo-sites ---
+            {
+              std::vector<LayerJob> jobs;
+              const FuSide* sides[2] = {&G.sa, &G.sb};
+              double* qs[2] = {qa, qbuf};
+              T* outs[2] = {lred_a, lred_b};
+              const long long lsz[2] = {G.la_size, G.lb_size};
+              for (int side = 0; side < 2; ++side) {
+                const FuSide& F = *sides[side];
+                int pdm[6], pev[6];
+                for (int k = 0; k < 6; ++k) {
+                  pdm[k] = F.leg[k];
+                  pev[k] = F.even[k];
+                }
+                pdm[2] = F.leg[3];
+                pev[2] = F.even[3];
+                pdm[3] = 1;
+                pev[3] = 1;
+                pdm[F.core] = F.rcap;
+                pev[F.core] = F.recap;
+                for (int s = 0; s < cur; ++s) {
+                  LayerJob J{};
+                  J.ket = qs[side] + (long long)s * qmax;
+                  J.bra = J.ket;
+                  for (int k = 0; k < 6; ++k) {
+                    J.dk[k] = J.db[k] = pdm[k];
+                    J.ek[k] = J.eb[k] = pev[k];
+                  }
+                  J.out = outs[side] + (long long)s * lsz[side];
+                  jobs.push_back(J);
+                }
+              }
+              SPP_CHECK(hipStreamSynchronize(st));
+              SPP_CHECK(hipMemcpyAsync(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), hipMemcpyHostToDevice, st));
+              SPP_CHECK(launch_double_layers<T>(djobs, (int)jobs.size(), std::max(G.la_size, G.lb_size), st));
+            }
+            // --- bond metric N[R, S] (R = (rk, rb) pairs of A, S of B) ---
+            const Leg Ra = pair_leg(G.sa.rcap, G.sa.recap), Sb = pair_leg(G.sb.rcap, G.sb.recap);
+            const long long nfsz = (long long)Ra.dim * Sb.dim;
+            {
+              const size_t m2 = ctx.arena.mark();
+              TRef Nr = dense_ref(nfused, nfsz, "RS", {Ra, Sb});
+              if (horizontal) {
+                const LayerSpec& A_ = G.la;  // (W2, N2, R2, S2)
+                const LayerSpec& B_ = G.lb;  // (R2', N2, E2, S2)
+                TRef Ar = dense_ref(lred_a, G.la_size, "wnRs", {A_.w, A_.n, A_.e, A_.s});
+                TRef Br = dense_ref(lred_b, G.lb_size, "SveT", {B_.w, B_.n, B_.e, B_.s});
+                const Leg lb = leftm.leg[2], la = leftm.leg[0];
+                std::vector<Leg> xl = {lb, N.phys(x), N.bond(x + 1), leftm.leg[1]};
+                T* X = take(count_legs(xl));
+                TRef Xr = dense_ref(X, count_legs(xl), "bncw", xl);
+                contract<T>(ctx, Xr, with_letters(leftm, "awb"), N.ref(x, "anc"), cur);
+                (void)la;
+                std::vector<Leg> yl = {lb, N.bond(x + 1), Ra, A_.s};
+                T* Y = take(count_legs(yl));
+                TRef Yr = dense_ref(Y, count_legs(yl), "bcRs", yl);
+                contract<T>(ctx, Yr, Xr, Ar, cur);
+                std::vector<Leg> lm = {N.bond(x + 1), south_bond(S, Lx, x + 1), Ra};
+                T* Lm = take(count_legs(lm));
+                TRef Lr = dense_ref(Lm, count_legs(lm), "cfR", lm);
+                contract<T>(ctx, Lr, Yr, bottom_ref(S, Lx, x, "bsf", true), cur);
+                std::vector<Leg> ul = {N.bond(x + 1), N.phys(x + 1), rightm.leg[1], rightm.leg[2]};
+                T* U = take(count_legs(ul));
+                TRef Ur = dense_ref(U, count_legs(ul), "cveg", ul);
+                contract<T>(ctx, Ur, with_letters(rightm, "deg"), N.ref(x + 1, "cvd"), cur);
+                std::vector<Leg> vl = {N.bond(x + 1), rightm.leg[2], Sb, B_.s};
+                T* V = take(count_legs(vl));
+                TRef Vr = dense_ref(V, count_legs(vl), "cgST", vl);
+                contract<T>(ctx, Vr, Ur, Br, cur);
+                std::vector<Leg> rm = {N.bond(x + 1), south_bond(S, Lx, x + 1), Sb};
+                T* Rm = take(count_legs(rm));
+                TRef Rr = dense_ref(Rm, count_legs(rm), "cfS", rm);
+                contract<T>(ctx, Rr, Vr, bottom_ref(S, Lx, x + 1, "fTg", true), cur);
+                contract<T>(ctx, Nr, Lr, Rr, cur);
+              } else {
+                const LayerSpec& A_ = G.la;  // (W2, N2, E2, R2)
+                const LayerSpec& B_ = G.lb;  // (W2, R2', E2, S2)
+                TRef Ar = dense_ref(lred_a, G.la_size, "wneR", {A_.w, A_.n, A_.e, A_.s});
+                TRef Br = dense_ref(lred_b, G.lb_size, "vSfs", {B_.w, B_.n, B_.e, B_.s});
+                // Rb[c,e,b,v,S] = sum_{f,s} (right[c,e,f,d] bottom[b,s,d]) Bred[v,S,f,s]
+                std::vector<Leg> rbl = {rightm.leg[0], rightm.leg[1], leftm.leg[3], B_.w, Sb};
+                T* Rb = take(count_legs(rbl));
+                const size_t mg = ctx.arena.mark();
+                std::vector<Leg> g1 = {rightm.leg[0], rightm.leg[1], rightm.leg[2], leftm.leg[3], B_.s};
+                T* G1 = take(count_legs(g1));
+                TRef G1r = dense_ref(G1, count_legs(g1), "cefbs", g1);
+                contract<T>(ctx, G1r, with_letters(rightm, "cefd"), bottom_ref(S, Lx, x, "bsd", true), cur);
+                TRef Rbr = dense_ref(Rb, count_legs(rbl), "cebvS", rbl);
+                contract<T>(ctx, Rbr, G1r, Br, cur);
+                ctx.arena.release(mg);  // G1 is no longer needed
+                std::vector<Leg> xl = {leftm.leg[1], leftm.leg[2], leftm.leg[3], N.phys(x), N.bond(x + 1)};
+                T* X = take(count_legs(xl));
+                TRef Xr = dense_ref(X, count_legs(xl), "wvbnc", xl);
+                contract<T>(ctx, Xr, with_letters(leftm, "awvb"), N.ref(x, "anc"), cur);
+                std::vector<Leg> ql = {A_.w, A_.n, A_.e, Sb};
+                T* Qt = take(count_legs(ql));
+                TRef Qr = dense_ref(Qt, count_legs(ql), "wneS", ql);
+                contract<T>(ctx, Qr, Xr, Rbr, cur);
+                contract<T>(ctx, Nr, Ar, Qr, cur);
+              }
+              ctx.arena.release(m2);
+            }
+            // --- the bond solve ---
+            const fu::Opts o = P.opts;
+            hipLaunchKernelGGL(fu_core_kernel<T>, dim3((cur + 63) / 64), dim3(64), 0, st, nfused, nfsz, tgt, rlive,
+                               G.sa.rcap, G.sa.recap, G.sb.rcap, G.sb.recap, d, ed, G.Dcap, G.eDcap, o, aout, bout, drec,
+                               blive, cwork, coremax, cur, nb);
+            SPP_CHECK(hipGetLastError());
+            SPP_CHECK(hipMemcpyAsync(hrec.data(), drec, sizeof(double) * 12 * cur, hipMemcpyDeviceToHost, st));
+            SPP_CHECK(hipStreamSynchronize(st));
+            for (int s = 0; s < cur; ++s) {
+              const int code = (int)hrec[(size_t)s * 12 + 7];
+              if (code != 0) {
+                static const char* msg[] = {"ok", "nonfinite FU norm metric", "zero FU norm metric",
+                                            "gate produced a zero target", "PSD repair of the FU metric needed",
+                                            "nonfinite FU residual/target", "singular FU gauge", "FU workspace"};
+                throw HipError(std::string("full update: ") + msg[std::min(code, 7)]);
+              }
+              if (out.records) {
+                double* rr = out.records + ((size_t)(c0 + s) * nrec_bonds + rec_bond) * 8;
+                rr[0] = hrec[(size_t)s * 12 + 0];
+                rr[1] = hrec[(size_t)s * 12 + 1];
+                rr[2] = hrec[(size_t)s * 12 + 2];
+                rr[3] = hrec[(size_t)s * 12 + 4];
+                rr[4] = hrec[(size_t)s * 12 + 5];
+                rr[5] = hrec[(size_t)s * 12 + 6];
+                rr[6] = hrec[(size_t)s * 12 + 8];
+                rr[7] = hrec[(size_t)s * 12 + 9];
+              }
+            }
+            ++rec_bond;
+            // --- restore the sites, their bond bookkeeping and layers ---
+            {
+              long long tmax = std::max(s0.sites[ia].size(), s0.sites[ib].size());
+              dim3 grid((unsigned)((tmax + 255) / 256), (unsigned)cur, 2);
+              hipLaunchKernelGGL(fu_restore_kernel, grid, dim3(256), 0, st, rjobs, cur, G.sa, G.sb, d, qa, qmax, qbuf,
+                                 qmax, aout, bout, G.Dcap, nb);
+              SPP_CHECK(hipGetLastError());
+              const int va = horizontal ? 2 : 3, vbl = horizontal ? 0 : 1;
+              hipLaunchKernelGGL(fu_set_live_kernel, dim3((cur + 63) / 64), dim3(64), 0, st, live, ns, ia, va, ib, vbl,
+                                 blive, cur);
+              SPP_CHECK(hipGetLastError());
+              relayer({ia, ib});
+            }
+            // --- rolling message ---
+            const int column = reverse ? finalx : x;
+            if (x != (reverse ? 0 : xmax - 1)) {
+              std::vector<Leg> ol;
+              TRef Rin = dense_ref(Roll, count_legs(rl), ml, rl);
+              TRef Or = reverse ? right_step(column, Rin, Rnew, ol) : left_step(Rin, column, Rnew, ol);
+              normalize(Or, s1(column));
+              std::swap(Roll, Rnew);
+              rl = ol;
+            }
+          }
+          ctx.arena.release(mark);
+          if (reverse) absorb_row(lastrow, true);
+          else absorb_row(y, false);
+        }
+        // environment errors: the far side's chain and the incremental one
+        const double* ends[2] = {reverse ? north[Ly].max_trunc : south[0].max_trunc,
+                                 reverse ? south[horizontal ? 0 : 1].max_trunc : north[ymax].max_trunc};
+        for (const double* e : ends) {
+          SPP_CHECK(hipMemcpyAsync(hmt.data(), e, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+          SPP_CHECK(hipStreamSynchronize(st));
+          for (int s = 0; s < cur; ++s) env_err[s] = std::max(env_err[s], hmt[s]);
+        }
+      }
+      SPP_CHECK(hipMemcpyAsync(hstatus.data(), status, sizeof(int) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipStreamSynchronize(st));
+      for (int s = 0; s < cur; ++s)
+        if (hstatus[s] != 0) throw HipError("full update: nonfinite row environment");
+      // normalize_state! (finite.jl:89-100): site norms into log_amplitude, row-major sites
+      {
+        std::vector<double*> hp((size_t)ns * cur);
+        for (int i = 0; i < ns; ++i)
+          for (int s = 0; s < cur; ++s) hp[(size_t)i * cur + s] = states[c0 + s]->sites[i].d;
+        double** dp = (double**)ctx.arena.take(sizeof(double*) * hp.size());
+        double* dn = (double*)ctx.arena.take(sizeof(double) * hp.size());
+        SPP_CHECK(hipMemcpyAsync(dp, hp.data(), sizeof(double*) * hp.size(), hipMemcpyHostToDevice, st));
+        std::vector<long long> sizes(ns);
+        for (int i = 0; i < ns; ++i) sizes[i] = s0.sites[i].size();
+        for (int i = 0; i < ns; ++i) {
+          hipLaunchKernelGGL(site_sumsq_kernel, dim3(cur), dim3(256), 0, st, dp + (size_t)i * cur, sizes[i],
+                             dn + (size_t)i * cur);
+          SPP_CHECK(hipGetLastError());
+        }
+        std::vector<double> hn(hp.size());
+        SPP_CHECK(hipMemcpyAsync(hn.data(), dn, sizeof(double) * hn.size(), hipMemcpyDeviceToHost, st));
+        SPP_CHECK(hipStreamSynchronize(st));
+        for (size_t k = 0; k < hn.size(); ++k) {
+          const double v = std::sqrt(hn[k]);
+          if (!(v > 0.0 && std::isfinite(v))) throw HipError("zero or nonfinite tensor after the full update");
+          hn[k] = v;
+        }
+        for (int i = 0; i < ns; ++i)
+          for (int s = 0; s < cur; ++s) {
+            const double v = hn[(size_t)i * cur + s];
+            states[c0 + s]->log_amplitude += std::log(v);
+            if (out.normalizations) out.normalizations[(size_t)(c0 + s) * ns + i] = std::log(v);
+            hn[(size_t)i * cur + s] = 1.0 / v;
+          }
+        SPP_CHECK(hipMemcpyAsync(dn, hn.data(), sizeof(double) * hn.size(), hipMemcpyHostToDevice, st));
+        for (int i = 0; i < ns; ++i) {
+          hipLaunchKernelGGL(site_scale_kernel, dim3(cur, (unsigned)std::min<long long>((sizes[i] + 255) / 256, 64)),
+                             dim3(256), 0, st, dp + (size_t)i * cur, sizes[i], dn + (size_t)i * cur);
+          SPP_CHECK(hipGetLastError());
+        }
+        SPP_CHECK(hipStreamSynchronize(st));
+        ctx.arena.release(0);
+      }
+      // bond bookkeeping back to the states
+      {
+        std::vector<int> hl((size_t)ns * 8 * nb);
+        SPP_CHECK(hipMemcpyAsync(hl.data(), live, sizeof(int) * hl.size(), hipMemcpyDeviceToHost, st));
+        SPP_CHECK(hipStreamSynchronize(st));
+        for (int s = 0; s < cur; ++s)
+          std::memcpy(states[c0 + s]->live.data(), hl.data() + (size_t)s * ns * 8, sizeof(int) * ns * 8);
+      }
+      for (int s = 0; s < cur; ++s) {
+        states[c0 + s]->beta += 2.0 * P.delta;
+        if (out.max_env_trunc) out.max_env_trunc[c0 + s] = env_err[s];
+      }
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(st);
+    ctx.arena.release(0);
+    cleanup();
+    throw;
+  }
+  SPP_CHECK(hipStreamSynchronize(st));
+  cleanup();
+}
+
+template void fu_evolution_step<float>(Context&, const std::vector<StateDev*>&, int, const FuParams&, FuOutputs&);
+template void fu_evolution_step<double>(Context&, const std::vector<StateDev*>&, int, const FuParams&, FuOutputs&);
 
 }  // namespace spp

@@ -19,6 +19,9 @@ struct StateDev {
   int Lx = 0, Ly = 0, kind = 0;  // kind 0: PEPO, 1: PEPS
   std::vector<SiteDev> sites;    // x + y*Lx
   double beta = 0, log_amplitude = 0;
+  // per site, per virtual leg (W, N, E, S): live even / odd counts of the
+  // bond (the sectors in use; the rest of the leg's capacity is zero)
+  std::vector<int> live;
   ~StateDev();
 };
 

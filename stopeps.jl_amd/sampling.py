@@ -101,7 +101,7 @@ class Context:
 class DeviceState:
     """A ``FiniteState`` uploaded to the device (``spp_state_upload``)."""
 
-    def __init__(self, ctx, state):
+    def __init__(self, ctx, state, live=None):
         self.ctx = ctx
         self.state = state
         L = state.lattice
@@ -121,6 +121,38 @@ class DeviceState:
                                                          _lib.iptr(evens), ptrs, state.beta,
                                                          state.log_amplitude, ctypes.byref(h)))
         self.handle = h
+        if live is not None:
+            lv = np.ascontiguousarray(live, dtype=np.int32).reshape(-1)
+            _lib.check(ctx.handle, ctx._lib.spp_state_set_live(ctx.handle, h, _lib.iptr(lv)))
+
+    def live(self):
+        """Live (even, odd) counts per site and virtual leg W, N, E, S: [Lx*Ly, 4, 2]."""
+        L = self.state.lattice
+        lv = np.zeros(L.Lx * L.Ly * 8, dtype=np.int32)
+        bufs = [np.empty_like(a) for a in self._arrays]
+        ptrs = (_lib.c_double_p * len(bufs))(*[_lib.dptr(b) for b in bufs])
+        _lib.check(self.ctx.handle, self.ctx._lib.spp_state_download(self.ctx.handle, self.handle, ptrs, None, None,
+                                                                     _lib.iptr(lv)))
+        return lv.reshape(L.Lx * L.Ly, 4, 2)
+
+    def download(self):
+        """The device state as a ``FiniteState`` in the leg order of the uploaded one."""
+        L = self.state.lattice
+        bufs = [np.empty_like(a) for a in self._arrays]
+        ptrs = (_lib.c_double_p * len(bufs))(*[_lib.dptr(b) for b in bufs])
+        beta = ctypes.c_double()
+        la = ctypes.c_double()
+        _lib.check(self.ctx.handle, self.ctx._lib.spp_state_download(self.ctx.handle, self.handle, ptrs,
+                                                                     ctypes.byref(beta), ctypes.byref(la), None))
+        sites = [[None] * L.Ly for _ in range(L.Lx)]
+        for b, (x, y) in zip(bufs, L.site_order()):
+            src = self.state[(x, y)]
+            a = b
+            for k, p in enumerate(src.parity):
+                order = np.argsort(p, kind="stable")
+                a = np.take(a, np.argsort(order), axis=k)
+            sites[x][y] = Site(np.ascontiguousarray(a), src.parity)
+        return FiniteState(L, sites, self.state.kind, beta=beta.value, log_amplitude=la.value)
 
     def __del__(self):
         try:

@@ -159,6 +159,39 @@ def random_peps(lattice, D=(3, 3), seed=7, kind="peps"):
     return FiniteState(lattice, sites, kind)
 
 
+def pad_bonds(state, capacity):
+    """Embed every interior virtual leg in a bond space of (even, odd) =
+    ``capacity`` by appending zero slots (the original entries come first in
+    each parity, so they are the live ones on the device). Returns the padded
+    state and the live counts [Lx*Ly, 4, 2] for ``DeviceState(..., live=...)``."""
+    L = state.lattice
+    ce, co = capacity
+    legs = (0, 1, 4, 5)
+    sites = [[None] * L.Ly for _ in range(L.Lx)]
+    live = np.zeros((L.Lx * L.Ly, 4, 2), dtype=np.int32)
+    for (x, y) in L.site_order():
+        s = state[(x, y)]
+        a = s.data
+        par = list(s.parity)
+        for v, k in enumerate(legs):
+            p = par[k]
+            e, o = int(np.sum(p == 0)), int(np.sum(p == 1))
+            live[x + y * L.Lx, v] = (e, o)
+            if len(p) == 1 and ((k == 0 and x == 0) or (k == 1 and y == 0) or (k == 4 and x == L.Lx - 1)
+                                or (k == 5 and y == L.Ly - 1)):
+                continue
+            if e > ce or o > co:
+                raise ValueError("bond space exceeds the capacity")
+            extra = np.array([0] * (ce - e) + [1] * (co - o), dtype=np.int64)
+            if len(extra):
+                shape = list(a.shape)
+                shape[k] = len(extra)
+                a = np.concatenate([a, np.zeros(shape)], axis=k)
+                par[k] = np.concatenate([p, extra])
+        sites[x][y] = Site(a, par)
+    return FiniteState(L, sites, state.kind, beta=state.beta, log_amplitude=state.log_amplitude), live
+
+
 def parity_sorted(site):
     """Dense array with every leg sorted by parity (stable), and even counts."""
     a = site.data

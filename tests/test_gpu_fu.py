@@ -1,0 +1,115 @@
+"""GPU parity of the full-update evolution step (spp_fu_evolution_step,
+through the C ABI) against the FP64 oracle's evolution_step: the evolved
+state as a dense vector (gauge invariant), the per-bond FU costs in sweep
+order, and the accumulated log amplitude."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import to_product_state
+from oracle.model import Lattice, TFIM, State, random_state
+from oracle.z2 import Z2
+from oracle import fu as ofu
+from test_oracle_fu import dense_vector
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def to_oracle(spp_state):
+    L = spp_state.lattice
+    sites = [[Z2(spp_state[(x, y)].data, spp_state[(x, y)].parity) for y in range(L.Ly)] for x in range(L.Lx)]
+    return State(Lattice(L.Lx, L.Ly), sites, spp_state.kind, beta=spp_state.beta,
+                 log_amplitude=spp_state.log_amplitude)
+
+
+def _compare(spp, ctx, ostates, m, delta, chi, caps, opts, pad=None, steps=1, tol=1e-8):
+    ctx.set_precision("fp64")
+    devs = []
+    for o in ostates:
+        s = to_product_state(spp, o)
+        if pad is not None:
+            s, live = spp.pad_bonds(s, pad)
+            devs.append(spp.DeviceState(ctx, s, live=live))
+        else:
+            devs.append(spp.DeviceState(ctx, s))
+    model = spp.TFIM(m.J, m.h, symmetric=m.symmetric)
+    method = spp.FullUpdate(opts.max_iterations, opts.error_tolerance, opts.diagonal_shift,
+                            opts.linear_max_iterations, opts.linear_tolerance)
+    D = sum(caps)
+    for step in range(steps):
+        got = spp.evolution_steps(ctx, devs, model, delta, chi, method)
+        for k, o in enumerate(ostates):
+            ref = ofu.evolution_step(o, m, delta, D, caps, opts, chi)
+            costs = np.array([r["final_cost"] for r in ref["records"]])
+            assert got["records"].shape[1] == len(costs)
+            # relative squared residuals: below ~1e-9 they sit at the linear-solve / roundoff floor
+            np.testing.assert_allclose(got["records"][k, :, 1], costs, rtol=1e-6, atol=1e-9)
+            its = np.array([r["iterations"] for r in ref["records"]])
+            assert np.mean(got["records"][k, :, 3] == its) > 0.9
+            ev = to_oracle(devs[k].download())
+            v_dev, v_ref = dense_vector(ev), dense_vector(o)
+            err = np.linalg.norm(v_dev - v_ref) / np.linalg.norm(v_ref)
+            say(f"  step {step} state {k}: |dpsi|/|psi| = {err:.2e}, max FU cost {costs.max():.3e}")
+            assert err <= tol
+            assert abs(ev.log_amplitude - o.log_amplitude) <= 1e-8 * max(1.0, abs(o.log_amplitude))
+            assert abs(ev.beta - o.beta) < 1e-14
+    return devs
+
+
+def test_fu_step_2x2_peps(spp, gpu_ctx):
+    say("FU step 2x2 Z2 PEPS D=(1,1)")
+    m = TFIM(J=1.0, h=0.9, symmetric=True)
+    _compare(spp, gpu_ctx, [random_state(Lattice(2, 2), (1, 1), "peps", seed=21)], m, 0.1, 64, (1, 1),
+             ofu.FullUpdate())
+
+
+def test_fu_step_3x3_batch(spp, gpu_ctx):
+    say("FU step 3x3 Z2 PEPS D=(1,1), batch of 3, 40 ALS iterations")
+    m = TFIM(J=1.0, h=2.0, symmetric=True)
+    st = [random_state(Lattice(3, 3), (1, 1), "peps", seed=22 + k) for k in range(3)]
+    _compare(spp, gpu_ctx, st, m, 0.1, 64, (1, 1), ofu.FullUpdate(max_iterations=40, linear_max_iterations=100))
+
+
+def test_fu_step_padded_capacity_two_steps(spp, gpu_ctx):
+    """Bonds uploaded at capacity (1,2) while the state lives in (1,1): the
+    device works in the live sectors exactly as the reference's growing bond
+    space. Truncation binds (costs 1e-5..1e-2); the configuration is one whose
+    oracle costs move by < 1e-11 under 1e-13 input noise (many small FU
+    problems are not: a flat ALS landscape or metric eigenvalues at the gauge
+    floor make the result depend on roundoff, in the reference as well)."""
+    say("FU two steps 3x2 Z2 PEPS live (1,1) in capacity (1,2)")
+    m = TFIM(J=1.0, h=2.0, symmetric=True)
+    _compare(spp, gpu_ctx, [random_state(Lattice(3, 2), (1, 1), "peps", seed=35)], m, 0.3, 64, (1, 2),
+             ofu.FullUpdate(max_iterations=40, linear_max_iterations=200), pad=(1, 2), steps=2)
+
+
+def test_fu_step_pepo(spp, gpu_ctx):
+    say("FU step 2x2 Z2 PEPO D=(1,1) (purification)")
+    m = TFIM(J=1.0, h=0.7, symmetric=True)
+    st = random_state(Lattice(2, 2), (1, 1), "pepo", seed=26)
+    ctx = gpu_ctx
+    ctx.set_precision("fp64")
+    dev = spp.DeviceState(ctx, to_product_state(spp, st))
+    got = spp.evolution_step(ctx, dev, spp.TFIM(m.J, m.h, symmetric=True), 0.05, 64)
+    ref = ofu.evolution_step(st, m, 0.05, 2, (1, 1), ofu.FullUpdate(), 64)
+    np.testing.assert_allclose([r["final_cost"] for r in got["records"]],
+                               [r["final_cost"] for r in ref["records"]], rtol=1e-6, atol=1e-9)
+    ev = to_oracle(dev.download())
+    # PEPO amplitude over (p, a) pairs
+    from test_oracle_pins import dense_network
+    a1 = dense_network(ev) * np.exp(ev.log_amplitude)
+    a2 = dense_network(st) * np.exp(st.log_amplitude)
+    assert np.linalg.norm(a1 - a2) <= 1e-8 * np.linalg.norm(a2)
+
+
+def test_fu_step_dense_states(spp, gpu_ctx):
+    say("FU step 2x3 dense PEPS D=2")
+    m = TFIM(J=1.0, h=0.8, symmetric=False)
+    _compare(spp, gpu_ctx, [random_state(Lattice(2, 3), 2, "peps", seed=27, symmetric=False)], m, 0.1, 64, (2, 0),
+             ofu.FullUpdate(max_iterations=20, linear_max_iterations=60))
