@@ -29,7 +29,8 @@ fu::Result fu_bond_host(const fu::Dims& g, const fu::Opts& o, const double* N4, 
   std::vector<double> ws((size_t)cap);
   fu::Work w{ws.data(), 1, 0, cap};
   fu::View n{const_cast<double*>(N4), 1}, t{const_cast<double*>(target), 1}, a{A, 1}, b{B, 1};
-  return fu::fu_bond(g, o, n, t, a, b, w);
+  const fu::Team one{0, 1, nullptr};
+  return fu::fu_bond(one, g, o, n, t, a, b, w);
 }
 
 namespace {
@@ -67,14 +68,16 @@ struct PairJob {
 
 FU_HD inline int vidx(int leg) { return leg == 0 ? 0 : leg == 1 ? 1 : leg == 4 ? 2 : 3; }
 
-// Householder QR of both sites of every sample (thread per sample) into
-// pseudo-site Q tensors and cores, then the gated target. Work arrays are
-// interleaved by sample (stride nbtot).
-FU_HD inline void fu_pair_sample(const PairJob& J, int s, const FuSide& sa, const FuSide& sb, const double* gate,
-                                  int d, int ed, double* qa, long long qa_stride, double* qb, long long qb_stride,
-                                  double* ca, double* cb, double* target, int* rlive, double* work,
-                                  long long work_per, int nbtot) {
-  fu::Work w{work + s, nbtot, 0, work_per};
+// Householder QR of both sites of one sample into pseudo-site Q tensors and
+// cores, then the gated target; solved by a Team of lanes (fu_core.h). The
+// sample's workspace is contiguous (work_per doubles); the cores and the
+// target are interleaved by sample (stride nbtot).
+FU_HD inline void fu_pair_sample(const fu::Team& t, const PairJob& J, int s, const FuSide& sa, const FuSide& sb,
+                                  const double* gate, int d, int ed, double* qa, long long qa_stride, double* qb,
+                                  long long qb_stride, double* ca, double* cb, double* target, int* rlive,
+                                  double* work, long long work_per, int nbtot) {
+  const int L = t.lane, NT = t.size;
+  fu::Work w{work + (long long)s * work_per, 1, 0, work_per};
   const FuSide* sides[2] = {&sa, &sb};
   double* qout[2] = {qa + (long long)s * qa_stride, qb + (long long)s * qb_stride};
   double* cout[2] = {ca, cb};
@@ -97,65 +100,61 @@ FU_HD inline void fu_pair_sample(const PairJob& J, int s, const FuSide& sa, cons
     for (int k = 4; k >= 0; --k) ps[k] = ps[k + 1] * pd[k + 1];
     double* Q = qout[side];
     const long long qsize = ps[0] * pd[0];
-    for (long long i = 0; i < qsize; ++i) Q[i] = 0.0;
+    for (long long i = L; i < qsize; i += NT) Q[i] = 0.0;
     const int ncore = F.leg[F.core];
     // cores C[r][p][k] (interleaved)
     fu::View C{cout[side] + s, nbtot};
-    for (long long i = 0; i < (long long)F.rcap * d * ncore; ++i) C[i] = 0.0;
+    for (long long i = L; i < (long long)F.rcap * d * ncore; i += NT) C[i] = 0.0;
+    t.sync();
     const int dv0 = F.leg[F.rowv[0]], dv1 = F.leg[F.rowv[1]], dv2 = F.leg[F.rowv[2]], dax = F.leg[3];
+    const int l0 = F.rowv[0], l1 = F.rowv[1], l2 = F.rowv[2];
+    auto row_live = [&](int i0, int i1, int i2) {
+      return live_slot(i0, F.even[l0], lv[2 * vidx(l0)], lv[2 * vidx(l0) + 1]) &&
+             live_slot(i1, F.even[l1], lv[2 * vidx(l1)], lv[2 * vidx(l1) + 1]) &&
+             live_slot(i2, F.even[l2], lv[2 * vidx(l2)], lv[2 * vidx(l2) + 1]);
+    };
+    auto row_par = [&](int i0, int i1, int i2, int ax) {
+      return fu::par(i0, F.even[l0]) ^ fu::par(i1, F.even[l1]) ^ fu::par(i2, F.even[l2]) ^ fu::par(ax, F.even[3]);
+    };
+    auto col_live = [&](int k) {
+      return live_slot(k, F.even[F.core], lv[2 * vidx(F.core)], lv[2 * vidx(F.core) + 1]);
+    };
     int rl[2] = {0, 0};
     for (int q = 0; q < 2; ++q) {
       const long long mark = w.top;
-      // live rows / columns of parity q
+      // live rows / columns of parity q (every lane counts)
       int mq = 0, nq = 0;
       for (int i0 = 0; i0 < dv0; ++i0)
         for (int i1 = 0; i1 < dv1; ++i1)
           for (int i2 = 0; i2 < dv2; ++i2)
-            for (int ax = 0; ax < dax; ++ax) {
-              const int l0 = F.rowv[0], l1 = F.rowv[1], l2 = F.rowv[2];
-              if (!live_slot(i0, F.even[l0], lv[2 * vidx(l0)], lv[2 * vidx(l0) + 1]) ||
-                  !live_slot(i1, F.even[l1], lv[2 * vidx(l1)], lv[2 * vidx(l1) + 1]) ||
-                  !live_slot(i2, F.even[l2], lv[2 * vidx(l2)], lv[2 * vidx(l2) + 1]))
-                continue;
-              const int pr = fu::par(i0, F.even[l0]) ^ fu::par(i1, F.even[l1]) ^ fu::par(i2, F.even[l2]) ^
-                             fu::par(ax, F.even[3]);
-              mq += pr == q;
-            }
+            for (int ax = 0; ax < dax; ++ax)
+              if (row_live(i0, i1, i2)) mq += row_par(i0, i1, i2, ax) == q;
       for (int p = 0; p < d; ++p)
-        for (int k = 0; k < ncore; ++k) {
-          if (!live_slot(k, F.even[F.core], lv[2 * vidx(F.core)], lv[2 * vidx(F.core) + 1])) continue;
-          nq += (fu::par(p, ed) ^ fu::par(k, F.even[F.core])) == q;
-        }
+        for (int k = 0; k < ncore; ++k)
+          if (col_live(k)) nq += (fu::par(p, ed) ^ fu::par(k, F.even[F.core])) == q;
       if (mq == 0 || nq == 0) continue;
       fu::View M = w.take((long long)mq * nq);
       fu::View rowid = w.take(mq), colid = w.take(nq);
-      {
+      if (L == 0) {
         int bi = 0;
         for (int i0 = 0; i0 < dv0; ++i0)
           for (int i1 = 0; i1 < dv1; ++i1)
             for (int i2 = 0; i2 < dv2; ++i2)
               for (int ax = 0; ax < dax; ++ax) {
-                const int l0 = F.rowv[0], l1 = F.rowv[1], l2 = F.rowv[2];
-                if (!live_slot(i0, F.even[l0], lv[2 * vidx(l0)], lv[2 * vidx(l0) + 1]) ||
-                    !live_slot(i1, F.even[l1], lv[2 * vidx(l1)], lv[2 * vidx(l1) + 1]) ||
-                    !live_slot(i2, F.even[l2], lv[2 * vidx(l2)], lv[2 * vidx(l2) + 1]))
-                  continue;
-                const int pr = fu::par(i0, F.even[l0]) ^ fu::par(i1, F.even[l1]) ^ fu::par(i2, F.even[l2]) ^
-                               fu::par(ax, F.even[3]);
-                if (pr != q) continue;
+                if (!row_live(i0, i1, i2) || row_par(i0, i1, i2, ax) != q) continue;
                 rowid[bi] = (double)(((i0 * dv1 + i1) * dv2 + i2) * dax + ax);
                 ++bi;
               }
         int bj = 0;
         for (int p = 0; p < d; ++p)
           for (int k = 0; k < ncore; ++k) {
-            if (!live_slot(k, F.even[F.core], lv[2 * vidx(F.core)], lv[2 * vidx(F.core) + 1])) continue;
-            if ((fu::par(p, ed) ^ fu::par(k, F.even[F.core])) != q) continue;
+            if (!col_live(k) || (fu::par(p, ed) ^ fu::par(k, F.even[F.core])) != q) continue;
             colid[bj] = (double)(p * ncore + k);
             ++bj;
           }
       }
-      for (int bi = 0; bi < mq; ++bi) {
+      t.sync();
+      for (int bi = L; bi < mq; bi += NT) {
         int rid = (int)rowid[bi];
         const int ax = rid % dax;
         rid /= dax;
@@ -174,10 +173,11 @@ FU_HD inline void fu_pair_sample(const PairJob& J, int s, const FuSide& sa, cons
       const int kq = mq < nq ? mq : nq;
       fu::View Qb = w.take((long long)mq * kq), Rb = w.take((long long)kq * nq);
       fu::View vs = w.take((long long)mq * kq), tau = w.take(kq);
-      fu::householder_qr(M, mq, nq, Qb, Rb, vs, tau);
+      t.sync();
+      fu::householder_qr(t, M, mq, nq, Qb, Rb, vs, tau);
       const int off = q == 0 ? 0 : F.recap;
       rl[q] = kq;
-      for (int bi = 0; bi < mq; ++bi) {
+      for (int bi = L; bi < mq; bi += NT) {
         int rid = (int)rowid[bi];
         const int ax = rid % dax;
         rid /= dax;
@@ -197,60 +197,73 @@ FU_HD inline void fu_pair_sample(const PairJob& J, int s, const FuSide& sa, cons
           Q[o] = Qb[(long long)bi * kq + c];
         }
       }
-      for (int c = 0; c < kq; ++c)
-        for (int bj = 0; bj < nq; ++bj) {
-          const int cid = (int)colid[bj];
-          const int p = cid / ncore, k = cid % ncore;
-          C[((long long)(off + c) * d + p) * ncore + k] = Rb[(long long)c * nq + bj];
-        }
+      for (long long idx = L; idx < (long long)kq * nq; idx += NT) {
+        const int c = (int)(idx / nq), bj = (int)(idx % nq);
+        const int cid = (int)colid[bj];
+        const int p = cid / ncore, k = cid % ncore;
+        C[((long long)(off + c) * d + p) * ncore + k] = Rb[idx];
+      }
+      t.sync();
       w.top = mark;
     }
-    rlive[(long long)s * 4 + side * 2] = rl[0];
-    rlive[(long long)s * 4 + side * 2 + 1] = rl[1];
+    if (L == 0) {
+      rlive[(long long)s * 4 + side * 2] = rl[0];
+      rlive[(long long)s * 4 + side * 2 + 1] = rl[1];
+    }
   }
+  t.sync();
   // target[r,p,q,s] = sum_{u,k,v} Ca[r,u,k] Cb[s,v,k] gate[p,q,u,v]
   const int ra = sa.rcap, rb = sb.rcap, nk = sa.leg[sa.core];
   fu::View Ca{ca + s, nbtot}, Cb{cb + s, nbtot}, Tg{target + s, nbtot};
-  for (int r = 0; r < ra; ++r)
-    for (int p = 0; p < d; ++p)
-      for (int q = 0; q < d; ++q)
-        for (int t = 0; t < rb; ++t) {
-          double acc = 0.0;
-          for (int u = 0; u < d; ++u)
-            for (int v = 0; v < d; ++v) {
-              const double gv = gate[((p * d + q) * d + u) * d + v];
-              if (gv == 0.0) continue;
-              double ab = 0.0;
-              for (int k = 0; k < nk; ++k) ab += Ca[((long long)r * d + u) * nk + k] * Cb[((long long)t * d + v) * nk + k];
-              acc += gv * ab;
-            }
-          Tg[(((long long)r * d + p) * d + q) * rb + t] = acc;
-        }
+  for (long long idx = L; idx < (long long)ra * d * d * rb; idx += NT) {
+    const int tt = (int)(idx % rb);
+    const long long rpq = idx / rb;
+    const int q = (int)(rpq % d), p = (int)((rpq / d) % d), r = (int)(rpq / ((long long)d * d));
+    double acc = 0.0;
+    for (int u = 0; u < d; ++u)
+      for (int v = 0; v < d; ++v) {
+        const double gv = gate[((p * d + q) * d + u) * d + v];
+        if (gv == 0.0) continue;
+        double ab = 0.0;
+        for (int k = 0; k < nk; ++k) ab += Ca[((long long)r * d + u) * nk + k] * Cb[((long long)tt * d + v) * nk + k];
+        acc += gv * ab;
+      }
+    Tg[idx] = acc;
+  }
+  t.sync();
 }
 
-__global__ void fu_pair_kernel(const PairJob* jobs, int nb, FuSide sa, FuSide sb, const double* gate, int d,
-                               int ed, double* qa, long long qa_stride, double* qb, long long qb_stride,
-                               double* ca, double* cb, double* target, int* rlive, double* work,
-                               long long work_per, int nbtot) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(64)
+fu_pair_kernel(const PairJob* jobs, int nb, FuSide sa, FuSide sb, const double* gate, int d, int ed, double* qa,
+               long long qa_stride, double* qb, long long qb_stride, double* ca, double* cb, double* target,
+               int* rlive, double* work, long long work_per, int nbtot) {
+  const int s = blockIdx.x;
   if (s >= nb) return;
-  fu_pair_sample(jobs[s], s, sa, sb, gate, d, ed, qa, qa_stride, qb, qb_stride, ca, cb, target, rlive, work,
+  __shared__ double red[3 * 64];
+  const fu::Team tm{(int)threadIdx.x, 64, red};
+  fu_pair_sample(tm, jobs[s], s, sa, sb, gate, d, ed, qa, qa_stride, qb, qb_stride, ca, cb, target, rlive, work,
                  work_per, nbtot);
 }
 
-// Bond solve of every sample (thread per sample): compact the metric and the
-// target to the live r dimensions, solve, expand the cores.
+// Bond solve of every sample, one 64-lane workgroup per sample (fu_core.h's
+// Team): compact the metric and the target to the live r dimensions, solve,
+// expand the cores. Each sample's workspace is contiguous (work_per doubles).
+constexpr int kCoreLanes = 64;
+
 template <typename T>
-__global__ void fu_core_kernel(const T* nf, long long nf_stride, const double* target, const int* rlive, int ra,
-                               int era, int rb, int erb, int d, int ed, int D, int eD, fu::Opts o, double* aout,
-                               double* bout, double* rec, int* bond_live, double* work, long long work_per,
-                               int nb, int nbtot) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kCoreLanes)
+fu_core_kernel(const T* nf, long long nf_stride, const double* target, const int* rlive, int ra, int era, int rb,
+               int erb, int d, int ed, int D, int eD, fu::Opts o, double* aout, double* bout, double* rec,
+               int* bond_live, double* work, long long work_per, int nb, int nbtot) {
+  const int s = blockIdx.x;
   if (s >= nb) return;
+  __shared__ double red[3 * kCoreLanes];
+  const fu::Team tm{(int)threadIdx.x, kCoreLanes, red};
+  const int L = tm.lane;
   const int la[2] = {rlive[(long long)s * 4], rlive[(long long)s * 4 + 1]};
   const int lb[2] = {rlive[(long long)s * 4 + 2], rlive[(long long)s * 4 + 3]};
   fu::Dims g{la[0] + la[1], la[0], lb[0] + lb[1], lb[0], d, ed, D, eD};
-  fu::Work w{work + s, nbtot, 0, work_per};
+  fu::Work w{work + (long long)s * work_per, 1, 0, work_per};
   const long long na = g.na, nbk = g.nb;
   fu::View N4 = w.take(na * nbk * na * nbk), tg = w.take(na * d * d * nbk);
   fu::View A = w.take(na * d * D), B = w.take((long long)D * d * nbk);
@@ -261,52 +274,62 @@ __global__ void fu_core_kernel(const T* nf, long long nf_stride, const double* t
   // N[rb,sb,rk,sk] = Nf[pair(rk, rb)][pair(sk, sb)]. The contraction kernel
   // writes only the parity-allowed blocks of its output (pair parities equal);
   // the others hold whatever the buffer held before and are zero by symmetry.
-  for (int i = 0; i < g.na; ++i)
-    for (int j = 0; j < g.nb; ++j)
-      for (int k = 0; k < g.na; ++k)
-        for (int l = 0; l < g.nb; ++l) {
-          const int pR = fu::par(i, la[0]) ^ fu::par(k, la[0]);
-          const int pS = fu::par(j, lb[0]) ^ fu::par(l, lb[0]);
-          double v = 0.0;
-          if (pR == pS) {
-            const int R = pair_index(slot_a(k), slot_a(i), ra, era);
-            const int S = pair_index(slot_b(l), slot_b(j), rb, erb);
-            v = (double)N[(long long)R * nbr + S];
-          }
-          N4[(((long long)i * g.nb + j) * g.na + k) * g.nb + l] = v;
-        }
-  fu::View Tg{const_cast<double*>(target) + s, nbtot};
-  for (int r = 0; r < g.na; ++r)
-    for (int p = 0; p < d; ++p)
-      for (int q = 0; q < d; ++q)
-        for (int t = 0; t < g.nb; ++t)
-          tg[(((long long)r * d + p) * d + q) * g.nb + t] = Tg[(((long long)slot_a(r) * d + p) * d + q) * rb + slot_b(t)];
-  fu::Work inner{work + s, nbtot, w.top, work_per - w.top};
-  fu::Result res = fu::fu_bond(g, o, N4, tg, A, B, inner);
-  fu::View Ao{aout + s, nbtot}, Bo{bout + s, nbtot};
-  for (long long i = 0; i < (long long)ra * d * D; ++i) Ao[i] = 0.0;
-  for (long long i = 0; i < (long long)D * d * rb; ++i) Bo[i] = 0.0;
-  if (res.status == 0) {
-    for (int r = 0; r < g.na; ++r)
-      for (long long j = 0; j < (long long)d * D; ++j) Ao[(long long)slot_a(r) * d * D + j] = A[(long long)r * d * D + j];
-    for (long long kq = 0; kq < (long long)D * d; ++kq)
-      for (int t = 0; t < g.nb; ++t) Bo[kq * rb + slot_b(t)] = B[kq * g.nb + t];
+  const long long nn = na * nbk * na * nbk;
+  for (long long idx = L; idx < nn; idx += kCoreLanes) {
+    const int l = (int)(idx % nbk), k = (int)((idx / nbk) % na);
+    const int j = (int)((idx / (nbk * na)) % nbk), i = (int)(idx / (nbk * na * nbk));
+    const int pR = fu::par(i, la[0]) ^ fu::par(k, la[0]);
+    const int pS = fu::par(j, lb[0]) ^ fu::par(l, lb[0]);
+    double v = 0.0;
+    if (pR == pS) {
+      const int R = pair_index(slot_a(k), slot_a(i), ra, era);
+      const int S = pair_index(slot_b(l), slot_b(j), rb, erb);
+      v = (double)N[(long long)R * nbr + S];
+    }
+    N4[idx] = v;
   }
-  double* rr = rec + (long long)s * 12;
-  rr[0] = res.initial_cost;
-  rr[1] = res.final_cost;
-  rr[2] = res.residual;
-  rr[3] = res.target_norm;
-  rr[4] = res.iterations;
-  rr[5] = res.limit_reached;
-  rr[6] = res.stop;
-  rr[7] = res.status;
-  rr[8] = res.kept_even;
-  rr[9] = res.kept_odd;
-  rr[10] = res.floored;
-  rr[11] = 0.0;
-  bond_live[(long long)s * 2] = res.kept_even;
-  bond_live[(long long)s * 2 + 1] = res.kept_odd;
+  fu::View Tg{const_cast<double*>(target) + s, nbtot};
+  const long long nt = na * d * d * nbk;
+  for (long long idx = L; idx < nt; idx += kCoreLanes) {
+    const int t = (int)(idx % nbk);
+    const long long rpq = idx / nbk;
+    const int q = (int)(rpq % d), p = (int)((rpq / d) % d), r = (int)(rpq / ((long long)d * d));
+    tg[idx] = Tg[(((long long)slot_a(r) * d + p) * d + q) * rb + slot_b(t)];
+  }
+  tm.sync();
+  fu::Work inner{w.base, 1, w.top, work_per - w.top};
+  const fu::Result res = fu::fu_bond(tm, g, o, N4, tg, A, B, inner);
+  fu::View Ao{aout + s, nbtot}, Bo{bout + s, nbtot};
+  for (long long i = L; i < (long long)ra * d * D; i += kCoreLanes) Ao[i] = 0.0;
+  for (long long i = L; i < (long long)D * d * rb; i += kCoreLanes) Bo[i] = 0.0;
+  tm.sync();
+  if (res.status == 0) {
+    for (long long idx = L; idx < na * d * D; idx += kCoreLanes) {
+      const int r = (int)(idx / ((long long)d * D));
+      Ao[(long long)slot_a(r) * d * D + idx % ((long long)d * D)] = A[idx];
+    }
+    for (long long idx = L; idx < (long long)D * d * nbk; idx += kCoreLanes) {
+      const long long kq = idx / nbk;
+      Bo[kq * rb + slot_b((int)(idx % nbk))] = B[idx];
+    }
+  }
+  if (L == 0) {
+    double* rr = rec + (long long)s * 12;
+    rr[0] = res.initial_cost;
+    rr[1] = res.final_cost;
+    rr[2] = res.residual;
+    rr[3] = res.target_norm;
+    rr[4] = res.iterations;
+    rr[5] = res.limit_reached;
+    rr[6] = res.stop;
+    rr[7] = res.status;
+    rr[8] = res.kept_even;
+    rr[9] = res.kept_odd;
+    rr[10] = res.floored;
+    rr[11] = 0.0;
+    bond_live[(long long)s * 2] = res.kept_even;
+    bond_live[(long long)s * 2 + 1] = res.kept_odd;
+  }
 }
 
 // New site = Q * core, written into the state's site array (thread per entry).
@@ -452,7 +475,8 @@ int fu_pair_host(int horizontal, const int* dims_a, const int* evens_a, const in
   J.live[0] = live_a;
   J.live[1] = live_b;
   int rl[4] = {0, 0, 0, 0};
-  fu_pair_sample(J, 0, sa, sb, gate, d, ed, qa, 0, qb, 0, ca, cb, target, rl, ws.data(), work, 1);
+  const fu::Team one{0, 1, nullptr};
+  fu_pair_sample(one, J, 0, sa, sb, gate, d, ed, qa, 0, qb, 0, ca, cb, target, rl, ws.data(), work, 1);
   geometry[0] = sa.rcap;
   geometry[1] = sa.recap;
   geometry[2] = sb.rcap;
@@ -905,7 +929,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
               SPP_CHECK(hipMemcpyAsync(pjobs, pjd.data(), sizeof(PairJob) * cur, hipMemcpyHostToDevice, st));
               SPP_CHECK(hipMemcpyAsync(rjobs, rj.data(), sizeof(RestoreJob) * cur, hipMemcpyHostToDevice, st));
             }
-            hipLaunchKernelGGL(fu_pair_kernel, dim3((cur + 63) / 64), dim3(64), 0, st, pjobs, cur, G.sa, G.sb, gate, d,
+            hipLaunchKernelGGL(fu_pair_kernel, dim3(cur), dim3(64), 0, st, pjobs, cur, G.sa, G.sb, gate, d,
                                ed, qa, qmax, qbuf, qmax, ca, cb, tgt, rlive, pwork, pairwork, nb);
             SPP_CHECK(hipGetLastError());
             // --- reduced layers of the Q pseudo-sites ---
@@ -1012,7 +1036,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
             }
             // --- the bond solve ---
             const fu::Opts o = P.opts;
-            hipLaunchKernelGGL(fu_core_kernel<T>, dim3((cur + 63) / 64), dim3(64), 0, st, nfused, nfsz, tgt, rlive,
+            hipLaunchKernelGGL(fu_core_kernel<T>, dim3(cur), dim3(kCoreLanes), 0, st, nfused, nfsz, tgt, rlive,
                                G.sa.rcap, G.sa.recap, G.sb.rcap, G.sb.recap, d, ed, G.Dcap, G.eDcap, o, aout, bout, drec,
                                blive, cwork, coremax, cur, nb);
             SPP_CHECK(hipGetLastError());
