@@ -239,7 +239,7 @@ fu_pair_kernel(const PairJob* jobs, int nb, FuSide sa, FuSide sb, const double* 
                int* rlive, double* work, long long work_per, int nbtot) {
   const int s = blockIdx.x;
   if (s >= nb) return;
-  __shared__ double red[3 * 64];
+  __shared__ double red[3];
   const fu::Team tm{(int)threadIdx.x, 64, red};
   fu_pair_sample(tm, jobs[s], s, sa, sb, gate, d, ed, qa, qa_stride, qb, qb_stride, ca, cb, target, rlive, work,
                  work_per, nbtot);
@@ -257,7 +257,7 @@ fu_core_kernel(const T* nf, long long nf_stride, const double* target, const int
                int* bond_live, double* work, long long work_per, int nb, int nbtot) {
   const int s = blockIdx.x;
   if (s >= nb) return;
-  __shared__ double red[3 * kCoreLanes];
+  __shared__ double red[3 * (kCoreLanes / 64)];
   const fu::Team tm{(int)threadIdx.x, kCoreLanes, red};
   const int L = tm.lane;
   const int la[2] = {rlive[(long long)s * 4], rlive[(long long)s * 4 + 1]};

@@ -38,10 +38,13 @@ struct Work {
 };
 
 // The lanes that solve one sample together: a workgroup on the device (lane =
-// thread, size = a power of two; red: LDS scratch of 3 * size doubles), one
-// lane on the host. All lanes run the same control flow and take the same
-// workspace; a matrix row i is "owned" by lane i % size wherever a routine
-// keeps rows private to a lane between its reductions.
+// thread, size = a multiple of the 64-lane wavefront; red: LDS scratch of
+// 3 * size / 64 doubles), one lane on the host. All lanes run the same control
+// flow and take the same workspace; a matrix row i is "owned" by lane
+// i % size wherever a routine keeps rows private to a lane between its
+// reductions. Reductions are xor butterflies inside a wavefront (every lane
+// ends with the bitwise same value, since each step adds the same two
+// operands on both partners), then a fixed-order sum over wavefronts.
 struct Team {
   int lane, size;
   double* red;
@@ -50,68 +53,56 @@ struct Team {
     __syncthreads();
 #endif
   }
+#ifdef __HIP_DEVICE_COMPILE__
+  __device__ void waves(double* v, int nv, bool is_max) const {
+    for (int m = 32; m >= 1; m >>= 1)
+      for (int k = 0; k < nv; ++k) {
+        const double o = __shfl_xor(v[k], m, 64);
+        v[k] = is_max ? (o > v[k] ? o : v[k]) : v[k] + o;
+      }
+    if (size > 64) {
+      const int nw = size >> 6;
+      __syncthreads();  // earlier readers of red are done
+      if ((lane & 63) == 0)
+        for (int k = 0; k < nv; ++k) red[k * nw + (lane >> 6)] = v[k];
+      __syncthreads();
+      for (int k = 0; k < nv; ++k) {
+        double r = red[k * nw];
+        for (int w = 1; w < nw; ++w) r = is_max ? (red[k * nw + w] > r ? red[k * nw + w] : r) : r + red[k * nw + w];
+        v[k] = r;
+      }
+    }
+  }
+#endif
   // all lanes receive the sum of v over lanes
   FU_HD double sum(double v) const {
 #ifdef __HIP_DEVICE_COMPILE__
-    red[lane] = v;
-    __syncthreads();
-    for (int w = size / 2; w > 0; w >>= 1) {
-      if (lane < w) red[lane] += red[lane + w];
-      __syncthreads();
-    }
-    const double r = red[0];
-    __syncthreads();
-    return r;
-#else
-    return v;
+    waves(&v, 1, false);
 #endif
+    return v;
   }
   FU_HD double max(double v) const {
 #ifdef __HIP_DEVICE_COMPILE__
-    red[lane] = v;
-    __syncthreads();
-    for (int w = size / 2; w > 0; w >>= 1) {
-      if (lane < w && red[lane + w] > red[lane]) red[lane] = red[lane + w];
-      __syncthreads();
-    }
-    const double r = red[0];
-    __syncthreads();
-    return r;
-#else
+    waves(&v, 1, true);
+#endif
     return v;
+  }
+  // two or three sums in one pass
+  FU_HD void sum2(double& a, double& b) const {
+#ifdef __HIP_DEVICE_COMPILE__
+    double v[2] = {a, b};
+    waves(v, 2, false);
+    a = v[0];
+    b = v[1];
 #endif
   }
-  // three sums in one pass
   FU_HD void sum3(double& a, double& b, double& c) const {
 #ifdef __HIP_DEVICE_COMPILE__
-    red[lane] = a;
-    red[size + lane] = b;
-    red[2 * size + lane] = c;
-    __syncthreads();
-    for (int w = size / 2; w > 0; w >>= 1) {
-      if (lane < w) {
-        red[lane] += red[lane + w];
-        red[size + lane] += red[size + lane + w];
-        red[2 * size + lane] += red[2 * size + lane + w];
-      }
-      __syncthreads();
-    }
-    a = red[0];
-    b = red[size];
-    c = red[2 * size];
-    __syncthreads();
-#endif
-  }
-  // lane 0's value to every lane
-  FU_HD double bcast(double v) const {
-#ifdef __HIP_DEVICE_COMPILE__
-    if (lane == 0) red[0] = v;
-    __syncthreads();
-    const double r = red[0];
-    __syncthreads();
-    return r;
-#else
-    return v;
+    double v[3] = {a, b, c};
+    waves(v, 3, false);
+    a = v[0];
+    b = v[1];
+    c = v[2];
 #endif
   }
 };
@@ -152,14 +143,22 @@ FU_HD inline bool finite(double x) { return fabs(x) <= 1.7976931348623157e308; }
 // dense kernels on small row-major matrices
 // ---------------------------------------------------------------------------
 
-// Cyclic Jacobi eigensolver of the symmetric n x n matrix A (destroyed):
-// eigenvalues in w, eigenvectors as columns of V (V[i*n+k] = component i of k).
-// The lanes share each rotation's row and column updates.
-FU_HD inline void jacobi_eigh(const Team& t, View A, int n, View w, View V) {
+// Jacobi eigensolver of the symmetric n x n matrix A (destroyed) in
+// round-robin (tournament) order: each round rotates n/2 disjoint index pairs
+// at once, so the lanes share a round's rotations and every round costs three
+// barriers. Eigenvalues in w, eigenvectors as columns of V (V[i*n+k] =
+// component i of k), unordered. cs: 3 * (n / 2 + 1) scratch. The bond gauge
+// uses only V max(w, floor) V^T, which does not depend on the order or on the
+// basis chosen inside a degenerate eigenspace.
+FU_HD inline void jacobi_eigh(const Team& t, View A, int n, View w, View V, View cs) {
   for (long long i = t.lane; i < (long long)n * n; i += t.size) V[i] = (i / n) == (i % n) ? 1.0 : 0.0;
   double fl = 0.0;
   for (long long i = t.lane; i < (long long)n * n; i += t.size) fl += A[i] * A[i];
   const double fro = t.sum(fl);
+  const int m = n + (n & 1);  // one idle slot for odd n
+  const int np = m / 2;
+  // pair i of round r: slots (i, m-1-i) of the sequence [0, r rotations of 1..m-1]
+  auto slot = [&](int r, int j) { return j == 0 ? 0 : 1 + (j - 1 + r) % (m - 1); };
   for (int sweep = 0; sweep < 80; ++sweep) {
     double ol = 0.0;
     for (long long i = t.lane; i < (long long)n * n; i += t.size) {
@@ -168,36 +167,59 @@ FU_HD inline void jacobi_eigh(const Team& t, View A, int n, View w, View V) {
     }
     const double off = t.sum(ol);
     if (off <= 1e-34 * fro || off == 0.0) break;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        const double apq = A[(long long)p * n + q];
-        if (apq == 0.0 || fabs(apq) <= 1e-300) continue;
-        const double app = A[(long long)p * n + p], aqq = A[(long long)q * n + q];
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        const double c = 1.0 / sqrt(tt * tt + 1.0), s = tt * c;
-        t.sync();  // every lane has read the pivot entries
-        for (int k = t.lane; k < n; k += t.size) {
-          const double akp = A[(long long)k * n + p], akq = A[(long long)k * n + q];
-          A[(long long)k * n + p] = c * akp - s * akq;
-          A[(long long)k * n + q] = s * akp + c * akq;
-          const double vkp = V[(long long)k * n + p], vkq = V[(long long)k * n + q];
-          V[(long long)k * n + p] = c * vkp - s * vkq;
-          V[(long long)k * n + q] = s * vkp + c * vkq;
+    for (int r = 0; r < m - 1; ++r) {
+      // rotation of each pair (c = 2 marks "no rotation")
+      for (int i = t.lane; i < np; i += t.size) {
+        int p = slot(r, i), q = slot(r, m - 1 - i);
+        if (p > q) {
+          const int x = p;
+          p = q;
+          q = x;
         }
-        t.sync();
-        for (int k = t.lane; k < n; k += t.size) {
-          const double apk = A[(long long)p * n + k], aqk = A[(long long)q * n + k];
-          A[(long long)p * n + k] = c * apk - s * aqk;
-          A[(long long)q * n + k] = s * apk + c * aqk;
+        double c = 2.0, sn = 0.0;
+        if (q < n) {
+          const double apq = A[(long long)p * n + q];
+          if (!(apq == 0.0 || fabs(apq) <= 1e-300)) {
+            const double app = A[(long long)p * n + p], aqq = A[(long long)q * n + q];
+            const double theta = (aqq - app) / (2.0 * apq);
+            const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            c = 1.0 / sqrt(tt * tt + 1.0);
+            sn = tt * c;
+          }
         }
-        t.sync();
-        if (t.lane == 0) {
-          A[(long long)p * n + q] = 0.0;
-          A[(long long)q * n + p] = 0.0;
-        }
-        t.sync();
+        cs[3 * i] = c;
+        cs[3 * i + 1] = sn;
+        cs[3 * i + 2] = p + q * 65536.0;
       }
+      t.sync();
+      // columns p, q of A and V: A <- A J, V <- V J
+      for (long long task = t.lane; task < (long long)np * n; task += t.size) {
+        const int i = (int)(task / n), k = (int)(task % n);
+        const double c = cs[3 * i];
+        if (c == 2.0) continue;
+        const double sn = cs[3 * i + 1];
+        const int pq = (int)cs[3 * i + 2], p = pq & 65535, q = pq >> 16;
+        const double akp = A[(long long)k * n + p], akq = A[(long long)k * n + q];
+        A[(long long)k * n + p] = c * akp - sn * akq;
+        A[(long long)k * n + q] = sn * akp + c * akq;
+        const double vkp = V[(long long)k * n + p], vkq = V[(long long)k * n + q];
+        V[(long long)k * n + p] = c * vkp - sn * vkq;
+        V[(long long)k * n + q] = sn * vkp + c * vkq;
+      }
+      t.sync();
+      // rows p, q: A <- J^T A; the rotated pair's off-diagonal entries become 0
+      for (long long task = t.lane; task < (long long)np * n; task += t.size) {
+        const int i = (int)(task / n), k = (int)(task % n);
+        const double c = cs[3 * i];
+        if (c == 2.0) continue;
+        const double sn = cs[3 * i + 1];
+        const int pq = (int)cs[3 * i + 2], p = pq & 65535, q = pq >> 16;
+        const double apk = A[(long long)p * n + k], aqk = A[(long long)q * n + k];
+        A[(long long)p * n + k] = k == q ? 0.0 : c * apk - sn * aqk;
+        A[(long long)q * n + k] = k == p ? 0.0 : sn * apk + c * aqk;
+      }
+      t.sync();
+    }
   }
   for (int i = t.lane; i < n; i += t.size) w[i] = A[(long long)i * n + i];
   t.sync();
@@ -480,7 +502,7 @@ FU_HD inline bool triangular_inverse(const Team& t, View R, int n, int e, View R
 // theta[r][p][q][s] (na, d, d, nb); cores A[r][p][k] (na, d, D), B[k][q][s].
 // ---------------------------------------------------------------------------
 
-// out[r,p,q,s] = sum_{t,u} N[r,s,t,u] theta[t,p,q,u]
+// out[r,p,q,s] = sum_{t,u} N[r,s,t,u] theta[t,p,q,u] for a Z2-symmetric N
 FU_HD inline void apply_metric(const Team& tm, View N, View th, View out, const Dims& g) {
   const int na = g.na, nb = g.nb, d = g.d;
   const long long tot = (long long)na * nb * d * d;
@@ -492,10 +514,16 @@ FU_HD inline void apply_metric(const Team& tm, View N, View th, View out, const 
     r_ /= d;
     const int s = (int)(r_ % nb);
     const int r = (int)(r_ / nb);
+    // N[r,s,t,u] vanishes unless par(r) + par(s) = par(t) + par(u): u runs
+    // over the one parity sector each t allows
+    const int prs = par(r, g.ea) ^ par(s, g.eb);
     double acc = 0.0;
-    for (int t = 0; t < na; ++t)
-      for (int u = 0; u < nb; ++u)
+    for (int t = 0; t < na; ++t) {
+      const int pu = prs ^ par(t, g.ea);
+      const int u0 = pu == 0 ? 0 : g.eb, u1 = pu == 0 ? g.eb : nb;
+      for (int u = u0; u < u1; ++u)
         acc += N[(((long long)r * nb + s) * na + t) * nb + u] * th[(((long long)t * d + p) * d + q) * nb + u];
+    }
     out[(((long long)r * d + p) * d + q) * nb + s] = acc;
   }
   tm.sync();
@@ -604,19 +632,27 @@ FU_HD inline double solve_core(const Team& tm, View M, int n, int nc, View rhs, 
   double rz = dot(tm, r, z, nv);
   for (int it = 0; it < o.linear_max_iterations; ++it) {
     action(p, Ap);
-    const double curv = dot(tm, p, Ap, nv);
-    const double pn = sqrt(dot(tm, p, p, nv)), an = sqrt(dot(tm, Ap, Ap, nv));
-    const double lim = pn * an > kTiny ? pn * an : kTiny;
+    double curv = 0.0, pp = 0.0, aa = 0.0;
+    for (long long i = tm.lane; i < nv; i += tm.size) {
+      curv += p[i] * Ap[i];
+      pp += p[i] * p[i];
+      aa += Ap[i] * Ap[i];
+    }
+    tm.sum3(curv, pp, aa);
+    const double lim = sqrt(pp) * sqrt(aa) > kTiny ? sqrt(pp) * sqrt(aa) : kTiny;
     if (!(curv > kEps * lim)) break;
     const double alpha = rz / curv;
+    double rr = 0.0, nxt = 0.0;
     for (long long i = tm.lane; i < nv; i += tm.size) {
       x[i] += alpha * p[i];
       r[i] -= alpha * Ap[i];
+      z[i] = r[i] / pre[i / nc];
+      rr += r[i] * r[i];
+      nxt += r[i] * z[i];
     }
-    residual = sqrt(dot(tm, r, r, nv)) / rhsn;
+    tm.sum2(rr, nxt);
+    residual = sqrt(rr) / rhsn;
     if (residual <= o.linear_tolerance) break;
-    for (long long i = tm.lane; i < nv; i += tm.size) z[i] = r[i] / pre[i / nc];
-    const double nxt = dot(tm, r, z, nv);
     if (!(nxt >= 0)) break;
     const double beta = nxt / (rz > kTiny ? rz : kTiny);
     for (long long i = tm.lane; i < nv; i += tm.size) p[i] = z[i] + beta * p[i];
@@ -701,7 +737,7 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
         ++fill[q];
       }
     }
-    View Hq = w.take(n * n);
+    View Hq = w.take(n * n), cs = w.take(3 * (n / 2 + 1));
     View evals = w.take(n), evecs = w.take(n * n);  // block-major storage
     t.sync();
     long long eoff = 0, voff = 0;
@@ -716,7 +752,7 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
       t.sync();
       View wq{&evals[eoff], evals.s};
       View vq{&evecs[voff], evecs.s};
-      jacobi_eigh(t, Hq, m, wq, vq);
+      jacobi_eigh(t, Hq, m, wq, vq, cs);
       eoff += m;
       voff += (long long)m * m;
     }
@@ -1005,12 +1041,16 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
       for (long long idx = L; idx < (long long)nl * nl; idx += NT) {
         const long long fb_ = idx / nl, go = idx % nl;
         const int f = (int)(fb_ / D), b = (int)(fb_ % D), gg = (int)(go / D), oo = (int)(go % D);
+        // Z2 structure: the entry vanishes unless par(f) + par(b) = par(g) + par(o),
+        // and metric[f,s,g,n] unless par(n) = par(f) + par(s) + par(g)
         double acc = 0.0;
-        for (int s = 0; s < nb; ++s)
-          for (int nn = 0; nn < nb; ++nn) {
-            const double mv = metric[(((long long)f * nb + s) * na + gg) * nb + nn];
-            if (mv == 0.0) continue;
-            acc += mv * BB[((long long)b * nb + s) * nr + (long long)oo * nb + nn];
+        if ((pa(f) ^ pD(b)) == (pa(gg) ^ pD(oo)))
+          for (int s = 0; s < nb; ++s) {
+            const int pn = pa(f) ^ pb(s) ^ pa(gg);
+            const int n0 = pn == 0 ? 0 : g.eb, n1 = pn == 0 ? g.eb : nb;
+            for (int nn = n0; nn < n1; ++nn)
+              acc += metric[(((long long)f * nb + s) * na + gg) * nb + nn] *
+                     BB[((long long)b * nb + s) * nr + (long long)oo * nb + nn];
           }
         NL[idx] = acc;
       }
@@ -1057,12 +1097,16 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
       for (long long idx = L; idx < (long long)nr * nr; idx += NT) {
         const long long bs = idx / nr, on = idx % nr;
         const int b = (int)(bs / nb), s = (int)(bs % nb), oo = (int)(on / nb), nn = (int)(on % nb);
+        // vanishes unless par(b) + par(s) = par(o) + par(n); metric[w,s,e,n]
+        // unless par(e) = par(w) + par(s) + par(n)
         double acc = 0.0;
-        for (int ww = 0; ww < na; ++ww)
-          for (int e = 0; e < na; ++e) {
-            const double mv = metric[(((long long)ww * nb + s) * na + e) * nb + nn];
-            if (mv == 0.0) continue;
-            acc += mv * AA[((long long)b * na + ww) * nl + (long long)oo * na + e];
+        if ((pD(b) ^ pb(s)) == (pD(oo) ^ pb(nn)))
+          for (int ww = 0; ww < na; ++ww) {
+            const int pe = pa(ww) ^ pb(s) ^ pb(nn);
+            const int e0 = pe == 0 ? 0 : g.ea, e1 = pe == 0 ? g.ea : na;
+            for (int e = e0; e < e1; ++e)
+              acc += metric[(((long long)ww * nb + s) * na + e) * nb + nn] *
+                     AA[((long long)b * na + ww) * nl + (long long)oo * na + e];
           }
         NR[idx] = acc;
       }
