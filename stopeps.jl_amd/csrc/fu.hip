@@ -29,7 +29,7 @@ fu::Result fu_bond_host(const fu::Dims& g, const fu::Opts& o, const double* N4, 
   std::vector<double> ws((size_t)cap);
   fu::Work w{ws.data(), 1, 0, cap};
   fu::View n{const_cast<double*>(N4), 1}, t{const_cast<double*>(target), 1}, a{A, 1}, b{B, 1};
-  const fu::Team one{0, 1, nullptr};
+  const fu::Team one{0, 1, nullptr, nullptr, 0};
   return fu::fu_bond(one, g, o, n, t, a, b, w);
 }
 
@@ -240,25 +240,26 @@ fu_pair_kernel(const PairJob* jobs, int nb, FuSide sa, FuSide sb, const double* 
   const int s = blockIdx.x;
   if (s >= nb) return;
   __shared__ double red[3];
-  const fu::Team tm{(int)threadIdx.x, 64, red};
+  const fu::Team tm{(int)threadIdx.x, 64, red, nullptr, 0};
   fu_pair_sample(tm, jobs[s], s, sa, sb, gate, d, ed, qa, qa_stride, qb, qb_stride, ca, cb, target, rlive, work,
                  work_per, nbtot);
 }
 
-// Bond solve of every sample, one 64-lane workgroup per sample (fu_core.h's
+// Bond solve of every sample, one 256-lane workgroup per sample (fu_core.h's
 // Team): compact the metric and the target to the live r dimensions, solve,
 // expand the cores. Each sample's workspace is contiguous (work_per doubles).
-constexpr int kCoreLanes = 64;
+constexpr int kCoreLanes = 256;
 
 template <typename T>
 __global__ void __launch_bounds__(kCoreLanes)
 fu_core_kernel(const T* nf, long long nf_stride, const double* target, const int* rlive, int ra, int era, int rb,
                int erb, int d, int ed, int D, int eD, fu::Opts o, double* aout, double* bout, double* rec,
-               int* bond_live, double* work, long long work_per, int nb, int nbtot) {
+               int* bond_live, double* work, long long work_per, int nb, int nbtot, long long lds_doubles) {
   const int s = blockIdx.x;
   if (s >= nb) return;
   __shared__ double red[3 * (kCoreLanes / 64)];
-  const fu::Team tm{(int)threadIdx.x, kCoreLanes, red};
+  extern __shared__ double fu_lds[];
+  const fu::Team tm{(int)threadIdx.x, kCoreLanes, red, fu_lds, lds_doubles};
   const int L = tm.lane;
   const int la[2] = {rlive[(long long)s * 4], rlive[(long long)s * 4 + 1]};
   const int lb[2] = {rlive[(long long)s * 4 + 2], rlive[(long long)s * 4 + 3]};
@@ -475,7 +476,7 @@ int fu_pair_host(int horizontal, const int* dims_a, const int* evens_a, const in
   J.live[0] = live_a;
   J.live[1] = live_b;
   int rl[4] = {0, 0, 0, 0};
-  const fu::Team one{0, 1, nullptr};
+  const fu::Team one{0, 1, nullptr, nullptr, 0};
   fu_pair_sample(one, J, 0, sa, sb, gate, d, ed, qa, 0, qb, 0, ca, cb, target, rl, ws.data(), work, 1);
   geometry[0] = sa.rcap;
   geometry[1] = sa.recap;
@@ -536,6 +537,15 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
     int Dcap, eDcap;
     long long qa_size, qb_size, la_size, lb_size;
     LayerSpec la, lb;
+  };
+  // doubles of LDS the bond solve stages: the larger gauge eigenproblem block
+  // with its eigenvectors and rotations, or one normal-matrix operand
+  auto core_lds_doubles = [](const BondGeo& G) -> long long {
+    const long long ra = G.sa.rcap, era = G.sa.recap, rb = G.sb.rcap, erb = G.sb.recap;
+    const long long c0 = era * erb + (ra - era) * (rb - erb), c1 = era * (rb - erb) + (ra - era) * erb;
+    const long long m = std::max(c0, c1);
+    const long long nl = ra * G.Dcap, nr = G.Dcap * rb;
+    return std::max(2 * m * m + 3 * (m / 2 + 1), std::max(nl * nl, nr * nr));
   };
   auto geo = [&](int ia, int ib, bool horizontal) {
     BondGeo g{};
@@ -672,6 +682,24 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
     double* bout = (double*)dev(sizeof(double) * (size_t)cmax * nb);
     double* pwork = (double*)dev(sizeof(double) * (size_t)pairwork * nb);
     double* cwork = (double*)dev(sizeof(double) * (size_t)coremax * nb);
+    // LDS pool of the bond solve: what the largest bond needs, within the
+    // device's per-workgroup limit (the kernel stages only what fits)
+    long long lds_limit = 0;
+    {
+      int maxlds = 0, optin = 0;
+      SPP_CHECK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx.device));
+      if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, ctx.device) == hipSuccess)
+        maxlds = std::max(maxlds, optin);
+      (void)hipGetLastError();
+      long long need = 0;
+      for (const auto* set : {&hb, &vb})
+        for (const BondGeo& g : *set) need = std::max(need, core_lds_doubles(g));
+      lds_limit = std::min<long long>(need, (maxlds - 256) / (long long)sizeof(double));
+      if (lds_limit < 0) lds_limit = 0;
+      if (lds_limit * (long long)sizeof(double) > 65536)
+        SPP_CHECK(hipFuncSetAttribute((const void*)fu_core_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(lds_limit * sizeof(double))));
+    }
     double* drec = (double*)dev(sizeof(double) * 12 * (size_t)nb);
     int* rlive = (int*)dev(sizeof(int) * 4 * (size_t)nb);
     int* blive = (int*)dev(sizeof(int) * 2 * (size_t)nb);
@@ -1036,9 +1064,10 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
             }
             // --- the bond solve ---
             const fu::Opts o = P.opts;
-            hipLaunchKernelGGL(fu_core_kernel<T>, dim3(cur), dim3(kCoreLanes), 0, st, nfused, nfsz, tgt, rlive,
-                               G.sa.rcap, G.sa.recap, G.sb.rcap, G.sb.recap, d, ed, G.Dcap, G.eDcap, o, aout, bout, drec,
-                               blive, cwork, coremax, cur, nb);
+            const long long lds = std::min(core_lds_doubles(G), lds_limit);
+            hipLaunchKernelGGL(fu_core_kernel<T>, dim3(cur), dim3(kCoreLanes), (unsigned)(lds * sizeof(double)), st,
+                               nfused, nfsz, tgt, rlive, G.sa.rcap, G.sa.recap, G.sb.rcap, G.sb.recap, d, ed, G.Dcap,
+                               G.eDcap, o, aout, bout, drec, blive, cwork, coremax, cur, nb, lds);
             SPP_CHECK(hipGetLastError());
             SPP_CHECK(hipMemcpyAsync(hrec.data(), drec, sizeof(double) * 12 * cur, hipMemcpyDeviceToHost, st));
             SPP_CHECK(hipStreamSynchronize(st));

@@ -42,19 +42,24 @@ struct Work {
 // 3 * size / 64 doubles), one lane on the host. All lanes run the same control
 // flow and take the same workspace; a matrix row i is "owned" by lane
 // i % size wherever a routine keeps rows private to a lane between its
-// reductions. Reductions are xor butterflies inside a wavefront (every lane
+// reductions. Hot working sets (the gauge eigenproblem, the normal-matrix
+// operands) are staged in the LDS pool when they fit, in the workspace
+// otherwise. Reductions are xor butterflies inside a wavefront (every lane
 // ends with the bitwise same value, since each step adds the same two
 // operands on both partners), then a fixed-order sum over wavefronts.
 struct Team {
   int lane, size;
   double* red;
+  double* lds;        // the workgroup's LDS pool (device), null on the host
+  long long lds_cap;  // its size in doubles (0: stage nothing)
   FU_HD void sync() const {
 #ifdef __HIP_DEVICE_COMPILE__
-    __syncthreads();
+    if (size > 1) __syncthreads();
 #endif
   }
 #ifdef __HIP_DEVICE_COMPILE__
   __device__ void waves(double* v, int nv, bool is_max) const {
+    if (size == 1) return;
     for (int m = 32; m >= 1; m >>= 1)
       for (int k = 0; k < nv; ++k) {
         const double o = __shfl_xor(v[k], m, 64);
@@ -74,6 +79,38 @@ struct Team {
     }
   }
 #endif
+  // exclusive prefix sum of v over lanes (lane order); total: the full sum
+  FU_HD int scan(int v, int& total) const {
+#ifdef __HIP_DEVICE_COMPILE__
+    if (size > 1) {
+      const int wl = lane & 63;
+      int x = v;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (wl >= o) x += y;
+      }
+      if (size > 64) {
+        const int nw = size >> 6, me = lane >> 6;
+        __syncthreads();  // earlier readers of red are done
+        if (wl == 63) red[me] = (double)x;
+        __syncthreads();
+        int before = 0, tot = 0;
+        for (int w = 0; w < nw; ++w) {
+          const int c = (int)red[w];
+          if (w < me) before += c;
+          tot += c;
+        }
+        __syncthreads();
+        total = tot;
+        return before + x - v;
+      }
+      total = __shfl(x, 63, 64);
+      return x - v;
+    }
+#endif
+    total = v;
+    return 0;
+  }
   // all lanes receive the sum of v over lanes
   FU_HD double sum(double v) const {
 #ifdef __HIP_DEVICE_COMPILE__
@@ -150,10 +187,12 @@ FU_HD inline bool finite(double x) { return fabs(x) <= 1.7976931348623157e308; }
 // component i of k), unordered. cs: 3 * (n / 2 + 1) scratch. The bond gauge
 // uses only V max(w, floor) V^T, which does not depend on the order or on the
 // basis chosen inside a degenerate eigenspace.
+constexpr int kU = 4;  // tasks whose loads a lane issues together
+
 FU_HD inline void jacobi_eigh(const Team& t, View A, int n, View w, View V, View cs) {
-  for (long long i = t.lane; i < (long long)n * n; i += t.size) V[i] = (i / n) == (i % n) ? 1.0 : 0.0;
+  for (int i = t.lane; i < (int)n * n; i += t.size) V[i] = (i / n) == (i % n) ? 1.0 : 0.0;
   double fl = 0.0;
-  for (long long i = t.lane; i < (long long)n * n; i += t.size) fl += A[i] * A[i];
+  for (int i = t.lane; i < (int)n * n; i += t.size) fl += A[i] * A[i];
   const double fro = t.sum(fl);
   const int m = n + (n & 1);  // one idle slot for odd n
   const int np = m / 2;
@@ -161,7 +200,7 @@ FU_HD inline void jacobi_eigh(const Team& t, View A, int n, View w, View V, View
   auto slot = [&](int r, int j) { return j == 0 ? 0 : 1 + (j - 1 + r) % (m - 1); };
   for (int sweep = 0; sweep < 80; ++sweep) {
     double ol = 0.0;
-    for (long long i = t.lane; i < (long long)n * n; i += t.size) {
+    for (int i = t.lane; i < (int)n * n; i += t.size) {
       const int p = (int)(i / n), q = (int)(i % n);
       if (q > p) ol += A[i] * A[i];
     }
@@ -178,9 +217,9 @@ FU_HD inline void jacobi_eigh(const Team& t, View A, int n, View w, View V, View
         }
         double c = 2.0, sn = 0.0;
         if (q < n) {
-          const double apq = A[(long long)p * n + q];
+          const double apq = A[(int)p * n + q];
           if (!(apq == 0.0 || fabs(apq) <= 1e-300)) {
-            const double app = A[(long long)p * n + p], aqq = A[(long long)q * n + q];
+            const double app = A[(int)p * n + p], aqq = A[(int)q * n + q];
             const double theta = (aqq - app) / (2.0 * apq);
             const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
             c = 1.0 / sqrt(tt * tt + 1.0);
@@ -192,36 +231,76 @@ FU_HD inline void jacobi_eigh(const Team& t, View A, int n, View w, View V, View
         cs[3 * i + 2] = p + q * 65536.0;
       }
       t.sync();
-      // columns p, q of A and V: A <- A J, V <- V J
-      for (long long task = t.lane; task < (long long)np * n; task += t.size) {
-        const int i = (int)(task / n), k = (int)(task % n);
-        const double c = cs[3 * i];
-        if (c == 2.0) continue;
-        const double sn = cs[3 * i + 1];
-        const int pq = (int)cs[3 * i + 2], p = pq & 65535, q = pq >> 16;
-        const double akp = A[(long long)k * n + p], akq = A[(long long)k * n + q];
-        A[(long long)k * n + p] = c * akp - sn * akq;
-        A[(long long)k * n + q] = sn * akp + c * akq;
-        const double vkp = V[(long long)k * n + p], vkq = V[(long long)k * n + q];
-        V[(long long)k * n + p] = c * vkp - sn * vkq;
-        V[(long long)k * n + q] = sn * vkp + c * vkq;
+      // columns p, q of A and V: A <- A J, V <- V J. Tasks touch disjoint
+      // entries; each lane loads kU tasks before it stores any, so that
+      // their loads are in flight together.
+      for (int t0 = t.lane; t0 < (int)np * n; t0 += (int)kU * t.size) {
+        double c[kU], sn[kU], akp[kU], akq[kU], vkp[kU], vkq[kU];
+        int ip[kU], iq[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int task = t0 + (int)u * t.size;
+          c[u] = 2.0;
+          if (task < (int)np * n) {
+            const int i = (int)(task / n), k = (int)(task % n);
+            c[u] = cs[3 * i];
+            sn[u] = cs[3 * i + 1];
+            const int pq = (int)cs[3 * i + 2];
+            ip[u] = (int)k * n + (pq & 65535);
+            iq[u] = (int)k * n + (pq >> 16);
+            if (c[u] != 2.0) {
+              akp[u] = A[ip[u]];
+              akq[u] = A[iq[u]];
+              vkp[u] = V[ip[u]];
+              vkq[u] = V[iq[u]];
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+          if (c[u] != 2.0) {
+            A[ip[u]] = c[u] * akp[u] - sn[u] * akq[u];
+            A[iq[u]] = sn[u] * akp[u] + c[u] * akq[u];
+            V[ip[u]] = c[u] * vkp[u] - sn[u] * vkq[u];
+            V[iq[u]] = sn[u] * vkp[u] + c[u] * vkq[u];
+          }
       }
       t.sync();
       // rows p, q: A <- J^T A; the rotated pair's off-diagonal entries become 0
-      for (long long task = t.lane; task < (long long)np * n; task += t.size) {
-        const int i = (int)(task / n), k = (int)(task % n);
-        const double c = cs[3 * i];
-        if (c == 2.0) continue;
-        const double sn = cs[3 * i + 1];
-        const int pq = (int)cs[3 * i + 2], p = pq & 65535, q = pq >> 16;
-        const double apk = A[(long long)p * n + k], aqk = A[(long long)q * n + k];
-        A[(long long)p * n + k] = k == q ? 0.0 : c * apk - sn * aqk;
-        A[(long long)q * n + k] = k == p ? 0.0 : sn * apk + c * aqk;
+      for (int t0 = t.lane; t0 < (int)np * n; t0 += (int)kU * t.size) {
+        double c[kU], sn[kU], apk[kU], aqk[kU];
+        int ip[kU], iq[kU];
+        bool zp[kU], zq[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int task = t0 + (int)u * t.size;
+          c[u] = 2.0;
+          if (task < (int)np * n) {
+            const int i = (int)(task / n), k = (int)(task % n);
+            c[u] = cs[3 * i];
+            sn[u] = cs[3 * i + 1];
+            const int pq = (int)cs[3 * i + 2], p = pq & 65535, q = pq >> 16;
+            ip[u] = (int)p * n + k;
+            iq[u] = (int)q * n + k;
+            zp[u] = k == q;
+            zq[u] = k == p;
+            if (c[u] != 2.0) {
+              apk[u] = A[ip[u]];
+              aqk[u] = A[iq[u]];
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+          if (c[u] != 2.0) {
+            A[ip[u]] = zp[u] ? 0.0 : c[u] * apk[u] - sn[u] * aqk[u];
+            A[iq[u]] = zq[u] ? 0.0 : sn[u] * apk[u] + c[u] * aqk[u];
+          }
       }
       t.sync();
     }
   }
-  for (int i = t.lane; i < n; i += t.size) w[i] = A[(long long)i * n + i];
+  for (int i = t.lane; i < n; i += t.size) w[i] = A[(int)i * n + i];
   t.sync();
 }
 
@@ -233,21 +312,21 @@ FU_HD inline void householder_qr(const Team& t, View M, int m, int n, View Q, Vi
   for (int j = 0; j < k; ++j) {
     double nl = 0.0;
     for (int i = t.lane; i < m; i += t.size)
-      if (i >= j) nl += M[(long long)i * n + j] * M[(long long)i * n + j];
+      if (i >= j) nl += M[(int)i * n + j] * M[(int)i * n + j];
     const double nrm = sqrt(t.sum(nl));  // the sum's barrier also publishes M[j][j]
     if (nrm == 0.0) {
       if (t.lane == 0) tau[j] = 0.0;
-      for (int i = t.lane; i < m; i += t.size) vs[(long long)i * k + j] = 0.0;
+      for (int i = t.lane; i < m; i += t.size) vs[(int)i * k + j] = 0.0;
       continue;
     }
-    const double x0 = M[(long long)j * n + j];
+    const double x0 = M[(int)j * n + j];
     const double alpha = x0 >= 0 ? -nrm : nrm;
     // v = x - alpha e_j, normalised so that v_j = 1
     const double v0 = x0 - alpha;
     double vl = 0.0;
     for (int i = t.lane; i < m; i += t.size) {
-      const double v = i < j ? 0.0 : (i == j ? 1.0 : M[(long long)i * n + j] / v0);
-      vs[(long long)i * k + j] = v;
+      const double v = i < j ? 0.0 : (i == j ? 1.0 : M[(int)i * n + j] / v0);
+      vs[(int)i * k + j] = v;
       vl += v * v;
     }
     const double tj = 2.0 / t.sum(vl);
@@ -255,39 +334,39 @@ FU_HD inline void householder_qr(const Team& t, View M, int m, int n, View Q, Vi
     for (int c = j; c < n; ++c) {
       double dl = 0.0;
       for (int i = t.lane; i < m; i += t.size)
-        if (i >= j) dl += vs[(long long)i * k + j] * M[(long long)i * n + c];
+        if (i >= j) dl += vs[(int)i * k + j] * M[(int)i * n + c];
       const double dt = t.sum(dl) * tj;
       for (int i = t.lane; i < m; i += t.size)
-        if (i >= j) M[(long long)i * n + c] -= dt * vs[(long long)i * k + j];
+        if (i >= j) M[(int)i * n + c] -= dt * vs[(int)i * k + j];
     }
   }
   t.sync();
   // the sign of each R diagonal entry, read from the final M, flips its row of
   // R and its column of Q
-  for (long long idx = t.lane; idx < (long long)k * n; idx += t.size) {
+  for (int idx = t.lane; idx < (int)k * n; idx += t.size) {
     const int i = (int)(idx / n), c = (int)(idx % n);
-    const double sg = M[(long long)i * n + i] < 0.0 ? -1.0 : 1.0;
+    const double sg = M[(int)i * n + i] < 0.0 ? -1.0 : 1.0;
     R[idx] = c >= i ? sg * M[idx] : 0.0;
   }
   if (Q.p != nullptr) {
     // Q = H_0 ... H_{k-1} I[:, :k]
     for (int i = t.lane; i < m; i += t.size)
-      for (int c = 0; c < k; ++c) Q[(long long)i * k + c] = i == c ? 1.0 : 0.0;
+      for (int c = 0; c < k; ++c) Q[(int)i * k + c] = i == c ? 1.0 : 0.0;
     for (int j = k - 1; j >= 0; --j) {
       const double tj = tau[j];
       if (tj == 0.0) continue;
       for (int c = 0; c < k; ++c) {
         double dl = 0.0;
         for (int i = t.lane; i < m; i += t.size)
-          if (i >= j) dl += vs[(long long)i * k + j] * Q[(long long)i * k + c];
+          if (i >= j) dl += vs[(int)i * k + j] * Q[(int)i * k + c];
         const double dt = t.sum(dl) * tj;
         for (int i = t.lane; i < m; i += t.size)
-          if (i >= j) Q[(long long)i * k + c] -= dt * vs[(long long)i * k + j];
+          if (i >= j) Q[(int)i * k + c] -= dt * vs[(int)i * k + j];
       }
     }
     for (int i = t.lane; i < m; i += t.size)
       for (int c = 0; c < k; ++c)
-        if (M[(long long)c * n + c] < 0.0) Q[(long long)i * k + c] = -Q[(long long)i * k + c];
+        if (M[(int)c * n + c] < 0.0) Q[(int)i * k + c] = -Q[(int)i * k + c];
   }
   t.sync();
 }
@@ -297,14 +376,14 @@ FU_HD inline void householder_qr(const Team& t, View M, int m, int n, View Q, Vi
 // and Vw are lane-owned; the three column products of a pair are one reduction.
 FU_HD inline void jacobi_svd_tall(const Team& t, View M, int m, int n, View U, View S, View Vt, View Vw) {
   for (int i = t.lane; i < n; i += t.size)
-    for (int j = 0; j < n; ++j) Vw[(long long)i * n + j] = i == j ? 1.0 : 0.0;
+    for (int j = 0; j < n; ++j) Vw[(int)i * n + j] = i == j ? 1.0 : 0.0;
   for (int sweep = 0; sweep < 80; ++sweep) {
     bool rotated = false;
     for (int p = 0; p < n - 1; ++p)
       for (int q = p + 1; q < n; ++q) {
         double a = 0.0, b = 0.0, g = 0.0;
         for (int i = t.lane; i < m; i += t.size) {
-          const double x = M[(long long)i * n + p], y = M[(long long)i * n + q];
+          const double x = M[(int)i * n + p], y = M[(int)i * n + q];
           a += x * x;
           b += y * y;
           g += x * y;
@@ -316,14 +395,14 @@ FU_HD inline void jacobi_svd_tall(const Team& t, View M, int m, int n, View U, V
         const double tt = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
         const double c = 1.0 / sqrt(1.0 + tt * tt), s = c * tt;
         for (int i = t.lane; i < m; i += t.size) {
-          const double x = M[(long long)i * n + p], y = M[(long long)i * n + q];
-          M[(long long)i * n + p] = c * x - s * y;
-          M[(long long)i * n + q] = s * x + c * y;
+          const double x = M[(int)i * n + p], y = M[(int)i * n + q];
+          M[(int)i * n + p] = c * x - s * y;
+          M[(int)i * n + q] = s * x + c * y;
         }
         for (int i = t.lane; i < n; i += t.size) {
-          const double x = Vw[(long long)i * n + p], y = Vw[(long long)i * n + q];
-          Vw[(long long)i * n + p] = c * x - s * y;
-          Vw[(long long)i * n + q] = s * x + c * y;
+          const double x = Vw[(int)i * n + p], y = Vw[(int)i * n + q];
+          Vw[(int)i * n + p] = c * x - s * y;
+          Vw[(int)i * n + q] = s * x + c * y;
         }
       }
     if (!rotated) break;
@@ -331,7 +410,7 @@ FU_HD inline void jacobi_svd_tall(const Team& t, View M, int m, int n, View U, V
   // column norms, then a selection sort into descending order
   for (int j = 0; j < n; ++j) {
     double al = 0.0;
-    for (int i = t.lane; i < m; i += t.size) al += M[(long long)i * n + j] * M[(long long)i * n + j];
+    for (int i = t.lane; i < m; i += t.size) al += M[(int)i * n + j] * M[(int)i * n + j];
     const double a = sqrt(t.sum(al));
     if (t.lane == 0) S[j] = a;
   }
@@ -348,22 +427,22 @@ FU_HD inline void jacobi_svd_tall(const Team& t, View M, int m, int n, View U, V
         S[best] = x;
       }
       for (int i = t.lane; i < m; i += t.size) {
-        const double x = M[(long long)i * n + j];
-        M[(long long)i * n + j] = M[(long long)i * n + best];
-        M[(long long)i * n + best] = x;
+        const double x = M[(int)i * n + j];
+        M[(int)i * n + j] = M[(int)i * n + best];
+        M[(int)i * n + best] = x;
       }
       for (int i = t.lane; i < n; i += t.size) {
-        const double x = Vw[(long long)i * n + j];
-        Vw[(long long)i * n + j] = Vw[(long long)i * n + best];
-        Vw[(long long)i * n + best] = x;
+        const double x = Vw[(int)i * n + j];
+        Vw[(int)i * n + j] = Vw[(int)i * n + best];
+        Vw[(int)i * n + best] = x;
       }
     }
     t.sync();
   }
   for (int j = 0; j < n; ++j) {
     const double inv = S[j] > 0.0 ? 1.0 / S[j] : 0.0;
-    for (int i = t.lane; i < m; i += t.size) U[(long long)i * n + j] = M[(long long)i * n + j] * inv;
-    for (int i = t.lane; i < n; i += t.size) Vt[(long long)j * n + i] = Vw[(long long)i * n + j];
+    for (int i = t.lane; i < m; i += t.size) U[(int)i * n + j] = M[(int)i * n + j] * inv;
+    for (int i = t.lane; i < n; i += t.size) Vt[(int)j * n + i] = Vw[(int)i * n + j];
   }
   t.sync();
 }
@@ -373,30 +452,30 @@ FU_HD inline void jacobi_svd_tall(const Team& t, View M, int m, int n, View U, V
 FU_HD inline void svd_compact(const Team& t, View M, int m, int n, View U, View S, View Vt, Work& w) {
   const long long mark = w.top;
   if (m >= n) {
-    View T = w.take((long long)m * n);
-    for (long long i = t.lane; i < (long long)m * n; i += t.size) T[i] = M[i];
-    View Vw = w.take((long long)n * n);
+    View T = w.take((int)m * n);
+    for (int i = t.lane; i < (int)m * n; i += t.size) T[i] = M[i];
+    View Vw = w.take((int)n * n);
     t.sync();
     jacobi_svd_tall(t, T, m, n, U, S, Vt, Vw);
   } else {
     // M^T = U' S V'^T  =>  M = V' S U'^T
-    View T = w.take((long long)m * n);
-    for (long long idx = t.lane; idx < (long long)m * n; idx += t.size) {
+    View T = w.take((int)m * n);
+    for (int idx = t.lane; idx < (int)m * n; idx += t.size) {
       const int i = (int)(idx / n), j = (int)(idx % n);
-      T[(long long)j * m + i] = M[idx];
+      T[(int)j * m + i] = M[idx];
     }
-    View Uu = w.take((long long)n * m);
-    View Vv = w.take((long long)m * m);
-    View Vw = w.take((long long)m * m);
+    View Uu = w.take((int)n * m);
+    View Vv = w.take((int)m * m);
+    View Vw = w.take((int)m * m);
     t.sync();
     jacobi_svd_tall(t, T, n, m, Uu, S, Vv, Vw);
-    for (long long idx = t.lane; idx < (long long)m * m; idx += t.size) {
+    for (int idx = t.lane; idx < (int)m * m; idx += t.size) {
       const int i = (int)(idx / m), k = (int)(idx % m);
-      U[idx] = Vv[(long long)k * m + i];
+      U[idx] = Vv[(int)k * m + i];
     }
-    for (long long idx = t.lane; idx < (long long)m * n; idx += t.size) {
+    for (int idx = t.lane; idx < (int)m * n; idx += t.size) {
       const int k = (int)(idx / n), j = (int)(idx % n);
-      Vt[idx] = Uu[(long long)j * m + k];
+      Vt[idx] = Uu[(int)j * m + k];
     }
     t.sync();
   }
@@ -417,45 +496,49 @@ FU_HD inline void block_qr(const Team& t, View M, int m, int n, RP rpar, CP cpar
   const int K = kcap[0] + kcap[1];
   const bool want_q = Q.p != nullptr;
   if (want_q)
-    for (long long i = t.lane; i < (long long)m * K; i += t.size) Q[i] = 0.0;
-  for (long long i = t.lane; i < (long long)K * n; i += t.size) R[i] = 0.0;
+    for (int i = t.lane; i < m * K; i += t.size) Q[i] = 0.0;
+  for (int i = t.lane; i < K * n; i += t.size) R[i] = 0.0;
+  // each lane walks a contiguous range of rows; a prefix count over lanes
+  // gives the block row index of its first row of parity q
+  const int per = (m + t.size - 1) / t.size;
+  const int r0 = t.lane * per < m ? t.lane * per : m;
+  const int r1 = r0 + per < m ? r0 + per : m;
   for (int q = 0; q < 2; ++q) {
     const long long mark = w.top;
-    int mq = 0, nq = 0;
-    for (int i = 0; i < m; ++i) mq += rpar(i) == q;
+    int cnt = 0;
+    for (int i = r0; i < r1; ++i) cnt += rpar(i) == q;
+    int mq = 0;
+    const int first = t.scan(cnt, mq);
+    int nq = 0;
     for (int j = 0; j < n; ++j) nq += cpar(j) == q;
     if (mq == 0 || nq == 0) continue;
-    View B = w.take((long long)mq * nq);
-    // block row bi is gathered by its owner lane bi % size
+    View B = w.take(mq * nq);
     {
-      int bi = 0;
-      for (int i = 0; i < m; ++i) {
+      int bi = first;
+      for (int i = r0; i < r1; ++i) {
         if (rpar(i) != q) continue;
-        if (bi % t.size == t.lane) {
-          int bj = 0;
-          for (int j = 0; j < n; ++j) {
-            if (cpar(j) != q) continue;
-            B[(long long)bi * nq + bj] = M[(long long)i * n + j];
-            ++bj;
-          }
+        int bj = 0;
+        for (int j = 0; j < n; ++j) {
+          if (cpar(j) != q) continue;
+          B[bi * nq + bj] = M[i * n + j];
+          ++bj;
         }
         ++bi;
       }
     }
     const int kq = mq < nq ? mq : nq;
-    View Qb = want_q ? w.take((long long)mq * kq) : View{nullptr, 0};
-    View Rb = w.take((long long)kq * nq);
-    View vs = w.take((long long)mq * kq), tau = w.take(kq);
+    View Qb = want_q ? w.take(mq * kq) : View{nullptr, 0};
+    View Rb = w.take(kq * nq);
+    View vs = w.take(mq * kq), tau = w.take(kq);
     t.sync();
     householder_qr(t, B, mq, nq, Qb, Rb, vs, tau);
     const int off = q == 0 ? 0 : kcap[0];
     const int kput = kq < kcap[q] ? kq : kcap[q];
     if (want_q) {
-      int bi = 0;
-      for (int i = 0; i < m; ++i) {
+      int bi = first;
+      for (int i = r0; i < r1; ++i) {
         if (rpar(i) != q) continue;
-        if (bi % t.size == t.lane)
-          for (int c = 0; c < kput; ++c) Q[(long long)i * K + off + c] = Qb[(long long)bi * kq + c];
+        for (int c = 0; c < kput; ++c) Q[i * K + off + c] = Qb[bi * kq + c];
         ++bi;
       }
     }
@@ -463,7 +546,7 @@ FU_HD inline void block_qr(const Team& t, View M, int m, int n, RP rpar, CP cpar
       int bj = 0;
       for (int j = 0; j < n; ++j) {
         if (cpar(j) != q) continue;
-        R[(long long)(off + c) * n + j] = Rb[(long long)c * nq + bj];
+        R[(off + c) * n + j] = Rb[c * nq + bj];
         ++bj;
       }
     }
@@ -477,7 +560,7 @@ FU_HD inline void block_qr(const Team& t, View M, int m, int n, RP rpar, CP cpar
 // parity-sorted indices (even count e), one column per lane. Returns false
 // (on every lane) on a zero or nonfinite pivot.
 FU_HD inline bool triangular_inverse(const Team& t, View R, int n, int e, View Ri) {
-  for (long long i = t.lane; i < (long long)n * n; i += t.size) Ri[i] = 0.0;
+  for (int i = t.lane; i < (int)n * n; i += t.size) Ri[i] = 0.0;
   t.sync();
   double bad = 0.0;
   for (int c = t.lane; c < n; c += t.size) {
@@ -485,13 +568,13 @@ FU_HD inline bool triangular_inverse(const Team& t, View R, int n, int e, View R
     // solve R x = e_c for the column c of Ri, rows lo..c
     for (int i = c; i >= lo; --i) {
       double s = i == c ? 1.0 : 0.0;
-      for (int k = i + 1; k <= c; ++k) s -= R[(long long)i * n + k] * Ri[(long long)k * n + c];
-      const double piv = R[(long long)i * n + i];
+      for (int k = i + 1; k <= c; ++k) s -= R[(int)i * n + k] * Ri[(int)k * n + c];
+      const double piv = R[(int)i * n + i];
       if (piv == 0.0 || !finite(piv)) {
         bad = 1.0;
         break;
       }
-      Ri[(long long)i * n + c] = s / piv;
+      Ri[(int)i * n + c] = s / piv;
     }
   }
   return t.sum(bad) == 0.0;
@@ -505,9 +588,9 @@ FU_HD inline bool triangular_inverse(const Team& t, View R, int n, int e, View R
 // out[r,p,q,s] = sum_{t,u} N[r,s,t,u] theta[t,p,q,u] for a Z2-symmetric N
 FU_HD inline void apply_metric(const Team& tm, View N, View th, View out, const Dims& g) {
   const int na = g.na, nb = g.nb, d = g.d;
-  const long long tot = (long long)na * nb * d * d;
-  for (long long idx = tm.lane; idx < tot; idx += tm.size) {
-    long long r_ = idx;
+  const int tot = (int)na * nb * d * d;
+  for (int idx = tm.lane; idx < tot; idx += tm.size) {
+    int r_ = idx;
     const int q = (int)(r_ % d);
     r_ /= d;
     const int p = (int)(r_ % d);
@@ -522,41 +605,41 @@ FU_HD inline void apply_metric(const Team& tm, View N, View th, View out, const 
       const int pu = prs ^ par(t, g.ea);
       const int u0 = pu == 0 ? 0 : g.eb, u1 = pu == 0 ? g.eb : nb;
       for (int u = u0; u < u1; ++u)
-        acc += N[(((long long)r * nb + s) * na + t) * nb + u] * th[(((long long)t * d + p) * d + q) * nb + u];
+        acc += N[(((int)r * nb + s) * na + t) * nb + u] * th[(((int)t * d + p) * d + q) * nb + u];
     }
-    out[(((long long)r * d + p) * d + q) * nb + s] = acc;
+    out[(((int)r * d + p) * d + q) * nb + s] = acc;
   }
   tm.sync();
 }
 
 FU_HD inline void pair_tensor(const Team& tm, View A, View B, View out, const Dims& g) {
   const int na = g.na, nb = g.nb, d = g.d, D = g.D;
-  const long long tot = (long long)na * d * d * nb;
-  for (long long idx = tm.lane; idx < tot; idx += tm.size) {
+  const int tot = (int)na * d * d * nb;
+  for (int idx = tm.lane; idx < tot; idx += tm.size) {
     const int s = (int)(idx % nb);
-    const long long rpq = idx / nb;
+    const int rpq = idx / nb;
     const int q = (int)(rpq % d);
-    const long long rp = rpq / d;
+    const int rp = rpq / d;
     double acc = 0.0;
-    for (int k = 0; k < D; ++k) acc += A[rp * D + k] * B[((long long)k * d + q) * nb + s];
+    for (int k = 0; k < D; ++k) acc += A[rp * D + k] * B[((int)k * d + q) * nb + s];
     out[idx] = acc;
   }
   tm.sync();
 }
 
-FU_HD inline double dot(const Team& tm, View a, View b, long long n) {
+FU_HD inline double dot(const Team& tm, View a, View b, int n) {
   double s = 0.0;
-  for (long long i = tm.lane; i < n; i += tm.size) s += a[i] * b[i];
+  for (int i = tm.lane; i < n; i += tm.size) s += a[i] * b[i];
   return tm.sum(s);
 }
 
 // res . N res with res = pair(A, B) - target
 FU_HD inline double residual_cost(const Team& tm, View N, View A, View B, View target, const Dims& g, Work& w) {
   const long long mark = w.top;
-  const long long n = (long long)g.na * g.d * g.d * g.nb;
+  const int n = (int)g.na * g.d * g.d * g.nb;
   View res = w.take(n), nr = w.take(n);
   pair_tensor(tm, A, B, res, g);
-  for (long long i = tm.lane; i < n; i += tm.size) res[i] -= target[i];
+  for (int i = tm.lane; i < n; i += tm.size) res[i] -= target[i];
   tm.sync();
   apply_metric(tm, N, res, nr, g);
   const double v = dot(tm, res, nr, n);
@@ -567,18 +650,18 @@ FU_HD inline double residual_cost(const Team& tm, View N, View A, View B, View t
 // A'[i,p,k] = sum_x Li[i][x] A[x,p,k];  B'[k,q,i] = sum_y Ri[i][y] B[k,q,y]
 FU_HD inline void original_cores(const Team& tm, View A, View B, View Li, View Ri, View oA, View oB, const Dims& g) {
   const int na = g.na, nb = g.nb, d = g.d, D = g.D;
-  for (long long idx = tm.lane; idx < (long long)na * d * D; idx += tm.size) {
-    const long long pk = idx % ((long long)d * D);
-    const int i = (int)(idx / ((long long)d * D));
+  for (int idx = tm.lane; idx < (int)na * d * D; idx += tm.size) {
+    const int pk = idx % ((int)d * D);
+    const int i = (int)(idx / ((int)d * D));
     double acc = 0.0;
-    for (int x = 0; x < na; ++x) acc += Li[(long long)i * na + x] * A[(long long)x * d * D + pk];
+    for (int x = 0; x < na; ++x) acc += Li[(int)i * na + x] * A[(int)x * d * D + pk];
     oA[idx] = acc;
   }
-  for (long long idx = tm.lane; idx < (long long)D * d * nb; idx += tm.size) {
+  for (int idx = tm.lane; idx < (int)D * d * nb; idx += tm.size) {
     const int i = (int)(idx % nb);
-    const long long kq = idx / nb;
+    const int kq = idx / nb;
     double acc = 0.0;
-    for (int y = 0; y < nb; ++y) acc += Ri[(long long)i * nb + y] * B[kq * nb + y];
+    for (int y = 0; y < nb; ++y) acc += Ri[(int)i * nb + y] * B[kq * nb + y];
     oB[idx] = acc;
   }
   tm.sync();
@@ -588,11 +671,11 @@ FU_HD inline void original_cores(const Team& tm, View A, View B, View Li, View R
 // on the n x n matrix M for nc right-hand-side columns (x, rhs: n x nc).
 FU_HD inline double solve_core(const Team& tm, View M, int n, int nc, View rhs, View x, const Opts& o, Work& w) {
   const long long mark = w.top;
-  const long long nv = (long long)n * nc;
+  const int nv = (int)n * nc;
   View pre = w.take(n), r = w.take(nv), z = w.take(nv), p = w.take(nv), Ap = w.take(nv);
   double sl = 0.0;
   for (int i = tm.lane; i < n; i += tm.size) {
-    const double dg = M[(long long)i * n + i] > 0.0 ? M[(long long)i * n + i] : 0.0;
+    const double dg = M[(int)i * n + i] > 0.0 ? M[(int)i * n + i] : 0.0;
     pre[i] = dg;
     if (dg > sl) sl = dg;
   }
@@ -604,17 +687,17 @@ FU_HD inline double solve_core(const Team& tm, View M, int n, int nc, View rhs, 
   for (int i = tm.lane; i < n; i += tm.size) pre[i] = (pre[i] + shift) > floor ? (pre[i] + shift) : floor;
   auto action = [&](View v, View out) {
     tm.sync();
-    for (long long idx = tm.lane; idx < nv; idx += tm.size) {
+    for (int idx = tm.lane; idx < nv; idx += tm.size) {
       const int i = (int)(idx / nc), c = (int)(idx % nc);
       double acc = 0.0;
-      for (int k = 0; k < n; ++k) acc += M[(long long)i * n + k] * v[(long long)k * nc + c];
+      for (int k = 0; k < n; ++k) acc += M[(int)i * n + k] * v[(int)k * nc + c];
       out[idx] = acc + shift * v[idx];
     }
     tm.sync();
   };
   action(x, Ap);
   double rl = 0.0;
-  for (long long i = tm.lane; i < nv; i += tm.size) {
+  for (int i = tm.lane; i < nv; i += tm.size) {
     r[i] = rhs[i] - Ap[i];
     rl += rhs[i] * rhs[i];
   }
@@ -625,7 +708,7 @@ FU_HD inline double solve_core(const Team& tm, View M, int n, int nc, View rhs, 
     w.top = mark;
     return residual;
   }
-  for (long long i = tm.lane; i < nv; i += tm.size) {
+  for (int i = tm.lane; i < nv; i += tm.size) {
     z[i] = r[i] / pre[i / nc];
     p[i] = z[i];
   }
@@ -633,7 +716,7 @@ FU_HD inline double solve_core(const Team& tm, View M, int n, int nc, View rhs, 
   for (int it = 0; it < o.linear_max_iterations; ++it) {
     action(p, Ap);
     double curv = 0.0, pp = 0.0, aa = 0.0;
-    for (long long i = tm.lane; i < nv; i += tm.size) {
+    for (int i = tm.lane; i < nv; i += tm.size) {
       curv += p[i] * Ap[i];
       pp += p[i] * p[i];
       aa += Ap[i] * Ap[i];
@@ -643,7 +726,7 @@ FU_HD inline double solve_core(const Team& tm, View M, int n, int nc, View rhs, 
     if (!(curv > kEps * lim)) break;
     const double alpha = rz / curv;
     double rr = 0.0, nxt = 0.0;
-    for (long long i = tm.lane; i < nv; i += tm.size) {
+    for (int i = tm.lane; i < nv; i += tm.size) {
       x[i] += alpha * p[i];
       r[i] -= alpha * Ap[i];
       z[i] = r[i] / pre[i / nc];
@@ -655,12 +738,73 @@ FU_HD inline double solve_core(const Team& tm, View M, int n, int nc, View rhs, 
     if (residual <= o.linear_tolerance) break;
     if (!(nxt >= 0)) break;
     const double beta = nxt / (rz > kTiny ? rz : kTiny);
-    for (long long i = tm.lane; i < nv; i += tm.size) p[i] = z[i] + beta * p[i];
+    for (int i = tm.lane; i < nv; i += tm.size) p[i] = z[i] + beta * p[i];
     rz = nxt;
   }
   tm.sync();
   w.top = mark;
   return residual;
+}
+
+// One ALS normal matrix from a Z2-symmetric metric (full_update.jl:402-416),
+// written for both sides:
+//   out[(x,b),(y,o)] = sum_{xi,yi} M[x,xi,y,yi] P[(b,xi),(o,yi)]
+// with (x, y) the outer pair (no entries, even count eo), (xi, yi) the summed
+// pair (ni, ei), b, o the kept bond (D, eD). The metric vanishes unless
+// par(x) + par(xi) = par(y) + par(yi), so yi runs over one parity sector, and
+// the result vanishes unless par(x) + par(b) = par(y) + par(o). One lane per
+// (x, y, b) keeps its row of D entries in registers; P is read by all lanes
+// (staged in LDS by the caller when it fits). The terms of every entry are
+// added in the order (xi outer, yi inner).
+struct NormalShape {
+  int no, eo, ni, ei;
+  int mx, my, mxi, myi;  // metric strides
+  int pxi, pyi, pb, po;  // P strides
+  int ox, oy, ob, oo;    // output strides
+};
+
+constexpr int kTileD = 8;  // register rows cover kept bonds up to this size
+
+FU_HD inline void normal_matrix(const Team& t, View M, View P, View out, const NormalShape& h, int D, int eD) {
+  for (int task = t.lane; task < (int)h.no * h.no * D; task += t.size) {
+    const int b = (int)(task % D);
+    const int xy = task / D;
+    const int x = (int)(xy / h.no), y = (int)(xy % h.no);
+    const int px = par(x, h.eo), py = par(y, h.eo);
+    const int obase = x * h.ox + y * h.oy + b * h.ob;
+    if (D <= kTileD) {
+      double acc[kTileD];
+#pragma unroll
+      for (int o = 0; o < kTileD; ++o) acc[o] = 0.0;
+      for (int xi = 0; xi < h.ni; ++xi) {
+        const int pyi = px ^ py ^ par(xi, h.ei);
+        const int y0 = pyi == 0 ? 0 : h.ei, y1 = pyi == 0 ? h.ei : h.ni;
+        for (int yi = y0; yi < y1; ++yi) {
+          const double m = M[x * h.mx + y * h.my + xi * h.mxi + yi * h.myi];
+          const int pbase = xi * h.pxi + yi * h.pyi + b * h.pb;
+#pragma unroll
+          for (int o = 0; o < kTileD; ++o)
+            if (o < D) acc[o] += m * P[pbase + o * h.po];
+        }
+      }
+#pragma unroll
+      for (int o = 0; o < kTileD; ++o)
+        if (o < D) out[obase + o * h.oo] = (px ^ par(b, eD)) == (py ^ par(o, eD)) ? acc[o] : 0.0;
+    } else {
+      for (int o = 0; o < D; ++o) {
+        double acc = 0.0;
+        if ((px ^ par(b, eD)) == (py ^ par(o, eD)))
+          for (int xi = 0; xi < h.ni; ++xi) {
+            const int pyi = px ^ py ^ par(xi, h.ei);
+            const int y0 = pyi == 0 ? 0 : h.ei, y1 = pyi == 0 ? h.ei : h.ni;
+            for (int yi = y0; yi < y1; ++yi)
+              acc += M[x * h.mx + y * h.my + xi * h.mxi + yi * h.myi] * P[xi * h.pxi + yi * h.pyi + b * h.pb + o * h.po];
+          }
+        out[obase + o * h.oo] = acc;
+      }
+    }
+  }
+  t.sync();
 }
 
 // Doubles of workspace fu_bond needs for these dimensions.
@@ -689,10 +833,11 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
   Result res{};
   res.stop = o.max_iterations == 0 ? kDisabled : kIterationLimit;
   const int na = g.na, nb = g.nb, d = g.d, D = g.D;
-  const long long n = (long long)na * nb;
-  const long long tn = (long long)na * d * d * nb;
+  const int n = (int)na * nb;
+  const int tn = (int)na * d * d * nb;
   const int L = t.lane, NT = t.size;
-  if (workspace_doubles(g) > w.cap) {
+  // indices below are 32-bit: every array lies inside a workspace of < 2^31 doubles
+  if (workspace_doubles(g) > w.cap || workspace_doubles(g) > 0x7fffffff) {
     res.status = kWorkspace;
     return res;
   }
@@ -703,7 +848,7 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
   double scale = 0.0;
   {
     double sl = 0.0, bad = 0.0;
-    for (long long i = L; i < n * n; i += NT) {
+    for (int i = L; i < n * n; i += NT) {
       const double v = N4[i];
       if (!finite(v)) bad = 1.0;
       else sl += v * v;
@@ -727,10 +872,10 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
     View bmap = w.take(n);      // block-major position -> (rk, sk) index
     View binv = w.take(n);      // (rk, sk) index -> position inside its block
     int cnt[2] = {0, 0};
-    for (long long I = 0; I < n; ++I) ++cnt[pa((int)(I / nb)) ^ pb((int)(I % nb))];
+    for (int I = 0; I < n; ++I) ++cnt[pa((int)(I / nb)) ^ pb((int)(I % nb))];
     if (L == 0) {
       int fill[2] = {0, 0};
-      for (long long I = 0; I < n; ++I) {
+      for (int I = 0; I < n; ++I) {
         const int q = pa((int)(I / nb)) ^ pb((int)(I % nb));
         bmap[(q == 0 ? 0 : cnt[0]) + fill[q]] = (double)I;
         binv[I] = fill[q];
@@ -740,72 +885,83 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
     View Hq = w.take(n * n), cs = w.take(3 * (n / 2 + 1));
     View evals = w.take(n), evecs = w.take(n * n);  // block-major storage
     t.sync();
-    long long eoff = 0, voff = 0;
+    int eoff = 0, voff = 0;
     for (int q = 0; q < 2; ++q) {
       const int m = cnt[q];
       if (m == 0) continue;
-      const long long boff = q == 0 ? 0 : cnt[0];
-      for (long long idx = L; idx < (long long)m * m; idx += NT) {
-        const long long I = (long long)bmap[boff + idx / m], K = (long long)bmap[boff + idx % m];
+      const int boff = q == 0 ? 0 : cnt[0];
+      for (int idx = L; idx < (int)m * m; idx += NT) {
+        const int I = (int)bmap[boff + idx / m], K = (int)bmap[boff + idx % m];
         Hq[idx] = 0.5 * (N4[I * n + K] + N4[K * n + I]) / scale;
       }
       t.sync();
       View wq{&evals[eoff], evals.s};
       View vq{&evecs[voff], evecs.s};
-      jacobi_eigh(t, Hq, m, wq, vq, cs);
+      const int mm = (int)m * m;
+      if (2 * mm + 3 * (m / 2 + 1) <= t.lds_cap) {
+        // the block and its eigenvectors in LDS
+        const View As{t.lds, 1}, Vs{t.lds + mm, 1}, cl{t.lds + 2 * mm, 1};
+        for (int i = L; i < mm; i += NT) As[i] = Hq[i];
+        t.sync();
+        jacobi_eigh(t, As, m, wq, Vs, cl);
+        for (int i = L; i < mm; i += NT) vq[i] = Vs[i];
+        t.sync();
+      } else {
+        jacobi_eigh(t, Hq, m, wq, vq, cs);
+      }
       eoff += m;
-      voff += (long long)m * m;
+      voff += (int)m * m;
     }
     // eigenvalues of both blocks set the spectral scale
     double spectral = 0.0;
-    for (long long k = 0; k < eoff; ++k)
+    for (int k = 0; k < eoff; ++k)
       if (fabs(evals[k]) > spectral) spectral = fabs(evals[k]);
     if (spectral == 0.0) spectral = 1.0;
     double rel = sqrt(kEps);
     if ((double)n * kEps > rel) rel = (double)n * kEps;
     const double floorv = rel * spectral;
-    for (long long k = 0; k < eoff; ++k)
+    for (int k = 0; k < eoff; ++k)
       if (evals[k] < floorv) ++res.floored;
     // Z row (block q, eigenvector k) = sqrt(max(lambda_k, floor)) v_k on the
     // block's (rk, sk) entries, zero elsewhere
-    for (long long idx = L; idx < n * n; idx += NT) {
-      const long long row = idx / n, K = idx % n;
+    for (int idx = L; idx < n * n; idx += NT) {
+      const int row = idx / n, K = idx % n;
       const int qr = row < cnt[0] ? 0 : 1;
       const int k = (int)(qr == 0 ? row : row - cnt[0]);
       const int m = cnt[qr];
-      const long long ev0 = qr == 0 ? 0 : cnt[0], vv0 = qr == 0 ? 0 : (long long)cnt[0] * cnt[0];
+      const int ev0 = qr == 0 ? 0 : cnt[0], vv0 = qr == 0 ? 0 : (int)cnt[0] * cnt[0];
       const int qk = pa((int)(K / nb)) ^ pb((int)(K % nb));
       double v = 0.0;
       if (qk == qr) {
         const double lam = evals[ev0 + k];
         const double root = sqrt(lam > floorv ? lam : floorv);
-        v = root * evecs[vv0 + (long long)binv[K] * m + k];
+        v = root * evecs[vv0 + (int)binv[K] * m + k];
       }
       Z[idx] = v;
     }
-    for (long long row = L; row < n; row += NT) zpar[row] = row < cnt[0] ? 0 : 1;
+    for (int row = L; row < n; row += NT) zpar[row] = row < cnt[0] ? 0 : 1;
     t.sync();
     w.top = mark;
   }
   // left = R of ((k, sk) | rk), right = R of ((k, rk) | sk)
-  View left = w.take((long long)na * na), right = w.take((long long)nb * nb);
-  View Li = w.take((long long)na * na), Ri = w.take((long long)nb * nb);
+  View left = w.take((int)na * na), right = w.take((int)nb * nb);
+  View Li = w.take((int)na * na), Ri = w.take((int)nb * nb);
   {
     const long long mark = w.top;
     const View none{nullptr, 0};
     View Ml = w.take(n * nb * na);
-    for (long long idx = L; idx < n * nb * na; idx += NT) {
-      const long long kl = idx / na;
+    for (int idx = L; idx < n * nb * na; idx += NT) {
+      const int kl = idx / na;
       const int a = (int)(idx % na);
-      Ml[idx] = Z[(kl / nb) * n + (long long)a * nb + kl % nb];
+      Ml[idx] = Z[(kl / nb) * n + (int)a * nb + kl % nb];
     }
     t.sync();
     int kc[2] = {g.ea, na - g.ea};
     block_qr(t, Ml, (int)(n * nb), na, [&](int i) { return ((int)zpar[i / nb]) ^ pb(i % nb); }, pa, kc, none, left, w);
     w.top = mark;
     View Mr = w.take(n * na * nb);
-    for (long long idx = L; idx < n * na * nb; idx += NT) {
-      const long long ka = idx / nb;
+    for (int idx = L; idx < n * na * nb; idx += NT) {
+      const int ka = idx / nb;
       const int l = (int)(idx % nb);
       Mr[idx] = Z[(ka / na) * n + (ka % na) * nb + l];
     }
@@ -825,35 +981,35 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
   {
     const long long mark = w.top;
     View T1 = w.take(n * n);
-    for (long long idx = L; idx < n * n; idx += NT) {
+    for (int idx = L; idx < n * n; idx += NT) {
       const int x = (int)(idx / (nb * n));
-      const long long rest = idx % (nb * n);
+      const int rest = idx % (nb * n);
       double acc = 0.0;
-      for (int i = 0; i < na; ++i) acc += Li[(long long)i * na + x] * N4[(long long)i * nb * n + rest];
+      for (int i = 0; i < na; ++i) acc += Li[(int)i * na + x] * N4[(int)i * nb * n + rest];
       T1[idx] = acc;
     }
     t.sync();
-    for (long long idx = L; idx < n * n; idx += NT) {
-      const long long xy = idx / n, rest = idx % n;
+    for (int idx = L; idx < n * n; idx += NT) {
+      const int xy = idx / n, rest = idx % n;
       const int x = (int)(xy / nb), y = (int)(xy % nb);
       double acc = 0.0;
-      for (int j = 0; j < nb; ++j) acc += Ri[(long long)j * nb + y] * T1[((long long)x * nb + j) * n + rest];
+      for (int j = 0; j < nb; ++j) acc += Ri[(int)j * nb + y] * T1[((int)x * nb + j) * n + rest];
       metric[idx] = acc;
     }
     t.sync();
-    for (long long idx = L; idx < n * n; idx += NT) {
-      const long long xy = idx / n;
+    for (int idx = L; idx < n * n; idx += NT) {
+      const int xy = idx / n;
       const int z = (int)((idx % n) / nb), l = (int)(idx % nb);
       double acc = 0.0;
-      for (int k = 0; k < na; ++k) acc += Li[(long long)k * na + z] * metric[xy * n + (long long)k * nb + l];
+      for (int k = 0; k < na; ++k) acc += Li[(int)k * na + z] * metric[xy * n + (int)k * nb + l];
       T1[idx] = acc;
     }
     t.sync();
-    for (long long idx = L; idx < n * n; idx += NT) {
-      const long long xyz = idx / nb;
+    for (int idx = L; idx < n * n; idx += NT) {
+      const int xyz = idx / nb;
       const int ww = (int)(idx % nb);
       double acc = 0.0;
-      for (int l = 0; l < nb; ++l) acc += Ri[(long long)l * nb + ww] * T1[xyz * nb + l];
+      for (int l = 0; l < nb; ++l) acc += Ri[(int)l * nb + ww] * T1[xyz * nb + l];
       metric[idx] = acc;
     }
     t.sync();
@@ -861,16 +1017,16 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
   }
   // working target wt[x,p,q,y] = sum target[r,p,q,s] left[x,r] right[y,s]
   View wt = w.take(tn);
-  for (long long idx = L; idx < tn; idx += NT) {
+  for (int idx = L; idx < tn; idx += NT) {
     const int y = (int)(idx % nb);
-    const long long xpq = idx / nb;
-    const int q = (int)(xpq % d), p = (int)((xpq / d) % d), x = (int)(xpq / ((long long)d * d));
+    const int xpq = idx / nb;
+    const int q = (int)(xpq % d), p = (int)((xpq / d) % d), x = (int)(xpq / ((int)d * d));
     double acc = 0.0;
     for (int r = 0; r < na; ++r) {
-      const double lx = left[(long long)x * na + r];
+      const double lx = left[(int)x * na + r];
       if (lx == 0.0) continue;
       for (int s = 0; s < nb; ++s)
-        acc += lx * right[(long long)y * nb + s] * target[(((long long)r * d + p) * d + q) * nb + s];
+        acc += lx * right[(int)y * nb + s] * target[(((int)r * d + p) * d + q) * nb + s];
     }
     wt[idx] = acc;
   }
@@ -879,13 +1035,13 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
   // The cores live in the kept bond (ke even + ko odd, ke + ko <= D) from here
   // on, as the reference's bond space does; the output is expanded to the
   // device's padded capacity (even slots [0, eD), odd slots [eD, D)) at the end.
-  const long long ca0 = (long long)na * d * D, cb0 = (long long)D * d * nb;
+  const int ca0 = (int)na * d * D, cb0 = (int)D * d * nb;
   View A = w.take(ca0), B = w.take(cb0);
   int ke = 0, ko = 0;
   {
     const long long mark = w.top;
-    for (long long i = L; i < ca0; i += NT) A[i] = 0.0;
-    for (long long i = L; i < cb0; i += NT) B[i] = 0.0;
+    for (int i = L; i < ca0; i += NT) A[i] = 0.0;
+    for (int i = L; i < cb0; i += NT) B[i] = 0.0;
     const int mr = na * d, mc = d * nb;
     const double wn = sqrt(dot(t, wt, wt, tn));
     int mxd = na > nb ? na : nb;
@@ -899,10 +1055,10 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
     for (int q = 0; q < 2; ++q) {
       kq[q] = mq[q] < nq[q] ? mq[q] : nq[q];
       Sv[q] = w.take(kq[q] > 0 ? kq[q] : 1);
-      Uv[q] = w.take((long long)mq[q] * (kq[q] > 0 ? kq[q] : 1));
-      Vv[q] = w.take((long long)(kq[q] > 0 ? kq[q] : 1) * nq[q]);
+      Uv[q] = w.take((int)mq[q] * (kq[q] > 0 ? kq[q] : 1));
+      Vv[q] = w.take((int)(kq[q] > 0 ? kq[q] : 1) * nq[q]);
       if (kq[q] == 0) continue;
-      View Bm = w.take((long long)mq[q] * nq[q]);
+      View Bm = w.take((int)mq[q] * nq[q]);
       if (L == 0) {
         int bi = 0;
         for (int i = 0; i < mr; ++i) {
@@ -910,7 +1066,7 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
           int bj = 0;
           for (int j = 0; j < mc; ++j) {
             if ((pd(j / nb) ^ pb(j % nb)) != q) continue;
-            Bm[(long long)bi * nq[q] + bj] = wt[(long long)i * mc + j];
+            Bm[(int)bi * nq[q] + bj] = wt[(int)i * mc + j];
             ++bj;
           }
           ++bi;
@@ -957,14 +1113,14 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
         for (int i = 0; i < mr; ++i) {
           if ((pa(i / d) ^ pd(i % d)) != q) continue;
           for (int k = 0; k < take_q[q]; ++k)
-            A[(long long)i * Dk + off + k] = Uv[q][(long long)bi * kq[q] + k] * sqrt(Sv[q][k]);
+            A[(int)i * Dk + off + k] = Uv[q][(int)bi * kq[q] + k] * sqrt(Sv[q][k]);
           ++bi;
         }
         for (int k = 0; k < take_q[q]; ++k) {
           int bj = 0;
           for (int j = 0; j < mc; ++j) {
             if ((pd(j / nb) ^ pb(j % nb)) != q) continue;
-            B[(long long)(off + k) * mc + j] = sqrt(Sv[q][k]) * Vv[q][(long long)k * nq[q] + bj];
+            B[(int)(off + k) * mc + j] = sqrt(Sv[q][k]) * Vv[q][(int)k * nq[q] + bj];
             ++bj;
           }
         }
@@ -979,7 +1135,7 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
   // ---- ALS (full_update.jl:418-482) ----
   {
   const int D = gk.D;
-  const long long ca = (long long)na * d * D, cb = (long long)D * d * nb;
+  const int ca = (int)na * d * D, cb = (int)D * d * nb;
   auto pD = [&](int i) { return par(i, gk.eD); };
   View Nt = w.take(tn);
   apply_metric(t, N4, target, Nt, g);
@@ -1002,8 +1158,8 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
     fallback = relv < 0.0;
     return true;
   };
-  auto keep = [&](View dst, View src, long long cnt) {
-    for (long long i = L; i < cnt; i += NT) dst[i] = src[i];
+  auto keep = [&](View dst, View src, int cnt) {
+    for (int i = L; i < cnt; i += NT) dst[i] = src[i];
   };
   double best = 0.0;
   bool fb = false;
@@ -1011,7 +1167,7 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
   res.initial_cost = best;
   keep(bestA, oA, ca);
   keep(bestB, oB, cb);
-  View fQA = w.take(ca), fC = w.take((long long)D * D), fQB = w.take(cb);
+  View fQA = w.take(ca), fC = w.take((int)D * D), fQB = w.take(cb);
   bool have = false;
   if (fb) {
     res.stop = kResidualFallback;
@@ -1021,133 +1177,118 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
     View wtw = w.take(tn);
     apply_metric(t, metric, wt, wtw, g);
     const int nl = na * D, nr = D * nb;
-    View NL = w.take((long long)nl * nl), yl = w.take((long long)nl * d), xl = w.take((long long)nl * d);
-    View NR = w.take((long long)nr * nr), yr = w.take((long long)nr * d), xr = w.take((long long)nr * d);
-    View BB = w.take((long long)nr * nr), AA = w.take((long long)nl * nl);
-    View QA = w.take(ca), Rm = w.take((long long)D * D), rinit = w.take(cb), C = w.take((long long)D * D);
+    View NL = w.take((int)nl * nl), yl = w.take((int)nl * d), xl = w.take((int)nl * d);
+    View NR = w.take((int)nr * nr), yr = w.take((int)nr * d), xr = w.take((int)nr * d);
+    View BB = w.take((int)nr * nr), AA = w.take((int)nl * nl);
+    // BB and AA are never live together: one LDS region holds either
+    const bool stage = (long long)nr * nr <= t.lds_cap && (long long)nl * nl <= t.lds_cap;
+    const View BBs = stage ? View{t.lds, 1} : BB, AAs = stage ? View{t.lds, 1} : AA;
+    View QA = w.take(ca), Rm = w.take((int)D * D), rinit = w.take(cb), C = w.take((int)D * D);
     View QB = w.take(cb), solm = w.take(ca), solt = w.take(cb), Qt = w.take(cb);
     const int kcD[2] = {gk.eD, D - gk.eD};
     for (int it = 1; it <= o.max_iterations; ++it) {
       // left normal equation (full_update.jl:402-408):
       // NL[(f,b),(g,o)] = sum_{s,n} metric[f,s,g,n] BB[(b,s),(o,n)], BB = sum_p B B
-      for (long long idx = L; idx < (long long)nr * nr; idx += NT) {
-        const long long bs = idx / nr, on = idx % nr;
+      for (int idx = L; idx < (int)nr * nr; idx += NT) {
+        const int bs = idx / nr, on = idx % nr;
         const int b = (int)(bs / nb), s = (int)(bs % nb), oo = (int)(on / nb), nn = (int)(on % nb);
         double bb = 0.0;
-        for (int p = 0; p < d; ++p) bb += B[((long long)b * d + p) * nb + s] * B[((long long)oo * d + p) * nb + nn];
-        BB[idx] = bb;
+        for (int p = 0; p < d; ++p) bb += B[((int)b * d + p) * nb + s] * B[((int)oo * d + p) * nb + nn];
+        BBs[idx] = bb;
       }
       t.sync();
-      for (long long idx = L; idx < (long long)nl * nl; idx += NT) {
-        const long long fb_ = idx / nl, go = idx % nl;
-        const int f = (int)(fb_ / D), b = (int)(fb_ % D), gg = (int)(go / D), oo = (int)(go % D);
-        // Z2 structure: the entry vanishes unless par(f) + par(b) = par(g) + par(o),
-        // and metric[f,s,g,n] unless par(n) = par(f) + par(s) + par(g)
-        double acc = 0.0;
-        if ((pa(f) ^ pD(b)) == (pa(gg) ^ pD(oo)))
-          for (int s = 0; s < nb; ++s) {
-            const int pn = pa(f) ^ pb(s) ^ pa(gg);
-            const int n0 = pn == 0 ? 0 : g.eb, n1 = pn == 0 ? g.eb : nb;
-            for (int nn = n0; nn < n1; ++nn)
-              acc += metric[(((long long)f * nb + s) * na + gg) * nb + nn] *
-                     BB[((long long)b * nb + s) * nr + (long long)oo * nb + nn];
-          }
-        NL[idx] = acc;
+      {
+        const NormalShape h{na, g.ea, nb, g.eb,
+                            (int)nb * na * nb, nb, (int)na * nb, 1,
+                            nr, 1, (int)nb * nr, nb,
+                            (int)D * nl, D, nl, 1};
+        normal_matrix(t, metric, BBs, NL, h, D, gk.eD);
       }
-      for (long long idx = L; idx < (long long)nl * d; idx += NT) {
+      for (int idx = L; idx < (int)nl * d; idx += NT) {
         const int p = (int)(idx % d);
-        const long long fb_ = idx / d;
+        const int fb_ = idx / d;
         const int f = (int)(fb_ / D), b = (int)(fb_ % D);
         double acc = 0.0;
         for (int q = 0; q < d; ++q)
           for (int s = 0; s < nb; ++s)
-            acc += wtw[(((long long)f * d + p) * d + q) * nb + s] * B[((long long)b * d + q) * nb + s];
+            acc += wtw[(((int)f * d + p) * d + q) * nb + s] * B[((int)b * d + q) * nb + s];
         yl[idx] = acc;
-        xl[idx] = A[((long long)f * d + p) * D + b];
+        xl[idx] = A[((int)f * d + p) * D + b];
       }
       t.sync();
       const double ra = solve_core(t, NL, nl, d, yl, xl, o, w);
       // QA, R = left_orth(sol as ((r, p) | k))
-      for (long long idx = L; idx < ca; idx += NT) {
+      for (int idx = L; idx < ca; idx += NT) {
         const int k = (int)(idx % D);
-        const long long rp = idx / D;
+        const int rp = idx / D;
         const int r = (int)(rp / d), p = (int)(rp % d);
-        solm[idx] = xl[((long long)r * D + k) * d + p];
+        solm[idx] = xl[((int)r * D + k) * d + p];
       }
       t.sync();
       block_qr(t, solm, na * d, D, [&](int i) { return pa(i / d) ^ pd(i % d); }, pD, kcD, QA, Rm, w);
       // right_initial = R B
-      for (long long idx = L; idx < cb; idx += NT) {
-        const int k = (int)(idx / ((long long)d * nb));
-        const long long j = idx % ((long long)d * nb);
+      for (int idx = L; idx < cb; idx += NT) {
+        const int k = (int)(idx / ((int)d * nb));
+        const int j = idx % ((int)d * nb);
         double acc = 0.0;
-        for (int m = 0; m < D; ++m) acc += Rm[(long long)k * D + m] * B[(long long)m * d * nb + j];
+        for (int m = 0; m < D; ++m) acc += Rm[(int)k * D + m] * B[(int)m * d * nb + j];
         rinit[idx] = acc;
       }
       // right normal equation (full_update.jl:410-416) with A = QA:
       // NR[(b,s),(o,n)] = sum_{w,e} metric[w,s,e,n] AA[(b,w),(o,e)], AA = sum_p QA QA
-      for (long long idx = L; idx < (long long)nl * nl; idx += NT) {
-        const long long bw = idx / nl, oe = idx % nl;
+      for (int idx = L; idx < (int)nl * nl; idx += NT) {
+        const int bw = idx / nl, oe = idx % nl;
         const int b = (int)(bw / na), ww = (int)(bw % na), oo = (int)(oe / na), e = (int)(oe % na);
         double aa = 0.0;
-        for (int p = 0; p < d; ++p) aa += QA[((long long)ww * d + p) * D + b] * QA[((long long)e * d + p) * D + oo];
-        AA[idx] = aa;
+        for (int p = 0; p < d; ++p) aa += QA[((int)ww * d + p) * D + b] * QA[((int)e * d + p) * D + oo];
+        AAs[idx] = aa;
       }
       t.sync();
-      for (long long idx = L; idx < (long long)nr * nr; idx += NT) {
-        const long long bs = idx / nr, on = idx % nr;
-        const int b = (int)(bs / nb), s = (int)(bs % nb), oo = (int)(on / nb), nn = (int)(on % nb);
-        // vanishes unless par(b) + par(s) = par(o) + par(n); metric[w,s,e,n]
-        // unless par(e) = par(w) + par(s) + par(n)
-        double acc = 0.0;
-        if ((pD(b) ^ pb(s)) == (pD(oo) ^ pb(nn)))
-          for (int ww = 0; ww < na; ++ww) {
-            const int pe = pa(ww) ^ pb(s) ^ pb(nn);
-            const int e0 = pe == 0 ? 0 : g.ea, e1 = pe == 0 ? g.ea : na;
-            for (int e = e0; e < e1; ++e)
-              acc += metric[(((long long)ww * nb + s) * na + e) * nb + nn] *
-                     AA[((long long)b * na + ww) * nl + (long long)oo * na + e];
-          }
-        NR[idx] = acc;
+      {
+        const NormalShape h{nb, g.eb, na, g.ea,
+                            (int)na * nb, 1, (int)nb * na * nb, nb,
+                            nl, 1, (int)na * nl, na,
+                            nr, 1, (int)nb * nr, nb};
+        normal_matrix(t, metric, AAs, NR, h, D, gk.eD);
       }
-      for (long long idx = L; idx < (long long)nr * d; idx += NT) {
+      for (int idx = L; idx < (int)nr * d; idx += NT) {
         const int p = (int)(idx % d);
-        const long long bs = idx / d;
+        const int bs = idx / d;
         const int b = (int)(bs / nb), s = (int)(bs % nb);
         double acc = 0.0;
         for (int ww = 0; ww < na; ++ww)
           for (int q = 0; q < d; ++q)
-            acc += wtw[(((long long)ww * d + q) * d + p) * nb + s] * QA[((long long)ww * d + q) * D + b];
+            acc += wtw[(((int)ww * d + q) * d + p) * nb + s] * QA[((int)ww * d + q) * D + b];
         yr[idx] = acc;
-        xr[idx] = rinit[((long long)b * d + p) * nb + s];
+        xr[idx] = rinit[((int)b * d + p) * nb + s];
       }
       t.sync();
       const double rb = solve_core(t, NR, nr, d, yr, xr, o, w);
       // C, QB = right_orth(sol as (k | (q, s))) via QR of its transpose
-      for (long long idx = L; idx < cb; idx += NT) {
+      for (int idx = L; idx < cb; idx += NT) {
         const int k = (int)(idx % D);
-        const long long qs = idx / D;
+        const int qs = idx / D;
         const int q = (int)(qs / nb), s = (int)(qs % nb);
-        solt[idx] = xr[((long long)k * nb + s) * d + q];
+        solt[idx] = xr[((int)k * nb + s) * d + q];
       }
       t.sync();
       block_qr(t, solt, d * nb, D, [&](int i) { return pd(i / nb) ^ pb(i % nb); }, pD, kcD, Qt, Rm, w);
-      for (long long idx = L; idx < (long long)D * D; idx += NT) {
+      for (int idx = L; idx < (int)D * D; idx += NT) {
         const int k = (int)(idx / D), m = (int)(idx % D);
-        C[idx] = Rm[(long long)m * D + k];
+        C[idx] = Rm[(int)m * D + k];
       }
-      for (long long idx = L; idx < cb; idx += NT) {
-        const int m = (int)(idx / ((long long)d * nb));
-        const long long j = idx % ((long long)d * nb);
+      for (int idx = L; idx < cb; idx += NT) {
+        const int m = (int)(idx / ((int)d * nb));
+        const int j = idx % ((int)d * nb);
         QB[idx] = Qt[j * D + m];
       }
       t.sync();
       // A = QA C, B = QB
-      for (long long idx = L; idx < ca; idx += NT) {
-        const long long rp = idx / D;
+      for (int idx = L; idx < ca; idx += NT) {
+        const int rp = idx / D;
         const int m = (int)(idx % D);
         double acc = 0.0;
-        for (int k = 0; k < D; ++k) acc += QA[rp * D + k] * C[(long long)k * D + m];
+        for (int k = 0; k < D; ++k) acc += QA[rp * D + k] * C[(int)k * D + m];
         A[idx] = acc;
       }
       keep(B, QB, cb);
@@ -1167,7 +1308,7 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
         keep(bestA, oA, ca);
         keep(bestB, oB, cb);
         keep(fQA, QA, ca);
-        keep(fC, C, (long long)D * D);
+        keep(fC, C, (int)D * D);
         keep(fQB, QB, cb);
         have = true;
       }
@@ -1183,35 +1324,35 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
   if (have) {
     // _fu_balance_bond (full_update.jl:44-52): compact SVD of C per parity block
     const long long mark = w.top;
-    View Us = w.take((long long)D * D), Vs = w.take((long long)D * D);
-    for (long long i = L; i < (long long)D * D; i += NT) Us[i] = Vs[i] = 0.0;
+    View Us = w.take((int)D * D), Vs = w.take((int)D * D);
+    for (int i = L; i < (int)D * D; i += NT) Us[i] = Vs[i] = 0.0;
     for (int q = 0; q < 2; ++q) {
       const int lo = q == 0 ? 0 : gk.eD, m = q == 0 ? gk.eD : D - gk.eD;
       if (m == 0) continue;
-      View Cm = w.take((long long)m * m), U = w.take((long long)m * m), S = w.take(m), Vt = w.take((long long)m * m);
-      for (long long idx = L; idx < (long long)m * m; idx += NT)
-        Cm[idx] = fC[(long long)(lo + idx / m) * D + lo + idx % m];
+      View Cm = w.take((int)m * m), U = w.take((int)m * m), S = w.take(m), Vt = w.take((int)m * m);
+      for (int idx = L; idx < (int)m * m; idx += NT)
+        Cm[idx] = fC[(int)(lo + idx / m) * D + lo + idx % m];
       t.sync();
       svd_compact(t, Cm, m, m, U, S, Vt, w);
-      for (long long idx = L; idx < (long long)m * m; idx += NT) {
+      for (int idx = L; idx < (int)m * m; idx += NT) {
         const int i = (int)(idx / m), k = (int)(idx % m);
-        Us[(long long)(lo + i) * D + lo + k] = U[(long long)i * m + k] * sqrt(S[k]);
-        Vs[(long long)(lo + k) * D + lo + i] = sqrt(S[k]) * Vt[(long long)k * m + i];
+        Us[(int)(lo + i) * D + lo + k] = U[(int)i * m + k] * sqrt(S[k]);
+        Vs[(int)(lo + k) * D + lo + i] = sqrt(S[k]) * Vt[(int)k * m + i];
       }
     }
     t.sync();
-    for (long long idx = L; idx < ca; idx += NT) {
-      const long long rp = idx / D;
+    for (int idx = L; idx < ca; idx += NT) {
+      const int rp = idx / D;
       const int m = (int)(idx % D);
       double acc = 0.0;
-      for (int k = 0; k < D; ++k) acc += fQA[rp * D + k] * Us[(long long)k * D + m];
+      for (int k = 0; k < D; ++k) acc += fQA[rp * D + k] * Us[(int)k * D + m];
       A[idx] = acc;
     }
-    for (long long idx = L; idx < cb; idx += NT) {
-      const int m = (int)(idx / ((long long)d * nb));
-      const long long j = idx % ((long long)d * nb);
+    for (int idx = L; idx < cb; idx += NT) {
+      const int m = (int)(idx / ((int)d * nb));
+      const int j = idx % ((int)d * nb);
       double acc = 0.0;
-      for (int k = 0; k < D; ++k) acc += Vs[(long long)m * D + k] * fQB[(long long)k * d * nb + j];
+      for (int k = 0; k < D; ++k) acc += Vs[(int)m * D + k] * fQB[(int)k * d * nb + j];
       B[idx] = acc;
     }
     t.sync();
@@ -1219,19 +1360,19 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
     w.top = mark;
   }
   // expand the kept bond into the padded capacity
-  for (long long i = L; i < (long long)na * d * g.D; i += NT) Aout[i] = 0.0;
-  for (long long i = L; i < (long long)g.D * d * nb; i += NT) Bout[i] = 0.0;
+  for (int i = L; i < (int)na * d * g.D; i += NT) Aout[i] = 0.0;
+  for (int i = L; i < (int)g.D * d * nb; i += NT) Bout[i] = 0.0;
   t.sync();
   const int Dk = ke + ko;
-  for (long long idx = L; idx < (long long)na * d * Dk; idx += NT) {
-    const long long rp = idx / Dk;
+  for (int idx = L; idx < (int)na * d * Dk; idx += NT) {
+    const int rp = idx / Dk;
     const int k = (int)(idx % Dk);
     Aout[rp * g.D + (k < ke ? k : g.eD + k - ke)] = bestA[idx];
   }
-  for (long long idx = L; idx < (long long)Dk * d * nb; idx += NT) {
-    const int k = (int)(idx / ((long long)d * nb));
-    const long long j = idx % ((long long)d * nb);
-    Bout[(long long)(k < ke ? k : g.eD + k - ke) * d * nb + j] = bestB[idx];
+  for (int idx = L; idx < (int)Dk * d * nb; idx += NT) {
+    const int k = (int)(idx / ((int)d * nb));
+    const int j = idx % ((int)d * nb);
+    Bout[(int)(k < ke ? k : g.eD + k - ke) * d * nb + j] = bestB[idx];
   }
   t.sync();
   }
