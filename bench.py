@@ -127,6 +127,37 @@ def adjoint_timing(spp, reps=2, batch=8, chi=36):
             "batch": batch, "batch_s_per_state": tb / batch}
 
 
+def fu_timing(spp, batch=16, chi=32, J=1.0, h=2.0, delta=0.05):
+    """The full-update evolution workload: one second-order step (sweeps H->,
+    V->, V<-, H<-; 224 bond updates) of a batch of 8x8 Z2 PEPS, D=6 (3+3),
+    d=2, i.i.d. N(0,1) blocks scaled to unit norm (seeds 100, 101, ...), TFIM
+    gates, boundary chi=32, FullUpdate with 40 ALS and 100 PCG iterations
+    (the options of the reference's example configurations), FP64. One timed
+    step: a step is tens of seconds and allocates only its scratch."""
+    lat = spp.Lattice(8, 8)
+    ctx = spp.Context(0, "fp64")
+    dev = [spp.DeviceState(ctx, spp.random_peps(lat, (3, 3), seed=100 + k)) for k in range(batch)]
+    model = spp.TFIM(J, h, symmetric=True)
+    method = spp.FullUpdate(max_iterations=40, error_tolerance=1e-8, diagonal_shift=1e-10,
+                            linear_max_iterations=100, linear_tolerance=1e-9)
+    ctx.synchronize()
+    ctx.reset_counters()
+    t0 = time.perf_counter()
+    r = spp.evolution_steps(ctx, dev, model, delta, chi, method)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    cnt = ctx.counters()
+    rec = r["records"]
+    return {"value": batch / dt, "unit": "FU sample-steps/s", "dtype": "fp64",
+            "workload": f"full-update step (2nd order, delta={delta}), 8x8 Z2 PEPS D=6 (3+3), d=2, chi={chi}, "
+                        f"TFIM J={J} h={h}, 40 ALS x 100 PCG iterations, {batch} states (seeds 100..)",
+            "batch": batch, "s_per_step": dt, "s_per_sample_step": dt / batch,
+            "bond_updates_per_step": int(rec.shape[1]), "mean_als_iterations": float(rec[:, :, 3].mean()),
+            "max_final_cost": float(rec[:, :, 1].max()),
+            "max_environment_truncation": float(np.max(r["max_environment_truncation"])),
+            "contraction_tflops_over_step": cnt["gemm_flops"] / dt / 1e12}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +177,8 @@ def main():
     ap.add_argument("--profile-only", action="store_true", help="timed loop only (for rocprofv3)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="time only the log-norm gradient workload (for rocprofv3)")
+    ap.add_argument("--fu-only", action="store_true", help="time only the full-update workload")
+    ap.add_argument("--fu-batch", type=int, default=16, help="states per full-update step (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed loop, copy 256 MiB device to device (HBM counter calibration)")
     args = ap.parse_args()
@@ -158,6 +191,9 @@ def main():
     spp = ge.load_package()
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_timing(spp)}), flush=True)
+        return
+    if args.fu_only:
+        print(json.dumps({"full_update": fu_timing(spp, batch=max(args.fu_batch, 1))}), flush=True)
         return
     L = args.L
     lat = spp.Lattice(L, L)
@@ -266,6 +302,9 @@ def main():
     result["energy_measurement"] = {"value": k / t_energy, "unit": "energy measurements/s",
                                     "dtype": args.precision, "samples": int(k), "chi": args.chi}
     result["adjoint"] = adjoint_timing(spp)
+    if args.fu_batch > 0:
+        log("full-update step ...")
+        result["full_update"] = fu_timing(spp, batch=args.fu_batch, J=args.J, h=args.h)
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_baseline(L=L, chi=args.chi)
