@@ -166,6 +166,19 @@ int spp_measure_energy_batch(spp_ctx* ctx, const spp_cache* cache, const spp_sta
                              double J, double h, double* energy_total, double* energy_per_site, double* log_norm,
                              double* tensor_log_norm, double* max_trunc, int* status, int* fail_site);
 
+/* Batched `measure_energy(state, model, BoundaryMPS(chi))`
+ * (src/measurement/energy.jl:31-78) of n uploaded states (PEPS or PEPO, all on
+ * the same lattice with the same leg spaces; the double layers trace the
+ * auxiliary leg), e.g. the particles of `_observe` after `_evolve!`
+ * (src/driver/particles.jl:35-40, 86-95). Contractions in the context's
+ * precision. Operators, outputs and status codes as in
+ * spp_measure_energy_batch; log_norm adds each state's 2 log_amplitude and
+ * tensor_log_norm is that of the stored tensors. */
+int spp_measure_energy_states(spp_ctx* ctx, int n, const spp_state* const* states, int chi, const double* X,
+                              const double* F, const double* G, int factor_odd, double J, double h,
+                              double* energy_total, double* energy_per_site, double* log_norm,
+                              double* tensor_log_norm, double* max_trunc, int* status, int* fail_site);
+
 /* Batched `lognorm(state, BoundaryMPS(chi))` (src/contraction/boundary_mps.jl:162-191)
  * of n uploaded states (PEPS or PEPO, all on the same lattice with the same
  * leg spaces), e.g. the particles of `_observe` with energy = false

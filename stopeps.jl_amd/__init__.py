@@ -11,7 +11,8 @@ from .states import (Lattice, Site, FiniteState, FinitePEPO, FinitePEPS, identit
                      LocalBasis, SymmetryBasis, XBasis, resolve_basis)
 from .sampling import (Context, DeviceState, SamplingCache, BornError, sample_configuration,
                        sample_configurations, sample_uniforms, project_configuration, splitmix64)
-from .measurement import TFIM, spin_operators, operator_factors, EnergyError, measure_energies, measure_energy
+from .measurement import (TFIM, spin_operators, operator_factors, EnergyError, measure_energies, measure_energy,
+                          measure_energy_states)
 from .contraction import NormError, lognorm, lognorms, lognorm_gradient, lognorm_gradients
 from .evolution import FullUpdate, trotter_gate, evolution_step, evolution_steps
 from . import _lib
@@ -20,6 +21,6 @@ __all__ = ["synthetic_thermal_pepo", "random_peps", "Lattice", "Site", "FiniteSt
            "LocalBasis", "SymmetryBasis", "XBasis", "resolve_basis", "Context", "DeviceState",
            "SamplingCache", "BornError", "sample_configuration", "sample_configurations", "sample_uniforms",
            "project_configuration", "splitmix64", "TFIM", "spin_operators", "operator_factors", "EnergyError",
-           "measure_energies", "measure_energy", "NormError", "lognorm", "lognorms",
+           "measure_energies", "measure_energy", "measure_energy_states", "NormError", "lognorm", "lognorms",
            "lognorm_gradient", "lognorm_gradients", "pad_bonds", "FullUpdate", "trotter_gate",
            "evolution_step", "evolution_steps"]

@@ -35,6 +35,10 @@ _SIGNATURES = {
                                                 c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_double,
                                                 ctypes.c_double, c_double_p, c_double_p, c_double_p, c_double_p,
                                                 c_double_p, c_int_p, c_int_p]),
+    "spp_measure_energy_states": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.POINTER(c_void_p), ctypes.c_int,
+                                                 c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_double,
+                                                 ctypes.c_double, c_double_p, c_double_p, c_double_p, c_double_p,
+                                                 c_double_p, c_int_p, c_int_p]),
     "spp_lognorm_batch": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.POINTER(c_void_p), ctypes.c_int,
                                          c_double_p, c_double_p, c_double_p, c_int_p]),
     "spp_lognorm_grad_batch": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.POINTER(c_void_p), ctypes.c_int,

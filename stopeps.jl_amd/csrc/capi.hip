@@ -347,6 +347,41 @@ int spp_measure_energy_batch(spp_ctx* ctx, const spp_cache* cache, const spp_sta
   });
 }
 
+int spp_measure_energy_states(spp_ctx* ctx, int n, const spp_state* const* states, int chi, const double* X,
+                              const double* F, const double* G, int factor_odd, double J, double h,
+                              double* energy_total, double* energy_per_site, double* log_norm,
+                              double* tensor_log_norm, double* max_trunc, int* status, int* fail_site) {
+  return guarded(ctx, [&] {
+    if (n < 0) throw HipError("negative state count");
+    if (n > 0 && !states) throw HipError("null state list");
+    if (!X || !F || !G) throw HipError("null operator");
+    if (factor_odd != 0 && factor_odd != 1) throw HipError("factor_odd must be 0 or 1");
+    SPP_CHECK(hipSetDevice(ctx->c.device));
+    std::vector<const StateDev*> v(n);
+    for (int i = 0; i < n; ++i) {
+      if (!states[i]) throw HipError("null state");
+      v[i] = &states[i]->s;
+    }
+    EnergyOps ops;
+    std::memcpy(ops.X, X, sizeof(ops.X));
+    std::memcpy(ops.F, F, sizeof(ops.F));
+    std::memcpy(ops.G, G, sizeof(ops.G));
+    ops.odd = factor_odd;
+    ops.J = J;
+    ops.h = h;
+    EnergyOutputs o;
+    o.energy_total = energy_total;
+    o.energy_per_site = energy_per_site;
+    o.log_norm = log_norm;
+    o.tensor_log_norm = tensor_log_norm;
+    o.max_trunc = max_trunc;
+    o.status = status;
+    o.fail_site = fail_site;
+    if (ctx->c.precision == 1) measure_energy_states<float>(ctx->c, v, chi, ops, o);
+    else measure_energy_states<double>(ctx->c, v, chi, ops, o);
+  });
+}
+
 int spp_lognorm_batch(spp_ctx* ctx, int n, const spp_state* const* states, int chi, double* log_norm,
                       double* tensor_log_norm, double* max_trunc, int* status) {
   return guarded(ctx, [&] {

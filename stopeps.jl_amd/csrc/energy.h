@@ -38,4 +38,12 @@ template <typename T>
 void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>& cache, int chi, int n,
                           const int* labels, const EnergyOps& ops, EnergyOutputs& out);
 
+// Energies of uploaded states (PEPS or PEPO, all on one lattice with the same
+// leg spaces; double layers trace the auxiliary leg), e.g. particles after
+// full-update evolution. log_norm adds each state's 2 log_amplitude;
+// tensor_log_norm is that of the stored tensors.
+template <typename T>
+void measure_energy_states(Context& ctx, const std::vector<const StateDev*>& states, int chi, const EnergyOps& ops,
+                           EnergyOutputs& out);
+
 }  // namespace spp

@@ -84,10 +84,60 @@ bool op_respects_grading(const double* op, int even, int odd_op) {
   return true;
 }
 
+// One d = 2 physical space on every site, operators that respect its grading.
+void check_energy_ops(const StateDev& st, const EnergyOps& ops) {
+  const int pe = st.sites[0].even[2];
+  for (const SiteDev& sd : st.sites)
+    if (sd.dim[2] != 2 || sd.even[2] != pe) throw HipError("energy needs one d = 2 physical space on every site");
+  if (pe == 2 && ops.odd) throw HipError("an odd factor leg needs a Z2-graded physical space");
+  if (!op_respects_grading(ops.X, pe, 0) || !op_respects_grading(ops.F, pe, ops.odd) ||
+      !op_respects_grading(ops.G, pe, ops.odd))
+    throw HipError("operator does not respect the Z2 grading of the physical space");
+}
+
+// Layer specs of every kind at one site (plain spec p; the factor leg of the
+// two-site halves fused into E, W, S or N).
+void impurity_specs(const SiteDev& sd, const LayerSpec& p, int odd, LayerSpec out[kKinds]) {
+  auto flip = [&](int k) { return odd ? flipped_pair_leg(sd.dim[k], sd.even[k]) : pair_leg(sd.dim[k], sd.even[k]); };
+  out[kPlain] = p;
+  out[kOnsite] = p;
+  out[kHorzC] = LayerSpec{p.w, p.n, flip(4), p.s};
+  out[kHorzD] = LayerSpec{flip(0), p.n, p.e, p.s};
+  out[kVertC] = LayerSpec{p.w, p.n, p.e, flip(5)};
+  out[kVertD] = LayerSpec{p.w, flip(1), p.e, p.s};
+}
+
+// Whether a kind's layer is read at site (x, y).
+bool kind_used(int kind, int x, int y, int Lx, int Ly) {
+  return kind == kPlain || kind == kOnsite || (kind == kHorzC && x < Lx - 1) || (kind == kHorzD && x > 0) ||
+         (kind == kVertC && y < Ly - 1) || (kind == kVertD && y > 0);
+}
+
+// Impurity layer job: the site with the kind's operator on the ket's physical
+// leg (and its factor leg fused into a virtual leg); aux traced (use_aux 0) or
+// projected on a basis vector.
+LayerJob impurity_job(const SiteDev& sd, int kind, const EnergyOps& ops, void* out) {
+  LayerJob J{};
+  J.ket = sd.d;
+  J.bra = sd.d;
+  for (int l = 0; l < 6; ++l) {
+    J.dk[l] = J.db[l] = sd.dim[l];
+    J.ek[l] = J.eb[l] = sd.even[l];
+  }
+  J.out = out;
+  if (kind != kPlain) {
+    J.use_op = 1;
+    const double* op = kind == kOnsite ? ops.X : (kind == kHorzC || kind == kVertC) ? ops.F : ops.G;
+    std::memcpy(J.op, op, sizeof(double) * 4);
+    const int fl = kind == kHorzC ? 4 : kind == kHorzD ? 0 : kind == kVertC ? 5 : kind == kVertD ? 1 : -1;
+    if (fl >= 0) J.flipk[fl] = ops.odd;
+  }
+  return J;
+}
+
 template <typename T>
 struct EnergyRun {
   Context& ctx;
-  const CacheDev<T>& c;
   int Lx, Ly, ns, chi, nb = 0, cur = 0;
   long long D = 1;                          // largest double-layer leg
   std::vector<LayerSpec> spec[kKinds];      // [kind][site]
@@ -96,8 +146,7 @@ struct EnergyRun {
   double *xsum = nullptr, *zzsum = nullptr, *den = nullptr, *num = nullptr, *numh = nullptr;
   double *fac = nullptr, *facp = nullptr, *nrm = nullptr;
 
-  EnergyRun(Context& cx, const CacheDev<T>& cc, int chi_)
-      : ctx(cx), c(cc), Lx(cc.Lx), Ly(cc.Ly), ns(cc.Lx * cc.Ly), chi(chi_) {}
+  EnergyRun(Context& cx, int lx, int ly, int chi_) : ctx(cx), Lx(lx), Ly(ly), ns(lx * ly), chi(chi_) {}
 
   void* const* table(int kind, int site) const { return tables + ((long long)kind * ns + site) * nb; }
   TRef layer(int kind, int site, const std::string& letters) const {
@@ -299,15 +348,9 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
   if (!labels_h) throw HipError("labels are required");
   for (long long i = 0; i < (long long)n * ns; ++i)
     if (labels_h[i] != 0 && labels_h[i] != 1) throw HipError("labels must be 0 or 1");
-  const int pe = pepo.sites[0].even[2];
-  for (const SiteDev& sd : pepo.sites)
-    if (sd.dim[2] != 2 || sd.even[2] != pe) throw HipError("energy needs one d = 2 physical space on every site");
-  if (pe == 2 && ops.odd) throw HipError("an odd factor leg needs a Z2-graded physical space");
-  if (!op_respects_grading(ops.X, pe, 0) || !op_respects_grading(ops.F, pe, ops.odd) ||
-      !op_respects_grading(ops.G, pe, ops.odd))
-    throw HipError("operator does not respect the Z2 grading of the physical space");
+  check_energy_ops(pepo, ops);
 
-  EnergyRun<T> run(ctx, c, chi);
+  EnergyRun<T> run(ctx, Lx, Ly, chi);
   // ---------------- impurity layers ----------------
   std::vector<void*> owned;
   std::vector<void*> base((size_t)kKinds * ns * 2, nullptr);
@@ -317,18 +360,12 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
   for (int i = 0; i < ns; ++i) {
     const SiteDev& sd = pepo.sites[i];
     const int x = i % Lx, y = i / Lx;
-    const LayerSpec& p = c.spec[i];
-    auto flip = [&](int k) { return ops.odd ? flipped_pair_leg(sd.dim[k], sd.even[k]) : pair_leg(sd.dim[k], sd.even[k]); };
-    run.spec[kPlain][i] = p;
-    run.spec[kOnsite][i] = p;
-    run.spec[kHorzC][i] = LayerSpec{p.w, p.n, flip(4), p.s};
-    run.spec[kHorzD][i] = LayerSpec{flip(0), p.n, p.e, p.s};
-    run.spec[kVertC][i] = LayerSpec{p.w, p.n, p.e, flip(5)};
-    run.spec[kVertD][i] = LayerSpec{p.w, flip(1), p.e, p.s};
+    LayerSpec ks[kKinds];
+    impurity_specs(sd, c.spec[i], ops.odd, ks);
+    for (int k = 0; k < kKinds; ++k) run.spec[k][i] = ks[k];
     for (int k = 0; k < 2; ++k) base[((size_t)kPlain * ns + i) * 2 + k] = c.cand[k][i];
     for (int kind = kOnsite; kind < kKinds; ++kind) {
-      const bool used = kind == kOnsite || (kind == kHorzC && x < Lx - 1) || (kind == kHorzD && x > 0) ||
-                        (kind == kVertC && y < Ly - 1) || (kind == kVertD && y > 0);
+      const bool used = kind_used(kind, x, y, Lx, Ly);
       for (int k = 0; k < 2; ++k) {
         void*& slot = base[((size_t)kind * ns + i) * 2 + k];
         if (!used) {
@@ -342,22 +379,10 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
         SPP_CHECK(hipMalloc(&buf, sizeof(T) * cnt));
         owned.push_back(buf);
         slot = buf;
-        LayerJob J{};
-        J.ket = sd.d;
-        J.bra = sd.d;
-        for (int l = 0; l < 6; ++l) {
-          J.dk[l] = J.db[l] = sd.dim[l];
-          J.ek[l] = J.eb[l] = sd.even[l];
-        }
-        J.out = buf;
+        LayerJob J = impurity_job(sd, kind, ops, buf);
         J.use_aux = 1;
         J.aux[0] = c.basis[0 * 2 + k];
         J.aux[1] = c.basis[1 * 2 + k];
-        J.use_op = 1;
-        const double* op = kind == kOnsite ? ops.X : (kind == kHorzC || kind == kVertC) ? ops.F : ops.G;
-        std::memcpy(J.op, op, sizeof(double) * 4);
-        const int fl = kind == kHorzC ? 4 : kind == kHorzD ? 0 : kind == kVertC ? 5 : kind == kVertD ? 1 : -1;
-        if (fl >= 0) J.flipk[fl] = ops.odd;
         jobs.push_back(J);
       }
     }
@@ -539,6 +564,270 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
   SPP_CHECK(hipStreamSynchronize(st));
   cleanup();
 }
+
+// Per chunk of states: the plain double layers of every site, the south
+// boundaries, then the north sweep; before row y the impurity layers of row y
+// (onsite, both horizontal halves, the upper vertical half) and the lower
+// vertical half of row y+1 are built into per-(kind, column) buffers, so the
+// impurity storage is two rows' worth whatever the lattice height.
+template <typename T>
+void measure_energy_states(Context& ctx, const std::vector<const StateDev*>& states, int chi, const EnergyOps& ops,
+                           EnergyOutputs& out) {
+  hipStream_t st = ctx.stream;
+  const int n = (int)states.size();
+  if (n == 0) return;
+  if (chi < 1) throw HipError("chi must be positive");
+  const StateDev& s0 = *states[0];
+  const int Lx = s0.Lx, Ly = s0.Ly, ns = Lx * Ly;
+  for (const StateDev* sd : states) {
+    if (!sd || sd->Lx != Lx || sd->Ly != Ly || (int)sd->sites.size() != ns)
+      throw HipError("energy batch: states must share the lattice");
+    for (int i = 0; i < ns; ++i)
+      if (!batchable_sites(sd->sites[i], s0.sites[i]))
+        throw HipError("energy batch: states must share every leg space (up to the auxiliary charge)");
+  }
+  check_energy_ops(s0, ops);
+
+  EnergyRun<T> run(ctx, Lx, Ly, chi);
+  for (int k = 0; k < kKinds; ++k) run.spec[k].resize(ns);
+  std::vector<long long> lsize(ns);
+  std::vector<long long> isize((size_t)kKinds * Lx, 0);  // per (kind, column): largest over rows
+  long long maxout = 1;
+  for (int i = 0; i < ns; ++i) {
+    const SiteDev& sd = s0.sites[i];
+    const int x = i % Lx, y = i / Lx;
+    const LayerSpec p{pair_leg(sd.dim[0], sd.even[0]), pair_leg(sd.dim[1], sd.even[1]),
+                      pair_leg(sd.dim[4], sd.even[4]), pair_leg(sd.dim[5], sd.even[5])};
+    LayerSpec ks[kKinds];
+    impurity_specs(sd, p, ops.odd, ks);
+    for (int k = 0; k < kKinds; ++k) {
+      run.spec[k][i] = ks[k];
+      const long long cnt = (long long)ks[k].w.dim * ks[k].n.dim * ks[k].e.dim * ks[k].s.dim;
+      if (!kind_used(k, x, y, Lx, Ly)) continue;
+      maxout = std::max(maxout, cnt);
+      if (k == kPlain) lsize[i] = cnt;
+      else isize[(size_t)k * Lx + x] = std::max(isize[(size_t)k * Lx + x], cnt);
+      run.D = std::max<long long>(run.D, std::max(std::max(ks[k].w.dim, ks[k].n.dim), std::max(ks[k].e.dim, ks[k].s.dim)));
+    }
+  }
+  long long layer_elems = 0, imp_elems = 0;
+  for (long long v : lsize) layer_elems += v;
+  for (long long v : isize) imp_elems += v;
+
+  std::vector<Boundary<T>> south(Ly + 1);
+  Boundary<T> NB[2];
+  std::vector<void*> scratch;
+  auto dev = [&](size_t bytes) {
+    void* p = nullptr;
+    SPP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+    scratch.push_back(p);
+    return p;
+  };
+  auto cleanup = [&]() {
+    for (auto& b : south) b.release();
+    NB[0].release();
+    NB[1].release();
+    for (void* p : scratch) (void)hipFree(p);
+    scratch.clear();
+  };
+  try {
+    // ---------------- chunk size from the memory budget ----------------
+    const long long D = run.D;
+    const double ts = sizeof(T);
+    const double small = (double)chi * D * chi, mid = small * D, big = mid * D;
+    const double row_b = ts * ((Lx + 5) * small + 2.0 * mid);
+    const double slab_b = ts * ((Lx + 4) * mid + 2.0 * big);
+    const double absorb_b = absorb_bytes_per_sample(Lx, chi, D, sizeof(T), ctx.ksub);
+    const size_t per_arena = (size_t)(1.15 * std::max(std::max(row_b, slab_b), absorb_b)) + 8192;
+    long long pmax = 1;
+    for (const LayerSpec& q : run.spec[kPlain]) pmax = std::max<long long>(pmax, std::max(q.n.dim, q.s.dim));
+    const long long capS = (long long)chi * pmax * chi;
+    const size_t per_persist = (size_t)(ts * (capS * Lx * (Ly + 3) + layer_elems + imp_elems)) +
+                               sizeof(int) * (Lx + 1) * (Ly + 3) + sizeof(void*) * kKinds * ns +
+                               sizeof(LayerJob) * (ns + kKinds * Lx) + 128;
+    SPP_CHECK(hipStreamSynchronize(st));
+    size_t freeb = 0, totalb = 0;
+    SPP_CHECK(hipMemGetInfo(&freeb, &totalb));
+    const double avail = 0.6 * (double)freeb + (double)ctx.arena.capacity();
+    long long chunk = std::max<long long>(1, (long long)(avail / (double)(per_arena + per_persist)));
+    if (ctx.chunk > 0) chunk = std::min<long long>(chunk, ctx.chunk);
+    chunk = std::min<long long>(chunk, n);
+    // double-layer grid: one row per job (a site, or a row's impurity layers, of each sample)
+    chunk = std::min<long long>(chunk, std::max(1, 65535 / std::max(ns, (int)kKinds * Lx)));
+    const int nb = (int)chunk;
+    ctx.arena.reserve(per_arena * (size_t)nb);
+    run.nb = nb;
+    for (int y = 0; y <= Ly; ++y) prepare_boundary(south[y], Lx, nb, capS);
+    prepare_boundary(NB[0], Lx, nb, capS);
+    prepare_boundary(NB[1], Lx, nb, capS);
+    T* layers = (T*)dev(sizeof(T) * (size_t)layer_elems * nb);
+    T* imp = (T*)dev(sizeof(T) * (size_t)std::max(imp_elems, 1LL) * nb);
+    std::vector<T*> site_base(ns), imp_base((size_t)kKinds * Lx, nullptr);
+    {
+      long long off = 0;
+      for (int i = 0; i < ns; ++i) {
+        site_base[i] = layers + off * nb;
+        off += lsize[i];
+      }
+      off = 0;
+      for (size_t k = 0; k < isize.size(); ++k) {
+        imp_base[k] = imp + off * nb;
+        off += isize[k];
+      }
+    }
+    run.tables = (void**)dev(sizeof(void*) * (size_t)kKinds * ns * nb);
+    // host mirror of the tables; unused (kind, site) entries point at the plain layer
+    std::vector<void*> htab((size_t)kKinds * ns * nb);
+    for (int k = 0; k < kKinds; ++k)
+      for (int i = 0; i < ns; ++i)
+        for (int s = 0; s < nb; ++s) htab[((size_t)k * ns + i) * nb + s] = site_base[i] + (long long)s * lsize[i];
+    LayerJob* djobs = (LayerJob*)dev(sizeof(LayerJob) * (size_t)std::max(ns, kKinds * Lx) * nb);
+    run.status = (int*)dev(sizeof(int) * nb);
+    run.fail = (int*)dev(sizeof(int) * nb);
+    run.xsum = (double*)dev(sizeof(double) * nb);
+    run.zzsum = (double*)dev(sizeof(double) * nb);
+    run.den = (double*)dev(sizeof(double) * nb);
+    run.num = (double*)dev(sizeof(double) * nb);
+    run.numh = (double*)dev(sizeof(double) * nb);
+    run.fac = (double*)dev(sizeof(double) * nb);
+    run.facp = (double*)dev(sizeof(double) * nb);
+    run.nrm = (double*)dev(sizeof(double) * nb);
+    double* mant = (double*)dev(sizeof(double) * nb);
+    long long* dsst = (long long*)dev(sizeof(long long) * Lx);
+    T** dtens = (T**)dev(sizeof(T*) * Lx);
+    std::vector<int> h_status(nb), h_fail(nb);
+    std::vector<double> h_x(nb), h_zz(nb), h_mant(nb), h_ls(nb), h_mtn(nb), h_mts(nb);
+    std::vector<LayerJob> jobs;
+    // build the given jobs (the job table and the pointer tables may still be
+    // read by queued kernels: wait before rewriting them)
+    auto run_jobs = [&]() {
+      SPP_CHECK(hipStreamSynchronize(st));
+      SPP_CHECK(hipMemcpyAsync(run.tables, htab.data(), sizeof(void*) * htab.size(), hipMemcpyHostToDevice, st));
+      if (!jobs.empty()) {
+        SPP_CHECK(hipMemcpyAsync(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), hipMemcpyHostToDevice, st));
+        SPP_CHECK(launch_double_layers<T>(djobs, (int)jobs.size(), maxout, st));
+      }
+    };
+    for (long long c0 = 0; c0 < n; c0 += nb) {
+      const int cur = (int)std::min<long long>(nb, n - c0);
+      run.cur = cur;
+      jobs.clear();
+      for (int i = 0; i < ns; ++i)
+        for (int s = 0; s < cur; ++s)
+          jobs.push_back(impurity_job(states[c0 + s]->sites[i], kPlain, ops, site_base[i] + (long long)s * lsize[i]));
+      run_jobs();
+      SPP_CHECK(launch_fill<int>(run.status, nb, 0, st));
+      SPP_CHECK(launch_fill<int>(run.fail, nb, -1, st));
+      SPP_CHECK(launch_fill<double>(run.xsum, nb, 0.0, st));
+      SPP_CHECK(launch_fill<double>(run.zzsum, nb, 0.0, st));
+      // south boundaries, rotated frame
+      for (auto& b : south) b.nb = cur;
+      const std::vector<LayerSpec>& P = run.spec[kPlain];
+      std::vector<Leg> physS(Lx);
+      for (int xr = 0; xr < Lx; ++xr) physS[xr] = P[(Lx - 1 - xr) + (Ly - 1) * Lx].s;
+      set_trivial(ctx, south[Ly], physS);
+      for (int y = Ly - 1; y >= 0; --y) {
+        RowLayers<T> rr;
+        rr.L = Lx;
+        rr.rotated = true;
+        rr.spec.resize(Lx);
+        rr.ptrs.resize(Lx);
+        for (int xr = 0; xr < Lx; ++xr) {
+          const int x = Lx - 1 - xr;
+          const LayerSpec& sp = P[x + y * Lx];
+          rr.spec[xr] = LayerSpec{sp.e, sp.s, sp.w, sp.n};
+          rr.ptrs[xr] = run.table(kPlain, x + y * Lx);
+        }
+        absorb_forward<T>(ctx, south[y + 1], rr, chi, south[y]);
+      }
+      // north sweep with the row and slab terms
+      NB[0].nb = cur;
+      NB[1].nb = cur;
+      std::vector<Leg> physN(Lx);
+      for (int x = 0; x < Lx; ++x) physN[x] = P[x].n;
+      set_trivial(ctx, NB[0], physN);
+      int which = 0;
+      for (int y = 0; y < Ly; ++y) {
+        jobs.clear();
+        auto add = [&](int kind, int x, int yy) {
+          const int i = x + yy * Lx;
+          if (yy >= Ly || !kind_used(kind, x, yy, Lx, Ly)) return;
+          const LayerSpec& q = run.spec[kind][i];
+          const long long cnt = (long long)q.w.dim * q.n.dim * q.e.dim * q.s.dim;
+          T* base = imp_base[(size_t)kind * Lx + x];
+          for (int s = 0; s < cur; ++s) {
+            T* o = base + (long long)s * cnt;
+            htab[((size_t)kind * ns + i) * nb + s] = o;
+            jobs.push_back(impurity_job(states[c0 + s]->sites[i], kind, ops, o));
+          }
+        };
+        for (int x = 0; x < Lx; ++x) {
+          add(kOnsite, x, y);
+          add(kHorzC, x, y);
+          add(kHorzD, x, y);
+          add(kVertC, x, y);
+          add(kVertD, x, y + 1);
+        }
+        run_jobs();
+        Boundary<T>& Nc = NB[which];
+        run.row_terms(Nc, south[y + 1], y);
+        if (y < Ly - 1) run.slab_terms(Nc, south[y + 2], y);
+        RowLayers<T> row;
+        row.L = Lx;
+        row.spec.assign(P.begin() + y * Lx, P.begin() + (y + 1) * Lx);
+        row.ptrs.resize(Lx);
+        for (int x = 0; x < Lx; ++x) row.ptrs[x] = run.table(kPlain, x + y * Lx);
+        absorb_forward<T>(ctx, Nc, row, chi, NB[1 - which]);
+        which = 1 - which;
+      }
+      // log norm (lognorm, boundary_mps.jl:180-191)
+      Boundary<T>& Nf = NB[which];
+      for (int x = 0; x <= Lx; ++x)
+        if (Nf.bdim[x] != 1) throw HipError("final boundary must have unit bonds");
+      std::vector<long long> sst(Lx);
+      for (int x = 0; x < Lx; ++x) sst[x] = Nf.tsize(x);
+      SPP_CHECK(hipMemcpyAsync(dsst, sst.data(), sizeof(long long) * Lx, hipMemcpyHostToDevice, st));
+      SPP_CHECK(hipMemcpyAsync(dtens, Nf.t.data(), sizeof(T*) * Lx, hipMemcpyHostToDevice, st));
+      SPP_CHECK(launch_close_unit<T>(dtens, dsst, Lx, mant, cur, st));
+      SPP_CHECK(hipMemcpyAsync(h_status.data(), run.status, sizeof(int) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_fail.data(), run.fail, sizeof(int) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_x.data(), run.xsum, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_zz.data(), run.zzsum, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_mant.data(), mant, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_ls.data(), Nf.log_scale, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_mtn.data(), Nf.max_trunc, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_mts.data(), south[0].max_trunc, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipStreamSynchronize(st));
+      for (int s = 0; s < cur; ++s) {
+        const long long g = c0 + s;
+        int stt = h_status[s];
+        const double m = h_mant[s];
+        if (stt == 0 && !(m > 0.0 && std::isfinite(m))) stt = 6;
+        const double tln = std::log(m) + h_ls[s];
+        const double e = stt == 0 ? -ops.J * h_zz[s] - ops.h * h_x[s] : NAN;
+        if (out.energy_total) out.energy_total[g] = e;
+        if (out.energy_per_site) out.energy_per_site[g] = e / ns;
+        if (out.tensor_log_norm) out.tensor_log_norm[g] = stt == 0 ? tln : NAN;
+        if (out.log_norm) out.log_norm[g] = stt == 0 ? tln + 2.0 * states[g]->log_amplitude : NAN;
+        if (out.max_trunc) out.max_trunc[g] = std::max(h_mtn[s], h_mts[s]);
+        if (out.status) out.status[g] = stt;
+        if (out.fail_site) out.fail_site[g] = stt == 0 ? -1 : h_fail[s];
+      }
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(st);
+    ctx.arena.release(0);
+    cleanup();
+    throw;
+  }
+  SPP_CHECK(hipStreamSynchronize(st));
+  cleanup();
+}
+
+template void measure_energy_states<float>(Context&, const std::vector<const StateDev*>&, int, const EnergyOps&,
+                                           EnergyOutputs&);
+template void measure_energy_states<double>(Context&, const std::vector<const StateDev*>&, int, const EnergyOps&,
+                                            EnergyOutputs&);
 
 template void measure_energy_batch<float>(Context&, const StateDev&, const CacheDev<float>&, int, int, const int*,
                                           const EnergyOps&, EnergyOutputs&);

@@ -64,6 +64,7 @@ struct FuSide {
 struct PairJob {
   const double* site[2];   // per sample: A, B
   const int* live[2];      // per sample: live counts [4 legs][2] of A and B
+  int aux_even[2];         // per sample: even count of A's and B's auxiliary leg
 };
 
 FU_HD inline int vidx(int leg) { return leg == 0 ? 0 : leg == 1 ? 1 : leg == 4 ? 2 : 3; }
@@ -114,7 +115,7 @@ FU_HD inline void fu_pair_sample(const fu::Team& t, const PairJob& J, int s, con
              live_slot(i2, F.even[l2], lv[2 * vidx(l2)], lv[2 * vidx(l2) + 1]);
     };
     auto row_par = [&](int i0, int i1, int i2, int ax) {
-      return fu::par(i0, F.even[l0]) ^ fu::par(i1, F.even[l1]) ^ fu::par(i2, F.even[l2]) ^ fu::par(ax, F.even[3]);
+      return fu::par(i0, F.even[l0]) ^ fu::par(i1, F.even[l1]) ^ fu::par(i2, F.even[l2]) ^ fu::par(ax, J.aux_even[side]);
     };
     auto col_live = [&](int k) {
       return live_slot(k, F.even[F.core], lv[2 * vidx(F.core)], lv[2 * vidx(F.core) + 1]);
@@ -419,7 +420,9 @@ long long count_legs(const std::vector<Leg>& l) {
   return n;
 }
 
-FuSide make_side(const SiteDev& sd, int rowv0, int rowv1, int rowv2, int core) {
+// any_aux: size the r capacity for either parity split of the auxiliary leg
+// (a batch whose samples carry different auxiliary charges).
+FuSide make_side(const SiteDev& sd, int rowv0, int rowv1, int rowv2, int core, bool any_aux = false) {
   FuSide F{};
   for (int k = 0; k < 6; ++k) {
     F.leg[k] = sd.dim[k];
@@ -430,19 +433,25 @@ FuSide make_side(const SiteDev& sd, int rowv0, int rowv1, int rowv2, int core) {
   F.rowv[2] = rowv2;
   F.core = core;
   // r capacity per parity: min(rows of parity q, columns of parity q)
-  long long rq[2] = {0, 0}, cq[2] = {0, 0};
-  for (int a = 0; a < sd.dim[rowv0]; ++a)
-    for (int b = 0; b < sd.dim[rowv1]; ++b)
-      for (int c = 0; c < sd.dim[rowv2]; ++c)
-        for (int x = 0; x < sd.dim[3]; ++x)
-          ++rq[fu::par(a, sd.even[rowv0]) ^ fu::par(b, sd.even[rowv1]) ^ fu::par(c, sd.even[rowv2]) ^
-               fu::par(x, sd.even[3])];
+  long long cq[2] = {0, 0};
   for (int p = 0; p < sd.dim[2]; ++p)
     for (int k = 0; k < sd.dim[core]; ++k) ++cq[fu::par(p, sd.even[2]) ^ fu::par(k, sd.even[core])];
-  const long long k0 = std::min(rq[0], cq[0]), k1 = std::min(rq[1], cq[1]);
+  long long k0 = 0, k1 = 0;
+  for (int flip = 0; flip < (any_aux ? 2 : 1); ++flip) {
+    const int ea = flip ? sd.dim[3] - sd.even[3] : sd.even[3];
+    long long rq[2] = {0, 0};
+    for (int a = 0; a < sd.dim[rowv0]; ++a)
+      for (int b = 0; b < sd.dim[rowv1]; ++b)
+        for (int c = 0; c < sd.dim[rowv2]; ++c)
+          for (int x = 0; x < sd.dim[3]; ++x)
+            ++rq[fu::par(a, sd.even[rowv0]) ^ fu::par(b, sd.even[rowv1]) ^ fu::par(c, sd.even[rowv2]) ^
+                 fu::par(x, ea)];
+    k0 = std::max(k0, std::min(rq[0], cq[0]));
+    k1 = std::max(k1, std::min(rq[1], cq[1]));
+  }
   F.recap = (int)k0;
   F.rcap = (int)(k0 + k1);
-  F.rows = rq[0] + rq[1];
+  F.rows = (long long)sd.dim[rowv0] * sd.dim[rowv1] * sd.dim[rowv2] * sd.dim[3];
   return F;
 }
 
@@ -475,6 +484,8 @@ int fu_pair_host(int horizontal, const int* dims_a, const int* evens_a, const in
   J.site[1] = site_b;
   J.live[0] = live_a;
   J.live[1] = live_b;
+  J.aux_even[0] = evens_a[3];
+  J.aux_even[1] = evens_b[3];
   int rl[4] = {0, 0, 0, 0};
   const fu::Team one{0, 1, nullptr, nullptr, 0};
   fu_pair_sample(one, J, 0, sa, sb, gate, d, ed, qa, 0, qb, 0, ca, cb, target, rl, ws.data(), work, 1);
@@ -503,9 +514,8 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
     if (!sd || sd->Lx != Lx || sd->Ly != Ly || (int)sd->sites.size() != ns)
       throw HipError("full update: states must share the lattice");
     for (int i = 0; i < ns; ++i)
-      for (int k = 0; k < 6; ++k)
-        if (sd->sites[i].dim[k] != s0.sites[i].dim[k] || sd->sites[i].even[k] != s0.sites[i].even[k])
-          throw HipError("full update: states must share every leg space");
+      if (!batchable_sites(sd->sites[i], s0.sites[i]))
+        throw HipError("full update: states must share every leg space (up to the auxiliary charge)");
     if ((int)sd->live.size() != ns * 8) throw HipError("full update: state without bond bookkeeping");
   }
   const int d = s0.sites[0].dim[2], ed = s0.sites[0].even[2];
@@ -553,14 +563,19 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
     g.ib = ib;
     const SiteDev& A = s0.sites[ia];
     const SiteDev& B = s0.sites[ib];
+    bool mixed_a = false, mixed_b = false;
+    for (const StateDev* sd : states) {
+      mixed_a = mixed_a || sd->sites[ia].even[3] != A.even[3];
+      mixed_b = mixed_b || sd->sites[ib].even[3] != B.even[3];
+    }
     if (horizontal) {
-      g.sa = make_side(A, 0, 1, 5, 4);
-      g.sb = make_side(B, 1, 4, 5, 0);
+      g.sa = make_side(A, 0, 1, 5, 4, mixed_a);
+      g.sb = make_side(B, 1, 4, 5, 0, mixed_b);
       g.Dcap = A.dim[4];
       g.eDcap = A.even[4];
     } else {
-      g.sa = make_side(A, 0, 1, 4, 5);
-      g.sb = make_side(B, 0, 4, 5, 1);
+      g.sa = make_side(A, 0, 1, 4, 5, mixed_a);
+      g.sb = make_side(B, 0, 4, 5, 1, mixed_b);
       g.Dcap = A.dim[5];
       g.eDcap = A.even[5];
     }
@@ -942,6 +957,8 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
               for (int s = 0; s < cur; ++s) {
                 pj[s].site[0] = states[c0 + s]->sites[ia].d;
                 pj[s].site[1] = states[c0 + s]->sites[ib].d;
+                pj[s].aux_even[0] = states[c0 + s]->sites[ia].even[3];
+                pj[s].aux_even[1] = states[c0 + s]->sites[ib].even[3];
                 rj[s].site[0] = states[c0 + s]->sites[ia].d;
                 rj[s].site[1] = states[c0 + s]->sites[ib].d;
               }

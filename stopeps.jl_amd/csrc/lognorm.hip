@@ -25,9 +25,8 @@ void lognorm_batch(Context& ctx, const std::vector<const StateDev*>& states, int
     if (!sd || sd->Lx != Lx || sd->Ly != Ly || (int)sd->sites.size() != ns)
       throw HipError("lognorm batch: states must share the lattice");
     for (int i = 0; i < ns; ++i)
-      for (int k = 0; k < 6; ++k)
-        if (sd->sites[i].dim[k] != s0.sites[i].dim[k] || sd->sites[i].even[k] != s0.sites[i].even[k])
-          throw HipError("lognorm batch: states must share every leg space");
+      if (!batchable_sites(sd->sites[i], s0.sites[i]))
+        throw HipError("lognorm batch: states must share every leg space (up to the auxiliary charge)");
   }
   std::vector<LayerSpec> spec(ns);
   std::vector<long long> lsize(ns);

@@ -15,6 +15,16 @@ struct SiteDev {
   }
 };
 
+// Sites that one batched call can hold side by side: equal dimensions on every
+// leg and equal parity splits on every leg but the auxiliary one. The double
+// layers trace the auxiliary leg, so its parity split (the Z2 charge of a
+// projected sample's label) may differ per sample.
+inline bool batchable_sites(const SiteDev& a, const SiteDev& b) {
+  for (int k = 0; k < 6; ++k)
+    if (a.dim[k] != b.dim[k] || (k != 3 && a.even[k] != b.even[k])) return false;
+  return true;
+}
+
 struct StateDev {
   int Lx = 0, Ly = 0, kind = 0;  // kind 0: PEPO, 1: PEPS
   std::vector<SiteDev> sites;    // x + y*Lx

@@ -79,6 +79,22 @@ def _raise_status(lattice, status, fail):
     raise EnergyError("norm requires a finite positive real value")
 
 
+def _device_operators(model, physical_parity):
+    """X and the rank-one split F (x) G of the model's ZZ in the device's
+    parity-sorted physical basis, and the factor leg's parity."""
+    ppar = np.asarray(physical_parity)
+    pe = int(np.sum(ppar == 0))
+    X, ZZ = spin_operators(model)
+    if model.symmetric != (pe < 2):
+        raise ValueError("model physical space does not match the state")
+    # the device holds the physical leg parity-sorted (even indices first)
+    perm = np.argsort(ppar, kind="stable")
+    X = X[np.ix_(perm, perm)]
+    ZZ = ZZ[np.ix_(perm, perm, perm, perm)]
+    F, G, odd = operator_factors(ZZ, pe)
+    return np.ascontiguousarray(X), np.ascontiguousarray(F), np.ascontiguousarray(G), odd
+
+
 def measure_energies(cache, configurations, model, chi=None, check=True):
     """Batched ``measure_energy(project_configuration(pepo, c), model,
     BoundaryMPS(chi))`` for configurations [n, Lx, Ly] of basis labels.
@@ -93,19 +109,7 @@ def measure_energies(cache, configurations, model, chi=None, check=True):
     n = cfg.shape[0]
     idx = np.vectorize(cache.basis.index)(cfg) if n else np.zeros((0, L.Lx, L.Ly), dtype=int)
     labels = np.ascontiguousarray(np.transpose(idx, (0, 2, 1)).reshape(n, ns), dtype=np.int32)
-    ppar = np.asarray(cache.state[(0, 0)].parity[2])
-    pe = int(np.sum(ppar == 0))
-    X, ZZ = spin_operators(model)
-    if model.symmetric != (pe < 2):
-        raise ValueError("model physical space does not match the state")
-    # the device holds the physical leg parity-sorted (even indices first)
-    perm = np.argsort(ppar, kind="stable")
-    X = X[np.ix_(perm, perm)]
-    ZZ = ZZ[np.ix_(perm, perm, perm, perm)]
-    F, G, odd = operator_factors(ZZ, pe)
-    X = np.ascontiguousarray(X)
-    F = np.ascontiguousarray(F)
-    G = np.ascontiguousarray(G)
+    X, F, G, odd = _device_operators(model, cache.state[(0, 0)].parity[2])
     out = {k: np.full(n, np.nan) for k in ("energy_total", "energy_per_site", "log_norm", "tensor_log_norm",
                                             "max_environment_truncation")}
     status = np.zeros(n, dtype=np.int32)
@@ -131,3 +135,35 @@ def measure_energy(cache, configuration, model, chi=None):
     return {"energy_total": float(r["energy_total"][0]), "energy_per_site": float(r["energy_per_site"][0]),
             "log_norm": float(r["log_norm"][0]), "tensor_log_norm": float(r["tensor_log_norm"][0]),
             "max_environment_truncation": float(r["max_environment_truncation"][0]), "Se": None}
+
+
+def measure_energy_states(ctx, states, model, chi, check=True):
+    """Batched ``measure_energy(state, model, BoundaryMPS(chi))``
+    (energy.jl:31-78) of resident states (``DeviceState``, or host states that
+    are uploaded for the call): PEPS or PEPO on one lattice with the same leg
+    spaces, e.g. the particles after ``_evolve!`` (particles.jl:86-95). Returns
+    the dict of ``measure_energies``; log_norm includes each state's
+    2 log_amplitude."""
+    from .sampling import DeviceState
+    import ctypes
+    devs = [s if isinstance(s, DeviceState) else DeviceState(ctx, s) for s in states]
+    n = len(devs)
+    out = {k: np.full(n, np.nan) for k in ("energy_total", "energy_per_site", "log_norm", "tensor_log_norm",
+                                            "max_environment_truncation")}
+    status = np.zeros(n, dtype=np.int32)
+    fail = np.full(n, -1, dtype=np.int32)
+    if n == 0:
+        out["status"], out["fail_site"] = status, fail
+        return out
+    X, F, G, odd = _device_operators(model, devs[0].state[(0, 0)].parity[2])
+    handles = (ctypes.c_void_p * n)(*[d.handle.value for d in devs])
+    _lib.check(ctx.handle, ctx._lib.spp_measure_energy_states(
+        ctx.handle, n, handles, int(chi), _lib.dptr(X), _lib.dptr(F), _lib.dptr(G), odd, model.J, model.h,
+        _lib.dptr(out["energy_total"]), _lib.dptr(out["energy_per_site"]), _lib.dptr(out["log_norm"]),
+        _lib.dptr(out["tensor_log_norm"]), _lib.dptr(out["max_environment_truncation"]), _lib.iptr(status),
+        _lib.iptr(fail)))
+    if check:
+        _raise_status(devs[0].state.lattice, status, fail)
+    out["status"] = status
+    out["fail_site"] = fail
+    return out

@@ -8,7 +8,8 @@ import numpy as np
 import pytest
 
 from conftest import to_product_state
-from oracle.model import Lattice, TFIM, State, random_state
+from oracle.model import Lattice, TFIM, State, embed_identity_pepo, random_state
+from oracle import born as ob
 from oracle.z2 import Z2
 from oracle import fu as ofu
 from test_oracle_fu import dense_vector
@@ -113,3 +114,17 @@ def test_fu_step_dense_states(spp, gpu_ctx):
     m = TFIM(J=1.0, h=0.8, symmetric=False)
     _compare(spp, gpu_ctx, [random_state(Lattice(2, 3), 2, "peps", seed=27, symmetric=False)], m, 0.1, 64, (2, 0),
              ofu.FullUpdate(max_iterations=20, linear_max_iterations=60))
+
+
+def test_fu_step_projected_particles_mixed_charges(spp, gpu_ctx):
+    """Projected samples of one PEPO carry their label's Z2 charge on the
+    auxiliary leg, so a batch of the reference's particles mixes auxiliary
+    charges; each evolves as the oracle's single state."""
+    say("FU step on projected particles with mixed auxiliary charges, 3x3")
+    L = Lattice(3, 3)
+    pepo = embed_identity_pepo(L, (1, 1), 0.5, seed=31)
+    rng = np.random.default_rng(8)
+    cfgs = np.asarray([1, 2])[rng.integers(0, 2, size=(3, 3, 3))]
+    states = [ob.project_configuration(pepo, c)[0] for c in cfgs]
+    m = TFIM(J=1.0, h=2.0, symmetric=True)
+    _compare(spp, gpu_ctx, states, m, 0.1, 64, (1, 1), ofu.FullUpdate(max_iterations=40, linear_max_iterations=100))
