@@ -1,0 +1,363 @@
+"""Particles, weights and ensemble statistics of stochastic PEPS: the FP64 host
+bookkeeping around the device path (src/driver/particles.jl:2-102,
+src/statistics/ensemble.jl:8-40, src/driver/config.jl:63-76,
+src/driver/run.jl:45-66 and 87-130, src/resample/selection.jl:122-140).
+
+Every tensor operation runs on the device: Born draws (``sample_configurations``),
+observation (``measure_energies`` on projected samples, ``measure_energy_states``
+or ``lognorms`` on resident states) and evolution (``evolution_steps``). What
+stays here is scalar per particle, kept in FP64 as the reference keeps it:
+relative log weights, multiplicities, beta schedules and the ratio estimators.
+Particles of one beta node are observed and evolved as one batch.
+"""
+import math
+import time
+
+import numpy as np
+
+from .contraction import lognorms
+from .evolution import FullUpdate, evolution_steps
+from .measurement import measure_energies, measure_energy_states
+from .sampling import DeviceState, SamplingCache, project_configuration, sample_configurations, sample_uniforms
+from .states import normalize_state, pad_bonds, resolve_basis
+
+EPS = np.finfo(np.float64).eps
+
+
+# ----------------------------------------------------------------------------
+# beta grids (config.jl:63-76, particles.jl:55-64)
+# ----------------------------------------------------------------------------
+
+def same_beta(a, b):
+    """``_same_beta`` (config.jl:63): equal up to 64 ulps of max(1, |a|, |b|)."""
+    return abs(a - b) <= 64 * EPS * max(1.0, abs(a), abs(b))
+
+
+def measure_at(beta_measure, beta):
+    """``_measure_at`` (config.jl:64): energy is measured at this node."""
+    return any(same_beta(b, beta) for b in beta_measure)
+
+
+def temperature_nodes(beta0, beta_log, beta_measure):
+    """``temperature_nodes(c)`` (config.jl:68-76): sorted grid of 0, beta0,
+    beta_log and beta_measure, near-equal values merged (beta0 wins)."""
+    out = []
+    for b in sorted([0.0, float(beta0)] + [float(v) for v in beta_log] + [float(v) for v in beta_measure]):
+        if same_beta(b, beta0):
+            b = float(beta0)
+        if out and same_beta(out[-1], b):
+            continue
+        out.append(b)
+    return out
+
+
+def step_schedule(start, target, delta_tau):
+    """``_step_schedule`` (particles.jl:55-64): beta endpoints of the Trotter
+    steps from ``start`` to ``target``, each advancing beta by 2 delta_tau, the
+    last one shortened to land on the node."""
+    steps = []
+    b = float(start)
+    while not same_beta(b, target):
+        if not target > b:
+            raise ValueError("evolution requires increasing beta endpoints")
+        b = min(float(target), b + 2 * delta_tau)
+        if same_beta(b, target):
+            b = float(target)
+        steps.append(b)
+    return steps
+
+
+# ----------------------------------------------------------------------------
+# ensemble statistics (ensemble.jl:8-40)
+# ----------------------------------------------------------------------------
+
+def ensemble_statistics(relative, multiplicity, energy, Ns, log_scale=0.0, nsites=1):
+    """``ensemble_statistics`` (ensemble.jl:8-25): self-normalised estimators
+    with weights w = exp(relative - max(relative)) times multiplicities, the
+    effective-sample fraction eta = clamp(mu1^2/mu2, 1/Nv, 1), delta-method
+    standard errors, lnZ = log_scale + pivot + log mu1 and eta_p."""
+    relative = np.asarray(relative, dtype=np.float64)
+    m = np.asarray(multiplicity, dtype=np.float64)
+    energy = np.asarray(energy, dtype=np.float64)
+    if not (len(relative) == len(m) == len(energy) > 0):
+        raise ValueError("empty or inconsistent ensemble")
+    if not (np.all(np.isfinite(relative)) and np.all(m > 0)):
+        raise ValueError("invalid particle weights/counts")
+    Nv = float(m.sum())
+    pivot = float(relative.max())
+    w = np.exp(relative - pivot)
+    mass = m * w
+    total = float(mass.sum())
+    mu1 = total / Nv
+    mu2 = float(np.sum(m * w * w)) / Nv
+    eta = min(max(mu1 * mu1 / mu2, 1.0 / Nv), 1.0)
+
+    def sem(f):
+        if Nv <= 1:
+            return math.nan
+        return math.sqrt(max(0.0, float(np.sum(m * np.abs(f) ** 2)) / (Nv * (Nv - 1))))
+
+    measured = bool(np.all(np.isfinite(energy)))
+    E = float(np.sum(mass * energy)) / total if measured else math.nan
+    E_err = sem(w * (energy - E) / mu1) if measured else math.nan
+    return {"energy_total": nsites * E, "energy_per_site": E, "energy_per_site_error": E_err,
+            "lnZ": log_scale + pivot + math.log(mu1), "lnZ_error": sem((w - mu1) / mu1), "eta": eta,
+            "eta_error": sem((2 * mu1 / mu2) * (w - mu1) - (mu1 * mu1 / (mu2 * mu2)) * (w * w - mu2)),
+            "ESS": Nv * eta,
+            "eta_p": min(max(total * total / (Ns * float(np.sum(mass * mass))), 1.0 / Ns), len(relative) / Ns),
+            "target_samples": Ns, "virtual_samples": Nv, "representatives": len(relative)}
+
+
+def group_configurations(configurations):
+    """``_group_configurations`` (selection.jl:122-140): identical draws share
+    one particle. Returns first_draw (index of each group's first draw),
+    multiplicity and draw_to_group, groups in order of first appearance."""
+    first, mult, mapping, lookup = [], [], [], {}
+    for d, cfg in enumerate(configurations):
+        key = np.asarray(cfg).tobytes()
+        j = lookup.get(key)
+        if j is None:
+            j = len(first)
+            lookup[key] = j
+            first.append(d)
+            mult.append(0)
+        mult[j] += 1
+        mapping.append(j)
+    return {"first_draw": first, "multiplicity": mult, "draw_to_group": mapping}
+
+
+# ----------------------------------------------------------------------------
+# particles (particles.jl:2-24, 42-53, 86-102)
+# ----------------------------------------------------------------------------
+
+class Particle:
+    """``Particle`` (particles.jl:2-24): a Born sample with its resident state
+    and estimator weight. ``state`` is a ``DeviceState``."""
+
+    __slots__ = ("id", "generation", "sample_index", "seed", "count", "configuration", "state", "log_p0",
+                 "log_q_proposal", "relative_log_weight", "weight_log_scale", "physical_log_norm",
+                 "tensor_log_norm", "energy_total", "energy_per_site", "max_state_truncation",
+                 "max_environment_truncation", "max_boundary_entropy", "time_seconds", "measurement_seconds")
+
+    def __init__(self, **kw):
+        for k in self.__slots__:
+            setattr(self, k, kw.get(k))
+
+    @property
+    def beta(self):
+        return self.state.state.beta
+
+    def record(self, owner=0):
+        """``_particle_scalar`` (particles.jl:103-118): the scalar record."""
+        return {"id": self.id, "generation": self.generation, "sample_index": self.sample_index,
+                "seed": self.seed, "count": self.count, "configuration": np.array(self.configuration),
+                "owner": owner, "beta": self.beta, "log_p0": self.log_p0, "log_q_proposal": self.log_q_proposal,
+                "relative_log_weight": self.relative_log_weight, "weight_log_scale": self.weight_log_scale,
+                "log_weight": self.weight_log_scale + self.relative_log_weight,
+                "physical_log_norm": self.physical_log_norm, "energy_total": self.energy_total,
+                "energy_per_site": self.energy_per_site, "max_state_truncation": self.max_state_truncation,
+                "max_environment_truncation": self.max_environment_truncation,
+                "max_boundary_entropy": self.max_boundary_entropy, "time_seconds": self.time_seconds,
+                "measurement_seconds": self.measurement_seconds}
+
+
+def observe(ctx, states, model, chi, energy):
+    """``_observe`` (particles.jl:35-41) for a batch of resident states:
+    ``measure_energy`` when ``energy``, else ``lognorm`` with NaN energies."""
+    t0 = time.perf_counter()
+    if energy:
+        r = measure_energy_states(ctx, states, model, chi)
+    else:
+        r = lognorms(ctx, states, chi)
+        r["energy_total"] = np.full(len(states), np.nan)
+        r["energy_per_site"] = np.full(len(states), np.nan)
+    r["measurement_seconds"] = (time.perf_counter() - t0) / max(len(states), 1)
+    return r
+
+
+def new_particles(ctx, cache, draws, model, chi_measurement, energy, bond_space=None, pepo_live=None, seed=0,
+                  first_id=1):
+    """``_initial_particles`` after the draws (run.jl:45-66) with ``_new_particle``
+    (particles.jl:42-53) for each distinct configuration: project the PEPO
+    (host slice + ``normalize_state!``), upload it as a resident PEPS (bonds
+    embedded in ``bond_space`` = (even, odd) capacities when given, live
+    sectors ``pepo_live`` [Lx*Ly, 4, 2] of the PEPO), observe the batch and set
+    relative_log_weight = log_norm - log q.
+
+    ``draws`` is the dict of ``sample_configurations``. When ``energy`` and
+    ``chi_measurement`` allow it, the projected samples are measured through
+    the sampling cache's shared layers (same values as measuring the uploaded
+    states)."""
+    grouping = group_configurations(draws["configuration"])
+    reps = grouping["first_draw"]
+    pepo = cache.state
+    states = []
+    for i in reps:
+        st, _ = project_configuration(pepo, draws["configuration"][i], cache.basis)
+        live = pepo_live
+        if bond_space is not None:
+            st, live0 = pad_bonds(st, bond_space)
+            live = live0 if pepo_live is None else pepo_live
+        states.append(DeviceState(ctx, st, live=live))
+    t0 = time.perf_counter()
+    if energy:
+        m = measure_energies(cache, draws["configuration"][reps], model, chi_measurement)
+    else:
+        m = lognorms(ctx, states, chi_measurement)
+        m["energy_total"] = np.full(len(reps), np.nan)
+        m["energy_per_site"] = np.full(len(reps), np.nan)
+    mt_sec = (time.perf_counter() - t0) / max(len(reps), 1)
+    out = []
+    for u, i in enumerate(reps):
+        log_q = float(draws["log_probability"][i])
+        out.append(Particle(
+            id=first_id + i, generation=0, sample_index=first_id + i, seed=seed,
+            count=grouping["multiplicity"][u], configuration=np.array(draws["configuration"][i]),
+            state=states[u], log_p0=log_q, log_q_proposal=log_q,
+            relative_log_weight=float(m["log_norm"][u]) - log_q, weight_log_scale=0.0,
+            physical_log_norm=float(m["log_norm"][u]), tensor_log_norm=float(m["tensor_log_norm"][u]),
+            energy_total=float(m["energy_total"][u]), energy_per_site=float(m["energy_per_site"][u]),
+            max_state_truncation=0.0,
+            max_environment_truncation=max(float(draws["max_environment_truncation"][i]),
+                                           float(m["max_environment_truncation"][u])),
+            max_boundary_entropy=math.nan, time_seconds=0.0, measurement_seconds=mt_sec))
+    return out, grouping
+
+
+def advance_particles(ctx, particles, model, steps, chi_environment, chi_measurement, method, energy, order=2,
+                      on_step=None):
+    """``_advance_particle!`` (particles.jl:86-102) with ``_evolve!``
+    (particles.jl:66-85) for a batch of particles at the same beta: one
+    full-update step per endpoint in ``steps`` (delta = (beta - state.beta)/2),
+    then ``_observe``; relative_log_weight += 2 sum(log_normalization) +
+    (tensor_log_norm_new - tensor_log_norm_old)."""
+    if not particles:
+        return particles
+    devs = [p.state for p in particles]
+    normalization = np.zeros(len(particles))
+    maxstate = np.zeros(len(particles))
+    maxenv = np.zeros(len(particles))
+    total = 0.0
+    for beta in steps:
+        b0 = devs[0].state.beta
+        if any(not same_beta(d.state.beta, b0) for d in devs):
+            raise ValueError("particles of one batch must share beta")
+        t0 = time.perf_counter()
+        r = evolution_steps(ctx, devs, model, (beta - b0) / 2, chi_environment, method, order)
+        elapsed = time.perf_counter() - t0
+        for d in devs:
+            d.state.beta = beta
+        normalization += r["normalizations"].sum(axis=1)
+        if r["records"].shape[1]:
+            maxstate = np.maximum(maxstate, r["records"][:, :, 1].max(axis=1))
+        maxenv = np.maximum(maxenv, r["max_environment_truncation"])
+        total += elapsed
+        if on_step is not None:
+            on_step({"beta": beta, "time_seconds": elapsed, **r})
+    m = observe(ctx, devs, model, chi_measurement, energy)
+    for k, p in enumerate(particles):
+        old = p.tensor_log_norm
+        p.relative_log_weight += 2.0 * float(normalization[k]) + (float(m["tensor_log_norm"][k]) - old)
+        p.physical_log_norm = float(m["log_norm"][k])
+        p.tensor_log_norm = float(m["tensor_log_norm"][k])
+        p.energy_total = float(m["energy_total"][k])
+        p.energy_per_site = float(m["energy_per_site"][k])
+        p.max_state_truncation = float(maxstate[k])
+        p.max_environment_truncation = max(float(maxenv[k]), float(m["max_environment_truncation"][k]))
+        p.time_seconds = total / len(particles)
+        p.measurement_seconds = m["measurement_seconds"]
+    return particles
+
+
+def aligned_weights(records):
+    """``_aligned_weights`` (ensemble.jl:26-30): relative weights on a common scale."""
+    scale = max(r["weight_log_scale"] for r in records)
+    return [r["relative_log_weight"] + (r["weight_log_scale"] - scale) for r in records], scale
+
+
+def statistics(records, Ns, nsites):
+    """``_statistics`` (ensemble.jl:31-40) over particle records."""
+    relative, scale = aligned_weights(records)
+    s = ensemble_statistics(relative, [r["count"] for r in records], [r["energy_per_site"] for r in records], Ns,
+                            log_scale=scale, nsites=nsites)
+    s.update({"beta": records[0]["beta"],
+              "max_state_truncation": max(r["max_state_truncation"] for r in records),
+              "max_environment_truncation": max(r["max_environment_truncation"] for r in records),
+              "max_boundary_entropy": max(r["max_boundary_entropy"] for r in records),
+              "distinct_configurations": len({np.asarray(r["configuration"]).tobytes() for r in records})})
+    return s
+
+
+# ----------------------------------------------------------------------------
+# drivers (run.jl:2-43, 45-66, 87-130), without I/O, MPI or resampling
+# ----------------------------------------------------------------------------
+
+def run_purification(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, chi_environment, chi_measurement,
+                     bond_space, method=None, order=2):
+    """``run_purification`` (run.jl:2-43) from the identity PEPO
+    (``identity_pepo``; normalised here as the reference does), on the device:
+    FU steps to every node <= beta0 and an observation there. Returns the
+    resident PEPO (``DeviceState``, bonds at the ``bond_space`` capacity with
+    their live sectors) and the rows (beta, energy_total, energy_per_site,
+    lnZ, truncations, seconds)."""
+    method = method or FullUpdate()
+    st, live = pad_bonds(pepo, bond_space)
+    normalize_state(st)
+    dev = DeviceState(ctx, st, live=live)
+    rows = []
+    for beta in [b for b in temperature_nodes(beta0, beta_log, beta_measure) if b <= beta0]:
+        steps = step_schedule(dev.state.beta, beta, delta_tau)
+        p = Particle(state=dev, tensor_log_norm=0.0, relative_log_weight=0.0)
+        advance_particles(ctx, [p], model, steps, chi_environment, chi_measurement, method,
+                          measure_at(beta_measure, beta), order)
+        rows.append({"beta": beta, "source": "purification", "energy_total": p.energy_total,
+                     "energy_per_site": p.energy_per_site,
+                     "energy_per_site_error": 0.0 if measure_at(beta_measure, beta) else math.nan,
+                     "lnZ": p.physical_log_norm, "lnZ_error": 0.0, "max_state_truncation": p.max_state_truncation,
+                     "max_environment_truncation": p.max_environment_truncation, "time_seconds": p.time_seconds,
+                     "measurement_seconds": p.measurement_seconds})
+    return dev, rows
+
+
+def run_stochastic(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, Ns, seed, chi_sampling,
+                   chi_environment, chi_measurement, bond_space=None, method=None, order=2, basis=None,
+                   uniforms=None, on_node=None):
+    """``run_stochastic`` (run.jl:87-130) without resampling: Ns Born draws
+    from the beta0 PEPO (``pepo``: a host ``FiniteState`` or a resident
+    ``DeviceState``), one particle per distinct configuration, then FU
+    evolution through every node >= beta0 with an observation and the
+    ensemble statistics at each. Uniforms default to ``sample_uniforms(seed,
+    i, Lx*Ly)`` for sample indices i = 1..Ns. Returns (particles, rows)."""
+    method = method or FullUpdate()
+    live = None
+    if isinstance(pepo, DeviceState):
+        live = pepo.live()
+        host = pepo.download()
+    else:
+        host = pepo
+    if not same_beta(host.beta, beta0):
+        raise ValueError("checkpoint beta0 mismatch")
+    L = host.lattice
+    ns = L.Lx * L.Ly
+    cache = SamplingCache(ctx, host, chi_sampling, basis=resolve_basis(host, basis))
+    if uniforms is None:
+        uniforms = np.stack([sample_uniforms(seed, i, ns) for i in range(1, Ns + 1)])
+    draws = sample_configurations(cache, uniforms=uniforms)
+    nodes = [b for b in temperature_nodes(beta0, beta_log, beta_measure) if b >= beta0]
+    particles, grouping = new_particles(ctx, cache, draws, model, chi_measurement,
+                                        measure_at(beta_measure, nodes[0]), bond_space=bond_space,
+                                        pepo_live=live, seed=seed)
+    rows = []
+    for node, beta in enumerate(nodes):
+        if node > 0:
+            steps = step_schedule(nodes[node - 1], beta, delta_tau)
+            advance_particles(ctx, particles, model, steps, chi_environment, chi_measurement, method,
+                              measure_at(beta_measure, beta), order)
+        records = [p.record() for p in particles]
+        stats = statistics(records, Ns, ns)
+        stats["source"] = "sampling"
+        stats["time_seconds"] = sum(p.time_seconds for p in particles)
+        rows.append(stats)
+        if on_node is not None:
+            on_node(stats)
+    return particles, rows

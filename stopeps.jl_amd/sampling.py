@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .states import parity_sorted, resolve_basis, FiniteState, Site
+from .states import parity_sorted, resolve_basis, normalize_state, FiniteState, Site
 
 MASK64 = (1 << 64) - 1
 PRECISIONS = {"fp64": 0, "fp32": 1}
@@ -310,12 +310,4 @@ def project_configuration(pepo, configuration, basis=None):
             par[3] = np.array([charge])
             sites[x][y] = Site(np.ascontiguousarray(a), par)
     st = FiniteState(L, sites, "peps", beta=pepo.beta, log_amplitude=pepo.log_amplitude)
-    records = []
-    for (x, y) in L.site_order():
-        v = float(np.linalg.norm(st[(x, y)].data))
-        if not (np.isfinite(v) and v > 0):
-            raise ValueError(f"zero or nonfinite tensor at {(x + 1, y + 1)}")
-        st.sites[x][y] = Site(st[(x, y)].data / v, st[(x, y)].parity)
-        st.log_amplitude += np.log(v)
-        records.append({"site": (x + 1, y + 1), "log_normalization": float(np.log(v))})
-    return st, records
+    return st, normalize_state(st)

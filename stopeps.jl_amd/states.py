@@ -94,6 +94,21 @@ def identity_pepo(lattice, symmetric=True):
     return FiniteState(lattice, sites, "pepo")
 
 
+def normalize_state(state):
+    """``normalize_state!`` (src/states/finite.jl:89-100): divide every site by
+    its Frobenius norm in row-major order and add the logs to log_amplitude,
+    in place. Returns the records {"site": (x+1, y+1), "log_normalization"}."""
+    records = []
+    for (x, y) in state.lattice.site_order():
+        v = float(np.linalg.norm(state[(x, y)].data))
+        if not (np.isfinite(v) and v > 0):
+            raise ValueError(f"zero or nonfinite tensor at {(x + 1, y + 1)}")
+        state.sites[x][y] = Site(state[(x, y)].data / v, state[(x, y)].parity)
+        state.log_amplitude += np.log(v)
+        records.append({"site": (x + 1, y + 1), "log_normalization": float(np.log(v))})
+    return records
+
+
 def synthetic_thermal_pepo(lattice, D=(3, 3), eps=0.3, seed=1):
     """Synthetic Z2 PEPO at production shapes: the beta = 0 identity PEPO
     embedded in bond spaces Z2Space(0=>D[0], 1=>D[1]) plus a seeded Gaussian
