@@ -92,9 +92,10 @@ int spp_state_download(spp_ctx* ctx, const spp_state* state, double* const* data
 
 /* `evolution_step!(state, model, delta, D, FullUpdate(...), BoundaryMPS(chi);
  * order, sector_space)` (src/evolution/full_update.jl:563-597) for n states on
- * the same lattice with the same leg spaces, updated in place. The bond
- * capacity is the uploaded bond space and doubles as the sector budget
- * (`sector_space`), D = its dimension. options = {max_iterations,
+ * the same lattice with the same leg spaces (up to the auxiliary charge),
+ * updated in place. The bond capacity is the uploaded bond space and doubles
+ * as the sector budget (`sector_space`); D is the truncrank of the bond
+ * (D <= 0: the capacity's dimension). options = {max_iterations,
  * error_tolerance, diagonal_shift, linear_max_iterations, linear_tolerance}
  * (`FullUpdate`, full_update.jl:11-17). gates_h [Ly*(Lx-1)][d^4] and gates_v
  * [(Ly-1)*Lx][d^4] are `trotter_gate(model, lattice, i, j, delta_gate)`
@@ -108,7 +109,7 @@ int spp_state_download(spp_ctx* ctx, const spp_state* state, double* const* data
  * state's log_amplitude); max_env_trunc [n]. The PSD repair of the bond metric
  * (full_update.jl:177-330) is not available: a bond that needs it fails the
  * call with the reference's message. */
-int spp_fu_evolution_step(spp_ctx* ctx, int n, spp_state* const* states, int chi, int order, double delta,
+int spp_fu_evolution_step(spp_ctx* ctx, int n, spp_state* const* states, int chi, int order, double delta, int D,
                           const double* options, const double* gates_h, const double* gates_v, double* records,
                           double* normalizations, double* max_env_trunc);
 

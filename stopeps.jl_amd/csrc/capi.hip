@@ -225,7 +225,7 @@ int spp_state_download(spp_ctx* ctx, const spp_state* state, double* const* data
   });
 }
 
-int spp_fu_evolution_step(spp_ctx* ctx, int n, spp_state* const* states, int chi, int order, double delta,
+int spp_fu_evolution_step(spp_ctx* ctx, int n, spp_state* const* states, int chi, int order, double delta, int D,
                           const double* options, const double* gates_h, const double* gates_v, double* records,
                           double* normalizations, double* max_env_trunc) {
   return guarded(ctx, [&] {
@@ -240,7 +240,7 @@ int spp_fu_evolution_step(spp_ctx* ctx, int n, spp_state* const* states, int chi
       v[i] = &states[i]->s;
     }
     FuParams P;
-    P.opts = fu::Opts{(int)options[0], options[1], options[2], (int)options[3], options[4]};
+    P.opts = fu::Opts{(int)options[0], options[1], options[2], (int)options[3], options[4], D > 0 ? D : 0};
     if (P.opts.max_iterations < 0 || P.opts.linear_max_iterations < 0) throw HipError("negative iteration limit");
     P.order = order;
     P.delta = delta;

@@ -157,6 +157,7 @@ struct Opts {
   double diagonal_shift;
   int linear_max_iterations;
   double linear_tolerance;
+  int rank = 0;  // truncrank(D) of the SVD start; 0: the bond capacity's dimension
 };
 
 enum Stop { kDisabled = 0, kIterationLimit = 1, kErrorTolerance = 2, kResidualFallback = 3 };
@@ -1077,8 +1078,9 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
     }
     // truncrank(D) over both sectors (ties: sector 0 first, earlier first),
     // truncspace caps (the bond capacity), trunctol(atol)
+    const int rank = o.rank > 0 && o.rank < D ? o.rank : D;
     int take_q[2] = {0, 0};
-    for (int r = 0; r < D; ++r) {
+    for (int r = 0; r < rank; ++r) {
       int pick = -1;
       double bestv = -1.0;
       for (int q = 0; q < 2; ++q)

@@ -69,11 +69,13 @@ def _gates(model, lattice, delta, ppar):
     return gh, gv
 
 
-def evolution_steps(ctx, states, model, delta, chi, method=None, order=2):
+def evolution_steps(ctx, states, model, delta, chi, method=None, order=2, D=None):
     """``evolution_step!(state, model, delta, D, FullUpdate(...), BoundaryMPS(chi);
     order, sector_space)`` for every uploaded state (``DeviceState``; same lattice
-    and leg spaces), in place on the device. D and ``sector_space`` are the
-    states' uploaded bond spaces. Returns a dict: records [n, updates, 8]
+    and leg spaces up to the auxiliary charge), in place on the device.
+    ``sector_space`` is the states' uploaded bond space (the per-parity
+    capacity); ``D`` the truncrank of every bond (None: the capacity's
+    dimension). Returns a dict: records [n, updates, 8]
     (initial_cost, final_cost, residual, iterations, iteration_limit_reached,
     stop, kept_even, kept_odd), normalizations [n, Lx*Ly],
     max_environment_truncation [n]."""
@@ -99,16 +101,17 @@ def evolution_steps(ctx, states, model, delta, chi, method=None, order=2):
     handles = (ctypes.c_void_p * n)(*[d.handle.value for d in devs])
     o = method.options()
     _lib.check(ctx.handle, ctx._lib.spp_fu_evolution_step(
-        ctx.handle, n, handles, int(chi), int(order), float(delta), _lib.dptr(o), _lib.dptr(gh), _lib.dptr(gv),
+        ctx.handle, n, handles, int(chi), int(order), float(delta), int(D or 0), _lib.dptr(o), _lib.dptr(gh),
+        _lib.dptr(gv),
         _lib.dptr(rec), _lib.dptr(norms), _lib.dptr(mt)))
     for d in devs:
         d.state.beta += 2 * delta
     return {"records": rec, "normalizations": norms, "max_environment_truncation": mt}
 
 
-def evolution_step(ctx, state, model, delta, chi, method=None, order=2):
+def evolution_step(ctx, state, model, delta, chi, method=None, order=2, D=None):
     """One state: the report of :func:`evolution_steps` for it."""
-    r = evolution_steps(ctx, [state], model, delta, chi, method, order)
+    r = evolution_steps(ctx, [state], model, delta, chi, method, order, D)
     recs = r["records"][0]
     return {"records": [{"initial_cost": float(v[0]), "final_cost": float(v[1]), "residual": float(v[2]),
                          "iterations": int(v[3]), "iteration_limit_reached": bool(v[4]),

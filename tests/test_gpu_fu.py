@@ -128,3 +128,30 @@ def test_fu_step_projected_particles_mixed_charges(spp, gpu_ctx):
     states = [ob.project_configuration(pepo, c)[0] for c in cfgs]
     m = TFIM(J=1.0, h=2.0, symmetric=True)
     _compare(spp, gpu_ctx, states, m, 0.1, 64, (1, 1), ofu.FullUpdate(max_iterations=40, linear_max_iterations=100))
+
+
+def test_fu_step_rank_below_sector_budget(spp, gpu_ctx):
+    """truncrank(D) without a sector budget (the reference's bond_space =
+    nothing): the device holds each bond at (D, D) per parity and truncates to
+    rank D, as the oracle with caps None does. Two steps, 3x2, D=2."""
+    say("FU two steps, truncrank D=2 with capacity (2,2) vs no sector budget")
+    m = TFIM(J=1.0, h=2.0, symmetric=True)
+    opts = ofu.FullUpdate(max_iterations=40, linear_max_iterations=200)
+    o = random_state(Lattice(3, 2), (1, 1), "peps", seed=35)
+    ctx = gpu_ctx
+    ctx.set_precision("fp64")
+    s, live = spp.pad_bonds(to_product_state(spp, o), (2, 2))
+    dev = spp.DeviceState(ctx, s, live=live)
+    model = spp.TFIM(m.J, m.h, symmetric=True)
+    method = spp.FullUpdate(opts.max_iterations, opts.error_tolerance, opts.diagonal_shift,
+                            opts.linear_max_iterations, opts.linear_tolerance)
+    for step in range(2):
+        got = spp.evolution_steps(ctx, [dev], model, 0.3, 64, method, D=2)
+        ref = ofu.evolution_step(o, m, 0.3, 2, None, opts, 64)
+        np.testing.assert_allclose(got["records"][0, :, 1], [r["final_cost"] for r in ref["records"]],
+                                   rtol=1e-6, atol=1e-9)
+        assert np.all(got["records"][0, :, 6] + got["records"][0, :, 7] <= 2)
+        ev = to_oracle(dev.download())
+        err = np.linalg.norm(dense_vector(ev) - dense_vector(o)) / np.linalg.norm(dense_vector(o))
+        say(f"  step {step}: |dpsi|/|psi| = {err:.2e}")
+        assert err <= 1e-8
