@@ -7,7 +7,7 @@ package as ``stopeps_jl_amd``). Compute runs in ``libstopeps_hip.so``
 is the host-side mirror of the reference's Julia API for that path.
 """
 from .states import (Lattice, Site, FiniteState, FinitePEPO, FinitePEPS, identity_pepo, parity_sorted,
-                     synthetic_thermal_pepo, random_peps, pad_bonds, normalize_state,
+                     synthetic_thermal_pepo, random_peps, pad_bonds, normalize_state, reembed_bonds,
                      LocalBasis, SymmetryBasis, XBasis, resolve_basis)
 from .sampling import (Context, DeviceState, SamplingCache, BornError, sample_configuration,
                        sample_configurations, sample_uniforms, project_configuration, splitmix64)
@@ -15,8 +15,10 @@ from .measurement import (TFIM, spin_operators, operator_factors, EnergyError, m
                           measure_energy_states)
 from .contraction import NormError, lognorm, lognorms, lognorm_gradient, lognorm_gradients
 from .evolution import FullUpdate, trotter_gate, evolution_step, evolution_steps
-from .particles import (Particle, ensemble_statistics, temperature_nodes, step_schedule, run_purification,
-                        run_stochastic, new_particles, advance_particles, statistics)
+from .particles import (Particle, ensemble_statistics, step_schedule, purify, stochastic_ensemble, new_particles,
+                        advance_particles, statistics, group_configurations)
+from .config import RunConfig, ResamplingOptions, validate_config, temperature_nodes, parameter_dict
+from .driver import ThermalCheckpoint, run_purification, run_stochastic
 from . import _lib, particles
 
 __all__ = ["synthetic_thermal_pepo", "random_peps", "Lattice", "Site", "FiniteState", "FinitePEPO", "FinitePEPS", "identity_pepo", "parity_sorted",
@@ -26,5 +28,6 @@ __all__ = ["synthetic_thermal_pepo", "random_peps", "Lattice", "Site", "FiniteSt
            "measure_energies", "measure_energy", "measure_energy_states", "NormError", "lognorm", "lognorms",
            "lognorm_gradient", "lognorm_gradients", "pad_bonds", "FullUpdate", "trotter_gate",
            "evolution_step", "evolution_steps", "Particle", "ensemble_statistics", "temperature_nodes",
-           "step_schedule", "run_purification", "run_stochastic", "new_particles", "advance_particles", "statistics",
-           "normalize_state"]
+           "step_schedule", "purify", "stochastic_ensemble", "new_particles", "advance_particles", "statistics",
+           "group_configurations", "normalize_state", "reembed_bonds", "RunConfig", "ResamplingOptions",
+           "validate_config", "parameter_dict", "ThermalCheckpoint", "run_purification", "run_stochastic"]

@@ -19,7 +19,7 @@ from .contraction import lognorms
 from .evolution import FullUpdate, evolution_steps
 from .measurement import measure_energies, measure_energy_states
 from .sampling import DeviceState, SamplingCache, project_configuration, sample_configurations, sample_uniforms
-from .states import normalize_state, pad_bonds, resolve_basis
+from .states import normalize_state, pad_bonds, reembed_bonds, resolve_basis
 
 EPS = np.finfo(np.float64).eps
 
@@ -181,7 +181,8 @@ def new_particles(ctx, cache, draws, model, chi_measurement, energy, bond_space=
     (particles.jl:42-53) for each distinct configuration: project the PEPO
     (host slice + ``normalize_state!``), upload it as a resident PEPS (bonds
     embedded in ``bond_space`` = (even, odd) capacities when given, live
-    sectors ``pepo_live`` [Lx*Ly, 4, 2] of the PEPO), observe the batch and set
+    sectors ``pepo_live`` [Lx*Ly, 4, 2] of the PEPO, the PEPO's zero slots
+    dropped), observe the batch and set
     relative_log_weight = log_norm - log q.
 
     ``draws`` is the dict of ``sample_configurations``. When ``energy`` and
@@ -196,8 +197,10 @@ def new_particles(ctx, cache, draws, model, chi_measurement, energy, bond_space=
         st, _ = project_configuration(pepo, draws["configuration"][i], cache.basis)
         live = pepo_live
         if bond_space is not None:
-            st, live0 = pad_bonds(st, bond_space)
-            live = live0 if pepo_live is None else pepo_live
+            if pepo_live is None:
+                st, live = pad_bonds(st, bond_space)
+            else:
+                st, live = reembed_bonds(st, pepo_live, bond_space)
         states.append(DeviceState(ctx, st, live=live))
     t0 = time.perf_counter()
     if energy:
@@ -225,12 +228,13 @@ def new_particles(ctx, cache, draws, model, chi_measurement, energy, bond_space=
 
 
 def advance_particles(ctx, particles, model, steps, chi_environment, chi_measurement, method, energy, order=2,
-                      on_step=None):
+                      on_step=None, D=None):
     """``_advance_particle!`` (particles.jl:86-102) with ``_evolve!``
     (particles.jl:66-85) for a batch of particles at the same beta: one
     full-update step per endpoint in ``steps`` (delta = (beta - state.beta)/2),
     then ``_observe``; relative_log_weight += 2 sum(log_normalization) +
-    (tensor_log_norm_new - tensor_log_norm_old)."""
+    (tensor_log_norm_new - tensor_log_norm_old). ``D`` is the bonds' truncrank
+    (None: the resident bond capacity's dimension)."""
     if not particles:
         return particles
     devs = [p.state for p in particles]
@@ -243,7 +247,7 @@ def advance_particles(ctx, particles, model, steps, chi_environment, chi_measure
         if any(not same_beta(d.state.beta, b0) for d in devs):
             raise ValueError("particles of one batch must share beta")
         t0 = time.perf_counter()
-        r = evolution_steps(ctx, devs, model, (beta - b0) / 2, chi_environment, method, order)
+        r = evolution_steps(ctx, devs, model, (beta - b0) / 2, chi_environment, method, order, D)
         elapsed = time.perf_counter() - t0
         for d in devs:
             d.state.beta = beta
@@ -289,12 +293,13 @@ def statistics(records, Ns, nsites):
 
 
 # ----------------------------------------------------------------------------
-# drivers (run.jl:2-43, 45-66, 87-130), without I/O, MPI or resampling
+# ensemble drivers (run.jl:2-43, 45-66, 87-130) on explicit arguments; the
+# RunConfig entry points are in driver.py
 # ----------------------------------------------------------------------------
 
-def run_purification(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, chi_environment, chi_measurement,
-                     bond_space, method=None, order=2):
-    """``run_purification`` (run.jl:2-43) from the identity PEPO
+def purify(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, chi_environment, chi_measurement, bond_space,
+           method=None, order=2, D=None):
+    """The evolution of ``run_purification`` (run.jl:2-43) from the identity PEPO
     (``identity_pepo``; normalised here as the reference does), on the device:
     FU steps to every node <= beta0 and an observation there. Returns the
     resident PEPO (``DeviceState``, bonds at the ``bond_space`` capacity with
@@ -309,7 +314,7 @@ def run_purification(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau,
         steps = step_schedule(dev.state.beta, beta, delta_tau)
         p = Particle(state=dev, tensor_log_norm=0.0, relative_log_weight=0.0)
         advance_particles(ctx, [p], model, steps, chi_environment, chi_measurement, method,
-                          measure_at(beta_measure, beta), order)
+                          measure_at(beta_measure, beta), order, D=D)
         rows.append({"beta": beta, "source": "purification", "energy_total": p.energy_total,
                      "energy_per_site": p.energy_per_site,
                      "energy_per_site_error": 0.0 if measure_at(beta_measure, beta) else math.nan,
@@ -319,10 +324,10 @@ def run_purification(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau,
     return dev, rows
 
 
-def run_stochastic(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, Ns, seed, chi_sampling,
-                   chi_environment, chi_measurement, bond_space=None, method=None, order=2, basis=None,
-                   uniforms=None, on_node=None):
-    """``run_stochastic`` (run.jl:87-130) without resampling: Ns Born draws
+def stochastic_ensemble(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, Ns, seed, chi_sampling,
+                        chi_environment, chi_measurement, bond_space=None, method=None, order=2, basis=None,
+                        uniforms=None, on_node=None, D=None, draws_out=None):
+    """The ensemble of ``run_stochastic`` (run.jl:87-130) without resampling: Ns Born draws
     from the beta0 PEPO (``pepo``: a host ``FiniteState`` or a resident
     ``DeviceState``), one particle per distinct configuration, then FU
     evolution through every node >= beta0 with an observation and the
@@ -343,6 +348,8 @@ def run_stochastic(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, N
     if uniforms is None:
         uniforms = np.stack([sample_uniforms(seed, i, ns) for i in range(1, Ns + 1)])
     draws = sample_configurations(cache, uniforms=uniforms)
+    if draws_out is not None:
+        draws_out.update(draws)
     nodes = [b for b in temperature_nodes(beta0, beta_log, beta_measure) if b >= beta0]
     particles, grouping = new_particles(ctx, cache, draws, model, chi_measurement,
                                         measure_at(beta_measure, nodes[0]), bond_space=bond_space,
@@ -352,12 +359,12 @@ def run_stochastic(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, N
         if node > 0:
             steps = step_schedule(nodes[node - 1], beta, delta_tau)
             advance_particles(ctx, particles, model, steps, chi_environment, chi_measurement, method,
-                              measure_at(beta_measure, beta), order)
+                              measure_at(beta_measure, beta), order, D=D)
         records = [p.record() for p in particles]
         stats = statistics(records, Ns, ns)
         stats["source"] = "sampling"
         stats["time_seconds"] = sum(p.time_seconds for p in particles)
         rows.append(stats)
         if on_node is not None:
-            on_node(stats)
+            on_node(stats, records)
     return particles, rows

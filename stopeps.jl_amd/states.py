@@ -207,6 +207,45 @@ def pad_bonds(state, capacity):
     return FiniteState(L, sites, state.kind, beta=state.beta, log_amplitude=state.log_amplitude), live
 
 
+def reembed_bonds(state, live, capacity):
+    """Re-embed every interior virtual leg in a bond space of (even, odd) =
+    ``capacity``: keep its live slots (the first live[site, leg, q] indices of
+    parity q, in leg order) and append zero slots. ``live`` [Lx*Ly, 4, 2] as in
+    :func:`pad_bonds`. Returns the state and its live counts; a live sector
+    larger than the capacity raises."""
+    L = state.lattice
+    ce, co = capacity
+    legs = (0, 1, 4, 5)
+    live = np.asarray(live).reshape(L.Lx * L.Ly, 4, 2)
+    sites = [[None] * L.Ly for _ in range(L.Lx)]
+    for (x, y) in L.site_order():
+        s = state[(x, y)]
+        a = s.data
+        par = list(s.parity)
+        for v, k in enumerate(legs):
+            p = par[k]
+            edge = len(p) == 1 and ((k == 0 and x == 0) or (k == 1 and y == 0) or (k == 4 and x == L.Lx - 1)
+                                    or (k == 5 and y == L.Ly - 1))
+            if edge:
+                continue
+            e, o = int(live[x + y * L.Lx, v, 0]), int(live[x + y * L.Lx, v, 1])
+            if e > ce or o > co:
+                raise ValueError("live bond sectors exceed the capacity")
+            keep = np.concatenate([np.nonzero(p == 0)[0][:e], np.nonzero(p == 1)[0][:o]])
+            keep.sort(kind="stable")
+            a = np.take(a, keep, axis=k)
+            p = p[keep]
+            extra = np.array([0] * (ce - e) + [1] * (co - o), dtype=np.int64)
+            if len(extra):
+                shape = list(a.shape)
+                shape[k] = len(extra)
+                a = np.concatenate([a, np.zeros(shape)], axis=k)
+                p = np.concatenate([p, extra])
+            par[k] = p
+        sites[x][y] = Site(a, par)
+    return FiniteState(L, sites, state.kind, beta=state.beta, log_amplitude=state.log_amplitude), live.copy()
+
+
 def parity_sorted(site):
     """Dense array with every leg sorted by parity (stable), and even counts."""
     a = site.data

@@ -76,15 +76,16 @@ def oracle_pipeline():
 
 
 def test_stochastic_pipeline_matches_oracle(spp, gpu_ctx):
-    from stopeps_jl_amd import particles as P
-    say("stochastic PEPS pipeline 3x3 Z2 D=(1,1) chi=16: purification -> draws -> FU -> statistics")
+    say("stochastic PEPS pipeline 3x3 Z2 D=(1,1) chi=16: RunConfig -> run_purification -> run_stochastic")
     ctx = gpu_ctx
     ctx.set_precision("fp64")
-    model = spp.TFIM(1.0, H, symmetric=True)
-    method = spp.FullUpdate(max_iterations=40, linear_max_iterations=100)
-    lat = spp.Lattice(*L3)
-    dev, rows = P.run_purification(ctx, spp.identity_pepo(lat), model, BETA0, [], [BETA0, BETA1], DT, CHI, CHI,
-                                   CAPS, method)
+    c = spp.RunConfig(lattice=spp.Lattice(*L3), model=spp.TFIM(1.0, H, symmetric=True), beta0=BETA0, beta_log=[],
+                      beta_measure=[BETA0, BETA1], delta_tau_pur=DT, delta_tau_peps=DT, D_pur=sum(CAPS),
+                      D_peps=sum(CAPS), bond_space_pur=CAPS, bond_space_peps=CAPS, chi_environment=CHI,
+                      update=spp.FullUpdate(max_iterations=40, linear_max_iterations=100), Ns=NS, seed=1,
+                      resampling=spp.ResamplingOptions(enabled=False))
+    cp = spp.run_purification(c, ctx)
+    rows = cp.purification
     pur, cfgs, grp, parts, stats = oracle_pipeline()
     assert [r["beta"] for r in rows] == [0.0, BETA0]
     assert abs(rows[-1]["energy_total"] - pur[0]["energy_total"]) < 1e-9
@@ -92,9 +93,13 @@ def test_stochastic_pipeline_matches_oracle(spp, gpu_ctx):
     # beta = 0 (not a measurement node, so only lognorm): the normalised
     # identity PEPO has lnZ = log Tr 1 = Lx*Ly*log 2
     assert np.isnan(rows[0]["energy_total"]) and abs(rows[0]["lnZ"] - 9 * np.log(2.0)) < 1e-12
-    particles, srows = P.run_stochastic(ctx, dev, model, BETA0, [], [BETA0, BETA1], DT, NS, 1, CHI, CHI, CHI,
-                                        bond_space=CAPS, method=method)
+    result = spp.run_stochastic(c, cp, ctx)
+    assert result["schema_version"] == "2.7.0" and result["metadata"]["complete"]
+    curve = result["observables"]["curve"]
+    assert [r["beta"] for r in curve] == spp.temperature_nodes(c)
+    particles = result["particles"]
     assert len(particles) == len(grp["first_draw"])
+    assert len(result["observables"]["samples"]) == 2 * len(particles)
     for p, i, mult in zip(particles, grp["first_draw"], grp["multiplicity"]):
         assert np.array_equal(p.configuration, cfgs[i]) and p.count == mult
     for p, q in zip(particles, parts):
@@ -103,6 +108,7 @@ def test_stochastic_pipeline_matches_oracle(spp, gpu_ctx):
         assert abs(p.relative_log_weight - q["rel"]) < 1e-8
         assert abs(p.energy_per_site - q["e"]) < 1e-9
         assert abs(p.beta - BETA1) < 1e-15
+    srows = [r for r in curve if r.get("source") == "sampling"]
     assert [r["beta"] for r in srows] == [BETA0, BETA1]
     for r, s in zip(srows, stats):
         for k in ("energy_per_site", "energy_per_site_error", "lnZ", "lnZ_error", "eta", "ESS"):

@@ -97,3 +97,28 @@ def test_normalize_state(spp):
     assert len(rec) == 4 and all(abs(r["log_normalization"] - np.log(3.0)) < 1e-14 for r in rec)
     assert abs(st.log_amplitude - 4 * np.log(3.0)) < 1e-13
     assert all(abs(np.linalg.norm(st[xy].data) - 1.0) < 1e-14 for xy in st.lattice.site_order())
+
+
+def test_run_config_mirror(spp):
+    """RunConfig defaults, derived chis, validation and the beta grid
+    (config.jl:20-115)."""
+    c = spp.RunConfig()
+    assert c.chi_sampling == c.chi_measurement == c.chi_proposal == c.chi_environment == 32
+    assert c.beta0 == 0.1 and c.Ns == 16 and c.seed == 20260924 and c.resampling.enabled
+    assert spp.temperature_nodes(c) == [0.0, 0.05, 0.1, 0.15, 0.2, 0.25, 0.3]
+    spp.validate_config(c)
+    for bad in (dict(delta_tau_pur=0.0), dict(Ns=0), dict(trotter_order=3), dict(beta0=-1.0),
+                dict(bond_space_pur=(0, 0), model=spp.TFIM(1.0, 1.0, symmetric=True))):
+        with pytest.raises(ValueError):
+            spp.validate_config(spp.RunConfig(**bad))
+    p = spp.parameter_dict(spp.RunConfig(model=spp.TFIM(1.0, 3.04, symmetric=True), lattice=spp.Lattice(4, 4)))
+    assert p["model"] == {"type": "TFIM", "J": 1.0, "h": 3.04, "physical_space": "Z2Space(0=>1,1=>1)"}
+    assert p["lattice"] == {"type": "Lattice", "Lx": 4, "Ly": 4} and p["contraction"] == "BoundaryMPS-Row"
+    assert p["sampling_basis"]["labels"] == [1, 2]
+    from stopeps_jl_amd.config import bond_capacity, device_support
+    z2 = spp.RunConfig(model=spp.TFIM(1.0, 1.0, symmetric=True))
+    assert bond_capacity(z2, 6, (3, 3)) == (3, 3) and bond_capacity(z2, 4, None) == (4, 4)
+    with pytest.raises(NotImplementedError):
+        device_support(spp.RunConfig(), stochastic=True)  # resampling enabled by default
+    with pytest.raises(NotImplementedError):
+        device_support(spp.RunConfig(contraction="BoundaryMPSDiagonal"), stochastic=False)
