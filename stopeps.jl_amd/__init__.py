@@ -13,7 +13,7 @@ from .sampling import (Context, DeviceState, SamplingCache, BornError, sample_co
                        sample_configurations, sample_uniforms, project_configuration, splitmix64)
 from .measurement import (TFIM, spin_operators, operator_factors, EnergyError, measure_energies, measure_energy,
                           measure_energy_states)
-from .contraction import NormError, lognorm, lognorms, lognorm_gradient, lognorm_gradients
+from .contraction import NormError, lognorm, lognorms, lognorm_gradient, lognorm_gradients, lognorm_rrule
 from .evolution import FullUpdate, trotter_gate, evolution_step, evolution_steps
 from .particles import (Particle, ensemble_statistics, step_schedule, purify, stochastic_ensemble, new_particles,
                         advance_particles, statistics, group_configurations)
@@ -26,7 +26,7 @@ __all__ = ["synthetic_thermal_pepo", "random_peps", "Lattice", "Site", "FiniteSt
            "SamplingCache", "BornError", "sample_configuration", "sample_configurations", "sample_uniforms",
            "project_configuration", "splitmix64", "TFIM", "spin_operators", "operator_factors", "EnergyError",
            "measure_energies", "measure_energy", "measure_energy_states", "NormError", "lognorm", "lognorms",
-           "lognorm_gradient", "lognorm_gradients", "pad_bonds", "FullUpdate", "trotter_gate",
+           "lognorm_gradient", "lognorm_gradients", "lognorm_rrule", "pad_bonds", "FullUpdate", "trotter_gate",
            "evolution_step", "evolution_steps", "Particle", "ensemble_statistics", "temperature_nodes",
            "step_schedule", "purify", "stochastic_ensemble", "new_particles", "advance_particles", "statistics",
            "group_configurations", "normalize_state", "reembed_bonds", "RunConfig", "ResamplingOptions",

@@ -91,3 +91,16 @@ def lognorm_gradient(ctx, state, chi):
     """One state: (grads dict (x, y) -> array, log_norm)."""
     grads, info = lognorm_gradients(ctx, [state], chi)
     return grads[0], float(info["log_norm"][0])
+
+
+def lognorm_rrule(ctx, state, chi):
+    """Reverse rule of ``lognorm(state, BoundaryMPS(chi))`` in the ChainRules
+    form: returns (log_norm, pullback) with pullback(dy) -> {(x, y): dy * grad}
+    the cotangent of every site tensor. One forward-and-backward sweep
+    (``spp_lognorm_grad_batch``) serves both; the gradient holds the boundary
+    truncations fixed, so it is exact once chi reaches the exact boundary bond."""
+    grads, log_norm = lognorm_gradient(ctx, state, chi)
+
+    def pullback(dy):
+        return {xy: float(dy) * g for xy, g in grads.items()}
+    return log_norm, pullback

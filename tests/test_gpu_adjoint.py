@@ -124,6 +124,20 @@ def test_gradient_product_state_closed_form(spp, gpu_ctx):
     assert abs(ln - 6 * np.log(2.0)) < 1e-12
 
 
+def test_lognorm_rrule_pullback(spp, gpu_ctx):
+    """The ChainRules form: the value is lognorm and pullback(dy) scales the
+    gradient (identity PEPO closed form: grad_i = 2 A_i / |A_i|^2)."""
+    say("lognorm rrule pullback")
+    L = Lattice(3, 2)
+    ostate = identity_pepo(L, symmetric=True)
+    y, pullback = spp.lognorm_rrule(gpu_ctx, to_product_state(spp, ostate), 4)
+    assert abs(y - 6 * np.log(2.0)) < 1e-12
+    ct = pullback(-0.5)
+    for xy in L.site_order():
+        a = ostate[xy].a
+        np.testing.assert_allclose(ct[xy], -0.5 * 2.0 * a / np.sum(a * a), atol=1e-12)
+
+
 def test_gradient_empty_batch(spp, gpu_ctx):
     say("lognorm gradient empty batch")
     grads, info = spp.lognorm_gradients(gpu_ctx, [], 4)
