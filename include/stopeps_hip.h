@@ -106,12 +106,15 @@ int spp_state_download(spp_ctx* ctx, const spp_state* state, double* const* data
  * iterations, iteration_limit_reached, stop (0 disabled, 1 iteration limit,
  * 2 error tolerance, 3 residual fallback), kept_even, kept_odd};
  * normalizations [n][Lx*Ly] (`normalize_state!` log norms, added to each
- * state's log_amplitude); max_env_trunc [n]. The PSD repair of the bond metric
+ * state's log_amplitude); max_env_trunc [n]; max_boundary_entropy [n] the
+ * largest pre-truncation Schmidt entropy (nats) over every boundary cut of the
+ * step (`record_entropy=true`, full_update.jl:560-591; NULL: not recorded).
+ * The PSD repair of the bond metric
  * (full_update.jl:177-330) is not available: a bond that needs it fails the
  * call with the reference's message. */
 int spp_fu_evolution_step(spp_ctx* ctx, int n, spp_state* const* states, int chi, int order, double delta, int D,
                           const double* options, const double* gates_h, const double* gates_v, double* records,
-                          double* normalizations, double* max_env_trunc);
+                          double* normalizations, double* max_env_trunc, double* max_boundary_entropy);
 
 /* `SamplingCache(state, BoundaryMPS(chi); kind=:auxiliary, basis)`
  * (src/sampling/born.jl:56-73): double layers, aux-projected candidate layers
@@ -161,11 +164,19 @@ int spp_born_sample_batch(spp_ctx* ctx, spp_cache* cache, int n, const double* u
  *                      (boundary_mps.jl:258), 4 zero local norm environment
  *                      and 5 nonreal/nonfinite observable (energy.jl:1-6),
  *                      6 norm not finite and positive (boundary_mps.jl:188);
- *  fail_site [n]       row-major site of the failing term, or -1. */
+ *  fail_site [n]       row-major site of the failing term, or -1;
+ *  boundary_entropy [n][2][Ly-1][Lx-1]  `Se` of `record_entropy=true`
+ *                      (boundary_mps.jl:20-57, 193-209): Se.north then
+ *                      Se.south [row_cut][bond], pre-truncation Schmidt
+ *                      entropy in nats, NaN where undefined (NULL: not
+ *                      recorded);
+ *  max_boundary_entropy [n]  its maximum over every cut of the north and
+ *                      south boundaries (NULL: not recorded). */
 int spp_measure_energy_batch(spp_ctx* ctx, const spp_cache* cache, const spp_state* pepo, int chi, int n,
                              const int* labels, const double* X, const double* F, const double* G, int factor_odd,
                              double J, double h, double* energy_total, double* energy_per_site, double* log_norm,
-                             double* tensor_log_norm, double* max_trunc, int* status, int* fail_site);
+                             double* tensor_log_norm, double* max_trunc, int* status, int* fail_site,
+                             double* boundary_entropy, double* max_boundary_entropy);
 
 /* Batched `measure_energy(state, model, BoundaryMPS(chi))`
  * (src/measurement/energy.jl:31-78) of n uploaded states (PEPS or PEPO, all on
@@ -178,7 +189,8 @@ int spp_measure_energy_batch(spp_ctx* ctx, const spp_cache* cache, const spp_sta
 int spp_measure_energy_states(spp_ctx* ctx, int n, const spp_state* const* states, int chi, const double* X,
                               const double* F, const double* G, int factor_odd, double J, double h,
                               double* energy_total, double* energy_per_site, double* log_norm,
-                              double* tensor_log_norm, double* max_trunc, int* status, int* fail_site);
+                              double* tensor_log_norm, double* max_trunc, int* status, int* fail_site,
+                              double* boundary_entropy, double* max_boundary_entropy);
 
 /* Batched `lognorm(state, BoundaryMPS(chi))` (src/contraction/boundary_mps.jl:162-191)
  * of n uploaded states (PEPS or PEPO, all on the same lattice with the same
@@ -250,6 +262,14 @@ int spp_selftest_fu_pair(int horizontal, const int* dims_a, const int* evens_a, 
  * transpose [K][M]; bit 1: B is given as its transpose [N][K]. */
 int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, int evenM, int evenN, int evenK,
                       int layout, const double* A, const double* B, double* C);
+
+/* Self-test of the boundary-entropy eigensolver (no reference counterpart;
+ * the reference reads the spectrum off its compact SVD, boundary_mps.jl:33-43):
+ * all eigenvalues, ascending, of nprob symmetric blocks[nprob][ld][ld]
+ * (row-major; only the leading n[i] x n[i] part of block i is used and its
+ * two triangles are averaged) into eigenvalues[nprob][ld] (entries past n[i]
+ * are 0). */
+int spp_selftest_spectrum(spp_ctx* ctx, int nprob, int ld, const int* n, const double* blocks, double* eigenvalues);
 
 /* Calibration copy (no reference counterpart): one launch of the library's
  * plain copy kernel moving n FP32 values between two device buffers it

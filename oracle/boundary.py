@@ -67,6 +67,17 @@ def schmidt_entropy(spectra):
     return max(0.0, float(-np.sum(p * np.log(p))))
 
 
+def record_entropy(entropy, side, row_cut, bond, value):
+    """``_record_entropy!`` (boundary_mps.jl:50-56): ``entropy`` is a dict with
+    "maximum" and, when a profile is kept, "north"/"south" arrays
+    [Ly-1, Lx-1]. Julia's ``max`` propagates NaN."""
+    m = entropy["maximum"]
+    entropy["maximum"] = float("nan") if (np.isnan(m) or np.isnan(value)) else max(m, value)
+    prof = entropy.get(side)
+    if prof is not None and 1 <= row_cut <= prof.shape[0]:
+        prof[row_cut - 1, bond - 1] = value
+
+
 def compress_boundary(B, chi, reverse=False, observer=None):
     """``compress_boundary!`` (boundary_mps.jl:94-134). Modifies the list ``B``."""
     if chi < 1:
@@ -171,12 +182,7 @@ def boundary_environments(layers, chi, side="both", entropy=None):
     def observer_for(s, row_cut):
         if entropy is None:
             return None
-
-        def obs(bond, value):
-            entropy["maximum"] = max(entropy["maximum"], value)
-            if 1 <= row_cut <= Ly - 1:
-                entropy[s][row_cut - 1, bond - 1] = value
-        return obs
+        return lambda bond, value: record_entropy(entropy, s, row_cut, bond, value)
 
     if side != "south":
         for y in range(Ly):

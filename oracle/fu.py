@@ -12,6 +12,7 @@ import numpy as np
 from .z2 import Z2, EPS, ein, permute, left_orth, right_orth, tsvd, eigh_blocks, as_matrix, fuse
 from .boundary import (act, double_layer, double_layers, boundary_environments, absorb, row_environments,
                        left_step, right_step, safe_scale, Slab, row)
+from .boundary import record_entropy as _record_entropy
 from .model import normalize_state
 
 RESIDUAL_ROUNDOFF_TOLERANCE = 1e-8  # full_update.jl:18
@@ -399,12 +400,20 @@ def update_bond(state, i, j, gate, D, caps, opts, env, direction):
 # sweeps and the evolution step (full_update.jl:510-597)
 # ---------------------------------------------------------------------------
 
-def sweep(state, model, delta, D, caps, opts, chi, direction, reverse):
-    """``_sweep!`` (full_update.jl:510-561) for the row boundary-MPS contraction."""
+def sweep(state, model, delta, D, caps, opts, chi, direction, reverse, entropy=None):
+    """``_sweep!`` (full_update.jl:510-561) for the row boundary-MPS contraction.
+    ``entropy``: the ``BoundaryEntropy`` dict ({"maximum": ...}) every absorb
+    reports to, or None."""
     L = state.lattice
     Lx, Ly = L.Lx, L.Ly
     layers = double_layers(state)
-    north, south, env_err = boundary_environments(layers, chi, side="north" if reverse else "south")
+    north, south, env_err = boundary_environments(layers, chi, side="north" if reverse else "south",
+                                                  entropy=entropy)
+
+    def obs(side, row_cut):
+        if entropy is None:
+            return None
+        return lambda bond, value: _record_entropy(entropy, side, row_cut, bond, value)
     ymax = Ly if direction == "horizontal" else Ly - 1
     xmax = Lx - 1 if direction == "horizontal" else Lx
     records = []
@@ -443,19 +452,20 @@ def sweep(state, model, delta, D, caps, opts, chi, direction, reverse):
                 else:
                     rolling = safe_scale(left_step(rolling, top[column], E, bottom[column]))
         if reverse:
-            south[lastrow] = absorb(south[lastrow + 1], row(layers, lastrow), chi, "south")
+            south[lastrow] = absorb(south[lastrow + 1], row(layers, lastrow), chi, "south", obs("south", lastrow - 1))
             env_err = max(env_err, south[lastrow].max_truncation)
         else:
-            north[y + 1] = absorb(north[y], row(layers, y), chi, "north")
+            north[y + 1] = absorb(north[y], row(layers, y), chi, "north", obs("north", y + 1))
             env_err = max(env_err, north[y + 1].max_truncation)
     return records, env_err
 
 
-def evolution_step(state, model, delta, D, caps, opts, chi, order=2):
+def evolution_step(state, model, delta, D, caps, opts, chi, order=2, record_entropy=False):
     """``evolution_step!`` (full_update.jl:563-597) with FullUpdate and
     BoundaryMPS(chi). ``caps`` = (even, odd) sector budget of the bond space, or None."""
     records = []
     env_err = 0.0
+    entropy = {"maximum": 0.0} if record_entropy else None
     gate_delta = delta / 2 if order == 2 else delta
     sweeps = ((("horizontal", False), ("vertical", False), ("vertical", True), ("horizontal", True))
               if order == 2 else (("horizontal", False), ("vertical", False)))
@@ -466,11 +476,12 @@ def evolution_step(state, model, delta, D, caps, opts, chi, order=2):
         for direction, reverse in sweeps:
             if not L.bonds(direction):
                 continue
-            r, e = sweep(state, model, gate_delta, D, caps, opts, chi, direction, reverse)
+            r, e = sweep(state, model, gate_delta, D, caps, opts, chi, direction, reverse, entropy)
             records += r
             env_err = max(env_err, e)
     state.beta += 2 * delta
     norms = normalize_state(state)
     return {"beta": state.beta, "records": records, "normalizations": norms,
             "max_environment_truncation": env_err,
+            "max_boundary_entropy": float("nan") if entropy is None else entropy["maximum"],
             "max_state_truncation": max((r["final_cost"] for r in records), default=0.0)}

@@ -34,11 +34,11 @@ _SIGNATURES = {
     "spp_measure_energy_batch": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, c_int_p,
                                                 c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_double,
                                                 ctypes.c_double, c_double_p, c_double_p, c_double_p, c_double_p,
-                                                c_double_p, c_int_p, c_int_p]),
+                                                c_double_p, c_int_p, c_int_p, c_double_p, c_double_p]),
     "spp_measure_energy_states": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.POINTER(c_void_p), ctypes.c_int,
                                                  c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_double,
                                                  ctypes.c_double, c_double_p, c_double_p, c_double_p, c_double_p,
-                                                 c_double_p, c_int_p, c_int_p]),
+                                                 c_double_p, c_int_p, c_int_p, c_double_p, c_double_p]),
     "spp_lognorm_batch": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.POINTER(c_void_p), ctypes.c_int,
                                          c_double_p, c_double_p, c_double_p, c_int_p]),
     "spp_lognorm_grad_batch": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.POINTER(c_void_p), ctypes.c_int,
@@ -53,11 +53,12 @@ _SIGNATURES = {
                                           c_int_p]),
     "spp_fu_evolution_step": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.POINTER(c_void_p), ctypes.c_int,
                                              ctypes.c_int, ctypes.c_double, ctypes.c_int, c_double_p, c_double_p,
-                                             c_double_p, c_double_p, c_double_p, c_double_p]),
+                                             c_double_p, c_double_p, c_double_p, c_double_p, c_double_p]),
     "spp_selftest_fu_pair": (ctypes.c_int, [ctypes.c_int, c_int_p, c_int_p, c_int_p, c_int_p, c_double_p, c_double_p,
                                             c_int_p, c_int_p, c_double_p, c_double_p, c_double_p, c_double_p,
                                             c_double_p, c_double_p, c_int_p]),
     "spp_selftest_copy": (ctypes.c_int, [c_void_p, ctypes.c_longlong]),
+    "spp_selftest_spectrum": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, c_int_p, c_double_p, c_double_p]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -205,6 +205,9 @@ def parameter_dict(c):
     data["full_update_linear_system"] = "environment_and_internal_gauge_coordinates; relative_diagonal_shift_and_CG_residual"
     data["full_update_internal_gauge"] = "QR_after_left_solve; LQ_after_right_solve; best_center_SVD_without_truncation"
     data["environment_truncation_metric"] = "canonical_SVD_relative_discarded_weight"
-    data["boundary_entropy_convention"] = "not_recorded"
+    data["boundary_entropy_convention"] = "pretruncation_Schmidt_entropy; nats; sector_dimension_weighted"
+    data["boundary_entropy_log_scope"] = "maximum_over_FU_boundary_cuts_in_output_interval"
+    data["boundary_entropy_profile_scope"] = ("energy_measurement_boundaries; Se.north[y,b] and Se.south[y,b]; "
+                                              "cut_after_row_y_and_column_b")
     data["sampling_rng"] = "counter_splitmix64(seed, sample_index, site)"
     return data

@@ -227,7 +227,7 @@ int spp_state_download(spp_ctx* ctx, const spp_state* state, double* const* data
 
 int spp_fu_evolution_step(spp_ctx* ctx, int n, spp_state* const* states, int chi, int order, double delta, int D,
                           const double* options, const double* gates_h, const double* gates_v, double* records,
-                          double* normalizations, double* max_env_trunc) {
+                          double* normalizations, double* max_env_trunc, double* max_boundary_entropy) {
   return guarded(ctx, [&] {
     if (n < 0) throw HipError("negative state count");
     if (n > 0 && !states) throw HipError("null state list");
@@ -250,6 +250,7 @@ int spp_fu_evolution_step(spp_ctx* ctx, int n, spp_state* const* states, int chi
     o.records = records;
     o.normalizations = normalizations;
     o.max_env_trunc = max_env_trunc;
+    o.max_entropy = max_boundary_entropy;
     if (ctx->c.precision == 1) fu_evolution_step<float>(ctx->c, v, chi, P, o);
     else fu_evolution_step<double>(ctx->c, v, chi, P, o);
   });
@@ -319,7 +320,8 @@ int spp_born_sample_batch(spp_ctx* ctx, spp_cache* cache, int n, const double* u
 int spp_measure_energy_batch(spp_ctx* ctx, const spp_cache* cache, const spp_state* pepo, int chi, int n,
                              const int* labels, const double* X, const double* F, const double* G, int factor_odd,
                              double J, double h, double* energy_total, double* energy_per_site, double* log_norm,
-                             double* tensor_log_norm, double* max_trunc, int* status, int* fail_site) {
+                             double* tensor_log_norm, double* max_trunc, int* status, int* fail_site,
+                             double* boundary_entropy, double* max_boundary_entropy) {
   return guarded(ctx, [&] {
     if (!cache || !pepo) throw HipError("null cache or state");
     if (!X || !F || !G) throw HipError("null operator");
@@ -342,6 +344,8 @@ int spp_measure_energy_batch(spp_ctx* ctx, const spp_cache* cache, const spp_sta
     o.max_trunc = max_trunc;
     o.status = status;
     o.fail_site = fail_site;
+    o.entropy_profile = boundary_entropy;
+    o.entropy_max = max_boundary_entropy;
     if (cache->precision == 1) measure_energy_batch<float>(ctx->c, pepo->s, *cache->f, chi, n, labels, ops, o);
     else measure_energy_batch<double>(ctx->c, pepo->s, *cache->d, chi, n, labels, ops, o);
   });
@@ -350,7 +354,8 @@ int spp_measure_energy_batch(spp_ctx* ctx, const spp_cache* cache, const spp_sta
 int spp_measure_energy_states(spp_ctx* ctx, int n, const spp_state* const* states, int chi, const double* X,
                               const double* F, const double* G, int factor_odd, double J, double h,
                               double* energy_total, double* energy_per_site, double* log_norm,
-                              double* tensor_log_norm, double* max_trunc, int* status, int* fail_site) {
+                              double* tensor_log_norm, double* max_trunc, int* status, int* fail_site,
+                              double* boundary_entropy, double* max_boundary_entropy) {
   return guarded(ctx, [&] {
     if (n < 0) throw HipError("negative state count");
     if (n > 0 && !states) throw HipError("null state list");
@@ -377,6 +382,8 @@ int spp_measure_energy_states(spp_ctx* ctx, int n, const spp_state* const* state
     o.max_trunc = max_trunc;
     o.status = status;
     o.fail_site = fail_site;
+    o.entropy_profile = boundary_entropy;
+    o.entropy_max = max_boundary_entropy;
     if (ctx->c.precision == 1) measure_energy_states<float>(ctx->c, v, chi, ops, o);
     else measure_energy_states<double>(ctx->c, v, chi, ops, o);
   });
@@ -508,6 +515,14 @@ int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, int even
     (void)hipFree(dC);
     if (tA) (void)hipFree(tA);
     if (tB) (void)hipFree(tB);
+  });
+}
+
+int spp_selftest_spectrum(spp_ctx* ctx, int nprob, int ld, const int* n, const double* blocks, double* eigenvalues) {
+  return guarded(ctx, [&] {
+    if (!n || !blocks || !eigenvalues) throw HipError("null argument");
+    SPP_CHECK(hipSetDevice(ctx->c.device));
+    spectrum_selftest(ctx->c, nprob, ld, n, blocks, eigenvalues);
   });
 }
 

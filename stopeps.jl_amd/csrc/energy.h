@@ -29,6 +29,11 @@ struct EnergyOutputs {
   // 5 nonfinite observable, 6 norm not finite and positive
   int* status = nullptr;
   int* fail_site = nullptr;           // [n] row-major site of the failure, or -1
+  // Boundary entropy (record_entropy, energy.jl:31-78; nullptr: not recorded):
+  // [n][2][Ly-1][Lx-1] Se.north then Se.south (NaN where undefined) and [n]
+  // the maximum over every cut of the south and north absorbs.
+  double* entropy_profile = nullptr;
+  double* entropy_max = nullptr;
 };
 
 // Energies of the PEPS samples obtained by projecting the auxiliary legs of

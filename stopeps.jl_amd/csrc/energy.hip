@@ -135,6 +135,65 @@ LayerJob impurity_job(const SiteDev& sd, int kind, const EnergyOps& ops, void* o
   return J;
 }
 
+// Boundary-entropy observer of a measurement chunk (boundary_environments
+// with an entropy argument, boundary_mps.jl:193-209, as measure_energy runs it,
+// energy.jl:31-78): the per-cut entropies of every south and north absorb and
+// their running maximum. On the host the cuts map to Se.north / Se.south
+// [Ly-1][Lx-1]: the north absorb of row y (0-based) is row_cut y+1, its cut
+// after site x is bond x+1; the south absorb of row y is row_cut y, and its
+// rotated cut after site xr is bond Lx-1-xr. Rows outside 1..Ly-1 only feed
+// the maximum.
+struct EntropyObserver {
+  int Lx = 0, Ly = 0, nb = 0, cur = 0;
+  double* cuts = nullptr;  // device [2][Ly][Lx-1][nb]: south absorbs, then north absorbs, by row
+  double* emax = nullptr;  // device [nb]
+  std::vector<double> hc, hm;
+  bool on() const { return emax != nullptr; }
+  template <typename Dev>
+  void init(int lx, int ly, int nbb, Dev&& dev) {
+    Lx = lx;
+    Ly = ly;
+    nb = nbb;
+    if (Lx > 1) cuts = (double*)dev(sizeof(double) * 2 * (size_t)Ly * (Lx - 1) * nb);
+    emax = (double*)dev(sizeof(double) * (size_t)nb);
+    hc.assign(Lx > 1 ? 2 * (size_t)Ly * (Lx - 1) * nb : 0, 0.0);
+    hm.assign(nb, 0.0);
+  }
+  // the absorb into b of row y on side 0 (south) or 1 (north)
+  template <typename B>
+  void attach(B& b, int side, int y) const {
+    if (!on()) return;
+    b.entropy = cuts ? cuts + ((size_t)side * Ly + y) * (size_t)(Lx - 1) * nb : nullptr;
+    b.entropy_max = emax;
+  }
+  void start(int c, hipStream_t st) {
+    cur = c;
+    if (on()) SPP_CHECK(launch_fill<double>(emax, nb, 0.0, st));
+  }
+  void fetch(hipStream_t st) {
+    if (!on()) return;
+    if (cuts) SPP_CHECK(hipMemcpyAsync(hc.data(), cuts, sizeof(double) * hc.size(), hipMemcpyDeviceToHost, st));
+    SPP_CHECK(hipMemcpyAsync(hm.data(), emax, sizeof(double) * (size_t)nb, hipMemcpyDeviceToHost, st));
+  }
+  // after fetch and a stream synchronisation: sample s of the chunk is output g
+  void write(int s, long long g, double* profile, double* maximum) const {
+    if (!on()) return;
+    const int C = Lx - 1, R = Ly - 1;
+    if (profile && C > 0 && R > 0) {
+      double* P = profile + (size_t)g * 2 * R * C;
+      for (int i = 0; i < 2 * R * C; ++i) P[i] = NAN;
+      auto at = [&](int side, int y, int cut) {
+        return hc[((size_t)side * Ly + y) * (size_t)C * nb + (size_t)cut * cur + s];
+      };
+      for (int y = 0; y + 1 <= R; ++y)
+        for (int x = 0; x < C; ++x) P[(size_t)y * C + x] = at(1, y, x);
+      for (int y = 1; y <= R; ++y)
+        for (int xr = 0; xr < C; ++xr) P[(size_t)R * C + (size_t)(y - 1) * C + (C - 1 - xr)] = at(0, y, xr);
+    }
+    if (maximum) maximum[g] = hm[s];
+  }
+};
+
 template <typename T>
 struct EnergyRun {
   Context& ctx;
@@ -448,6 +507,8 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
     for (int y = 0; y <= Ly; ++y) prepare_boundary(south[y], Lx, nb, capS);
     prepare_boundary(NB[0], Lx, nb, capS);
     prepare_boundary(NB[1], Lx, nb, capS);
+    EntropyObserver eobs;
+    if (out.entropy_profile || out.entropy_max) eobs.init(Lx, Ly, nb, dev);
     run.tables = (void**)dev(sizeof(void*) * (size_t)kKinds * ns * nb);
     dbase = (void**)dev(sizeof(void*) * base.size());
     dlabels = (int*)dev(sizeof(int) * (size_t)n * ns);
@@ -481,6 +542,7 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
       SPP_CHECK(launch_fill<int>(run.fail, nb, -1, st));
       SPP_CHECK(launch_fill<double>(run.xsum, nb, 0.0, st));
       SPP_CHECK(launch_fill<double>(run.zzsum, nb, 0.0, st));
+      eobs.start(cur, st);
       // south boundaries of the projected samples, rotated frame
       for (auto& b : south) b.nb = cur;
       std::vector<Leg> physS(Lx);
@@ -498,6 +560,7 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
           rr.spec[xr] = LayerSpec{sp.e, sp.s, sp.w, sp.n};
           rr.ptrs[xr] = run.table(kPlain, x + y * Lx);
         }
+        eobs.attach(south[y], 0, y);
         absorb_forward<T>(ctx, south[y + 1], rr, chi, south[y]);
       }
       // north sweep with the row and slab terms
@@ -516,6 +579,7 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
         row.spec.assign(c.spec.begin() + y * Lx, c.spec.begin() + (y + 1) * Lx);
         row.ptrs.resize(Lx);
         for (int x = 0; x < Lx; ++x) row.ptrs[x] = run.table(kPlain, x + y * Lx);
+        eobs.attach(NB[1 - which], 1, y);
         absorb_forward<T>(ctx, Nc, row, chi, NB[1 - which]);
         which = 1 - which;
       }
@@ -538,6 +602,7 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
       SPP_CHECK(hipMemcpyAsync(h_mtn.data(), Nf.max_trunc, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_mts.data(), south[0].max_trunc, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_amp.data(), lnamp, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      eobs.fetch(st);
       SPP_CHECK(hipStreamSynchronize(st));
       for (int s = 0; s < cur; ++s) {
         const long long g = s0 + s;
@@ -553,6 +618,7 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
         if (out.max_trunc) out.max_trunc[g] = std::max(h_mtn[s], h_mts[s]);
         if (out.status) out.status[g] = stt;
         if (out.fail_site) out.fail_site[g] = stt == 0 ? -1 : h_fail[s];
+        eobs.write(s, g, out.entropy_profile, out.entropy_max);
       }
     }
   } catch (...) {
@@ -660,6 +726,8 @@ void measure_energy_states(Context& ctx, const std::vector<const StateDev*>& sta
     for (int y = 0; y <= Ly; ++y) prepare_boundary(south[y], Lx, nb, capS);
     prepare_boundary(NB[0], Lx, nb, capS);
     prepare_boundary(NB[1], Lx, nb, capS);
+    EntropyObserver eobs;
+    if (out.entropy_profile || out.entropy_max) eobs.init(Lx, Ly, nb, dev);
     T* layers = (T*)dev(sizeof(T) * (size_t)layer_elems * nb);
     T* imp = (T*)dev(sizeof(T) * (size_t)std::max(imp_elems, 1LL) * nb);
     std::vector<T*> site_base(ns), imp_base((size_t)kKinds * Lx, nullptr);
@@ -720,6 +788,7 @@ void measure_energy_states(Context& ctx, const std::vector<const StateDev*>& sta
       SPP_CHECK(launch_fill<int>(run.fail, nb, -1, st));
       SPP_CHECK(launch_fill<double>(run.xsum, nb, 0.0, st));
       SPP_CHECK(launch_fill<double>(run.zzsum, nb, 0.0, st));
+      eobs.start(cur, st);
       // south boundaries, rotated frame
       for (auto& b : south) b.nb = cur;
       const std::vector<LayerSpec>& P = run.spec[kPlain];
@@ -738,6 +807,7 @@ void measure_energy_states(Context& ctx, const std::vector<const StateDev*>& sta
           rr.spec[xr] = LayerSpec{sp.e, sp.s, sp.w, sp.n};
           rr.ptrs[xr] = run.table(kPlain, x + y * Lx);
         }
+        eobs.attach(south[y], 0, y);
         absorb_forward<T>(ctx, south[y + 1], rr, chi, south[y]);
       }
       // north sweep with the row and slab terms
@@ -777,6 +847,7 @@ void measure_energy_states(Context& ctx, const std::vector<const StateDev*>& sta
         row.spec.assign(P.begin() + y * Lx, P.begin() + (y + 1) * Lx);
         row.ptrs.resize(Lx);
         for (int x = 0; x < Lx; ++x) row.ptrs[x] = run.table(kPlain, x + y * Lx);
+        eobs.attach(NB[1 - which], 1, y);
         absorb_forward<T>(ctx, Nc, row, chi, NB[1 - which]);
         which = 1 - which;
       }
@@ -797,6 +868,7 @@ void measure_energy_states(Context& ctx, const std::vector<const StateDev*>& sta
       SPP_CHECK(hipMemcpyAsync(h_ls.data(), Nf.log_scale, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_mtn.data(), Nf.max_trunc, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_mts.data(), south[0].max_trunc, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      eobs.fetch(st);
       SPP_CHECK(hipStreamSynchronize(st));
       for (int s = 0; s < cur; ++s) {
         const long long g = c0 + s;
@@ -812,6 +884,7 @@ void measure_energy_states(Context& ctx, const std::vector<const StateDev*>& sta
         if (out.max_trunc) out.max_trunc[g] = std::max(h_mtn[s], h_mts[s]);
         if (out.status) out.status[g] = stt;
         if (out.fail_site) out.fail_site[g] = stt == 0 ? -1 : h_fail[s];
+        eobs.write(s, g, out.entropy_profile, out.entropy_max);
       }
     }
   } catch (...) {

@@ -116,6 +116,32 @@ struct LaunchTimer {
   std::vector<int> pending_prec;  // 0: FP64 kernel, 1: FP32 kernel
 };
 
+// Cut spectra waiting for the batched eigenvalue solve of the boundary
+// entropy (compact rho blocks copied aside during an absorb).
+constexpr int kEntropyCuts = 16;
+struct EntropyTable {
+  double* mat[kEntropyCuts];      // [2 nb][ld][ld] blocks (overwritten)
+  double* w[kEntropyCuts];        // [2 nb][wstride] eigenvalues
+  const int* cnt[kEntropyCuts];   // [2 nb] block dimensions
+  long long wstride[kEntropyCuts];
+  int ld[kEntropyCuts];
+  int cut[kEntropyCuts];          // cut index (after site cut) in the absorb frame
+  int count;
+};
+struct EntropyWork {
+  char* buf = nullptr;
+  size_t cap = 0, used = 0;
+  struct Cut {
+    size_t mat, w, cnt;
+    int ld, cut;
+  };
+  std::vector<Cut> pending;
+  EntropyWork() = default;
+  EntropyWork(const EntropyWork&) = delete;
+  EntropyWork& operator=(const EntropyWork&) = delete;
+  ~EntropyWork();
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -131,8 +157,24 @@ struct Context {
   int last_iterations = 0;
   int profile = 0;          // 1: time every contraction launch with HIP events
   LaunchTimer timer;
+  EntropyWork entropy;      // boundary-entropy spectra of the current absorb
   void flush_timer();       // accumulate finished event pairs into counters
 };
+
+// Boundary entropy of one cut from eigenvalues already at hand (w [2 nb][wstride],
+// cnt [2 nb]); out[cut * nb + s] (if out) and the NaN-propagating running
+// maximum run_max [nb] (if run_max).
+void entropy_cut(Context& ctx, const double* w, long long wstride, const int* cnt, int nb, int cut, double* out,
+                 double* run_max);
+// Copy the compact blocks rho [2 nb][ld][ld] of one cut aside for the batched
+// solve (flushing first when the workspace is full).
+void entropy_defer(Context& ctx, const double* rho, long long rps, int ld, const int* cnt, int nb, int cut,
+                   int cuts_left, double* out, double* run_max);
+// Solve every deferred cut and write its entropies.
+void entropy_flush(Context& ctx, int nb, double* out, double* run_max);
+// All eigenvalues (ascending) of nprob symmetric blocks [nprob][ld][ld]
+// (host), of dimensions n_h[i] <= ld, into eig_h [nprob][ld] (host).
+void spectrum_selftest(Context& ctx, int nprob, int ld, const int* n_h, const double* mats_h, double* eig_h);
 
 // Contraction C = alpha * A.B (+ C): letters shared by A and B are summed.
 template <typename T>
@@ -153,6 +195,11 @@ struct Boundary {
   std::vector<T*> t;              // device tensors [nb][bdim[x]][pdim[x]][bdim[x+1]]
   double* log_scale = nullptr;    // device [nb]
   double* max_trunc = nullptr;    // device [nb]
+  // Boundary-entropy recording of absorbs into this boundary (not owned):
+  // entropy [L-1][nb] per cut of the last absorb, entropy_max [nb] running
+  // maximum; nullptr: not recorded.
+  double* entropy = nullptr;
+  double* entropy_max = nullptr;
   bool owns = true;
   Leg bond(int b) const {
     Leg l;

@@ -1363,6 +1363,13 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
                        er.vstride, er.ldv, er.avail, ncount, tr, chi, cap, U, us, (long long)sp.s.dim * cap,
                        (long long)cap, out.bsplit[x + 1], out.max_trunc, nullptr);
     SPP_CHECK(hipGetLastError());
+    if (out.entropy || out.entropy_max) {
+      // pre-truncation spectrum of this cut (eigenvalues of rho)
+      if (nmax <= ctx.ksub)
+        entropy_cut(ctx, er.w, er.wstride, ncount, nb, x, out.entropy, out.entropy_max);
+      else
+        entropy_defer(ctx, rho, rps, (int)nmax, ncount, nb, x, L - 1 - x, out.entropy, out.entropy_max);
+    }
     out.bdim[x] = kb.dim;
     out.beven[x] = kb.even;
     Leg nbond;
@@ -1378,6 +1385,7 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
     Rl = {nbond, br, sp.e};
     kb = nbond;
   }
+  if (out.entropy || out.entropy_max) entropy_flush(ctx, nb, out.entropy, out.entropy_max);
   ctx.arena.release(mark);
 }
 

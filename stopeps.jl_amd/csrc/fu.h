@@ -31,6 +31,9 @@ struct FuOutputs {
                                      // limit_reached, stop, kept_even, kept_odd (sweep order)
   double* normalizations = nullptr;  // [n][Lx*Ly]: log of each site norm divided out
   double* max_env_trunc = nullptr;   // [n]
+  // [n] maximum pre-truncation Schmidt entropy over every boundary cut of the
+  // step (record_entropy, full_update.jl:560-591); nullptr: not recorded
+  double* max_entropy = nullptr;
 };
 
 // `evolution_step!` (full_update.jl:563-597) with FullUpdate on every state

@@ -733,6 +733,10 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
     std::vector<double> hrec((size_t)12 * nb);
     std::vector<int> hstatus(nb);
     std::vector<double> hmt(nb);
+    // boundary entropy: every absorb of the step raises a per-sample maximum
+    double* emax = out.max_entropy ? (double*)dev(sizeof(double) * (size_t)nb) : nullptr;
+    for (auto* side : {&north, &south})
+      for (Boundary<T>& b : *side) b.entropy_max = emax;
     for (long long c0 = 0; c0 < n; c0 += nb) {
       const int cur = (int)std::min<long long>(nb, n - c0);
       // per-sample bookkeeping
@@ -744,6 +748,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
       }
       SPP_CHECK(launch_fill<int>(status, nb, 0, st));
       SPP_CHECK(launch_fill<int>(fail, nb, -1, st));
+      if (emax) SPP_CHECK(launch_fill<double>(emax, nb, 0.0, st));
       std::vector<double> env_err(cur, 0.0);
       auto relayer = [&](const std::vector<int>& sites) {
         std::vector<LayerJob> jobs;
@@ -1197,9 +1202,14 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
         for (int s = 0; s < cur; ++s)
           std::memcpy(states[c0 + s]->live.data(), hl.data() + (size_t)s * ns * 8, sizeof(int) * ns * 8);
       }
+      if (emax) {
+        SPP_CHECK(hipMemcpyAsync(hmt.data(), emax, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+        SPP_CHECK(hipStreamSynchronize(st));
+      }
       for (int s = 0; s < cur; ++s) {
         states[c0 + s]->beta += 2.0 * P.delta;
         if (out.max_env_trunc) out.max_env_trunc[c0 + s] = env_err[s];
+        if (emax) out.max_entropy[c0 + s] = hmt[s];
       }
     }
   } catch (...) {

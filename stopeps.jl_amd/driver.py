@@ -65,9 +65,10 @@ def run_purification(c, ctx=None):
     cap = bond_capacity(c, c.D_pur, c.bond_space_pur)
     history = [{"phase": "initialization", "beta": 0.0}]
     dev, rows = purify(ctx, pepo, c.model, c.beta0, c.beta_log, c.beta_measure, c.delta_tau_pur, c.chi_environment,
-                       c.chi_measurement, cap, c.update, c.trotter_order, D=c.D_pur)
+                       c.chi_measurement, cap, c.update, c.trotter_order, D=c.D_pur,
+                       record_entropy=c.record_boundary_entropy)
     for r in rows:
-        r.update({"eta": float("nan"), "eta_error": float("nan"), "max_boundary_entropy": float("nan"), "Se": None})
+        r.update({"eta": float("nan"), "eta_error": float("nan")})
         out["observables"]["purification"].append(r)
         out["observables"]["curve"].append(r)
     out["diagnostics"]["history"] = history
@@ -116,7 +117,7 @@ def run_stochastic(c, checkpoint, ctx=None, uniforms=None):
                                        c.Ns, c.seed, c.chi_sampling, c.chi_environment, c.chi_measurement,
                                        bond_space=cap, method=c.update, order=c.trotter_order,
                                        basis=sampling_basis(c), uniforms=uniforms, on_node=on_node, D=c.D_peps,
-                                       draws_out=draws)
+                                       draws_out=draws, record_entropy=c.record_boundary_entropy)
     out["diagnostics"]["born_draws"] = [
         {"index": i + 1, "configuration": draws["configuration"][i], "log_probability": float(draws["log_probability"][i]),
          "max_environment_truncation": float(draws["max_environment_truncation"][i])}

@@ -69,22 +69,24 @@ def _gates(model, lattice, delta, ppar):
     return gh, gv
 
 
-def evolution_steps(ctx, states, model, delta, chi, method=None, order=2, D=None):
+def evolution_steps(ctx, states, model, delta, chi, method=None, order=2, D=None, record_entropy=False):
     """``evolution_step!(state, model, delta, D, FullUpdate(...), BoundaryMPS(chi);
-    order, sector_space)`` for every uploaded state (``DeviceState``; same lattice
+    order, sector_space, record_entropy)`` for every uploaded state (``DeviceState``; same lattice
     and leg spaces up to the auxiliary charge), in place on the device.
     ``sector_space`` is the states' uploaded bond space (the per-parity
     capacity); ``D`` the truncrank of every bond (None: the capacity's
     dimension). Returns a dict: records [n, updates, 8]
     (initial_cost, final_cost, residual, iterations, iteration_limit_reached,
     stop, kept_even, kept_odd), normalizations [n, Lx*Ly],
-    max_environment_truncation [n]."""
+    max_environment_truncation [n], max_boundary_entropy [n] (the largest
+    pre-truncation Schmidt entropy over every boundary cut of the step; NaN
+    unless ``record_entropy``)."""
     method = method or FullUpdate()
     devs = list(states)
     n = len(devs)
     if n == 0:
         return {"records": np.zeros((0, 0, 8)), "normalizations": np.zeros((0, 0)),
-                "max_environment_truncation": np.zeros(0)}
+                "max_environment_truncation": np.zeros(0), "max_boundary_entropy": np.zeros(0)}
     L = devs[0].state.lattice
     if any(not isinstance(d, DeviceState) for d in devs):
         raise TypeError("evolution_steps takes DeviceState objects (the states evolve in place on the device)")
@@ -98,23 +100,26 @@ def evolution_steps(ctx, states, model, delta, chi, method=None, order=2, D=None
     rec = np.zeros((n, nup, 8))
     norms = np.zeros((n, L.Lx * L.Ly))
     mt = np.zeros(n)
+    ent = np.zeros(n) if record_entropy else None
     handles = (ctypes.c_void_p * n)(*[d.handle.value for d in devs])
     o = method.options()
     _lib.check(ctx.handle, ctx._lib.spp_fu_evolution_step(
         ctx.handle, n, handles, int(chi), int(order), float(delta), int(D or 0), _lib.dptr(o), _lib.dptr(gh),
         _lib.dptr(gv),
-        _lib.dptr(rec), _lib.dptr(norms), _lib.dptr(mt)))
+        _lib.dptr(rec), _lib.dptr(norms), _lib.dptr(mt), _lib.dptr(ent)))
     for d in devs:
         d.state.beta += 2 * delta
-    return {"records": rec, "normalizations": norms, "max_environment_truncation": mt}
+    return {"records": rec, "normalizations": norms, "max_environment_truncation": mt,
+            "max_boundary_entropy": ent if ent is not None else np.full(n, np.nan)}
 
 
-def evolution_step(ctx, state, model, delta, chi, method=None, order=2, D=None):
+def evolution_step(ctx, state, model, delta, chi, method=None, order=2, D=None, record_entropy=False):
     """One state: the report of :func:`evolution_steps` for it."""
-    r = evolution_steps(ctx, [state], model, delta, chi, method, order, D)
+    r = evolution_steps(ctx, [state], model, delta, chi, method, order, D, record_entropy)
     recs = r["records"][0]
     return {"records": [{"initial_cost": float(v[0]), "final_cost": float(v[1]), "residual": float(v[2]),
                          "iterations": int(v[3]), "iteration_limit_reached": bool(v[4]),
                          "stop_reason": STOP_REASONS[int(v[5])], "kept": (int(v[6]), int(v[7]))} for v in recs],
             "normalizations": r["normalizations"][0], "max_environment_truncation": float(r["max_environment_truncation"][0]),
+            "max_boundary_entropy": float(r["max_boundary_entropy"][0]),
             "max_state_truncation": float(np.max(recs[:, 1])) if len(recs) else 0.0}

@@ -95,12 +95,32 @@ def _device_operators(model, physical_parity):
     return np.ascontiguousarray(X), np.ascontiguousarray(F), np.ascontiguousarray(G), odd
 
 
-def measure_energies(cache, configurations, model, chi=None, check=True):
+def _entropy_buffers(lattice, n, record_entropy):
+    if not record_entropy:
+        return None, None
+    return (np.full((n, 2, lattice.Ly - 1, lattice.Lx - 1), np.nan), np.full(n, np.nan))
+
+
+def _entropy_outputs(out, prof, emax):
+    """``Se`` (north, south [Ly-1, Lx-1] arrays indexed [row_cut-1, bond-1],
+    NaN where undefined) and ``max_boundary_entropy`` per sample."""
+    if prof is None:
+        out["Se"] = [None] * len(out["energy_total"])
+        out["max_boundary_entropy"] = np.full(len(out["energy_total"]), np.nan)
+    else:
+        out["Se"] = [{"north": prof[i, 0], "south": prof[i, 1]} for i in range(prof.shape[0])]
+        out["max_boundary_entropy"] = emax
+
+
+def measure_energies(cache, configurations, model, chi=None, check=True, record_entropy=False):
     """Batched ``measure_energy(project_configuration(pepo, c), model,
-    BoundaryMPS(chi))`` for configurations [n, Lx, Ly] of basis labels.
-    ``chi`` defaults to the cache's (sampling) chi. Returns a dict of arrays:
-    energy_total, energy_per_site, log_norm, tensor_log_norm,
-    max_environment_truncation, status, fail_site."""
+    BoundaryMPS(chi); record_entropy)`` for configurations [n, Lx, Ly] of basis
+    labels. ``chi`` defaults to the cache's (sampling) chi. Returns a dict of
+    arrays: energy_total, energy_per_site, log_norm, tensor_log_norm,
+    max_environment_truncation, status, fail_site, and per sample ``Se``
+    ({"north", "south"} pre-truncation Schmidt entropies [Ly-1, Lx-1], or None)
+    and max_boundary_entropy (the maximum over every boundary cut; NaN when
+    not recorded)."""
     L = cache.lattice
     ns = L.Lx * L.Ly
     cfg = np.asarray(configurations)
@@ -114,31 +134,34 @@ def measure_energies(cache, configurations, model, chi=None, check=True):
                                             "max_environment_truncation")}
     status = np.zeros(n, dtype=np.int32)
     fail = np.full(n, -1, dtype=np.int32)
+    prof, emax = _entropy_buffers(L, n, record_entropy)
     ctx = cache.ctx
     _lib.check(ctx.handle, ctx._lib.spp_measure_energy_batch(
         ctx.handle, cache.handle, cache._dev.handle, int(chi if chi is not None else cache.chi), n,
         _lib.iptr(labels), _lib.dptr(X), _lib.dptr(F), _lib.dptr(G), odd, model.J, model.h,
         _lib.dptr(out["energy_total"]), _lib.dptr(out["energy_per_site"]), _lib.dptr(out["log_norm"]),
         _lib.dptr(out["tensor_log_norm"]), _lib.dptr(out["max_environment_truncation"]), _lib.iptr(status),
-        _lib.iptr(fail)))
+        _lib.iptr(fail), _lib.dptr(prof), _lib.dptr(emax)))
     if check:
         _raise_status(L, status, fail)
     out["status"] = status
     out["fail_site"] = fail
+    _entropy_outputs(out, prof, emax)
     return out
 
 
-def measure_energy(cache, configuration, model, chi=None):
+def measure_energy(cache, configuration, model, chi=None, record_entropy=False):
     """One sample: the NamedTuple fields of ``measure_energy`` (energy.jl:73-77)
-    as a dict (``Se`` is None: entropy recording is not offered on the device)."""
-    r = measure_energies(cache, np.asarray(configuration)[None], model, chi)
+    as a dict (``Se`` is None unless ``record_entropy``)."""
+    r = measure_energies(cache, np.asarray(configuration)[None], model, chi, record_entropy=record_entropy)
     return {"energy_total": float(r["energy_total"][0]), "energy_per_site": float(r["energy_per_site"][0]),
             "log_norm": float(r["log_norm"][0]), "tensor_log_norm": float(r["tensor_log_norm"][0]),
-            "max_environment_truncation": float(r["max_environment_truncation"][0]), "Se": None}
+            "max_environment_truncation": float(r["max_environment_truncation"][0]), "Se": r["Se"][0],
+            "max_boundary_entropy": float(r["max_boundary_entropy"][0])}
 
 
-def measure_energy_states(ctx, states, model, chi, check=True):
-    """Batched ``measure_energy(state, model, BoundaryMPS(chi))``
+def measure_energy_states(ctx, states, model, chi, check=True, record_entropy=False):
+    """Batched ``measure_energy(state, model, BoundaryMPS(chi); record_entropy)``
     (energy.jl:31-78) of resident states (``DeviceState``, or host states that
     are uploaded for the call): PEPS or PEPO on one lattice with the same leg
     spaces, e.g. the particles after ``_evolve!`` (particles.jl:86-95). Returns
@@ -154,16 +177,20 @@ def measure_energy_states(ctx, states, model, chi, check=True):
     fail = np.full(n, -1, dtype=np.int32)
     if n == 0:
         out["status"], out["fail_site"] = status, fail
+        _entropy_outputs(out, None, None)
         return out
+    L = devs[0].state.lattice
+    prof, emax = _entropy_buffers(L, n, record_entropy)
     X, F, G, odd = _device_operators(model, devs[0].state[(0, 0)].parity[2])
     handles = (ctypes.c_void_p * n)(*[d.handle.value for d in devs])
     _lib.check(ctx.handle, ctx._lib.spp_measure_energy_states(
         ctx.handle, n, handles, int(chi), _lib.dptr(X), _lib.dptr(F), _lib.dptr(G), odd, model.J, model.h,
         _lib.dptr(out["energy_total"]), _lib.dptr(out["energy_per_site"]), _lib.dptr(out["log_norm"]),
         _lib.dptr(out["tensor_log_norm"]), _lib.dptr(out["max_environment_truncation"]), _lib.iptr(status),
-        _lib.iptr(fail)))
+        _lib.iptr(fail), _lib.dptr(prof), _lib.dptr(emax)))
     if check:
-        _raise_status(devs[0].state.lattice, status, fail)
+        _raise_status(L, status, fail)
     out["status"] = status
     out["fail_site"] = fail
+    _entropy_outputs(out, prof, emax)
     return out

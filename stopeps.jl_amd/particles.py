@@ -137,7 +137,7 @@ class Particle:
     __slots__ = ("id", "generation", "sample_index", "seed", "count", "configuration", "state", "log_p0",
                  "log_q_proposal", "relative_log_weight", "weight_log_scale", "physical_log_norm",
                  "tensor_log_norm", "energy_total", "energy_per_site", "max_state_truncation",
-                 "max_environment_truncation", "max_boundary_entropy", "time_seconds", "measurement_seconds")
+                 "max_environment_truncation", "max_boundary_entropy", "Se", "time_seconds", "measurement_seconds")
 
     def __init__(self, **kw):
         for k in self.__slots__:
@@ -157,26 +157,37 @@ class Particle:
                 "physical_log_norm": self.physical_log_norm, "energy_total": self.energy_total,
                 "energy_per_site": self.energy_per_site, "max_state_truncation": self.max_state_truncation,
                 "max_environment_truncation": self.max_environment_truncation,
-                "max_boundary_entropy": self.max_boundary_entropy, "time_seconds": self.time_seconds,
-                "measurement_seconds": self.measurement_seconds}
+                "max_boundary_entropy": self.max_boundary_entropy,
+                "Se": None if self.Se is None else {k: np.array(v) for k, v in self.Se.items()},
+                "time_seconds": self.time_seconds, "measurement_seconds": self.measurement_seconds}
 
 
-def observe(ctx, states, model, chi, energy):
+def nan_max(values, init=None):
+    """Julia's ``max``/``maximum`` on floats: NaN if any operand is NaN."""
+    vals = list(values) if init is None else [init, *values]
+    if not vals:
+        raise ValueError("empty maximum")
+    return math.nan if any(math.isnan(v) for v in vals) else max(vals)
+
+
+def observe(ctx, states, model, chi, energy, record_entropy=False):
     """``_observe`` (particles.jl:35-41) for a batch of resident states:
-    ``measure_energy`` when ``energy``, else ``lognorm`` with NaN energies."""
+    ``measure_energy`` (with the ``Se`` profiles when ``record_entropy``) when
+    ``energy``, else ``lognorm`` with NaN energies and no ``Se``."""
     t0 = time.perf_counter()
     if energy:
-        r = measure_energy_states(ctx, states, model, chi)
+        r = measure_energy_states(ctx, states, model, chi, record_entropy=record_entropy)
     else:
         r = lognorms(ctx, states, chi)
         r["energy_total"] = np.full(len(states), np.nan)
         r["energy_per_site"] = np.full(len(states), np.nan)
+        r["Se"] = [None] * len(states)
     r["measurement_seconds"] = (time.perf_counter() - t0) / max(len(states), 1)
     return r
 
 
 def new_particles(ctx, cache, draws, model, chi_measurement, energy, bond_space=None, pepo_live=None, seed=0,
-                  first_id=1):
+                  first_id=1, record_entropy=False):
     """``_initial_particles`` after the draws (run.jl:45-66) with ``_new_particle``
     (particles.jl:42-53) for each distinct configuration: project the PEPO
     (host slice + ``normalize_state!``), upload it as a resident PEPS (bonds
@@ -188,7 +199,8 @@ def new_particles(ctx, cache, draws, model, chi_measurement, energy, bond_space=
     ``draws`` is the dict of ``sample_configurations``. When ``energy`` and
     ``chi_measurement`` allow it, the projected samples are measured through
     the sampling cache's shared layers (same values as measuring the uploaded
-    states)."""
+    states). ``record_entropy`` records each particle's ``Se`` profiles
+    (``record_boundary_entropy``)."""
     grouping = group_configurations(draws["configuration"])
     reps = grouping["first_draw"]
     pepo = cache.state
@@ -204,11 +216,13 @@ def new_particles(ctx, cache, draws, model, chi_measurement, energy, bond_space=
         states.append(DeviceState(ctx, st, live=live))
     t0 = time.perf_counter()
     if energy:
-        m = measure_energies(cache, draws["configuration"][reps], model, chi_measurement)
+        m = measure_energies(cache, draws["configuration"][reps], model, chi_measurement,
+                             record_entropy=record_entropy)
     else:
         m = lognorms(ctx, states, chi_measurement)
         m["energy_total"] = np.full(len(reps), np.nan)
         m["energy_per_site"] = np.full(len(reps), np.nan)
+        m["Se"] = [None] * len(reps)
     mt_sec = (time.perf_counter() - t0) / max(len(reps), 1)
     out = []
     for u, i in enumerate(reps):
@@ -223,31 +237,36 @@ def new_particles(ctx, cache, draws, model, chi_measurement, energy, bond_space=
             max_state_truncation=0.0,
             max_environment_truncation=max(float(draws["max_environment_truncation"][i]),
                                            float(m["max_environment_truncation"][u])),
-            max_boundary_entropy=math.nan, time_seconds=0.0, measurement_seconds=mt_sec))
+            max_boundary_entropy=math.nan, Se=m["Se"][u], time_seconds=0.0, measurement_seconds=mt_sec))
     return out, grouping
 
 
 def advance_particles(ctx, particles, model, steps, chi_environment, chi_measurement, method, energy, order=2,
-                      on_step=None, D=None):
+                      on_step=None, D=None, record_entropy=False):
     """``_advance_particle!`` (particles.jl:86-102) with ``_evolve!``
     (particles.jl:66-85) for a batch of particles at the same beta: one
     full-update step per endpoint in ``steps`` (delta = (beta - state.beta)/2),
     then ``_observe``; relative_log_weight += 2 sum(log_normalization) +
     (tensor_log_norm_new - tensor_log_norm_old). ``D`` is the bonds' truncrank
-    (None: the resident bond capacity's dimension)."""
+    (None: the resident bond capacity's dimension). With ``record_entropy``
+    every particle's max_boundary_entropy is the maximum over the FU boundary
+    cuts of these steps and ``Se`` holds the measurement's profiles; without
+    recording, or without steps, max_boundary_entropy is NaN (``_evolve!``)."""
     if not particles:
         return particles
     devs = [p.state for p in particles]
     normalization = np.zeros(len(particles))
     maxstate = np.zeros(len(particles))
     maxenv = np.zeros(len(particles))
+    maxentropy = [0.0 if record_entropy and steps else math.nan] * len(particles)
     total = 0.0
     for beta in steps:
         b0 = devs[0].state.beta
         if any(not same_beta(d.state.beta, b0) for d in devs):
             raise ValueError("particles of one batch must share beta")
         t0 = time.perf_counter()
-        r = evolution_steps(ctx, devs, model, (beta - b0) / 2, chi_environment, method, order, D)
+        r = evolution_steps(ctx, devs, model, (beta - b0) / 2, chi_environment, method, order, D,
+                            record_entropy=record_entropy)
         elapsed = time.perf_counter() - t0
         for d in devs:
             d.state.beta = beta
@@ -255,10 +274,11 @@ def advance_particles(ctx, particles, model, steps, chi_environment, chi_measure
         if r["records"].shape[1]:
             maxstate = np.maximum(maxstate, r["records"][:, :, 1].max(axis=1))
         maxenv = np.maximum(maxenv, r["max_environment_truncation"])
+        maxentropy = [nan_max([a, float(b)]) for a, b in zip(maxentropy, r["max_boundary_entropy"])]
         total += elapsed
         if on_step is not None:
             on_step({"beta": beta, "time_seconds": elapsed, **r})
-    m = observe(ctx, devs, model, chi_measurement, energy)
+    m = observe(ctx, devs, model, chi_measurement, energy, record_entropy)
     for k, p in enumerate(particles):
         old = p.tensor_log_norm
         p.relative_log_weight += 2.0 * float(normalization[k]) + (float(m["tensor_log_norm"][k]) - old)
@@ -268,6 +288,8 @@ def advance_particles(ctx, particles, model, steps, chi_environment, chi_measure
         p.energy_per_site = float(m["energy_per_site"][k])
         p.max_state_truncation = float(maxstate[k])
         p.max_environment_truncation = max(float(maxenv[k]), float(m["max_environment_truncation"][k]))
+        p.max_boundary_entropy = maxentropy[k]
+        p.Se = m["Se"][k]
         p.time_seconds = total / len(particles)
         p.measurement_seconds = m["measurement_seconds"]
     return particles
@@ -287,7 +309,7 @@ def statistics(records, Ns, nsites):
     s.update({"beta": records[0]["beta"],
               "max_state_truncation": max(r["max_state_truncation"] for r in records),
               "max_environment_truncation": max(r["max_environment_truncation"] for r in records),
-              "max_boundary_entropy": max(r["max_boundary_entropy"] for r in records),
+              "max_boundary_entropy": nan_max(r["max_boundary_entropy"] for r in records),
               "distinct_configurations": len({np.asarray(r["configuration"]).tobytes() for r in records})})
     return s
 
@@ -298,13 +320,13 @@ def statistics(records, Ns, nsites):
 # ----------------------------------------------------------------------------
 
 def purify(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, chi_environment, chi_measurement, bond_space,
-           method=None, order=2, D=None):
+           method=None, order=2, D=None, record_entropy=False):
     """The evolution of ``run_purification`` (run.jl:2-43) from the identity PEPO
     (``identity_pepo``; normalised here as the reference does), on the device:
     FU steps to every node <= beta0 and an observation there. Returns the
     resident PEPO (``DeviceState``, bonds at the ``bond_space`` capacity with
     their live sectors) and the rows (beta, energy_total, energy_per_site,
-    lnZ, truncations, seconds)."""
+    lnZ, truncations, max_boundary_entropy, Se, seconds)."""
     method = method or FullUpdate()
     st, live = pad_bonds(pepo, bond_space)
     normalize_state(st)
@@ -314,19 +336,20 @@ def purify(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, chi_envir
         steps = step_schedule(dev.state.beta, beta, delta_tau)
         p = Particle(state=dev, tensor_log_norm=0.0, relative_log_weight=0.0)
         advance_particles(ctx, [p], model, steps, chi_environment, chi_measurement, method,
-                          measure_at(beta_measure, beta), order, D=D)
+                          measure_at(beta_measure, beta), order, D=D, record_entropy=record_entropy)
         rows.append({"beta": beta, "source": "purification", "energy_total": p.energy_total,
                      "energy_per_site": p.energy_per_site,
                      "energy_per_site_error": 0.0 if measure_at(beta_measure, beta) else math.nan,
                      "lnZ": p.physical_log_norm, "lnZ_error": 0.0, "max_state_truncation": p.max_state_truncation,
-                     "max_environment_truncation": p.max_environment_truncation, "time_seconds": p.time_seconds,
+                     "max_environment_truncation": p.max_environment_truncation,
+                     "max_boundary_entropy": p.max_boundary_entropy, "Se": p.Se, "time_seconds": p.time_seconds,
                      "measurement_seconds": p.measurement_seconds})
     return dev, rows
 
 
 def stochastic_ensemble(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, Ns, seed, chi_sampling,
                         chi_environment, chi_measurement, bond_space=None, method=None, order=2, basis=None,
-                        uniforms=None, on_node=None, D=None, draws_out=None):
+                        uniforms=None, on_node=None, D=None, draws_out=None, record_entropy=False):
     """The ensemble of ``run_stochastic`` (run.jl:87-130) without resampling: Ns Born draws
     from the beta0 PEPO (``pepo``: a host ``FiniteState`` or a resident
     ``DeviceState``), one particle per distinct configuration, then FU
@@ -353,13 +376,13 @@ def stochastic_ensemble(ctx, pepo, model, beta0, beta_log, beta_measure, delta_t
     nodes = [b for b in temperature_nodes(beta0, beta_log, beta_measure) if b >= beta0]
     particles, grouping = new_particles(ctx, cache, draws, model, chi_measurement,
                                         measure_at(beta_measure, nodes[0]), bond_space=bond_space,
-                                        pepo_live=live, seed=seed)
+                                        pepo_live=live, seed=seed, record_entropy=record_entropy)
     rows = []
     for node, beta in enumerate(nodes):
         if node > 0:
             steps = step_schedule(nodes[node - 1], beta, delta_tau)
             advance_particles(ctx, particles, model, steps, chi_environment, chi_measurement, method,
-                              measure_at(beta_measure, beta), order, D=D)
+                              measure_at(beta_measure, beta), order, D=D, record_entropy=record_entropy)
         records = [p.record() for p in particles]
         stats = statistics(records, Ns, ns)
         stats["source"] = "sampling"

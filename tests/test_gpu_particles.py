@@ -41,11 +41,14 @@ def oracle_pipeline():
     normalize_state(pepo)
     pur = []
     b = 0.0
+    pur_entropy = 0.0
     while b < BETA0 - 1e-12:
-        ofu.evolution_step(pepo, m, DT, D, CAPS, opts, CHI)
+        rep = ofu.evolution_step(pepo, m, DT, D, CAPS, opts, CHI, record_entropy=True)
+        pur_entropy = max(pur_entropy, rep["max_boundary_entropy"])
         b += 2 * DT
     pepo.beta = BETA0
-    pur.append(oe.measure_energy(pepo, m, CHI))
+    pur.append(oe.measure_energy(pepo, m, CHI, record_entropy=True))
+    pur[-1]["fu_entropy"] = pur_entropy
     cache = ob.SamplingCache(pepo, CHI)
     ns = lat.Lx * lat.Ly
     draws = [ob.sample_configuration(cache, uniforms=ob.counter_uniforms(1, i, ns)) for i in range(1, NS + 1)]
@@ -54,19 +57,23 @@ def oracle_pipeline():
     parts = []
     for u, i in enumerate(grp["first_draw"]):
         ps, _ = ob.project_configuration(pepo, cfgs[i])
-        r = oe.measure_energy(ps, m, CHI)
+        r = oe.measure_energy(ps, m, CHI, record_entropy=True)
         parts.append({"state": ps, "rel": r["log_norm"] - draws[i]["log_probability"],
-                      "tln": r["tensor_log_norm"], "e": r["energy_per_site"], "count": grp["multiplicity"][u]})
+                      "tln": r["tensor_log_norm"], "e": r["energy_per_site"], "count": grp["multiplicity"][u],
+                      "Se": r["Se"]})
     stats = [oens.ensemble_statistics([p["rel"] for p in parts], [p["count"] for p in parts],
                                       [p["e"] for p in parts], NS, nsites=ns)]
     for p in parts:
         norm = 0.0
         bb = BETA0
+        p["fu_entropy"] = 0.0
         while bb < BETA1 - 1e-12:
-            rep = ofu.evolution_step(p["state"], m, DT, D, CAPS, opts, CHI)
+            rep = ofu.evolution_step(p["state"], m, DT, D, CAPS, opts, CHI, record_entropy=True)
             norm += sum(x["log_normalization"] for x in rep["normalizations"])
+            p["fu_entropy"] = max(p["fu_entropy"], rep["max_boundary_entropy"])
             bb += 2 * DT
-        r = oe.measure_energy(p["state"], m, CHI)
+        r = oe.measure_energy(p["state"], m, CHI, record_entropy=True)
+        p["Se"] = r["Se"]
         p["rel"] += 2 * norm + (r["tensor_log_norm"] - p["tln"])
         p["tln"] = r["tensor_log_norm"]
         p["e"] = r["energy_per_site"]
@@ -93,6 +100,12 @@ def test_stochastic_pipeline_matches_oracle(spp, gpu_ctx):
     # beta = 0 (not a measurement node, so only lognorm): the normalised
     # identity PEPO has lnZ = log Tr 1 = Lx*Ly*log 2
     assert np.isnan(rows[0]["energy_total"]) and abs(rows[0]["lnZ"] - 9 * np.log(2.0)) < 1e-12
+    # boundary entropy (record_boundary_entropy defaults to true): no steps
+    # before beta = 0, the FU maximum of the steps to beta0, and the measured Se
+    assert np.isnan(rows[0]["max_boundary_entropy"]) and rows[0]["Se"] is None
+    assert abs(rows[-1]["max_boundary_entropy"] - pur[0]["fu_entropy"]) < 1e-8
+    for side in ("north", "south"):
+        np.testing.assert_allclose(rows[-1]["Se"][side], pur[0]["Se"][side], atol=1e-9)
     result = spp.run_stochastic(c, cp, ctx)
     assert result["schema_version"] == "2.7.0" and result["metadata"]["complete"]
     curve = result["observables"]["curve"]
@@ -108,9 +121,15 @@ def test_stochastic_pipeline_matches_oracle(spp, gpu_ctx):
         assert abs(p.relative_log_weight - q["rel"]) < 1e-8
         assert abs(p.energy_per_site - q["e"]) < 1e-9
         assert abs(p.beta - BETA1) < 1e-15
+        assert abs(p.max_boundary_entropy - q["fu_entropy"]) < 1e-8
+        for side in ("north", "south"):
+            np.testing.assert_allclose(p.Se[side], q["Se"][side], atol=1e-9)
     srows = [r for r in curve if r.get("source") == "sampling"]
     assert [r["beta"] for r in srows] == [BETA0, BETA1]
     for r, s in zip(srows, stats):
         for k in ("energy_per_site", "energy_per_site_error", "lnZ", "lnZ_error", "eta", "ESS"):
             assert abs(r[k] - s[k]) <= 1e-8 * max(1.0, abs(s[k])), (k, r[k], s[k])
         assert r["virtual_samples"] == NS and r["representatives"] == len(parts)
+    # new particles carry no FU entropy (NaN); after the steps the maximum over particles
+    assert np.isnan(srows[0]["max_boundary_entropy"])
+    assert abs(srows[1]["max_boundary_entropy"] - max(q["fu_entropy"] for q in parts)) < 1e-8

@@ -188,6 +188,78 @@ def test_compression_matches_exact_mps_svd():
     assert err < 10 * np.linalg.norm(ve) * np.sqrt(trunc.max_truncation) + 1e-12
 
 
+# ------------------------------ boundary entropy -----------------------------
+
+def test_schmidt_entropy_closed_forms():
+    """``_schmidt_entropy`` (boundary_mps.jl:20-30): k equal values give log k,
+    one value 0, two values the binary entropy, a zero spectrum NaN."""
+    for k in (1, 2, 5, 16):
+        assert bm.schmidt_entropy([np.full(k, 0.3)]) == pytest.approx(np.log(k), abs=1e-14)
+    assert bm.schmidt_entropy([np.array([2.0]), np.zeros(3)]) == 0.0
+    s = np.array([np.sqrt(0.8), np.sqrt(0.2)])
+    h = -(0.8 * np.log(0.8) + 0.2 * np.log(0.2))
+    assert bm.schmidt_entropy([s[:1], s[1:]]) == pytest.approx(h, abs=1e-15)
+    assert np.isnan(bm.schmidt_entropy([np.zeros(2)]))
+    e = {"maximum": 0.0}
+    bm.record_entropy(e, "north", 1, 1, 0.5)
+    bm.record_entropy(e, "north", 1, 1, float("nan"))
+    bm.record_entropy(e, "north", 1, 1, 0.7)
+    assert np.isnan(e["maximum"])  # Julia's max propagates NaN
+
+
+def _rows_vector(layers, ys, open_side):
+    """Dense contraction of the double-layer rows ``ys`` (top to bottom) with
+    np.einsum; open legs: the south legs of the last row (open_side "south")
+    or the north legs of the first row ("north"), in site order."""
+    Lx = len(layers)
+    counter = itertools.count()
+    lab = {}
+
+    def g(k):
+        if k not in lab:
+            lab[k] = next(counter)
+        return lab[k]
+    args = []
+    for y in ys:
+        for x in range(Lx):
+            E = layers[x][y].a
+            args += [E, [g(("h", x, y)), g(("v", x, y)), g(("h", x + 1, y)), g(("v", x, y + 1))]]
+    yo = ys[-1] + 1 if open_side == "south" else ys[0]
+    out = [g(("v", x, yo)) for x in range(Lx)]
+    return np.einsum(*args, out, optimize=True)
+
+
+def _exact_entropy(v, b):
+    s = np.linalg.svd(v.reshape(int(np.prod(v.shape[:b])), -1), compute_uv=False)
+    p = s ** 2 / np.sum(s ** 2)
+    p = p[p > 0]
+    return float(-np.sum(p * np.log(p)))
+
+
+@pytest.mark.parametrize("kind,seed", [("peps", 21), ("pepo", 22)])
+def test_boundary_entropy_profile_is_exact_schmidt_entropy(kind, seed):
+    """Without truncation the pre-truncation spectrum of every cut is the exact
+    Schmidt spectrum of the boundary vector, so Se.north[r-1, b-1] is the
+    entropy of rows 1..r contracted densely and cut after column b, and
+    Se.south[r-1, b-1] that of rows r+1..Ly (boundary_mps.jl:193-209,
+    energy.jl:31-78)."""
+    L = Lattice(3, 3)
+    st = random_state(L, (1, 1), kind, seed)
+    m = TFIM(J=1.0, h=0.9, symmetric=True)
+    r = oe.measure_energy(st, m, 256, record_entropy=True)
+    layers = bm.double_layers(st)
+    Se = r["Se"]
+    assert Se["north"].shape == (2, 2) and Se["south"].shape == (2, 2)
+    for rc in (1, 2):
+        vn = _rows_vector(layers, list(range(rc)), "south")
+        vs = _rows_vector(layers, list(range(rc, 3)), "north")
+        for b in (1, 2):
+            assert Se["north"][rc - 1, b - 1] == pytest.approx(_exact_entropy(vn, b), abs=1e-10)
+            assert Se["south"][rc - 1, b - 1] == pytest.approx(_exact_entropy(vs, b), abs=1e-10)
+    # the first and last absorbs close on unit legs: their cuts carry entropy 0
+    assert r["max_boundary_entropy"] == pytest.approx(max(Se["north"].max(), Se["south"].max()), abs=1e-14)
+
+
 # ------------------------------ truncation rules -----------------------------
 
 def test_select_kept_truncrank_and_intersection():
