@@ -466,6 +466,7 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
     return p;
   };
   auto cleanup = [&]() {
+    entropy_reset(ctx);  // pending cuts write into buffers freed below
     for (auto& b : south) b.release();
     NB[0].release();
     NB[1].release();
@@ -602,6 +603,7 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
       SPP_CHECK(hipMemcpyAsync(h_mtn.data(), Nf.max_trunc, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_mts.data(), south[0].max_trunc, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_amp.data(), lnamp, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      if (eobs.on()) entropy_flush(ctx);
       eobs.fetch(st);
       SPP_CHECK(hipStreamSynchronize(st));
       for (int s = 0; s < cur; ++s) {
@@ -690,6 +692,7 @@ void measure_energy_states(Context& ctx, const std::vector<const StateDev*>& sta
     return p;
   };
   auto cleanup = [&]() {
+    entropy_reset(ctx);  // pending cuts write into buffers freed below
     for (auto& b : south) b.release();
     NB[0].release();
     NB[1].release();
@@ -868,6 +871,7 @@ void measure_energy_states(Context& ctx, const std::vector<const StateDev*>& sta
       SPP_CHECK(hipMemcpyAsync(h_ls.data(), Nf.log_scale, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_mtn.data(), Nf.max_trunc, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_mts.data(), south[0].max_trunc, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      if (eobs.on()) entropy_flush(ctx);
       eobs.fetch(st);
       SPP_CHECK(hipStreamSynchronize(st));
       for (int s = 0; s < cur; ++s) {

@@ -117,15 +117,18 @@ struct LaunchTimer {
 };
 
 // Cut spectra waiting for the batched eigenvalue solve of the boundary
-// entropy (compact rho blocks copied aside during an absorb).
-constexpr int kEntropyCuts = 16;
+// entropy (compact rho blocks copied aside during absorbs, solved together).
+constexpr int kEntropyCuts = 32;
 struct EntropyTable {
   double* mat[kEntropyCuts];      // [2 nb][ld][ld] blocks (overwritten)
   double* w[kEntropyCuts];        // [2 nb][wstride] eigenvalues
   const int* cnt[kEntropyCuts];   // [2 nb] block dimensions
+  double* out[kEntropyCuts];      // [cuts][nb] entropies of the absorb, or nullptr
+  double* run_max[kEntropyCuts];  // [nb] running maximum, or nullptr
   long long wstride[kEntropyCuts];
   int ld[kEntropyCuts];
   int cut[kEntropyCuts];          // cut index (after site cut) in the absorb frame
+  int nb[kEntropyCuts];           // samples
   int count;
 };
 struct EntropyWork {
@@ -133,7 +136,9 @@ struct EntropyWork {
   size_t cap = 0, used = 0;
   struct Cut {
     size_t mat, w, cnt;
-    int ld, cut;
+    int ld, cut, nb;
+    double* out;
+    double* run_max;
   };
   std::vector<Cut> pending;
   EntropyWork() = default;
@@ -167,11 +172,15 @@ struct Context {
 void entropy_cut(Context& ctx, const double* w, long long wstride, const int* cnt, int nb, int cut, double* out,
                  double* run_max);
 // Copy the compact blocks rho [2 nb][ld][ld] of one cut aside for the batched
-// solve (flushing first when the workspace is full).
+// solve (solving what is pending first when the workspace is full). The
+// outputs are written by entropy_flush, which the caller runs before reading
+// them.
 void entropy_defer(Context& ctx, const double* rho, long long rps, int ld, const int* cnt, int nb, int cut,
-                   int cuts_left, double* out, double* run_max);
+                   double* out, double* run_max);
 // Solve every deferred cut and write its entropies.
-void entropy_flush(Context& ctx, int nb, double* out, double* run_max);
+void entropy_flush(Context& ctx);
+// Drop pending cuts (their outputs are being freed) and the workspace.
+void entropy_reset(Context& ctx);
 // All eigenvalues (ascending) of nprob symmetric blocks [nprob][ld][ld]
 // (host), of dimensions n_h[i] <= ld, into eig_h [nprob][ld] (host).
 void spectrum_selftest(Context& ctx, int nprob, int ld, const int* n_h, const double* mats_h, double* eig_h);

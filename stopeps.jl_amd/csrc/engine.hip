@@ -1368,7 +1368,7 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
       if (nmax <= ctx.ksub)
         entropy_cut(ctx, er.w, er.wstride, ncount, nb, x, out.entropy, out.entropy_max);
       else
-        entropy_defer(ctx, rho, rps, (int)nmax, ncount, nb, x, L - 1 - x, out.entropy, out.entropy_max);
+        entropy_defer(ctx, rho, rps, (int)nmax, ncount, nb, x, out.entropy, out.entropy_max);
     }
     out.bdim[x] = kb.dim;
     out.beven[x] = kb.even;
@@ -1385,7 +1385,6 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
     Rl = {nbond, br, sp.e};
     kb = nbond;
   }
-  if (out.entropy || out.entropy_max) entropy_flush(ctx, nb, out.entropy, out.entropy_max);
   ctx.arena.release(mark);
 }
 

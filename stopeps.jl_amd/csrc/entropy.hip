@@ -10,7 +10,8 @@
 //
 // Blocks that the direct Jacobi solve of the truncation already diagonalised
 // give their eigenvalues straight away. Larger blocks are copied aside during
-// the sweep and solved together at the end of the absorb, one workgroup per
+// the sweeps and solved together when the caller flushes (once per batch of
+// samples, so that thousands of blocks fill the device), one workgroup per
 // block: Householder tridiagonalisation in place (the rank-two update of one
 // step fused with the matrix-vector product of the next, so every step streams
 // the trailing block once) followed by Sturm-count bisection for every
@@ -94,6 +95,7 @@ __device__ int sturm_count(const double* d, const double* ee, int n, double x, d
 __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(EntropyTable tab) {
   const int c = blockIdx.y;
   const int prob = blockIdx.x;
+  if (prob >= 2 * tab.nb[c]) return;
   const int ld = tab.ld[c];
   const int n = tab.cnt[c][prob];
   double* A = tab.mat[c] + (long long)prob * ld * ld;
@@ -212,12 +214,13 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(EntropyTable tab
 __device__ inline double nan_max(double m, double v) { return (isnan(m) || isnan(v)) ? NAN : fmax(m, v); }
 
 // Entropy of every sample (blockIdx.x) at every cut of the table, from the
-// spectra of its two parity blocks; out[cut * nb + s] and the running maximum.
-__global__ __launch_bounds__(256) void entropy_kernel(EntropyTable tab, int nb, double* out, double* run_max) {
+// spectra of its two parity blocks; out[cut * nb + s] and the running maximum
+// (one block per sample updates it entry after entry).
+__global__ __launch_bounds__(256) void entropy_kernel(EntropyTable tab) {
   const int s = blockIdx.x;
   __shared__ double part[256];
-  double m = run_max ? run_max[s] : 0.0;
   for (int c = 0; c < tab.count; ++c) {
+    if (s >= tab.nb[c]) continue;
     double tot = 0.0;
     for (int q = 0; q < 2; ++q) {
       const double* wq = tab.w[c] + (long long)(2 * s + q) * tab.wstride[c];
@@ -252,11 +255,10 @@ __global__ __launch_bounds__(256) void entropy_kernel(EntropyTable tab, int nb, 
     const double S = A > 0.0 ? fmax(0.0, part[0]) : NAN;
     __syncthreads();
     if (threadIdx.x == 0) {
-      if (out) out[(long long)tab.cut[c] * nb + s] = S;
-      m = nan_max(m, S);
+      if (tab.out[c]) tab.out[c][(long long)tab.cut[c] * tab.nb[c] + s] = S;
+      if (tab.run_max[c]) tab.run_max[c][s] = nan_max(tab.run_max[c][s], S);
     }
   }
-  if (threadIdx.x == 0 && run_max) run_max[s] = m;
 }
 
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -274,16 +276,19 @@ void entropy_cut(Context& ctx, const double* w, long long wstride, const int* cn
   tab.wstride[0] = wstride;
   tab.cnt[0] = cnt;
   tab.cut[0] = cut;
+  tab.nb[0] = nb;
+  tab.out[0] = out;
+  tab.run_max[0] = run_max;
   tab.count = 1;
-  hipLaunchKernelGGL(entropy_kernel, dim3(nb), dim3(256), 0, ctx.stream, tab, nb, out, run_max);
+  hipLaunchKernelGGL(entropy_kernel, dim3(nb), dim3(256), 0, ctx.stream, tab);
   SPP_CHECK(hipGetLastError());
 }
 
-void entropy_flush(Context& ctx, int nb, double* out, double* run_max) {
+void entropy_flush(Context& ctx) {
   EntropyWork& ew = ctx.entropy;
   for (size_t g = 0; g < ew.pending.size(); g += kEntropyCuts) {
     EntropyTable tab{};
-    int ldmax = 1;
+    int ldmax = 1, nbmax = 1;
     for (size_t i = g; i < ew.pending.size() && tab.count < kEntropyCuts; ++i) {
       const EntropyWork::Cut& c = ew.pending[i];
       const int k = tab.count++;
@@ -293,7 +298,11 @@ void entropy_flush(Context& ctx, int nb, double* out, double* run_max) {
       tab.wstride[k] = c.ld;
       tab.ld[k] = c.ld;
       tab.cut[k] = c.cut;
+      tab.nb[k] = c.nb;
+      tab.out[k] = c.out;
+      tab.run_max[k] = c.run_max;
       ldmax = std::max(ldmax, c.ld);
+      nbmax = std::max(nbmax, c.nb);
     }
     const size_t lds = sizeof(double) * (kSpecWaves + 6 * (size_t)ldmax);
     if (lds > 64 * 1024) {
@@ -307,37 +316,51 @@ void entropy_flush(Context& ctx, int nb, double* out, double* run_max) {
       SPP_CHECK(hipFuncSetAttribute((const void*)spectrum_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
     }
-    hipLaunchKernelGGL(spectrum_kernel, dim3(2 * nb, tab.count), dim3(kSpecThreads), lds, ctx.stream, tab);
+    // blocks with rows past a block's dimension idle; blockIdx.x covers the largest batch
+    hipLaunchKernelGGL(spectrum_kernel, dim3(2 * nbmax, tab.count), dim3(kSpecThreads), lds, ctx.stream, tab);
     SPP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(entropy_kernel, dim3(nb), dim3(256), 0, ctx.stream, tab, nb, out, run_max);
+    hipLaunchKernelGGL(entropy_kernel, dim3(nbmax), dim3(256), 0, ctx.stream, tab);
     SPP_CHECK(hipGetLastError());
   }
   ew.pending.clear();
   ew.used = 0;
 }
 
+void entropy_reset(Context& ctx) {
+  EntropyWork& ew = ctx.entropy;
+  ew.pending.clear();
+  ew.used = 0;
+  if (ew.buf) {
+    (void)hipStreamSynchronize(ctx.stream);
+    (void)hipFree(ew.buf);
+  }
+  ew.buf = nullptr;
+  ew.cap = 0;
+}
+
 void entropy_defer(Context& ctx, const double* rho, long long rps, int ld, const int* cnt, int nb, int cut,
-                   int cuts_left, double* out, double* run_max) {
+                   double* out, double* run_max) {
   EntropyWork& ew = ctx.entropy;
   const size_t bm = align256(sizeof(double) * 2 * (size_t)nb * (size_t)ld * ld);
   const size_t bw = align256(sizeof(double) * 2 * (size_t)nb * ld);
   const size_t bc = align256(sizeof(int) * 2 * (size_t)nb);
   const size_t need = bm + bw + bc;
   if (ew.used + need > ew.cap) {
-    entropy_flush(ctx, nb, out, run_max);
+    entropy_flush(ctx);
     if (need > ew.cap) {
+      // room for a few hundred cuts of this size, within a quarter of the free memory
       size_t freeb = 0, totalb = 0;
       SPP_CHECK(hipStreamSynchronize(ctx.stream));
       if (ew.buf) SPP_CHECK(hipFree(ew.buf));
       ew.buf = nullptr;
       ew.cap = 0;
       SPP_CHECK(hipMemGetInfo(&freeb, &totalb));
-      const size_t want = std::max(need, std::min(need * (size_t)std::max(1, cuts_left), freeb / 4));
+      const size_t want = std::max(need, std::min(need * 256, freeb / 4));
       SPP_CHECK(hipMalloc((void**)&ew.buf, want));
       ew.cap = want;
     }
   }
-  EntropyWork::Cut c{ew.used, ew.used + bm, ew.used + bm + bw, ld, cut};
+  EntropyWork::Cut c{ew.used, ew.used + bm, ew.used + bm + bw, ld, cut, nb, out, run_max};
   SPP_CHECK(hipMemcpyAsync(ew.buf + c.mat, rho, sizeof(double) * 2 * (size_t)nb * (size_t)rps,
                            hipMemcpyDeviceToDevice, ctx.stream));
   SPP_CHECK(hipMemcpyAsync(ew.buf + c.cnt, cnt, sizeof(int) * 2 * (size_t)nb, hipMemcpyDeviceToDevice, ctx.stream));
@@ -366,6 +389,7 @@ void spectrum_selftest(Context& ctx, int nprob, int ld, const int* n_h, const do
     tab.cnt[0] = c;
     tab.wstride[0] = ld;
     tab.ld[0] = ld;
+    tab.nb[0] = (nprob + 1) / 2;
     tab.count = 1;
     const size_t lds = sizeof(double) * (kSpecWaves + 6 * (size_t)ld);
     if (lds > 64 * 1024) {

@@ -656,6 +656,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
     return p;
   };
   auto cleanup = [&]() {
+    entropy_reset(ctx);  // pending cuts write into buffers freed below
     for (auto& b : north) b.release();
     for (auto& b : south) b.release();
     for (void* p : scratch) (void)hipFree(p);
@@ -1203,6 +1204,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
           std::memcpy(states[c0 + s]->live.data(), hl.data() + (size_t)s * ns * 8, sizeof(int) * ns * 8);
       }
       if (emax) {
+        entropy_flush(ctx);
         SPP_CHECK(hipMemcpyAsync(hmt.data(), emax, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
         SPP_CHECK(hipStreamSynchronize(st));
       }
