@@ -101,7 +101,9 @@ int spp_state_download(spp_ctx* ctx, const spp_state* state, double* const* data
  * [(Ly-1)*Lx][d^4] are `trotter_gate(model, lattice, i, j, delta_gate)`
  * (src/models/tfim.jl:48) as G[p][q][u][v] in the parity-sorted physical
  * basis, bond (x,y)-(x+1,y) at y*(Lx-1)+x and (x,y)-(x,y+1) at y*Lx+x, with
- * delta_gate = delta/2 for order 2. Outputs (may be null): records
+ * delta_gate = delta/2 for order 2. On a one-site lattice gates_h is instead
+ * exp(-delta onsite_hamiltonian) as G[p][u] (full_update.jl:568-570) and
+ * gates_v is unused. Outputs (may be null): records
  * [n][bond updates in sweep order][8] = {initial_cost, final_cost, residual,
  * iterations, iteration_limit_reached, stop (0 disabled, 1 iteration limit,
  * 2 error tolerance, 3 residual fallback), kept_even, kept_odd};

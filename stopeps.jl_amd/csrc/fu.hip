@@ -393,6 +393,22 @@ __global__ void fu_set_live_kernel(int* live, int ns, int site_a, int va, int si
 }
 
 // sum of squares of each sample's site array (block per sample)
+// A[.., p, ..] <- sum_u G[p][u] A[.., u, ..] on the physical leg (leg 2) of one
+// site per sample (blockIdx.x); outer = dims of legs 0-1, inner = legs 3-5.
+__global__ void onsite_gate_kernel(double* const* p, long long outer, int d, long long inner, const double* G) {
+  double* a = p[blockIdx.x];
+  for (long long t = threadIdx.x; t < outer * inner; t += blockDim.x) {
+    const long long o = t / inner, i = t % inner;
+    double v[8];
+    for (int u = 0; u < d; ++u) v[u] = a[(o * d + u) * inner + i];
+    for (int q = 0; q < d; ++q) {
+      double acc = 0.0;
+      for (int u = 0; u < d; ++u) acc += G[q * d + u] * v[u];
+      a[(o * d + q) * inner + i] = acc;
+    }
+  }
+}
+
 __global__ void site_sumsq_kernel(double* const* p, long long n, double* out) {
   const double* a = p[blockIdx.x];
   __shared__ double part[256];
@@ -501,6 +517,67 @@ int fu_pair_host(int horizontal, const int* dims_a, const int* evens_a, const in
 // the evolution step
 // ---------------------------------------------------------------------------
 
+// One-site lattice (full_update.jl:568-570): A <- exp(-delta h_onsite) on the
+// physical leg, then normalize_state!. The gate is P.gates_h [d][d].
+static void onsite_step(Context& ctx, const std::vector<StateDev*>& states, const FuParams& P, FuOutputs& out) {
+  hipStream_t st = ctx.stream;
+  const int n = (int)states.size();
+  const SiteDev& s0 = states[0]->sites[0];
+  const int d = s0.dim[2], ed = s0.even[2];
+  if (!P.gates_h) throw HipError("full update: the onsite gate is required");
+  if (d > 8) throw HipError("full update: physical dimension above 8");
+  for (int q = 0; q < d; ++q)
+    for (int u = 0; u < d; ++u)
+      if ((q >= ed) != (u >= ed) && P.gates_h[q * d + u] != 0.0)
+        throw HipError("full update: the onsite gate does not respect the Z2 grading of the physical space");
+  const long long outer = (long long)s0.dim[0] * s0.dim[1];
+  const long long inner = (long long)s0.dim[3] * s0.dim[4] * s0.dim[5];
+  const long long size = s0.size();
+  std::vector<double*> hp(n);
+  for (int s = 0; s < n; ++s) hp[s] = states[s]->sites[0].d;
+  double** dp = nullptr;
+  double *dg = nullptr, *dn = nullptr;
+  auto release = [&]() {
+    if (dp) (void)hipFree(dp);
+    if (dg) (void)hipFree(dg);
+    if (dn) (void)hipFree(dn);
+  };
+  try {
+    SPP_CHECK(hipMalloc((void**)&dp, sizeof(double*) * n));
+    SPP_CHECK(hipMalloc((void**)&dg, sizeof(double) * d * d));
+    SPP_CHECK(hipMalloc((void**)&dn, sizeof(double) * n));
+    SPP_CHECK(hipMemcpyAsync(dp, hp.data(), sizeof(double*) * n, hipMemcpyHostToDevice, st));
+    SPP_CHECK(hipMemcpyAsync(dg, P.gates_h, sizeof(double) * d * d, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(onsite_gate_kernel, dim3(n), dim3(256), 0, st, dp, outer, d, inner, (const double*)dg);
+    SPP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(site_sumsq_kernel, dim3(n), dim3(256), 0, st, dp, size, dn);
+    SPP_CHECK(hipGetLastError());
+    std::vector<double> hn(n);
+    SPP_CHECK(hipMemcpyAsync(hn.data(), dn, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    SPP_CHECK(hipStreamSynchronize(st));
+    for (int s = 0; s < n; ++s) {
+      const double v = std::sqrt(hn[s]);
+      if (!(v > 0.0 && std::isfinite(v))) throw HipError("zero or nonfinite tensor after the full update");
+      states[s]->log_amplitude += std::log(v);
+      if (out.normalizations) out.normalizations[s] = std::log(v);
+      if (out.max_env_trunc) out.max_env_trunc[s] = 0.0;
+      if (out.max_entropy) out.max_entropy[s] = 0.0;  // no boundary cuts
+      states[s]->beta += 2.0 * P.delta;
+      hn[s] = 1.0 / v;
+    }
+    SPP_CHECK(hipMemcpyAsync(dn, hn.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(site_scale_kernel, dim3(n, (unsigned)std::min<long long>((size + 255) / 256, 64)), dim3(256),
+                       0, st, dp, size, (const double*)dn);
+    SPP_CHECK(hipGetLastError());
+    SPP_CHECK(hipStreamSynchronize(st));
+  } catch (...) {
+    (void)hipStreamSynchronize(st);
+    release();
+    throw;
+  }
+  release();
+}
+
 template <typename T>
 void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int chi, const FuParams& P,
                        FuOutputs& out) {
@@ -521,7 +598,10 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
   const int d = s0.sites[0].dim[2], ed = s0.sites[0].even[2];
   for (int i = 0; i < ns; ++i)
     if (s0.sites[i].dim[2] != d || s0.sites[i].even[2] != ed) throw HipError("full update: physical legs differ");
-  if (ns == 1) throw HipError("full update: a single-site lattice has no bonds");
+  if (ns == 1) {
+    onsite_step(ctx, states, P, out);
+    return;
+  }
   const int nh = Ly * (Lx - 1), nv = (Ly - 1) * Lx;
   if (!P.gates_h || !P.gates_v) throw HipError("full update: gates are required");
   // leg specs of the double layers (fixed capacities)

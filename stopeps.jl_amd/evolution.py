@@ -56,6 +56,14 @@ def trotter_gate(model, lattice, a, b, delta):
     return ((V * np.exp(-delta * w)) @ V.T).reshape(2, 2, 2, 2)
 
 
+def onsite_gate(model, delta):
+    """``exp(-delta onsite_hamiltonian)`` (tfim.jl:43, full_update.jl:568-570)
+    with ``onsite_hamiltonian = -h X``, as G[p, u]."""
+    X, _ = spin_operators(model)
+    w, V = np.linalg.eigh(-model.h * X)
+    return (V * np.exp(-delta * w)) @ V.T
+
+
 def _gates(model, lattice, delta, ppar):
     perm = np.argsort(ppar, kind="stable")
     def sorted_gate(a, b):
@@ -95,6 +103,9 @@ def evolution_steps(ctx, states, model, delta, chi, method=None, order=2, D=None
         raise ValueError("model physical space does not match the state")
     gate_delta = delta / 2 if order == 2 else delta
     gh, gv = _gates(model, L, gate_delta, ppar)
+    if L.Lx * L.Ly == 1:  # one site: the onsite gate at the full delta
+        perm = np.argsort(ppar, kind="stable")
+        gh = np.ascontiguousarray(onsite_gate(model, delta)[np.ix_(perm, perm)])
     nh, nv = L.Ly * (L.Lx - 1), (L.Ly - 1) * L.Lx
     nup = (2 if order == 2 else 1) * (nh + nv)
     rec = np.zeros((n, nup, 8))

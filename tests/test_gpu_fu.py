@@ -183,3 +183,22 @@ def test_fu_step_fp32_contractions(spp, gpu_ctx):
         say(f"  state {k}: |dpsi|/|psi| = {err:.2e}")
         assert err <= 1e-5
         assert abs(ev.log_amplitude - o.log_amplitude) <= 1e-5 * max(1.0, abs(o.log_amplitude))
+
+
+@pytest.mark.parametrize("kind", ["peps", "pepo"])
+def test_fu_step_single_site(spp, gpu_ctx, kind):
+    """One-site lattice (full_update.jl:568-570): the onsite gate
+    exp(-delta h_onsite) on the physical leg at the full delta, then
+    normalize_state!, against the oracle's step."""
+    say(f"full-update step on a 1x1 {kind}")
+    m = TFIM(J=1.0, h=0.8, symmetric=True)
+    o = random_state(Lattice(1, 1), (1, 1), kind, seed=5)
+    dev = spp.DeviceState(gpu_ctx, to_product_state(spp, o))
+    got = spp.evolution_steps(gpu_ctx, [dev], spp.TFIM(m.J, m.h, symmetric=True), 0.1, 8, record_entropy=True)
+    ref = ofu.evolution_step(o, m, 0.1, 2, (1, 1), ofu.FullUpdate(), 8)
+    ev = to_oracle(dev.download())
+    np.testing.assert_allclose(ev[(0, 0)].a, o[(0, 0)].a, atol=1e-13)
+    assert abs(ev.log_amplitude - o.log_amplitude) < 1e-13 and abs(ev.beta - o.beta) < 1e-15
+    assert got["records"].shape == (1, 0, 8)
+    assert abs(got["normalizations"][0, 0] - ref["normalizations"][0]["log_normalization"]) < 1e-13
+    assert got["max_environment_truncation"][0] == 0.0 and got["max_boundary_entropy"][0] == 0.0
