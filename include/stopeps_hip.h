@@ -55,7 +55,11 @@ int spp_set_precision(spp_ctx* ctx, int precision);
  * filter degree of a subspace step, 1 = plain power steps; default 8),
  * "chunk" (samples per device batch,
  * 0 = sized from free memory), "profile" (1: time each contraction launch
- * with HIP events on the library stream). */
+ * with HIP events on the library stream), "boundary_cap_even" /
+ * "boundary_cap_odd" (the sector budget of `BoundaryMPS(chi, sector_space)`,
+ * boundary_mps.jl:11-18: every boundary compression keeps truncrank(chi)
+ * intersected with at most this many values of each bond parity; negative =
+ * no budget, the default; applies to the calls made while it is set). */
 int spp_set_option(spp_ctx* ctx, const char* name, double value);
 
 /* counters[0] = contraction flops issued (2*M*N*K over parity blocks),

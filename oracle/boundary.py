@@ -78,8 +78,26 @@ def record_entropy(entropy, side, row_cut, bond, value):
         prof[row_cut - 1, bond - 1] = value
 
 
+class BoundaryMPS:
+    """``BoundaryMPS(chi, sector_space)`` (boundary_mps.jl:11-18): every
+    compression keeps ``truncrank(chi) & truncspace(sector_space)``; the
+    sector space is an (even, odd) budget, or None. Every function of this
+    oracle that takes ``chi`` also takes a ``BoundaryMPS``."""
+
+    def __init__(self, chi, sector_space=None):
+        self.chi = int(chi)
+        self.caps = None if sector_space is None else tuple(int(c) for c in sector_space)
+
+
+def _method(chi):
+    return chi if isinstance(chi, BoundaryMPS) else BoundaryMPS(chi)
+
+
 def compress_boundary(B, chi, reverse=False, observer=None):
-    """``compress_boundary!`` (boundary_mps.jl:94-134). Modifies the list ``B``."""
+    """``compress_boundary!`` (boundary_mps.jl:94-134). Modifies the list ``B``.
+    ``chi``: the bond, or a ``BoundaryMPS`` with a sector budget."""
+    method = _method(chi)
+    chi, caps = method.chi, method.caps
     if chi < 1:
         raise ValueError("chi must be positive")
     N = len(B)
@@ -90,7 +108,7 @@ def compress_boundary(B, chi, reverse=False, observer=None):
             B[x] = Q
             B[x - 1] = ein("lpk,kr->lpr", B[x - 1], L)
         for x in range(N - 1):
-            U, S, V, bp, err, spectra = tsvd(B[x], 2, chi=chi)
+            U, S, V, bp, err, spectra = tsvd(B[x], 2, chi=chi, caps=caps)
             nM = B[x].norm()
             if observer is not None:
                 observer(x + 1, schmidt_entropy(spectra))
@@ -104,7 +122,7 @@ def compress_boundary(B, chi, reverse=False, observer=None):
             B[x] = Q
             B[x + 1] = ein("lk,kpr->lpr", R, B[x + 1])
         for x in range(N - 1, 0, -1):
-            U, S, V, bp, err, spectra = tsvd(B[x], 1, chi=chi)
+            U, S, V, bp, err, spectra = tsvd(B[x], 1, chi=chi, caps=caps)
             nM = B[x].norm()
             if observer is not None:
                 observer(x, schmidt_entropy(spectra))

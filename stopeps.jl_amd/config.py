@@ -6,7 +6,7 @@ schema.
 Spaces are written as this library writes them: a Z2 bond space as its
 (even, odd) sector dimensions, a dense one as an int. Fields that select
 reference features this library does not provide (diagonal or variational
-contraction, boundary sector caps, annealed resampling, file output) are kept
+contraction, annealed resampling, file output) are kept
 so that a configuration reads the same; the drivers reject them with a
 message naming the feature.
 """
@@ -149,7 +149,8 @@ def validate_config(c):
         raise ValueError("invalid full-update options")
     if not all(np.isfinite(x) for x in (c.model.J, c.model.h)):
         raise ValueError("nonfinite model parameter")
-    for V in (c.bond_space_pur, c.bond_space_peps):
+    for V in (c.bond_space_pur, c.bond_space_peps, c.environment_space, c.sampling_space, c.measurement_space,
+              c.proposal_space):
         if V is None:
             continue
         if c.model.symmetric and not (len(V) == 2 and min(V) >= 0 and sum(V) > 0):
@@ -164,8 +165,6 @@ def device_support(c, stochastic):
     """Raise for configuration choices this library does not run on the device."""
     if c.contraction not in ("BoundaryMPS", "BoundaryMPSRow"):
         raise NotImplementedError(f"contraction {c.contraction}: only the row boundary MPS is provided")
-    if any(v is not None for v in (c.environment_space, c.sampling_space, c.measurement_space, c.proposal_space)):
-        raise NotImplementedError("boundary sector caps (environment/sampling/measurement spaces) are not provided")
     if stochastic and c.resampling.enabled:
         raise NotImplementedError("annealed Born resampling is not provided: set resampling=ResamplingOptions(enabled=False)")
 

@@ -13,18 +13,20 @@ class NormError(ValueError):
     """``_positive_real`` failure (boundary_mps.jl:187-190)."""
 
 
-def lognorms(ctx, states, chi, check=True):
-    """Batched ``lognorm(state, BoundaryMPS(chi))`` of states on one lattice
-    with the same leg spaces (PEPS or PEPO). Returns a dict of arrays:
-    log_norm, tensor_log_norm, max_environment_truncation, status."""
+def lognorms(ctx, states, chi, check=True, sector_space=None):
+    """Batched ``lognorm(state, BoundaryMPS(chi, sector_space))`` of states on
+    one lattice with the same leg spaces (PEPS or PEPO). Returns a dict of
+    arrays: log_norm, tensor_log_norm, max_environment_truncation, status."""
     devs = [s if isinstance(s, DeviceState) else DeviceState(ctx, s) for s in states]
     n = len(devs)
     handles = (ctypes.c_void_p * max(n, 1))(*[d.handle.value for d in devs])
     out = {k: np.full(n, np.nan) for k in ("log_norm", "tensor_log_norm", "max_environment_truncation")}
     status = np.zeros(n, dtype=np.int32)
-    _lib.check(ctx.handle, ctx._lib.spp_lognorm_batch(
-        ctx.handle, n, handles, int(chi), _lib.dptr(out["log_norm"]), _lib.dptr(out["tensor_log_norm"]),
-        _lib.dptr(out["max_environment_truncation"]), _lib.iptr(status)))
+    with ctx.boundary_space(sector_space):
+        rc = ctx._lib.spp_lognorm_batch(
+            ctx.handle, n, handles, int(chi), _lib.dptr(out["log_norm"]), _lib.dptr(out["tensor_log_norm"]),
+            _lib.dptr(out["max_environment_truncation"]), _lib.iptr(status))
+    _lib.check(ctx.handle, rc)
     if check and np.any(status):
         raise NormError("norm requires a finite positive real value")
     out["status"] = status

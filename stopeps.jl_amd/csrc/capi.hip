@@ -103,6 +103,9 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value) {
     } else if (n == "profile") {
       ctx->c.flush_timer();
       ctx->c.profile = value != 0 ? 1 : 0;
+    } else if (n == "boundary_cap_even" || n == "boundary_cap_odd") {
+      if (!(value == value) || value > 1e9) throw HipError(n + " must be an integer (negative: no budget)");
+      ctx->c.bcap[n == "boundary_cap_even" ? 0 : 1] = value < 0 ? -1 : (int)value;
     } else if (n == "chunk") {
       if (value < 0) throw HipError("chunk must be >= 0");
       ctx->c.chunk = (long long)value;

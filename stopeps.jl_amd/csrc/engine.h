@@ -155,6 +155,10 @@ struct Context {
   int max_subspace_iter = 40;
   double subspace_tol = 1e-12;
   int cheb_degree = 8;      // largest Chebyshev filter degree per subspace step (1: plain power step)
+  // Boundary sector budget (BoundaryMPS(chi, sector_space)): at most bcap[q]
+  // kept values of bond parity q per cut, intersected with truncrank(chi);
+  // negative: no budget.
+  int bcap[2] = {-1, -1};
   Arena arena;
   Counters counters;
   std::string last_error;

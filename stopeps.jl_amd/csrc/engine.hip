@@ -946,15 +946,17 @@ __global__ void trace_kernel(const double* rho, long long pstride, int ld, const
 }
 
 // Select the top chi eigenpairs over both parity blocks (truncrank, ties to
-// parity 0 and lower index), scatter them into U[k,p,c] (dense, zeroed
-// beforehand) with the new bond parity-sorted, and record the new even count,
-// the discarded weight fraction and its running maximum.
+// parity 0 and lower index) intersected with the sector budget caps0/caps1
+// (truncspace; negative: none), scatter them into U[k,p,c] (dense, zeroed
+// beforehand, bond of capacity bcap) with the new bond parity-sorted, and
+// record the new even count, the discarded weight fraction and its running
+// maximum. cap bounds the ranked count (the rank of the cut).
 template <typename T>
 __global__ void truncate_select_kernel(Group gkp, const double* w, long long wstride, const double* v,
                                        long long vstride, int ldv, const int* avail_of, const int* n_of,
-                                       const double* tr, int chi, int cap, T* U, long long ustride,
-                                       long long ustride_k, long long ustride_p, int* new_even, double* maxtrunc,
-                                       double* discarded_out) {
+                                       const double* tr, int chi, int cap, int caps0, int caps1, int bcap, T* U,
+                                       long long ustride, long long ustride_k, long long ustride_p, int* new_even,
+                                       double* maxtrunc, double* discarded_out) {
   const int s = blockIdx.x;
   __shared__ int m0s, m1s;
   __shared__ GroupView g;
@@ -966,15 +968,21 @@ __global__ void truncate_select_kernel(Group gkp, const double* w, long long wst
     int a1 = min(avail_of[2 * s + 1], n_of[2 * s + 1]);
     int want = min(chi, min(cap, a0 + a1));
     int i0 = 0, i1 = 0;
-    double kept = 0.0;
     while (i0 + i1 < want) {
       bool take0;
       if (i0 >= a0) take0 = false;
       else if (i1 >= a1) take0 = true;
       else take0 = (w0[i0] >= w1[i1]);
-      if (take0) kept += fmax(w0[i0++], 0.0);
-      else kept += fmax(w1[i1++], 0.0);
+      if (take0) ++i0;
+      else ++i1;
     }
+    if (caps0 >= 0) i0 = min(i0, caps0);
+    if (caps1 >= 0) i1 = min(i1, caps1);
+    i1 = min(i1, bcap);  // the bond holds both parities' kept columns
+    i0 = min(i0, bcap - i1);
+    double kept = 0.0;
+    for (int i = 0; i < i0; ++i) kept += fmax(w0[i], 0.0);
+    for (int i = 0; i < i1; ++i) kept += fmax(w1[i], 0.0);
     m0s = i0;
     m1s = i1;
     double total = tr[2 * s] + tr[2 * s + 1];
@@ -983,7 +991,7 @@ __global__ void truncate_select_kernel(Group gkp, const double* w, long long wst
     double frac = total > 0.0 ? disc / total : 0.0;
     if (frac > maxtrunc[s]) maxtrunc[s] = frac;
     if (discarded_out) discarded_out[s] = frac;
-    new_even[s] = cap - i1;
+    new_even[s] = bcap - i1;
   }
   __syncthreads();
   const int m0 = m0s, m1 = m1s;
@@ -992,7 +1000,7 @@ __global__ void truncate_select_kernel(Group gkp, const double* w, long long wst
   for (int q = 0; q < 2; ++q) {
     const int n = n_of[2 * s + q];
     const int m = q == 0 ? m0 : m1;
-    const int cbase = q == 0 ? 0 : cap - m1;
+    const int cbase = q == 0 ? 0 : bcap - m1;
     const double* vb = v + (long long)(2 * s + q) * vstride;
     for (long long i = threadIdx.x; i < (long long)n * m; i += blockDim.x) {
       int r = (int)(i / m), j = (int)(i % m);
@@ -1353,15 +1361,24 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
     }
     int cap = (int)std::min<long long>(chi, std::min(nq[0], kR[x + 1][0]) + std::min(nq[1], kR[x + 1][1]));
     if (cap < 1) cap = 1;
+    // bond capacity under the sector budget (the ranking still runs over the
+    // top cap; a parity keeps at most its row bound nq and its budget)
+    long long capq[2];
+    for (int q = 0; q < 2; ++q) {
+      capq[q] = nq[q];
+      if (ctx.bcap[q] >= 0) capq[q] = std::min<long long>(capq[q], ctx.bcap[q]);
+    }
+    int bcap = (int)std::min<long long>(cap, capq[0] + capq[1]);
+    if (bcap < 1) bcap = 1;
     EigResult er{};
     eig_topk(ctx, rho, rps, (int)nmax, kb, sp.s, ncount, (int)nmax, nb, cap, er);
     // U[k,p,c] into the output boundary storage
-    long long us = (long long)kb.dim * sp.s.dim * cap;
+    long long us = (long long)kb.dim * sp.s.dim * bcap;
     T* U = out.t[x];
     SPP_CHECK(launch_fill<T>(U, nb * us, T(0), st));
     hipLaunchKernelGGL(truncate_select_kernel<T>, dim3(nb), dim3(256), 0, st, gkp, er.w, er.wstride, er.v,
-                       er.vstride, er.ldv, er.avail, ncount, tr, chi, cap, U, us, (long long)sp.s.dim * cap,
-                       (long long)cap, out.bsplit[x + 1], out.max_trunc, nullptr);
+                       er.vstride, er.ldv, er.avail, ncount, tr, chi, cap, ctx.bcap[0], ctx.bcap[1], bcap, U, us,
+                       (long long)sp.s.dim * bcap, (long long)bcap, out.bsplit[x + 1], out.max_trunc, nullptr);
     SPP_CHECK(hipGetLastError());
     if (out.entropy || out.entropy_max) {
       // pre-truncation spectrum of this cut (eigenvalues of rho)
@@ -1373,11 +1390,11 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
     out.bdim[x] = kb.dim;
     out.beven[x] = kb.even;
     Leg nbond;
-    nbond.dim = cap;
+    nbond.dim = bcap;
     nbond.split = out.bsplit[x + 1];
-    nbond.even = cap;
+    nbond.even = bcap;
     // R = U^T Y -> (c, r, e)
-    long long rs = (long long)cap * br.dim * sp.e.dim;
+    long long rs = (long long)bcap * br.dim * sp.e.dim;
     contract<T>(ctx, dense_ref(Rn, rs, "cre", {nbond, br, sp.e}), dense_ref(U, us, "kpc", {kb, sp.s, nbond}),
                 Yr, nb);
     ctx.arena.release(xm);

@@ -66,7 +66,8 @@ def run_purification(c, ctx=None):
     history = [{"phase": "initialization", "beta": 0.0}]
     dev, rows = purify(ctx, pepo, c.model, c.beta0, c.beta_log, c.beta_measure, c.delta_tau_pur, c.chi_environment,
                        c.chi_measurement, cap, c.update, c.trotter_order, D=c.D_pur,
-                       record_entropy=c.record_boundary_entropy)
+                       record_entropy=c.record_boundary_entropy, environment_space=c.environment_space,
+                       measurement_space=c.measurement_space)
     for r in rows:
         r.update({"eta": float("nan"), "eta_error": float("nan")})
         out["observables"]["purification"].append(r)
@@ -117,7 +118,9 @@ def run_stochastic(c, checkpoint, ctx=None, uniforms=None):
                                        c.Ns, c.seed, c.chi_sampling, c.chi_environment, c.chi_measurement,
                                        bond_space=cap, method=c.update, order=c.trotter_order,
                                        basis=sampling_basis(c), uniforms=uniforms, on_node=on_node, D=c.D_peps,
-                                       draws_out=draws, record_entropy=c.record_boundary_entropy)
+                                       draws_out=draws, record_entropy=c.record_boundary_entropy,
+                                       environment_space=c.environment_space, sampling_space=c.sampling_space,
+                                       measurement_space=c.measurement_space)
     out["diagnostics"]["born_draws"] = [
         {"index": i + 1, "configuration": draws["configuration"][i], "log_probability": float(draws["log_probability"][i]),
          "max_environment_truncation": float(draws["max_environment_truncation"][i])}

@@ -77,7 +77,8 @@ def _gates(model, lattice, delta, ppar):
     return gh, gv
 
 
-def evolution_steps(ctx, states, model, delta, chi, method=None, order=2, D=None, record_entropy=False):
+def evolution_steps(ctx, states, model, delta, chi, method=None, order=2, D=None, record_entropy=False,
+                    boundary_space=None):
     """``evolution_step!(state, model, delta, D, FullUpdate(...), BoundaryMPS(chi);
     order, sector_space, record_entropy)`` for every uploaded state (``DeviceState``; same lattice
     and leg spaces up to the auxiliary charge), in place on the device.
@@ -88,7 +89,8 @@ def evolution_steps(ctx, states, model, delta, chi, method=None, order=2, D=None
     stop, kept_even, kept_odd), normalizations [n, Lx*Ly],
     max_environment_truncation [n], max_boundary_entropy [n] (the largest
     pre-truncation Schmidt entropy over every boundary cut of the step; NaN
-    unless ``record_entropy``)."""
+    unless ``record_entropy``). ``boundary_space``: the sector budget of the
+    environment boundaries (``BoundaryMPS(chi, sector_space)``)."""
     method = method or FullUpdate()
     devs = list(states)
     n = len(devs)
@@ -114,19 +116,21 @@ def evolution_steps(ctx, states, model, delta, chi, method=None, order=2, D=None
     ent = np.zeros(n) if record_entropy else None
     handles = (ctypes.c_void_p * n)(*[d.handle.value for d in devs])
     o = method.options()
-    _lib.check(ctx.handle, ctx._lib.spp_fu_evolution_step(
-        ctx.handle, n, handles, int(chi), int(order), float(delta), int(D or 0), _lib.dptr(o), _lib.dptr(gh),
-        _lib.dptr(gv),
-        _lib.dptr(rec), _lib.dptr(norms), _lib.dptr(mt), _lib.dptr(ent)))
+    with ctx.boundary_space(boundary_space):
+        rc = ctx._lib.spp_fu_evolution_step(
+            ctx.handle, n, handles, int(chi), int(order), float(delta), int(D or 0), _lib.dptr(o), _lib.dptr(gh),
+            _lib.dptr(gv), _lib.dptr(rec), _lib.dptr(norms), _lib.dptr(mt), _lib.dptr(ent))
+    _lib.check(ctx.handle, rc)
     for d in devs:
         d.state.beta += 2 * delta
     return {"records": rec, "normalizations": norms, "max_environment_truncation": mt,
             "max_boundary_entropy": ent if ent is not None else np.full(n, np.nan)}
 
 
-def evolution_step(ctx, state, model, delta, chi, method=None, order=2, D=None, record_entropy=False):
+def evolution_step(ctx, state, model, delta, chi, method=None, order=2, D=None, record_entropy=False,
+                   boundary_space=None):
     """One state: the report of :func:`evolution_steps` for it."""
-    r = evolution_steps(ctx, [state], model, delta, chi, method, order, D, record_entropy)
+    r = evolution_steps(ctx, [state], model, delta, chi, method, order, D, record_entropy, boundary_space)
     recs = r["records"][0]
     return {"records": [{"initial_cost": float(v[0]), "final_cost": float(v[1]), "residual": float(v[2]),
                          "iterations": int(v[3]), "iteration_limit_reached": bool(v[4]),

@@ -112,7 +112,7 @@ def _entropy_outputs(out, prof, emax):
         out["max_boundary_entropy"] = emax
 
 
-def measure_energies(cache, configurations, model, chi=None, check=True, record_entropy=False):
+def measure_energies(cache, configurations, model, chi=None, check=True, record_entropy=False, sector_space=None):
     """Batched ``measure_energy(project_configuration(pepo, c), model,
     BoundaryMPS(chi); record_entropy)`` for configurations [n, Lx, Ly] of basis
     labels. ``chi`` defaults to the cache's (sampling) chi. Returns a dict of
@@ -120,7 +120,8 @@ def measure_energies(cache, configurations, model, chi=None, check=True, record_
     max_environment_truncation, status, fail_site, and per sample ``Se``
     ({"north", "south"} pre-truncation Schmidt entropies [Ly-1, Lx-1], or None)
     and max_boundary_entropy (the maximum over every boundary cut; NaN when
-    not recorded)."""
+    not recorded). ``sector_space``: the boundary sector budget of the
+    measurement method (``BoundaryMPS(chi, sector_space)``)."""
     L = cache.lattice
     ns = L.Lx * L.Ly
     cfg = np.asarray(configurations)
@@ -136,12 +137,14 @@ def measure_energies(cache, configurations, model, chi=None, check=True, record_
     fail = np.full(n, -1, dtype=np.int32)
     prof, emax = _entropy_buffers(L, n, record_entropy)
     ctx = cache.ctx
-    _lib.check(ctx.handle, ctx._lib.spp_measure_energy_batch(
-        ctx.handle, cache.handle, cache._dev.handle, int(chi if chi is not None else cache.chi), n,
-        _lib.iptr(labels), _lib.dptr(X), _lib.dptr(F), _lib.dptr(G), odd, model.J, model.h,
-        _lib.dptr(out["energy_total"]), _lib.dptr(out["energy_per_site"]), _lib.dptr(out["log_norm"]),
-        _lib.dptr(out["tensor_log_norm"]), _lib.dptr(out["max_environment_truncation"]), _lib.iptr(status),
-        _lib.iptr(fail), _lib.dptr(prof), _lib.dptr(emax)))
+    with ctx.boundary_space(sector_space):
+        rc = ctx._lib.spp_measure_energy_batch(
+            ctx.handle, cache.handle, cache._dev.handle, int(chi if chi is not None else cache.chi), n,
+            _lib.iptr(labels), _lib.dptr(X), _lib.dptr(F), _lib.dptr(G), odd, model.J, model.h,
+            _lib.dptr(out["energy_total"]), _lib.dptr(out["energy_per_site"]), _lib.dptr(out["log_norm"]),
+            _lib.dptr(out["tensor_log_norm"]), _lib.dptr(out["max_environment_truncation"]), _lib.iptr(status),
+            _lib.iptr(fail), _lib.dptr(prof), _lib.dptr(emax))
+    _lib.check(ctx.handle, rc)
     if check:
         _raise_status(L, status, fail)
     out["status"] = status
@@ -150,17 +153,18 @@ def measure_energies(cache, configurations, model, chi=None, check=True, record_
     return out
 
 
-def measure_energy(cache, configuration, model, chi=None, record_entropy=False):
+def measure_energy(cache, configuration, model, chi=None, record_entropy=False, sector_space=None):
     """One sample: the NamedTuple fields of ``measure_energy`` (energy.jl:73-77)
     as a dict (``Se`` is None unless ``record_entropy``)."""
-    r = measure_energies(cache, np.asarray(configuration)[None], model, chi, record_entropy=record_entropy)
+    r = measure_energies(cache, np.asarray(configuration)[None], model, chi, record_entropy=record_entropy,
+                         sector_space=sector_space)
     return {"energy_total": float(r["energy_total"][0]), "energy_per_site": float(r["energy_per_site"][0]),
             "log_norm": float(r["log_norm"][0]), "tensor_log_norm": float(r["tensor_log_norm"][0]),
             "max_environment_truncation": float(r["max_environment_truncation"][0]), "Se": r["Se"][0],
             "max_boundary_entropy": float(r["max_boundary_entropy"][0])}
 
 
-def measure_energy_states(ctx, states, model, chi, check=True, record_entropy=False):
+def measure_energy_states(ctx, states, model, chi, check=True, record_entropy=False, sector_space=None):
     """Batched ``measure_energy(state, model, BoundaryMPS(chi); record_entropy)``
     (energy.jl:31-78) of resident states (``DeviceState``, or host states that
     are uploaded for the call): PEPS or PEPO on one lattice with the same leg
@@ -183,11 +187,13 @@ def measure_energy_states(ctx, states, model, chi, check=True, record_entropy=Fa
     prof, emax = _entropy_buffers(L, n, record_entropy)
     X, F, G, odd = _device_operators(model, devs[0].state[(0, 0)].parity[2])
     handles = (ctypes.c_void_p * n)(*[d.handle.value for d in devs])
-    _lib.check(ctx.handle, ctx._lib.spp_measure_energy_states(
-        ctx.handle, n, handles, int(chi), _lib.dptr(X), _lib.dptr(F), _lib.dptr(G), odd, model.J, model.h,
-        _lib.dptr(out["energy_total"]), _lib.dptr(out["energy_per_site"]), _lib.dptr(out["log_norm"]),
-        _lib.dptr(out["tensor_log_norm"]), _lib.dptr(out["max_environment_truncation"]), _lib.iptr(status),
-        _lib.iptr(fail), _lib.dptr(prof), _lib.dptr(emax)))
+    with ctx.boundary_space(sector_space):
+        rc = ctx._lib.spp_measure_energy_states(
+            ctx.handle, n, handles, int(chi), _lib.dptr(X), _lib.dptr(F), _lib.dptr(G), odd, model.J, model.h,
+            _lib.dptr(out["energy_total"]), _lib.dptr(out["energy_per_site"]), _lib.dptr(out["log_norm"]),
+            _lib.dptr(out["tensor_log_norm"]), _lib.dptr(out["max_environment_truncation"]), _lib.iptr(status),
+            _lib.iptr(fail), _lib.dptr(prof), _lib.dptr(emax))
+    _lib.check(ctx.handle, rc)
     if check:
         _raise_status(L, status, fail)
     out["status"] = status

@@ -170,15 +170,15 @@ def nan_max(values, init=None):
     return math.nan if any(math.isnan(v) for v in vals) else max(vals)
 
 
-def observe(ctx, states, model, chi, energy, record_entropy=False):
+def observe(ctx, states, model, chi, energy, record_entropy=False, sector_space=None):
     """``_observe`` (particles.jl:35-41) for a batch of resident states:
     ``measure_energy`` (with the ``Se`` profiles when ``record_entropy``) when
     ``energy``, else ``lognorm`` with NaN energies and no ``Se``."""
     t0 = time.perf_counter()
     if energy:
-        r = measure_energy_states(ctx, states, model, chi, record_entropy=record_entropy)
+        r = measure_energy_states(ctx, states, model, chi, record_entropy=record_entropy, sector_space=sector_space)
     else:
-        r = lognorms(ctx, states, chi)
+        r = lognorms(ctx, states, chi, sector_space=sector_space)
         r["energy_total"] = np.full(len(states), np.nan)
         r["energy_per_site"] = np.full(len(states), np.nan)
         r["Se"] = [None] * len(states)
@@ -187,7 +187,7 @@ def observe(ctx, states, model, chi, energy, record_entropy=False):
 
 
 def new_particles(ctx, cache, draws, model, chi_measurement, energy, bond_space=None, pepo_live=None, seed=0,
-                  first_id=1, record_entropy=False):
+                  first_id=1, record_entropy=False, measurement_space=None):
     """``_initial_particles`` after the draws (run.jl:45-66) with ``_new_particle``
     (particles.jl:42-53) for each distinct configuration: project the PEPO
     (host slice + ``normalize_state!``), upload it as a resident PEPS (bonds
@@ -217,9 +217,9 @@ def new_particles(ctx, cache, draws, model, chi_measurement, energy, bond_space=
     t0 = time.perf_counter()
     if energy:
         m = measure_energies(cache, draws["configuration"][reps], model, chi_measurement,
-                             record_entropy=record_entropy)
+                             record_entropy=record_entropy, sector_space=measurement_space)
     else:
-        m = lognorms(ctx, states, chi_measurement)
+        m = lognorms(ctx, states, chi_measurement, sector_space=measurement_space)
         m["energy_total"] = np.full(len(reps), np.nan)
         m["energy_per_site"] = np.full(len(reps), np.nan)
         m["Se"] = [None] * len(reps)
@@ -242,7 +242,7 @@ def new_particles(ctx, cache, draws, model, chi_measurement, energy, bond_space=
 
 
 def advance_particles(ctx, particles, model, steps, chi_environment, chi_measurement, method, energy, order=2,
-                      on_step=None, D=None, record_entropy=False):
+                      on_step=None, D=None, record_entropy=False, environment_space=None, measurement_space=None):
     """``_advance_particle!`` (particles.jl:86-102) with ``_evolve!``
     (particles.jl:66-85) for a batch of particles at the same beta: one
     full-update step per endpoint in ``steps`` (delta = (beta - state.beta)/2),
@@ -266,7 +266,7 @@ def advance_particles(ctx, particles, model, steps, chi_environment, chi_measure
             raise ValueError("particles of one batch must share beta")
         t0 = time.perf_counter()
         r = evolution_steps(ctx, devs, model, (beta - b0) / 2, chi_environment, method, order, D,
-                            record_entropy=record_entropy)
+                            record_entropy=record_entropy, boundary_space=environment_space)
         elapsed = time.perf_counter() - t0
         for d in devs:
             d.state.beta = beta
@@ -278,7 +278,7 @@ def advance_particles(ctx, particles, model, steps, chi_environment, chi_measure
         total += elapsed
         if on_step is not None:
             on_step({"beta": beta, "time_seconds": elapsed, **r})
-    m = observe(ctx, devs, model, chi_measurement, energy, record_entropy)
+    m = observe(ctx, devs, model, chi_measurement, energy, record_entropy, measurement_space)
     for k, p in enumerate(particles):
         old = p.tensor_log_norm
         p.relative_log_weight += 2.0 * float(normalization[k]) + (float(m["tensor_log_norm"][k]) - old)
@@ -320,7 +320,7 @@ def statistics(records, Ns, nsites):
 # ----------------------------------------------------------------------------
 
 def purify(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, chi_environment, chi_measurement, bond_space,
-           method=None, order=2, D=None, record_entropy=False):
+           method=None, order=2, D=None, record_entropy=False, environment_space=None, measurement_space=None):
     """The evolution of ``run_purification`` (run.jl:2-43) from the identity PEPO
     (``identity_pepo``; normalised here as the reference does), on the device:
     FU steps to every node <= beta0 and an observation there. Returns the
@@ -336,7 +336,8 @@ def purify(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, chi_envir
         steps = step_schedule(dev.state.beta, beta, delta_tau)
         p = Particle(state=dev, tensor_log_norm=0.0, relative_log_weight=0.0)
         advance_particles(ctx, [p], model, steps, chi_environment, chi_measurement, method,
-                          measure_at(beta_measure, beta), order, D=D, record_entropy=record_entropy)
+                          measure_at(beta_measure, beta), order, D=D, record_entropy=record_entropy,
+                          environment_space=environment_space, measurement_space=measurement_space)
         rows.append({"beta": beta, "source": "purification", "energy_total": p.energy_total,
                      "energy_per_site": p.energy_per_site,
                      "energy_per_site_error": 0.0 if measure_at(beta_measure, beta) else math.nan,
@@ -349,7 +350,8 @@ def purify(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, chi_envir
 
 def stochastic_ensemble(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, Ns, seed, chi_sampling,
                         chi_environment, chi_measurement, bond_space=None, method=None, order=2, basis=None,
-                        uniforms=None, on_node=None, D=None, draws_out=None, record_entropy=False):
+                        uniforms=None, on_node=None, D=None, draws_out=None, record_entropy=False,
+                        environment_space=None, sampling_space=None, measurement_space=None):
     """The ensemble of ``run_stochastic`` (run.jl:87-130) without resampling: Ns Born draws
     from the beta0 PEPO (``pepo``: a host ``FiniteState`` or a resident
     ``DeviceState``), one particle per distinct configuration, then FU
@@ -367,7 +369,7 @@ def stochastic_ensemble(ctx, pepo, model, beta0, beta_log, beta_measure, delta_t
         raise ValueError("checkpoint beta0 mismatch")
     L = host.lattice
     ns = L.Lx * L.Ly
-    cache = SamplingCache(ctx, host, chi_sampling, basis=resolve_basis(host, basis))
+    cache = SamplingCache(ctx, host, chi_sampling, basis=resolve_basis(host, basis), sector_space=sampling_space)
     if uniforms is None:
         uniforms = np.stack([sample_uniforms(seed, i, ns) for i in range(1, Ns + 1)])
     draws = sample_configurations(cache, uniforms=uniforms)
@@ -376,13 +378,15 @@ def stochastic_ensemble(ctx, pepo, model, beta0, beta_log, beta_measure, delta_t
     nodes = [b for b in temperature_nodes(beta0, beta_log, beta_measure) if b >= beta0]
     particles, grouping = new_particles(ctx, cache, draws, model, chi_measurement,
                                         measure_at(beta_measure, nodes[0]), bond_space=bond_space,
-                                        pepo_live=live, seed=seed, record_entropy=record_entropy)
+                                        pepo_live=live, seed=seed, record_entropy=record_entropy,
+                                        measurement_space=measurement_space)
     rows = []
     for node, beta in enumerate(nodes):
         if node > 0:
             steps = step_schedule(nodes[node - 1], beta, delta_tau)
             advance_particles(ctx, particles, model, steps, chi_environment, chi_measurement, method,
-                              measure_at(beta_measure, beta), order, D=D, record_entropy=record_entropy)
+                              measure_at(beta_measure, beta), order, D=D, record_entropy=record_entropy,
+                              environment_space=environment_space, measurement_space=measurement_space)
         records = [p.record() for p in particles]
         stats = statistics(records, Ns, ns)
         stats["source"] = "sampling"

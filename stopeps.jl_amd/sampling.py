@@ -7,6 +7,7 @@ that replaces the per-sample loop of ``_initial_particles``
 (src/driver/run.jl:45-66). All contraction work runs in the HIP library;
 this module only marshals arrays.
 """
+import contextlib
 import ctypes
 
 import numpy as np
@@ -56,6 +57,20 @@ class Context:
 
     def reset_counters(self):
         self._lib.spp_reset_counters(self.handle)
+
+    @contextlib.contextmanager
+    def boundary_space(self, space):
+        """Run the enclosed calls with the boundary sector budget of
+        ``BoundaryMPS(chi, space)`` (boundary_mps.jl:11-18): an (even, odd)
+        pair for a Z2 state, an int for a dense one, or None."""
+        caps = boundary_caps(space)
+        self.set_option("boundary_cap_even", caps[0])
+        self.set_option("boundary_cap_odd", caps[1])
+        try:
+            yield
+        finally:
+            self.set_option("boundary_cap_even", -1)
+            self.set_option("boundary_cap_odd", -1)
 
     def synchronize(self):
         _lib.check(self.handle, self._lib.spp_synchronize(self.handle))
@@ -162,11 +177,28 @@ class DeviceState:
             pass
 
 
-class SamplingCache:
-    """``SamplingCache(state, BoundaryMPS(chi); kind=:auxiliary, basis=:auto)``
-    (src/sampling/born.jl:45-73). The state must be a PEPO."""
+def boundary_caps(space):
+    """(even, odd) budgets of a boundary sector space; -1 = none. A dense
+    space n is one trivial sector, which the device holds as the even one."""
+    if space is None:
+        return (-1, -1)
+    if isinstance(space, (tuple, list)):
+        e, o = (int(v) for v in space)
+        if min(e, o) < 0 or e + o <= 0:
+            raise ValueError("a boundary sector space needs nonnegative budgets and a nonzero dimension")
+        return (e, o)
+    n = int(space)
+    if n <= 0:
+        raise ValueError("a boundary sector space needs a positive dimension")
+    return (n, -1)
 
-    def __init__(self, ctx, state, chi, basis=None, kind="auxiliary"):
+
+class SamplingCache:
+    """``SamplingCache(state, BoundaryMPS(chi, sector_space); kind=:auxiliary,
+    basis=:auto)`` (src/sampling/born.jl:45-73). The state must be a PEPO; the
+    draws from the cache compress with the same boundary method."""
+
+    def __init__(self, ctx, state, chi, basis=None, kind="auxiliary", sector_space=None):
         if kind != "auxiliary":
             raise NotImplementedError("physical-kind sampling (resampling proposals) is not part of this library")
         if state.kind != "pepo":
@@ -174,13 +206,16 @@ class SamplingCache:
         self.ctx = ctx
         self.state = state
         self.chi = int(chi)
+        self.sector_space = sector_space
+        boundary_caps(sector_space)
         self.basis = resolve_basis(state, basis)
         self.lattice = state.lattice
         self._dev = DeviceState(ctx, state)
         U = np.ascontiguousarray(self.basis.U, dtype=np.float64)
         h = ctypes.c_void_p()
-        _lib.check(ctx.handle, ctx._lib.spp_sampling_cache_build(ctx.handle, self._dev.handle, self.chi,
-                                                                 _lib.dptr(U), ctypes.byref(h)))
+        with ctx.boundary_space(sector_space):
+            _lib.check(ctx.handle, ctx._lib.spp_sampling_cache_build(ctx.handle, self._dev.handle, self.chi,
+                                                                     _lib.dptr(U), ctypes.byref(h)))
         self.handle = h
         smt = np.zeros(self.lattice.Ly + 1)
         ctx._lib.spp_sampling_cache_info(self.handle, _lib.dptr(smt))
@@ -264,9 +299,10 @@ def sample_configurations(cache, uniforms=None, configurations=None, check=True)
     status = np.zeros(n, dtype=np.int32)
     fail = np.zeros(n, dtype=np.int32)
     ctx = cache.ctx
-    _lib.check(ctx.handle, ctx._lib.spp_born_sample_batch(
-        ctx.handle, cache.handle, n, _lib.dptr(uni), _lib.iptr(labels_in), _lib.iptr(labels), _lib.dptr(logp),
-        _lib.dptr(cond), _lib.dptr(mt), _lib.dptr(tln), _lib.dptr(lamp), _lib.iptr(status), _lib.iptr(fail)))
+    with ctx.boundary_space(cache.sector_space):
+        _lib.check(ctx.handle, ctx._lib.spp_born_sample_batch(
+            ctx.handle, cache.handle, n, _lib.dptr(uni), _lib.iptr(labels_in), _lib.iptr(labels), _lib.dptr(logp),
+            _lib.dptr(cond), _lib.dptr(mt), _lib.dptr(tln), _lib.dptr(lamp), _lib.iptr(status), _lib.iptr(fail)))
     if check:
         _raise_status(cache, status, fail)
     lab = np.asarray(cache.basis.labels)[labels].reshape(n, L.Ly, L.Lx).transpose(0, 2, 1)

@@ -260,6 +260,31 @@ def test_boundary_entropy_profile_is_exact_schmidt_entropy(kind, seed):
     assert r["max_boundary_entropy"] == pytest.approx(max(Se["north"].max(), Se["south"].max()), abs=1e-14)
 
 
+def test_boundary_sector_budget():
+    """``BoundaryMPS(chi, sector_space)`` (boundary_mps.jl:11-18): a budget of
+    chi per parity changes nothing, an active one bounds every bond's sector
+    counts and leaves the total within chi."""
+    L = Lattice(3, 3)
+    st = random_state(L, (2, 1), "pepo", 31)
+    layers = bm.double_layers(st)
+    plain = bm.boundary_environments(layers, 8)
+    wide = bm.boundary_environments(layers, bm.BoundaryMPS(8, (8, 8)))
+    for a, b in zip(plain[0] + plain[1], wide[0] + wide[1]):
+        for x, y in zip(a.tensors, b.tensors):
+            assert np.array_equal(x.a, y.a)
+    north, south, _ = bm.boundary_environments(layers, bm.BoundaryMPS(8, (4, 2)))
+    narrowed = False
+    for env in north[1:] + south[:-1]:
+        for A in env.tensors[1:]:
+            p = A.p[0]
+            assert np.sum(p == 0) <= 4 and np.sum(p == 1) <= 2 and len(p) <= 8
+            narrowed |= len(p) == 6
+    assert narrowed  # the unbudgeted bonds here are (4, 4) and (5, 3)
+    ln_plain = bm.lognorm(st, 8)["log_norm"]
+    ln_caps = bm.lognorm(st, bm.BoundaryMPS(8, (4, 2)))["log_norm"]
+    assert abs(ln_caps - ln_plain) > 1e-4
+
+
 # ------------------------------ truncation rules -----------------------------
 
 def test_select_kept_truncrank_and_intersection():
