@@ -139,3 +139,14 @@ def test_fu_step_max_entropy(spp, gpu_ctx):
         ref = ofu.evolution_step(o, m, 0.05, 2, (1, 1), opts, 8, record_entropy=True)
         say(f"  state {k}: {got['max_boundary_entropy'][k]:.12f} vs {ref['max_boundary_entropy']:.12f}")
         assert abs(got["max_boundary_entropy"][k] - ref["max_boundary_entropy"]) <= 1e-7
+
+
+def test_entropy_profiles_dense_model(spp, gpu_ctx):
+    """Se on a dense (non-symmetric) TFIM state: one trivial sector, so the
+    spectrum sits in a single block per cut."""
+    say("boundary entropy of a dense-model PEPO")
+    m = TFIM(J=0.8, h=1.1, symmetric=False)
+    states = [random_state(Lattice(3, 3), 2, "pepo", seed=95, symmetric=False)]
+    devs = [spp.DeviceState(gpu_ctx, to_product_state(spp, s)) for s in states]
+    got = spp.measure_energy_states(gpu_ctx, devs, spp.TFIM(m.J, m.h, symmetric=False), 6, record_entropy=True)
+    _check_profiles(got, [oe.measure_energy(s, m, 6, record_entropy=True) for s in states])

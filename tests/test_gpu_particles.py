@@ -13,6 +13,7 @@ import pytest
 
 from oracle.model import Lattice, TFIM, identity_pepo, normalize_state
 from oracle import born as ob
+from oracle import boundary as obm
 from oracle import energy as oe
 from oracle import ensemble as oens
 from oracle import fu as ofu
@@ -32,8 +33,9 @@ def say(*a):
     sys.stdout.flush()
 
 
-def oracle_pipeline():
+def oracle_pipeline(bspace=None):
     lat = Lattice(*L3)
+    chi = CHI if bspace is None else obm.BoundaryMPS(CHI, bspace)
     m = TFIM(J=1.0, h=H, symmetric=True)
     opts = ofu.FullUpdate(max_iterations=40, linear_max_iterations=100)
     D = sum(CAPS)
@@ -43,13 +45,13 @@ def oracle_pipeline():
     b = 0.0
     pur_entropy = 0.0
     while b < BETA0 - 1e-12:
-        rep = ofu.evolution_step(pepo, m, DT, D, CAPS, opts, CHI, record_entropy=True)
+        rep = ofu.evolution_step(pepo, m, DT, D, CAPS, opts, chi, record_entropy=True)
         pur_entropy = max(pur_entropy, rep["max_boundary_entropy"])
         b += 2 * DT
     pepo.beta = BETA0
-    pur.append(oe.measure_energy(pepo, m, CHI, record_entropy=True))
+    pur.append(oe.measure_energy(pepo, m, chi, record_entropy=True))
     pur[-1]["fu_entropy"] = pur_entropy
-    cache = ob.SamplingCache(pepo, CHI)
+    cache = ob.SamplingCache(pepo, chi)
     ns = lat.Lx * lat.Ly
     draws = [ob.sample_configuration(cache, uniforms=ob.counter_uniforms(1, i, ns)) for i in range(1, NS + 1)]
     cfgs = [d["configuration"] for d in draws]
@@ -57,7 +59,7 @@ def oracle_pipeline():
     parts = []
     for u, i in enumerate(grp["first_draw"]):
         ps, _ = ob.project_configuration(pepo, cfgs[i])
-        r = oe.measure_energy(ps, m, CHI, record_entropy=True)
+        r = oe.measure_energy(ps, m, chi, record_entropy=True)
         parts.append({"state": ps, "rel": r["log_norm"] - draws[i]["log_probability"],
                       "tln": r["tensor_log_norm"], "e": r["energy_per_site"], "count": grp["multiplicity"][u],
                       "Se": r["Se"]})
@@ -68,11 +70,11 @@ def oracle_pipeline():
         bb = BETA0
         p["fu_entropy"] = 0.0
         while bb < BETA1 - 1e-12:
-            rep = ofu.evolution_step(p["state"], m, DT, D, CAPS, opts, CHI, record_entropy=True)
+            rep = ofu.evolution_step(p["state"], m, DT, D, CAPS, opts, chi, record_entropy=True)
             norm += sum(x["log_normalization"] for x in rep["normalizations"])
             p["fu_entropy"] = max(p["fu_entropy"], rep["max_boundary_entropy"])
             bb += 2 * DT
-        r = oe.measure_energy(p["state"], m, CHI, record_entropy=True)
+        r = oe.measure_energy(p["state"], m, chi, record_entropy=True)
         p["Se"] = r["Se"]
         p["rel"] += 2 * norm + (r["tensor_log_norm"] - p["tln"])
         p["tln"] = r["tensor_log_norm"]
@@ -82,18 +84,20 @@ def oracle_pipeline():
     return pur, cfgs, grp, parts, stats
 
 
-def test_stochastic_pipeline_matches_oracle(spp, gpu_ctx):
-    say("stochastic PEPS pipeline 3x3 Z2 D=(1,1) chi=16: RunConfig -> run_purification -> run_stochastic")
+@pytest.mark.parametrize("bspace", [None, (2, 1)])
+def test_stochastic_pipeline_matches_oracle(spp, gpu_ctx, bspace):
+    say(f"stochastic PEPS pipeline 3x3 Z2 D=(1,1) chi=16, boundary space {bspace}: "
+        "RunConfig -> run_purification -> run_stochastic")
     ctx = gpu_ctx
     ctx.set_precision("fp64")
     c = spp.RunConfig(lattice=spp.Lattice(*L3), model=spp.TFIM(1.0, H, symmetric=True), beta0=BETA0, beta_log=[],
                       beta_measure=[BETA0, BETA1], delta_tau_pur=DT, delta_tau_peps=DT, D_pur=sum(CAPS),
                       D_peps=sum(CAPS), bond_space_pur=CAPS, bond_space_peps=CAPS, chi_environment=CHI,
                       update=spp.FullUpdate(max_iterations=40, linear_max_iterations=100), Ns=NS, seed=1,
-                      resampling=spp.ResamplingOptions(enabled=False))
+                      resampling=spp.ResamplingOptions(enabled=False), environment_space=bspace)
     cp = spp.run_purification(c, ctx)
     rows = cp.purification
-    pur, cfgs, grp, parts, stats = oracle_pipeline()
+    pur, cfgs, grp, parts, stats = oracle_pipeline(bspace)
     assert [r["beta"] for r in rows] == [0.0, BETA0]
     assert abs(rows[-1]["energy_total"] - pur[0]["energy_total"]) < 1e-9
     assert abs(rows[-1]["lnZ"] - pur[0]["log_norm"]) < 1e-9
