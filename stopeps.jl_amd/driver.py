@@ -48,7 +48,9 @@ def _new_record(c, run_id, stage):
 
 
 def _context(ctx):
-    return ctx if ctx is not None else Context(0, "fp64")
+    """The given context, or an FP64 one with four lanes (full-update batches
+    split over four streams)."""
+    return ctx if ctx is not None else Context(0, "fp64", lanes=4)
 
 
 def run_purification(c, ctx=None):
