@@ -535,7 +535,10 @@ __global__ void random_block_kernel(double* q, long long pstride, int ld, int nc
     int r = (int)(i / ncol), c = (int)(i % ncol);
     double v = 0.0;
     if (r < n) {
-      unsigned long long h = mix64(((unsigned long long)prob << 40) ^ ((unsigned long long)r << 20) ^ (unsigned long long)c);
+      // seeded by the parity only: a sample's start block (and so its result)
+      // does not depend on where it sits in the batch
+      unsigned long long h =
+          mix64(((unsigned long long)(prob & 1) << 40) ^ ((unsigned long long)r << 20) ^ (unsigned long long)c);
       v = ((double)(h >> 11) * 0x1.0p-53) - 0.5;
     }
     b[(long long)r * ld + c] = v;
