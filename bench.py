@@ -127,16 +127,15 @@ def adjoint_timing(spp, reps=2, batch=8, chi=36):
             "batch": batch, "batch_s_per_state": tb / batch}
 
 
-def fu_timing(spp, batch=32, chi=32, J=1.0, h=2.0, delta=0.05, lanes=4):
+def fu_timing(spp, batch=16, chi=32, J=1.0, h=2.0, delta=0.05):
     """The full-update evolution workload: one second-order step (sweeps H->,
     V->, V<-, H<-; 224 bond updates) of a batch of 8x8 Z2 PEPS, D=6 (3+3),
     d=2, i.i.d. N(0,1) blocks scaled to unit norm (seeds 100, 101, ...), TFIM
     gates, boundary chi=32, FullUpdate with 40 ALS and 100 PCG iterations
-    (the options of the reference's example configurations), FP64, the batch
-    split over `lanes` streams (concurrent host threads). One timed step: a
-    step is tens of seconds and allocates only its scratch."""
+    (the options of the reference's example configurations), FP64. One timed
+    step: a step is tens of seconds and allocates only its scratch."""
     lat = spp.Lattice(8, 8)
-    ctx = spp.Context(0, "fp64", lanes=lanes)
+    ctx = spp.Context(0, "fp64")
     dev = [spp.DeviceState(ctx, spp.random_peps(lat, (3, 3), seed=100 + k)) for k in range(batch)]
     model = spp.TFIM(J, h, symmetric=True)
     method = spp.FullUpdate(max_iterations=40, error_tolerance=1e-8, diagonal_shift=1e-10,
@@ -151,9 +150,8 @@ def fu_timing(spp, batch=32, chi=32, J=1.0, h=2.0, delta=0.05, lanes=4):
     rec = r["records"]
     return {"value": batch / dt, "unit": "FU sample-steps/s", "dtype": "fp64",
             "workload": f"full-update step (2nd order, delta={delta}), 8x8 Z2 PEPS D=6 (3+3), d=2, chi={chi}, "
-                        f"TFIM J={J} h={h}, 40 ALS x 100 PCG iterations, {batch} states (seeds 100..) "
-                        f"over {lanes} streams",
-            "batch": batch, "lanes": lanes, "s_per_step": dt, "s_per_sample_step": dt / batch,
+                        f"TFIM J={J} h={h}, 40 ALS x 100 PCG iterations, {batch} states (seeds 100..)",
+            "batch": batch, "s_per_step": dt, "s_per_sample_step": dt / batch,
             "bond_updates_per_step": int(rec.shape[1]), "mean_als_iterations": float(rec[:, :, 3].mean()),
             "max_final_cost": float(rec[:, :, 1].max()),
             "max_environment_truncation": float(np.max(r["max_environment_truncation"])),
@@ -180,8 +178,7 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true",
                     help="time only the log-norm gradient workload (for rocprofv3)")
     ap.add_argument("--fu-only", action="store_true", help="time only the full-update workload")
-    ap.add_argument("--fu-batch", type=int, default=32, help="states per full-update step (0: skip)")
-    ap.add_argument("--fu-lanes", type=int, default=4, help="streams the full-update batch is split over")
+    ap.add_argument("--fu-batch", type=int, default=16, help="states per full-update step (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed loop, copy 256 MiB device to device (HBM counter calibration)")
     args = ap.parse_args()
@@ -196,8 +193,7 @@ def main():
         print(json.dumps({"adjoint": adjoint_timing(spp)}), flush=True)
         return
     if args.fu_only:
-        print(json.dumps({"full_update": fu_timing(spp, batch=max(args.fu_batch, 1), lanes=args.fu_lanes)}),
-              flush=True)
+        print(json.dumps({"full_update": fu_timing(spp, batch=max(args.fu_batch, 1))}), flush=True)
         return
     L = args.L
     lat = spp.Lattice(L, L)
@@ -308,7 +304,7 @@ def main():
     result["adjoint"] = adjoint_timing(spp)
     if args.fu_batch > 0:
         log("full-update step ...")
-        result["full_update"] = fu_timing(spp, batch=args.fu_batch, J=args.J, h=args.h, lanes=args.fu_lanes)
+        result["full_update"] = fu_timing(spp, batch=args.fu_batch, J=args.J, h=args.h)
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_baseline(L=L, chi=args.chi)

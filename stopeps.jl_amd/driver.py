@@ -48,9 +48,7 @@ def _new_record(c, run_id, stage):
 
 
 def _context(ctx):
-    """The given context, or an FP64 one with four lanes (full-update batches
-    split over four streams)."""
-    return ctx if ctx is not None else Context(0, "fp64", lanes=4)
+    return ctx if ctx is not None else Context(0, "fp64")
 
 
 def run_purification(c, ctx=None):

@@ -6,7 +6,6 @@ The sweeps, environments, bond metrics and bond solves run in
 ``spp_fu_evolution_step``; this module builds the gates and marshals arrays.
 """
 import ctypes
-import threading
 
 import numpy as np
 
@@ -115,38 +114,13 @@ def evolution_steps(ctx, states, model, delta, chi, method=None, order=2, D=None
     norms = np.zeros((n, L.Lx * L.Ly))
     mt = np.zeros(n)
     ent = np.zeros(n) if record_entropy else None
+    handles = (ctypes.c_void_p * n)(*[d.handle.value for d in devs])
     o = method.options()
-    # the batch is split over the context's lanes (part k holds states k,
-    # k + lanes, ...), each part one call on its own stream from its own host thread
-    lanes = ctx.lanes[:min(len(ctx.lanes), n)]
-    parts = [list(range(k, n, len(lanes))) for k in range(len(lanes))]
-    outs = [(np.zeros((len(p), nup, 8)), np.zeros((len(p), L.Lx * L.Ly)), np.zeros(len(p)),
-             np.zeros(len(p)) if record_entropy else None) for p in parts]
-    rcs = [0] * len(lanes)
-
-    def run(k):
-        lane, idx = lanes[k], parts[k]
-        handles = (ctypes.c_void_p * len(idx))(*[devs[i].handle.value for i in idx])
-        r, nm, m_, e_ = outs[k]
-        rcs[k] = lane._lib.spp_fu_evolution_step(
-            lane.handle, len(idx), handles, int(chi), int(order), float(delta), int(D or 0), _lib.dptr(o),
-            _lib.dptr(gh), _lib.dptr(gv), _lib.dptr(r), _lib.dptr(nm), _lib.dptr(m_), _lib.dptr(e_))
     with ctx.boundary_space(boundary_space):
-        if len(lanes) == 1:
-            run(0)
-        else:
-            threads = [threading.Thread(target=run, args=(k,)) for k in range(len(lanes))]
-            for t in threads:
-                t.start()
-            for t in threads:
-                t.join()
-    for k, lane in enumerate(lanes):
-        _lib.check(lane.handle, rcs[k])
-    for k, idx in enumerate(parts):
-        r, nm, m_, e_ = outs[k]
-        rec[idx], norms[idx], mt[idx] = r, nm, m_
-        if ent is not None:
-            ent[idx] = e_
+        rc = ctx._lib.spp_fu_evolution_step(
+            ctx.handle, n, handles, int(chi), int(order), float(delta), int(D or 0), _lib.dptr(o), _lib.dptr(gh),
+            _lib.dptr(gv), _lib.dptr(rec), _lib.dptr(norms), _lib.dptr(mt), _lib.dptr(ent))
+    _lib.check(ctx.handle, rc)
     for d in devs:
         d.state.beta += 2 * delta
     return {"records": rec, "normalizations": norms, "max_environment_truncation": mt,

@@ -24,51 +24,31 @@ class BornError(ValueError):
 
 
 class Context:
-    """One device context (one HIP stream) of the library.
+    """One device context (one HIP stream) of the library."""
 
-    ``lanes`` > 1 adds lanes - 1 further native contexts on the same device,
-    each with its own stream and workspace. Full-update steps split their
-    batch over the lanes and run them from concurrent host threads, so one
-    part's latency-bound bond solves overlap another part's contractions.
-    Precision and options apply to every lane."""
-
-    def __init__(self, device=0, precision="fp64", lanes=1, **options):
+    def __init__(self, device=0, precision="fp64", **options):
         self._lib = _lib.load()
         h = ctypes.c_void_p()
         rc = self._lib.spp_create(int(device), ctypes.byref(h))
         if rc != 0:
             raise _lib.HIPLibraryError(f"spp_create failed ({rc}): no usable HIP device")
         self.handle = h
-        if int(lanes) < 1:
-            raise ValueError("lanes must be positive")
-        self.lanes = [self] + [Context(device, precision, **options) for _ in range(int(lanes) - 1)]
         self.set_precision(precision)
         for k, v in options.items():
             self.set_option(k, v)
 
     def set_precision(self, precision):
-        for c in self.lanes:
-            c.precision = precision
-            _lib.check(c.handle, c._lib.spp_set_precision(c.handle, PRECISIONS[precision]))
+        self.precision = precision
+        _lib.check(self.handle, self._lib.spp_set_precision(self.handle, PRECISIONS[precision]))
 
     def set_option(self, name, value):
-        for c in self.lanes:
-            _lib.check(c.handle, c._lib.spp_set_option(c.handle, name.encode(), float(value)))
+        _lib.check(self.handle, self._lib.spp_set_option(self.handle, name.encode(), float(value)))
 
     def counters(self):
-        """The library counters, summed over the lanes (the workspace peak is
-        the largest lane's, the subspace iterations the first lane's)."""
         out = np.zeros(15)
-        for k, c in enumerate(self.lanes):
-            part = np.zeros(15)
-            rc = c._lib.spp_get_counters(c.handle, _lib.dptr(part), 15)
-            if rc < 0:
-                _lib.check(c.handle, rc)
-            if k == 0:
-                out[:] = part
-            else:
-                out[3] = max(out[3], part[3])
-                out[[0, 1] + list(range(4, 15))] += part[[0, 1] + list(range(4, 15))]
+        rc = self._lib.spp_get_counters(self.handle, _lib.dptr(out), 15)
+        if rc < 0:
+            _lib.check(self.handle, rc)
         return {"gemm_flops": out[0], "gemm_launches": out[1], "subspace_iterations": out[2],
                 "workspace_peak_bytes": out[3], "gemm_ms": out[4], "gemm_flops_profiled": out[5],
                 "fp64": {"launches": out[6], "ms": out[7], "flops": out[8]},
@@ -76,8 +56,7 @@ class Context:
                 "eig_calls": out[12], "eig_iterations": out[13], "eig_filter_gemms": out[14]}
 
     def reset_counters(self):
-        for c in self.lanes:
-            c._lib.spp_reset_counters(c.handle)
+        self._lib.spp_reset_counters(self.handle)
 
     @contextlib.contextmanager
     def boundary_space(self, space):
@@ -94,12 +73,9 @@ class Context:
             self.set_option("boundary_cap_odd", -1)
 
     def synchronize(self):
-        for c in self.lanes:
-            _lib.check(c.handle, c._lib.spp_synchronize(c.handle))
+        _lib.check(self.handle, self._lib.spp_synchronize(self.handle))
 
     def close(self):
-        for c in getattr(self, "lanes", [])[1:]:
-            c.close()
         if getattr(self, "handle", None):
             self._lib.spp_destroy(self.handle)
             self.handle = None

@@ -202,26 +202,3 @@ def test_fu_step_single_site(spp, gpu_ctx, kind):
     assert got["records"].shape == (1, 0, 8)
     assert abs(got["normalizations"][0, 0] - ref["normalizations"][0]["log_normalization"]) < 1e-13
     assert got["max_environment_truncation"][0] == 0.0 and got["max_boundary_entropy"][0] == 0.0
-
-
-def test_fu_step_lanes_match_one_stream(spp, gpu_ctx):
-    """A context with three lanes splits the batch over three streams and
-    host threads; every state's result equals the one-stream step."""
-    say("full-update step split over three lanes")
-    m = TFIM(J=1.0, h=1.4, symmetric=True)
-    ostates = [random_state(Lattice(3, 3), (1, 1), "peps", seed=50 + k) for k in range(5)]
-    model = spp.TFIM(m.J, m.h, symmetric=True)
-    method = spp.FullUpdate(20, 1e-8, 1e-10, 60, 1e-9)
-    one = [spp.DeviceState(gpu_ctx, to_product_state(spp, o)) for o in ostates]
-    r1 = spp.evolution_steps(gpu_ctx, one, model, 0.05, 16, method, record_entropy=True)
-    ctx3 = spp.Context(0, "fp64", lanes=3)
-    try:
-        three = [spp.DeviceState(ctx3, to_product_state(spp, o)) for o in ostates]
-        r3 = spp.evolution_steps(ctx3, three, model, 0.05, 16, method, record_entropy=True)
-        for k in range(len(ostates)):
-            v1, v3 = dense_vector(to_oracle(one[k].download())), dense_vector(to_oracle(three[k].download()))
-            assert np.linalg.norm(v1 - v3) <= 1e-12 * np.linalg.norm(v1)
-        for key in ("records", "normalizations", "max_environment_truncation", "max_boundary_entropy"):
-            np.testing.assert_allclose(r3[key], r1[key], rtol=1e-12, atol=1e-14)
-    finally:
-        ctx3.close()
