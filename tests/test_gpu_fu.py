@@ -202,3 +202,30 @@ def test_fu_step_single_site(spp, gpu_ctx, kind):
     assert got["records"].shape == (1, 0, 8)
     assert abs(got["normalizations"][0, 0] - ref["normalizations"][0]["log_normalization"]) < 1e-13
     assert got["max_environment_truncation"][0] == 0.0 and got["max_boundary_entropy"][0] == 0.0
+
+
+def test_results_do_not_depend_on_batch_mates(spp, gpu_ctx):
+    """With the subspace truncation eigensolver in use (ksub below the block
+    size), a state's lognorm and full-update step come out bit for bit the same
+    whether it is stepped alone or in a batch with other states."""
+    say("batch independence of the iterative truncation: lognorm and FU step")
+    m = TFIM(J=1.0, h=1.5, symmetric=True)
+    model = spp.TFIM(m.J, m.h, symmetric=True)
+    method = spp.FullUpdate(10, 1e-8, 1e-10, 40, 1e-9)
+    ostates = [random_state(Lattice(3, 3), (2, 1), "peps", seed=38 + k) for k in range(3)]
+    gpu_ctx.set_option("ksub", 12)
+    try:
+        batch = spp.lognorms(gpu_ctx, [to_product_state(spp, o) for o in ostates], 8)
+        alone = spp.lognorms(gpu_ctx, [to_product_state(spp, ostates[2])], 8)
+        assert batch["log_norm"][2] == alone["log_norm"][0]
+        devs = [spp.DeviceState(gpu_ctx, to_product_state(spp, o)) for o in ostates]
+        rb = spp.evolution_steps(gpu_ctx, devs, model, 0.05, 8, method)
+        solo = spp.DeviceState(gpu_ctx, to_product_state(spp, ostates[2]))
+        rs = spp.evolution_steps(gpu_ctx, [solo], model, 0.05, 8, method)
+    finally:
+        gpu_ctx.set_option("ksub", 64)
+    np.testing.assert_array_equal(rb["records"][2], rs["records"][0])
+    a, b = devs[2].download(), solo.download()
+    assert a.log_amplitude == b.log_amplitude
+    for xy in a.lattice.site_order():
+        np.testing.assert_array_equal(a[xy].data, b[xy].data)
