@@ -759,6 +759,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
       for (int i = 0; i < ns; ++i)
         for (int s = 0; s < nb; ++s) hptr[(size_t)i * nb + s] = site_base[i] + (long long)s * lsize[i];
       SPP_CHECK(hipMemcpyAsync(ptrs, hptr.data(), sizeof(void*) * hptr.size(), hipMemcpyHostToDevice, st));
+      SPP_CHECK(hipStreamSynchronize(st));  // hptr is released at the end of this block
     }
     auto table = [&](int site) { return (void* const*)(ptrs + (size_t)site * nb); };
     LayerJob* djobs = (LayerJob*)dev(sizeof(LayerJob) * (size_t)std::max(ns, 2) * nb);
@@ -826,6 +827,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
         for (int s = 0; s < cur; ++s)
           std::memcpy(hl.data() + (size_t)s * ns * 8, states[c0 + s]->live.data(), sizeof(int) * ns * 8);
         SPP_CHECK(hipMemcpyAsync(live, hl.data(), sizeof(int) * hl.size(), hipMemcpyHostToDevice, st));
+        SPP_CHECK(hipStreamSynchronize(st));  // hl is released at the end of this block
       }
       SPP_CHECK(launch_fill<int>(status, nb, 0, st));
       SPP_CHECK(launch_fill<int>(fail, nb, -1, st));
@@ -849,6 +851,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
         // the job table may still be read by queued kernels: wait before rewriting it
         SPP_CHECK(hipStreamSynchronize(st));
         SPP_CHECK(hipMemcpyAsync(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), hipMemcpyHostToDevice, st));
+        SPP_CHECK(hipStreamSynchronize(st));  // jobs is released when this call returns
         SPP_CHECK(launch_double_layers<T>(djobs, (int)jobs.size(), maxout, st));
       };
       {
@@ -1059,6 +1062,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
               SPP_CHECK(hipStreamSynchronize(st));
               SPP_CHECK(hipMemcpyAsync(pjobs, pjd.data(), sizeof(PairJob) * cur, hipMemcpyHostToDevice, st));
               SPP_CHECK(hipMemcpyAsync(rjobs, rj.data(), sizeof(RestoreJob) * cur, hipMemcpyHostToDevice, st));
+              SPP_CHECK(hipStreamSynchronize(st));  // pjd and rj are released at the end of this block
             }
             hipLaunchKernelGGL(fu_pair_kernel, dim3(cur), dim3(64), 0, st, pjobs, cur, G.sa, G.sb, gate, d,
                                ed, qa, qmax, qbuf, qmax, ca, cb, tgt, rlive, pwork, pairwork, nb);
@@ -1097,6 +1101,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
               }
               SPP_CHECK(hipStreamSynchronize(st));
               SPP_CHECK(hipMemcpyAsync(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), hipMemcpyHostToDevice, st));
+              SPP_CHECK(hipStreamSynchronize(st));  // jobs is released at the end of this block
               SPP_CHECK(launch_double_layers<T>(djobs, (int)jobs.size(), std::max(G.la_size, G.lb_size), st));
             }
             // --- bond metric N[R, S] (R = (rk, rb) pairs of A, S of B) ---
