@@ -127,6 +127,32 @@ def adjoint_timing(spp, reps=2, batch=8, chi=36):
             "batch": batch, "batch_s_per_state": tb / batch}
 
 
+def entropy_timing(spp, n=16, chi=32, J=1.0, h=2.0):
+    """The boundary-entropy observer's cost: FP64 energy measurement of n
+    projected samples of the synthetic 8x8 D=6 PEPO (eps 1.0, seed 1) at
+    chi=32, without and with record_entropy (every south and north cut's
+    pre-truncation spectrum), each timed once after a warm-up."""
+    lat = spp.Lattice(8, 8)
+    pepo = spp.synthetic_thermal_pepo(lat, (3, 3), 1.0, seed=1)
+    ctx = spp.Context(0, "fp64")
+    cache = spp.SamplingCache(ctx, pepo, chi)
+    model = spp.TFIM(J, h, symmetric=True)
+    labels = np.asarray(cache.basis.labels)
+    cfg = labels[np.random.default_rng(0).integers(0, 2, size=(n, 8, 8))]
+    spp.measure_energies(cache, cfg[:2], model)
+    out = {}
+    for rec in (False, True):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        r = spp.measure_energies(cache, cfg, model, record_entropy=rec)
+        ctx.synchronize()
+        out[rec] = (time.perf_counter() - t0, r)
+    return {"workload": f"FP64 energy of {n} projected samples, 8x8 Z2 PEPO D=6, chi={chi}",
+            "s_without_entropy": out[False][0], "s_with_entropy": out[True][0],
+            "overhead": out[True][0] / out[False][0] - 1.0,
+            "max_boundary_entropy": float(np.max(out[True][1]["max_boundary_entropy"]))}
+
+
 def fu_timing(spp, batch=16, chi=32, J=1.0, h=2.0, delta=0.05):
     """The full-update evolution workload: one second-order step (sweeps H->,
     V->, V<-, H<-; 224 bond updates) of a batch of 8x8 Z2 PEPS, D=6 (3+3),
@@ -175,6 +201,8 @@ def main():
     ap.add_argument("--h", type=float, default=2.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-only", action="store_true", help="timed loop only (for rocprofv3)")
+    ap.add_argument("--entropy-only", action="store_true",
+                    help="time only the boundary-entropy observer (energy with and without record_entropy)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="time only the log-norm gradient workload (for rocprofv3)")
     ap.add_argument("--fu-only", action="store_true", help="time only the full-update workload")
@@ -189,6 +217,9 @@ def main():
 
     import __graft_entry__ as ge
     spp = ge.load_package()
+    if args.entropy_only:
+        print(json.dumps({"boundary_entropy": entropy_timing(spp)}), flush=True)
+        return
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_timing(spp)}), flush=True)
         return
