@@ -216,12 +216,12 @@ struct EnergyRun {
   }
   T* take(long long elems) { return (T*)ctx.arena.take(sizeof(T) * (size_t)cur * (size_t)std::max(1LL, elems)); }
   void dot(const TRef& a, const TRef& b, double* out) {
-    SPP_CHECK(launch_dot<T>(full_view(a), full_view(b), out, cur, ctx.stream));
+    SPP_CHECK(launch_dot<T>(full_view(a), full_view(b), out, cur, ctx.stream, dot_partials(ctx, cur)));
   }
   // _safe_scale in place; the factor stays in `fac`
   void normalize(const TRef& t, int site) {
     FullView fv = full_view(t);
-    SPP_CHECK(launch_dot<T>(fv, fv, nrm, cur, ctx.stream));
+    SPP_CHECK(launch_dot<T>(fv, fv, nrm, cur, ctx.stream, dot_partials(ctx, cur)));
     SPP_CHECK(launch_safe_factor(nrm, fac, status, fail, site, cur, ctx.stream));
     SPP_CHECK(launch_scale<T>(fv, fac, cur, ctx.stream));
   }

@@ -147,6 +147,16 @@ struct EntropyWork {
   ~EntropyWork();
 };
 
+// Device buffer owned by a context, grown on demand (stream-ordered reuse).
+struct DeviceScratch {
+  double* p = nullptr;
+  size_t n = 0;
+  DeviceScratch() = default;
+  DeviceScratch(const DeviceScratch&) = delete;
+  DeviceScratch& operator=(const DeviceScratch&) = delete;
+  ~DeviceScratch();
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -167,6 +177,7 @@ struct Context {
   int profile = 0;          // 1: time every contraction launch with HIP events
   LaunchTimer timer;
   EntropyWork entropy;      // boundary-entropy spectra of the current absorb
+  DeviceScratch dot_part;   // partial sums of the chunked dot products
   void flush_timer();       // accumulate finished event pairs into counters
 };
 
@@ -188,6 +199,9 @@ void entropy_reset(Context& ctx);
 // All eigenvalues (ascending) of nprob symmetric blocks [nprob][ld][ld]
 // (host), of dimensions n_h[i] <= ld, into eig_h [nprob][ld] (host).
 void spectrum_selftest(Context& ctx, int nprob, int ld, const int* n_h, const double* mats_h, double* eig_h);
+
+// Scratch for launch_dot's partial sums of nsamples samples on ctx.stream.
+double* dot_partials(Context& ctx, int nsamples);
 
 // Contraction C = alpha * A.B (+ C): letters shared by A and B are summed.
 template <typename T>

@@ -31,6 +31,23 @@ void Arena::reserve(size_t bytes) {
   cap_ = bytes;
 }
 
+DeviceScratch::~DeviceScratch() {
+  if (p) (void)hipFree(p);
+}
+
+double* dot_partials(Context& ctx, int nsamples) {
+  const size_t need = (size_t)std::max(nsamples, 1) * kDotChunks;
+  if (ctx.dot_part.n < need) {
+    SPP_CHECK(hipStreamSynchronize(ctx.stream));  // earlier dot launches may still read the old buffer
+    if (ctx.dot_part.p) SPP_CHECK(hipFree(ctx.dot_part.p));
+    ctx.dot_part.p = nullptr;
+    ctx.dot_part.n = 0;
+    SPP_CHECK(hipMalloc((void**)&ctx.dot_part.p, sizeof(double) * need));
+    ctx.dot_part.n = need;
+  }
+  return ctx.dot_part.p;
+}
+
 void* Arena::take(size_t bytes) {
   size_t aligned = (bytes + 255) & ~size_t(255);
   if (top_ + aligned > cap_)
@@ -1315,7 +1332,7 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
       // last tensor: normalise and store (k, p, 1)
       if (br.dim != 1 || sp.e.dim != 1) throw HipError("absorb: last site must close with unit legs");
       FullView fv = full_view(Yr);
-      SPP_CHECK(launch_dot<T>(fv, fv, nrm, nb, st));
+      SPP_CHECK(launch_dot<T>(fv, fv, nrm, nb, st, dot_partials(ctx, nb)));
       hipLaunchKernelGGL(norm_finish_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, nrm, out.log_scale, fac,
                          nullptr, nb);
       SPP_CHECK(hipGetLastError());

@@ -914,7 +914,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
           };
           auto normalize = [&](const TRef& tr, int site) {
             FullView fv = full_view(tr);
-            SPP_CHECK(launch_dot<T>(fv, fv, nrm, cur, st));
+            SPP_CHECK(launch_dot<T>(fv, fv, nrm, cur, st, dot_partials(ctx, cur)));
             SPP_CHECK(launch_safe_factor(nrm, fac, status, fail, site, cur, st));
             SPP_CHECK(launch_scale<T>(fv, fac, cur, st));
           };

@@ -16,8 +16,13 @@ struct FullView {
 };
 
 // out[s] = sum over neutral entries of x*y (x == y gives the squared norm).
+// With partials (device scratch of nsamples * kDotChunks doubles) each sample
+// is split over kDotChunks workgroups and the partial sums are added in a
+// fixed order.
+constexpr int kDotChunks = 32;
 template <typename T>
-hipError_t launch_dot(const FullView& x, const FullView& y, double* out, int nsamples, hipStream_t st);
+hipError_t launch_dot(const FullView& x, const FullView& y, double* out, int nsamples, hipStream_t st,
+                      double* partials = nullptr);
 
 // x[entry] *= f[s] when f[s] != 0 (f given as 1/norm or 0 to leave untouched).
 template <typename T>

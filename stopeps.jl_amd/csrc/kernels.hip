@@ -45,10 +45,54 @@ __global__ void dot_kernel(FullView x, FullView y, double* out) {
   if (threadIdx.x == 0) out[s] = red[0];
 }
 
+// Partial sums of chunk blockIdx.x of sample blockIdx.y (entries i with
+// i / 256 = chunk mod kDotChunks) into part[s * kDotChunks + chunk].
 template <typename T>
-hipError_t launch_dot(const FullView& x, const FullView& y, double* out, int nsamples, hipStream_t st) {
+__global__ void dot_chunk_kernel(FullView x, FullView y, double* part) {
+  const int s = blockIdx.y, c = blockIdx.x;
+  __shared__ GroupView g;
+  __shared__ double red[256];
+  if (threadIdx.x == 0) resolve_group(x.g, s, g);
+  __syncthreads();
+  const long long n = group_count(g, 0);
+  const T* xb = fv_base<T>(x, s);
+  const T* yb = fv_base<T>(y, s);
+  double acc = 0.0;
+  int legs[kMaxLegs];
+  for (long long i = (long long)c * blockDim.x + threadIdx.x; i < n; i += (long long)kDotChunks * blockDim.x) {
+    decode(g, 0, i, legs);
+    acc += (double)xb[fv_off(x, g, legs)] * (double)yb[fv_off(y, g, legs)];
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(long long)s * kDotChunks + c] = red[0];
+}
+
+__global__ void dot_finish_kernel(const double* part, double* out, int nsamples) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsamples) return;
+  double acc = 0.0;
+  for (int c = 0; c < kDotChunks; ++c) acc += part[(long long)s * kDotChunks + c];
+  out[s] = acc;
+}
+
+template <typename T>
+hipError_t launch_dot(const FullView& x, const FullView& y, double* out, int nsamples, hipStream_t st,
+                      double* partials) {
   if (nsamples <= 0) return hipSuccess;
-  hipLaunchKernelGGL(dot_kernel<T>, dim3(nsamples), dim3(256), 0, st, x, y, out);
+  if (!partials) {
+    hipLaunchKernelGGL(dot_kernel<T>, dim3(nsamples), dim3(256), 0, st, x, y, out);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(dot_chunk_kernel<T>, dim3(kDotChunks, nsamples), dim3(256), 0, st, x, y, partials);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(dot_finish_kernel, dim3((nsamples + 255) / 256), dim3(256), 0, st, (const double*)partials,
+                     out, nsamples);
   return hipGetLastError();
 }
 
@@ -482,8 +526,8 @@ hipError_t launch_jacobi(const double* a, long long astride, int lda, double* w,
   return hipGetLastError();
 }
 
-template hipError_t launch_dot<float>(const FullView&, const FullView&, double*, int, hipStream_t);
-template hipError_t launch_dot<double>(const FullView&, const FullView&, double*, int, hipStream_t);
+template hipError_t launch_dot<float>(const FullView&, const FullView&, double*, int, hipStream_t, double*);
+template hipError_t launch_dot<double>(const FullView&, const FullView&, double*, int, hipStream_t, double*);
 template hipError_t launch_scale<float>(const FullView&, const double*, int, hipStream_t);
 template hipError_t launch_scale<double>(const FullView&, const double*, int, hipStream_t);
 template hipError_t launch_select_copy<float>(const FullView&, const FullView&, const FullView&, const int*,

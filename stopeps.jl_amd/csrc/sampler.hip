@@ -434,7 +434,7 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
           TRef Rr = dense_ref(R[x], Rs[x], "awb", Rl[x]);
           contract<T>(ctx, Rr, Yr, bottom_ref(sb, Lx, x, "bsd"), cur);
           FullView fv = full_view(Rr);
-          SPP_CHECK(launch_dot<T>(fv, fv, nrm, cur, st));
+          SPP_CHECK(launch_dot<T>(fv, fv, nrm, cur, st, dot_partials(ctx, cur)));
           SPP_CHECK(launch_safe_factor(nrm, fac, status, fail, x + y * Lx, cur, st));
           SPP_CHECK(launch_scale<T>(fv, fac, cur, st));
           ctx.arena.release(xm);
@@ -469,7 +469,7 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
             TRef Or = dense_ref(O[k], os, "ced", Ol);
             contract<T>(ctx, Or, Yr, bottom_ref(sb, Lx, x, "bsd"), cur);
             SPP_CHECK(launch_dot<T>(full_view(Or), full_view(dense_ref(R[x + 1], Rs[x + 1], "ced", Rl[x + 1])),
-                                    rawk[k], cur, st));
+                                    rawk[k], cur, st, dot_partials(ctx, cur)));
           }
           hipLaunchKernelGGL(born_select_kernel, dim3(gb), dim3(tb), 0, st, raw0, raw1, duni, dlabels_in, s0, ns,
                              site, choice, labels, logp, cond, status, fail, done, cur);
@@ -483,7 +483,7 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
             SPP_CHECK(launch_select_copy<T>(full_view(Lnew), full_view(dense_ref(O[0], os, "ced", Ol)),
                                             full_view(dense_ref(O[1], os, "ced", Ol)), choice, nullptr, cur, st));
             FullView fv = full_view(Lnew);
-            SPP_CHECK(launch_dot<T>(fv, fv, nrm, cur, st));
+            SPP_CHECK(launch_dot<T>(fv, fv, nrm, cur, st, dot_partials(ctx, cur)));
             SPP_CHECK(launch_safe_factor(nrm, fac, status, fail, site, cur, st));
             SPP_CHECK(launch_scale<T>(fv, fac, cur, st));
             Ll = Ol;
