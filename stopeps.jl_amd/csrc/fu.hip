@@ -246,10 +246,10 @@ fu_pair_kernel(const PairJob* jobs, int nb, FuSide sa, FuSide sb, const double* 
                  work_per, nbtot);
 }
 
-// Bond solve of every sample, one 256-lane workgroup per sample (fu_core.h's
+// Bond solve of every sample, one 512-lane workgroup per sample (fu_core.h's
 // Team): compact the metric and the target to the live r dimensions, solve,
 // expand the cores. Each sample's workspace is contiguous (work_per doubles).
-constexpr int kCoreLanes = 256;
+constexpr int kCoreLanes = 512;
 
 template <typename T>
 __global__ void __launch_bounds__(kCoreLanes)
