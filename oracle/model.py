@@ -181,7 +181,7 @@ def exact_thermal_observables(lattice, model, beta):
 
 def random_state(lattice, D, kind, seed, symmetric=True, d=2, dtype=np.float64):
     """Seeded i.i.d. N(0,1) entries in every allowed block, each site scaled to unit
-    Frobenius norm (the survey's timing / differentiation input). ``D`` is an
+    Frobenius norm (the random input of the timing and differentiation workloads). ``D`` is an
     (even, odd) pair for symmetric states or an int for dense ones. ``kind``
     "pepo": aux leg = physical space; "peps": aux leg of dim 1 and parity 0."""
     rng = np.random.default_rng(seed)
