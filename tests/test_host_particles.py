@@ -122,3 +122,45 @@ def test_run_config_mirror(spp):
         device_support(spp.RunConfig(), stochastic=True)  # resampling enabled by default
     with pytest.raises(NotImplementedError):
         device_support(spp.RunConfig(contraction="BoundaryMPSDiagonal"), stochastic=False)
+
+
+def test_nan_max_follows_julia(P):
+    """``max``/``maximum`` on floats in the reference propagate NaN (a new
+    particle's max_boundary_entropy is NaN, so is the node's maximum)."""
+    assert P.nan_max([0.1, 0.3, 0.2]) == 0.3
+    assert math.isnan(P.nan_max([0.1, math.nan, 0.2]))
+    assert math.isnan(P.nan_max([0.5], init=math.nan))
+    assert P.nan_max([], init=0.0) == 0.0
+    with pytest.raises(ValueError):
+        P.nan_max([])
+
+
+def test_boundary_space_parsing(spp):
+    """Boundary sector budgets of ``BoundaryMPS(chi, sector_space)``: an
+    (even, odd) pair for Z2, one trivial sector for a dense space."""
+    from stopeps_jl_amd.sampling import boundary_caps
+    assert boundary_caps(None) == (-1, -1)
+    assert boundary_caps((4, 2)) == (4, 2) and boundary_caps([0, 3]) == (0, 3)
+    assert boundary_caps(6) == (6, -1)
+    for bad in ((0, 0), (-1, 2), 0, -3):
+        with pytest.raises(ValueError):
+            boundary_caps(bad)
+
+
+def test_run_config_boundary_spaces_and_entropy_convention(spp):
+    """Sampling and measurement spaces default to the environment space
+    (config.jl:44-47); the entropy conventions are the reference's strings
+    (config.jl:147-151)."""
+    c = spp.RunConfig(model=spp.TFIM(1.0, 1.0, symmetric=True), environment_space=(4, 2))
+    assert c.sampling_space == c.measurement_space == c.proposal_space == (4, 2)
+    spp.validate_config(c)
+    from stopeps_jl_amd.config import device_support
+    device_support(spp.RunConfig(environment_space=(4, 2), resampling=spp.ResamplingOptions(enabled=False)),
+                   stochastic=True)
+    with pytest.raises(ValueError):
+        spp.validate_config(spp.RunConfig(model=spp.TFIM(1.0, 1.0, symmetric=True), measurement_space=(0, 0)))
+    p = spp.parameter_dict(c)
+    assert p["boundary_entropy_convention"] == "pretruncation_Schmidt_entropy; nats; sector_dimension_weighted"
+    assert p["boundary_entropy_log_scope"] == "maximum_over_FU_boundary_cuts_in_output_interval"
+    assert p["boundary_entropy_profile_scope"] == ("energy_measurement_boundaries; Se.north[y,b] and Se.south[y,b]; "
+                                                   "cut_after_row_y_and_column_b")
