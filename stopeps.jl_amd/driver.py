@@ -51,11 +51,13 @@ def _context(ctx):
     return ctx if ctx is not None else Context(0, "fp64")
 
 
-def run_purification(c, ctx=None):
+def run_purification(c, ctx=None, on_step=None):
     """``run_purification(c)`` (run.jl:2-43): the identity PEPO, normalised,
     evolved by full update (D_pur, bond_space_pur, delta_tau_pur) to every node
     up to beta0 with an observation at each. Returns the ``ThermalCheckpoint``
-    at beta0; its ``result`` attribute holds the preparation's result record."""
+    at beta0; its ``result`` attribute holds the preparation's result record.
+    ``on_step(info)`` is called after every full-update step (beta, seconds
+    and the step's outputs)."""
     validate_config(c)
     device_support(c, stochastic=False)
     ctx = _context(ctx)
@@ -67,7 +69,7 @@ def run_purification(c, ctx=None):
     dev, rows = purify(ctx, pepo, c.model, c.beta0, c.beta_log, c.beta_measure, c.delta_tau_pur, c.chi_environment,
                        c.chi_measurement, cap, c.update, c.trotter_order, D=c.D_pur,
                        record_entropy=c.record_boundary_entropy, environment_space=c.environment_space,
-                       measurement_space=c.measurement_space)
+                       measurement_space=c.measurement_space, on_step=on_step)
     for r in rows:
         r.update({"eta": float("nan"), "eta_error": float("nan")})
         out["observables"]["purification"].append(r)

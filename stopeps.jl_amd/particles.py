@@ -320,13 +320,16 @@ def statistics(records, Ns, nsites):
 # ----------------------------------------------------------------------------
 
 def purify(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, chi_environment, chi_measurement, bond_space,
-           method=None, order=2, D=None, record_entropy=False, environment_space=None, measurement_space=None):
+           method=None, order=2, D=None, record_entropy=False, environment_space=None, measurement_space=None,
+           on_step=None):
     """The evolution of ``run_purification`` (run.jl:2-43) from the identity PEPO
     (``identity_pepo``; normalised here as the reference does), on the device:
     FU steps to every node <= beta0 and an observation there. Returns the
     resident PEPO (``DeviceState``, bonds at the ``bond_space`` capacity with
     their live sectors) and the rows (beta, energy_total, energy_per_site,
-    lnZ, truncations, max_boundary_entropy, Se, seconds)."""
+    lnZ, truncations, max_boundary_entropy, Se, seconds). ``on_step`` is
+    called after every full-update step with its beta, time and outputs (the
+    progress the reference writes to its run log)."""
     method = method or FullUpdate()
     st, live = pad_bonds(pepo, bond_space)
     normalize_state(st)
@@ -336,8 +339,9 @@ def purify(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, chi_envir
         steps = step_schedule(dev.state.beta, beta, delta_tau)
         p = Particle(state=dev, tensor_log_norm=0.0, relative_log_weight=0.0)
         advance_particles(ctx, [p], model, steps, chi_environment, chi_measurement, method,
-                          measure_at(beta_measure, beta), order, D=D, record_entropy=record_entropy,
-                          environment_space=environment_space, measurement_space=measurement_space)
+                          measure_at(beta_measure, beta), order, on_step=on_step, D=D,
+                          record_entropy=record_entropy, environment_space=environment_space,
+                          measurement_space=measurement_space)
         rows.append({"beta": beta, "source": "purification", "energy_total": p.energy_total,
                      "energy_per_site": p.energy_per_site,
                      "energy_per_site_error": 0.0 if measure_at(beta_measure, beta) else math.nan,
