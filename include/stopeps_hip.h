@@ -25,8 +25,11 @@
  *    `status` (0 ok, 1 invalid Born conditional, 2 zero conditional mass,
  *    3 nonfinite environment), so the caller can raise the reference's
  *    messages (born.jl:137-142, boundary_mps.jl:131,258).
- *  - One context = one HIP stream; a context is not re-entrant. Separate
- *    contexts may be used from separate threads.
+ *  - One context = one HIP stream; a context is not re-entrant. Drive all
+ *    contexts of a process from one host thread: calls on separate contexts
+ *    from concurrent host threads are not supported (a full-update batch split
+ *    over concurrent contexts gave results that varied from run to run).
+ *    Contexts used one after the other give the results of a single context.
  */
 #ifndef STOPEPS_HIP_H
 #define STOPEPS_HIP_H

@@ -193,7 +193,7 @@ void lognorm_grad_batch(Context& ctx, const std::vector<const StateDev*>& states
     std::vector<void*> hptr((size_t)ns * nb);
     for (int i = 0; i < ns; ++i)
       for (int s = 0; s < nb; ++s) hptr[(size_t)i * nb + s] = site_base[i] + (long long)s * lsize[i];
-    SPP_CHECK(hipMemcpyAsync(ptrs, hptr.data(), sizeof(void*) * hptr.size(), hipMemcpyHostToDevice, st));
+    upload(ptrs, hptr.data(), sizeof(void*) * hptr.size(), st);
     const double** aptr = (const double**)dev(sizeof(double*) * (size_t)ns * nb);
     LayerJob* djobs = (LayerJob*)dev(sizeof(LayerJob) * (size_t)ns * nb);
     double* G = (double*)dev(sizeof(double) * (size_t)goff[ns] * nb);
@@ -229,9 +229,9 @@ void lognorm_grad_batch(Context& ctx, const std::vector<const StateDev*>& states
           jobs.push_back(J);
           haptr[(size_t)i * nb + s] = sd.d;
         }
-      SPP_CHECK(hipMemcpyAsync(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), hipMemcpyHostToDevice, st));
+      upload(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), st);
       SPP_CHECK(launch_double_layers<T>(djobs, (int)jobs.size(), maxout, st));
-      SPP_CHECK(hipMemcpyAsync(aptr, haptr.data(), sizeof(double*) * haptr.size(), hipMemcpyHostToDevice, st));
+      upload(aptr, haptr.data(), sizeof(double*) * haptr.size(), st);
       SPP_CHECK(launch_fill<int>(status, nb, 0, st));
       SPP_CHECK(launch_fill<int>(fail, nb, -1, st));
       auto table = [&](int site) { return (void* const*)(ptrs + (size_t)site * nb); };
@@ -366,8 +366,8 @@ void lognorm_grad_batch(Context& ctx, const std::vector<const StateDev*>& states
         if (Nf.bdim[x] != 1) throw HipError("final boundary must have unit bonds");
       std::vector<long long> sst(Lx);
       for (int x = 0; x < Lx; ++x) sst[x] = Nf.tsize(x);
-      SPP_CHECK(hipMemcpyAsync(dsst, sst.data(), sizeof(long long) * Lx, hipMemcpyHostToDevice, st));
-      SPP_CHECK(hipMemcpyAsync(dtens, Nf.t.data(), sizeof(T*) * Lx, hipMemcpyHostToDevice, st));
+      upload(dsst, sst.data(), sizeof(long long) * Lx, st);
+      upload(dtens, Nf.t.data(), sizeof(T*) * Lx, st);
       SPP_CHECK(launch_close_unit<T>(dtens, dsst, Lx, mant, cur, st));
       SPP_CHECK(hipMemcpyAsync(h_mant.data(), mant, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_ls.data(), Nf.log_scale, sizeof(double) * cur, hipMemcpyDeviceToHost, st));

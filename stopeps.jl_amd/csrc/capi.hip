@@ -169,7 +169,7 @@ int spp_state_upload(spp_ctx* ctx, int Lx, int Ly, int kind, const int* dims, co
         throw HipError("vertical space mismatch");
       long long n = d.size();
       SPP_CHECK(hipMalloc(&d.d, sizeof(double) * n));
-      SPP_CHECK(hipMemcpyAsync(d.d, data[i], sizeof(double) * n, hipMemcpyHostToDevice, ctx->c.stream));
+      upload(d.d, data[i], sizeof(double) * n, ctx->c.stream);
     }
     s.live.assign((size_t)Lx * Ly * 8, 0);
     for (int i = 0; i < Lx * Ly; ++i) {
@@ -305,6 +305,12 @@ int spp_born_sample_batch(spp_ctx* ctx, spp_cache* cache, int n, const double* u
     if (!cache) throw HipError("null cache");
     if (n < 0) throw HipError("negative sample count");
     if (!uniforms && !labels_in) throw HipError("an explicit RNG is required");
+    if (labels_in) {
+      // evaluation mode: every label indexes one of the two basis vectors
+      const int ns = cache->precision == 1 ? cache->f->Lx * cache->f->Ly : cache->d->Lx * cache->d->Ly;
+      for (long long i = 0; i < (long long)n * ns; ++i)
+        if (labels_in[i] != 0 && labels_in[i] != 1) throw HipError("labels must be 0 or 1");
+    }
     SPP_CHECK(hipSetDevice(ctx->c.device));
     DrawOutputs o;
     o.labels = labels_out;
@@ -482,8 +488,8 @@ int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, int even
     SPP_CHECK(hipMalloc(&dA, sizeof(double) * M * K));
     SPP_CHECK(hipMalloc(&dB, sizeof(double) * K * N));
     SPP_CHECK(hipMalloc(&dC, sizeof(double) * M * N));
-    SPP_CHECK(hipMemcpyAsync(dA, A, sizeof(double) * M * K, hipMemcpyHostToDevice, st));
-    SPP_CHECK(hipMemcpyAsync(dB, B, sizeof(double) * K * N, hipMemcpyHostToDevice, st));
+    upload(dA, A, sizeof(double) * M * K, st);
+    upload(dB, B, sizeof(double) * K * N, st);
     Leg lm = fixed_leg(M, evenM), ln = fixed_leg(N, evenN), lk = fixed_leg(K, evenK);
     if (layout < 0 || layout > 3) throw HipError("layout must lie in [0, 3]");
     // storage order of the operands: A as [M][K] or [K][M], B as [K][N] or [N][K]

@@ -478,7 +478,7 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
   try {
     if (!jobs.empty()) {
       djobs = (LayerJob*)dev(sizeof(LayerJob) * jobs.size());
-      SPP_CHECK(hipMemcpyAsync(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), hipMemcpyHostToDevice, st));
+      upload(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), st);
       SPP_CHECK(launch_double_layers<T>(djobs, (int)jobs.size(), maxout, st));
     }
     // ---------------- chunk size from the memory budget ----------------
@@ -527,8 +527,8 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
     lnamp = (double*)dev(sizeof(double) * nb);
     dsst = (long long*)dev(sizeof(long long) * Lx);
     dtens = (T**)dev(sizeof(T*) * Lx);
-    SPP_CHECK(hipMemcpyAsync(dbase, base.data(), sizeof(void*) * base.size(), hipMemcpyHostToDevice, st));
-    SPP_CHECK(hipMemcpyAsync(dlabels, labels_h, sizeof(int) * (size_t)n * ns, hipMemcpyHostToDevice, st));
+    upload(dbase, base.data(), sizeof(void*) * base.size(), st);
+    upload(dlabels, labels_h, sizeof(int) * (size_t)n * ns, st);
     std::vector<int> h_status(nb), h_fail(nb);
     std::vector<double> h_x(nb), h_zz(nb), h_mant(nb), h_ls(nb), h_mtn(nb), h_mts(nb), h_amp(nb);
     for (long long s0 = 0; s0 < n; s0 += nb) {
@@ -590,8 +590,8 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
         if (Nf.bdim[x] != 1) throw HipError("final boundary must have unit bonds");
       std::vector<long long> sst(Lx);
       for (int x = 0; x < Lx; ++x) sst[x] = Nf.tsize(x);
-      SPP_CHECK(hipMemcpyAsync(dsst, sst.data(), sizeof(long long) * Lx, hipMemcpyHostToDevice, st));
-      SPP_CHECK(hipMemcpyAsync(dtens, Nf.t.data(), sizeof(T*) * Lx, hipMemcpyHostToDevice, st));
+      upload(dsst, sst.data(), sizeof(long long) * Lx, st);
+      upload(dtens, Nf.t.data(), sizeof(T*) * Lx, st);
       SPP_CHECK(launch_close_unit<T>(dtens, dsst, Lx, mant, cur, st));
       SPP_CHECK(launch_gather_lognorm(c.cand_lognorm, lab, ns, lnamp, cur, st));
       SPP_CHECK(hipMemcpyAsync(h_status.data(), run.status, sizeof(int) * cur, hipMemcpyDeviceToHost, st));
@@ -773,9 +773,9 @@ void measure_energy_states(Context& ctx, const std::vector<const StateDev*>& sta
     // read by queued kernels: wait before rewriting them)
     auto run_jobs = [&]() {
       SPP_CHECK(hipStreamSynchronize(st));
-      SPP_CHECK(hipMemcpyAsync(run.tables, htab.data(), sizeof(void*) * htab.size(), hipMemcpyHostToDevice, st));
+      upload(run.tables, htab.data(), sizeof(void*) * htab.size(), st);
       if (!jobs.empty()) {
-        SPP_CHECK(hipMemcpyAsync(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), hipMemcpyHostToDevice, st));
+        upload(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), st);
         SPP_CHECK(launch_double_layers<T>(djobs, (int)jobs.size(), maxout, st));
       }
     };
@@ -860,8 +860,8 @@ void measure_energy_states(Context& ctx, const std::vector<const StateDev*>& sta
         if (Nf.bdim[x] != 1) throw HipError("final boundary must have unit bonds");
       std::vector<long long> sst(Lx);
       for (int x = 0; x < Lx; ++x) sst[x] = Nf.tsize(x);
-      SPP_CHECK(hipMemcpyAsync(dsst, sst.data(), sizeof(long long) * Lx, hipMemcpyHostToDevice, st));
-      SPP_CHECK(hipMemcpyAsync(dtens, Nf.t.data(), sizeof(T*) * Lx, hipMemcpyHostToDevice, st));
+      upload(dsst, sst.data(), sizeof(long long) * Lx, st);
+      upload(dtens, Nf.t.data(), sizeof(T*) * Lx, st);
       SPP_CHECK(launch_close_unit<T>(dtens, dsst, Lx, mant, cur, st));
       SPP_CHECK(hipMemcpyAsync(h_status.data(), run.status, sizeof(int) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_fail.data(), run.fail, sizeof(int) * cur, hipMemcpyDeviceToHost, st));

@@ -28,6 +28,14 @@ struct HipError : std::runtime_error {
                             __FILE__ + ":" + std::to_string(__LINE__));                       \
   } while (0)
 
+// Host-to-device copy on the stream that returns once the copy is complete:
+// the host source is often a temporary (a job table, a pointer list) released
+// right after the call, and a pageable source must outlive its transfer.
+inline void upload(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  SPP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+  SPP_CHECK(hipStreamSynchronize(st));
+}
+
 // Device arena with stack-like release.
 class Arena {
  public:

@@ -380,8 +380,8 @@ void spectrum_selftest(Context& ctx, int nprob, int ld, const int* n_h, const do
   try {
     SPP_CHECK(hipMalloc((void**)&w, wb));
     SPP_CHECK(hipMalloc((void**)&c, sizeof(int) * nprob));
-    SPP_CHECK(hipMemcpyAsync(m, mats_h, mb, hipMemcpyHostToDevice, st));
-    SPP_CHECK(hipMemcpyAsync(c, n_h, sizeof(int) * nprob, hipMemcpyHostToDevice, st));
+    upload(m, mats_h, mb, st);
+    upload(c, n_h, sizeof(int) * nprob, st);
     SPP_CHECK(launch_fill<double>(w, (long long)nprob * ld, 0.0, st));
     EntropyTable tab{};
     tab.mat[0] = m;

@@ -546,8 +546,8 @@ static void onsite_step(Context& ctx, const std::vector<StateDev*>& states, cons
     SPP_CHECK(hipMalloc((void**)&dp, sizeof(double*) * n));
     SPP_CHECK(hipMalloc((void**)&dg, sizeof(double) * d * d));
     SPP_CHECK(hipMalloc((void**)&dn, sizeof(double) * n));
-    SPP_CHECK(hipMemcpyAsync(dp, hp.data(), sizeof(double*) * n, hipMemcpyHostToDevice, st));
-    SPP_CHECK(hipMemcpyAsync(dg, P.gates_h, sizeof(double) * d * d, hipMemcpyHostToDevice, st));
+    upload(dp, hp.data(), sizeof(double*) * n, st);
+    upload(dg, P.gates_h, sizeof(double) * d * d, st);
     hipLaunchKernelGGL(onsite_gate_kernel, dim3(n), dim3(256), 0, st, dp, outer, d, inner, (const double*)dg);
     SPP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(site_sumsq_kernel, dim3(n), dim3(256), 0, st, dp, size, dn);
@@ -565,7 +565,7 @@ static void onsite_step(Context& ctx, const std::vector<StateDev*>& states, cons
       states[s]->beta += 2.0 * P.delta;
       hn[s] = 1.0 / v;
     }
-    SPP_CHECK(hipMemcpyAsync(dn, hn.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+    upload(dn, hn.data(), sizeof(double) * n, st);
     hipLaunchKernelGGL(site_scale_kernel, dim3(n, (unsigned)std::min<long long>((size + 255) / 256, 64)), dim3(256),
                        0, st, dp, size, (const double*)dn);
     SPP_CHECK(hipGetLastError());
@@ -758,7 +758,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
       std::vector<void*> hptr((size_t)ns * nb);
       for (int i = 0; i < ns; ++i)
         for (int s = 0; s < nb; ++s) hptr[(size_t)i * nb + s] = site_base[i] + (long long)s * lsize[i];
-      SPP_CHECK(hipMemcpyAsync(ptrs, hptr.data(), sizeof(void*) * hptr.size(), hipMemcpyHostToDevice, st));
+      upload(ptrs, hptr.data(), sizeof(void*) * hptr.size(), st);
       SPP_CHECK(hipStreamSynchronize(st));  // hptr is released at the end of this block
     }
     auto table = [&](int site) { return (void* const*)(ptrs + (size_t)site * nb); };
@@ -802,10 +802,8 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
     int* blive = (int*)dev(sizeof(int) * 2 * (size_t)nb);
     int* live = (int*)dev(sizeof(int) * (size_t)ns * 8 * nb);
     double* gate_dev = (double*)dev(sizeof(double) * (size_t)(nh + nv) * d * d * d * d);
-    SPP_CHECK(hipMemcpyAsync(gate_dev, P.gates_h, sizeof(double) * (size_t)nh * d * d * d * d, hipMemcpyHostToDevice,
-                             st));
-    SPP_CHECK(hipMemcpyAsync(gate_dev + (size_t)nh * d * d * d * d, P.gates_v, sizeof(double) * (size_t)nv * d * d * d * d,
-                             hipMemcpyHostToDevice, st));
+    upload(gate_dev, P.gates_h, sizeof(double) * (size_t)nh * d * d * d * d, st);
+    upload(gate_dev + (size_t)nh * d * d * d * d, P.gates_v, sizeof(double) * (size_t)nv * d * d * d * d, st);
     PairJob* pjobs = (PairJob*)dev(sizeof(PairJob) * (size_t)nb);
     RestoreJob* rjobs = (RestoreJob*)dev(sizeof(RestoreJob) * (size_t)nb);
     int* status = (int*)dev(sizeof(int) * nb);
@@ -826,7 +824,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
         std::vector<int> hl((size_t)ns * 8 * nb, 0);
         for (int s = 0; s < cur; ++s)
           std::memcpy(hl.data() + (size_t)s * ns * 8, states[c0 + s]->live.data(), sizeof(int) * ns * 8);
-        SPP_CHECK(hipMemcpyAsync(live, hl.data(), sizeof(int) * hl.size(), hipMemcpyHostToDevice, st));
+        upload(live, hl.data(), sizeof(int) * hl.size(), st);
         SPP_CHECK(hipStreamSynchronize(st));  // hl is released at the end of this block
       }
       SPP_CHECK(launch_fill<int>(status, nb, 0, st));
@@ -850,7 +848,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
           }
         // the job table may still be read by queued kernels: wait before rewriting it
         SPP_CHECK(hipStreamSynchronize(st));
-        SPP_CHECK(hipMemcpyAsync(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), hipMemcpyHostToDevice, st));
+        upload(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), st);
         SPP_CHECK(hipStreamSynchronize(st));  // jobs is released when this call returns
         SPP_CHECK(launch_double_layers<T>(djobs, (int)jobs.size(), maxout, st));
       };
@@ -1060,8 +1058,8 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
               }
               // the previous bond's restore kernel reads rjobs: wait before rewriting the tables
               SPP_CHECK(hipStreamSynchronize(st));
-              SPP_CHECK(hipMemcpyAsync(pjobs, pjd.data(), sizeof(PairJob) * cur, hipMemcpyHostToDevice, st));
-              SPP_CHECK(hipMemcpyAsync(rjobs, rj.data(), sizeof(RestoreJob) * cur, hipMemcpyHostToDevice, st));
+              upload(pjobs, pjd.data(), sizeof(PairJob) * cur, st);
+              upload(rjobs, rj.data(), sizeof(RestoreJob) * cur, st);
               SPP_CHECK(hipStreamSynchronize(st));  // pjd and rj are released at the end of this block
             }
             hipLaunchKernelGGL(fu_pair_kernel, dim3(cur), dim3(64), 0, st, pjobs, cur, G.sa, G.sb, gate, d,
@@ -1100,7 +1098,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
                 }
               }
               SPP_CHECK(hipStreamSynchronize(st));
-              SPP_CHECK(hipMemcpyAsync(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), hipMemcpyHostToDevice, st));
+              upload(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), st);
               SPP_CHECK(hipStreamSynchronize(st));  // jobs is released at the end of this block
               SPP_CHECK(launch_double_layers<T>(djobs, (int)jobs.size(), std::max(G.la_size, G.lb_size), st));
             }
@@ -1248,7 +1246,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
           for (int s = 0; s < cur; ++s) hp[(size_t)i * cur + s] = states[c0 + s]->sites[i].d;
         double** dp = (double**)ctx.arena.take(sizeof(double*) * hp.size());
         double* dn = (double*)ctx.arena.take(sizeof(double) * hp.size());
-        SPP_CHECK(hipMemcpyAsync(dp, hp.data(), sizeof(double*) * hp.size(), hipMemcpyHostToDevice, st));
+        upload(dp, hp.data(), sizeof(double*) * hp.size(), st);
         std::vector<long long> sizes(ns);
         for (int i = 0; i < ns; ++i) sizes[i] = s0.sites[i].size();
         for (int i = 0; i < ns; ++i) {
@@ -1271,7 +1269,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
             if (out.normalizations) out.normalizations[(size_t)(c0 + s) * ns + i] = std::log(v);
             hn[(size_t)i * cur + s] = 1.0 / v;
           }
-        SPP_CHECK(hipMemcpyAsync(dn, hn.data(), sizeof(double) * hn.size(), hipMemcpyHostToDevice, st));
+        upload(dn, hn.data(), sizeof(double) * hn.size(), st);
         for (int i = 0; i < ns; ++i) {
           hipLaunchKernelGGL(site_scale_kernel, dim3(cur, (unsigned)std::min<long long>((sizes[i] + 255) / 256, 64)),
                              dim3(256), 0, st, dp + (size_t)i * cur, sizes[i], dn + (size_t)i * cur);

@@ -87,7 +87,7 @@ void lognorm_batch(Context& ctx, const std::vector<const StateDev*>& states, int
     std::vector<void*> hptr((size_t)ns * nb);
     for (int i = 0; i < ns; ++i)
       for (int s = 0; s < nb; ++s) hptr[(size_t)i * nb + s] = site_base[i] + (long long)s * lsize[i];
-    SPP_CHECK(hipMemcpyAsync(ptrs, hptr.data(), sizeof(void*) * hptr.size(), hipMemcpyHostToDevice, st));
+    upload(ptrs, hptr.data(), sizeof(void*) * hptr.size(), st);
     LayerJob* djobs = (LayerJob*)dev(sizeof(LayerJob) * (size_t)ns * nb);
     prepare_boundary(NB[0], Lx, nb, cap);
     prepare_boundary(NB[1], Lx, nb, cap);
@@ -112,7 +112,7 @@ void lognorm_batch(Context& ctx, const std::vector<const StateDev*>& states, int
           J.out = site_base[i] + (long long)s * lsize[i];
           jobs.push_back(J);
         }
-      SPP_CHECK(hipMemcpyAsync(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), hipMemcpyHostToDevice, st));
+      upload(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), st);
       SPP_CHECK(launch_double_layers<T>(djobs, (int)jobs.size(), maxout, st));
       NB[0].nb = cur;
       NB[1].nb = cur;
@@ -134,8 +134,8 @@ void lognorm_batch(Context& ctx, const std::vector<const StateDev*>& states, int
         if (Nf.bdim[x] != 1) throw HipError("final boundary must have unit bonds");
       std::vector<long long> sst(Lx);
       for (int x = 0; x < Lx; ++x) sst[x] = Nf.tsize(x);
-      SPP_CHECK(hipMemcpyAsync(dsst, sst.data(), sizeof(long long) * Lx, hipMemcpyHostToDevice, st));
-      SPP_CHECK(hipMemcpyAsync(dtens, Nf.t.data(), sizeof(T*) * Lx, hipMemcpyHostToDevice, st));
+      upload(dsst, sst.data(), sizeof(long long) * Lx, st);
+      upload(dtens, Nf.t.data(), sizeof(T*) * Lx, st);
       SPP_CHECK(launch_close_unit<T>(dtens, dsst, Lx, mant, cur, st));
       SPP_CHECK(hipMemcpyAsync(h_mant.data(), mant, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_ls.data(), Nf.log_scale, sizeof(double) * cur, hipMemcpyDeviceToHost, st));

@@ -269,7 +269,7 @@ void build_cache(Context& ctx, const StateDev& pepo, int chi, const double* basi
   }
   LayerJob* djobs = nullptr;
   SPP_CHECK(hipMalloc(&djobs, sizeof(LayerJob) * jobs.size()));
-  SPP_CHECK(hipMemcpyAsync(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), hipMemcpyHostToDevice, st));
+  upload(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), st);
   SPP_CHECK(launch_double_layers<T>(djobs, (int)jobs.size(), maxout, st));
   SPP_CHECK(hipMalloc(&c.cand_lognorm, sizeof(double) * 2 * ns));
   for (int i = 0; i < ns; ++i) {
@@ -304,7 +304,7 @@ void build_cache(Context& ctx, const StateDev& pepo, int chi, const double* basi
         J.out = l64[i];
         jobs64.push_back(J);
       }
-      SPP_CHECK(hipMemcpyAsync(djobs, jobs64.data(), sizeof(LayerJob) * ns, hipMemcpyHostToDevice, st));
+      upload(djobs, jobs64.data(), sizeof(LayerJob) * ns, st);
       SPP_CHECK(launch_double_layers<double>(djobs, ns, maxout, st));
       build_south<double>(ctx, c.spec, l64, Lx, Ly, chi, s64);
       c.south.resize(Ly + 1);
@@ -381,9 +381,9 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
   if (labels_in) SPP_CHECK(hipMalloc(&dlabels_in, sizeof(int) * (long long)n * ns));
   if (uniforms) SPP_CHECK(hipMalloc(&duni, sizeof(double) * (long long)n * ns));
   if (labels_in)
-    SPP_CHECK(hipMemcpyAsync(dlabels_in, labels_in, sizeof(int) * (long long)n * ns, hipMemcpyHostToDevice, st));
+    upload(dlabels_in, labels_in, sizeof(int) * (long long)n * ns, st);
   if (uniforms)
-    SPP_CHECK(hipMemcpyAsync(duni, uniforms, sizeof(double) * (long long)n * ns, hipMemcpyHostToDevice, st));
+    upload(duni, uniforms, sizeof(double) * (long long)n * ns, st);
   const int tb = 256, gb = (nb + tb - 1) / tb;
   std::vector<int> h_status(nb), h_fail(nb), h_labels((size_t)nb * ns);
   std::vector<double> h_logp(nb), h_cond((size_t)nb * ns), h_mt(nb), h_mant(nb), h_ls(nb), h_amp(nb);
@@ -507,8 +507,8 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
         if (Nf.bdim[x] != 1) throw HipError("final boundary must have unit bonds");
       std::vector<long long> sst(Lx);
       for (int x = 0; x < Lx; ++x) sst[x] = Nf.tsize(x);
-      SPP_CHECK(hipMemcpyAsync(dsst, sst.data(), sizeof(long long) * Lx, hipMemcpyHostToDevice, st));
-      SPP_CHECK(hipMemcpyAsync(dtens, Nf.t.data(), sizeof(T*) * Lx, hipMemcpyHostToDevice, st));
+      upload(dsst, sst.data(), sizeof(long long) * Lx, st);
+      upload(dtens, Nf.t.data(), sizeof(T*) * Lx, st);
       SPP_CHECK(launch_close_unit<T>(dtens, dsst, Lx, mant, cur, st));
       SPP_CHECK(launch_gather_lognorm(c.cand_lognorm, labels, ns, lnamp, cur, st));
       SPP_CHECK(hipMemcpyAsync(h_status.data(), status, sizeof(int) * cur, hipMemcpyDeviceToHost, st));
