@@ -8,6 +8,7 @@ is the host-side mirror of the reference's Julia API for that path.
 """
 from .states import (Lattice, Site, FiniteState, FinitePEPO, FinitePEPS, identity_pepo, parity_sorted,
                      synthetic_thermal_pepo, random_peps, pad_bonds, normalize_state, reembed_bonds,
+                     save_state_npz, load_state_npz,
                      LocalBasis, SymmetryBasis, XBasis, resolve_basis)
 from .sampling import (Context, DeviceState, SamplingCache, BornError, sample_configuration,
                        sample_configurations, sample_uniforms, project_configuration, splitmix64)
@@ -30,4 +31,5 @@ __all__ = ["synthetic_thermal_pepo", "random_peps", "Lattice", "Site", "FiniteSt
            "evolution_step", "evolution_steps", "Particle", "ensemble_statistics", "temperature_nodes",
            "step_schedule", "purify", "stochastic_ensemble", "new_particles", "advance_particles", "statistics",
            "group_configurations", "normalize_state", "reembed_bonds", "RunConfig", "ResamplingOptions",
-           "validate_config", "parameter_dict", "ThermalCheckpoint", "run_purification", "run_stochastic"]
+           "validate_config", "parameter_dict", "ThermalCheckpoint", "run_purification", "run_stochastic",
+           "save_state_npz", "load_state_npz"]

@@ -290,3 +290,35 @@ def resolve_basis(state, basis=None):
         return basis
     symmetric = bool(np.any(state[(0, 0)].parity[2]))
     return SymmetryBasis() if symmetric else XBasis()
+
+
+def save_state_npz(path, state, **extra):
+    """Write a ``FiniteState`` to an ``.npz`` file: one dense array and six
+    parity vectors per site (row-major order), kind, beta and log_amplitude,
+    plus ``extra`` arrays. Loaded by :func:`load_state_npz`."""
+    arrs = {}
+    for i, (x, y) in enumerate(state.lattice.site_order()):
+        s = state[(x, y)]
+        arrs[f"site{i}"] = s.data
+        for k in range(6):
+            arrs[f"par{i}_{k}"] = s.parity[k].astype(np.int8)
+    np.savez_compressed(path, Lx=state.lattice.Lx, Ly=state.lattice.Ly, kind=state.kind, beta=state.beta,
+                        log_amplitude=state.log_amplitude, **arrs, **extra)
+
+
+def load_state_npz(path):
+    """(``FiniteState``, dict of the remaining arrays) of a file written by
+    :func:`save_state_npz` (read without unpickling)."""
+    f = np.load(path, allow_pickle=False)
+    Lx, Ly = int(f["Lx"]), int(f["Ly"])
+    lat = Lattice(Lx, Ly)
+    sites = [[None] * Ly for _ in range(Lx)]
+    used = {"Lx", "Ly", "kind", "beta", "log_amplitude"}
+    for i in range(Lx * Ly):
+        x, y = i % Lx, i // Lx
+        sites[x][y] = Site(f[f"site{i}"], [f[f"par{i}_{k}"].astype(np.int64) for k in range(6)])
+        used.add(f"site{i}")
+        used.update(f"par{i}_{k}" for k in range(6))
+    kind = str(f["kind"]) if "kind" in f.files else "pepo"
+    st = FiniteState(lat, sites, kind, beta=float(f["beta"]), log_amplitude=float(f["log_amplitude"]))
+    return st, {k: f[k] for k in f.files if k not in used}

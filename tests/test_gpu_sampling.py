@@ -36,6 +36,21 @@ def test_gemm_selftest(spp, gpu_ctx, precision, tol, shape, layout):
     assert err < tol, err
 
 
+def boundary_distance(basis, ref, cfg_ref, cfg_got, u, L):
+    """Distance of the uniform from the oracle's cumulative-probability
+    boundary at the first site (row-major) where two draws differ: with two
+    outcomes, ``_categorical`` (src/resample/selection.jl:68-80) picks the
+    first when u < p_first, so a draw can flip only if u lies within the
+    conditional's error of p_first."""
+    for j in range(L.Lx * L.Ly):
+        x, y = j % L.Lx, j // L.Lx
+        if cfg_ref[x][y] != cfg_got[x][y]:
+            c = ref["conditionals"][j]
+            p_first = c if basis.index(cfg_ref[x][y]) == 0 else 1.0 - c
+            return abs(u[j] - p_first)
+    raise AssertionError("draws do not differ")
+
+
 def _compare_draws(spp, ctx, pepo, chi, nsamp, seed, tol_logp, tol_cond, precision="fp64", check_norm=True):
     L = pepo.lattice
     ns = L.Lx * L.Ly
@@ -47,12 +62,12 @@ def _compare_draws(spp, ctx, pepo, chi, nsamp, seed, tol_logp, tol_cond, precisi
     ocache = ob.SamplingCache(pepo, chi)
     np.testing.assert_allclose(cache.south_max_truncation, [e.max_truncation for e in ocache.south],
                                atol=1e-10)
-    mism = 0
+    mism = []
     for i in range(nsamp):
         ref = ob.sample_configuration(ocache, uniforms=uni[i])
         cfg = oracle_cfg(ref, L)
         if not np.array_equal(cfg, got["configuration"][i]):
-            mism += 1
+            mism.append(boundary_distance(ocache.basis, ref, cfg, got["configuration"][i], uni[i], L))
             continue
         assert abs(ref["log_probability"] - got["log_probability"][i]) <= tol_logp * max(1, abs(ref["log_probability"]))
         np.testing.assert_allclose(got["conditionals"][i], ref["conditionals"], rtol=tol_cond, atol=tol_cond)
@@ -69,28 +84,28 @@ def test_draw_exact_chi_fp64(spp, gpu_ctx):
     say("draw exact chi fp64")
     L = Lattice(3, 3)
     pepo = embed_identity_pepo(L, (1, 1), 0.5, seed=11)
-    assert _compare_draws(spp, gpu_ctx, pepo, 16, 6, 1, 1e-11, 1e-11) == 0
+    assert _compare_draws(spp, gpu_ctx, pepo, 16, 6, 1, 1e-11, 1e-11) == []
 
 
 def test_draw_truncated_fp64(spp, gpu_ctx):
     say("draw truncated fp64")
     L = Lattice(4, 4)
     pepo = embed_identity_pepo(L, (1, 1), 0.6, seed=12)
-    assert _compare_draws(spp, gpu_ctx, pepo, 6, 8, 2, 1e-9, 1e-9) == 0
+    assert _compare_draws(spp, gpu_ctx, pepo, 6, 8, 2, 1e-9, 1e-9) == []
 
 
 def test_draw_truncated_fp64_d3(spp, gpu_ctx):
     say("draw truncated fp64 D=(2,1) uneven")
     L = Lattice(3, 4)
     pepo = embed_identity_pepo(L, (2, 1), 0.4, seed=13)
-    assert _compare_draws(spp, gpu_ctx, pepo, 10, 6, 3, 1e-9, 1e-9) == 0
+    assert _compare_draws(spp, gpu_ctx, pepo, 10, 6, 3, 1e-9, 1e-9) == []
 
 
 def test_draw_dense_xbasis_fp64(spp, gpu_ctx):
     say("draw dense X basis fp64")
     L = Lattice(3, 3)
     pepo = embed_identity_pepo(L, 2, 0.5, seed=14, symmetric=False)
-    assert _compare_draws(spp, gpu_ctx, pepo, 8, 6, 4, 1e-10, 1e-10) == 0
+    assert _compare_draws(spp, gpu_ctx, pepo, 8, 6, 4, 1e-10, 1e-10) == []
 
 
 def test_draw_fp32(spp, gpu_ctx):
@@ -99,7 +114,10 @@ def test_draw_fp32(spp, gpu_ctx):
     pepo = embed_identity_pepo(L, (1, 1), 0.6, seed=12)
     mism = _compare_draws(spp, gpu_ctx, pepo, 6, 8, 2, 2e-5, 2e-5, precision="fp32")
     gpu_ctx.set_precision("fp64")
-    assert mism <= 1
+    # FP32 conditionals agree with the oracle's to 2e-5 (checked above on the
+    # identical draws): a draw may differ only where the uniform lies within
+    # that bound of the cumulative boundary
+    assert all(d <= 2e-5 for d in mism), mism
 
 
 def test_evaluation_mode_sums_to_one(spp, gpu_ctx):

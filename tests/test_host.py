@@ -164,3 +164,61 @@ def test_gradient_unsort_inverts_parity_sort(spp):
     site = spp.Site(a, par)
     srt, _ = spp.parity_sorted(site)
     np.testing.assert_array_equal(contraction._unsort(srt, site.parity), a)
+
+
+def test_flop_counter_block_products_match_brute_force(spp):
+    """The roofline yardstick counts sum_pi |M_pi||K_pi||N_pi| multiply-adds
+    for a contraction of Z2-neutral tensors: checked against enumerating every
+    (m, k, n) index triple with parity(m) = parity(k) = parity(n)."""
+    from stopeps_jl_amd import flops
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        groups = [[(int(rng.integers(0, 3)), int(rng.integers(0, 3))) for _ in range(rng.integers(1, 3))]
+                  for _ in range(3)]
+        if any(flops.dim(flops.fuse(*g)) == 0 for g in groups):
+            continue
+
+        def parities(g):
+            out = np.zeros(1, dtype=int)
+            for (e, o) in g:
+                out = (out[:, None] + np.array([0] * e + [1] * o)[None, :]).reshape(-1) % 2
+            return out
+        pm, pk, pn = (parities(g) for g in groups)
+        brute = int(np.sum((pm[:, None, None] == pk[None, :, None]) & (pk[None, :, None] == pn[None, None, :])))
+        assert flops.macs(*groups) == brute
+
+
+def test_flop_counter_reproduces_dense_draw_formula(spp):
+    """With every leg even (a dense model) the draw count equals the
+    closed-form per-site sum of right, candidate and selected left steps and
+    the absorb product over the row boundary bonds c(b) = min(chi, Delta^min(b, L-b))."""
+    from stopeps_jl_amd import flops
+    L, D, chi, d = 4, 3, 20, 2
+    Dl = D * D
+    layers = {}
+    for x in range(L):
+        for y in range(L):
+            w = 1 if x == 0 else Dl
+            n = 1 if y == 0 else Dl
+            e = 1 if x == L - 1 else Dl
+            s = 1 if y == L - 1 else Dl
+            layers[(x, y)] = ((w, 0), (n, 0), (e, 0), (s, 0))
+    c = [1] + [min(chi, Dl ** min(b, L - b)) for b in range(1, L)] + [1]
+    total = 0
+    for y in range(L):
+        for x in range(L):
+            (w, _), (n, _), (e, _), (s, _) = layers[(x, y)]
+            tl, tr = (1, 1) if y == 0 else (c[x], c[x + 1])
+            bl, br = (1, 1) if y == L - 1 else (c[x], c[x + 1])
+            right = tl * n * e * br * tr + tl * w * s * br * n * e + tl * w * bl * s * br
+            left = w * bl * n * tr * tl + bl * tr * e * s * w * n + tr * e * br * bl * s
+            absorb = tl * tr * w * e * s * n
+            total += right + d * left + (x < L - 1) * left + absorb
+    # a dense bond is one (even) sector: count with all-even boundary bonds
+    orig = flops.even_split
+    try:
+        flops.even_split = lambda cc: (cc, 0)
+        got = flops.draw_macs(layers, L, L, chi)
+    finally:
+        flops.even_split = orig
+    assert got == total
