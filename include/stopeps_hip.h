@@ -62,7 +62,11 @@ int spp_set_precision(spp_ctx* ctx, int precision);
  * "boundary_cap_odd" (the sector budget of `BoundaryMPS(chi, sector_space)`,
  * boundary_mps.jl:11-18: every boundary compression keeps truncrank(chi)
  * intersected with at most this many values of each bond parity; negative =
- * no budget, the default; applies to the calls made while it is set). */
+ * no budget, the default; applies to the calls made while it is set);
+ * "fu_fail_sample", "fu_fail_bond", "fu_fail_code" (a test hook of the full
+ * update's failure handling: bond update fu_fail_bond, in sweep order, of
+ * sample fu_fail_sample of the next steps is treated as failed with
+ * fu_fail_code; a negative sample turns it off). */
 int spp_set_option(spp_ctx* ctx, const char* name, double value);
 
 /* counters[0] = contraction flops issued (2*M*N*K over parity blocks),
@@ -117,13 +121,24 @@ int spp_state_download(spp_ctx* ctx, const spp_state* state, double* const* data
  * normalizations [n][Lx*Ly] (`normalize_state!` log norms, added to each
  * state's log_amplitude); max_env_trunc [n]; max_boundary_entropy [n] the
  * largest pre-truncation Schmidt entropy (nats) over every boundary cut of the
- * step (`record_entropy=true`, full_update.jl:560-591; NULL: not recorded).
- * The PSD repair of the bond metric
- * (full_update.jl:177-330) is not available: a bond that needs it fails the
- * call with the reference's message. */
+ * step (`record_entropy=true`, full_update.jl:560-591; NULL: not recorded);
+ * status [n] the per-sample outcome: 0 ok, else the failure of the first bond
+ * that failed (1 nonfinite norm metric, 2 zero norm metric, 3 zero gated
+ * target, 4 negative residual needing the PSD repair, 5 nonfinite
+ * residual/target, 6 singular gauge, 7 workspace, 8 eigen/SVD sweep cap
+ * reached without convergence), 9 nonfinite row environment, 10 zero or
+ * nonfinite site norm.
+ * Failure semantics: a sample that fails keeps its sites from the failing
+ * bond on (no later bond of the step writes it), the others finish the step;
+ * a failed state is marked and every compute entry point refuses it
+ * afterwards ("state invalidated by a failed full-update step"). With status
+ * NULL the call finishes the step and then returns an error carrying the
+ * first failure's message. The reference raises at the failing bond
+ * (full_update.jl:491-501, svd.jl:1-11). */
 int spp_fu_evolution_step(spp_ctx* ctx, int n, spp_state* const* states, int chi, int order, double delta, int D,
                           const double* options, const double* gates_h, const double* gates_v, double* records,
-                          double* normalizations, double* max_env_trunc, double* max_boundary_entropy);
+                          double* normalizations, double* max_env_trunc, double* max_boundary_entropy,
+                          int* status);
 
 /* `SamplingCache(state, BoundaryMPS(chi); kind=:auxiliary, basis)`
  * (src/sampling/born.jl:56-73): double layers, aux-projected candidate layers
@@ -244,10 +259,21 @@ int spp_lognorm_grad_batch(spp_ctx* ctx, int n, const spp_state* const* states, 
  * final_cost, residual, target_norm, iterations, limit_reached, stop (0
  * disabled, 1 iteration limit, 2 error tolerance, 3 residual fallback),
  * status (0 ok, 1 nonfinite metric, 2 zero metric, 3 zero target, 4 PSD repair
- * needed, 5 nonfinite cost, 6 singular gauge), kept_even, kept_odd,
- * floored eigenvalues, 0}. Returns 0, or -3 on bad arguments. */
+ * needed, 5 nonfinite cost, 6 singular gauge, 7 workspace, 8 an eigen/SVD
+ * solve reached its sweep cap), kept_even, kept_odd, floored eigenvalues,
+ * nonconverged eigen/SVD solves}. Returns 0, or -3 on bad arguments. */
 int spp_selftest_fu_bond(const int* dims, const double* options, const double* metric, const double* target,
                          double* left, double* right, double* record);
+
+/* Host run of the bond solve's Jacobi eigensolver (kind 0: symmetric n x n A,
+ * m = n; eigenvalues unordered) or one-sided Jacobi SVD (kind 1: m x n A,
+ * m >= n; singular values descending) with at most max_sweeps sweeps
+ * (selftest; no GPU needed). The device solve caps both at 80 sweeps and
+ * reports a solve that reaches the cap (record slot 11, status 8), where the
+ * reference retries gesdd -> gesvd and then raises (src/states/svd.jl:1-11).
+ * converged: 1 when the sweeps stopped before the cap. Returns 0, or -3 on bad
+ * arguments. */
+int spp_selftest_jacobi(int kind, int m, int n, const double* A, int max_sweeps, double* values, int* converged);
 
 /* Host run of the reduced-pair stage that the device FU kernel runs
  * (selftest; no GPU needed; `_reduced_pair`, src/evolution/full_update.jl:21-29):

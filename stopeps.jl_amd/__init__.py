@@ -15,7 +15,7 @@ from .sampling import (Context, DeviceState, SamplingCache, BornError, sample_co
 from .measurement import (TFIM, spin_operators, operator_factors, EnergyError, measure_energies, measure_energy,
                           measure_energy_states)
 from .contraction import NormError, lognorm, lognorms, lognorm_gradient, lognorm_gradients, lognorm_rrule
-from .evolution import FullUpdate, trotter_gate, evolution_step, evolution_steps
+from .evolution import FullUpdate, FullUpdateError, trotter_gate, evolution_step, evolution_steps
 from .particles import (Particle, ensemble_statistics, step_schedule, purify, stochastic_ensemble, new_particles,
                         advance_particles, statistics, group_configurations)
 from .config import RunConfig, ResamplingOptions, validate_config, temperature_nodes, parameter_dict
@@ -32,4 +32,4 @@ __all__ = ["synthetic_thermal_pepo", "random_peps", "Lattice", "Site", "FiniteSt
            "step_schedule", "purify", "stochastic_ensemble", "new_particles", "advance_particles", "statistics",
            "group_configurations", "normalize_state", "reembed_bonds", "RunConfig", "ResamplingOptions",
            "validate_config", "parameter_dict", "ThermalCheckpoint", "run_purification", "run_stochastic",
-           "save_state_npz", "load_state_npz"]
+           "save_state_npz", "load_state_npz", "FullUpdateError"]

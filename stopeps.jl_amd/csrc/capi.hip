@@ -25,6 +25,13 @@ struct spp_cache {
   std::unique_ptr<CacheDev<double>> d;
 };
 
+// A state on which a full-update step failed is refused by the compute entry
+// points (it keeps its sites from the failing bond on).
+static void usable(const StateDev& s) {
+  if (s.failed)
+    throw HipError(std::string("state invalidated by a failed full-update step: ") + fu_failure_message(s.failed));
+}
+
 template <typename F>
 static int guarded(spp_ctx* ctx, F&& f) {
   try {
@@ -106,6 +113,13 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value) {
     } else if (n == "boundary_cap_even" || n == "boundary_cap_odd") {
       if (!(value == value) || value > 1e9) throw HipError(n + " must be an integer (negative: no budget)");
       ctx->c.bcap[n == "boundary_cap_even" ? 0 : 1] = value < 0 ? -1 : (int)value;
+    } else if (n == "fu_fail_sample") {
+      ctx->c.fu_fail_sample = value < 0 ? -1 : (long long)value;
+    } else if (n == "fu_fail_bond") {
+      ctx->c.fu_fail_bond = (int)value;
+    } else if (n == "fu_fail_code") {
+      if (value < 0 || value > 10) throw HipError("fu_fail_code must lie in [0, 10]");
+      ctx->c.fu_fail_code = (int)value;
     } else if (n == "chunk") {
       if (value < 0) throw HipError("chunk must be >= 0");
       ctx->c.chunk = (long long)value;
@@ -230,7 +244,8 @@ int spp_state_download(spp_ctx* ctx, const spp_state* state, double* const* data
 
 int spp_fu_evolution_step(spp_ctx* ctx, int n, spp_state* const* states, int chi, int order, double delta, int D,
                           const double* options, const double* gates_h, const double* gates_v, double* records,
-                          double* normalizations, double* max_env_trunc, double* max_boundary_entropy) {
+                          double* normalizations, double* max_env_trunc, double* max_boundary_entropy,
+                          int* status) {
   return guarded(ctx, [&] {
     if (n < 0) throw HipError("negative state count");
     if (n > 0 && !states) throw HipError("null state list");
@@ -240,6 +255,7 @@ int spp_fu_evolution_step(spp_ctx* ctx, int n, spp_state* const* states, int chi
     std::vector<StateDev*> v(n);
     for (int i = 0; i < n; ++i) {
       if (!states[i]) throw HipError("null state");
+      usable(states[i]->s);
       v[i] = &states[i]->s;
     }
     FuParams P;
@@ -254,6 +270,7 @@ int spp_fu_evolution_step(spp_ctx* ctx, int n, spp_state* const* states, int chi
     o.normalizations = normalizations;
     o.max_env_trunc = max_env_trunc;
     o.max_entropy = max_boundary_entropy;
+    o.status = status;
     if (ctx->c.precision == 1) fu_evolution_step<float>(ctx->c, v, chi, P, o);
     else fu_evolution_step<double>(ctx->c, v, chi, P, o);
   });
@@ -272,6 +289,7 @@ int spp_sampling_cache_build(spp_ctx* ctx, const spp_state* pepo, int chi, const
     if (!pepo || !out) throw HipError("null argument");
     if (chi < 1) throw HipError("chi must be positive");
     if (pepo->s.kind != 0) throw HipError("the sampling cache needs a PEPO");
+    usable(pepo->s);
     SPP_CHECK(hipSetDevice(ctx->c.device));
     c->precision = ctx->c.precision;
     if (c->precision == 1) {
@@ -374,6 +392,7 @@ int spp_measure_energy_states(spp_ctx* ctx, int n, const spp_state* const* state
     std::vector<const StateDev*> v(n);
     for (int i = 0; i < n; ++i) {
       if (!states[i]) throw HipError("null state");
+      usable(states[i]->s);
       v[i] = &states[i]->s;
     }
     EnergyOps ops;
@@ -407,6 +426,7 @@ int spp_lognorm_batch(spp_ctx* ctx, int n, const spp_state* const* states, int c
     std::vector<const StateDev*> v(n);
     for (int i = 0; i < n; ++i) {
       if (!states[i]) throw HipError("null state");
+      usable(states[i]->s);
       v[i] = &states[i]->s;
     }
     LognormOutputs o;
@@ -428,6 +448,7 @@ int spp_lognorm_grad_batch(spp_ctx* ctx, int n, const spp_state* const* states, 
     std::vector<const StateDev*> v(n);
     for (int i = 0; i < n; ++i) {
       if (!states[i]) throw HipError("null state");
+      usable(states[i]->s);
       v[i] = &states[i]->s;
     }
     GradOutputs o;
@@ -453,8 +474,20 @@ int spp_selftest_fu_bond(const int* dims, const double* options, const double* m
     fu::Result r = fu_bond_host(g, o, metric, target, left, right);
     const double v[12] = {r.initial_cost, r.final_cost, r.residual, r.target_norm, (double)r.iterations,
                           (double)r.limit_reached, (double)r.stop, (double)r.status, (double)r.kept_even,
-                          (double)r.kept_odd, (double)r.floored, 0.0};
+                          (double)r.kept_odd, (double)r.floored, (double)r.nonconverged};
     for (int i = 0; i < 12; ++i) record[i] = v[i];
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
+int spp_selftest_jacobi(int kind, int m, int n, const double* A, int max_sweeps, double* values, int* converged) {
+  try {
+    if ((kind != 0 && kind != 1) || m <= 0 || n <= 0 || (kind == 0 && m != n) || (kind == 1 && m < n) || !A ||
+        !values || !converged || max_sweeps < 0)
+      return -3;
+    *converged = jacobi_host(kind, m, n, A, max_sweeps, values) ? 1 : 0;
     return 0;
   } catch (...) {
     return -1;

@@ -186,6 +186,11 @@ struct Context {
   LaunchTimer timer;
   EntropyWork entropy;      // boundary-entropy spectra of the current absorb
   DeviceScratch dot_part;   // partial sums of the chunked dot products
+  // Test hook of the full update's failure handling: treat bond update
+  // fu_fail_bond (sweep order) of sample fu_fail_sample as failed with
+  // fu_fail_code (fu_fail_sample < 0: off).
+  long long fu_fail_sample = -1;
+  int fu_fail_bond = -1, fu_fail_code = 0;
   void flush_timer();       // accumulate finished event pairs into counters
 };
 

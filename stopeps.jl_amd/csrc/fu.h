@@ -9,6 +9,12 @@ namespace spp {
 fu::Result fu_bond_host(const fu::Dims& g, const fu::Opts& o, const double* N4, const double* target, double* A,
                         double* B);
 
+// Host run of the bond solve's Jacobi eigensolver (kind 0: symmetric n x n A,
+// eigenvalues unordered) or one-sided Jacobi SVD (kind 1: m x n A with m >= n,
+// singular values descending) with a sweep cap; returns whether it converged
+// within max_sweeps (selftest).
+bool jacobi_host(int kind, int m, int n, const double* A, int max_sweeps, double* values);
+
 // Host run of the reduced-pair stage of the device FU for one sample
 // (selftest): QR of both sites (live sectors only) into pseudo-site Q
 // tensors (W,N,P,A,E,S slots with Aux in P and r in the bond leg's slot),
@@ -26,6 +32,25 @@ struct FuParams {
   const double* gates_v = nullptr;  // host [(Ly-1)*Lx][d^4]: bond (x,y)-(x,y+1) at y*Lx+x
 };
 
+// Per-sample outcome of a full-update step: 0 ok, 1..8 the bond-solve status
+// of the first bond that failed (fu::Status), or the codes below.
+constexpr int kFuNonfiniteEnvironment = 9;  // a row environment of the sample became nonfinite
+constexpr int kFuZeroTensor = 10;           // a site norm was zero or nonfinite at normalize_state!
+inline const char* fu_failure_message(int code) {
+  static const char* msg[] = {"ok",
+                              "nonfinite FU norm metric",
+                              "zero FU norm metric",
+                              "gate produced a zero target",
+                              "negative FU residual needs PSD repair",
+                              "nonfinite FU residual/target",
+                              "singular FU gauge",
+                              "FU workspace exhausted",
+                              "FU eigen/SVD solve did not converge",
+                              "nonfinite row environment",
+                              "zero or nonfinite tensor after the full update"};
+  return (code >= 0 && code <= 10) ? msg[code] : "unknown failure";
+}
+
 struct FuOutputs {
   double* records = nullptr;         // [n][bond updates][8]: initial_cost, final_cost, residual, iterations,
                                      // limit_reached, stop, kept_even, kept_odd (sweep order)
@@ -34,6 +59,10 @@ struct FuOutputs {
   // [n] maximum pre-truncation Schmidt entropy over every boundary cut of the
   // step (record_entropy, full_update.jl:560-591); nullptr: not recorded
   double* max_entropy = nullptr;
+  // [n] per-sample outcome (0 ok, else a failure code above); a failed sample
+  // keeps its sites from the failing bond on and its state is marked failed.
+  // nullptr: the call fails after finishing the step when any sample failed.
+  int* status = nullptr;
 };
 
 // `evolution_step!` (full_update.jl:563-597) with FullUpdate on every state

@@ -33,6 +33,19 @@ fu::Result fu_bond_host(const fu::Dims& g, const fu::Opts& o, const double* N4, 
   return fu::fu_bond(one, g, o, n, t, a, b, w);
 }
 
+bool jacobi_host(int kind, int m, int n, const double* A, int max_sweeps, double* values) {
+  const fu::Team one{0, 1, nullptr, nullptr, 0};
+  std::vector<double> a(A, A + (size_t)m * n);
+  fu::View av{a.data(), 1}, w{values, 1};
+  if (kind == 0) {
+    std::vector<double> V((size_t)n * n), cs(3 * (size_t)(n / 2 + 1));
+    return fu::jacobi_eigh(one, av, n, w, fu::View{V.data(), 1}, fu::View{cs.data(), 1}, max_sweeps);
+  }
+  std::vector<double> U((size_t)m * n), Vt((size_t)n * n), Vw((size_t)n * n);
+  return fu::jacobi_svd_tall(one, av, m, n, fu::View{U.data(), 1}, w, fu::View{Vt.data(), 1},
+                             fu::View{Vw.data(), 1}, max_sweeps);
+}
+
 namespace {
 
 // (ket i, bra j) pair index in the parity-sorted pair order of the double layers
@@ -328,7 +341,7 @@ fu_core_kernel(const T* nf, long long nf_stride, const double* target, const int
     rr[8] = res.kept_even;
     rr[9] = res.kept_odd;
     rr[10] = res.floored;
-    rr[11] = 0.0;
+    rr[11] = res.nonconverged;
     bond_live[(long long)s * 2] = res.kept_even;
     bond_live[(long long)s * 2 + 1] = res.kept_odd;
   }
@@ -341,10 +354,10 @@ struct RestoreJob {
 
 __global__ void fu_restore_kernel(const RestoreJob* jobs, int nb, FuSide sa, FuSide sb, int d, const double* qa,
                                   long long qa_stride, const double* qb, long long qb_stride, const double* aout,
-                                  const double* bout, int D, int nbtot) {
+                                  const double* bout, int D, int nbtot, const int* failed) {
   const int s = blockIdx.y;
   const int side = blockIdx.z;
-  if (s >= nb) return;
+  if (s >= nb || failed[s]) return;  // a failed sample keeps its sites
   const FuSide& F = side == 0 ? sa : sb;
   long long total = 1;
   for (int k = 0; k < 6; ++k) total *= F.leg[k];
@@ -383,9 +396,9 @@ __global__ void fu_restore_kernel(const RestoreJob* jobs, int nb, FuSide sa, FuS
 
 // live[(s*ns + site)*8 + v*2 + q] <- bond_live[s*2 + q] for the bond legs of A and B
 __global__ void fu_set_live_kernel(int* live, int ns, int site_a, int va, int site_b, int vb, const int* bond_live,
-                                   int nb) {
+                                   int nb, const int* failed) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nb) return;
+  if (s >= nb || failed[s]) return;
   for (int q = 0; q < 2; ++q) {
     live[((long long)s * ns + site_a) * 8 + va * 2 + q] = bond_live[(long long)s * 2 + q];
     live[((long long)s * ns + site_b) * 8 + vb * 2 + q] = bond_live[(long long)s * 2 + q];
@@ -600,6 +613,8 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
     if (s0.sites[i].dim[2] != d || s0.sites[i].even[2] != ed) throw HipError("full update: physical legs differ");
   if (ns == 1) {
     onsite_step(ctx, states, P, out);
+    if (out.status)
+      for (int s = 0; s < n; ++s) out.status[s] = 0;
     return;
   }
   const int nh = Ly * (Lx - 1), nv = (Ly - 1) * Lx;
@@ -743,6 +758,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
     scratch.clear();
   };
   const int nrec_bonds = P.order == 2 ? 2 * (nh + nv) : nh + nv;
+  int first_failure = 0;  // first per-sample failure code of the call (reported when no status array)
   try {
     T* layers = (T*)dev(sizeof(T) * (size_t)layer_elems * nb);
     std::vector<T*> site_base(ns);
@@ -808,6 +824,8 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
     RestoreJob* rjobs = (RestoreJob*)dev(sizeof(RestoreJob) * (size_t)nb);
     int* status = (int*)dev(sizeof(int) * nb);
     int* fail = (int*)dev(sizeof(int) * nb);
+    int* dfailed = (int*)dev(sizeof(int) * nb);  // per sample: first failure code of this step, 0 ok
+    std::vector<int> hfailed(nb, 0);
     double* nrm = (double*)dev(sizeof(double) * nb);
     double* fac = (double*)dev(sizeof(double) * nb);
     std::vector<double> hrec((size_t)12 * nb);
@@ -829,6 +847,8 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
       }
       SPP_CHECK(launch_fill<int>(status, nb, 0, st));
       SPP_CHECK(launch_fill<int>(fail, nb, -1, st));
+      SPP_CHECK(launch_fill<int>(dfailed, nb, 0, st));
+      std::fill(hfailed.begin(), hfailed.end(), 0);
       if (emax) SPP_CHECK(launch_fill<double>(emax, nb, 0.0, st));
       std::vector<double> env_err(cur, 0.0);
       auto relayer = [&](const std::vector<int>& sites) {
@@ -1177,14 +1197,17 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
             SPP_CHECK(hipGetLastError());
             SPP_CHECK(hipMemcpyAsync(hrec.data(), drec, sizeof(double) * 12 * cur, hipMemcpyDeviceToHost, st));
             SPP_CHECK(hipStreamSynchronize(st));
+            bool newly_failed = false;
             for (int s = 0; s < cur; ++s) {
-              const int code = (int)hrec[(size_t)s * 12 + 7];
-              if (code != 0) {
-                static const char* msg[] = {"ok", "nonfinite FU norm metric", "zero FU norm metric",
-                                            "gate produced a zero target", "PSD repair of the FU metric needed",
-                                            "nonfinite FU residual/target", "singular FU gauge", "FU workspace"};
-                throw HipError(std::string("full update: ") + msg[std::min(code, 7)]);
+              int code = (int)hrec[(size_t)s * 12 + 7];
+              if (ctx.fu_fail_sample == c0 + s && ctx.fu_fail_bond == rec_bond) code = ctx.fu_fail_code;
+              if (code != 0 && hfailed[s] == 0) {
+                // the sample keeps its sites from here on (restores are skipped)
+                // and reports the first failure; the others go on
+                hfailed[s] = code;
+                newly_failed = true;
               }
+              if (hfailed[s] != 0) continue;
               if (out.records) {
                 double* rr = out.records + ((size_t)(c0 + s) * nrec_bonds + rec_bond) * 8;
                 rr[0] = hrec[(size_t)s * 12 + 0];
@@ -1197,17 +1220,18 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
                 rr[7] = hrec[(size_t)s * 12 + 9];
               }
             }
+            if (newly_failed) upload(dfailed, hfailed.data(), sizeof(int) * cur, st);
             ++rec_bond;
             // --- restore the sites, their bond bookkeeping and layers ---
             {
               long long tmax = std::max(s0.sites[ia].size(), s0.sites[ib].size());
               dim3 grid((unsigned)((tmax + 255) / 256), (unsigned)cur, 2);
               hipLaunchKernelGGL(fu_restore_kernel, grid, dim3(256), 0, st, rjobs, cur, G.sa, G.sb, d, qa, qmax, qbuf,
-                                 qmax, aout, bout, G.Dcap, nb);
+                                 qmax, aout, bout, G.Dcap, nb, dfailed);
               SPP_CHECK(hipGetLastError());
               const int va = horizontal ? 2 : 3, vbl = horizontal ? 0 : 1;
               hipLaunchKernelGGL(fu_set_live_kernel, dim3((cur + 63) / 64), dim3(64), 0, st, live, ns, ia, va, ib, vbl,
-                                 blive, cur);
+                                 blive, cur, dfailed);
               SPP_CHECK(hipGetLastError());
               relayer({ia, ib});
             }
@@ -1238,7 +1262,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
       SPP_CHECK(hipMemcpyAsync(hstatus.data(), status, sizeof(int) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipStreamSynchronize(st));
       for (int s = 0; s < cur; ++s)
-        if (hstatus[s] != 0) throw HipError("full update: nonfinite row environment");
+        if (hstatus[s] != 0 && hfailed[s] == 0) hfailed[s] = kFuNonfiniteEnvironment;
       // normalize_state! (finite.jl:89-100): site norms into log_amplitude, row-major sites
       {
         std::vector<double*> hp((size_t)ns * cur);
@@ -1259,8 +1283,9 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
         SPP_CHECK(hipStreamSynchronize(st));
         for (size_t k = 0; k < hn.size(); ++k) {
           const double v = std::sqrt(hn[k]);
-          if (!(v > 0.0 && std::isfinite(v))) throw HipError("zero or nonfinite tensor after the full update");
-          hn[k] = v;
+          const int s = (int)(k % cur);
+          if (!(v > 0.0 && std::isfinite(v)) && hfailed[s] == 0) hfailed[s] = kFuZeroTensor;
+          hn[k] = v > 0.0 && std::isfinite(v) ? v : 1.0;  // a failed sample is left unscaled
         }
         for (int i = 0; i < ns; ++i)
           for (int s = 0; s < cur; ++s) {
@@ -1295,6 +1320,11 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
         states[c0 + s]->beta += 2.0 * P.delta;
         if (out.max_env_trunc) out.max_env_trunc[c0 + s] = env_err[s];
         if (emax) out.max_entropy[c0 + s] = hmt[s];
+        if (out.status) out.status[c0 + s] = hfailed[s];
+        if (hfailed[s] != 0) {
+          states[c0 + s]->failed = hfailed[s];
+          if (first_failure == 0) first_failure = hfailed[s];
+        }
       }
     }
   } catch (...) {
@@ -1305,6 +1335,7 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
   }
   SPP_CHECK(hipStreamSynchronize(st));
   cleanup();
+  if (first_failure != 0 && !out.status) throw HipError(std::string("full update: ") + fu_failure_message(first_failure));
 }
 
 template void fu_evolution_step<float>(Context&, const std::vector<StateDev*>&, int, const FuParams&, FuOutputs&);

@@ -162,12 +162,19 @@ struct Opts {
 
 enum Stop { kDisabled = 0, kIterationLimit = 1, kErrorTolerance = 2, kResidualFallback = 3 };
 enum Status { kOk = 0, kNonfiniteMetric = 1, kZeroMetric = 2, kZeroTarget = 3, kRepairNeeded = 4,
-              kNonfiniteCost = 5, kSingularGauge = 6, kWorkspace = 7 };
+              kNonfiniteCost = 5, kSingularGauge = 6, kWorkspace = 7, kNonconverged = 8 };
+
+// Sweep cap of the Jacobi eigensolver and SVD. Both converge quadratically
+// once rotations are small; a solve that reaches the cap is reported
+// (Result::nonconverged, status kNonconverged), as the reference raises when
+// its SVD fails to converge after the gesdd -> gesvd retry (src/states/svd.jl:1-11).
+constexpr int kJacobiSweeps = 80;
 
 struct Result {
   double initial_cost, final_cost, residual, target_norm;
   int iterations, limit_reached, stop, status;
   int kept_even, kept_odd, floored;
+  int nonconverged;  // eigen/SVD solves of this bond that reached the sweep cap
 };
 
 constexpr double kEps = 2.220446049250313e-16;
@@ -190,7 +197,8 @@ FU_HD inline bool finite(double x) { return fabs(x) <= 1.7976931348623157e308; }
 // basis chosen inside a degenerate eigenspace.
 constexpr int kU = 4;  // tasks whose loads a lane issues together
 
-FU_HD inline void jacobi_eigh(const Team& t, View A, int n, View w, View V, View cs) {
+FU_HD inline bool jacobi_eigh(const Team& t, View A, int n, View w, View V, View cs,
+                               int max_sweeps = kJacobiSweeps) {
   for (int i = t.lane; i < (int)n * n; i += t.size) V[i] = (i / n) == (i % n) ? 1.0 : 0.0;
   double fl = 0.0;
   for (int i = t.lane; i < (int)n * n; i += t.size) fl += A[i] * A[i];
@@ -199,14 +207,18 @@ FU_HD inline void jacobi_eigh(const Team& t, View A, int n, View w, View V, View
   const int np = m / 2;
   // pair i of round r: slots (i, m-1-i) of the sequence [0, r rotations of 1..m-1]
   auto slot = [&](int r, int j) { return j == 0 ? 0 : 1 + (j - 1 + r) % (m - 1); };
-  for (int sweep = 0; sweep < 80; ++sweep) {
+  bool converged = false;
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     double ol = 0.0;
     for (int i = t.lane; i < (int)n * n; i += t.size) {
       const int p = (int)(i / n), q = (int)(i % n);
       if (q > p) ol += A[i] * A[i];
     }
     const double off = t.sum(ol);
-    if (off <= 1e-34 * fro || off == 0.0) break;
+    if (off <= 1e-34 * fro || off == 0.0) {
+      converged = true;
+      break;
+    }
     for (int r = 0; r < m - 1; ++r) {
       // rotation of each pair (c = 2 marks "no rotation")
       for (int i = t.lane; i < np; i += t.size) {
@@ -303,6 +315,7 @@ FU_HD inline void jacobi_eigh(const Team& t, View A, int n, View w, View V, View
   }
   for (int i = t.lane; i < n; i += t.size) w[i] = A[(int)i * n + i];
   t.sync();
+  return converged;
 }
 
 // Householder QR of the m x n matrix M (row-major, destroyed): Q (m x k; not
@@ -375,10 +388,12 @@ FU_HD inline void householder_qr(const Team& t, View M, int m, int n, View Q, Vi
 // One-sided Jacobi SVD of the m x n matrix M (row-major, destroyed) with
 // m >= n: U (m x n), S (n, descending), Vt (n x n). Vw: n*n scratch. Rows of M
 // and Vw are lane-owned; the three column products of a pair are one reduction.
-FU_HD inline void jacobi_svd_tall(const Team& t, View M, int m, int n, View U, View S, View Vt, View Vw) {
+FU_HD inline bool jacobi_svd_tall(const Team& t, View M, int m, int n, View U, View S, View Vt, View Vw,
+                                   int max_sweeps = kJacobiSweeps) {
   for (int i = t.lane; i < n; i += t.size)
     for (int j = 0; j < n; ++j) Vw[(int)i * n + j] = i == j ? 1.0 : 0.0;
-  for (int sweep = 0; sweep < 80; ++sweep) {
+  bool converged = false;
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     bool rotated = false;
     for (int p = 0; p < n - 1; ++p)
       for (int q = p + 1; q < n; ++q) {
@@ -406,7 +421,10 @@ FU_HD inline void jacobi_svd_tall(const Team& t, View M, int m, int n, View U, V
           Vw[(int)i * n + q] = s * x + c * y;
         }
       }
-    if (!rotated) break;
+    if (!rotated) {
+      converged = true;
+      break;
+    }
   }
   // column norms, then a selection sort into descending order
   for (int j = 0; j < n; ++j) {
@@ -446,18 +464,21 @@ FU_HD inline void jacobi_svd_tall(const Team& t, View M, int m, int n, View U, V
     for (int i = t.lane; i < n; i += t.size) Vt[(int)j * n + i] = Vw[(int)i * n + j];
   }
   t.sync();
+  return converged;
 }
 
 // Compact SVD of an m x n matrix (any shape): U (m x k), S (k), Vt (k x n),
-// k = min(m, n). work: m*n + k*k + max(m,n)*k doubles.
-FU_HD inline void svd_compact(const Team& t, View M, int m, int n, View U, View S, View Vt, Work& w) {
+// k = min(m, n). work: m*n + k*k + max(m,n)*k doubles. Returns false when the
+// Jacobi sweeps reached their cap.
+FU_HD inline bool svd_compact(const Team& t, View M, int m, int n, View U, View S, View Vt, Work& w) {
   const long long mark = w.top;
+  bool converged;
   if (m >= n) {
     View T = w.take((int)m * n);
     for (int i = t.lane; i < (int)m * n; i += t.size) T[i] = M[i];
     View Vw = w.take((int)n * n);
     t.sync();
-    jacobi_svd_tall(t, T, m, n, U, S, Vt, Vw);
+    converged = jacobi_svd_tall(t, T, m, n, U, S, Vt, Vw);
   } else {
     // M^T = U' S V'^T  =>  M = V' S U'^T
     View T = w.take((int)m * n);
@@ -469,7 +490,7 @@ FU_HD inline void svd_compact(const Team& t, View M, int m, int n, View U, View 
     View Vv = w.take((int)m * m);
     View Vw = w.take((int)m * m);
     t.sync();
-    jacobi_svd_tall(t, T, n, m, Uu, S, Vv, Vw);
+    converged = jacobi_svd_tall(t, T, n, m, Uu, S, Vv, Vw);
     for (int idx = t.lane; idx < (int)m * m; idx += t.size) {
       const int i = (int)(idx / m), k = (int)(idx % m);
       U[idx] = Vv[(int)k * m + i];
@@ -481,6 +502,7 @@ FU_HD inline void svd_compact(const Team& t, View M, int m, int n, View U, View 
     t.sync();
   }
   w.top = mark;
+  return converged;
 }
 
 // ---------------------------------------------------------------------------
@@ -904,11 +926,11 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
         const View As{t.lds, 1}, Vs{t.lds + mm, 1}, cl{t.lds + 2 * mm, 1};
         for (int i = L; i < mm; i += NT) As[i] = Hq[i];
         t.sync();
-        jacobi_eigh(t, As, m, wq, Vs, cl);
+        if (!jacobi_eigh(t, As, m, wq, Vs, cl)) ++res.nonconverged;
         for (int i = L; i < mm; i += NT) vq[i] = Vs[i];
         t.sync();
       } else {
-        jacobi_eigh(t, Hq, m, wq, vq, cs);
+        if (!jacobi_eigh(t, Hq, m, wq, vq, cs)) ++res.nonconverged;
       }
       eoff += m;
       voff += (int)m * m;
@@ -1074,7 +1096,7 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
         }
       }
       t.sync();
-      svd_compact(t, Bm, mq[q], nq[q], Uv[q], Sv[q], Vv[q], w);
+      if (!svd_compact(t, Bm, mq[q], nq[q], Uv[q], Sv[q], Vv[q], w)) ++res.nonconverged;
     }
     // truncrank(D) over both sectors (ties: sector 0 first, earlier first),
     // truncspace caps (the bond capacity), trunctol(atol)
@@ -1335,7 +1357,7 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
       for (int idx = L; idx < (int)m * m; idx += NT)
         Cm[idx] = fC[(int)(lo + idx / m) * D + lo + idx % m];
       t.sync();
-      svd_compact(t, Cm, m, m, U, S, Vt, w);
+      if (!svd_compact(t, Cm, m, m, U, S, Vt, w)) ++res.nonconverged;
       for (int idx = L; idx < (int)m * m; idx += NT) {
         const int i = (int)(idx / m), k = (int)(idx % m);
         Us[(int)(lo + i) * D + lo + k] = U[(int)i * m + k] * sqrt(S[k]);
@@ -1378,6 +1400,7 @@ FU_HD inline Result fu_bond(const Team& t, const Dims& g, const Opts& o, View N4
   }
   t.sync();
   }
+  if (res.status == kOk && res.nonconverged > 0) res.status = kNonconverged;
   return res;
 }
 

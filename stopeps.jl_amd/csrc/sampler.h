@@ -32,6 +32,9 @@ struct StateDev {
   // per site, per virtual leg (W, N, E, S): live even / odd counts of the
   // bond (the sectors in use; the rest of the leg's capacity is zero)
   std::vector<int> live;
+  // nonzero: a full-update step failed on this state (its failure code); the
+  // compute entry points refuse it
+  int failed = 0;
   ~StateDev();
 };
 

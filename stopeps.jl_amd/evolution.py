@@ -34,6 +34,25 @@ class FullUpdate:
 
 STOP_REASONS = ("disabled", "iteration_limit", "error_tolerance", "residual_fallback")
 
+# per-sample outcomes of spp_fu_evolution_step (the reference's messages, full_update.jl:355-369, 491-501)
+FAILURES = ("ok", "nonfinite FU norm metric", "zero FU norm metric", "gate produced a zero target",
+            "negative FU residual needs PSD repair", "nonfinite FU residual/target", "singular FU gauge",
+            "FU workspace exhausted", "FU eigen/SVD solve did not converge", "nonfinite row environment",
+            "zero or nonfinite tensor after the full update")
+
+
+class FullUpdateError(RuntimeError):
+    """A full-update step failed for some samples; ``status`` holds every
+    sample's outcome (0 ok). Failed states keep their sites from the failing
+    bond on and are refused by later calls."""
+
+    def __init__(self, status):
+        self.status = np.asarray(status)
+        bad = np.nonzero(self.status)[0]
+        code = int(self.status[bad[0]])
+        super().__init__(f"full update failed for sample(s) {list(map(int, bad))}: "
+                         f"{FAILURES[code] if 0 <= code < len(FAILURES) else 'unknown failure'}")
+
 
 def _degree(lattice, i):
     """``_degree`` (tfim.jl:43), 0-based coordinates."""
@@ -78,7 +97,7 @@ def _gates(model, lattice, delta, ppar):
 
 
 def evolution_steps(ctx, states, model, delta, chi, method=None, order=2, D=None, record_entropy=False,
-                    boundary_space=None):
+                    boundary_space=None, check=True):
     """``evolution_step!(state, model, delta, D, FullUpdate(...), BoundaryMPS(chi);
     order, sector_space, record_entropy)`` for every uploaded state (``DeviceState``; same lattice
     and leg spaces up to the auxiliary charge), in place on the device.
@@ -90,13 +109,17 @@ def evolution_steps(ctx, states, model, delta, chi, method=None, order=2, D=None
     max_environment_truncation [n], max_boundary_entropy [n] (the largest
     pre-truncation Schmidt entropy over every boundary cut of the step; NaN
     unless ``record_entropy``). ``boundary_space``: the sector budget of the
-    environment boundaries (``BoundaryMPS(chi, sector_space)``)."""
+    environment boundaries (``BoundaryMPS(chi, sector_space)``). ``status`` [n]
+    is every sample's outcome (0 ok, else the index into ``FAILURES``); with
+    ``check`` a failure raises :class:`FullUpdateError` after the step (the
+    other samples have finished it)."""
     method = method or FullUpdate()
     devs = list(states)
     n = len(devs)
     if n == 0:
         return {"records": np.zeros((0, 0, 8)), "normalizations": np.zeros((0, 0)),
-                "max_environment_truncation": np.zeros(0), "max_boundary_entropy": np.zeros(0)}
+                "max_environment_truncation": np.zeros(0), "max_boundary_entropy": np.zeros(0),
+                "status": np.zeros(0, dtype=np.int32)}
     L = devs[0].state.lattice
     if any(not isinstance(d, DeviceState) for d in devs):
         raise TypeError("evolution_steps takes DeviceState objects (the states evolve in place on the device)")
@@ -114,17 +137,21 @@ def evolution_steps(ctx, states, model, delta, chi, method=None, order=2, D=None
     norms = np.zeros((n, L.Lx * L.Ly))
     mt = np.zeros(n)
     ent = np.zeros(n) if record_entropy else None
+    status = np.zeros(n, dtype=np.int32)
     handles = (ctypes.c_void_p * n)(*[d.handle.value for d in devs])
     o = method.options()
     with ctx.boundary_space(boundary_space):
         rc = ctx._lib.spp_fu_evolution_step(
             ctx.handle, n, handles, int(chi), int(order), float(delta), int(D or 0), _lib.dptr(o), _lib.dptr(gh),
-            _lib.dptr(gv), _lib.dptr(rec), _lib.dptr(norms), _lib.dptr(mt), _lib.dptr(ent))
+            _lib.dptr(gv), _lib.dptr(rec), _lib.dptr(norms), _lib.dptr(mt), _lib.dptr(ent), _lib.iptr(status))
     _lib.check(ctx.handle, rc)
     for d in devs:
         d.state.beta += 2 * delta
-    return {"records": rec, "normalizations": norms, "max_environment_truncation": mt,
-            "max_boundary_entropy": ent if ent is not None else np.full(n, np.nan)}
+    out = {"records": rec, "normalizations": norms, "max_environment_truncation": mt,
+           "max_boundary_entropy": ent if ent is not None else np.full(n, np.nan), "status": status}
+    if check and np.any(status):
+        raise FullUpdateError(status)
+    return out
 
 
 def evolution_step(ctx, state, model, delta, chi, method=None, order=2, D=None, record_entropy=False,

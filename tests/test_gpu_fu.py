@@ -229,3 +229,43 @@ def test_results_do_not_depend_on_batch_mates(spp, gpu_ctx):
     assert a.log_amplitude == b.log_amplitude
     for xy in a.lattice.site_order():
         np.testing.assert_array_equal(a[xy].data, b[xy].data)
+
+
+def test_failed_sample_is_isolated_and_refused(spp, gpu_ctx):
+    """Failure handling of a batched step, forced from the host (the test hook
+    marks one bond update of one sample as failed, every kernel runs
+    normally): the other samples finish the step exactly as without the
+    failure, the failed sample reports its code and keeps its sites, and later
+    calls refuse it."""
+    say("full update: a failed sample in a batch of 3")
+    ctx = gpu_ctx
+    ctx.set_precision("fp64")
+    L = Lattice(3, 3)
+    m = TFIM(J=1.0, h=2.0, symmetric=True)
+    model = spp.TFIM(m.J, m.h, symmetric=True)
+    method = spp.FullUpdate(10, 1e-8, 1e-10, 50, 1e-9)
+    ostates = [random_state(L, (1, 1), "peps", seed=70 + k) for k in range(3)]
+    ref = [spp.DeviceState(ctx, to_product_state(spp, o)) for o in ostates]
+    got = [spp.DeviceState(ctx, to_product_state(spp, o)) for o in ostates]
+    r_ref = spp.evolution_steps(ctx, ref, model, 0.05, 8, method)
+    try:
+        ctx.set_option("fu_fail_sample", 1)
+        ctx.set_option("fu_fail_bond", 3)
+        ctx.set_option("fu_fail_code", 4)
+        with pytest.raises(spp.FullUpdateError) as err:
+            spp.evolution_steps(ctx, got, model, 0.05, 8, method)
+    finally:
+        ctx.set_option("fu_fail_sample", -1)
+    assert list(err.value.status) == [0, 4, 0] and "needs PSD repair" in str(err.value)
+    for k in (0, 2):
+        a, b = got[k].download(), ref[k].download()
+        for xy in L.site_order():
+            np.testing.assert_allclose(a[xy].data, b[xy].data, rtol=1e-12, atol=1e-14)
+        assert abs(a.log_amplitude - b.log_amplitude) <= 1e-12 * max(1.0, abs(b.log_amplitude))
+    # the failed sample is refused by the compute entry points
+    with pytest.raises(spp._lib.HIPLibraryError, match="invalidated"):
+        spp.lognorms(ctx, [got[1]], 8)
+    with pytest.raises(spp._lib.HIPLibraryError, match="invalidated"):
+        spp.evolution_steps(ctx, [got[1]], model, 0.05, 8, method)
+    # it kept finite sites (no write after the failing bond)
+    assert all(np.all(np.isfinite(got[1].download()[xy].data)) for xy in L.site_order())

@@ -78,3 +78,32 @@ def test_device_bond_solve_matches_oracle(spp, direction, x, y, Dst, Dcap, its):
     assert abs(rec[1] - ref["final_cost"]) <= 1e-7 * max(ref["final_cost"], 1e-12) + 1e-14
     assert int(rec[4]) == ref["iterations"]
     assert int(rec[6]) == STOP[ref["stop_reason"]]
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_jacobi_solves_report_the_sweep_cap(spp, kind):
+    """The bond solve's eigensolver (kind 0) and SVD (kind 1), compiled for
+    the host: converged within the cap they match LAPACK; a solve stopped at
+    a cap too small to converge is reported as not converged (the device
+    turns that into bond status 8, where the reference raises after its
+    gesdd -> gesvd retry, src/states/svd.jl:1-11)."""
+    lib = spp._lib.load()
+    rng = np.random.default_rng(11 + kind)
+    m, n = (40, 40) if kind == 0 else (50, 30)
+    A = rng.standard_normal((m, n))
+    if kind == 0:
+        A = A + A.T
+    vals = np.zeros(n)
+    conv = np.zeros(1, dtype=np.int32)
+    assert lib.spp_selftest_jacobi(kind, m, n, spp._lib.dptr(np.ascontiguousarray(A)), 80, spp._lib.dptr(vals),
+                                   spp._lib.iptr(conv)) == 0
+    assert conv[0] == 1
+    ref = np.linalg.eigvalsh(A) if kind == 0 else np.linalg.svd(A, compute_uv=False)
+    got = np.sort(vals) if kind == 0 else vals
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12 * np.max(np.abs(ref)))
+    assert lib.spp_selftest_jacobi(kind, m, n, spp._lib.dptr(np.ascontiguousarray(A)), 1, spp._lib.dptr(vals),
+                                   spp._lib.iptr(conv)) == 0
+    assert conv[0] == 0
+    # bad shapes are rejected
+    assert lib.spp_selftest_jacobi(kind, 3, 5, spp._lib.dptr(np.zeros(15)), 80, spp._lib.dptr(np.zeros(5)),
+                                   spp._lib.iptr(conv)) == -3
