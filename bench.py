@@ -2,18 +2,26 @@
 """Born samples/s of the batched sampler on one MI355X.
 
 Workload (one "step"): one batched `sample_configuration` of B samples on the
-beta0 sampling cache of an 8x8 open-lattice Z2 PEPO with bond space
-Z2Space(0=>3,1=>3) (D=6), physical/auxiliary d=2, boundary chi=32, row-major
-draws with one injected uniform per site. The headline run computes the
-contraction GEMMs in FP32 on MFMA (FP64 eigensolvers, probabilities and
-log weights); the plain path is FP64 everywhere at batch 1. The cache build
-is timed separately and amortised over the whole ensemble.
+beta0 sampling cache of the 8x8 open TFIM (J=1, h=2.0) PEPO purified to
+beta0=0.5 (Z2 bond space Z2Space(0=>3,1=>3), D=6, physical/auxiliary d=2; the
+frozen fixture tests/golden/cfg2_pepo_beta05.npz), boundary chi=32, row-major
+draws with one injected uniform per site (sample indices 1, 2, ... of seed 1).
+The headline run computes the contraction GEMMs in FP32 on MFMA (FP64
+eigensolvers, probabilities and log weights); the plain path is FP64
+everywhere at batch 1, and the batched FP64 point runs the same batch in FP64.
+The cache build is timed separately and amortised over the whole ensemble.
+
+The roofline fraction uses the algorithmic work of a draw: the Z2 block-sparse
+GEMM flops of the reference's contraction sequence (stopeps.jl_amd/flops.py),
+per launch of the dominant kernel, over that kernel's average launch time.
 
 Prints one JSON line (rank 0).
 """
 import argparse
+import gc
 import json
 import os
+import platform
 import sys
 import time
 
@@ -25,58 +33,125 @@ sys.path.insert(0, ROOT)
 METRIC = ("Born samples/s, 8×8 TFIM D=6 χ=32: batched MFMA vs plain FP64, "
           "ΔE/site≤1e-6")
 PEAK = {"fp32": 157.3, "fp64": 78.6}  # MI355X dense matrix TFLOP/s
+PEPO_FIXTURE = os.path.join(ROOT, "tests", "golden", "cfg2_pepo_beta05.npz")
+ORACLE_FIXTURE = os.path.join(ROOT, "tests", "golden", "cfg2_oracle.npz")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(L=8, D=(3, 3), chi=32, seed=5, threads=None):
-    """Time the FP64 oracle on one row of a Born draw at the same shapes and
-    extrapolate to a draw (8 rows). One row = right row environments, the two
-    candidate left steps and the drawn left step per site, and the compressed
-    north absorb (src/sampling/born.jl:105-161). Boundaries are seeded random
-    tensors of the production shapes (cost depends on shapes only)."""
+def _cpu_row(args):
+    """One row of a Born draw by the FP64 oracle on one core (worker process):
+    right row environments, both candidate left steps and the drawn left step
+    per site, and the compressed north absorb (src/sampling/born.jl:105-161),
+    on the fixture PEPO's double layers; boundaries are seeded random Z2
+    tensors of the row's production shapes (the cost depends on shapes only)."""
+    y, chi, seed = args
     from threadpoolctl import threadpool_limits
-    from oracle.model import Lattice, random_state
-    from oracle.z2 import Z2, symmetrize
+    threadpool_limits(1)
+    sys.path.insert(0, ROOT)
     from oracle import boundary as bm
-    threads = threads or min(16, os.cpu_count() or 1)
-    rng = np.random.default_rng(seed)
-    lat = Lattice(L, L)
-    pepo = random_state(lat, D, "pepo", seed)
-    y = 2
-    rowl = [bm.double_layer(pepo[(x, y)]) for x in range(L)]
-    cands = [rowl[x] for x in range(L)]
+    from oracle import born as ob
+    from oracle.model import Lattice, State
+    from oracle.z2 import Z2, symmetrize
+    f = np.load(PEPO_FIXTURE, allow_pickle=False)
+    L = int(f["Lx"])
+    sites = [[Z2(f[f"site{x + yy * L}"], [f[f"par{x + yy * L}_{k}"].astype(np.int64) for k in range(6)])
+              for yy in range(L)] for x in range(L)]
+    pepo = State(Lattice(L, L), sites, "pepo")
+    basis = ob.resolve_basis(True)
+    cands = [[bm.double_layer(ob.project(pepo[(x, y)], basis, k)) for k in range(2)] for x in range(L)]
+    rowl = [c[0] for c in cands]
+    rng = np.random.default_rng(seed + y)
 
-    def bnd(phys_p):
-        half = np.array([0] * (chi // 2) + [1] * (chi - chi // 2))
+    def boundary(axis, trivial):
+        w = [E.p[0] for E in rowl]
+        e = [E.p[2] for E in rowl]
         out = []
         for x in range(L):
-            lp = np.array([0]) if x == 0 else half
-            rp = np.array([0]) if x == L - 1 else half
-            t = Z2(rng.standard_normal((len(lp), len(phys_p[x]), len(rp))), [lp, phys_p[x], rp])
+            def bond(b):
+                if trivial or b in (0, L):
+                    return np.array([0])
+                c = min(chi, int(np.prod([len(p) for p in e[:b]])), int(np.prod([len(p) for p in w[b:]])))
+                return np.array([0] * ((c + 1) // 2) + [1] * (c // 2))
+            lp, rp = bond(x), bond(x + 1)
+            t = Z2(rng.standard_normal((len(lp), len(rowl[x].p[axis]), len(rp))), [lp, rowl[x].p[axis], rp])
             out.append(symmetrize(t))
         return out
-    north = bnd([E.p[1] for E in rowl])
-    south = bnd([E.p[3] for E in rowl])
-    with threadpool_limits(threads):
+    north = boundary(1, y == 0)
+    south = boundary(3, y == L - 1)
+    t0 = time.perf_counter()
+    _, right = bm.row_environments(north, rowl, south, side="right")
+    left = bm.row_edge(north, rowl, south, "left")
+    for x in range(L):
+        steps = [bm.left_step(left, north[x], E, south[x]) for E in cands[x]]
+        _ = [bm.join_scalars(s, right[x + 1]) for s in steps]
+        if x < L - 1:
+            left = bm.safe_scale(bm.left_step(left, north[x], rowl[x], south[x]))
+    bm.absorb(bm.Boundary(north), rowl, chi, "north")
+    return y, time.perf_counter() - t0
+
+
+def host_info():
+    """Host CPU model, core count, ROCm version and BLAS library of this box."""
+    model = platform.processor()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        with open("/opt/rocm/.info/version") as fh:
+            rocm = fh.read().strip()
+    except OSError:
+        rocm = None
+    blas = None
+    try:
+        from threadpoolctl import threadpool_info
+        libs = [i for i in threadpool_info() if i.get("user_api") == "blas"]
+        if libs:
+            blas = f"{libs[0].get('internal_api')} {libs[0].get('version')}"
+    except Exception:
+        pass
+    return {"cpu": model, "nproc": os.cpu_count(), "rocm": rocm, "blas": blas, "numpy": np.__version__}
+
+
+def cpu_baseline(chi=32, seed=5, procs=None):
+    """The FP64 oracle (CPU restatement of the reference, not the reference,
+    which needs Julia) on the box's host cores: ``procs`` single-thread
+    processes, each timing one row of a Born draw; rows 0..7 are spread over
+    the processes and a draw's single-core time is the sum of the mean row
+    times. Throughput = procs / that time (each core drawing whole samples)."""
+    import multiprocessing as mp
+    procs = procs or min(16, os.cpu_count() or 1)
+    f = np.load(PEPO_FIXTURE, allow_pickle=False)
+    L = int(f["Ly"])
+    old = {k: os.environ.get(k) for k in ("OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS", "MKL_NUM_THREADS")}
+    for k in old:
+        os.environ[k] = "1"
+    try:
         t0 = time.perf_counter()
-        _, right = bm.row_environments(north, rowl, south, side="right")
-        left = bm.row_edge(north, rowl, south, "left")
-        for x in range(L):
-            steps = [bm.left_step(left, north[x], E, south[x]) for E in (cands[x], cands[x])]
-            _ = [bm.join_scalars(s, right[x + 1]) for s in steps]
-            left = bm.safe_scale(bm.left_step(left, north[x], cands[x], south[x]))
-        t_env = time.perf_counter() - t0
-        t1 = time.perf_counter()
-        bm.absorb(bm.Boundary(north), rowl, chi, "north")
-        t_abs = time.perf_counter() - t1
-    t_row = t_env + t_abs
-    return {"value": 1.0 / (L * t_row), "unit": "Born samples/s", "cores": threads, "kind": "port",
-            "sample": (f"FP64 NumPy oracle (CPU restatement, not the reference), one row of an {L}x{L} D=6 chi={chi} "
-                       f"draw timed ({t_env:.1f} s environments + {t_abs:.1f} s compressed absorb), x{L} rows "
-                       f"per draw; OpenBLAS {threads} threads")}
+        with mp.get_context("spawn").Pool(procs) as pool:
+            res = pool.map(_cpu_row, [(k % L, chi, seed) for k in range(max(procs, L))])
+        wall = time.perf_counter() - t0
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    per_row = {y: float(np.mean([t for (r, t) in res if r == y])) for y in range(L)}
+    t_draw = sum(per_row.values())
+    return {"value": procs / t_draw, "unit": "Born samples/s", "cores": procs, "kind": "port",
+            "sample": (f"FP64 NumPy oracle (CPU restatement, not the reference), {procs} single-thread processes, "
+                       f"each timing one row of an 8x8 D=6 chi={chi} draw on the fixture PEPO (rows 0..{L - 1} "
+                       f"spread over the processes; random boundaries of the production shapes); one draw = "
+                       f"{t_draw:.1f} core-seconds, {wall:.1f} s wall"),
+            "row_seconds": per_row, "host": host_info()}
 
 
 def measured_traffic(precision, batch, L, chi):
@@ -191,12 +266,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=128, help="Born samples per step")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--L", type=int, default=8)
     ap.add_argument("--chi", type=int, default=32)
-    ap.add_argument("--eps", type=float, default=1.0, help="perturbation size of the synthetic PEPO")
     ap.add_argument("--plain-draws", type=int, default=3, help="batch-1 FP64 draws for the plain path")
     ap.add_argument("--check-samples", type=int, default=8,
-                    help="drawn samples re-evaluated in FP64 (log p) and measured in both precisions (energy)")
+                    help="drawn samples measured in the energy-throughput figure")
     ap.add_argument("--J", type=float, default=1.0)
     ap.add_argument("--h", type=float, default=2.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -217,6 +290,7 @@ def main():
 
     import __graft_entry__ as ge
     spp = ge.load_package()
+    from stopeps_jl_amd import flops
     if args.entropy_only:
         print(json.dumps({"boundary_entropy": entropy_timing(spp)}), flush=True)
         return
@@ -226,11 +300,11 @@ def main():
     if args.fu_only:
         print(json.dumps({"full_update": fu_timing(spp, batch=max(args.fu_batch, 1))}), flush=True)
         return
-    L = args.L
-    lat = spp.Lattice(L, L)
-    pepo = spp.synthetic_thermal_pepo(lat, (3, 3), args.eps, seed=1)
+    pepo, _ = spp.load_state_npz(PEPO_FIXTURE)
+    L = pepo.lattice.Lx
     ns = L * L
     B = args.batch
+    w_draw = flops.draw_flops(pepo, args.chi)  # algorithmic flops of one draw
 
     ctx = spp.Context(0, args.precision)
     t0 = time.perf_counter()
@@ -258,30 +332,37 @@ def main():
     elapsed = time.perf_counter() - t0
     cnt = ctx.counters()
     ctx.set_option("profile", 0)
-    value = B * args.steps / elapsed
+    ndraws = B * args.steps
+    value = ndraws / elapsed
     log(f"timed {args.steps} x {B} draws in {elapsed:.2f} s -> {value:.3f} samples/s")
     traffic, traffic_src = measured_traffic(args.precision, B, L, args.chi)
     kc = cnt[args.precision]  # launches of the contraction kernel of this precision
-    gemm_tf = kc["flops"] / (kc["ms"] * 1e-3) / 1e12 if kc["ms"] > 0 else 0.0
     avg_ms = kc["ms"] / max(kc["launches"], 1)
+    draws_per_launch = ndraws / max(kc["launches"], 1)
+    achieved = w_draw * draws_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    executed_tf = kc["flops"] / (kc["ms"] * 1e-3) / 1e12 if kc["ms"] > 0 else 0.0
+    peak = PEAK[args.precision]
     result = {
         "metric": METRIC, "value": value, "unit": "Born samples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
-        "config": {"workload": f"batched Born draws, {L}x{L} Z2 PEPO D=6 (3+3), d=2, chi={args.chi}, "
-                               f"{B} samples per step", "lattice": [L, L], "D": 6, "chi": args.chi,
+        "config": {"workload": f"batched Born draws, {L}x{L} TFIM J=1 h=2.0 Z2 PEPO at beta0=0.5, D=6 (3+3), d=2, "
+                               f"chi={args.chi}, {B} samples per step", "lattice": [L, L], "D": 6, "chi": args.chi,
                    "samples_per_step": B, "cache_build_s": t_cache,
-                   "input": f"synthetic PEPO: identity embedded in D=6 bonds + Gaussian perturbation eps={args.eps}, seed 1"},
+                   "input": "tests/golden/cfg2_pepo_beta05.npz: the beta0=0.5 PEPO of full-update purification "
+                            "(5 steps of dtau=0.05, FP64), frozen; uniforms of sample indices 1.. of seed 1"},
         "roofline": {"bound": "mfma", "kernel": f"z2gemm_kernel<{'float' if args.precision == 'fp32' else 'double'}>",
-                     "achieved": gemm_tf, "peak": PEAK[args.precision], "unit": "TFLOP/s",
-                     "frac": gemm_tf / PEAK[args.precision], "traffic": traffic,
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": traffic,
                      "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
                      "traffic_source": traffic_src,
+                     "algorithmic_flops_per_draw": w_draw, "draws_per_launch": draws_per_launch,
                      "avg_launch_ms": avg_ms, "launches": kc["launches"],
-                     "flops_per_launch": kc["flops"] / max(kc["launches"], 1),
+                     "step_frac": w_draw * value / (peak * 1e12),
+                     "executed_tflops": executed_tf, "executed_frac": executed_tf / peak,
+                     "executed_flops_per_draw": cnt["gemm_flops"] / ndraws,
                      "kernel_share_of_step": kc["ms"] / (1e3 * elapsed),
-                     "all_contractions_share_of_step": cnt["gemm_ms"] / (1e3 * elapsed),
-                     "flops_per_draw": cnt["gemm_flops"] / (B * args.steps)},
+                     "all_contractions_share_of_step": cnt["gemm_ms"] / (1e3 * elapsed)},
     }
     if args.profile_only:
         if args.copy_calibration:
@@ -289,60 +370,95 @@ def main():
         print(json.dumps(result), flush=True)
         return
 
-    # plain FP64 path: the same kernels, FP64, one sample at a time
+    # batched FP64: the same batch, FP64 everywhere
     pctx = spp.Context(0, "fp64")
     pcache = spp.SamplingCache(pctx, pepo, args.chi)
-    spp.sample_configurations(pcache, uniforms=uniforms(10 ** 6, 1))  # warm-up
+    spp.sample_configurations(pcache, uniforms=uniforms(10 ** 6, 2))  # warm-up
     pctx.synchronize()
+    t0 = time.perf_counter()
+    spp.sample_configurations(pcache, uniforms=uniforms(index, B))
+    pctx.synchronize()
+    batched64 = B / (time.perf_counter() - t0)
+    log(f"batched FP64: {batched64:.3f} samples/s")
+    # plain FP64 path: the same kernels, FP64, one sample at a time
     t0 = time.perf_counter()
     for i in range(args.plain_draws):
         spp.sample_configurations(pcache, uniforms=uniforms(index + i, 1))
     pctx.synchronize()
     plain = args.plain_draws / (time.perf_counter() - t0)
+    log(f"plain FP64: {plain:.3f} samples/s")
     result["plain_fp64"] = {"value": plain, "unit": "Born samples/s", "batch": 1,
                             "speedup_of_value": value / plain}
-    # precision check: FP64 evaluation of FP32-drawn configurations
-    k = min(args.check_samples, B)
-    d0 = draws[0]
-    ev = spp.sample_configurations(pcache, configurations=d0["configuration"][:k])
-    dlogp = np.max(np.abs(ev["log_probability"] - d0["log_probability"][:k]))
-    dtln = np.max(np.abs(ev["tensor_log_norm"] - d0["tensor_log_norm"][:k]))
-    # energy of the same samples (TFIM J, h at the sampling chi), batched
-    # FP32-MFMA contractions vs the FP64 path; the ensemble estimate weighs
-    # each sample by exp(log_norm - log q) (particles.jl:43-52, ensemble.jl:17)
+    result["batched_fp64"] = {"value": batched64, "unit": "Born samples/s", "batch": B,
+                              "batching_gain": batched64 / plain, "precision_gain": value / batched64}
     model = spp.TFIM(args.J, args.h, symmetric=True)
-    cfg = d0["configuration"][:k]
+    result["precision_check"] = oracle_check(spp, ctx, cache, pcache, model, args)
+    # energy throughput of the headline's drawn samples
+    k = min(args.check_samples, B)
+    cfg = draws[0]["configuration"][:k]
     ctx.synchronize()
     t0 = time.perf_counter()
-    e_lo = spp.measure_energies(cache, cfg, model)
+    spp.measure_energies(cache, cfg, model)
     ctx.synchronize()
     t_energy = time.perf_counter() - t0
-    e_hi = spp.measure_energies(pcache, cfg, model)
-
-    def ensemble(e, lw):
-        w = np.exp(lw - np.max(lw))
-        return float(np.sum(w * e) / np.sum(w))
-    E_lo = ensemble(e_lo["energy_per_site"], e_lo["log_norm"] - d0["log_probability"][:k])
-    E_hi = ensemble(e_hi["energy_per_site"], e_hi["log_norm"] - ev["log_probability"])
-    result["precision_check"] = {
-        "samples": int(k), "max_abs_dlog_probability": float(dlogp), "max_abs_dtensor_log_norm": float(dtln),
-        "max_environment_truncation": float(np.max(d0["max_environment_truncation"])),
-        "max_abs_dE_per_site": float(np.max(np.abs(e_lo["energy_per_site"] - e_hi["energy_per_site"]))),
-        "ensemble_E_per_site": E_hi, "ensemble_dE_per_site": abs(E_lo - E_hi),
-        "model": f"TFIM J={args.J} h={args.h} (Z2)"}
     result["energy_measurement"] = {"value": k / t_energy, "unit": "energy measurements/s",
                                     "dtype": args.precision, "samples": int(k), "chi": args.chi}
+    # release the draw workspaces (the batched FP64 one holds ~0.1 TB) so the
+    # later workloads size their chunks from the whole device
+    del cache, pcache, draws
+    ctx.close()
+    pctx.close()
+    gc.collect()
     result["adjoint"] = adjoint_timing(spp)
     if args.fu_batch > 0:
         log("full-update step ...")
         result["full_update"] = fu_timing(spp, batch=args.fu_batch, J=args.J, h=args.h)
     if not args.no_cpu:
+        log("CPU baseline ...")
         try:
-            result["cpu_baseline"] = cpu_baseline(L=L, chi=args.chi)
+            result["cpu_baseline"] = cpu_baseline(chi=args.chi)
         except Exception as exc:  # the baseline must not hide the GPU result
             result["cpu_baseline"] = {"value": None, "error": repr(exc)}
     print(json.dumps(result), flush=True)
 
+
+def oracle_check(spp, ctx, cache, pcache, model, args):
+    """The headline's precision against the FP64 oracle's committed vectors on
+    the same PEPO (tests/golden/cfg2_oracle.npz): draws of the fixture's sample
+    indices in the headline precision and in FP64 (configurations, log p),
+    and the energy per site of the oracle's measured samples in both
+    precisions, per sample and as the weighted ensemble estimate
+    exp(log_norm - log q) (src/driver/particles.jl:43-52, ensemble.jl:17)."""
+    try:
+        ref = np.load(ORACLE_FIXTURE, allow_pickle=False)
+    except OSError:
+        return {"error": "oracle vectors not found"}
+    ns = cache.lattice.Lx * cache.lattice.Ly
+    idx = [int(i) for i in ref["sample_index"]]
+    uni = np.stack([spp.sample_uniforms(int(ref["seed"]), i, ns) for i in idx])
+    out = {"reference": "FP64 oracle, tests/golden/cfg2_oracle.npz", "samples": len(idx)}
+    for name, c in ((args.precision, cache), ("fp64", pcache)):
+        d = spp.sample_configurations(c, uniforms=uni)
+        same = [np.array_equal(d["configuration"][k], ref["configuration"][k]) for k in range(len(idx))]
+        dl = [abs(d["log_probability"][k] - ref["log_probability"][k]) for k in range(len(idx)) if same[k]]
+        out[name] = {"identical_configurations": int(sum(same)),
+                     "max_abs_dlog_probability": float(max(dl)) if dl else None}
+    m = np.isfinite(ref["energy_per_site"])
+    cfg = ref["configuration"][m].astype(np.int64)
+    for name, c in ((args.precision, cache), ("fp64", pcache)):
+        e = spp.measure_energies(c, cfg, model)
+        lw = e["log_norm"] - ref["log_probability"][m]
+        w = np.exp(lw - np.max(lw))
+        E = float(np.sum(w * e["energy_per_site"]) / np.sum(w))
+        lwr = ref["log_norm"][m] - ref["log_probability"][m]
+        wr = np.exp(lwr - np.max(lwr))
+        Er = float(np.sum(wr * ref["energy_per_site"][m]) / np.sum(wr))
+        out[name]["max_abs_dE_per_site"] = float(np.max(np.abs(e["energy_per_site"] - ref["energy_per_site"][m])))
+        out[name]["ensemble_dE_per_site"] = abs(E - Er)
+        out[name]["ensemble_E_per_site"] = E
+    out["max_environment_truncation"] = float(np.max(ref["max_environment_truncation"]))
+    out["model"] = f"TFIM J={args.J} h={args.h} (Z2)"
+    return out
 
 if __name__ == "__main__":
     main()
