@@ -227,6 +227,25 @@ int spp_measure_energy_states(spp_ctx* ctx, int n, const spp_state* const* state
 int spp_lognorm_batch(spp_ctx* ctx, int n, const spp_state* const* states, int chi, double* log_norm,
                       double* tensor_log_norm, double* max_trunc, int* status);
 
+/* Device bytes of the static scratch array behind spp_lognorm_batch_noalloc
+ * (one per GPU the library runs on, held until the process ends). */
+#define SPP_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* spp_lognorm_batch without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, outputs and results as spp_lognorm_batch; every device
+ * buffer of the call (double layers, boundaries, the absorb workspace) is
+ * carved from the library's static scratch array of
+ * SPP_NOALLOC_SCRATCH_BYTES on the context's GPU, and states are processed in
+ * chunks that fit it. A batch whose single state needs more than the array
+ * holds fails with a message before any launch. All calls on one GPU share
+ * that GPU's array: calls that overlap on one GPU (other contexts, other
+ * threads) race and must be serialised by the caller. Launch timing
+ * ("profile") is suspended during the call. */
+int spp_lognorm_batch_noalloc(spp_ctx* ctx, int n, const spp_state* const* states, int chi, double* log_norm,
+                              double* tensor_log_norm, double* max_trunc, int* status);
+
 /* Gradient of log <psi|psi> with respect to every site tensor of n uploaded
  * states (same lattice and leg spaces), the adjoint of `lognorm`
  * (src/contraction/boundary_mps.jl:162-191). The reference has no adjoint; the

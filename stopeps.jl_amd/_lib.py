@@ -41,6 +41,8 @@ _SIGNATURES = {
                                                  c_double_p, c_int_p, c_int_p, c_double_p, c_double_p]),
     "spp_lognorm_batch": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.POINTER(c_void_p), ctypes.c_int,
                                          c_double_p, c_double_p, c_double_p, c_int_p]),
+    "spp_lognorm_batch_noalloc": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.POINTER(c_void_p), ctypes.c_int,
+                                                 c_double_p, c_double_p, c_double_p, c_int_p]),
     "spp_lognorm_grad_batch": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.POINTER(c_void_p), ctypes.c_int,
                                               c_double_p, c_double_p, c_double_p, c_double_p, c_int_p]),
     "spp_selftest_gemm": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,

@@ -11,6 +11,12 @@
 
 using namespace spp;
 
+// Device memory of spp_lognorm_batch_noalloc: one static array per GPU, held
+// until the process ends. Host code never passes this name to a kernel or a
+// copy; it uses the per-device address resolved by hipGetSymbolAddress in
+// spp_create (Context::static_scratch).
+static __device__ __align__(256) char scratch[SPP_NOALLOC_SCRATCH_BYTES];
+
 struct spp_ctx {
   Context c;
 };
@@ -60,6 +66,17 @@ int spp_create(int device, spp_ctx** out) {
     ctx->c.device = device;
     SPP_CHECK(hipSetDevice(device));
     SPP_CHECK(hipStreamCreate(&ctx->c.stream));
+    // this device's copy of the static scratch array (the no-allocation entry
+    // point), resolved once per device with that device current
+    static std::vector<char*> scratch_of_device;
+    if ((int)scratch_of_device.size() < n) scratch_of_device.resize(n, nullptr);
+    if (!scratch_of_device[device]) {
+      void* ptr = nullptr;
+      SPP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+      if (!ptr) throw HipError("static scratch has no device address");
+      scratch_of_device[device] = (char*)ptr;
+    }
+    ctx->c.static_scratch = scratch_of_device[device];
   });
   if (rc != 0) {
     static thread_local std::string err;
@@ -436,6 +453,40 @@ int spp_lognorm_batch(spp_ctx* ctx, int n, const spp_state* const* states, int c
     o.status = status;
     if (ctx->c.precision == 1) lognorm_batch<float>(ctx->c, v, chi, o);
     else lognorm_batch<double>(ctx->c, v, chi, o);
+  });
+}
+
+int spp_lognorm_batch_noalloc(spp_ctx* ctx, int n, const spp_state* const* states, int chi, double* log_norm,
+                              double* tensor_log_norm, double* max_trunc, int* status) {
+  return guarded(ctx, [&] {
+    if (n < 0) throw HipError("negative state count");
+    if (n > 0 && !states) throw HipError("null state list");
+    if (!ctx->c.static_scratch) throw HipError("static scratch not resolved for this context");
+    SPP_CHECK(hipSetDevice(ctx->c.device));
+    std::vector<const StateDev*> v(n);
+    for (int i = 0; i < n; ++i) {
+      if (!states[i]) throw HipError("null state");
+      usable(states[i]->s);
+      v[i] = &states[i]->s;
+    }
+    LognormOutputs o;
+    o.log_norm = log_norm;
+    o.tensor_log_norm = tensor_log_norm;
+    o.max_trunc = max_trunc;
+    o.status = status;
+    // launch timing creates events: off for the duration of the call
+    const int profile = ctx->c.profile;
+    ctx->c.profile = 0;
+    try {
+      if (ctx->c.precision == 1)
+        lognorm_batch<float>(ctx->c, v, chi, o, ctx->c.static_scratch, SPP_NOALLOC_SCRATCH_BYTES);
+      else
+        lognorm_batch<double>(ctx->c, v, chi, o, ctx->c.static_scratch, SPP_NOALLOC_SCRATCH_BYTES);
+    } catch (...) {
+      ctx->c.profile = profile;
+      throw;
+    }
+    ctx->c.profile = profile;
   });
 }
 

@@ -46,10 +46,18 @@ class Arena {
   void release(size_t m) { top_ = m; }
   size_t capacity() const { return cap_; }
   size_t peak() const { return peak_; }
+  // Serve take() from caller-provided device memory [base, base + bytes)
+  // until restore(); reserve() then only checks the size (no hipMalloc or
+  // hipFree while borrowed). The owned block is kept and comes back on restore().
+  void borrow(char* base, size_t bytes);
+  void restore();
 
  private:
   char* base_ = nullptr;
   size_t cap_ = 0, top_ = 0, peak_ = 0;
+  bool borrowed_ = false;
+  char* own_base_ = nullptr;
+  size_t own_cap_ = 0, own_top_ = 0;
 };
 
 struct Leg {
@@ -186,6 +194,14 @@ struct Context {
   LaunchTimer timer;
   EntropyWork entropy;      // boundary-entropy spectra of the current absorb
   DeviceScratch dot_part;   // partial sums of the chunked dot products
+  // No-allocation mode (set by the entry points that must not allocate):
+  // the chunked dot products take their partial sums from this window of the
+  // library's static scratch array instead of growing dot_part.
+  double* noalloc_dot = nullptr;
+  size_t noalloc_dot_n = 0;
+  // Device address of the library's static scratch array on this context's
+  // device (resolved once, when the context is created).
+  char* static_scratch = nullptr;
   // Test hook of the full update's failure handling: treat bond update
   // fu_fail_bond (sweep order) of sample fu_fail_sample as failed with
   // fu_fail_code (fu_fail_sample < 0: off).

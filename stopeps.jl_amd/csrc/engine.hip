@@ -19,11 +19,34 @@ namespace spp {
 
 // ------------------------------- arena --------------------------------------
 Arena::~Arena() {
+  if (borrowed_) restore();
   if (base_) (void)hipFree(base_);
+}
+
+void Arena::borrow(char* base, size_t bytes) {
+  if (borrowed_) throw HipError("arena already borrows external memory");
+  own_base_ = base_;
+  own_cap_ = cap_;
+  own_top_ = top_;
+  base_ = base;
+  cap_ = bytes;
+  top_ = 0;
+  borrowed_ = true;
+}
+
+void Arena::restore() {
+  if (!borrowed_) return;
+  base_ = own_base_;
+  cap_ = own_cap_;
+  top_ = own_top_;
+  borrowed_ = false;
 }
 
 void Arena::reserve(size_t bytes) {
   if (bytes <= cap_) return;
+  if (borrowed_)
+    throw HipError("device workspace exhausted: need " + std::to_string(bytes) + " bytes, the scratch window holds " +
+                   std::to_string(cap_));
   if (top_ != 0) throw HipError("arena resize while in use");
   if (base_) SPP_CHECK(hipFree(base_));
   base_ = nullptr;
@@ -37,6 +60,10 @@ DeviceScratch::~DeviceScratch() {
 
 double* dot_partials(Context& ctx, int nsamples) {
   const size_t need = (size_t)std::max(nsamples, 1) * kDotChunks;
+  if (ctx.noalloc_dot) {
+    if (need > ctx.noalloc_dot_n) throw HipError("dot-product scratch window too small");
+    return ctx.noalloc_dot;
+  }
   if (ctx.dot_part.n < need) {
     SPP_CHECK(hipStreamSynchronize(ctx.stream));  // earlier dot launches may still read the old buffer
     if (ctx.dot_part.p) SPP_CHECK(hipFree(ctx.dot_part.p));

@@ -14,7 +14,13 @@ struct LognormOutputs {
 };
 
 // All states share the lattice and every site's leg dimensions and even counts.
+// With pool != nullptr every device buffer of the call (layers, pointer
+// tables, boundaries, the absorb workspace, dot partial sums) is carved from
+// [pool, pool + pool_bytes) and the call makes no device allocation; a state
+// batch whose single-state need exceeds pool_bytes is rejected before any
+// launch.
 template <typename T>
-void lognorm_batch(Context& ctx, const std::vector<const StateDev*>& states, int chi, LognormOutputs& out);
+void lognorm_batch(Context& ctx, const std::vector<const StateDev*>& states, int chi, LognormOutputs& out,
+                   char* pool = nullptr, size_t pool_bytes = 0);
 
 }  // namespace spp

@@ -14,7 +14,8 @@
 namespace spp {
 
 template <typename T>
-void lognorm_batch(Context& ctx, const std::vector<const StateDev*>& states, int chi, LognormOutputs& out) {
+void lognorm_batch(Context& ctx, const std::vector<const StateDev*>& states, int chi, LognormOutputs& out,
+                   char* pool, size_t pool_bytes) {
   hipStream_t st = ctx.stream;
   const int n = (int)states.size();
   if (n == 0) return;
@@ -47,32 +48,88 @@ void lognorm_batch(Context& ctx, const std::vector<const StateDev*>& states, int
   const double ts = sizeof(T);
   const size_t per_arena = (size_t)(1.15 * absorb_bytes_per_sample(Lx, chi, D, sizeof(T), ctx.ksub)) + 4096;
   const size_t per_persist = (size_t)(ts * (layer_elems + 2.0 * Lx * cap)) + sizeof(LayerJob) * ns + 256;
-  SPP_CHECK(hipStreamSynchronize(st));
-  size_t freeb = 0, totalb = 0;
-  SPP_CHECK(hipMemGetInfo(&freeb, &totalb));
-  const double avail = 0.6 * (double)freeb + (double)ctx.arena.capacity();
-  long long chunk = std::max<long long>(1, (long long)(avail / (double)(per_arena + per_persist)));
+  // device bytes of one chunk of nb samples carved from a pool (256-byte
+  // aligned pieces: the buffers below, the two boundaries, the dot partials)
+  auto pool_need = [&](long long nb) -> double {
+    const double al = 256.0;
+    const double bufs = ts * layer_elems * nb + 8.0 * ns * nb + sizeof(LayerJob) * (double)ns * nb + 8.0 * nb +
+                        8.0 * Lx + 8.0 * Lx + 6 * al;
+    const double bnd = 2.0 * (Lx * (ts * nb * cap + al) + (Lx - 1) * (4.0 * nb + al) + 2 * (8.0 * nb + al));
+    const double dots = 8.0 * (double)nb * kDotChunks + al;
+    return bufs + bnd + dots + (double)per_arena * nb;
+  };
+  long long chunk;
+  if (pool) {
+    if (pool_need(1) > (double)pool_bytes)
+      throw HipError("lognorm without allocation: one state needs " + std::to_string((long long)pool_need(1)) +
+                     " bytes of device scratch, the static scratch holds " + std::to_string((long long)pool_bytes));
+    chunk = 1;
+    while (chunk < n && pool_need(chunk + 1) <= (double)pool_bytes) ++chunk;
+  } else {
+    SPP_CHECK(hipStreamSynchronize(st));
+    size_t freeb = 0, totalb = 0;
+    SPP_CHECK(hipMemGetInfo(&freeb, &totalb));
+    const double avail = 0.6 * (double)freeb + (double)ctx.arena.capacity();
+    chunk = std::max<long long>(1, (long long)(avail / (double)(per_arena + per_persist)));
+  }
   if (ctx.chunk > 0) chunk = std::min<long long>(chunk, ctx.chunk);
   chunk = std::min<long long>(chunk, n);
   chunk = std::min<long long>(chunk, std::max(1, 65535 / ns));  // double-layer grid: one row per (sample, site)
   const int nb = (int)chunk;
-  ctx.arena.reserve(per_arena * (size_t)nb);
 
   Boundary<T> NB[2];
   std::vector<void*> scratch;
-  auto dev = [&](size_t bytes) {
+  size_t pool_top = 0;
+  auto dev = [&](size_t bytes) -> void* {
+    bytes = std::max<size_t>(bytes, 16);
+    if (pool) {
+      const size_t at = (pool_top + 255) & ~size_t(255);
+      if (at + bytes > pool_bytes) throw HipError("lognorm without allocation: scratch exhausted");
+      pool_top = at + bytes;
+      return pool + at;
+    }
     void* p = nullptr;
-    SPP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+    SPP_CHECK(hipMalloc(&p, bytes));
     scratch.push_back(p);
     return p;
+  };
+  auto make_boundary = [&](Boundary<T>& b) {
+    if (!pool) {
+      prepare_boundary(b, Lx, nb, cap);
+      return;
+    }
+    b.L = Lx;
+    b.nb = nb;
+    b.bdim.assign(Lx + 1, 1);
+    b.beven.assign(Lx + 1, 1);
+    b.bsplit.assign(Lx + 1, nullptr);
+    b.pdim.assign(Lx, 1);
+    b.peven.assign(Lx, 1);
+    b.t.assign(Lx, nullptr);
+    for (int x = 0; x < Lx; ++x) b.t[x] = (T*)dev(sizeof(T) * nb * cap);
+    for (int k = 1; k < Lx; ++k) b.bsplit[k] = (int*)dev(sizeof(int) * nb);
+    b.log_scale = (double*)dev(sizeof(double) * nb);
+    b.max_trunc = (double*)dev(sizeof(double) * nb);
+    b.owns = false;
   };
   auto cleanup = [&]() {
     NB[0].release();
     NB[1].release();
     for (void* p : scratch) (void)hipFree(p);
     scratch.clear();
+    if (pool) {
+      ctx.arena.restore();
+      ctx.noalloc_dot = nullptr;
+      ctx.noalloc_dot_n = 0;
+    }
   };
   try {
+    if (pool) {
+      ctx.noalloc_dot = (double*)dev(sizeof(double) * (size_t)nb * kDotChunks);
+      ctx.noalloc_dot_n = (size_t)nb * kDotChunks;
+    } else {
+      ctx.arena.reserve(per_arena * (size_t)nb);
+    }
     T* layers = (T*)dev(sizeof(T) * (size_t)layer_elems * nb);
     std::vector<T*> site_base(ns);
     {
@@ -89,11 +146,18 @@ void lognorm_batch(Context& ctx, const std::vector<const StateDev*>& states, int
       for (int s = 0; s < nb; ++s) hptr[(size_t)i * nb + s] = site_base[i] + (long long)s * lsize[i];
     upload(ptrs, hptr.data(), sizeof(void*) * hptr.size(), st);
     LayerJob* djobs = (LayerJob*)dev(sizeof(LayerJob) * (size_t)ns * nb);
-    prepare_boundary(NB[0], Lx, nb, cap);
-    prepare_boundary(NB[1], Lx, nb, cap);
+    make_boundary(NB[0]);
+    make_boundary(NB[1]);
     double* mant = (double*)dev(sizeof(double) * nb);
     long long* dsst = (long long*)dev(sizeof(long long) * Lx);
     T** dtens = (T**)dev(sizeof(T*) * Lx);
+    if (pool) {
+      // the absorb workspace is the rest of the pool
+      const size_t at = (pool_top + 255) & ~size_t(255);
+      if (at >= pool_bytes || pool_bytes - at < per_arena * (size_t)nb)
+        throw HipError("lognorm without allocation: scratch exhausted");
+      ctx.arena.borrow(pool + at, pool_bytes - at);
+    }
     std::vector<LayerJob> jobs;
     std::vector<double> h_mant(nb), h_ls(nb), h_mt(nb);
     for (long long s0i = 0; s0i < n; s0i += nb) {
@@ -162,7 +226,9 @@ void lognorm_batch(Context& ctx, const std::vector<const StateDev*>& states, int
   cleanup();
 }
 
-template void lognorm_batch<float>(Context&, const std::vector<const StateDev*>&, int, LognormOutputs&);
-template void lognorm_batch<double>(Context&, const std::vector<const StateDev*>&, int, LognormOutputs&);
+template void lognorm_batch<float>(Context&, const std::vector<const StateDev*>&, int, LognormOutputs&, char*,
+                                   size_t);
+template void lognorm_batch<double>(Context&, const std::vector<const StateDev*>&, int, LognormOutputs&, char*,
+                                    size_t);
 
 }  // namespace spp
