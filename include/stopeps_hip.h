@@ -76,7 +76,8 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value);
  * launches (profiling mode); counters[6..8] = launches, milliseconds and flops
  * of the timed FP64-kernel launches, counters[9..11] the same for the FP32
  * kernel; counters[12..14] = iterative truncation eigensolves, their summed
- * subspace iterations and their Chebyshev filter products. Returns the count
+ * subspace iterations and their Chebyshev filter products; counters[15] = full-
+ * update bonds repaired by the PSD projection of their hole. Returns the count
  * written. */
 int spp_get_counters(spp_ctx* ctx, double* counters, int n);
 int spp_reset_counters(spp_ctx* ctx);
@@ -127,7 +128,9 @@ int spp_state_download(spp_ctx* ctx, const spp_state* state, double* const* data
  * target, 4 negative residual needing the PSD repair, 5 nonfinite
  * residual/target, 6 singular gauge, 7 workspace, 8 eigen/SVD sweep cap
  * reached without convergence), 9 nonfinite row environment, 10 zero or
- * nonfinite site norm.
+ * nonfinite site norm, 11 the PSD repair did not remove the residual
+ * failure (see spp_fu_set_repair: with a repair callback set, a bond with
+ * status 4 is repaired instead of failing).
  * Failure semantics: a sample that fails keeps its sites from the failing
  * bond on (no later bond of the step writes it), the others finish the step;
  * a failed state is marked and every compute entry point refuses it
@@ -139,6 +142,48 @@ int spp_fu_evolution_step(spp_ctx* ctx, int n, spp_state* const* states, int chi
                           const double* options, const double* gates_h, const double* gates_v, double* records,
                           double* normalizations, double* max_env_trunc, double* max_boundary_entropy,
                           int* status);
+
+/* PSD repair of the full-update bond metric (`_positive_reduced_metric` and
+ * the repair branch of `update_bond!`, src/evolution/full_update.jl:177-340,
+ * 484-508). When the residual check of a bond requests it (status 4,
+ * full_update.jl:355-369), spp_fu_evolution_step contracts that sample's
+ * complete bond hole on the device and hands it to the host callback set here,
+ * together with the Q factors of the reduced pair; the callback writes the
+ * reduced metric of the hole's PSD projection, and the bond solve of that
+ * sample runs again on it. If the repaired solve still requests the repair,
+ * the sample fails with status 11 ("PSD repair did not remove the FU residual
+ * failure"). Without a callback the sample fails with status 4.
+ *
+ * Layouts: a double-layer (pair) leg of a virtual leg with dimension m and e
+ * even indices has m*m entries in the pair order (ket i, bra j) ->
+ *   both even: i*e + j; both odd: e*e + (i-e)*o + (j-e);
+ *   i even, j odd: e*e + o*o + i*o + (j-e); i odd, j even: e*e + o*o + e*o + (i-e)*e + j
+ * (o = m - e). hole: H over its six exterior pair legs, row-major; leg k is the
+ * pseudo-site slot hole_slot[k] of Q factor hole_q[k] (0: qa, 1: qb) and has
+ * virtual dimension hole_dims[k] with hole_evens[k] even indices. qa, qb: the Q
+ * factors as dense row-major pseudo sites (W,N,P,A,E,S) of dims *_dims, with
+ * the auxiliary leg in the P slot, a unit A slot and the bond r in slot
+ * *_core. The metric to write is N[pair(rk, rb)][pair(sk, sb)] over the r legs
+ * of qa and qb (dims qa_dims[qa_core] x ..., row-major), the contraction
+ * sum H+[(ket legs), (bra legs)] qa[ket.., aux, rk] qa[bra.., aux, rb]
+ * qb[ket.., aux', sk] qb[bra.., aux', sb] over the exterior legs and each
+ * factor's auxiliary leg, with H+ the PSD projection of H/||H|| as the
+ * reference computes it. The callback returns 0 on success. */
+typedef struct spp_fu_repair_problem {
+  int sample;      /* position of the state in the call's list */
+  int bond;        /* bond update in sweep order */
+  int horizontal;  /* 1: bond (x,y)-(x+1,y), 0: (x,y)-(x,y+1) */
+  int hole_dims[6], hole_evens[6], hole_q[6], hole_slot[6];
+  const double* hole;
+  int qa_dims[6], qa_evens[6], qb_dims[6], qb_evens[6];
+  int qa_core, qb_core;
+  const double* qa;
+  const double* qb;
+  double target_norm; /* <target, N target> of the unrepaired metric */
+} spp_fu_repair_problem;
+typedef int (*spp_fu_repair_fn)(void* user, const spp_fu_repair_problem* problem, double* metric);
+/* Set (fn non-null) or clear the repair callback of a context. */
+int spp_fu_set_repair(spp_ctx* ctx, spp_fu_repair_fn fn, void* user);
 
 /* `SamplingCache(state, BoundaryMPS(chi); kind=:auxiliary, basis)`
  * (src/sampling/born.jl:56-73): double layers, aux-projected candidate layers

@@ -3,9 +3,10 @@ src/evolution/full_update.jl; test infrastructure).
 
 Dense Z2 tensors (``oracle.z2``); real FP64 arithmetic, so conjugation is the
 identity and every ``_out`` / ``dual`` of the reference is a no-op. Leg order of
-a site is (W, N, P, Aux, E, S). The rare PSD repair of the bond metric
-(``_positive_reduced_metric``, full_update.jl:177-330) is not restated: the
-oracle raises where the reference would repair.
+a site is (W, N, P, Aux, E, S). The PSD repair of the bond metric
+(``_positive_reduced_metric``, full_update.jl:177-340) is restated on the
+complete bond hole as a dense matrix per Z2 block; ``update_bond`` runs it
+where the reference does (full_update.jl:491-501).
 """
 import numpy as np
 
@@ -133,6 +134,87 @@ def bond_hole_metric(env, Qa, Qb):
     Rh = ein("bsd,cefd->bscef", Bt, right)
     H = ein("wvbnc,bscef->wnevfs", Lh, Rh)
     return ein("wnevfs,wneRr,vfsQq->RQrq", H, A, B)
+
+
+def bond_hole(env):
+    """The complete bond hole (``bond_hole`` / ``_grouped_bond_hole``,
+    boundary_mps.jl:326-353, full_update.jl:228-257) over the six exterior
+    double legs: H[w,n,s,v,e,t] (horizontal: W, N, S of the first site, N, E,
+    S of the second) or H[w,n,e,v,f,s] (vertical: W, N, E of the upper site,
+    W, E, S of the lower)."""
+    left, right = env.left, env.right
+    if env.direction == "horizontal":
+        T1, T2 = env.top
+        B1, B2 = env.bottom
+        X = ein("awb,anc->wbnc", left, T1)
+        Lh = ein("wbnc,bsf->cwnfs", X, B1)
+        Rh = ein("deg,cvd,ftg->cveft", right, T2, B2)
+        return ein("cwnfs,cveft->wnsvet", Lh, Rh)
+    T = env.top[0]
+    Bt = env.bottom[0]
+    Lh = ein("awvb,anc->wvbnc", left, T)
+    Rh = ein("bsd,cefd->bscef", Bt, right)
+    return ein("wvbnc,bscef->wnevfs", Lh, Rh)
+
+
+def positive_reduced_metric(env, Qa, Qb):
+    """``_positive_reduced_metric`` (full_update.jl:289-331): the grouped hole
+    as a matrix (rows: the ket copies of the first site's three exterior legs,
+    then the second's; columns: the bra copies), scaled by its norm, made
+    symmetric, eigendecomposed per Z2 block; every positive eigenpair gives
+    W = sqrt(lambda) v' and the piece W'W contracted with the Q factors
+    (``_grouped_repair_piece``, :277-287). Returns (N[rb, sb, rk, sk], report)."""
+    H = bond_hole(env)
+    if not np.all(np.isfinite(H.a)):
+        raise ValueError("nonfinite norm metric")
+    dims = [int(np.sqrt(n)) for n in H.shape]
+    legpar = []  # parity of each virtual (ket or bra) index of each exterior leg
+    for k in range(6):
+        pk = H.p[k].reshape(dims[k], dims[k])  # fused pair parity = ket + bra
+        legpar.append(pk[:, 0] ^ pk[0, 0])     # ket parities (bra index 0 fixed)
+    A = H.a.reshape([d for m in dims for d in (m, m)])
+    A = np.transpose(A, [2 * k for k in range(6)] + [2 * k + 1 for k in range(6)])
+    n = int(np.prod(dims))
+    M = A.reshape(n, n)
+    rowpar = np.zeros(1, dtype=np.int64)
+    for k in range(6):
+        rowpar = ((rowpar[:, None] + legpar[k][None, :]) % 2).reshape(-1)
+    scale = float(np.linalg.norm(M))
+    if scale > 0:
+        M = M / scale
+    defect = float(np.sqrt(np.sum((M - M.T) ** 2)))
+    M = (M + M.T) / 2
+    # Q factors as [exterior (3 legs fused), aux, r]
+    qa = Qa.a.reshape(-1, Qa.shape[3], Qa.shape[4])
+    qb = Qb.a.reshape(-1, Qb.shape[3], Qb.shape[4])
+    na = qa.shape[0]
+    result = np.zeros((qa.shape[2], qb.shape[2], qa.shape[2], qb.shape[2]))  # [rb, sb, rk, sk]
+    minimum = spectral = negw = neg2 = spec2 = 0.0
+    nneg = 0
+    for q in (0, 1):
+        idx = np.nonzero(rowpar == q)[0]
+        if len(idx) == 0:
+            continue
+        w, V = np.linalg.eigh(M[np.ix_(idx, idx)])
+        minimum = min(minimum, float(np.min(w)))
+        spectral = max(spectral, float(np.max(np.abs(w))))
+        nneg += int(np.sum(w < 0))
+        negw += float(np.sum(w < 0))
+        neg2 += float(np.sum(np.minimum(w, 0) ** 2))
+        spec2 += float(np.sum(w ** 2))
+        for k in np.nonzero(w > 0)[0]:
+            v = np.zeros(n)
+            v[idx] = np.sqrt(w[k]) * V[:, k]             # row k of W (ket side)
+            Wk = v.reshape(na, -1)                       # [first site's legs, second site's legs]
+            ket = np.einsum("ab,axr,bys->xrys", Wk, qa, qb)
+            result += np.einsum("xRyS,xrys->RSrs", ket, ket)  # W'W with both Q factors, aux traced
+    tiny = np.finfo(np.float64).tiny
+    report = {"minimum_relative_eigenvalue": minimum / max(spectral, tiny), "negative_eigenvalues": negw,
+              "negative_multiplets": nneg, "negative_spectral_fraction": np.sqrt(neg2) / max(np.sqrt(spec2), tiny),
+              "hermiticity_defect": defect if scale > 0 else defect / tiny}
+    rp = Qa.p[4]
+    sp = Qb.p[4]
+    return Z2(result, [rp, sp, rp, sp]), report
 
 
 # ---------------------------------------------------------------------------
@@ -383,16 +465,33 @@ def balance_bond(QA, C, QB):
     return ein("rpk,kc->rpc", QA, Us), ein("ck,kqs->cqs", Vs, QB)
 
 
-def update_bond(state, i, j, gate, D, caps, opts, env, direction):
-    """``update_bond!`` (full_update.jl:484-508) without the PSD repair branch."""
+def update_bond(state, i, j, gate, D, caps, opts, env, direction, force_repair=False):
+    """``update_bond!`` (full_update.jl:484-508). When the residual check
+    requests it (or ``force_repair``, a test hook taking that branch at the
+    initial stage), the reduced metric of the PSD-projected complete hole
+    replaces N and the pair is initialised and fitted again; a second request
+    raises the reference's error."""
     pair = reduced_pair(state[i], state[j], gate, direction)
     N = reduced_metric(env, pair["Qa"], pair["Qb"])
     init = initialize_pair(pair, N, D, caps)
-    fit = optimize_pair(init, N, opts)
+    repairs = []
+    try:
+        if force_repair:
+            raise RepairNeeded("forced")
+        fit = optimize_pair(init, N, opts)
+    except RepairNeeded as trig:
+        repaired, report = positive_reduced_metric(env, pair["Qa"], pair["Qb"])
+        report["reason"] = str(trig)
+        repairs.append(report)
+        init = initialize_pair(pair, repaired, D, caps)
+        try:
+            fit = optimize_pair(init, repaired, opts)
+        except RepairNeeded:
+            raise ValueError("PSD repair did not remove the FU residual failure")
     state[i], state[j] = restore_pair(pair, fit["left"], fit["right"])
     rec = {k: fit[k] for k in ("initial_cost", "final_cost", "residual", "iterations", "iteration_limit_reached",
                                "stop_reason")}
-    rec.update(bond=(i, j), direction=direction)
+    rec.update(bond=(i, j), direction=direction, repairs=repairs)
     return rec
 
 
@@ -400,7 +499,7 @@ def update_bond(state, i, j, gate, D, caps, opts, env, direction):
 # sweeps and the evolution step (full_update.jl:510-597)
 # ---------------------------------------------------------------------------
 
-def sweep(state, model, delta, D, caps, opts, chi, direction, reverse, entropy=None):
+def sweep(state, model, delta, D, caps, opts, chi, direction, reverse, entropy=None, force=(), first=0):
     """``_sweep!`` (full_update.jl:510-561) for the row boundary-MPS contraction.
     ``entropy``: the ``BoundaryEntropy`` dict ({"maximum": ...}) every absorb
     reports to, or None."""
@@ -439,7 +538,8 @@ def sweep(state, model, delta, D, caps, opts, chi, direction, reverse, entropy=N
             env = bond_environment(top, bottom, x, direction, left, right)
             gate = model.trotter_gate(L, i, j, delta)
             gate = Z2(gate, [state[i].p[2], state[j].p[2], state[i].p[2], state[j].p[2]])
-            rec = update_bond(state, i, j, gate, D, caps, opts, env, direction)
+            rec = update_bond(state, i, j, gate, D, caps, opts, env, direction,
+                              force_repair=(first + len(records)) in force)
             rec["max_environment_truncation"] = max(north[y].max_truncation, south[lastrow + 1].max_truncation)
             records.append(rec)
             layers[i[0]][i[1]] = double_layer(state[i])
@@ -460,9 +560,10 @@ def sweep(state, model, delta, D, caps, opts, chi, direction, reverse, entropy=N
     return records, env_err
 
 
-def evolution_step(state, model, delta, D, caps, opts, chi, order=2, record_entropy=False):
+def evolution_step(state, model, delta, D, caps, opts, chi, order=2, record_entropy=False, force_repair=()):
     """``evolution_step!`` (full_update.jl:563-597) with FullUpdate and
-    BoundaryMPS(chi). ``caps`` = (even, odd) sector budget of the bond space, or None."""
+    BoundaryMPS(chi). ``caps`` = (even, odd) sector budget of the bond space, or None.
+    ``force_repair``: bond updates (sweep order) that take the PSD-repair branch."""
     records = []
     env_err = 0.0
     entropy = {"maximum": 0.0} if record_entropy else None
@@ -476,7 +577,8 @@ def evolution_step(state, model, delta, D, caps, opts, chi, order=2, record_entr
         for direction, reverse in sweeps:
             if not L.bonds(direction):
                 continue
-            r, e = sweep(state, model, gate_delta, D, caps, opts, chi, direction, reverse, entropy)
+            r, e = sweep(state, model, gate_delta, D, caps, opts, chi, direction, reverse, entropy,
+                         force=force_repair, first=len(records))
             records += r
             env_err = max(env_err, e)
     state.beta += 2 * delta

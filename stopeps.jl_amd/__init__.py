@@ -20,7 +20,7 @@ from .particles import (Particle, ensemble_statistics, step_schedule, purify, st
                         advance_particles, statistics, group_configurations)
 from .config import RunConfig, ResamplingOptions, validate_config, temperature_nodes, parameter_dict
 from .driver import ThermalCheckpoint, run_purification, run_stochastic
-from . import _lib, particles
+from . import _lib, particles, repair
 
 __all__ = ["synthetic_thermal_pepo", "random_peps", "Lattice", "Site", "FiniteState", "FinitePEPO", "FinitePEPS", "identity_pepo", "parity_sorted",
            "LocalBasis", "SymmetryBasis", "XBasis", "resolve_basis", "Context", "DeviceState",

@@ -56,6 +56,7 @@ _SIGNATURES = {
     "spp_fu_evolution_step": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.POINTER(c_void_p), ctypes.c_int,
                                              ctypes.c_int, ctypes.c_double, ctypes.c_int, c_double_p, c_double_p,
                                              c_double_p, c_double_p, c_double_p, c_double_p, c_double_p, c_int_p]),
+    "spp_fu_set_repair": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     "spp_selftest_jacobi": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, c_double_p, ctypes.c_int,
                                            c_double_p, c_int_p]),
     "spp_selftest_fu_pair": (ctypes.c_int, [ctypes.c_int, c_int_p, c_int_p, c_int_p, c_int_p, c_double_p, c_double_p,

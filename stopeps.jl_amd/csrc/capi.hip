@@ -150,12 +150,13 @@ int spp_get_counters(spp_ctx* ctx, double* counters, int n) {
   if (!ctx || !counters) return -3;
   if (guarded(ctx, [&] { ctx->c.flush_timer(); }) != 0) return -1;
   const Counters& k = ctx->c.counters;
-  double v[15] = {k.gemm_flops,        k.gemm_launches,    (double)ctx->c.last_iterations,
+  double v[16] = {k.gemm_flops,        k.gemm_launches,    (double)ctx->c.last_iterations,
                   (double)ctx->c.arena.peak(), k.gemm_ms, k.gemm_flops_profiled,
                   k.prec_launches[0],  k.prec_ms[0],       k.prec_flops[0],
                   k.prec_launches[1],  k.prec_ms[1],       k.prec_flops[1],
-                  k.eig_calls,         k.eig_iterations,   k.eig_filter_gemms};
-  int m = n < 15 ? n : 15;
+                  k.eig_calls,         k.eig_iterations,   k.eig_filter_gemms,
+                  k.fu_repairs};
+  int m = n < 16 ? n : 16;
   for (int i = 0; i < m; ++i) counters[i] = v[i];
   return m;
 }
@@ -291,6 +292,13 @@ int spp_fu_evolution_step(spp_ctx* ctx, int n, spp_state* const* states, int chi
     if (ctx->c.precision == 1) fu_evolution_step<float>(ctx->c, v, chi, P, o);
     else fu_evolution_step<double>(ctx->c, v, chi, P, o);
   });
+}
+
+int spp_fu_set_repair(spp_ctx* ctx, spp_fu_repair_fn fn, void* user) {
+  if (!ctx) return -3;
+  ctx->c.fu_repair_fn = (void*)fn;
+  ctx->c.fu_repair_user = fn ? user : nullptr;
+  return 0;
 }
 
 int spp_state_free(spp_state* state) {

@@ -123,6 +123,7 @@ struct Counters {
   double prec_flops[2] = {0, 0};
   // truncation eigensolver: iterative calls, subspace iterations, filter products
   double eig_calls = 0, eig_iterations = 0, eig_filter_gemms = 0;
+  double fu_repairs = 0;  // bonds repaired by the PSD projection of the hole
 };
 
 // Event pairs recorded around contraction launches when profiling is on.
@@ -207,6 +208,10 @@ struct Context {
   // fu_fail_code (fu_fail_sample < 0: off).
   long long fu_fail_sample = -1;
   int fu_fail_bond = -1, fu_fail_code = 0;
+  // Host callback of the full update's PSD repair (spp_fu_repair_fn of the C
+  // ABI) and its user pointer; null: a bond that needs the repair fails its sample.
+  void* fu_repair_fn = nullptr;
+  void* fu_repair_user = nullptr;
   void flush_timer();       // accumulate finished event pairs into counters
 };
 

@@ -36,6 +36,7 @@ struct FuParams {
 // of the first bond that failed (fu::Status), or the codes below.
 constexpr int kFuNonfiniteEnvironment = 9;  // a row environment of the sample became nonfinite
 constexpr int kFuZeroTensor = 10;           // a site norm was zero or nonfinite at normalize_state!
+constexpr int kFuRepairFailed = 11;         // the PSD repair did not remove the residual failure
 inline const char* fu_failure_message(int code) {
   static const char* msg[] = {"ok",
                               "nonfinite FU norm metric",
@@ -47,8 +48,9 @@ inline const char* fu_failure_message(int code) {
                               "FU workspace exhausted",
                               "FU eigen/SVD solve did not converge",
                               "nonfinite row environment",
-                              "zero or nonfinite tensor after the full update"};
-  return (code >= 0 && code <= 10) ? msg[code] : "unknown failure";
+                              "zero or nonfinite tensor after the full update",
+                              "PSD repair did not remove the FU residual failure"};
+  return (code >= 0 && code <= 11) ? msg[code] : "unknown failure";
 }
 
 struct FuOutputs {

@@ -15,6 +15,7 @@
 // then the sites' double layers and the rolling row message; rows are
 // re-absorbed into the incremental boundary (_sweep!, full_update.jl:510-561).
 #include "fu.h"
+#include "../../include/stopeps_hip.h"
 #include "boundary_ops.h"
 #include <algorithm>
 #include <cmath>
@@ -268,8 +269,9 @@ template <typename T>
 __global__ void __launch_bounds__(kCoreLanes)
 fu_core_kernel(const T* nf, long long nf_stride, const double* target, const int* rlive, int ra, int era, int rb,
                int erb, int d, int ed, int D, int eD, fu::Opts o, double* aout, double* bout, double* rec,
-               int* bond_live, double* work, long long work_per, int nb, int nbtot, long long lds_doubles) {
-  const int s = blockIdx.x;
+               int* bond_live, double* work, long long work_per, int nb, int nbtot, long long lds_doubles,
+               int s_first) {
+  const int s = blockIdx.x + s_first;  // s_first > 0: the solve of one sample again (PSD repair)
   if (s >= nb) return;
   __shared__ double red[3 * (kCoreLanes / 64)];
   extern __shared__ double fu_lds[];
@@ -1193,14 +1195,148 @@ void fu_evolution_step(Context& ctx, const std::vector<StateDev*>& states, int c
             const long long lds = std::min(core_lds_doubles(G), lds_limit);
             hipLaunchKernelGGL(fu_core_kernel<T>, dim3(cur), dim3(kCoreLanes), (unsigned)(lds * sizeof(double)), st,
                                nfused, nfsz, tgt, rlive, G.sa.rcap, G.sa.recap, G.sb.rcap, G.sb.recap, d, ed, G.Dcap,
-                               G.eDcap, o, aout, bout, drec, blive, cwork, coremax, cur, nb, lds);
+                               G.eDcap, o, aout, bout, drec, blive, cwork, coremax, cur, nb, lds, 0);
             SPP_CHECK(hipGetLastError());
             SPP_CHECK(hipMemcpyAsync(hrec.data(), drec, sizeof(double) * 12 * cur, hipMemcpyDeviceToHost, st));
             SPP_CHECK(hipStreamSynchronize(st));
+            // PSD repair (update_bond!, full_update.jl:491-501): the complete bond
+            // hole of one sample, its PSD projection by the host callback, and the
+            // bond solve of that sample again on the repaired reduced metric
+            auto repair = [&](int s) -> int {
+              const int rec_before = (int)hrec[(size_t)s * 12 + 7];
+              std::vector<void*> bufs;
+              auto dmal = [&](long long elems) {
+                void* p = nullptr;
+                SPP_CHECK(hipMalloc(&p, sizeof(T) * (size_t)std::max(elems, 1LL)));
+                bufs.push_back(p);
+                return (T*)p;
+              };
+              auto free_all = [&]() {
+                (void)hipStreamSynchronize(st);
+                for (void* p : bufs) (void)hipFree(p);
+                bufs.clear();
+              };
+              try {
+                auto view = [&](const TRef& t) {
+                  if (t.ptrs || t.compact || t.fold) throw HipError("PSD repair: unsupported operand layout");
+                  TRef v = t;
+                  v.base = (T*)t.base + (long long)s * t.sstride;
+                  for (Leg& l : v.leg)
+                    if (l.split) l.split += s;
+                  return v;
+                };
+                auto leg = [](const TRef& t, char c) {
+                  const size_t k = t.letters.find(c);
+                  if (k == std::string::npos) throw HipError("PSD repair: missing leg");
+                  return t.leg[k];
+                };
+                auto make = [&](const std::string& letters, const std::vector<Leg>& legs) {
+                  const long long n = count_legs(legs);
+                  return dense_ref(dmal(n), n, letters, legs);
+                };
+                TRef H;
+                if (horizontal) {
+                  TRef L = view(with_letters(leftm, "awb")), T1 = view(N.ref(x, "anc")),
+                       B1 = view(bottom_ref(S, Lx, x, "bsf", true)), T2 = view(N.ref(x + 1, "cvd")),
+                       B2 = view(bottom_ref(S, Lx, x + 1, "ftg", true)), R = view(with_letters(rightm, "deg"));
+                  TRef X = make("wbnc", {leg(L, 'w'), leg(L, 'b'), leg(T1, 'n'), leg(T1, 'c')});
+                  contract<T>(ctx, X, L, T1, 1);
+                  TRef Lh = make("wncsf", {leg(L, 'w'), leg(T1, 'n'), leg(T1, 'c'), leg(B1, 's'), leg(B1, 'f')});
+                  contract<T>(ctx, Lh, X, B1, 1);
+                  TRef U = make("cveg", {leg(T2, 'c'), leg(T2, 'v'), leg(R, 'e'), leg(R, 'g')});
+                  contract<T>(ctx, U, R, T2, 1);
+                  TRef Rh = make("cveft", {leg(T2, 'c'), leg(T2, 'v'), leg(R, 'e'), leg(B2, 'f'), leg(B2, 't')});
+                  contract<T>(ctx, Rh, U, B2, 1);
+                  H = make("wnsvet", {leg(L, 'w'), leg(T1, 'n'), leg(B1, 's'), leg(T2, 'v'), leg(R, 'e'),
+                                      leg(B2, 't')});
+                  contract<T>(ctx, H, Lh, Rh, 1);
+                } else {
+                  TRef L = view(with_letters(leftm, "awvb")), T1 = view(N.ref(x, "anc")),
+                       B1 = view(bottom_ref(S, Lx, x, "bsd", true)), R = view(with_letters(rightm, "cefd"));
+                  TRef Lh = make("wvbnc", {leg(L, 'w'), leg(L, 'v'), leg(L, 'b'), leg(T1, 'n'), leg(T1, 'c')});
+                  contract<T>(ctx, Lh, L, T1, 1);
+                  TRef Rh = make("bscef", {leg(B1, 'b'), leg(B1, 's'), leg(R, 'c'), leg(R, 'e'), leg(R, 'f')});
+                  contract<T>(ctx, Rh, B1, R, 1);
+                  H = make("wnevfs", {leg(L, 'w'), leg(T1, 'n'), leg(R, 'e'), leg(L, 'v'), leg(R, 'f'),
+                                      leg(B1, 's')});
+                  contract<T>(ctx, H, Lh, Rh, 1);
+                }
+                const long long hn = count_legs(H.leg);
+                std::vector<T> ht((size_t)hn);
+                SPP_CHECK(hipMemcpyAsync(ht.data(), H.base, sizeof(T) * hn, hipMemcpyDeviceToHost, st));
+                // Q factors of the reduced pair (pseudo sites, FP64)
+                const FuSide* F2[2] = {&G.sa, &G.sb};
+                const double* qsrc[2] = {qa + (long long)s * qmax, qbuf + (long long)s * qmax};
+                spp_fu_repair_problem pr{};
+                std::vector<double> hq[2];
+                for (int side = 0; side < 2; ++side) {
+                  const FuSide& F = *F2[side];
+                  int* qd = side == 0 ? pr.qa_dims : pr.qb_dims;
+                  int* qe = side == 0 ? pr.qa_evens : pr.qb_evens;
+                  for (int k = 0; k < 6; ++k) {
+                    qd[k] = F.leg[k];
+                    qe[k] = F.even[k];
+                  }
+                  qd[2] = F.leg[3];
+                  qe[2] = F.even[3];
+                  qd[3] = 1;
+                  qe[3] = 1;
+                  qd[F.core] = F.rcap;
+                  qe[F.core] = F.recap;
+                  (side == 0 ? pr.qa_core : pr.qb_core) = F.core;
+                  long long qn = 1;
+                  for (int k = 0; k < 6; ++k) qn *= qd[k];
+                  hq[side].resize((size_t)qn);
+                  SPP_CHECK(hipMemcpyAsync(hq[side].data(), qsrc[side], sizeof(double) * qn, hipMemcpyDeviceToHost,
+                                           st));
+                }
+                SPP_CHECK(hipStreamSynchronize(st));
+                std::vector<double> hd(ht.begin(), ht.end());
+                // exterior legs of the hole: (owner Q, pseudo-site slot)
+                static const int hq_h[6] = {0, 0, 0, 1, 1, 1}, hs_h[6] = {0, 1, 5, 1, 4, 5};
+                static const int hs_v[6] = {0, 1, 4, 0, 4, 5};
+                for (int k = 0; k < 6; ++k) {
+                  pr.hole_q[k] = hq_h[k];
+                  pr.hole_slot[k] = horizontal ? hs_h[k] : hs_v[k];
+                  const int* qd = pr.hole_q[k] == 0 ? pr.qa_dims : pr.qb_dims;
+                  const int* qe = pr.hole_q[k] == 0 ? pr.qa_evens : pr.qb_evens;
+                  pr.hole_dims[k] = qd[pr.hole_slot[k]];
+                  pr.hole_evens[k] = qe[pr.hole_slot[k]];
+                }
+                pr.sample = (int)(c0 + s);
+                pr.bond = rec_bond;
+                pr.horizontal = horizontal ? 1 : 0;
+                pr.hole = hd.data();
+                pr.qa = hq[0].data();
+                pr.qb = hq[1].data();
+                pr.target_norm = hrec[(size_t)s * 12 + 3];
+                std::vector<double> nm((size_t)nfsz, 0.0);
+                const int cb = ((spp_fu_repair_fn)ctx.fu_repair_fn)(ctx.fu_repair_user, &pr, nm.data());
+                if (cb != 0) throw HipError("PSD repair: the host callback failed");
+                std::vector<T> nt(nm.begin(), nm.end());
+                upload(nfused + (long long)s * nfsz, nt.data(), sizeof(T) * nfsz, st);
+                hipLaunchKernelGGL(fu_core_kernel<T>, dim3(1), dim3(kCoreLanes), (unsigned)(lds * sizeof(double)), st,
+                                   nfused, nfsz, tgt, rlive, G.sa.rcap, G.sa.recap, G.sb.rcap, G.sb.recap, d, ed,
+                                   G.Dcap, G.eDcap, o, aout, bout, drec, blive, cwork, coremax, cur, nb, lds, s);
+                SPP_CHECK(hipGetLastError());
+                SPP_CHECK(hipMemcpyAsync(hrec.data() + (size_t)s * 12, drec + (size_t)s * 12, sizeof(double) * 12,
+                                         hipMemcpyDeviceToHost, st));
+                SPP_CHECK(hipStreamSynchronize(st));
+                free_all();
+              } catch (...) {
+                free_all();
+                throw;
+              }
+              ctx.counters.fu_repairs += 1;
+              (void)rec_before;
+              const int after = (int)hrec[(size_t)s * 12 + 7];
+              return after == fu::kRepairNeeded ? kFuRepairFailed : after;
+            };
             bool newly_failed = false;
             for (int s = 0; s < cur; ++s) {
               int code = (int)hrec[(size_t)s * 12 + 7];
               if (ctx.fu_fail_sample == c0 + s && ctx.fu_fail_bond == rec_bond) code = ctx.fu_fail_code;
+              if (code == fu::kRepairNeeded && ctx.fu_repair_fn && hfailed[s] == 0) code = repair(s);
               if (code != 0 && hfailed[s] == 0) {
                 // the sample keeps its sites from here on (restores are skipped)
                 // and reports the first failure; the others go on

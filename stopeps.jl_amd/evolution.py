@@ -10,6 +10,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
+from .repair import RepairCallback
 from .measurement import spin_operators
 from .sampling import DeviceState
 from .states import FiniteState, Site
@@ -38,7 +39,7 @@ STOP_REASONS = ("disabled", "iteration_limit", "error_tolerance", "residual_fall
 FAILURES = ("ok", "nonfinite FU norm metric", "zero FU norm metric", "gate produced a zero target",
             "negative FU residual needs PSD repair", "nonfinite FU residual/target", "singular FU gauge",
             "FU workspace exhausted", "FU eigen/SVD solve did not converge", "nonfinite row environment",
-            "zero or nonfinite tensor after the full update")
+            "zero or nonfinite tensor after the full update", "PSD repair did not remove the FU residual failure")
 
 
 class FullUpdateError(RuntimeError):
@@ -97,7 +98,7 @@ def _gates(model, lattice, delta, ppar):
 
 
 def evolution_steps(ctx, states, model, delta, chi, method=None, order=2, D=None, record_entropy=False,
-                    boundary_space=None, check=True):
+                    boundary_space=None, check=True, repair=True):
     """``evolution_step!(state, model, delta, D, FullUpdate(...), BoundaryMPS(chi);
     order, sector_space, record_entropy)`` for every uploaded state (``DeviceState``; same lattice
     and leg spaces up to the auxiliary charge), in place on the device.
@@ -112,7 +113,11 @@ def evolution_steps(ctx, states, model, delta, chi, method=None, order=2, D=None
     environment boundaries (``BoundaryMPS(chi, sector_space)``). ``status`` [n]
     is every sample's outcome (0 ok, else the index into ``FAILURES``); with
     ``check`` a failure raises :class:`FullUpdateError` after the step (the
-    other samples have finished it)."""
+    other samples have finished it). ``repair``: a bond whose residual check
+    requests the PSD repair of its metric (full_update.jl:491-501) is repaired
+    on the host (:mod:`repair`; a ``RepairCallback`` may be passed to observe or
+    alter it) and its records are returned as ``repairs``; False: such a bond
+    fails its sample."""
     method = method or FullUpdate()
     devs = list(states)
     n = len(devs)
@@ -140,15 +145,27 @@ def evolution_steps(ctx, states, model, delta, chi, method=None, order=2, D=None
     status = np.zeros(n, dtype=np.int32)
     handles = (ctypes.c_void_p * n)(*[d.handle.value for d in devs])
     o = method.options()
-    with ctx.boundary_space(boundary_space):
-        rc = ctx._lib.spp_fu_evolution_step(
-            ctx.handle, n, handles, int(chi), int(order), float(delta), int(D or 0), _lib.dptr(o), _lib.dptr(gh),
-            _lib.dptr(gv), _lib.dptr(rec), _lib.dptr(norms), _lib.dptr(mt), _lib.dptr(ent), _lib.iptr(status))
+    cb = None
+    if repair:
+        cb = repair if isinstance(repair, RepairCallback) else RepairCallback()
+        cb.install(ctx)
+    try:
+        with ctx.boundary_space(boundary_space):
+            rc = ctx._lib.spp_fu_evolution_step(
+                ctx.handle, n, handles, int(chi), int(order), float(delta), int(D or 0), _lib.dptr(o),
+                _lib.dptr(gh), _lib.dptr(gv), _lib.dptr(rec), _lib.dptr(norms), _lib.dptr(mt), _lib.dptr(ent),
+                _lib.iptr(status))
+    finally:
+        if cb is not None:
+            RepairCallback.uninstall(ctx)
+    if rc != 0 and cb is not None and cb.error is not None:
+        raise RuntimeError(f"PSD repair callback failed: {cb.error!r}") from cb.error
     _lib.check(ctx.handle, rc)
     for d in devs:
         d.state.beta += 2 * delta
     out = {"records": rec, "normalizations": norms, "max_environment_truncation": mt,
-           "max_boundary_entropy": ent if ent is not None else np.full(n, np.nan), "status": status}
+           "max_boundary_entropy": ent if ent is not None else np.full(n, np.nan), "status": status,
+           "repairs": list(cb.records) if cb is not None else []}
     if check and np.any(status):
         raise FullUpdateError(status)
     return out

@@ -253,7 +253,7 @@ def test_failed_sample_is_isolated_and_refused(spp, gpu_ctx):
         ctx.set_option("fu_fail_bond", 3)
         ctx.set_option("fu_fail_code", 4)
         with pytest.raises(spp.FullUpdateError) as err:
-            spp.evolution_steps(ctx, got, model, 0.05, 8, method)
+            spp.evolution_steps(ctx, got, model, 0.05, 8, method, repair=False)
     finally:
         ctx.set_option("fu_fail_sample", -1)
     assert list(err.value.status) == [0, 4, 0] and "needs PSD repair" in str(err.value)
@@ -269,3 +269,72 @@ def test_failed_sample_is_isolated_and_refused(spp, gpu_ctx):
         spp.evolution_steps(ctx, [got[1]], model, 0.05, 8, method)
     # it kept finite sites (no write after the failing bond)
     assert all(np.all(np.isfinite(got[1].download()[xy].data)) for xy in L.site_order())
+
+
+def test_psd_repair_branch_matches_the_oracle(spp, gpu_ctx):
+    """The PSD-repair branch of update_bond! (full_update.jl:491-501), taken
+    at one bond update of one sample (forced from the host through the test
+    hook; every kernel runs normally): the device contracts that sample's
+    complete hole, the host projects it and the bond is solved again on the
+    repaired metric. Checked against the oracle taking the same branch at the
+    same bond, the other sample against the plain oracle step."""
+    say("full update: PSD repair of bond update 5 of sample 1")
+    ctx = gpu_ctx
+    ctx.set_precision("fp64")
+    L = Lattice(3, 3)
+    m = TFIM(J=1.0, h=2.0, symmetric=True)
+    model = spp.TFIM(m.J, m.h, symmetric=True)
+    opts = ofu.FullUpdate(10, 1e-8, 1e-10, 50, 1e-9)
+    method = spp.FullUpdate(opts.max_iterations, opts.error_tolerance, opts.diagonal_shift,
+                            opts.linear_max_iterations, opts.linear_tolerance)
+    ostates = [random_state(L, (1, 1), "peps", seed=90 + k) for k in range(2)]
+    devs = [spp.DeviceState(ctx, to_product_state(spp, o)) for o in ostates]
+    ctx.reset_counters()
+    try:
+        ctx.set_option("fu_fail_sample", 1)
+        ctx.set_option("fu_fail_bond", 5)
+        ctx.set_option("fu_fail_code", 4)
+        r = spp.evolution_steps(ctx, devs, model, 0.05, 16, method)
+    finally:
+        ctx.set_option("fu_fail_sample", -1)
+    assert list(r["status"]) == [0, 0]
+    assert len(r["repairs"]) == 1 and r["repairs"][0]["sample"] == 1 and r["repairs"][0]["bond"] == 5
+    assert ctx.counters()["fu_repairs"] == 1
+    for k, o in enumerate(ostates):
+        ref = ofu.evolution_step(o, m, 0.05, 2, (1, 1), opts, 16, force_repair=(5,) if k == 1 else ())
+        if k == 1:
+            assert len(ref["records"][5]["repairs"]) == 1
+        costs = np.array([rr["final_cost"] for rr in ref["records"]])
+        np.testing.assert_allclose(r["records"][k, :, 1], costs, rtol=1e-6, atol=1e-9)
+        v_dev, v_ref = dense_vector(to_oracle(devs[k].download())), dense_vector(o)
+        err = np.linalg.norm(v_dev - v_ref) / np.linalg.norm(v_ref)
+        say(f"  state {k}: |dpsi|/|psi| = {err:.2e}")
+        assert err <= 1e-8
+
+
+def test_psd_repair_that_does_not_remove_the_failure(spp, gpu_ctx):
+    """A repaired metric on which the residual check still fails (here the
+    host returns the negated projection) fails the sample with the
+    reference's message; the other sample finishes."""
+    say("full update: a repair that does not remove the residual failure")
+    ctx = gpu_ctx
+    ctx.set_precision("fp64")
+    L = Lattice(3, 3)
+    model = spp.TFIM(1.0, 2.0, symmetric=True)
+    method = spp.FullUpdate(10, 1e-8, 1e-10, 50, 1e-9)
+    devs = [spp.DeviceState(ctx, to_product_state(spp, random_state(L, (1, 1), "peps", seed=95 + k)))
+            for k in range(2)]
+
+    class Negated(spp.repair.RepairCallback):
+        def transform(self, metric):
+            metric *= -1.0
+    try:
+        ctx.set_option("fu_fail_sample", 0)
+        ctx.set_option("fu_fail_bond", 2)
+        ctx.set_option("fu_fail_code", 4)
+        r = spp.evolution_steps(ctx, devs, model, 0.05, 16, method, repair=Negated(), check=False)
+    finally:
+        ctx.set_option("fu_fail_sample", -1)
+    assert list(r["status"]) == [11, 0]
+    err = spp.FullUpdateError(r["status"])
+    assert "PSD repair did not remove the FU residual failure" in str(err)
