@@ -20,6 +20,7 @@ from .particles import (Particle, ensemble_statistics, step_schedule, purify, st
                         advance_particles, statistics, group_configurations)
 from .config import RunConfig, ResamplingOptions, validate_config, temperature_nodes, parameter_dict
 from .driver import ThermalCheckpoint, run_purification, run_stochastic
+from .results import save_result, load_result, save_checkpoint, load_checkpoint, RESULT_SCHEMA_VERSION
 from . import _lib, particles, repair
 
 __all__ = ["synthetic_thermal_pepo", "random_peps", "Lattice", "Site", "FiniteState", "FinitePEPO", "FinitePEPS", "identity_pepo", "parity_sorted",
@@ -32,4 +33,5 @@ __all__ = ["synthetic_thermal_pepo", "random_peps", "Lattice", "Site", "FiniteSt
            "step_schedule", "purify", "stochastic_ensemble", "new_particles", "advance_particles", "statistics",
            "group_configurations", "normalize_state", "reembed_bonds", "RunConfig", "ResamplingOptions",
            "validate_config", "parameter_dict", "ThermalCheckpoint", "run_purification", "run_stochastic",
-           "save_state_npz", "load_state_npz", "FullUpdateError"]
+           "save_state_npz", "load_state_npz", "FullUpdateError", "save_result", "load_result",
+           "save_checkpoint", "load_checkpoint", "RESULT_SCHEMA_VERSION"]

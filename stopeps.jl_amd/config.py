@@ -6,7 +6,7 @@ schema.
 Spaces are written as this library writes them: a Z2 bond space as its
 (even, odd) sector dimensions, a dense one as an int. Fields that select
 reference features this library does not provide (diagonal or variational
-contraction, annealed resampling, file output) are kept
+contraction, annealed resampling) are kept
 so that a configuration reads the same; the drivers reject them with a
 message naming the feature.
 """

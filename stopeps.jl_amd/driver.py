@@ -1,13 +1,18 @@
 """``run_purification`` / ``run_stochastic`` on a ``RunConfig``
 (src/driver/run.jl:1-135) with the result record of the reference
-(src/io/run_output.jl:4-60, docs/result-schema.md, schema 2.7.0), kept in
-memory: the checkpoint is a ``ThermalCheckpoint`` object whose PEPO stays
-resident on the GPU, and ``run_stochastic`` returns the result dictionary
-(parameters, model, metadata, observables {purification, curve, samples,
-replay}, diagnostics {history, events, born_draws, born_grouping}). File
-output, MPI and resampling are not provided; rows and records are plain dicts
-where the reference has NamedTuples.
+(src/io/run_output.jl:4-60, docs/result-schema.md, schema 2.7.0): the
+checkpoint is a ``ThermalCheckpoint`` object whose PEPO stays resident on the
+GPU, and ``run_stochastic`` returns the result dictionary (parameters, model,
+metadata, observables {purification, curve, samples, replay}, diagnostics
+{history, events, born_draws, born_grouping}). With ``output_directory`` set,
+the checkpoint is written to ``results/checkpoint.npz`` and the result to
+``results/result.npz`` (``results.py``: the reference's groups and atomic
+publication in an npz container instead of JLD2). MPI, the text logs and
+resampling are not provided; rows and records are plain dicts where the
+reference has NamedTuples.
 """
+import os
+
 import datetime
 import time
 
@@ -17,7 +22,7 @@ from .particles import purify, same_beta, stochastic_ensemble
 from .sampling import Context, DeviceState
 from .states import identity_pepo
 
-CHECKPOINT_FORMAT = "stopeps-tensorkit-thermal-1.0.0"
+from .results import CHECKPOINT_FORMAT, load_checkpoint, save_checkpoint, save_result
 
 
 class ThermalCheckpoint:
@@ -78,20 +83,33 @@ def run_purification(c, ctx=None, on_step=None):
     out["metadata"]["stage_complete"] = True
     cp = ThermalCheckpoint(dev, out["parameters"], list(rows), history, run_id)
     cp.result = out
+    if c.output_directory is not None:
+        path = os.path.join(c.output_directory, "results", "checkpoint.npz")
+        if os.path.lexists(path) and not c.overwrite_output:
+            raise FileExistsError("checkpoint already exists; choose another directory or overwrite_output=true")
+        save_checkpoint(path, cp, overwrite=c.overwrite_output)
     return cp
 
 
-def run_stochastic(c, checkpoint, ctx=None, uniforms=None):
-    """``run_stochastic(c; checkpoint)`` (run.jl:87-135) without resampling:
+def run_stochastic(c, checkpoint=None, ctx=None, uniforms=None, checkpoint_path=None):
+    """``run_stochastic(c; checkpoint, checkpoint_path)`` (run.jl:76-135) without resampling:
     Ns Born draws from the checkpoint's beta0 PEPO (sample indices 1..Ns,
     uniforms from ``sample_uniforms(seed, i, Lx*Ly)`` unless given), one
     particle per distinct configuration, full update (D_peps,
     bond_space_peps, delta_tau_peps) through every node from beta0, an
-    observation and the ensemble statistics at each. Returns the result
-    dictionary."""
+    observation and the ensemble statistics at each. Without ``checkpoint``
+    the checkpoint is loaded from ``checkpoint_path`` or the output
+    directory's ``results/checkpoint.npz`` (run.jl:80-88). Returns the result
+    dictionary, also saved to ``results/result.npz`` of the output directory."""
     validate_config(c)
     device_support(c, stochastic=True)
     ctx = _context(ctx)
+    if checkpoint is None:
+        if checkpoint_path is None:
+            if c.output_directory is None:
+                raise ValueError("supply a checkpoint or checkpoint_path")
+            checkpoint_path = os.path.join(c.output_directory, "results", "checkpoint.npz")
+        checkpoint = load_checkpoint(checkpoint_path, c, ctx)
     p = parameter_dict(c)
     pre = checkpoint.preparation
     if not (pre["model"] == p["model"] and pre["lattice"] == p["lattice"]):
@@ -129,5 +147,9 @@ def run_stochastic(c, checkpoint, ctx=None, uniforms=None):
         for i in range(len(draws["log_probability"]))]
     out["metadata"]["complete"] = True
     out["metadata"]["stage_complete"] = True
+    if c.output_directory is not None:
+        save_result(os.path.join(c.output_directory, "results", "result.npz"), method="stoPEPS", model=out["model"],
+                    parameters=out["parameters"], observables=out["observables"], diagnostics=out["diagnostics"],
+                    metadata=out["metadata"], overwrite=True)
     out["particles"] = particles
     return out
