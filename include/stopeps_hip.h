@@ -50,7 +50,10 @@ int spp_synchronize(spp_ctx* ctx);
 
 /* Arithmetic of the contraction GEMMs: 0 = FP64 everywhere, 1 = FP32 MFMA
  * contractions with FP64 eigensolvers, log weights and probabilities.
- * Takes effect for caches built afterwards. */
+ * 2 and 3 = as 1 (FP32 storage and accumulation), with each product formed on
+ * the BF16 matrix cores from BF16 pieces of the FP32 operands: 2 = three
+ * pieces and six piece products (FP32-equivalent), 3 = two pieces and three
+ * products (about 2^-16 relative). Takes effect for caches built afterwards. */
 int spp_set_precision(spp_ctx* ctx, int precision);
 
 /* Tuning: "ksub" (subspace size of the truncation eigensolver, <= 64),
@@ -358,7 +361,8 @@ int spp_selftest_fu_pair(int horizontal, const int* dims_a, const int* evens_a, 
  * leg whose first evenX entries are even. Only parity-matching (neutral)
  * entries of C are computed; the others are returned as NaN. Pass
  * evenX = X for a plain dense product. layout bit 0: A is given as its
- * transpose [K][M]; bit 1: B is given as its transpose [N][K]. */
+ * transpose [K][M]; bit 1: B is given as its transpose [N][K]. precision as
+ * spp_set_precision (for this call only). */
 int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, int evenM, int evenN, int evenK,
                       int layout, const double* A, const double* B, double* C);
 

@@ -104,8 +104,11 @@ int spp_synchronize(spp_ctx* ctx) {
 
 int spp_set_precision(spp_ctx* ctx, int precision) {
   return guarded(ctx, [&] {
-    if (precision != 0 && precision != 1) throw HipError("precision must be 0 (FP64) or 1 (FP32)");
-    ctx->c.precision = precision;
+    if (precision < 0 || precision > 3)
+      throw HipError("precision must be 0 (FP64), 1 (FP32), 2 (FP32 from six BF16 products) or 3 (FP32 from "
+                     "three BF16 products)");
+    ctx->c.precision = precision == 0 ? 0 : 1;
+    ctx->c.split = precision == 2 ? 3 : precision == 3 ? 2 : 0;
   });
 }
 
@@ -574,7 +577,16 @@ int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, int even
     if (M <= 0 || N <= 0 || K <= 0) throw HipError("dimensions must be positive");
     if (evenM < 0 || evenM > M || evenN < 0 || evenN > N || evenK < 0 || evenK > K)
       throw HipError("even counts must lie in [0, dim]");
+    if (precision < 0 || precision > 3) throw HipError("precision must lie in [0, 3]");
     hipStream_t st = ctx->c.stream;
+    const int split = ctx->c.split;
+    ctx->c.split = precision == 2 ? 3 : precision == 3 ? 2 : 0;
+    struct Restore {
+      Context& c;
+      int v;
+      ~Restore() { c.split = v; }
+    } restore{ctx->c, split};
+    if (precision > 1) precision = 1;
     size_t es = precision == 1 ? sizeof(float) : sizeof(double);
     void *dA, *dB, *dC, *tA = nullptr, *tB = nullptr;
     SPP_CHECK(hipMalloc(&dA, sizeof(double) * M * K));

@@ -178,6 +178,10 @@ struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   int precision = 0;        // 0: FP64, 1: FP32 contraction GEMMs (FP64 factorizations)
+  // FP32 contraction products: 0 on the FP32 matrix cores, 2 or 3 from that
+  // many BF16 pieces per operand on the BF16 matrix cores (3: six piece
+  // products, FP32-equivalent; 2: three products, about 2^-16 relative)
+  int split = 0;
   int ksub = 64;            // subspace size for truncation eigenproblems
   int max_subspace_iter = 40;
   double subspace_tol = 1e-12;

@@ -245,6 +245,12 @@ static std::vector<Leg> legs_of(const TRef& t, const std::string& letters) {
 }
 
 template <typename T>
+static hipError_t launch_gemm(const Context& ctx, const GemmArgs& g, long long mmax, long long nmax) {
+  if (sizeof(T) == 4 && ctx.split) return launch_z2gemm_split(g, ctx.split, mmax, nmax, ctx.stream);
+  return launch_z2gemm<T>(g, mmax, nmax, ctx.stream);
+}
+
+template <typename T>
 void contract(Context& ctx, const TRef& C, const TRef& A, const TRef& B, int nsamples, double alpha,
               bool accumulate, const int* pmask) {
   std::string M, N, K;
@@ -323,7 +329,7 @@ void contract(Context& ctx, const TRef& C, const TRef& A, const TRef& B, int nsa
     SPP_CHECK(hipEventCreate(&a));
     SPP_CHECK(hipEventCreate(&b));
     SPP_CHECK(hipEventRecord(a, ctx.stream));
-    SPP_CHECK(launch_z2gemm<T>(g, mmax, nmax, ctx.stream));
+    SPP_CHECK(launch_gemm<T>(ctx, g, mmax, nmax));
     SPP_CHECK(hipEventRecord(b, ctx.stream));
     ctx.timer.pending.emplace_back(a, b);
     ctx.timer.pending_flops.push_back(2.0 * fl * nsamples);
@@ -331,7 +337,7 @@ void contract(Context& ctx, const TRef& C, const TRef& A, const TRef& B, int nsa
     if (ctx.timer.pending.size() >= 4096) ctx.flush_timer();
     return;
   }
-  SPP_CHECK(launch_z2gemm<T>(g, mmax, nmax, ctx.stream));
+  SPP_CHECK(launch_gemm<T>(ctx, g, mmax, nmax));
 }
 
 void Context::flush_timer() {

@@ -59,6 +59,11 @@ struct GemmArgs {
 // bounds of the per-problem M and N counts.
 template <typename T>
 hipError_t launch_z2gemm(const GemmArgs& args, long long mmax, long long nmax, hipStream_t stream);
+// FP32 operands and result with the products formed from BF16 pieces on the
+// BF16 matrix cores (pieces = 3: six piece products, FP32-equivalent;
+// pieces = 2: three piece products, about 2^-16 relative).
+hipError_t launch_z2gemm_split(const GemmArgs& args, int pieces, long long mmax, long long nmax,
+                               hipStream_t stream);
 
 // ---------------------------------------------------------------------------
 // Group decoding, shared with the other kernels.

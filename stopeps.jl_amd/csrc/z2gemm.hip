@@ -354,6 +354,252 @@ __global__ __launch_bounds__(kThreads, MINW) void z2gemm_kernel(GemmArgs args) {
       }
 }
 
+// ---------------------------------------------------------------------------
+// BF16 split products (the "bf16x6" / "bf16x3" precision modes).
+//
+// FP32 operands are split when staged into LDS: hi = bf16(x), mid =
+// bf16(x - hi), lo = bf16(x - hi - mid) (round to nearest even), so that
+// x = hi + mid + lo to 2^-24 relative. The product sum_k a_k b_k is formed on
+// the BF16 matrix cores (v_mfma_f32_32x32x16_bf16, FP32 accumulation) from the
+// piece products whose orders add up to at most NP - 1: NP = 3 pieces take
+// hh, hm, mh, hl, lh, mm (six products; the omitted ones are below 2^-24
+// relative, so the result matches an FP32 product with FP32 accumulation),
+// NP = 2 pieces take hh, hm, mh (three products, about 2^-16 relative).
+// The LDS tiles are row-major with the K index contiguous, one array per piece,
+// so each MFMA operand (eight consecutive K values of one row) is one 16-byte read.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ inline unsigned short bf16_bits(float x) {  // round to nearest even
+  const unsigned u = __float_as_uint(x);
+  return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline float bf16_value(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+constexpr int SBK = 16;       // K tile of the split kernel: one MFMA step
+constexpr int SLD = SBK + 8;  // LDS row stride in BF16 elements (48 bytes)
+
+template <int BM, int BN, int WGM, int MINW, int AMF, int BKF, int NP>
+__global__ __launch_bounds__(kThreads, MINW) void z2gemm_split_kernel(GemmArgs args) {
+  constexpr int BK = SBK;
+  constexpr int WGN = 4 / WGM;
+  constexpr int WTM = BM / WGM, WTN = BN / WGN;
+  constexpr int TA = WTM / 32, TB = WTN / 32;
+  static_assert(TA * 32 == WTM && TB * 32 == WTN, "wave tile must be a multiple of 32");
+  using MA = TileMap<BM, BK, AMF>;
+  using MB = TileMap<BN, BK, 1 - BKF>;
+  __shared__ __align__(16) unsigned short As[NP][BM * SLD];
+  __shared__ __align__(16) unsigned short Bs[NP][BN * SLD];
+  __shared__ unsigned offAm[BM], offBn[BN];
+  __shared__ int offCm[BM], offCn[BN];
+  __shared__ unsigned offAk[KLDS], offBk[KLDS];
+  __shared__ GroupView gvM, gvN, gvK;
+
+  const int tid = threadIdx.x;
+  int bx, by, prob;
+  tile_order(bx, by, prob);
+  bx = __builtin_amdgcn_readfirstlane(bx);
+  by = __builtin_amdgcn_readfirstlane(by);
+  prob = __builtin_amdgcn_readfirstlane(prob);
+  const int sp = prob >> 1;
+  const int q = prob & 1;
+  if (args.pmask && !args.pmask[prob]) return;
+  const int s = args.sample_map ? args.sample_map[sp] : sp;
+
+  if (tid == 0) resolve_group(args.gm, s, gvM);
+  if (tid == 1) resolve_group(args.gn, s, gvN);
+  if (tid == 2) resolve_group(args.gk, s, gvK);
+  __syncthreads();
+  const int cM = (int)group_count(gvM, q);
+  const int cN = (int)group_count(gvN, q);
+  const int cK = (int)group_count(gvK, q);
+  const int m0 = by * BM;
+  const int n0 = bx * BN;
+  if (m0 >= cM || n0 >= cN) return;
+
+  int legs[kMaxLegs];
+  for (int r = tid; r < BM + BN; r += kThreads) {
+    if (r < BM) {
+      const int idx = m0 + r;
+      if (idx < cM) {
+        decode32(gvM, q, idx, legs);
+        offAm[r] = (unsigned)operand_offset(args.a.g[0], gvM, idx, legs) * (unsigned)sizeof(float);
+        offCm[r] = (int)operand_offset(args.c.g[0], gvM, idx, legs);
+      } else {
+        offAm[r] = 0;
+        offCm[r] = -1;
+      }
+    } else {
+      const int t = r - BM;
+      const int idx = n0 + t;
+      if (idx < cN) {
+        decode32(gvN, q, idx, legs);
+        offBn[t] = (unsigned)operand_offset(args.b.g[1], gvN, idx, legs) * (unsigned)sizeof(float);
+        offCn[t] = (int)operand_offset(args.c.g[1], gvN, idx, legs);
+      } else {
+        offBn[t] = 0;
+        offCn[t] = -1;
+      }
+    }
+  }
+
+  const __amdgpu_buffer_rsrc_t ra_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)operand_base<float>(args.a, s, q), 0, (int)args.a_bytes, kBufferFlags);
+  const __amdgpu_buffer_rsrc_t rb_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)operand_base<float>(args.b, s, q), 0, (int)args.b_bytes, kBufferFlags);
+  float* Cb = operand_base<float>(args.c, s, q);
+
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave / WGN) * WTM;
+  const int wn = (wave % WGN) * WTN;
+  const int li = lane & 31, lk = lane >> 5;
+  floatx16 acc[TA][TB];
+#pragma unroll
+  for (int a = 0; a < TA; ++a)
+#pragma unroll
+    for (int b = 0; b < TB; ++b)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+
+  float ra[MA::N], rb[MB::N];
+  auto load_tile = [&](int kt) {
+#pragma unroll
+    for (int j = 0; j < MA::N; ++j)
+      ra[j] = buffer_load(ra_rsrc, offAm[MA::row(tid, j)] + offAk[kt + MA::k(tid, j)], 0.f);
+#pragma unroll
+    for (int j = 0; j < MB::N; ++j)
+      rb[j] = buffer_load(rb_rsrc, offBn[MB::row(tid, j)] + offBk[kt + MB::k(tid, j)], 0.f);
+  };
+  auto store_tile = [&]() {
+#pragma unroll
+    for (int j = 0; j < MA::N; ++j) {
+      const int o = MA::row(tid, j) * SLD + MA::k(tid, j);
+      float x = ra[j];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        const unsigned short h = bf16_bits(x);
+        As[p][o] = h;
+        x -= bf16_value(h);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < MB::N; ++j) {
+      const int o = MB::row(tid, j) * SLD + MB::k(tid, j);
+      float x = rb[j];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        const unsigned short h = bf16_bits(x);
+        Bs[p][o] = h;
+        x -= bf16_value(h);
+      }
+    }
+  };
+
+  for (int kc = 0; kc < cK; kc += KLDS) {
+    const int klen = min(KLDS, cK - kc);
+    const int nt = (klen + BK - 1) / BK;
+    for (int k = tid; k < nt * BK; k += kThreads) {
+      if (k < klen) {
+        decode32(gvK, q, kc + k, legs);
+        offAk[k] = (unsigned)operand_offset(args.a.g[1], gvK, kc + k, legs) * (unsigned)sizeof(float);
+        offBk[k] = (unsigned)operand_offset(args.b.g[0], gvK, kc + k, legs) * (unsigned)sizeof(float);
+      } else {
+        offAk[k] = kInvalidK;
+        offBk[k] = kInvalidK;
+      }
+    }
+    __syncthreads();
+    load_tile(0);
+    for (int t = 0; t < nt; ++t) {
+      store_tile();
+      __syncthreads();
+      if (t + 1 < nt) load_tile((t + 1) * BK);
+      // operands of this K step: eight consecutive K values per lane and piece
+      bf16x8 av[NP][TA], bv[NP][TB];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+#pragma unroll
+        for (int ta = 0; ta < TA; ++ta)
+          av[p][ta] = *reinterpret_cast<const bf16x8*>(&As[p][(wm + 32 * ta + li) * SLD + 8 * lk]);
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb)
+          bv[p][tb] = *reinterpret_cast<const bf16x8*>(&Bs[p][(wn + 32 * tb + li) * SLD + 8 * lk]);
+      }
+#pragma unroll
+      for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb) {
+          // smallest products first, the leading one last
+#pragma unroll
+          for (int o = NP - 1; o >= 0; --o)
+#pragma unroll
+            for (int pa = 0; pa <= o; ++pa)
+              acc[ta][tb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[pa][ta], bv[o - pa][tb], acc[ta][tb], 0, 0,
+                                                                    0);
+        }
+      __syncthreads();
+    }
+  }
+  if (cK == 0) __syncthreads();
+
+  const float alpha = (float)args.alpha;
+#pragma unroll
+  for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < TB; ++tb)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int i = ta * 32 + (v & 3) + 4 * (lane >> 5) + 8 * (v >> 2);
+        const int j = tb * 32 + (lane & 31);
+        const int om = offCm[wm + i];
+        const int on = offCn[wn + j];
+        if (om >= 0 && on >= 0) {
+          float val = alpha * acc[ta][tb][v];
+          float* c = Cb + (unsigned)(om + on);
+          if (args.accumulate) val += *c;
+          *c = val;
+        }
+      }
+}
+
+template <int BM, int BN, int WGM, int MINW, int NP>
+static hipError_t launch_split_shape(const GemmArgs& args, long long mmax, long long nmax, hipStream_t stream) {
+  const long long gy = (mmax + BM - 1) / BM;
+  const long long gx = (nmax + BN - 1) / BN;
+  const long long gz = 2LL * args.nsamples;
+  if (gy > 65535 || gz > 65535 || gx > 2147483647LL || gx * gy * gz >= (1LL << 32))
+    return hipErrorInvalidConfiguration;
+  dim3 grid((unsigned)gx, (unsigned)gy, (unsigned)gz);
+  const int mode = (args.a_mfast ? 1 : 0) + (args.b_kfast ? 2 : 0);
+  if (mode == 0)
+    hipLaunchKernelGGL((z2gemm_split_kernel<BM, BN, WGM, MINW, 0, 0, NP>), grid, dim3(kThreads), 0, stream, args);
+  else if (mode == 1)
+    hipLaunchKernelGGL((z2gemm_split_kernel<BM, BN, WGM, MINW, 1, 0, NP>), grid, dim3(kThreads), 0, stream, args);
+  else if (mode == 2)
+    hipLaunchKernelGGL((z2gemm_split_kernel<BM, BN, WGM, MINW, 0, 1, NP>), grid, dim3(kThreads), 0, stream, args);
+  else
+    hipLaunchKernelGGL((z2gemm_split_kernel<BM, BN, WGM, MINW, 1, 1, NP>), grid, dim3(kThreads), 0, stream, args);
+  return hipGetLastError();
+}
+
+template <int NP>
+static hipError_t launch_split(const GemmArgs& args, long long mmax, long long nmax, hipStream_t stream) {
+  if (nmax <= 32) return launch_split_shape<256, 32, 4, 2, NP>(args, mmax, nmax, stream);
+  auto padded = [&](long long bm, long long bn) {
+    return ((mmax + bm - 1) / bm) * bm * (((nmax + bn - 1) / bn) * bn);
+  };
+  if (padded(256, 96) < padded(128, 128)) return launch_split_shape<256, 96, 4, 2, NP>(args, mmax, nmax, stream);
+  return launch_split_shape<128, 128, 2, 2, NP>(args, mmax, nmax, stream);
+}
+
+hipError_t launch_z2gemm_split(const GemmArgs& args, int pieces, long long mmax, long long nmax,
+                               hipStream_t stream) {
+  if (mmax <= 0 || nmax <= 0 || args.nsamples <= 0) return hipSuccess;
+  if (pieces == 2) return launch_split<2>(args, mmax, nmax, stream);
+  if (pieces == 3) return launch_split<3>(args, mmax, nmax, stream);
+  return hipErrorInvalidValue;
+}
+
 // Block tile shapes per element type: (BM, BN, wave rows, K tile, waves per
 // SIMD, double-buffered LDS).
 template <typename T>

@@ -16,7 +16,11 @@ from . import _lib
 from .states import parity_sorted, resolve_basis, normalize_state, FiniteState, Site
 
 MASK64 = (1 << 64) - 1
-PRECISIONS = {"fp64": 0, "fp32": 1}
+# Contraction arithmetic: FP64; FP32 on the FP32 matrix cores; FP32 operands
+# and accumulation with the products formed from three BF16 pieces per operand
+# (six piece products, FP32-equivalent) or two (three products, about 2^-16
+# relative) on the BF16 matrix cores.
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x6": 2, "bf16x3": 3}
 
 
 class BornError(ValueError):

@@ -10,7 +10,8 @@ src/sampling/born.jl:86-163), the projected samples' log norms
 
 Bars: FP64 configurations identical, log p and conditionals within 1e-9,
 tensor log norms within 1e-9 relative, energies within 1e-8; FP32 (mixed
-precision) energy per site within 1e-6 of the oracle on the weighted ensemble
+precision, with the products on the FP32 matrix cores or as six BF16 piece
+products) energy per site within 1e-6 of the oracle on the weighted ensemble
 exp(log_norm - log q) (the estimate the metric quotes) and within 2e-6 per
 sample (measured: 4.3e-7 .. 1.14e-6), and any FP32 draw that differs from the
 oracle's differs first at a site whose uniform lies within 1e-5 of the
@@ -95,11 +96,12 @@ def test_cfg2_fp64_draws_and_energies_match_oracle(spp, gpu_ctx, fixture):
         assert abs(e["log_norm"][j] - ref["log_norm"][k]) <= 1e-9 * max(1.0, abs(ref["log_norm"][k]))
 
 
-def test_cfg2_fp32_energy_per_site_within_1e6_of_oracle(spp, gpu_ctx, fixture):
+@pytest.mark.parametrize("precision", ["fp32", "bf16x6"])
+def test_cfg2_fp32_energy_per_site_within_1e6_of_oracle(spp, gpu_ctx, fixture, precision):
     pepo, ref = fixture
-    say("8x8 TFIM h=2 beta0=0.5 D=6 chi=32: FP32 draws and energies against the FP64 oracle")
+    say(f"8x8 TFIM h=2 beta0=0.5 D=6 chi=32: {precision} draws and energies against the FP64 oracle")
     try:
-        cache, uni, got = _draws(spp, gpu_ctx, pepo, ref, "fp32")
+        cache, uni, got = _draws(spp, gpu_ctx, pepo, ref, precision)
         for k in range(len(ref["sample_index"])):
             if np.array_equal(got["configuration"][k], ref["configuration"][k]):
                 lp = float(ref["log_probability"][k])
