@@ -69,7 +69,10 @@ int spp_set_precision(spp_ctx* ctx, int precision);
  * "fu_fail_sample", "fu_fail_bond", "fu_fail_code" (a test hook of the full
  * update's failure handling: bond update fu_fail_bond, in sweep order, of
  * sample fu_fail_sample of the next steps is treated as failed with
- * fu_fail_code; a negative sample turns it off). */
+ * fu_fail_code; a negative sample turns it off); "gram_chunk_bytes" (per-sample
+ * byte limit of one Gram transfer intermediate of a boundary absorb: the bra
+ * bond is taken in index ranges that keep it below the limit; default 2^30;
+ * results do not depend on it beyond summation order). */
 int spp_set_option(spp_ctx* ctx, const char* name, double value);
 
 /* counters[0] = contraction flops issued (2*M*N*K over parity blocks),

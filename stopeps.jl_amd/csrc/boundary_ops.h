@@ -1,6 +1,8 @@
 // Boundary-MPS helpers shared by the sampler and the energy measurement.
 #pragma once
 #include "engine.h"
+#include <algorithm>
+#include <cmath>
 
 namespace spp {
 
@@ -52,13 +54,28 @@ inline void freeze_splits(Context& ctx, Boundary<T>& b) {
   }
 }
 
+// Elements per sample and per index of the bra bond R of the three Gram
+// transfer intermediates of absorb_forward at one site (environment bond l,
+// layer legs w, n, e, s): the largest of lneF, lFwp and lwWN.
+inline long long gram_unit_elems(long long l, long long w, long long n, long long e, long long s) {
+  return std::max(l * n * e * e, std::max(l * e * w * s, l * w * w * n));
+}
+// Length of the R ranges of a site: all of R when that fits, else the largest
+// range that keeps one intermediate within `limit` bytes per sample (at least 1).
+inline int gram_rchunk(long long unit, long long rdim, size_t elem, double limit = kGramChunkBytes) {
+  const double fit = limit / ((double)elem * (double)std::max(unit, 1LL));
+  return (int)std::max(1.0, std::min((double)rdim, std::floor(fit)));
+}
+
 // Workspace bytes per sample of absorb_forward on a row of Lx layers with
-// double-layer legs of dimension at most D: the Gram transfers, the right Grams
-// and the truncation sweep (compact rho, subspace blocks, kept tensors).
+// double-layer legs of dimension at most D: the Gram transfers (over R ranges),
+// the right Grams and the truncation sweep (compact rho, subspace blocks, kept
+// tensors).
 inline double absorb_bytes_per_sample(long long Lx, long long chi, long long D, size_t elem, int ksub) {
   const double ts = (double)elem;
   const double half = chi * D / 2.0 + 1;
-  double z = ts * 2.0 * chi * chi * D * D * D;
+  const long long unit = chi * D * D * D;
+  double z = ts * 2.0 * (double)unit * gram_rchunk(unit, chi, elem);
   double g = ts * (Lx + 1) * (double)(chi * D) * (chi * D);
   double sweep = ts * 5.0 * chi * chi * D * D + (8.0 + ts) * 2.0 * half * half +
                  8.0 * 2.0 * (4.0 * half * ksub + 3.0 * ksub * ksub) + ts * Lx * chi * chi * D;

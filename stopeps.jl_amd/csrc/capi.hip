@@ -143,6 +143,9 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value) {
     } else if (n == "chunk") {
       if (value < 0) throw HipError("chunk must be >= 0");
       ctx->c.chunk = (long long)value;
+    } else if (n == "gram_chunk_bytes") {
+      if (!(value >= 1)) throw HipError("gram_chunk_bytes must be >= 1");
+      ctx->c.gram_chunk_bytes = value;
     } else {
       throw HipError("unknown option " + n);
     }

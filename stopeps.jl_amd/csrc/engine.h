@@ -97,6 +97,7 @@ struct TRef {
   // half, max(even, odd) slots; an entry's position in that half follows from
   // its index, and its parity from the neutrality of the stored entries.
   bool fold = false;
+  long long offset = 0;  // elements from each sample's base to this view (sub_leg views)
 };
 
 TRef dense_ref(void* base, long long sstride, const std::string& letters, const std::vector<Leg>& legs);
@@ -111,6 +112,18 @@ TRef with_letters(const TRef& t, const std::string& letters);
 TRef with_ptrs(const TRef& t, void* const* ptrs);
 // Permute a dense TRef's letters without moving data (e.g. read a south tensor in reversed order).
 TRef relabel(const TRef& t, const std::string& new_letters_in_storage_order);
+// View of the index range [r0, r0 + chunk.dim) of one leg of a dense operand:
+// same strides, the leg replaced by `chunk` (whose parity split must be that of
+// the range; see chunk_leg).
+TRef sub_leg(const TRef& t, char letter, int r0, const Leg& chunk);
+// Default of Context::gram_chunk_bytes.
+constexpr double kGramChunkBytes = 1 << 30;
+
+struct Context;
+// The leg of the index range [r0, r0 + c) of a parity-sorted leg: its even
+// count is clamp(even - r0, 0, c), per sample (written into `split_out`, nb
+// ints of device memory, on ctx.stream) when the leg's split varies per sample.
+Leg chunk_leg(Context& ctx, const Leg& l, int r0, int c, int* split_out, int nb);
 
 struct Counters {
   double gemm_flops = 0;  // 2*M*N*K summed over launched (sample, parity) problems
@@ -182,6 +195,9 @@ struct Context {
   // many BF16 pieces per operand on the BF16 matrix cores (3: six piece
   // products, FP32-equivalent; 2: three products, about 2^-16 relative)
   int split = 0;
+  // Per-sample byte limit of one Gram transfer intermediate of the boundary
+  // absorb (the bra bond is taken in index ranges that keep it below this).
+  double gram_chunk_bytes = kGramChunkBytes;
   int ksub = 64;            // subspace size for truncation eigenproblems
   int max_subspace_iter = 40;
   double subspace_tol = 1e-12;

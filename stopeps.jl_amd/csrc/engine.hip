@@ -10,6 +10,7 @@
 // sequence is: right Grams by four contractions per site, then per cut rho_x,
 // its top-chi eigenvectors over both parity blocks, U_x and R_x = U_x^T Y_x.
 #include "engine.h"
+#include "boundary_ops.h"
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -156,6 +157,34 @@ TRef with_ptrs(const TRef& t, void* const* ptrs) {
 
 TRef relabel(const TRef& t, const std::string& new_letters) { return with_letters(t, new_letters); }
 
+TRef sub_leg(const TRef& t, char letter, int r0, const Leg& chunk) {
+  const size_t k = t.letters.find(letter);
+  if (k == std::string::npos) throw HipError("sub_leg: no leg " + std::string(1, letter) + " in " + t.letters);
+  if (t.compact || (t.fold && k + 1 == t.leg.size())) throw HipError("sub_leg: needs a dense (unfolded) leg");
+  if (r0 < 0 || chunk.dim < 1 || r0 + chunk.dim > t.leg[k].dim) throw HipError("sub_leg: range outside the leg");
+  TRef r = t;
+  r.offset += (long long)r0 * t.stride[k];
+  r.leg[k] = chunk;
+  return r;
+}
+
+__global__ void chunk_split_kernel(const int* split, int r0, int c, int* out, int nb) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nb) out[s] = min(max(split[s] - r0, 0), c);
+}
+
+Leg chunk_leg(Context& ctx, const Leg& l, int r0, int c, int* split_out, int nb) {
+  if (l.flat) throw HipError("chunk_leg: ungraded leg");
+  Leg r = fixed_leg(c, std::min(std::max(l.even - r0, 0), c));
+  if (l.split) {
+    hipLaunchKernelGGL(chunk_split_kernel, dim3((nb + 255) / 256), dim3(256), 0, ctx.stream, l.split, r0, c, split_out,
+                       nb);
+    SPP_CHECK(hipGetLastError());
+    r.split = split_out;
+  }
+  return r;
+}
+
 static long long group_bound(const std::vector<Leg>& legs) {
   long long p = 1;
   bool flat = true, evensplit = false;
@@ -236,6 +265,7 @@ static void fill_operand(Operand& op, const TRef& t) {
   op.sstride = t.sstride;
   op.ptrs = t.ptrs;
   op.pstride = t.pstride;
+  op.offset = t.offset;
 }
 
 static std::vector<Leg> legs_of(const TRef& t, const std::string& letters) {
@@ -366,6 +396,7 @@ template void contract<double>(Context&, const TRef&, const TRef&, const TRef&, 
 FullView full_view(const TRef& t) {
   if (t.compact) throw HipError("full_view of a compact operand");
   if (t.fold) throw HipError("full_view of a folded operand");
+  if (t.offset) throw HipError("full_view of a sub-range view");
   FullView v{};
   fill_group(v.g, t.leg, "full_view");
   for (size_t j = 0; j < t.leg.size(); ++j) v.stride[j] = t.stride[j];
@@ -1280,16 +1311,21 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
   G[L] = (T*)ctx.arena.take(sizeof(T) * nb);
   SPP_CHECK(launch_fill<T>(G[L], nb, T(1), st));
   Gl[L] = {env.bond(L), row.spec[L - 1].e, env.bond(L), row.spec[L - 1].e};
+  // The three transfer intermediates carry the bra bond R of the environment
+  // as a free leg until the last product contracts it, so they are formed for
+  // index ranges of R at a time (G[x] accumulates over the ranges): each
+  // range's intermediates stay below ctx.gram_chunk_bytes per sample.
   long long zmax = 0;
+  std::vector<int> rchunk(L + 1, 1);
   for (int x = 1; x < L; ++x) {
     const LayerSpec& sp = row.spec[x];
-    long long a = (long long)env.bdim[x] * sp.n.dim * sp.e.dim * env.bdim[x + 1] * sp.e.dim;
-    long long b = (long long)env.bdim[x] * env.bdim[x + 1] * sp.e.dim * sp.w.dim * sp.s.dim;
-    long long c = (long long)env.bdim[x] * env.bdim[x + 1] * sp.w.dim * sp.w.dim * sp.n.dim;
-    zmax = std::max(zmax, std::max(a, std::max(b, c)));
+    const long long unit = gram_unit_elems(env.bdim[x], sp.w.dim, sp.n.dim, sp.e.dim, sp.s.dim);
+    rchunk[x] = gram_rchunk(unit, env.bdim[x + 1], sizeof(T), ctx.gram_chunk_bytes);
+    zmax = std::max(zmax, unit * rchunk[x]);
   }
   T* Z1 = zmax ? (T*)ctx.arena.take(sizeof(T) * nb * zmax) : nullptr;
   T* Z2 = zmax ? (T*)ctx.arena.take(sizeof(T) * nb * zmax) : nullptr;
+  int* rsplit = (int*)ctx.arena.take(sizeof(int) * nb);
   for (int x = L - 1; x >= 1; --x) {
     const LayerSpec& sp = row.spec[x];
     Leg bl = env.bond(x), br = env.bond(x + 1);
@@ -1299,14 +1335,20 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
     TRef A = env.ref(x, "lnr");
     TRef Gn = dense_ref(G[x + 1], (long long)Gl[x + 1][0].dim * Gl[x + 1][1].dim * Gl[x + 1][2].dim * Gl[x + 1][3].dim,
                         "reRF", Gl[x + 1]);
-    TRef z1 = folded_ref(Z1, zmax, "lneRF", {bl, sp.n, sp.e, br, sp.e});
-    contract<T>(ctx, z1, A, Gn, nb);
-    TRef z2 = folded_ref(Z2, zmax, "lRFwp", {bl, br, sp.e, sp.w, sp.s});
-    contract<T>(ctx, z2, z1, row.ref(x, "wnep"), nb);
-    TRef z3 = folded_ref(Z1, zmax, "lRwWN", {bl, br, sp.w, sp.w, sp.n});
-    contract<T>(ctx, z3, z2, row.ref(x, "WNFp"), nb);
     TRef g = dense_ref(G[x], gs, "lwLW", Gl[x]);
-    contract<T>(ctx, g, z3, env.ref(x, "LNR"), nb);
+    for (int r0 = 0; r0 < br.dim; r0 += rchunk[x]) {
+      const int c = std::min(rchunk[x], br.dim - r0);
+      const bool whole = c == br.dim;
+      const Leg rc = whole ? br : chunk_leg(ctx, br, r0, c, rsplit, nb);
+      TRef z1 = folded_ref(Z1, zmax, "lneRF", {bl, sp.n, sp.e, rc, sp.e});
+      contract<T>(ctx, z1, A, whole ? Gn : sub_leg(Gn, 'R', r0, rc), nb);
+      TRef z2 = folded_ref(Z2, zmax, "lRFwp", {bl, rc, sp.e, sp.w, sp.s});
+      contract<T>(ctx, z2, z1, row.ref(x, "wnep"), nb);
+      TRef z3 = folded_ref(Z1, zmax, "lRwWN", {bl, rc, sp.w, sp.w, sp.n});
+      contract<T>(ctx, z3, z2, row.ref(x, "WNFp"), nb);
+      TRef Ar = env.ref(x, "LNR");
+      contract<T>(ctx, g, z3, whole ? Ar : sub_leg(Ar, 'R', r0, rc), nb, 1.0, r0 > 0);
+    }
   }
   // ---------------- left-to-right truncation sweep --------------------------
   out.L = L;

@@ -38,6 +38,7 @@ struct Operand {
   long long sstride;             // elements between samples (0 = shared)
   void* const* ptrs;             // per-sample base pointers (overrides base/sstride) or nullptr
   long long pstride;             // elements between the two parity blocks (compact operands)
+  long long offset;              // elements added to every (sample, parity) base (sub-range views)
   OperandGroup g[2];             // A: (M,K)  B: (K,N)  C: (M,N)
 };
 

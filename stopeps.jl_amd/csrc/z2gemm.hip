@@ -135,7 +135,7 @@ struct WaveAcc<double, TA, TB, BK> {
 template <typename T>
 __device__ inline T* operand_base(const Operand& op, int s, int q) {
   T* p = op.ptrs ? reinterpret_cast<T*>(op.ptrs[s]) : reinterpret_cast<T*>(op.base) + (long long)s * op.sstride;
-  return p + (long long)q * op.pstride;
+  return p + (long long)q * op.pstride + op.offset;
 }
 
 // Raw buffer loads: offsets are bytes from the (sample, parity) base; an

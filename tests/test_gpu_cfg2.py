@@ -68,9 +68,22 @@ class _Ref:
         return self.c
 
 
-def test_cfg2_fp64_draws_and_energies_match_oracle(spp, gpu_ctx, fixture):
+@pytest.mark.parametrize("gram_bytes", [None, 3.7e7])
+def test_cfg2_fp64_draws_and_energies_match_oracle(spp, gpu_ctx, fixture, gram_bytes):
+    """gram_bytes 3.7e7: the Gram transfers of every boundary absorb take the
+    chi=32 bra bond in index ranges of 3 (32 x 36^3 FP64 entries per index,
+    11.9 MB; the last range holds 2), which must not change the results."""
     pepo, ref = fixture
-    say("8x8 TFIM h=2 beta0=0.5 D=6 chi=32: FP64 draws of samples 1..8 against the oracle")
+    say(f"8x8 TFIM h=2 beta0=0.5 D=6 chi=32: FP64 draws of samples 1..8 against the oracle (gram {gram_bytes})")
+    if gram_bytes:
+        gpu_ctx.set_option("gram_chunk_bytes", gram_bytes)
+    try:
+        _fp64_checks(spp, gpu_ctx, pepo, ref)
+    finally:
+        gpu_ctx.set_option("gram_chunk_bytes", 2.0 ** 30)
+
+
+def _fp64_checks(spp, gpu_ctx, pepo, ref):
     cache, uni, got = _draws(spp, gpu_ctx, pepo, ref, "fp64")
     np.testing.assert_allclose(cache.south_max_truncation, ref["south_max_truncation"], rtol=1e-6, atol=1e-12)
     for k in range(len(ref["sample_index"])):
