@@ -32,7 +32,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("Born samples/s, 8×8 TFIM D=6 χ=32: batched MFMA vs plain FP64, "
           "ΔE/site≤1e-6")
-PEAK = {"fp32": 157.3, "fp64": 78.6}  # MI355X dense matrix TFLOP/s
+# MI355X dense matrix TFLOP/s; the BF16 split modes: 2500 (BF16) over their
+# piece products per FP32 product
+PEAK = {"fp32": 157.3, "fp64": 78.6, "bf16x6": 2500.0 / 6, "bf16x3": 2500.0 / 3}
+KERNEL = {"fp32": "z2gemm_kernel<float>", "fp64": "z2gemm_kernel<double>", "bf16x6": "z2gemm_split_kernel<..., 3>",
+          "bf16x3": "z2gemm_split_kernel<..., 2>"}
 PEPO_FIXTURE = os.path.join(ROOT, "tests", "golden", "cfg2_pepo_beta05.npz")
 ORACLE_FIXTURE = os.path.join(ROOT, "tests", "golden", "cfg2_oracle.npz")
 
@@ -259,13 +263,60 @@ def fu_timing(spp, batch=16, chi=32, J=1.0, h=2.0, delta=0.05):
             "contraction_tflops_over_step": cnt["gemm_flops"] / dt / 1e12}
 
 
+def precision_sweep(spp, flops, L=6, chi=64, batch=8, D=(4, 4)):
+    """Born draws at the bond and boundary dimensions of the 12x12 workload
+    (D=8 as Z2Space(0=>4, 1=>4), chi=64) on a reduced LxL lattice, in every
+    contraction precision: FP64, FP32 on the FP32 matrix cores, and FP32 from
+    six or three BF16 piece products. The same uniforms (seed 1, indices
+    1..batch) in each; accuracy is stated against the FP64 draws. Input: the
+    synthetic thermal PEPO (beta=0 identity plus a seeded 0.3 perturbation,
+    unit-norm sites; no purified beta0=0.4 PEPO at these sizes is committed)."""
+    pepo = spp.synthetic_thermal_pepo(spp.Lattice(L, L), D=D, eps=0.3, seed=4)
+    uni = np.stack([spp.sample_uniforms(1, i, L * L) for i in range(1, batch + 1)])
+    out = {"workload": f"batched Born draws, {L}x{L} synthetic Z2 PEPO, D={sum(D)} ({D[0]}+{D[1]}), d=2, chi={chi}, "
+                       f"{batch} samples per step (reduced lattice of the 12x12 D=8 chi=64 workload)",
+           "algorithmic_flops_per_draw": flops.draw_flops(pepo, chi), "samples": batch}
+    ctx = spp.Context(0, "fp64")
+    ref = None
+    try:
+        for prec in ("fp64", "fp32", "bf16x6", "bf16x3"):
+            ctx.set_precision(prec)
+            t0 = time.perf_counter()
+            cache = spp.SamplingCache(ctx, pepo, chi)
+            ctx.synchronize()
+            tc = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            d = spp.sample_configurations(cache, uniforms=uni)
+            ctx.synchronize()
+            dt = time.perf_counter() - t0
+            row = {"value": batch / dt, "unit": "Born samples/s", "cache_build_s": tc}
+            if ref is None:
+                ref = d
+            else:
+                same = [np.array_equal(d["configuration"][k], ref["configuration"][k]) for k in range(batch)]
+                dl = [abs(d["log_probability"][k] - ref["log_probability"][k]) for k in range(batch) if same[k]]
+                row.update(identical_configurations=int(sum(same)),
+                           max_abs_dlog_probability_vs_fp64=float(max(dl)) if dl else None,
+                           max_rel_dtensor_log_norm_vs_fp64=float(np.max(
+                               np.abs(d["tensor_log_norm"] - ref["tensor_log_norm"])
+                               / np.maximum(1.0, np.abs(ref["tensor_log_norm"])))))
+            out[prec] = row
+            log(f"sweep {prec}: {row['value']:.3f} samples/s")
+            del cache, d
+        for prec in ("bf16x6", "bf16x3"):
+            out[prec]["gain_over_fp32"] = out[prec]["value"] / out["fp32"]["value"]
+    finally:
+        ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=128, help="Born samples per step")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "bf16x6", "bf16x3"])
     ap.add_argument("--chi", type=int, default=32)
     ap.add_argument("--plain-draws", type=int, default=3, help="batch-1 FP64 draws for the plain path")
     ap.add_argument("--check-samples", type=int, default=8,
@@ -280,6 +331,9 @@ def main():
                     help="time only the log-norm gradient workload (for rocprofv3)")
     ap.add_argument("--fu-only", action="store_true", help="time only the full-update workload")
     ap.add_argument("--fu-batch", type=int, default=16, help="states per full-update step (0: skip)")
+    ap.add_argument("--sweep-lattice", type=int, default=6,
+                    help="lattice edge of the D=8 chi=64 precision sweep (12: the full lattice; 0: skip)")
+    ap.add_argument("--sweep-batch", type=int, default=8, help="samples per precision in the sweep")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed loop, copy 256 MiB device to device (HBM counter calibration)")
     args = ap.parse_args()
@@ -336,12 +390,12 @@ def main():
     value = ndraws / elapsed
     log(f"timed {args.steps} x {B} draws in {elapsed:.2f} s -> {value:.3f} samples/s")
     traffic, traffic_src = measured_traffic(args.precision, B, L, args.chi)
-    kc = cnt[args.precision]  # launches of the contraction kernel of this precision
+    kc = cnt["fp64" if args.precision == "fp64" else "fp32"]  # launches of this precision's contraction kernel
     avg_ms = kc["ms"] / max(kc["launches"], 1)
     draws_per_launch = ndraws / max(kc["launches"], 1)
     achieved = w_draw * draws_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
     executed_tf = kc["flops"] / (kc["ms"] * 1e-3) / 1e12 if kc["ms"] > 0 else 0.0
-    peak = PEAK[args.precision]
+    peak = PEAK[args.precision]  # split modes: the BF16 peak over the piece products per FP32 product
     result = {
         "metric": METRIC, "value": value, "unit": "Born samples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
@@ -351,7 +405,7 @@ def main():
                    "samples_per_step": B, "cache_build_s": t_cache,
                    "input": "tests/golden/cfg2_pepo_beta05.npz: the beta0=0.5 PEPO of full-update purification "
                             "(5 steps of dtau=0.05, FP64), frozen; uniforms of sample indices 1.. of seed 1"},
-        "roofline": {"bound": "mfma", "kernel": f"z2gemm_kernel<{'float' if args.precision == 'fp32' else 'double'}>",
+        "roofline": {"bound": "mfma", "kernel": KERNEL[args.precision],
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic,
                      "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
@@ -370,6 +424,23 @@ def main():
         print(json.dumps(result), flush=True)
         return
 
+    # the headline batch with the products on the BF16 matrix cores (the
+    # context's workspace is already sized, so one timed step each)
+    if args.precision == "fp32":
+        split = {}
+        for prec in ("bf16x6", "bf16x3"):
+            ctx.set_precision(prec)
+            scache = spp.SamplingCache(ctx, pepo, args.chi)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            spp.sample_configurations(scache, uniforms=uniforms(1, B))
+            ctx.synchronize()
+            v = B / (time.perf_counter() - t0)
+            split[prec] = {"value": v, "unit": "Born samples/s", "gain_over_fp32": v / value}
+            log(f"{prec}: {v:.3f} samples/s")
+            del scache
+        ctx.set_precision(args.precision)
+        result["split_modes"] = split
     # batched FP64: the same batch, FP64 everywhere
     pctx = spp.Context(0, "fp64")
     pcache = spp.SamplingCache(pctx, pepo, args.chi)
@@ -409,6 +480,9 @@ def main():
     ctx.close()
     pctx.close()
     gc.collect()
+    if args.sweep_lattice > 0:
+        log("precision sweep ...")
+        result["precision_sweep"] = precision_sweep(spp, flops, L=args.sweep_lattice, batch=args.sweep_batch)
     result["adjoint"] = adjoint_timing(spp)
     if args.fu_batch > 0:
         log("full-update step ...")
