@@ -358,9 +358,10 @@ __global__ __launch_bounds__(kThreads, MINW) void z2gemm_kernel(GemmArgs args) {
 // BF16 split products (the "bf16x6" / "bf16x3" precision modes).
 //
 // FP32 operands are split when staged into LDS: hi = bf16(x), mid =
-// bf16(x - hi), lo = bf16(x - hi - mid) (round to nearest even), so that
-// x = hi + mid + lo to 2^-24 relative. The product sum_k a_k b_k is formed on
-// the BF16 matrix cores (v_mfma_f32_32x32x16_bf16, FP32 accumulation) from the
+// bf16(x - hi), lo = bf16(x - hi - mid) (round to nearest even, two values per
+// conversion), so that x = hi + mid + lo to 2^-24 relative (finite operands
+// below the BF16 overflow threshold, about 3.39e38). The product sum_k a_k b_k
+// is formed on the BF16 matrix cores (v_mfma_f32_32x32x16_bf16, FP32 accumulation) from the
 // piece products whose orders add up to at most NP - 1: NP = 3 pieces take
 // hh, hm, mh, hl, lh, mm (six products; the omitted ones are below 2^-24
 // relative, so the result matches an FP32 product with FP32 accumulation),
@@ -368,15 +369,34 @@ __global__ __launch_bounds__(kThreads, MINW) void z2gemm_kernel(GemmArgs args) {
 // The LDS tiles are row-major with the K index contiguous, one array per piece,
 // so each MFMA operand (eight consecutive K values of one row) is one 16-byte read.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
-__device__ inline unsigned short bf16_bits(float x) {  // round to nearest even
-  const unsigned u = __float_as_uint(x);
-  return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+// Two FP32 values -> two BF16 (round to nearest even, v_cvt_pk_bf16_f32),
+// element 0 in the low half.
+__device__ inline unsigned bf16_pair(float a, float b) {
+  return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a, b}, bf16x2));
 }
-__device__ inline float bf16_value(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
 constexpr int SBK = 16;       // K tile of the split kernel: one MFMA step
 constexpr int SLD = SBK + 8;  // LDS row stride in BF16 elements (48 bytes)
+
+// Thread -> (row, k pair) assignment of an R x 16 operand tile: 8 R pairs of
+// consecutive k, R / 32 per thread. Row-fast: consecutive lanes along rows;
+// K-fast: the 8 pairs of a row on consecutive lanes.
+template <int R, int ROWFAST>
+struct PairMap {
+  static constexpr int N = R / 32;
+  static_assert(N * 32 == R, "pair map rows");
+  __device__ static int row(int tid, int j) {
+    const int t = tid + kThreads * j;
+    return ROWFAST ? t % R : t >> 3;
+  }
+  __device__ static int pair(int tid, int j) {
+    const int t = tid + kThreads * j;
+    return ROWFAST ? t / R : t & 7;
+  }
+};
 
 template <int BM, int BN, int WGM, int MINW, int AMF, int BKF, int NP>
 __global__ __launch_bounds__(kThreads, MINW) void z2gemm_split_kernel(GemmArgs args) {
@@ -385,8 +405,8 @@ __global__ __launch_bounds__(kThreads, MINW) void z2gemm_split_kernel(GemmArgs a
   constexpr int WTM = BM / WGM, WTN = BN / WGN;
   constexpr int TA = WTM / 32, TB = WTN / 32;
   static_assert(TA * 32 == WTM && TB * 32 == WTN, "wave tile must be a multiple of 32");
-  using MA = TileMap<BM, BK, AMF>;
-  using MB = TileMap<BN, BK, 1 - BKF>;
+  using MA = PairMap<BM, AMF>;
+  using MB = PairMap<BN, 1 - BKF>;
   __shared__ __align__(16) unsigned short As[NP][BM * SLD];
   __shared__ __align__(16) unsigned short Bs[NP][BN * SLD];
   __shared__ unsigned offAm[BM], offBn[BN];
@@ -461,38 +481,43 @@ __global__ __launch_bounds__(kThreads, MINW) void z2gemm_split_kernel(GemmArgs a
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
 
-  float ra[MA::N], rb[MB::N];
+  float ra[MA::N][2], rb[MB::N][2];
   auto load_tile = [&](int kt) {
 #pragma unroll
-    for (int j = 0; j < MA::N; ++j)
-      ra[j] = buffer_load(ra_rsrc, offAm[MA::row(tid, j)] + offAk[kt + MA::k(tid, j)], 0.f);
-#pragma unroll
-    for (int j = 0; j < MB::N; ++j)
-      rb[j] = buffer_load(rb_rsrc, offBn[MB::row(tid, j)] + offBk[kt + MB::k(tid, j)], 0.f);
-  };
-  auto store_tile = [&]() {
-#pragma unroll
     for (int j = 0; j < MA::N; ++j) {
-      const int o = MA::row(tid, j) * SLD + MA::k(tid, j);
-      float x = ra[j];
-#pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        const unsigned short h = bf16_bits(x);
-        As[p][o] = h;
-        x -= bf16_value(h);
-      }
+      const unsigned o = offAm[MA::row(tid, j)];
+      const int k = kt + 2 * MA::pair(tid, j);
+      ra[j][0] = buffer_load(ra_rsrc, o + offAk[k], 0.f);
+      ra[j][1] = buffer_load(ra_rsrc, o + offAk[k + 1], 0.f);
     }
 #pragma unroll
     for (int j = 0; j < MB::N; ++j) {
-      const int o = MB::row(tid, j) * SLD + MB::k(tid, j);
-      float x = rb[j];
+      const unsigned o = offBn[MB::row(tid, j)];
+      const int k = kt + 2 * MB::pair(tid, j);
+      rb[j][0] = buffer_load(rb_rsrc, o + offBk[k], 0.f);
+      rb[j][1] = buffer_load(rb_rsrc, o + offBk[k + 1], 0.f);
+    }
+  };
+  // pieces of a pair of consecutive k values: one 32-bit LDS word per piece;
+  // each remainder x - piece is exact in FP32
+  auto split_pair = [&](unsigned short* dst, int stride, float x0, float x1) {
 #pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        const unsigned short h = bf16_bits(x);
-        Bs[p][o] = h;
-        x -= bf16_value(h);
+    for (int p = 0; p < NP; ++p) {
+      const unsigned w = bf16_pair(x0, x1);
+      *reinterpret_cast<unsigned*>(dst + p * stride) = w;
+      if (p + 1 < NP) {
+        x0 -= __uint_as_float(w << 16);
+        x1 -= __uint_as_float(w & 0xFFFF0000u);
       }
     }
+  };
+  auto store_tile = [&]() {
+#pragma unroll
+    for (int j = 0; j < MA::N; ++j)
+      split_pair(&As[0][MA::row(tid, j) * SLD + 2 * MA::pair(tid, j)], BM * SLD, ra[j][0], ra[j][1]);
+#pragma unroll
+    for (int j = 0; j < MB::N; ++j)
+      split_pair(&Bs[0][MB::row(tid, j) * SLD + 2 * MB::pair(tid, j)], BN * SLD, rb[j][0], rb[j][1]);
   };
 
   for (int kc = 0; kc < cK; kc += KLDS) {
