@@ -426,6 +426,7 @@ def main():
 
     # the headline batch with the products on the BF16 matrix cores (the
     # context's workspace is already sized, so one timed step each)
+    split_caches = {}
     if args.precision == "fp32":
         split = {}
         for prec in ("bf16x6", "bf16x3"):
@@ -436,9 +437,10 @@ def main():
             spp.sample_configurations(scache, uniforms=uniforms(1, B))
             ctx.synchronize()
             v = B / (time.perf_counter() - t0)
-            split[prec] = {"value": v, "unit": "Born samples/s", "gain_over_fp32": v / value}
+            split[prec] = {"value": v, "unit": "Born samples/s", "gain_over_fp32": v / value,
+                           "accuracy": "precision_check." + prec}
             log(f"{prec}: {v:.3f} samples/s")
-            del scache
+            split_caches[prec] = scache
         ctx.set_precision(args.precision)
         result["split_modes"] = split
     # batched FP64: the same batch, FP64 everywhere
@@ -463,7 +465,8 @@ def main():
     result["batched_fp64"] = {"value": batched64, "unit": "Born samples/s", "batch": B,
                               "batching_gain": batched64 / plain, "precision_gain": value / batched64}
     model = spp.TFIM(args.J, args.h, symmetric=True)
-    result["precision_check"] = oracle_check(spp, ctx, cache, pcache, model, args)
+    result["precision_check"] = oracle_check(spp, [(args.precision, cache), *split_caches.items(), ("fp64", pcache)],
+                                             model, args)
     # energy throughput of the headline's drawn samples
     k = min(args.check_samples, B)
     cfg = draws[0]["configuration"][:k]
@@ -476,7 +479,7 @@ def main():
                                     "dtype": args.precision, "samples": int(k), "chi": args.chi}
     # release the draw workspaces (the batched FP64 one holds ~0.1 TB) so the
     # later workloads size their chunks from the whole device
-    del cache, pcache, draws
+    del cache, pcache, draws, split_caches
     ctx.close()
     pctx.close()
     gc.collect()
@@ -496,22 +499,22 @@ def main():
     print(json.dumps(result), flush=True)
 
 
-def oracle_check(spp, ctx, cache, pcache, model, args):
-    """The headline's precision against the FP64 oracle's committed vectors on
-    the same PEPO (tests/golden/cfg2_oracle.npz): draws of the fixture's sample
-    indices in the headline precision and in FP64 (configurations, log p),
-    and the energy per site of the oracle's measured samples in both
-    precisions, per sample and as the weighted ensemble estimate
-    exp(log_norm - log q) (src/driver/particles.jl:43-52, ensemble.jl:17)."""
+def oracle_check(spp, caches, model, args):
+    """Each precision's caches (name, cache) against the FP64 oracle's committed
+    vectors on the same PEPO (tests/golden/cfg2_oracle.npz): draws of the
+    fixture's sample indices (configurations, log p), and the energy per site
+    of the oracle's measured samples, per sample and as the weighted ensemble
+    estimate exp(log_norm - log q) (src/driver/particles.jl:43-52,
+    ensemble.jl:17)."""
     try:
         ref = np.load(ORACLE_FIXTURE, allow_pickle=False)
     except OSError:
         return {"error": "oracle vectors not found"}
-    ns = cache.lattice.Lx * cache.lattice.Ly
+    ns = caches[0][1].lattice.Lx * caches[0][1].lattice.Ly
     idx = [int(i) for i in ref["sample_index"]]
     uni = np.stack([spp.sample_uniforms(int(ref["seed"]), i, ns) for i in idx])
     out = {"reference": "FP64 oracle, tests/golden/cfg2_oracle.npz", "samples": len(idx)}
-    for name, c in ((args.precision, cache), ("fp64", pcache)):
+    for name, c in caches:
         d = spp.sample_configurations(c, uniforms=uni)
         same = [np.array_equal(d["configuration"][k], ref["configuration"][k]) for k in range(len(idx))]
         dl = [abs(d["log_probability"][k] - ref["log_probability"][k]) for k in range(len(idx)) if same[k]]
@@ -519,7 +522,7 @@ def oracle_check(spp, ctx, cache, pcache, model, args):
                      "max_abs_dlog_probability": float(max(dl)) if dl else None}
     m = np.isfinite(ref["energy_per_site"])
     cfg = ref["configuration"][m].astype(np.int64)
-    for name, c in ((args.precision, cache), ("fp64", pcache)):
+    for name, c in caches:
         e = spp.measure_energies(c, cfg, model)
         lw = e["log_norm"] - ref["log_probability"][m]
         w = np.exp(lw - np.max(lw))
@@ -529,6 +532,7 @@ def oracle_check(spp, ctx, cache, pcache, model, args):
         Er = float(np.sum(wr * ref["energy_per_site"][m]) / np.sum(wr))
         out[name]["max_abs_dE_per_site"] = float(np.max(np.abs(e["energy_per_site"] - ref["energy_per_site"][m])))
         out[name]["ensemble_dE_per_site"] = abs(E - Er)
+        out[name]["meets_dE_per_site_1e-6"] = bool(abs(E - Er) <= 1e-6)
         out[name]["ensemble_E_per_site"] = E
     out["max_environment_truncation"] = float(np.max(ref["max_environment_truncation"]))
     out["model"] = f"TFIM J={args.J} h={args.h} (Z2)"
