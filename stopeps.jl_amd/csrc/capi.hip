@@ -27,6 +27,7 @@ struct spp_state {
 
 struct spp_cache {
   int precision = 0;
+  int split = 0;  // BF16 piece count of the FP32 products the cache was built with (0: FP32 matrix cores)
   std::unique_ptr<CacheDev<float>> f;
   std::unique_ptr<CacheDev<double>> d;
 };
@@ -37,6 +38,17 @@ static void usable(const StateDev& s) {
   if (s.failed)
     throw HipError(std::string("state invalidated by a failed full-update step: ") + fu_failure_message(s.failed));
 }
+
+// Calls through a cache compute their products the way the cache was built
+// (the context's precision setting takes effect for caches built afterwards).
+struct CacheProducts {
+  Context& c;
+  int saved;
+  CacheProducts(Context& ctx, const spp_cache& cache) : c(ctx), saved(ctx.split) { ctx.split = cache.split; }
+  ~CacheProducts() { c.split = saved; }
+  CacheProducts(const CacheProducts&) = delete;
+  CacheProducts& operator=(const CacheProducts&) = delete;
+};
 
 template <typename F>
 static int guarded(spp_ctx* ctx, F&& f) {
@@ -323,6 +335,7 @@ int spp_sampling_cache_build(spp_ctx* ctx, const spp_state* pepo, int chi, const
     usable(pepo->s);
     SPP_CHECK(hipSetDevice(ctx->c.device));
     c->precision = ctx->c.precision;
+    c->split = c->precision == 1 ? ctx->c.split : 0;
     if (c->precision == 1) {
       c->f.reset(new CacheDev<float>());
       build_cache<float>(ctx->c, pepo->s, chi, basis, *c->f);
@@ -370,6 +383,7 @@ int spp_born_sample_batch(spp_ctx* ctx, spp_cache* cache, int n, const double* u
     o.log_amplitude = log_amplitude;
     o.status = status;
     o.fail_site = fail_site;
+    CacheProducts products(ctx->c, *cache);
     if (cache->precision == 1) draw_batch<float>(ctx->c, *cache->f, n, uniforms, labels_in, o);
     else draw_batch<double>(ctx->c, *cache->d, n, uniforms, labels_in, o);
   });
@@ -404,6 +418,7 @@ int spp_measure_energy_batch(spp_ctx* ctx, const spp_cache* cache, const spp_sta
     o.fail_site = fail_site;
     o.entropy_profile = boundary_entropy;
     o.entropy_max = max_boundary_entropy;
+    CacheProducts products(ctx->c, *cache);
     if (cache->precision == 1) measure_energy_batch<float>(ctx->c, pepo->s, *cache->f, chi, n, labels, ops, o);
     else measure_energy_batch<double>(ctx->c, pepo->s, *cache->d, chi, n, labels, ops, o);
   });

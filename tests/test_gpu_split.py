@@ -85,3 +85,28 @@ def test_split_z2_gemm(spp, gpu_ctx, precision, layout):
     assert np.all(C[eM:, eN:] == 0.0)
     bound = _bound(A, B, precision)
     assert np.all(np.abs(C - A @ B)[neutral] <= bound[neutral])
+
+
+def test_cache_keeps_its_product_mode(spp):
+    """A sampling cache computes its draws with the products it was built with,
+    whatever the context is set to at draw time: a bf16x3 cache drawn while the
+    context is back on FP32 gives the draws of the same cache drawn in bf16x3
+    (bit for bit), and its log p differs from the FP32 cache's by about the
+    three-product rounding (2^-16 grade), not by zero."""
+    pepo = spp.synthetic_thermal_pepo(spp.Lattice(4, 4), D=(2, 2), eps=0.3, seed=5)
+    uni = np.stack([spp.sample_uniforms(1, i, 16) for i in range(1, 5)])
+    ctx = spp.Context(0, "bf16x3")
+    try:
+        split = spp.SamplingCache(ctx, pepo, 6)
+        a = spp.sample_configurations(split, uniforms=uni)
+        ctx.set_precision("fp32")
+        b = spp.sample_configurations(split, uniforms=uni)
+        plain = spp.SamplingCache(ctx, pepo, 6)
+        c = spp.sample_configurations(plain, uniforms=uni)
+    finally:
+        ctx.close()
+    say("bf16x3 cache vs fp32 cache dlogp", np.max(np.abs(a["log_probability"] - c["log_probability"])))
+    assert np.array_equal(a["configuration"], b["configuration"])
+    assert np.array_equal(a["log_probability"], b["log_probability"])
+    assert np.array_equal(a["conditionals"], b["conditionals"])
+    assert np.max(np.abs(a["log_probability"] - c["log_probability"])) > 0
