@@ -437,3 +437,119 @@ def test_lognorm_gradient_matches_finite_differences(kind, shape, D, sym):
             fd = (vals[0] - vals[1]) / (2 * h)
             g = grads[(x, y)].a[idx]
             assert abs(fd - g) <= 1e-6 * max(1.0, abs(g)), ((x, y), idx, fd, g)
+
+
+# ----------------------- physical-kind sampling (born.jl) -------------------
+
+def charged_peps(L, D, seed, aux_parities, symmetric=True):
+    """Random PEPS whose site i carries auxiliary charge aux_parities[i]
+    (row-major): a projected particle's leg structure."""
+    rng = np.random.default_rng(seed)
+    if symmetric:
+        bp = np.array([0] * D[0] + [1] * D[1])
+        pp = np.array([0, 1])
+    else:
+        bp = np.zeros(D, dtype=np.int64)
+        pp = np.zeros(2, dtype=np.int64)
+    u = np.array([0])
+    sites = []
+    for x in range(L.Lx):
+        col = []
+        for y in range(L.Ly):
+            ap = np.array([aux_parities[x + y * L.Lx] if symmetric else 0])
+            legs = [u if x == 0 else bp, u if y == 0 else bp, pp, ap,
+                    u if x == L.Lx - 1 else bp, u if y == L.Ly - 1 else bp]
+            t = symmetrize(Z2(rng.standard_normal([len(p) for p in legs]), legs))
+            col.append(t.scale(1.0 / t.norm()))
+        sites.append(col)
+    return State(L, sites, "peps")
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_physical_born_probabilities_brute_force(symmetric):
+    """Physical kind at chi >= exact bond: p(sigma) = |<sigma|Psi>|^2 / Z over
+    every physical configuration in the sampling basis (the exact site metric
+    is PSD, so the projection only clips round-off), the conditionals multiply
+    to p, sum p = 1, and configurations outside the state's total Z2 charge get
+    log p = -Inf from the charge check (born.jl:100-104, 114-115)."""
+    L = Lattice(2, 3)
+    aux = [1, 0, 1, 1, 0, 0]
+    st = charged_peps(L, (1, 1) if symmetric else 2, 11, aux, symmetric)
+    basis = ob.resolve_basis(symmetric)
+    cache = ob.SamplingCache(st, 64, basis=basis, kind="physical")
+    amp = dense_network(st)
+    idx_p = []
+    for y in range(3):
+        for x in range(2):
+            idx_p += [slice(None), 0]
+    amp = amp[tuple(idx_p)]  # physical legs only (unit aux legs)
+    # rotate every physical leg to the sampling basis: <u_k| psi>
+    for k in range(6):
+        amp = np.moveaxis(np.tensordot(basis.U.conj().T, amp, axes=([1], [k])), 0, k)
+    Z = np.sum(np.abs(amp) ** 2)
+    target = 0
+    for q in aux:
+        target ^= q
+    assert cache.target_charge == (target if symmetric else None)
+    total = 0.0
+    for ks in itertools.product([0, 1], repeat=6):
+        cfg = [[basis.labels[ks[x + 2 * y]] for y in range(3)] for x in range(2)]
+        r = ob.sample_configuration(cache, configuration=cfg)
+        exact = np.abs(amp[tuple(ks[x + 2 * y] for y in range(3) for x in range(2))]) ** 2 / Z
+        charge = 0
+        for k in ks:
+            charge ^= basis.charges[k]
+        if symmetric and charge != target:
+            assert r["log_probability"] == -np.inf and r["conditionals"] == []
+            assert exact == 0.0
+            continue
+        assert abs(np.exp(r["log_probability"]) - exact) < 1e-12
+        assert abs(np.sum(np.log(r["conditionals"])) - r["log_probability"]) < 1e-12
+        assert len([d for d in r["repairs"] if "hermiticity_defect" in d]) == 6
+        total += np.exp(r["log_probability"])
+    assert abs(total - 1.0) < 1e-12
+    # draws land on charge-allowed configurations and evaluate to their own log p
+    for i in range(1, 6):
+        d = ob.sample_configuration(cache, uniforms=ob.counter_uniforms(4, i, 6))
+        ev = ob.sample_configuration(cache, configuration=d["configuration"])
+        assert abs(ev["log_probability"] - d["log_probability"]) < 1e-13
+
+
+def test_positive_metric_closed_form():
+    """``_positive_metric`` (tensors.jl:65-94) on a Z2 matrix with a known
+    eigendecomposition per parity block: the repaired matrix is V max(D, 0) V^T
+    of the scaled input, the root satisfies root^T root = repaired, and the
+    report has its closed-form values (eigenvalues of M / ||M||)."""
+    rng = np.random.default_rng(3)
+    par = np.array([0, 0, 1, 1, 1])  # one leg, rows and columns
+    lam = {0: np.array([2.0, -0.5]), 1: np.array([1.0, 0.25, -0.1])}
+    M = np.zeros((5, 5))
+    Vs = {}
+    for q in (0, 1):
+        idx = np.nonzero(par == q)[0]
+        Q, _ = np.linalg.qr(rng.standard_normal((len(idx), len(idx))))
+        Vs[q] = (idx, Q)
+        M[np.ix_(idx, idx)] = (Q * lam[q][None, :]) @ Q.T
+    N = Z2(M, [par, par])
+    s = np.linalg.norm(M)
+    rep, report = ob.positive_metric(N, 1)
+    root, report2 = ob.positive_metric(N, 1, square_root=True)
+    exp = np.zeros((5, 5))
+    for q in (0, 1):
+        idx, Q = Vs[q]
+        exp[np.ix_(idx, idx)] = (Q * np.maximum(lam[q], 0)[None, :]) @ Q.T / s
+    assert np.allclose(rep, exp, atol=1e-14)
+    assert np.allclose(root.T @ root, exp, atol=1e-14)
+    allv = np.concatenate([lam[0], lam[1]]) / s
+    assert report == report2
+    assert abs(report["minimum_relative_eigenvalue"] - (-0.5 / 2.0)) < 1e-14
+    assert report["negative_eigenvalues"] == 2.0 and report["negative_multiplets"] == 2
+    assert abs(report["negative_spectral_fraction"]
+               - np.sqrt(np.sum(np.minimum(allv, 0) ** 2)) / np.sqrt(np.sum(allv ** 2))) < 1e-14
+    assert report["hermiticity_defect"] < 1e-15
+    # an antisymmetric part K of a symmetric S: defect = ||2K|| / ||S + K||, spectrum of S
+    K = np.zeros((5, 5))
+    K[0, 1], K[1, 0] = 0.3, -0.3
+    _, r3 = ob.positive_metric(Z2(M + K, [par, par]), 1)
+    assert abs(r3["hermiticity_defect"] - np.linalg.norm(2 * K) / np.linalg.norm(M + K)) < 1e-14
+    assert abs(r3["minimum_relative_eigenvalue"] - report["minimum_relative_eigenvalue"]) < 1e-14
