@@ -198,6 +198,19 @@ int spp_fu_set_repair(spp_ctx* ctx, spp_fu_repair_fn fn, void* user);
  * `SymmetryBasis()` on Z2Space(0=>1,1=>1); XBasis for dense TFIM). */
 int spp_sampling_cache_build(spp_ctx* ctx, const spp_state* pepo, int chi, const double* basis,
                              spp_cache** out);
+/* `SamplingCache(state, BoundaryMPS(chi); kind, basis)` (src/sampling/born.jl:56-73)
+ * of either kind: kind 0 = :auxiliary (state a PEPO; as spp_sampling_cache_build),
+ * kind 1 = :physical (state a PEPS with d = 2 and a one-index auxiliary leg; the
+ * resampling proposals of src/resample/event.jl:27-33): unprojected double
+ * layers, physically projected candidates, south boundaries, and for a Z2 state
+ * (which requires the identity basis, `SymmetryBasis()`) the state's total
+ * charge `_state_charge` and reachable charges `_charge_suffix`
+ * (src/sampling/basis.jl:89-100). Draws from a physical cache weight label k by
+ * ||root v_k||^2 with root the square root of the PSD projection of the site
+ * metric `_site_hole` (`_positive_metric(...; square_root=true)`,
+ * src/states/tensors.jl:65-94; born.jl:116-134). */
+int spp_sampling_cache_build_kind(spp_ctx* ctx, const spp_state* state, int chi, const double* basis, int kind,
+                                  spp_cache** out);
 int spp_sampling_cache_free(spp_cache* cache);
 /* south_max_trunc[Ly+1]: `max_truncation` of every south boundary. */
 int spp_sampling_cache_info(const spp_cache* cache, double* south_max_trunc);
@@ -221,6 +234,25 @@ int spp_sampling_cache_info(const spp_cache* cache, double* south_max_trunc);
 int spp_born_sample_batch(spp_ctx* ctx, spp_cache* cache, int n, const double* uniforms, const int* labels_in,
                           int* labels_out, double* log_prob, double* conditionals, double* max_trunc,
                           double* tensor_log_norm, double* log_amplitude, int* status, int* fail_site);
+
+/* spp_born_sample_batch for either kind of cache, with the per-site `repairs`
+ * of the physical kind (born.jl:118-131): repairs [n][Lx*Ly][6] (may be NULL)
+ * = {minimum_relative_eigenvalue, negative_eigenvalues, negative_multiplets,
+ * negative_spectral_fraction, hermiticity_defect} of the site metric's
+ * `_positive_metric` report (tensors.jl:78-85) and, last, the number of
+ * allowed outcomes when the proposal weights were all zero and the draw fell
+ * back to the uniform distribution over them (0 otherwise); sites not reached
+ * (the sample ended earlier) hold NaN. A physical draw has no projected PEPS:
+ * tensor_log_norm and log_amplitude are NaN, and the last row is not absorbed
+ * (max_trunc covers the absorbs made, born.jl:157-160). In evaluation mode a
+ * configuration outside a Z2 state's total charge gets log_prob = -Inf with
+ * no conditionals (born.jl:100-104). Status codes of the physical kind add 4
+ * nonfinite norm metric (tensors.jl:67), 5 no label fits the total charge
+ * (born.jl:116) and 6 the metric's eigensolver reached its iteration cap. */
+int spp_born_sample_batch_repairs(spp_ctx* ctx, spp_cache* cache, int n, const double* uniforms,
+                                  const int* labels_in, int* labels_out, double* log_prob, double* conditionals,
+                                  double* max_trunc, double* tensor_log_norm, double* log_amplitude, int* status,
+                                  int* fail_site, double* repairs);
 
 /* Batched `measure_energy(state, model, BoundaryMPS(chi))`
  * (src/measurement/energy.jl:31-78) of the projected samples
@@ -376,6 +408,23 @@ int spp_selftest_gemm(spp_ctx* ctx, int precision, int M, int N, int K, int even
  * two triangles are averaged) into eigenvalues[nprob][ld] (entries past n[i]
  * are 0). */
 int spp_selftest_spectrum(spp_ctx* ctx, int nprob, int ld, const int* n, const double* blocks, double* eigenvalues);
+
+/* Self-test of the physical-kind metric eigensolver (no reference
+ * counterpart; the reference calls eigh_full, tensors.jl:71): nprob symmetric
+ * blocks[nprob][ld][ld] (leading n[i] x n[i] part, triangles averaged) and two
+ * vectors[nprob][2][ld] each; returns every eigenvalue (unordered) in
+ * eigenvalues[nprob][ld], the projections u_j^T v of each vector on the
+ * matching eigenvectors in projections[nprob][2][ld], and failed[nprob] = 1
+ * where the QL iteration reached its cap. */
+int spp_selftest_psd(spp_ctx* ctx, int nprob, int ld, const int* n, const double* blocks, const double* vectors,
+                     double* eigenvalues, double* projections, int* failed);
+
+/* Host run of the implicit QL iteration of the physical-kind metric (selftest;
+ * no GPU needed): eigenvalues of the symmetric tridiagonal (d[n], off-diagonal
+ * e[0..n-2], e[n-1] scratch) into d, and nvec vectors[nvec][n] replaced by
+ * their projections on the eigenvectors. Returns 0, 1 when an eigenvalue
+ * needed more than maxit iterations, or -3 on bad arguments. */
+int spp_selftest_ql(int n, double* d, double* e, double* vectors, int nvec, int maxit);
 
 /* Calibration copy (no reference counterpart): one launch of the library's
  * plain copy kernel moving n FP32 values between two device buffers it

@@ -26,11 +26,16 @@ _SIGNATURES = {
     "spp_state_free": (ctypes.c_int, [c_void_p]),
     "spp_sampling_cache_build": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int, c_double_p,
                                                 ctypes.POINTER(c_void_p)]),
+    "spp_sampling_cache_build_kind": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int, c_double_p, ctypes.c_int,
+                                                     ctypes.POINTER(c_void_p)]),
     "spp_sampling_cache_free": (ctypes.c_int, [c_void_p]),
     "spp_sampling_cache_info": (ctypes.c_int, [c_void_p, c_double_p]),
     "spp_born_sample_batch": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int, c_double_p, c_int_p, c_int_p,
                                              c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
                                              c_int_p, c_int_p]),
+    "spp_born_sample_batch_repairs": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int, c_double_p, c_int_p,
+                                                     c_int_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                                                     c_double_p, c_int_p, c_int_p, c_double_p]),
     "spp_measure_energy_batch": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int, ctypes.c_int, c_int_p,
                                                 c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_double,
                                                 ctypes.c_double, c_double_p, c_double_p, c_double_p, c_double_p,
@@ -63,6 +68,10 @@ _SIGNATURES = {
                                             c_int_p, c_int_p, c_double_p, c_double_p, c_double_p, c_double_p,
                                             c_double_p, c_double_p, c_int_p]),
     "spp_selftest_copy": (ctypes.c_int, [c_void_p, ctypes.c_longlong]),
+    "spp_selftest_psd": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, c_int_p, c_double_p, c_double_p,
+                                        c_double_p, c_double_p, c_int_p]),
+    "spp_selftest_ql": (ctypes.c_int, [ctypes.c_int, c_double_p, c_double_p, c_double_p, ctypes.c_int,
+                                       ctypes.c_int]),
     "spp_selftest_spectrum": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, c_int_p, c_double_p, c_double_p]),
 }
 

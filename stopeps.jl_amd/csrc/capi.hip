@@ -5,6 +5,9 @@
 #include "lognorm.h"
 #include "adjoint.h"
 #include "fu.h"
+#include "sampler.h"
+#include "psd.h"
+#include "tridiag.h"
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -324,28 +327,35 @@ int spp_state_free(spp_state* state) {
   return 0;
 }
 
-int spp_sampling_cache_build(spp_ctx* ctx, const spp_state* pepo, int chi, const double* basis,
-                             spp_cache** out) {
+int spp_sampling_cache_build_kind(spp_ctx* ctx, const spp_state* state, int chi, const double* basis, int kind,
+                                  spp_cache** out) {
   if (out) *out = nullptr;
   std::unique_ptr<spp_cache> c(new spp_cache());
   int rc = guarded(ctx, [&] {
-    if (!pepo || !out) throw HipError("null argument");
+    if (!state || !out) throw HipError("null argument");
     if (chi < 1) throw HipError("chi must be positive");
-    if (pepo->s.kind != 0) throw HipError("the sampling cache needs a PEPO");
-    usable(pepo->s);
+    if (kind != 0 && kind != 1) throw HipError("unknown sampling kind");
+    if (kind == 0 && state->s.kind != 0) throw HipError("the sampling cache needs a PEPO");
+    if (kind == 1 && state->s.kind != 1) throw HipError("proposal requires PEPS");
+    usable(state->s);
     SPP_CHECK(hipSetDevice(ctx->c.device));
     c->precision = ctx->c.precision;
     c->split = c->precision == 1 ? ctx->c.split : 0;
     if (c->precision == 1) {
       c->f.reset(new CacheDev<float>());
-      build_cache<float>(ctx->c, pepo->s, chi, basis, *c->f);
+      build_cache<float>(ctx->c, state->s, chi, basis, *c->f, kind);
     } else {
       c->d.reset(new CacheDev<double>());
-      build_cache<double>(ctx->c, pepo->s, chi, basis, *c->d);
+      build_cache<double>(ctx->c, state->s, chi, basis, *c->d, kind);
     }
   });
   if (rc == 0) *out = c.release();
   return rc;
+}
+
+int spp_sampling_cache_build(spp_ctx* ctx, const spp_state* pepo, int chi, const double* basis,
+                             spp_cache** out) {
+  return spp_sampling_cache_build_kind(ctx, pepo, chi, basis, 0, out);
 }
 
 int spp_sampling_cache_free(spp_cache* cache) {
@@ -363,6 +373,14 @@ int spp_sampling_cache_info(const spp_cache* cache, double* south_max_trunc) {
 int spp_born_sample_batch(spp_ctx* ctx, spp_cache* cache, int n, const double* uniforms, const int* labels_in,
                           int* labels_out, double* log_prob, double* conditionals, double* max_trunc,
                           double* tensor_log_norm, double* log_amplitude, int* status, int* fail_site) {
+  return spp_born_sample_batch_repairs(ctx, cache, n, uniforms, labels_in, labels_out, log_prob, conditionals,
+                                       max_trunc, tensor_log_norm, log_amplitude, status, fail_site, nullptr);
+}
+
+int spp_born_sample_batch_repairs(spp_ctx* ctx, spp_cache* cache, int n, const double* uniforms,
+                                  const int* labels_in, int* labels_out, double* log_prob, double* conditionals,
+                                  double* max_trunc, double* tensor_log_norm, double* log_amplitude, int* status,
+                                  int* fail_site, double* repairs) {
   return guarded(ctx, [&] {
     if (!cache) throw HipError("null cache");
     if (n < 0) throw HipError("negative sample count");
@@ -383,6 +401,7 @@ int spp_born_sample_batch(spp_ctx* ctx, spp_cache* cache, int n, const double* u
     o.log_amplitude = log_amplitude;
     o.status = status;
     o.fail_site = fail_site;
+    o.repairs = repairs;
     CacheProducts products(ctx->c, *cache);
     if (cache->precision == 1) draw_batch<float>(ctx->c, *cache->f, n, uniforms, labels_in, o);
     else draw_batch<double>(ctx->c, *cache->d, n, uniforms, labels_in, o);
@@ -655,6 +674,23 @@ int spp_selftest_spectrum(spp_ctx* ctx, int nprob, int ld, const int* n, const d
     SPP_CHECK(hipSetDevice(ctx->c.device));
     spectrum_selftest(ctx->c, nprob, ld, n, blocks, eigenvalues);
   });
+}
+
+int spp_selftest_psd(spp_ctx* ctx, int nprob, int ld, const int* n, const double* blocks, const double* vectors,
+                     double* eigenvalues, double* projections, int* failed) {
+  return guarded(ctx, [&] {
+    if (nprob < 0 || ld < 1) throw HipError("bad sizes");
+    if (nprob > 0 && (!n || !blocks || !vectors || !eigenvalues || !projections || !failed))
+      throw HipError("null argument");
+    if ((size_t)(kTriWaves + 6 * (size_t)ld) * sizeof(double) > 160 * 1024) throw HipError("block too large");
+    SPP_CHECK(hipSetDevice(ctx->c.device));
+    psd_selftest(ctx->c, nprob, ld, n, blocks, vectors, eigenvalues, projections, failed);
+  });
+}
+
+int spp_selftest_ql(int n, double* d, double* e, double* vectors, int nvec, int maxit) {
+  if (n < 0 || nvec < 0 || (n > 0 && (!d || !e || (nvec > 0 && !vectors)))) return -3;
+  return ql_project(n, d, e, vectors, n, nvec, maxit);
 }
 
 int spp_selftest_copy(spp_ctx* ctx, long long n) {

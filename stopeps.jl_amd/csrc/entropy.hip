@@ -17,77 +17,15 @@
 // the trailing block once) followed by Sturm-count bisection for every
 // eigenvalue.
 #include "engine.h"
+#include "tridiag.h"
 #include <cfloat>
 
 namespace spp {
 
 namespace {
 
-constexpr int kSpecThreads = 512;
-constexpr int kSpecWaves = kSpecThreads / 64;
-
-__device__ inline double wave_sum(double v) {
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-  return v;
-}
-
-// Sum over the block; every thread returns the same value.
-__device__ inline double block_sum(double v, double* red) {
-  v = wave_sum(v);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double t = 0.0;
-  for (int i = 0; i < kSpecWaves; ++i) t += red[i];
-  return t;
-}
-
-__device__ inline double block_max(double v, double* red) {
-  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double t = red[0];
-  for (int i = 1; i < kSpecWaves; ++i) t = fmax(t, red[i]);
-  return t;
-}
-
-// Householder vector of row k (entries k+1..n-1, current values): writes
-// vout[k+1] = 1, vout[j > k+1], d[k] and e[k]; returns tau (block-uniform).
-__device__ double reflector(const double* A, int ld, int n, int k, double* vout, double* d, double* e, double* red) {
-  const double* rk = A + (long long)k * ld;
-  const double alpha = rk[k + 1];
-  double s = 0.0;
-  for (int j = k + 2 + (int)threadIdx.x; j < n; j += kSpecThreads) s += rk[j] * rk[j];
-  const double sigma = block_sum(s, red);
-  double tau = 0.0, scale = 0.0, off = alpha;
-  if (sigma > 0.0) {
-    const double beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
-    tau = (beta - alpha) / beta;
-    scale = 1.0 / (alpha - beta);
-    off = beta;
-  }
-  for (int j = k + 2 + (int)threadIdx.x; j < n; j += kSpecThreads) vout[j] = rk[j] * scale;
-  if (threadIdx.x == 0) {
-    vout[k + 1] = 1.0;
-    d[k] = rk[k];
-    e[k] = off;
-  }
-  __syncthreads();
-  return tau;
-}
-
-__device__ int sturm_count(const double* d, const double* ee, int n, double x, double pivmin) {
-  double q = d[0] - x;
-  if (fabs(q) <= pivmin) q = -pivmin;
-  int c = q < 0.0;
-  for (int j = 1; j < n; ++j) {
-    q = d[j] - x - ee[j - 1] / q;
-    if (fabs(q) <= pivmin) q = -pivmin;
-    c += q < 0.0;
-  }
-  return c;
-}
+constexpr int kSpecThreads = kTriThreads;
+constexpr int kSpecWaves = kTriWaves;
 
 // All eigenvalues (ascending) of each symmetric block: blockIdx.x = block
 // (sample * 2 + parity), blockIdx.y = table entry. The block is symmetrised and
@@ -108,81 +46,9 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(EntropyTable tab
   double* w = p + ld;             // [ld] p + K v, then squared off-diagonal
   double* d = w + ld;             // [ld] diagonal of the tridiagonal form
   double* e = d + ld;             // [ld] off-diagonal
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x;
   if (n <= 0) return;
-  for (long long i = tid; i < (long long)n * n; i += kSpecThreads) {
-    const int r = (int)(i / n), q = (int)(i % n);
-    if (r < q) {
-      const double s = 0.5 * (A[(long long)r * ld + q] + A[(long long)q * ld + r]);
-      A[(long long)r * ld + q] = s;
-      A[(long long)q * ld + r] = s;
-    }
-  }
-  __syncthreads();
-  if (n <= 2) {
-    if (tid == 0) {
-      d[0] = A[0];
-      if (n == 2) {
-        d[1] = A[(long long)ld + 1];
-        e[0] = A[1];
-      }
-    }
-    __syncthreads();
-  } else {
-    double tau = reflector(A, ld, n, 0, v, d, e, red);
-    for (int i = 1 + wave; i < n; i += kSpecWaves) {
-      const double* ri = A + (long long)i * ld;
-      double acc = 0.0;
-      for (int j = 1 + lane; j < n; j += 64) acc += ri[j] * v[j];
-      acc = wave_sum(acc);
-      if (lane == 0) p[i] = tau * acc;
-    }
-    __syncthreads();
-    for (int k = 0; k <= n - 3; ++k) {
-      // w = p + K v with K = -tau/2 (p . v): the step-k update is S -= v w^T + w v^T
-      double s = 0.0;
-      for (int i = k + 1 + tid; i < n; i += kSpecThreads) s += p[i] * v[i];
-      const double K = -0.5 * tau * block_sum(s, red);
-      for (int i = k + 1 + tid; i < n; i += kSpecThreads) w[i] = p[i] + K * v[i];
-      __syncthreads();
-      if (k == n - 3) {
-        if (tid == 0) {
-          const int a = n - 2, b = n - 1;
-          d[a] = A[(long long)a * ld + a] - 2.0 * v[a] * w[a];
-          d[b] = A[(long long)b * ld + b] - 2.0 * v[b] * w[b];
-          e[a] = A[(long long)a * ld + b] - (v[a] * w[b] + w[a] * v[b]);
-        }
-        __syncthreads();
-        break;
-      }
-      {
-        const int r = k + 1;
-        double* rr = A + (long long)r * ld;
-        for (int j = r + tid; j < n; j += kSpecThreads) rr[j] -= v[r] * w[j] + w[r] * v[j];
-      }
-      __syncthreads();
-      const double tau2 = reflector(A, ld, n, k + 1, v2, d, e, red);
-      // rows k+2.. : apply the step-k update (columns k+2..; column k+1 is not
-      // read again) and form the next product p = tau2 * S v2
-      for (int i = k + 2 + wave; i < n; i += kSpecWaves) {
-        double* ri = A + (long long)i * ld;
-        const double vi = v[i], wi = w[i];
-        double acc = 0.0;
-        for (int j = k + 2 + lane; j < n; j += 64) {
-          const double a = ri[j] - (vi * w[j] + wi * v[j]);
-          ri[j] = a;
-          acc += a * v2[j];
-        }
-        acc = wave_sum(acc);
-        if (lane == 0) p[i] = tau2 * acc;
-      }
-      __syncthreads();
-      double* t = v;
-      v = v2;
-      v2 = t;
-      tau = tau2;
-    }
-  }
+  tridiagonalize(A, ld, n, red, v, v2, p, w, d, e, nullptr);
   // bisection on the tridiagonal form (Gershgorin interval, Sturm counts)
   double* ee = w;
   for (int j = tid; j < n - 1; j += kSpecThreads) ee[j] = e[j] * e[j];
@@ -193,16 +59,16 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(EntropyTable tab
     hi = fmax(hi, d[i] + r);
     if (i < n - 1) emax = fmax(emax, e[i] * e[i]);
   }
-  lo = -block_max(-lo, red);
-  hi = block_max(hi, red);
-  emax = block_max(emax, red);
+  lo = -tri_block_max(-lo, red);
+  hi = tri_block_max(hi, red);
+  emax = tri_block_max(emax, red);
   const double pivmin = DBL_MIN * fmax(1.0, emax);
   const double tol = 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + pivmin;
   for (int i = tid; i < n; i += kSpecThreads) {
     double a = lo, b = hi;
     for (int it = 0; it < 128 && b - a > tol; ++it) {
       const double mid = 0.5 * (a + b);
-      if (sturm_count(d, ee, n, mid, pivmin) > i)
+      if (tri_sturm_count(d, ee, n, mid, pivmin) > i)
         b = mid;
       else
         a = mid;

@@ -1,0 +1,79 @@
+// Spectrum of a symmetric tridiagonal matrix with the projections of given
+// vectors onto its eigenvectors, for the PSD-projected site metric of
+// physical-kind Born sampling (`_positive_metric`, src/states/tensors.jl:65-94,
+// used at src/sampling/born.jl:117-134).
+//
+// Implicit QL iteration with Wilkinson shifts (Bowdler, Martin, Reinsch and
+// Wilkinson, Numer. Math. 11 (1968) 293-306): every plane rotation that acts on
+// columns (i, i+1) of the accumulated eigenvector matrix Z acts instead on the
+// entries (i, i+1) of y = Z^T w, so with y starting as w the result holds
+// z_j^T w for every eigenvector z_j without forming Z (O(n^2) work in all).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cmath>
+
+namespace spp {
+
+// d[0..n-1]: diagonal, overwritten by the eigenvalues (unordered);
+// e[0..n-1]: e[i] = T[i][i+1] for i < n-1, e[n-1] unused, overwritten;
+// y: ny vectors of length n (vector v at y + v * ystride), overwritten by the
+// projections. Returns 0, or 1 when an eigenvalue needed more than maxit
+// iterations (the values are then incomplete).
+__host__ __device__ inline int ql_project(int n, double* d, double* e, double* y, long long ystride, int ny,
+                                          int maxit) {
+  if (n <= 0) return 0;
+  e[n - 1] = 0.0;
+  for (int l = 0; l < n; ++l) {
+    int iter = 0;
+    for (;;) {
+      // the active block starts at l and ends at the first negligible coupling
+      int m = l;
+      for (; m < n - 1; ++m) {
+        const double dd = fabs(d[m]) + fabs(d[m + 1]);
+        if (fabs(e[m]) <= DBL_EPSILON * dd) break;
+      }
+      if (m == l) break;
+      if (iter++ >= maxit) return 1;
+      // shift: the eigenvalue of the leading 2x2 of the block closer to d[l]
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + copysign(r, g));
+      double s = 1.0, c = 1.0, p = 0.0;
+      bool split = false;
+      for (int i = m - 1; i >= l; --i) {
+        const double f = s * e[i];
+        const double b = c * e[i];
+        r = hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {
+          // the coupling underflowed: the block splits at i + 1
+          d[i + 1] -= p;
+          e[m] = 0.0;
+          split = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+        for (int v = 0; v < ny; ++v) {
+          double* yv = y + v * ystride;
+          const double hi = yv[i + 1];
+          yv[i + 1] = s * yv[i] + c * hi;
+          yv[i] = c * yv[i] - s * hi;
+        }
+      }
+      if (split) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  return 0;
+}
+
+}  // namespace spp

@@ -43,6 +43,67 @@ inline Leg pair_leg(int dim, int even) {  // (ket, bra) fused leg of a double la
   return fixed_leg(dim * dim, even * even + odd * odd);
 }
 
+// Physical-kind data of one site (resampling proposals, born.jl:116-134): the
+// FP64 site tensor and the ket tuples (row-major over W, N, E, S) of the two
+// parity blocks of its metric.
+struct PhysSite {
+  double* a = nullptr;   // site copy (W,N,P,A,E,S), A of dimension 1
+  int dim[6] = {1, 1, 1, 1, 1, 1};
+  int even[6] = {1, 1, 1, 1, 1, 1};
+  int nq[2] = {0, 0};    // block sizes
+  int* inv = nullptr;    // device [2][ld]: tuple index of (block, position)
+};
+
+// Device workspace of physical_conditionals for nb samples (blocks of at most
+// ld rows): M [nb][2][ld][ld], taus / d / e [nb][2][ld], y [nb][2][2][ld],
+// norms [nb][2][2], res [nb][2][8].
+struct PsdScratch {
+  int ld = 0;
+  double *M = nullptr, *taus = nullptr, *d = nullptr, *e = nullptr, *y = nullptr, *norms = nullptr, *res = nullptr;
+};
+inline size_t psd_scratch_doubles(int ld) { return 2 * (size_t)ld * ld + 3 * 2 * (size_t)ld + 4 * (size_t)ld + 4 + 16; }
+
+// Eigenvalue summaries and label weights of the PSD-projected metric of one
+// site for nb samples: H [nb][hs] pair-leg holes (FP32 when h_fp32), U the 2x2
+// basis (row-major U[p][k]). Results in w.res / w.norms for
+// physical_select_kernel.
+void physical_conditionals(Context& ctx, const void* H, bool h_fp32, long long hs, const PhysSite& ps,
+                           const double* U, int nb, const PsdScratch& w);
+// The site's report (repairs [nb][nsite][6]) and Born conditional of every
+// sample from physical_conditionals, then the draw with the uniforms (or the
+// given labels), as born_select_kernel does for the auxiliary kind. prefix
+// [nb]: charge of the labels chosen so far; target < 0: no charge constraint.
+void launch_physical_select(Context& ctx, const PsdScratch& w, const double* uniforms, const int* labels_in,
+                            long long sample0, int nsite, int site, const int* charges, int target, int reach_next,
+                            int* prefix, int* choice, int* labels, double* logp, double* cond, int* status,
+                            int* fail_site, int* done, double* repairs, int nb);
+// Host self-test of the PSD path: eigenvalues and label projections of
+// symmetric blocks [nprob][ld][ld] with vectors [nprob][2][ld].
+void psd_selftest(Context& ctx, int nprob, int ld, const int* n_h, const double* mats_h, const double* vecs_h,
+                  double* eig_h, double* proj_h, int* fail_h);
+
+// `_categorical(log.(p), u)` (src/resample/selection.jl:68-80) over npos
+// probabilities with the uniform u: the first index whose cumulative mass
+// exp(log p - logsumexp) exceeds u, skipping zero entries, else the last
+// positive one.
+__device__ inline int categorical_index(const double* p, int npos, double u) {
+  double pivot = -INFINITY;
+  for (int j = 0; j < npos; ++j) pivot = fmax(pivot, log(p[j]));
+  double sum = 0.0;
+  for (int j = 0; j < npos; ++j) sum += exp(log(p[j]) - pivot);
+  const double z = pivot + log(sum);
+  double cum = 0.0;
+  int last = -1;
+  for (int j = 0; j < npos; ++j) {
+    const double lm = log(p[j]);
+    if (lm == -INFINITY) continue;
+    last = j;
+    cum += exp(lm - z);
+    if (u < cum) return j;
+  }
+  return last;
+}
+
 template <typename T>
 struct CacheDev {
   int Lx = 0, Ly = 0, chi = 0, precision = 0;
@@ -55,11 +116,21 @@ struct CacheDev {
   std::vector<Boundary<T>> south;    // south[y], y = 0..Ly, rotated storage, shared
   std::vector<double> south_max_trunc;  // host copy per y
   double pepo_log_amplitude = 0, pepo_beta = 0;
+  // 0: auxiliary kind (PEPO), 1: physical kind (PEPS, resampling proposals)
+  int kind = 0;
+  // physical kind: per-site tensors and metric tables, the largest block, the
+  // state's total charge (-1: unconstrained) and the reachable-charge masks
+  // reach[i] (bit q: charge q can still be fused by sites i..; reach[ns] = 1)
+  std::vector<PhysSite> phys;
+  int psd_ld = 0;
+  int target_charge = -1;
+  std::vector<int> reach;
   ~CacheDev();
 };
 
+// kind 0: auxiliary kind of a PEPO; kind 1: physical kind of a PEPS.
 template <typename T>
-void build_cache(Context& ctx, const StateDev& pepo, int chi, const double* basis, CacheDev<T>& cache);
+void build_cache(Context& ctx, const StateDev& pepo, int chi, const double* basis, CacheDev<T>& cache, int kind = 0);
 
 struct DrawOutputs {
   int* labels = nullptr;           // [n][Lx*Ly] basis indices (0-based), row-major sites
@@ -69,7 +140,11 @@ struct DrawOutputs {
   double* tensor_log_norm = nullptr;  // [n] log norm of the normalised projected PEPS (null: skip)
   double* log_amplitude = nullptr; // [n] log amplitude of the projected PEPS (null: skip)
   int* status = nullptr;           // [n] 0 ok, 1 invalid conditional, 2 zero mass, 3 nonfinite
+                                   // environment; physical kind: 4 nonfinite norm metric,
+                                   // 5 no label fits the total charge, 6 eigensolver cap
   int* fail_site = nullptr;        // [n] row-major site index of the failure (or -1)
+  double* repairs = nullptr;       // physical kind [n][Lx*Ly][6] (null: skip): the site's
+                                   // _positive_metric report and the zero-weight fallback
 };
 
 // uniforms: [n][Lx*Ly] (row-major, x fastest) or null in evaluation mode;

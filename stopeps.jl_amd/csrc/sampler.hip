@@ -12,6 +12,7 @@
 // (born.jl:157-160).
 #include "sampler.h"
 #include "boundary_ops.h"
+#include "tridiag.h"
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -32,6 +33,10 @@ CacheDev<T>::~CacheDev() {
       if (p) (void)hipFree(p);
   if (cand_lognorm) (void)hipFree(cand_lognorm);
   for (auto& b : south) b.release();
+  for (auto& ps : phys) {
+    if (ps.a) (void)hipFree(ps.a);
+    if (ps.inv) (void)hipFree(ps.inv);
+  }
 }
 template struct CacheDev<float>;
 template struct CacheDev<double>;
@@ -132,6 +137,15 @@ __global__ void born_select_kernel(const double* raw0, const double* raw1, const
   labels[(long long)s * nsite + site] = k;
 }
 
+// max_environment_truncation of the samples still being drawn: a sample that
+// ends early (p = 0, or the charge check) keeps what it had when it ended
+// (born.jl:100-104, 146-148).
+__global__ void live_trunc_kernel(const double* mt, const int* done, double* out, int nb) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nb) return;
+  if (!done[s]) out[s] = mt[s];
+}
+
 __global__ void make_ptrs_kernel(const int* choice, void* c0, void* c1, void** ptrs, int nb) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nb) return;
@@ -147,6 +161,8 @@ static size_t draw_bytes_per_sample(const CacheDev<T>& c, const Context& ctx) {
   const long long Lx = c.Lx;
   const double ts = sizeof(T);
   double rowenv = ts * ((Lx + 4) * chi * D * chi + 3.0 * chi * D * D * chi) + 64 * 8;
+  if (c.kind == 1)  // the site hole and its PSD workspace
+    rowenv += ts * (double)D * D * D * D + 8.0 * (double)psd_scratch_doubles(c.psd_ld) + 8.0 * 6 * Lx * c.Ly;
   return (size_t)(1.15 * std::max(rowenv, absorb_bytes_per_sample(Lx, chi, D, sizeof(T), ctx.ksub))) + 4096;
 }
 
@@ -218,15 +234,94 @@ static void convert_boundary(Context& ctx, const Boundary<TI>& in, Boundary<TO>&
   SPP_CHECK(hipMemcpyAsync(out.max_trunc, in.max_trunc, sizeof(double) * in.nb, hipMemcpyDeviceToDevice, st));
 }
 
+// Physical-kind tables of a cache (born.jl:56-73): FP64 copies of the PEPS
+// sites, the metric block tables, the state's total charge
+// (`_state_charge`, basis.jl:89-92) and the reachable charges
+// (`_charge_suffix`, basis.jl:93-100) for the symmetry basis.
 template <typename T>
-void build_cache(Context& ctx, const StateDev& pepo, int chi, const double* basis, CacheDev<T>& c) {
+static void build_physical(Context& ctx, const StateDev& peps, CacheDev<T>& c) {
+  hipStream_t st = ctx.stream;
+  const int ns = peps.Lx * peps.Ly;
+  bool symmetric = false;
+  for (const SiteDev& s : peps.sites) symmetric = symmetric || s.even[2] < s.dim[2];
+  if (symmetric) {
+    const double* U = c.basis;
+    if (!(U[0] == 1.0 && U[1] == 0.0 && U[2] == 0.0 && U[3] == 1.0))
+      throw HipError("symmetric sampling requires SymmetryBasis()");
+    for (const SiteDev& s : peps.sites)
+      if (s.even[2] != 1) throw HipError("sampling requires one-dimensional irreps");
+  }
+  c.charges[0] = 0;
+  c.charges[1] = symmetric ? 1 : 0;
+  int target = 0;
+  for (const SiteDev& s : peps.sites) target ^= s.even[3] == 0 ? 1 : 0;
+  c.target_charge = symmetric ? target : -1;
+  c.reach.assign(ns + 1, 0);
+  c.reach[ns] = 1;
+  for (int i = ns - 1; i >= 0; --i)
+    for (int r = 0; r < 2; ++r)
+      if ((c.reach[i + 1] >> r) & 1)
+        for (int k = 0; k < 2; ++k) c.reach[i] |= 1 << (c.charges[k] ^ r);
+  c.phys.resize(ns);
+  int ld = 1;
+  std::vector<std::vector<int>> blocks(ns);
+  for (int i = 0; i < ns; ++i) {
+    const SiteDev& s = peps.sites[i];
+    PhysSite& ps = c.phys[i];
+    for (int k = 0; k < 6; ++k) {
+      ps.dim[k] = s.dim[k];
+      ps.even[k] = s.even[k];
+    }
+    std::vector<int> byq[2];
+    const int dW = s.dim[0], dN = s.dim[1], dE = s.dim[4], dS = s.dim[5];
+    for (int w = 0; w < dW; ++w)
+      for (int n = 0; n < dN; ++n)
+        for (int e = 0; e < dE; ++e)
+          for (int sv = 0; sv < dS; ++sv) {
+            const int q = ((w >= s.even[0]) + (n >= s.even[1]) + (e >= s.even[4]) + (sv >= s.even[5])) & 1;
+            byq[q].push_back(((w * dN + n) * dE + e) * dS + sv);
+          }
+    ps.nq[0] = (int)byq[0].size();
+    ps.nq[1] = (int)byq[1].size();
+    ld = std::max(ld, std::max(ps.nq[0], ps.nq[1]));
+    blocks[i] = byq[0];
+    blocks[i].insert(blocks[i].end(), byq[1].begin(), byq[1].end());
+  }
+  // a block must fit one workgroup's LDS vectors (6 ld doubles)
+  if ((size_t)(kTriWaves + 6 * (size_t)ld) * sizeof(double) > 160 * 1024)
+    throw HipError("physical-kind site metric block too large for the eigensolver (" + std::to_string(ld) +
+                   " rows)");
+  c.psd_ld = ld;
+  for (int i = 0; i < ns; ++i) {
+    const SiteDev& s = peps.sites[i];
+    PhysSite& ps = c.phys[i];
+    std::vector<int> inv(2 * (size_t)ld, 0);
+    for (int j = 0; j < ps.nq[0]; ++j) inv[j] = blocks[i][j];
+    for (int j = 0; j < ps.nq[1]; ++j) inv[ld + j] = blocks[i][ps.nq[0] + j];
+    SPP_CHECK(hipMalloc(&ps.inv, sizeof(int) * 2 * (size_t)ld));
+    upload(ps.inv, inv.data(), sizeof(int) * 2 * (size_t)ld, st);
+    SPP_CHECK(hipMalloc(&ps.a, sizeof(double) * s.size()));
+    SPP_CHECK(hipMemcpyAsync(ps.a, s.d, sizeof(double) * s.size(), hipMemcpyDeviceToDevice, st));
+  }
+}
+
+template <typename T>
+void build_cache(Context& ctx, const StateDev& pepo, int chi, const double* basis, CacheDev<T>& c, int kind) {
   hipStream_t st = ctx.stream;
   const int Lx = pepo.Lx, Ly = pepo.Ly, ns = Lx * Ly;
   c.Lx = Lx;
   c.Ly = Ly;
   c.chi = chi;
+  c.kind = kind;
   c.precision = sizeof(T) == 8 ? 0 : 1;
   if (basis) std::memcpy(c.basis, basis, sizeof(double) * 4);
+  if (kind == 1) {
+    if (pepo.kind != 1) throw HipError("proposal requires PEPS");
+    for (const SiteDev& s : pepo.sites)
+      if (s.dim[2] != 2 || s.dim[3] != 1)
+        throw HipError("physical-kind sampling cache needs a PEPS with d = 2 and a one-index auxiliary leg");
+    build_physical<T>(ctx, pepo, c);
+  }
   c.pepo_log_amplitude = pepo.log_amplitude;
   c.pepo_beta = pepo.beta;
   c.layers.assign(ns, nullptr);
@@ -237,7 +332,8 @@ void build_cache(Context& ctx, const StateDev& pepo, int chi, const double* basi
   long long maxout = 0;
   for (int i = 0; i < ns; ++i) {
     const SiteDev& s = pepo.sites[i];
-    if (s.dim[3] != 2 || s.dim[2] != 2) throw HipError("sampling cache needs a PEPO with d = 2 physical and aux legs");
+    if (kind == 0 && (s.dim[3] != 2 || s.dim[2] != 2))
+      throw HipError("sampling cache needs a PEPO with d = 2 physical and aux legs");
     LayerSpec sp{pair_leg(s.dim[0], s.even[0]), pair_leg(s.dim[1], s.even[1]), pair_leg(s.dim[4], s.even[4]),
                  pair_leg(s.dim[5], s.even[5])};
     c.spec[i] = sp;
@@ -258,11 +354,22 @@ void build_cache(Context& ctx, const StateDev& pepo, int chi, const double* basi
       if (v == 0) {
         J.out = c.layers[i];
         J.use_aux = 0;
-      } else {
+      } else if (kind == 0) {
         J.out = c.cand[v - 1][i];
         J.use_aux = 1;
         J.aux[0] = c.basis[0 * 2 + (v - 1)];
         J.aux[1] = c.basis[1 * 2 + (v - 1)];
+      } else {
+        // physical projector u_k u_k^T on the ket's physical leg (the bra is
+        // contracted over it): the double layer of _project(A, B, k, :physical)
+        J.out = c.cand[v - 1][i];
+        J.use_aux = 0;
+        J.use_op = 1;
+        const double u0 = c.basis[0 * 2 + (v - 1)], u1 = c.basis[1 * 2 + (v - 1)];
+        J.op[0] = u0 * u0;
+        J.op[1] = u0 * u1;
+        J.op[2] = u1 * u0;
+        J.op[3] = u1 * u1;
       }
       jobs.push_back(J);
     }
@@ -272,7 +379,8 @@ void build_cache(Context& ctx, const StateDev& pepo, int chi, const double* basi
   upload(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), st);
   SPP_CHECK(launch_double_layers<T>(djobs, (int)jobs.size(), maxout, st));
   SPP_CHECK(hipMalloc(&c.cand_lognorm, sizeof(double) * 2 * ns));
-  for (int i = 0; i < ns; ++i) {
+  if (kind == 1) SPP_CHECK(launch_fill<double>(c.cand_lognorm, 2 * ns, 0.0, st));
+  for (int i = 0; i < ns && kind == 0; ++i) {
     const SiteDev& s = pepo.sites[i];
     int nbef = s.dim[0] * s.dim[1] * s.dim[2], naft = s.dim[4] * s.dim[5];
     hipLaunchKernelGGL(proj_lognorm_kernel, dim3(1), dim3(256), 0, st, s.d, nbef, s.dim[3], naft, c.basis[0],
@@ -327,8 +435,8 @@ void build_cache(Context& ctx, const StateDev& pepo, int chi, const double* basi
   SPP_CHECK(hipFree(djobs));
 }
 
-template void build_cache<float>(Context&, const StateDev&, int, const double*, CacheDev<float>&);
-template void build_cache<double>(Context&, const StateDev&, int, const double*, CacheDev<double>&);
+template void build_cache<float>(Context&, const StateDev&, int, const double*, CacheDev<float>&, int);
+template void build_cache<double>(Context&, const StateDev&, int, const double*, CacheDev<double>&, int);
 
 template <typename T>
 void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, const int* labels_in,
@@ -378,6 +486,24 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
   SPP_CHECK(hipMalloc(&ptrs, sizeof(void*) * nb * Lx));
   SPP_CHECK(hipMalloc(&dsst, sizeof(long long) * Lx));
   SPP_CHECK(hipMalloc(&dtens, sizeof(T*) * Lx));
+  const bool phys = c.kind == 1;
+  int* prefix = nullptr;
+  double* mtlive = nullptr;
+  double* drep = nullptr;
+  SPP_CHECK(hipMalloc(&prefix, sizeof(int) * nb));
+  SPP_CHECK(hipMalloc(&mtlive, sizeof(double) * nb));
+  if (phys) SPP_CHECK(hipMalloc(&drep, sizeof(double) * (size_t)nb * ns * 6));
+  // evaluation mode of a charged physical cache: a configuration outside the
+  // state's total charge has log p = -Inf before any contraction (born.jl:100-104)
+  std::vector<int> h_rej(labels_in ? n : 0, 0);
+  bool any_rej = false;
+  if (labels_in && phys && c.target_charge >= 0)
+    for (int i = 0; i < n; ++i) {
+      int q = 0;
+      for (int k = 0; k < ns; ++k) q ^= c.charges[labels_in[(long long)i * ns + k]];
+      h_rej[i] = q != c.target_charge;
+      any_rej = any_rej || h_rej[i];
+    }
   if (labels_in) SPP_CHECK(hipMalloc(&dlabels_in, sizeof(int) * (long long)n * ns));
   if (uniforms) SPP_CHECK(hipMalloc(&duni, sizeof(double) * (long long)n * ns));
   if (labels_in)
@@ -387,6 +513,9 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
   const int tb = 256, gb = (nb + tb - 1) / tb;
   std::vector<int> h_status(nb), h_fail(nb), h_labels((size_t)nb * ns);
   std::vector<double> h_logp(nb), h_cond((size_t)nb * ns), h_mt(nb), h_mant(nb), h_ls(nb), h_amp(nb);
+  std::vector<double> h_rep(phys ? (size_t)nb * ns * 6 : 0);
+  std::vector<int> h_done0(nb);
+  std::vector<double> h_logp0(nb);
   double south_mt = 0.0;
   for (double v : c.south_max_trunc) south_mt = std::max(south_mt, v);
   try {
@@ -405,6 +534,17 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
       SPP_CHECK(launch_fill<double>(logp, nb, 0.0, st));
       SPP_CHECK(launch_fill<double>(cond, (long long)nb * ns, 0.0, st));
       SPP_CHECK(launch_fill<int>(choice, nb, 0, st));
+      SPP_CHECK(launch_fill<int>(prefix, nb, 0, st));
+      SPP_CHECK(launch_fill<double>(mtlive, nb, 0.0, st));
+      if (phys) SPP_CHECK(launch_fill<double>(drep, (long long)nb * ns * 6, NAN, st));
+      if (any_rej) {
+        for (int s = 0; s < cur; ++s) {
+          h_done0[s] = h_rej[s0 + s];
+          h_logp0[s] = h_rej[s0 + s] ? -INFINITY : 0.0;
+        }
+        upload(done, h_done0.data(), sizeof(int) * cur, st);
+        upload(logp, h_logp0.data(), sizeof(double) * cur, st);
+      }
       for (int y = 0; y < Ly; ++y) {
         Boundary<T>& Nc = NB[which];
         const Boundary<T>& sb = c.south[y + 1];
@@ -462,6 +602,51 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
           O[0] = (T*)ctx.arena.take(sizeof(T) * cur * os);
           O[1] = (T*)ctx.arena.take(sizeof(T) * cur * os);
           std::vector<Leg> Ol = {Nc.bond(x + 1), sp.e, sbr};
+          if (phys) {
+            // the site hole H[w,n,e,s] = X[w,b,c,n] (B[b,s,d] R[c,e,d]) (born.jl:75-83)
+            const long long y2s = (long long)sbl.dim * Nc.bdim[x + 1] * sp.e.dim * sp.s.dim;
+            T* Y2 = (T*)ctx.arena.take(sizeof(T) * cur * y2s);
+            TRef Y2r = dense_ref(Y2, y2s, "bces", {sbl, Nc.bond(x + 1), sp.e, sp.s});
+            contract<T>(ctx, Y2r, bottom_ref(sb, Lx, x, "bsd"), dense_ref(R[x + 1], Rs[x + 1], "ced", Rl[x + 1]),
+                        cur);
+            const long long hs = layer_count(sp);
+            T* Hb = (T*)ctx.arena.take(sizeof(T) * cur * hs);
+            contract<T>(ctx, dense_ref(Hb, hs, "wnes", {sp.w, sp.n, sp.e, sp.s}), Xr, Y2r, cur);
+            PsdScratch w;
+            w.ld = c.psd_ld;
+            const size_t l2 = (size_t)c.psd_ld;
+            w.M = (double*)ctx.arena.take(sizeof(double) * cur * 2 * l2 * l2);
+            w.taus = (double*)ctx.arena.take(sizeof(double) * cur * 2 * l2);
+            w.d = (double*)ctx.arena.take(sizeof(double) * cur * 2 * l2);
+            w.e = (double*)ctx.arena.take(sizeof(double) * cur * 2 * l2);
+            w.y = (double*)ctx.arena.take(sizeof(double) * cur * 4 * l2);
+            w.norms = (double*)ctx.arena.take(sizeof(double) * cur * 4);
+            w.res = (double*)ctx.arena.take(sizeof(double) * cur * 16);
+            physical_conditionals(ctx, Hb, sizeof(T) == sizeof(float), hs, c.phys[site], c.basis, cur, w);
+            launch_physical_select(ctx, w, duni, dlabels_in, s0, ns, site, c.charges, c.target_charge,
+                                   c.reach[site + 1], prefix, choice, labels, logp, cond, status, fail, done, drep,
+                                   cur);
+            hipLaunchKernelGGL(make_ptrs_kernel, dim3(gb), dim3(tb), 0, st, choice, (void*)c.cand[0][site],
+                               (void*)c.cand[1][site], ptrs + (long long)x * cur, cur);
+            SPP_CHECK(hipGetLastError());
+            if (x < Lx - 1) {
+              // left <- safe_scale(left_step of the drawn candidate)
+              TRef lay = dense_ref(nullptr, 0, "wnes", {sp.w, sp.n, sp.e, sp.s});
+              lay.ptrs = (void* const*)(ptrs + (long long)x * cur);
+              TRef Yr = folded_ref(Yk, ys, "bces", {sbl, Nc.bond(x + 1), sp.e, sp.s});
+              contract<T>(ctx, Yr, Xr, lay, cur);
+              TRef Lnew = dense_ref(Lb, os, "ced", Ol);
+              contract<T>(ctx, Lnew, Yr, bottom_ref(sb, Lx, x, "bsd"), cur);
+              FullView fv = full_view(Lnew);
+              SPP_CHECK(launch_dot<T>(fv, fv, nrm, cur, st, dot_partials(ctx, cur)));
+              SPP_CHECK(launch_safe_factor(nrm, fac, status, fail, site, cur, st));
+              SPP_CHECK(launch_scale<T>(fv, fac, cur, st));
+              Ll = Ol;
+              Ls = os;
+            }
+            ctx.arena.release(xm);
+            continue;
+          }
           double* rawk[2] = {raw0, raw1};
           for (int k = 0; k < 2; ++k) {
             TRef Yr = folded_ref(Yk, ys, "bces", {sbl, Nc.bond(x + 1), sp.e, sp.s});
@@ -493,6 +678,8 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
         }
         ctx.arena.release(mark);
         // ---------------- absorb the drawn row ----------------
+        // (the physical kind has no use for the last one, born.jl:157-160)
+        if (phys && y == Ly - 1) break;
         RowLayers<T> row;
         row.L = Lx;
         row.spec.assign(c.spec.begin() + y * Lx, c.spec.begin() + (y + 1) * Lx);
@@ -500,23 +687,32 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
         for (int x = 0; x < Lx; ++x) row.ptrs[x] = ptrs + (long long)x * cur;
         absorb_forward<T>(ctx, Nc, row, chi, NB[1 - which]);
         which = 1 - which;
+        hipLaunchKernelGGL(live_trunc_kernel, dim3(gb), dim3(tb), 0, st, NB[which].max_trunc, done, mtlive, cur);
+        SPP_CHECK(hipGetLastError());
       }
       // lognorm of the normalised projected sample from the final north boundary
       Boundary<T>& Nf = NB[which];
-      for (int x = 0; x <= Lx; ++x)
-        if (Nf.bdim[x] != 1) throw HipError("final boundary must have unit bonds");
-      std::vector<long long> sst(Lx);
-      for (int x = 0; x < Lx; ++x) sst[x] = Nf.tsize(x);
-      upload(dsst, sst.data(), sizeof(long long) * Lx, st);
-      upload(dtens, Nf.t.data(), sizeof(T*) * Lx, st);
-      SPP_CHECK(launch_close_unit<T>(dtens, dsst, Lx, mant, cur, st));
-      SPP_CHECK(launch_gather_lognorm(c.cand_lognorm, labels, ns, lnamp, cur, st));
+      if (!phys) {
+        for (int x = 0; x <= Lx; ++x)
+          if (Nf.bdim[x] != 1) throw HipError("final boundary must have unit bonds");
+        std::vector<long long> sst(Lx);
+        for (int x = 0; x < Lx; ++x) sst[x] = Nf.tsize(x);
+        upload(dsst, sst.data(), sizeof(long long) * Lx, st);
+        upload(dtens, Nf.t.data(), sizeof(T*) * Lx, st);
+        SPP_CHECK(launch_close_unit<T>(dtens, dsst, Lx, mant, cur, st));
+        SPP_CHECK(launch_gather_lognorm(c.cand_lognorm, labels, ns, lnamp, cur, st));
+      } else {
+        SPP_CHECK(launch_fill<double>(mant, cur, NAN, st));
+        SPP_CHECK(launch_fill<double>(lnamp, cur, NAN, st));
+        SPP_CHECK(hipMemcpyAsync(h_rep.data(), drep, sizeof(double) * (size_t)cur * ns * 6, hipMemcpyDeviceToHost,
+                                 st));
+      }
       SPP_CHECK(hipMemcpyAsync(h_status.data(), status, sizeof(int) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_fail.data(), fail, sizeof(int) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_labels.data(), labels, sizeof(int) * cur * ns, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_logp.data(), logp, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_cond.data(), cond, sizeof(double) * cur * ns, hipMemcpyDeviceToHost, st));
-      SPP_CHECK(hipMemcpyAsync(h_mt.data(), Nf.max_trunc, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipMemcpyAsync(h_mt.data(), mtlive, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_mant.data(), mant, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_ls.data(), Nf.log_scale, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(h_amp.data(), lnamp, sizeof(double) * cur, hipMemcpyDeviceToHost, st));
@@ -537,6 +733,8 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
           double m = h_mant[s];
           out.tensor_log_norm[g] = (m > 0.0 && std::isfinite(m)) ? std::log(m) + h_ls[s] - 2.0 * h_amp[s] : NAN;
         }
+        if (phys && out.repairs)
+          std::memcpy(out.repairs + g * ns * 6, h_rep.data() + (size_t)s * ns * 6, sizeof(double) * ns * 6);
       }
     }
   } catch (...) {
@@ -544,7 +742,8 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
     NB[1].release();
     for (void* p : {(void*)choice, (void*)labels, (void*)status, (void*)fail, (void*)done, (void*)logp, (void*)cond,
                     (void*)raw0, (void*)raw1, (void*)nrm, (void*)fac, (void*)lnamp, (void*)mant, (void*)ptrs,
-                    (void*)dlabels_in, (void*)duni, (void*)dsst, (void*)dtens})
+                    (void*)dlabels_in, (void*)duni, (void*)dsst, (void*)dtens, (void*)prefix, (void*)mtlive,
+                    (void*)drep})
       if (p) (void)hipFree(p);
     ctx.arena.release(0);
     throw;
@@ -553,7 +752,8 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
   NB[1].release();
   for (void* p : {(void*)choice, (void*)labels, (void*)status, (void*)fail, (void*)done, (void*)logp, (void*)cond,
                   (void*)raw0, (void*)raw1, (void*)nrm, (void*)fac, (void*)lnamp, (void*)mant, (void*)ptrs,
-                  (void*)dlabels_in, (void*)duni, (void*)dsst, (void*)dtens})
+                  (void*)dlabels_in, (void*)duni, (void*)dsst, (void*)dtens, (void*)prefix, (void*)mtlive,
+                  (void*)drep})
     if (p) SPP_CHECK(hipFree(p));
 }
 

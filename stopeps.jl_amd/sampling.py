@@ -198,29 +198,42 @@ def boundary_caps(space):
     return (n, -1)
 
 
+KINDS = {"auxiliary": 0, "physical": 1}
+
+
 class SamplingCache:
-    """``SamplingCache(state, BoundaryMPS(chi, sector_space); kind=:auxiliary,
-    basis=:auto)`` (src/sampling/born.jl:45-73). The state must be a PEPO; the
-    draws from the cache compress with the same boundary method."""
+    """``SamplingCache(state, BoundaryMPS(chi, sector_space); kind, basis)``
+    (src/sampling/born.jl:45-73). ``kind="auxiliary"`` (the default) samples a
+    PEPO's auxiliary indices; ``kind="physical"`` samples a PEPS's physical
+    indices with the PSD-projected site metric (the resampling proposals,
+    src/resample/event.jl:27-33). ``state`` is a host ``FiniteState`` or a
+    resident ``DeviceState`` (e.g. a particle). The draws from the cache
+    compress with the same boundary method."""
 
     def __init__(self, ctx, state, chi, basis=None, kind="auxiliary", sector_space=None):
-        if kind != "auxiliary":
-            raise NotImplementedError("physical-kind sampling (resampling proposals) is not part of this library")
-        if state.kind != "pepo":
+        if kind not in KINDS:
+            raise ValueError("unknown sampling kind")
+        dev = state if isinstance(state, DeviceState) else None
+        host = dev.state if dev is not None else state
+        if kind == "auxiliary" and host.kind != "pepo":
             raise ValueError("the auxiliary-kind sampling cache needs a PEPO")
+        if kind == "physical" and host.kind != "peps":
+            raise ValueError("proposal requires PEPS")
         self.ctx = ctx
-        self.state = state
+        self.state = host
+        self.kind = kind
         self.chi = int(chi)
         self.sector_space = sector_space
         boundary_caps(sector_space)
-        self.basis = resolve_basis(state, basis)
-        self.lattice = state.lattice
-        self._dev = DeviceState(ctx, state)
+        self.basis = resolve_basis(host, basis)
+        self.lattice = host.lattice
+        self._dev = dev if dev is not None else DeviceState(ctx, host)
         U = np.ascontiguousarray(self.basis.U, dtype=np.float64)
         h = ctypes.c_void_p()
         with ctx.boundary_space(sector_space):
-            _lib.check(ctx.handle, ctx._lib.spp_sampling_cache_build(ctx.handle, self._dev.handle, self.chi,
-                                                                     _lib.dptr(U), ctypes.byref(h)))
+            _lib.check(ctx.handle, ctx._lib.spp_sampling_cache_build_kind(ctx.handle, self._dev.handle, self.chi,
+                                                                          _lib.dptr(U), KINDS[kind],
+                                                                          ctypes.byref(h)))
         self.handle = h
         smt = np.zeros(self.lattice.Ly + 1)
         ctx._lib.spp_sampling_cache_info(self.handle, _lib.dptr(smt))
@@ -264,7 +277,37 @@ def _raise_status(cache, status, fail):
         raise BornError(f"invalid Born conditional at ({x},{y}); increase chi_sampling")
     if status[i] == 2:
         raise BornError(f"Born conditional has zero total mass at ({x},{y})")
+    if status[i] == 4:
+        raise BornError("nonfinite norm metric")
+    if status[i] == 5:
+        raise BornError("sampling prefix violates the total symmetry sector")
+    if status[i] == 6:
+        raise BornError(f"site metric eigensolver did not converge at ({x},{y})")
     raise BornError("nonfinite row environment")
+
+
+REPAIR_FIELDS = ("minimum_relative_eigenvalue", "negative_eigenvalues", "negative_multiplets",
+                 "negative_spectral_fraction", "hermiticity_defect")
+
+
+def repair_records(cache, rep):
+    """The reference's ``repairs`` list of one physical-kind draw (born.jl:118-131)
+    from its [Lx*Ly][6] array: one PSD report per site reached, in draw order,
+    each followed by a ``zero_proposal_weights`` record where the proposal fell
+    back to the uniform distribution."""
+    out = []
+    Lx = cache.lattice.Lx
+    for i, r in enumerate(np.asarray(rep)):
+        if np.isnan(r[0]):
+            break
+        site = (i % Lx + 1, i // Lx + 1)
+        d = {k: float(v) for k, v in zip(REPAIR_FIELDS, r[:5])}
+        d["negative_multiplets"] = int(r[2])
+        d["site"] = site
+        out.append(d)
+        if r[5] > 0:
+            out.append({"site": site, "reason": "zero_proposal_weights", "allowed_outcomes": int(r[5])})
+    return out
 
 
 def sample_configurations(cache, uniforms=None, configurations=None, check=True):
@@ -303,17 +346,24 @@ def sample_configurations(cache, uniforms=None, configurations=None, check=True)
     lamp = np.zeros(n)
     status = np.zeros(n, dtype=np.int32)
     fail = np.zeros(n, dtype=np.int32)
+    rep = np.zeros((n, ns, 6)) if cache.kind == "physical" else None
     ctx = cache.ctx
     with ctx.boundary_space(cache.sector_space):
-        _lib.check(ctx.handle, ctx._lib.spp_born_sample_batch(
+        _lib.check(ctx.handle, ctx._lib.spp_born_sample_batch_repairs(
             ctx.handle, cache.handle, n, _lib.dptr(uni), _lib.iptr(labels_in), _lib.iptr(labels), _lib.dptr(logp),
-            _lib.dptr(cond), _lib.dptr(mt), _lib.dptr(tln), _lib.dptr(lamp), _lib.iptr(status), _lib.iptr(fail)))
+            _lib.dptr(cond), _lib.dptr(mt), _lib.dptr(tln), _lib.dptr(lamp), _lib.iptr(status), _lib.iptr(fail),
+            _lib.dptr(rep)))
     if check:
         _raise_status(cache, status, fail)
     lab = np.asarray(cache.basis.labels)[labels].reshape(n, L.Ly, L.Lx).transpose(0, 2, 1)
-    return {"configuration": lab, "log_probability": logp, "conditionals": cond,
-            "max_environment_truncation": mt, "tensor_log_norm": tln, "log_amplitude": lamp,
-            "status": status, "fail_site": fail}
+    if labels_in is not None:
+        lab = np.array(configurations)
+    out = {"configuration": lab, "log_probability": logp, "conditionals": cond,
+           "max_environment_truncation": mt, "tensor_log_norm": tln, "log_amplitude": lamp,
+           "status": status, "fail_site": fail}
+    if rep is not None:
+        out["repairs"] = rep
+    return out
 
 
 def sample_configuration(cache, uniforms=None, configuration=None):
@@ -328,10 +378,13 @@ def sample_configuration(cache, uniforms=None, configuration=None):
     lp = float(r["log_probability"][0])
     cond = list(r["conditionals"][0])
     if lp == -np.inf:
-        k = int(np.argmin(np.asarray(cond) > 0))
-        cond = cond[:k + 1]
+        # the draw stopped at its first zero conditional, or before any site
+        # when a physical configuration missed the state's charge
+        charge_rejected = "repairs" in r and np.isnan(r["repairs"][0][0][0])
+        cond = [] if charge_rejected else cond[:int(np.argmin(np.asarray(cond) > 0)) + 1]
+    repairs = repair_records(cache, r["repairs"][0]) if "repairs" in r else []
     return {"configuration": r["configuration"][0], "log_probability": lp, "conditionals": cond,
-            "max_environment_truncation": float(r["max_environment_truncation"][0]), "repairs": []}
+            "max_environment_truncation": float(r["max_environment_truncation"][0]), "repairs": repairs}
 
 
 def project_configuration(pepo, configuration, basis=None):

@@ -36,3 +36,29 @@ def to_product_state(spp, ostate):
 
 def oracle_cfg(res, L):
     return np.array([[res["configuration"][x][y] for y in range(L.Ly)] for x in range(L.Lx)])
+
+
+def charged_peps(L, D, seed, aux_parities, symmetric=True):
+    """Random oracle PEPS whose site i carries auxiliary charge aux_parities[i]
+    (row-major): a projected particle's leg structure."""
+    from oracle.model import State
+    from oracle.z2 import Z2, symmetrize
+    rng = np.random.default_rng(seed)
+    if symmetric:
+        bp = np.array([0] * D[0] + [1] * D[1])
+        pp = np.array([0, 1])
+    else:
+        bp = np.zeros(D, dtype=np.int64)
+        pp = np.zeros(2, dtype=np.int64)
+    u = np.array([0])
+    sites = []
+    for x in range(L.Lx):
+        col = []
+        for y in range(L.Ly):
+            ap = np.array([aux_parities[x + y * L.Lx] if symmetric else 0])
+            legs = [u if x == 0 else bp, u if y == 0 else bp, pp, ap,
+                    u if x == L.Lx - 1 else bp, u if y == L.Ly - 1 else bp]
+            t = symmetrize(Z2(rng.standard_normal([len(p) for p in legs]), legs))
+            col.append(t.scale(1.0 / t.norm()))
+        sites.append(col)
+    return State(L, sites, "peps")

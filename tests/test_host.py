@@ -222,3 +222,31 @@ def test_flop_counter_reproduces_dense_draw_formula(spp):
     finally:
         flops.even_split = orig
     assert got == total
+
+
+def test_ql_projection_matches_numpy(spp):
+    """The implicit QL iteration of the physical-kind site metric (host run of
+    the device routine): eigenvalues of symmetric tridiagonal matrices, and the
+    projections u_j^T w of two vectors, against numpy's eigh — including a
+    block split by a zero coupling, repeated eigenvalues and n = 1, 2."""
+    _lib = spp._lib
+    lib = _lib.load()
+    rng = np.random.default_rng(2)
+    for n in (1, 2, 3, 17, 200):
+        for trial in range(3):
+            d = rng.standard_normal(n)
+            e = np.zeros(n)
+            e[:n - 1] = rng.standard_normal(n - 1)
+            if trial == 2 and n > 3:
+                e[n // 2] = 0.0
+                d = np.round(d)
+            T = np.diag(d) + np.diag(e[:n - 1], 1) + np.diag(e[:n - 1], -1)
+            w = rng.standard_normal((2, n))
+            dd, ee, yy = d.copy(), e.copy(), np.ascontiguousarray(w.copy())
+            assert lib.spp_selftest_ql(n, _lib.dptr(dd), _lib.dptr(ee), _lib.dptr(yy), 2, 60) == 0
+            lam, Z = np.linalg.eigh(T)
+            assert np.max(np.abs(np.sort(dd) - lam)) < 1e-12 * max(1.0, np.max(np.abs(lam)))
+            for v in range(2):
+                for f in (lambda x: np.maximum(x, 0.0), lambda x: x):
+                    ref = w[v] @ (Z * f(lam)[None, :]) @ Z.T @ w[v]
+                    assert abs(np.sum(f(dd) * yy[v] ** 2) - ref) < 1e-11 * max(1.0, abs(ref))

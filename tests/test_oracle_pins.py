@@ -14,6 +14,7 @@ from oracle import boundary as bm
 from oracle import born as ob
 from oracle import energy as oe
 from oracle import ensemble as oens
+from conftest import charged_peps
 
 
 def dense_network(state):
@@ -440,30 +441,6 @@ def test_lognorm_gradient_matches_finite_differences(kind, shape, D, sym):
 
 
 # ----------------------- physical-kind sampling (born.jl) -------------------
-
-def charged_peps(L, D, seed, aux_parities, symmetric=True):
-    """Random PEPS whose site i carries auxiliary charge aux_parities[i]
-    (row-major): a projected particle's leg structure."""
-    rng = np.random.default_rng(seed)
-    if symmetric:
-        bp = np.array([0] * D[0] + [1] * D[1])
-        pp = np.array([0, 1])
-    else:
-        bp = np.zeros(D, dtype=np.int64)
-        pp = np.zeros(2, dtype=np.int64)
-    u = np.array([0])
-    sites = []
-    for x in range(L.Lx):
-        col = []
-        for y in range(L.Ly):
-            ap = np.array([aux_parities[x + y * L.Lx] if symmetric else 0])
-            legs = [u if x == 0 else bp, u if y == 0 else bp, pp, ap,
-                    u if x == L.Lx - 1 else bp, u if y == L.Ly - 1 else bp]
-            t = symmetrize(Z2(rng.standard_normal([len(p) for p in legs]), legs))
-            col.append(t.scale(1.0 / t.norm()))
-        sites.append(col)
-    return State(L, sites, "peps")
-
 
 @pytest.mark.parametrize("symmetric", [True, False])
 def test_physical_born_probabilities_brute_force(symmetric):
