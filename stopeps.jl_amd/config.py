@@ -6,9 +6,8 @@ schema.
 Spaces are written as this library writes them: a Z2 bond space as its
 (even, odd) sector dimensions, a dense one as an int. Fields that select
 reference features this library does not provide (diagonal or variational
-contraction, annealed resampling) are kept
-so that a configuration reads the same; the drivers reject them with a
-message naming the feature.
+contraction) are kept so that a configuration reads the same; the drivers
+reject them with a message naming the feature.
 """
 import dataclasses
 from dataclasses import dataclass, field
@@ -165,8 +164,6 @@ def device_support(c, stochastic):
     """Raise for configuration choices this library does not run on the device."""
     if c.contraction not in ("BoundaryMPS", "BoundaryMPSRow"):
         raise NotImplementedError(f"contraction {c.contraction}: only the row boundary MPS is provided")
-    if stochastic and c.resampling.enabled:
-        raise NotImplementedError("annealed Born resampling is not provided: set resampling=ResamplingOptions(enabled=False)")
 
 
 def parameter_dict(c):

@@ -7,9 +7,10 @@ metadata, observables {purification, curve, samples, replay}, diagnostics
 {history, events, born_draws, born_grouping}). With ``output_directory`` set,
 the checkpoint is written to ``results/checkpoint.npz`` and the result to
 ``results/result.npz`` (``results.py``: the reference's groups and atomic
-publication in an npz container instead of JLD2). MPI, the text logs and
-resampling are not provided; rows and records are plain dicts where the
-reference has NamedTuples.
+publication in an npz container instead of JLD2). Annealed Born resampling
+events run as the reference runs them (``resample.py``). MPI and the text
+logs are not provided; rows and records are plain dicts where the reference
+has NamedTuples.
 """
 import os
 
@@ -92,14 +93,17 @@ def run_purification(c, ctx=None, on_step=None):
 
 
 def run_stochastic(c, checkpoint=None, ctx=None, uniforms=None, checkpoint_path=None):
-    """``run_stochastic(c; checkpoint, checkpoint_path)`` (run.jl:76-135) without resampling:
+    """``run_stochastic(c; checkpoint, checkpoint_path)`` (run.jl:76-135):
     Ns Born draws from the checkpoint's beta0 PEPO (sample indices 1..Ns,
     uniforms from ``sample_uniforms(seed, i, Lx*Ly)`` unless given), one
     particle per distinct configuration, full update (D_peps,
     bond_space_peps, delta_tau_peps) through every node from beta0, an
     observation and the ensemble statistics at each. Without ``checkpoint``
     the checkpoint is loaded from ``checkpoint_path`` or the output
-    directory's ``results/checkpoint.npz`` (run.jl:80-88). Returns the result
+    directory's ``results/checkpoint.npz`` (run.jl:80-88). With
+    ``c.resampling.enabled`` (the reference's default) a node whose eta falls
+    below eta_th runs an annealed Born resampling event (``resample.py``),
+    recorded under diagnostics.events and observables.replay. Returns the result
     dictionary, also saved to ``results/result.npz`` of the output directory."""
     validate_config(c)
     device_support(c, stochastic=True)
@@ -131,8 +135,8 @@ def run_stochastic(c, checkpoint=None, ctx=None, uniforms=None, checkpoint_path=
     def on_node(stats, records):
         out["observables"]["curve"].append(stats)
         out["observables"]["samples"].extend(records)
-        out["observables"]["replay"].append({"beta": stats["beta"]})
-        out["metadata"]["progress"] = {"phase": "stochastic", "beta": stats["beta"], "event": 0}
+        out["metadata"]["progress"] = {"phase": "stochastic", "beta": stats["beta"],
+                                       "event": len(out["diagnostics"]["events"])}
 
     particles, _ = stochastic_ensemble(ctx, state, c.model, c.beta0, c.beta_log, c.beta_measure, c.delta_tau_peps,
                                        c.Ns, c.seed, c.chi_sampling, c.chi_environment, c.chi_measurement,
@@ -140,7 +144,8 @@ def run_stochastic(c, checkpoint=None, ctx=None, uniforms=None, checkpoint_path=
                                        basis=sampling_basis(c), uniforms=uniforms, on_node=on_node, D=c.D_peps,
                                        draws_out=draws, record_entropy=c.record_boundary_entropy,
                                        environment_space=c.environment_space, sampling_space=c.sampling_space,
-                                       measurement_space=c.measurement_space)
+                                       measurement_space=c.measurement_space, resampling=c.resampling,
+                                       chi_proposal=c.chi_proposal, proposal_space=c.proposal_space, out=out)
     out["diagnostics"]["born_draws"] = [
         {"index": i + 1, "configuration": draws["configuration"][i], "log_probability": float(draws["log_probability"][i]),
          "max_environment_truncation": float(draws["max_environment_truncation"][i])}

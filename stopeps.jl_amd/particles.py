@@ -186,27 +186,21 @@ def observe(ctx, states, model, chi, energy, record_entropy=False, sector_space=
     return r
 
 
-def new_particles(ctx, cache, draws, model, chi_measurement, energy, bond_space=None, pepo_live=None, seed=0,
-                  first_id=1, record_entropy=False, measurement_space=None):
-    """``_initial_particles`` after the draws (run.jl:45-66) with ``_new_particle``
-    (particles.jl:42-53) for each distinct configuration: project the PEPO
-    (host slice + ``normalize_state!``), upload it as a resident PEPS (bonds
-    embedded in ``bond_space`` = (even, odd) capacities when given, live
-    sectors ``pepo_live`` [Lx*Ly, 4, 2] of the PEPO, the PEPO's zero slots
-    dropped), observe the batch and set
-    relative_log_weight = log_norm - log q.
-
-    ``draws`` is the dict of ``sample_configurations``. When ``energy`` and
-    ``chi_measurement`` allow it, the projected samples are measured through
-    the sampling cache's shared layers (same values as measuring the uploaded
-    states). ``record_entropy`` records each particle's ``Se`` profiles
-    (``record_boundary_entropy``)."""
-    grouping = group_configurations(draws["configuration"])
-    reps = grouping["first_draw"]
+def project_particles(ctx, cache, configurations, ids, counts, log_p0, log_q, sampling_error, model,
+                      chi_measurement, energy, bond_space=None, pepo_live=None, seed=0, generation=0,
+                      record_entropy=False, measurement_space=None):
+    """``_new_particle`` (particles.jl:42-53) for each given configuration:
+    project the PEPO of ``cache`` (host slice + ``normalize_state!``), upload
+    it as a resident PEPS (bonds embedded in ``bond_space`` = (even, odd)
+    capacities when given, live sectors ``pepo_live`` [Lx*Ly, 4, 2] of the
+    PEPO, the PEPO's zero slots dropped), observe the batch and set
+    relative_log_weight = log_norm - log_q. When ``energy`` is set the
+    projected samples are measured through the sampling cache's shared layers
+    (the same values as measuring the uploaded states)."""
     pepo = cache.state
     states = []
-    for i in reps:
-        st, _ = project_configuration(pepo, draws["configuration"][i], cache.basis)
+    for cfg in configurations:
+        st, _ = project_configuration(pepo, cfg, cache.basis)
         live = pepo_live
         if bond_space is not None:
             if pepo_live is None:
@@ -214,30 +208,47 @@ def new_particles(ctx, cache, draws, model, chi_measurement, energy, bond_space=
             else:
                 st, live = reembed_bonds(st, pepo_live, bond_space)
         states.append(DeviceState(ctx, st, live=live))
+    if not states:
+        return []
     t0 = time.perf_counter()
     if energy:
-        m = measure_energies(cache, draws["configuration"][reps], model, chi_measurement,
+        m = measure_energies(cache, np.asarray(configurations), model, chi_measurement,
                              record_entropy=record_entropy, sector_space=measurement_space)
     else:
         m = lognorms(ctx, states, chi_measurement, sector_space=measurement_space)
-        m["energy_total"] = np.full(len(reps), np.nan)
-        m["energy_per_site"] = np.full(len(reps), np.nan)
-        m["Se"] = [None] * len(reps)
-    mt_sec = (time.perf_counter() - t0) / max(len(reps), 1)
+        m["energy_total"] = np.full(len(states), np.nan)
+        m["energy_per_site"] = np.full(len(states), np.nan)
+        m["Se"] = [None] * len(states)
+    mt_sec = (time.perf_counter() - t0) / max(len(states), 1)
     out = []
-    for u, i in enumerate(reps):
-        log_q = float(draws["log_probability"][i])
+    for u, cfg in enumerate(configurations):
+        lq = float(log_q[u])
         out.append(Particle(
-            id=first_id + i, generation=0, sample_index=first_id + i, seed=seed,
-            count=grouping["multiplicity"][u], configuration=np.array(draws["configuration"][i]),
-            state=states[u], log_p0=log_q, log_q_proposal=log_q,
-            relative_log_weight=float(m["log_norm"][u]) - log_q, weight_log_scale=0.0,
+            id=int(ids[u]), generation=generation, sample_index=int(ids[u]), seed=seed, count=int(counts[u]),
+            configuration=np.array(cfg), state=states[u], log_p0=float(log_p0[u]), log_q_proposal=lq,
+            relative_log_weight=float(m["log_norm"][u]) - lq, weight_log_scale=0.0,
             physical_log_norm=float(m["log_norm"][u]), tensor_log_norm=float(m["tensor_log_norm"][u]),
             energy_total=float(m["energy_total"][u]), energy_per_site=float(m["energy_per_site"][u]),
             max_state_truncation=0.0,
-            max_environment_truncation=max(float(draws["max_environment_truncation"][i]),
-                                           float(m["max_environment_truncation"][u])),
+            max_environment_truncation=max(float(sampling_error[u]), float(m["max_environment_truncation"][u])),
             max_boundary_entropy=math.nan, Se=m["Se"][u], time_seconds=0.0, measurement_seconds=mt_sec))
+    return out
+
+
+def new_particles(ctx, cache, draws, model, chi_measurement, energy, bond_space=None, pepo_live=None, seed=0,
+                  first_id=1, record_entropy=False, measurement_space=None):
+    """``_initial_particles`` after the draws (run.jl:45-66): one particle per
+    distinct configuration (``project_particles``), id = sample index of its
+    first draw, multiplicity = its number of draws, log q = log p0 = the draw's
+    log probability. ``draws`` is the dict of ``sample_configurations``."""
+    grouping = group_configurations(draws["configuration"])
+    reps = grouping["first_draw"]
+    logp = [float(draws["log_probability"][i]) for i in reps]
+    out = project_particles(ctx, cache, [draws["configuration"][i] for i in reps], [first_id + i for i in reps],
+                            grouping["multiplicity"], logp, logp,
+                            [float(draws["max_environment_truncation"][i]) for i in reps], model, chi_measurement,
+                            energy, bond_space=bond_space, pepo_live=pepo_live, seed=seed,
+                            record_entropy=record_entropy, measurement_space=measurement_space)
     return out, grouping
 
 
@@ -355,13 +366,18 @@ def purify(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, chi_envir
 def stochastic_ensemble(ctx, pepo, model, beta0, beta_log, beta_measure, delta_tau, Ns, seed, chi_sampling,
                         chi_environment, chi_measurement, bond_space=None, method=None, order=2, basis=None,
                         uniforms=None, on_node=None, D=None, draws_out=None, record_entropy=False,
-                        environment_space=None, sampling_space=None, measurement_space=None):
-    """The ensemble of ``run_stochastic`` (run.jl:87-130) without resampling: Ns Born draws
-    from the beta0 PEPO (``pepo``: a host ``FiniteState`` or a resident
+                        environment_space=None, sampling_space=None, measurement_space=None, resampling=None,
+                        chi_proposal=None, proposal_space=None, out=None):
+    """The ensemble of ``run_stochastic`` (run.jl:87-130): Ns Born draws from
+    the beta0 PEPO (``pepo``: a host ``FiniteState`` or a resident
     ``DeviceState``), one particle per distinct configuration, then FU
     evolution through every node >= beta0 with an observation and the
     ensemble statistics at each. Uniforms default to ``sample_uniforms(seed,
-    i, Lx*Ly)`` for sample indices i = 1..Ns. Returns (particles, rows)."""
+    i, Lx*Ly)`` for sample indices i = 1..Ns. With ``resampling``
+    (``ResamplingOptions``, enabled) an annealed Born resampling event runs at
+    every node whose eta falls below eta_th (run.jl:117-122, ``resample.py``;
+    proposals at ``chi_proposal`` / ``proposal_space``, audit and replay rows
+    into the result dictionary ``out``). Returns (particles, rows)."""
     method = method or FullUpdate()
     live = None
     if isinstance(pepo, DeviceState):
@@ -384,10 +400,22 @@ def stochastic_ensemble(ctx, pepo, model, beta0, beta_log, beta_measure, delta_t
                                         measure_at(beta_measure, nodes[0]), bond_space=bond_space,
                                         pepo_live=live, seed=seed, record_entropy=record_entropy,
                                         measurement_space=measurement_space)
+    events = None
+    if resampling is not None and resampling.enabled:
+        from .resample import EventSettings
+        events = EventSettings(ctx, cache, Ns, L, seed, resampling,
+                               chi_sampling if chi_proposal is None else chi_proposal, proposal_space, model,
+                               chi_environment, chi_measurement, method, order, D, delta_tau, beta_measure,
+                               bond_space=bond_space, pepo_live=live, record_entropy=record_entropy,
+                               environment_space=environment_space, measurement_space=measurement_space)
     rows = []
+    replay = []
+    event = 0
+    next_sample_id = Ns + 1
     for node, beta in enumerate(nodes):
+        steps = [] if node == 0 else step_schedule(nodes[node - 1], beta, delta_tau)
+        replay.append({"beta": beta, "steps": list(steps)})
         if node > 0:
-            steps = step_schedule(nodes[node - 1], beta, delta_tau)
             advance_particles(ctx, particles, model, steps, chi_environment, chi_measurement, method,
                               measure_at(beta_measure, beta), order, D=D, record_entropy=record_entropy,
                               environment_space=environment_space, measurement_space=measurement_space)
@@ -395,6 +423,11 @@ def stochastic_ensemble(ctx, pepo, model, beta0, beta_log, beta_measure, delta_t
         stats = statistics(records, Ns, ns)
         stats["source"] = "sampling"
         stats["time_seconds"] = sum(p.time_seconds for p in particles)
+        if events is not None and stats["eta"] < resampling.eta_th:
+            from .resample import resample
+            event += 1
+            particles, records, stats, next_sample_id = resample(particles, records, events, replay, event,
+                                                                 next_sample_id, out)
         rows.append(stats)
         if on_node is not None:
             on_node(stats, records)

@@ -62,3 +62,12 @@ def charged_peps(L, D, seed, aux_parities, symmetric=True):
             col.append(t.scale(1.0 / t.norm()))
         sites.append(col)
     return State(L, sites, "peps")
+
+
+def to_oracle_state(pstate):
+    """Product FiniteState -> oracle State (same dense arrays and parities)."""
+    from oracle.model import Lattice, State
+    from oracle.z2 import Z2
+    L = pstate.lattice
+    sites = [[Z2(pstate[(x, y)].data, pstate[(x, y)].parity) for y in range(L.Ly)] for x in range(L.Lx)]
+    return State(Lattice(L.Lx, L.Ly), sites, pstate.kind, beta=pstate.beta, log_amplitude=pstate.log_amplitude)

@@ -118,8 +118,7 @@ def test_run_config_mirror(spp):
     from stopeps_jl_amd.config import bond_capacity, device_support
     z2 = spp.RunConfig(model=spp.TFIM(1.0, 1.0, symmetric=True))
     assert bond_capacity(z2, 6, (3, 3)) == (3, 3) and bond_capacity(z2, 4, None) == (4, 4)
-    with pytest.raises(NotImplementedError):
-        device_support(spp.RunConfig(), stochastic=True)  # resampling enabled by default
+    device_support(spp.RunConfig(), stochastic=True)  # resampling (enabled by default) runs on the device
     with pytest.raises(NotImplementedError):
         device_support(spp.RunConfig(contraction="BoundaryMPSDiagonal"), stochastic=False)
 
