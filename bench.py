@@ -180,7 +180,7 @@ def adjoint_timing(spp, reps=2, batch=8, chi=36):
     states with seeds 7, 8, ...)."""
     lat = spp.Lattice(8, 8)
     states = [spp.random_peps(lat, (3, 3), seed=7 + k) for k in range(batch)]
-    ctx = spp.Context(0, "fp64")
+    ctx = spp.Context(0, "fp64", autotune=0)  # timed on first calls: heuristic tile shapes
     dev = [spp.DeviceState(ctx, s) for s in states]
     spp.lognorm_gradients(ctx, dev[:1], chi)  # warm-up
     ctx.synchronize()
@@ -213,7 +213,7 @@ def entropy_timing(spp, n=16, chi=32, J=1.0, h=2.0):
     pre-truncation spectrum), each timed once after a warm-up."""
     lat = spp.Lattice(8, 8)
     pepo = spp.synthetic_thermal_pepo(lat, (3, 3), 1.0, seed=1)
-    ctx = spp.Context(0, "fp64")
+    ctx = spp.Context(0, "fp64", autotune=0)  # timed on first calls: heuristic tile shapes
     cache = spp.SamplingCache(ctx, pepo, chi)
     model = spp.TFIM(J, h, symmetric=True)
     labels = np.asarray(cache.basis.labels)
@@ -240,7 +240,7 @@ def fu_timing(spp, batch=16, chi=32, J=1.0, h=2.0, delta=0.05):
     (the options of the reference's example configurations), FP64. One timed
     step: a step is tens of seconds and allocates only its scratch."""
     lat = spp.Lattice(8, 8)
-    ctx = spp.Context(0, "fp64")
+    ctx = spp.Context(0, "fp64", autotune=0)  # timed on first calls: heuristic tile shapes
     dev = [spp.DeviceState(ctx, spp.random_peps(lat, (3, 3), seed=100 + k)) for k in range(batch)]
     model = spp.TFIM(J, h, symmetric=True)
     method = spp.FullUpdate(max_iterations=40, error_tolerance=1e-8, diagonal_shift=1e-10,
@@ -276,7 +276,7 @@ def precision_sweep(spp, flops, L=6, chi=64, batch=8, D=(4, 4)):
     out = {"workload": f"batched Born draws, {L}x{L} synthetic Z2 PEPO, D={sum(D)} ({D[0]}+{D[1]}), d=2, chi={chi}, "
                        f"{batch} samples per step (reduced lattice of the 12x12 D=8 chi=64 workload)",
            "algorithmic_flops_per_draw": flops.draw_flops(pepo, chi), "samples": batch}
-    ctx = spp.Context(0, "fp64")
+    ctx = spp.Context(0, "fp64", autotune=0)  # timed on first calls: heuristic tile shapes
     ref = None
     try:
         for prec in ("fp64", "fp32", "bf16x6", "bf16x3"):
@@ -375,6 +375,7 @@ def main():
         spp.sample_configurations(cache, uniforms=uniforms(index, B))
         index += B
     ctx.synchronize()
+    tuned = ctx.counters()  # the warm-up tunes the contraction tile shapes of every launch class
     ctx.reset_counters()
     ctx.set_option("profile", 1)
     draws = []
@@ -418,6 +419,9 @@ def main():
                      "kernel_share_of_step": kc["ms"] / (1e3 * elapsed),
                      "all_contractions_share_of_step": cnt["gemm_ms"] / (1e3 * elapsed)},
     }
+    result["autotune"] = {"tuned_classes": tuned["tuned_classes"], "classes_off_heuristic": tuned["tuned_changed"],
+                          "tuning_s": tuned["tune_ms"] / 1e3,
+                          "note": "tile shapes tuned per launch class during the warm-up; bit-identical results"}
     if args.profile_only:
         if args.copy_calibration:
             result["copy_calibration_bytes"] = ctx.copy_calibration(64 << 20)
@@ -426,12 +430,22 @@ def main():
 
     # the headline batch with the products on the BF16 matrix cores (the
     # context's workspace is already sized, so one timed step each)
+    # the same batch with the heuristic tile shapes instead of the tuned ones
+    ctx.set_option("autotune", 0)
+    t0 = time.perf_counter()
+    spp.sample_configurations(cache, uniforms=uniforms(index, B))
+    ctx.synchronize()
+    result["autotune"]["heuristic_value"] = B / (time.perf_counter() - t0)
+    result["autotune"]["gain"] = value / result["autotune"]["heuristic_value"]
+    ctx.set_option("autotune", 1)
+    log(f"heuristic tiles: {result['autotune']['heuristic_value']:.3f} samples/s")
     split_caches = {}
     if args.precision == "fp32":
         split = {}
         for prec in ("bf16x6", "bf16x3"):
             ctx.set_precision(prec)
             scache = spp.SamplingCache(ctx, pepo, args.chi)
+            spp.sample_configurations(scache, uniforms=uniforms(10 ** 6, B))  # warm-up (tunes this mode's tiles)
             ctx.synchronize()
             t0 = time.perf_counter()
             spp.sample_configurations(scache, uniforms=uniforms(1, B))
@@ -446,7 +460,8 @@ def main():
     # batched FP64: the same batch, FP64 everywhere
     pctx = spp.Context(0, "fp64")
     pcache = spp.SamplingCache(pctx, pepo, args.chi)
-    spp.sample_configurations(pcache, uniforms=uniforms(10 ** 6, 2))  # warm-up
+    spp.sample_configurations(pcache, uniforms=uniforms(10 ** 6, B))  # warm-up (tunes the FP64 tiles)
+    spp.sample_configurations(pcache, uniforms=uniforms(10 ** 6, 1))  # and those of the batch-1 path
     pctx.synchronize()
     t0 = time.perf_counter()
     spp.sample_configurations(pcache, uniforms=uniforms(index, B))
@@ -470,11 +485,13 @@ def main():
     # energy throughput of the headline's drawn samples
     k = min(args.check_samples, B)
     cfg = draws[0]["configuration"][:k]
+    ctx.set_option("autotune", 0)  # timed on its first call: heuristic tile shapes
     ctx.synchronize()
     t0 = time.perf_counter()
     spp.measure_energies(cache, cfg, model)
     ctx.synchronize()
     t_energy = time.perf_counter() - t0
+    ctx.set_option("autotune", 1)
     result["energy_measurement"] = {"value": k / t_energy, "unit": "energy measurements/s",
                                     "dtype": args.precision, "samples": int(k), "chi": args.chi}
     # release the draw workspaces (the batched FP64 one holds ~0.1 TB) so the

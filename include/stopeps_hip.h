@@ -72,7 +72,13 @@ int spp_set_precision(spp_ctx* ctx, int precision);
  * fu_fail_code; a negative sample turns it off); "gram_chunk_bytes" (per-sample
  * byte limit of one Gram transfer intermediate of a boundary absorb: the bra
  * bond is taken in index ranges that keep it below the limit; default 2^30;
- * results do not depend on it beyond summation order). */
+ * results do not depend on it beyond summation order); "autotune" (1, the
+ * default: the contraction kernel's tile shape is tuned per launch class — kernel
+ * family, operand load modes, sample count and the dimension and fixed parity
+ * split of every contracted and free leg — by timing every shape on the first
+ * launch of the class; 0: the padded-area heuristic; -1: forget the tuned
+ * shapes; every shape gives bit-identical results); "gemm_tile" (self-tests:
+ * force one tile shape index of the kernel family on every launch, -1 = off). */
 int spp_set_option(spp_ctx* ctx, const char* name, double value);
 
 /* counters[0] = contraction flops issued (2*M*N*K over parity blocks),
@@ -83,8 +89,9 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value);
  * of the timed FP64-kernel launches, counters[9..11] the same for the FP32
  * kernel; counters[12..14] = iterative truncation eigensolves, their summed
  * subspace iterations and their Chebyshev filter products; counters[15] = full-
- * update bonds repaired by the PSD projection of their hole. Returns the count
- * written. */
+ * update bonds repaired by the PSD projection of their hole; counters[16..18] = launch
+ * classes the tile autotuner tuned, those whose fastest shape differs from the
+ * heuristic's, and the milliseconds spent tuning. Returns the count written. */
 int spp_get_counters(spp_ctx* ctx, double* counters, int n);
 int spp_reset_counters(spp_ctx* ctx);
 

@@ -7,6 +7,7 @@
 // compression routine serves both directions.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <map>
 #include <string>
 #include <vector>
 #include <memory>
@@ -137,6 +138,9 @@ struct Counters {
   // truncation eigensolver: iterative calls, subspace iterations, filter products
   double eig_calls = 0, eig_iterations = 0, eig_filter_gemms = 0;
   double fu_repairs = 0;  // bonds repaired by the PSD projection of the hole
+  // contraction tile autotuner: launch classes tuned, classes whose fastest
+  // shape is not the heuristic's, and the milliseconds the tuning launches took
+  double tuned_classes = 0, tuned_changed = 0, tune_ms = 0;
 };
 
 // Event pairs recorded around contraction launches when profiling is on.
@@ -232,6 +236,18 @@ struct Context {
   // ABI) and its user pointer; null: a bond that needs the repair fails its sample.
   void* fu_repair_fn = nullptr;
   void* fu_repair_user = nullptr;
+  // Tile shapes of the contraction kernel. autotune = 1: the first launch of
+  // each launch class (kernel family, operand load modes, sample count, and
+  // the dimension and fixed parity split of every M, N and K leg, i.e. the
+  // block-size histogram of the launch) times every tile shape of its family
+  // on the real operands and the class keeps the fastest; launches that add
+  // into their output are never repeated, so a class first met there takes
+  // the heuristic shape until a plain launch tunes it. All shapes give
+  // bit-identical results. force_tile >= 0 (self-tests): every launch uses
+  // that shape index.
+  int autotune = 1;
+  int force_tile = -1;
+  std::map<std::vector<long long>, int> tile_table;
   void flush_timer();       // accumulate finished event pairs into counters
 };
 

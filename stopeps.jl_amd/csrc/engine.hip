@@ -10,6 +10,7 @@
 // sequence is: right Grams by four contractions per site, then per cut rho_x,
 // its top-chi eigenvectors over both parity blocks, U_x and R_x = U_x^T Y_x.
 #include "engine.h"
+#include <chrono>
 #include "boundary_ops.h"
 #include <algorithm>
 #include <array>
@@ -274,10 +275,56 @@ static std::vector<Leg> legs_of(const TRef& t, const std::string& letters) {
   return out;
 }
 
+// Times every tile shape of the family on this launch (each one writes the
+// same C; the first run warms the operands) and returns the fastest, keeping
+// the heuristic's shape unless another is faster by more than 2%.
+static int tune_tile(Context& ctx, const GemmArgs& g, int family, int pieces, long long mmax, long long nmax) {
+  const int def = z2gemm_default_tile(family, mmax, nmax);
+  const int count = z2gemm_tile_count(family);
+  const auto t0 = std::chrono::steady_clock::now();
+  SPP_CHECK(launch_z2gemm_tile(g, family, pieces, def, mmax, nmax, ctx.stream));
+  std::vector<hipEvent_t> ev(2 * count);
+  for (auto& e : ev) SPP_CHECK(hipEventCreate(&e));
+  std::vector<int> ok(count, 0);
+  for (int t = 0; t < count; ++t) {
+    SPP_CHECK(hipEventRecord(ev[2 * t], ctx.stream));
+    ok[t] = launch_z2gemm_tile(g, family, pieces, t, mmax, nmax, ctx.stream) == hipSuccess;
+    SPP_CHECK(hipEventRecord(ev[2 * t + 1], ctx.stream));
+  }
+  SPP_CHECK(hipStreamSynchronize(ctx.stream));
+  std::vector<float> ms(count, 0.f);
+  for (int t = 0; t < count; ++t) SPP_CHECK(hipEventElapsedTime(&ms[t], ev[2 * t], ev[2 * t + 1]));
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  int best = def;
+  for (int t = 0; t < count; ++t)
+    if (ok[t] && ms[t] < 0.98f * ms[best]) best = t;
+  ctx.counters.tuned_classes += 1;
+  ctx.counters.tuned_changed += best != def ? 1 : 0;
+  ctx.counters.tune_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return best;
+}
+
+// Tile shape of a launch: forced, tuned for its class (tuning it now when the
+// class is new and the launch may be repeated), or the heuristic's. Calls in
+// no-allocation mode never tune (the timing waits on the stream).
 template <typename T>
-static hipError_t launch_gemm(const Context& ctx, const GemmArgs& g, long long mmax, long long nmax) {
-  if (sizeof(T) == 4 && ctx.split) return launch_z2gemm_split(g, ctx.split, mmax, nmax, ctx.stream);
-  return launch_z2gemm<T>(g, mmax, nmax, ctx.stream);
+static int choose_tile(Context& ctx, const GemmArgs& g, long long mmax, long long nmax,
+                       const std::vector<long long>& cls, int& family, int& pieces) {
+  family = sizeof(T) == 4 ? (ctx.split ? 2 : 1) : 0;
+  pieces = family == 2 ? ctx.split : 0;
+  int tile = z2gemm_default_tile(family, mmax, nmax);
+  if (ctx.force_tile >= 0) {
+    if (ctx.force_tile < z2gemm_tile_count(family)) tile = ctx.force_tile;
+  } else if (ctx.autotune && mmax > 0 && nmax > 0 && g.nsamples > 0) {
+    auto it = ctx.tile_table.find(cls);
+    if (it != ctx.tile_table.end()) {
+      tile = it->second;
+    } else if (!g.accumulate && !ctx.noalloc_dot) {
+      tile = tune_tile(ctx, g, family, pieces, mmax, nmax);
+      ctx.tile_table[cls] = tile;
+    }
+  }
+  return tile;
 }
 
 template <typename T>
@@ -354,12 +401,24 @@ void contract(Context& ctx, const TRef& C, const TRef& A, const TRef& B, int nsa
   for (int q = 0; q < 2; ++q) fl += group_estimate(lm, q) * group_estimate(ln, q) * group_estimate(lk, q);
   ctx.counters.gemm_flops += 2.0 * fl * nsamples;
   ctx.counters.gemm_launches += 1;
+  // launch class of the tile autotuner: family, load modes, samples and the
+  // block structure of every leg (dimension; fixed even count or -1 per sample)
+  std::vector<long long> cls = {(long long)sizeof(T), ctx.split, g.a_mfast, g.b_kfast, nsamples, mmax, nmax};
+  for (const auto* grp : {&lm, &ln, &lk}) {
+    cls.push_back(-2);
+    for (const Leg& l : *grp) {
+      cls.push_back(l.dim);
+      cls.push_back(l.split ? -1 : l.even);
+    }
+  }
+  int family = 0, pieces = 0;
+  const int tile = choose_tile<T>(ctx, g, mmax, nmax, cls, family, pieces);
   if (ctx.profile) {
     hipEvent_t a, b;
     SPP_CHECK(hipEventCreate(&a));
     SPP_CHECK(hipEventCreate(&b));
     SPP_CHECK(hipEventRecord(a, ctx.stream));
-    SPP_CHECK(launch_gemm<T>(ctx, g, mmax, nmax));
+    SPP_CHECK(launch_z2gemm_tile(g, family, pieces, tile, mmax, nmax, ctx.stream));
     SPP_CHECK(hipEventRecord(b, ctx.stream));
     ctx.timer.pending.emplace_back(a, b);
     ctx.timer.pending_flops.push_back(2.0 * fl * nsamples);
@@ -367,7 +426,7 @@ void contract(Context& ctx, const TRef& C, const TRef& A, const TRef& B, int nsa
     if (ctx.timer.pending.size() >= 4096) ctx.flush_timer();
     return;
   }
-  SPP_CHECK(launch_gemm<T>(ctx, g, mmax, nmax));
+  SPP_CHECK(launch_z2gemm_tile(g, family, pieces, tile, mmax, nmax, ctx.stream));
 }
 
 void Context::flush_timer() {

@@ -66,6 +66,19 @@ hipError_t launch_z2gemm(const GemmArgs& args, long long mmax, long long nmax, h
 hipError_t launch_z2gemm_split(const GemmArgs& args, int pieces, long long mmax, long long nmax,
                                hipStream_t stream);
 
+// Block tile shapes by kernel family: 0 = FP64 kernel, 1 = FP32 kernel,
+// 2 = split-product kernel. Every shape of a family computes bit-identical
+// results (the same matrix-core instruction walks K in the same order); they
+// differ in speed only. The default is the padded-area heuristic; the
+// autotuner of the contraction layer may pick another shape per launch class.
+constexpr int kMaxTileShapes = 6;
+int z2gemm_tile_count(int family);
+int z2gemm_default_tile(int family, long long mmax, long long nmax);
+const char* z2gemm_tile_name(int family, int tile);
+// Launch with an explicit tile shape (pieces: the split kernel's BF16 pieces).
+hipError_t launch_z2gemm_tile(const GemmArgs& args, int family, int pieces, int tile, long long mmax,
+                              long long nmax, hipStream_t stream);
+
 // ---------------------------------------------------------------------------
 // Group decoding, shared with the other kernels.
 struct GroupView {

@@ -607,22 +607,22 @@ static hipError_t launch_split_shape(const GemmArgs& args, long long mmax, long 
   return hipGetLastError();
 }
 
+// Split-kernel shapes: 0 = 128x128, 1 = 256x96, 2 = 256x32, 3 = 128x64.
 template <int NP>
-static hipError_t launch_split(const GemmArgs& args, long long mmax, long long nmax, hipStream_t stream) {
-  if (nmax <= 32) return launch_split_shape<256, 32, 4, 2, NP>(args, mmax, nmax, stream);
-  auto padded = [&](long long bm, long long bn) {
-    return ((mmax + bm - 1) / bm) * bm * (((nmax + bn - 1) / bn) * bn);
-  };
-  if (padded(256, 96) < padded(128, 128)) return launch_split_shape<256, 96, 4, 2, NP>(args, mmax, nmax, stream);
-  return launch_split_shape<128, 128, 2, 2, NP>(args, mmax, nmax, stream);
+static hipError_t launch_split_tile(const GemmArgs& args, int tile, long long mmax, long long nmax,
+                                    hipStream_t stream) {
+  switch (tile) {
+    case 0: return launch_split_shape<128, 128, 2, 2, NP>(args, mmax, nmax, stream);
+    case 1: return launch_split_shape<256, 96, 4, 2, NP>(args, mmax, nmax, stream);
+    case 2: return launch_split_shape<256, 32, 4, 2, NP>(args, mmax, nmax, stream);
+    case 3: return launch_split_shape<128, 64, 2, 2, NP>(args, mmax, nmax, stream);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_z2gemm_split(const GemmArgs& args, int pieces, long long mmax, long long nmax,
                                hipStream_t stream) {
-  if (mmax <= 0 || nmax <= 0 || args.nsamples <= 0) return hipSuccess;
-  if (pieces == 2) return launch_split<2>(args, mmax, nmax, stream);
-  if (pieces == 3) return launch_split<3>(args, mmax, nmax, stream);
-  return hipErrorInvalidValue;
+  return launch_z2gemm_tile(args, 2, pieces, z2gemm_default_tile(2, mmax, nmax), mmax, nmax, stream);
 }
 
 // Block tile shapes per element type: (BM, BN, wave rows, K tile, waves per
@@ -662,17 +662,73 @@ static hipError_t launch_shape(const GemmArgs& args, long long mmax, long long n
   return hipGetLastError();
 }
 
+// Shapes 0-2 of a family are the Shapes<T> above; 3 and up are the extra
+// candidates of the autotuner: FP32 3 = 128x64 (2x2 waves, wave tile 64x32),
+// 4 = 256x64 (4x1, 64x64), 5 = 64x128 (1x4, 64x32); FP64 3 = 64x64 (2x2,
+// 32x32), 4 = 128x64 (2x2, 64x32).
 template <typename T>
-hipError_t launch_z2gemm(const GemmArgs& args, long long mmax, long long nmax, hipStream_t stream) {
-  using S = Shapes<T>;
-  if (mmax <= 0 || nmax <= 0 || args.nsamples <= 0) return hipSuccess;
-  if (nmax <= S::N2) return launch_shape<T, S::M2, S::N2, S::W2, S::K2, S::O2, S::D2>(args, mmax, nmax, stream);
+static hipError_t launch_tile_of(const GemmArgs& args, int tile, long long mmax, long long nmax, hipStream_t stream);
+template <>
+hipError_t launch_tile_of<float>(const GemmArgs& args, int tile, long long mmax, long long nmax, hipStream_t stream) {
+  using S = Shapes<float>;
+  switch (tile) {
+    case 0: return launch_shape<float, S::M0, S::N0, S::W0, S::K0, S::O0, S::D0>(args, mmax, nmax, stream);
+    case 1: return launch_shape<float, S::M1, S::N1, S::W1, S::K1, S::O1, S::D1>(args, mmax, nmax, stream);
+    case 2: return launch_shape<float, S::M2, S::N2, S::W2, S::K2, S::O2, S::D2>(args, mmax, nmax, stream);
+    case 3: return launch_shape<float, 128, 64, 2, 16, 3, 0>(args, mmax, nmax, stream);
+    case 4: return launch_shape<float, 256, 64, 4, 16, 3, 0>(args, mmax, nmax, stream);
+    case 5: return launch_shape<float, 64, 128, 1, 16, 3, 0>(args, mmax, nmax, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+template <>
+hipError_t launch_tile_of<double>(const GemmArgs& args, int tile, long long mmax, long long nmax,
+                                  hipStream_t stream) {
+  using S = Shapes<double>;
+  switch (tile) {
+    case 0: return launch_shape<double, S::M0, S::N0, S::W0, S::K0, S::O0, S::D0>(args, mmax, nmax, stream);
+    case 1: return launch_shape<double, S::M1, S::N1, S::W1, S::K1, S::O1, S::D1>(args, mmax, nmax, stream);
+    case 2: return launch_shape<double, S::M2, S::N2, S::W2, S::K2, S::O2, S::D2>(args, mmax, nmax, stream);
+    case 3: return launch_shape<double, 64, 64, 2, 16, 2, 0>(args, mmax, nmax, stream);
+    case 4: return launch_shape<double, 128, 64, 2, 16, 2, 0>(args, mmax, nmax, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int z2gemm_tile_count(int family) { return family == 1 ? 6 : family == 0 ? 5 : 4; }
+
+int z2gemm_default_tile(int family, long long mmax, long long nmax) {
+  // (BM, BN) of shapes 0, 1 and the N <= 32 shape 2 per family
+  const long long m0 = 128, n0 = 128, m1 = family == 0 ? 128 : 256, n1 = 96;
+  if (nmax <= 32) return 2;
   auto padded = [&](long long bm, long long bn) {
     return ((mmax + bm - 1) / bm) * bm * (((nmax + bn - 1) / bn) * bn);
   };
-  if (padded(S::M1, S::N1) < padded(S::M0, S::N0))
-    return launch_shape<T, S::M1, S::N1, S::W1, S::K1, S::O1, S::D1>(args, mmax, nmax, stream);
-  return launch_shape<T, S::M0, S::N0, S::W0, S::K0, S::O0, S::D0>(args, mmax, nmax, stream);
+  return padded(m1, n1) < padded(m0, n0) ? 1 : 0;
+}
+
+const char* z2gemm_tile_name(int family, int tile) {
+  static const char* f64[] = {"128x128", "128x96", "256x32", "64x64", "128x64"};
+  static const char* f32[] = {"128x128", "256x96", "256x32", "128x64", "256x64", "64x128"};
+  static const char* spl[] = {"128x128", "256x96", "256x32", "128x64"};
+  if (tile < 0 || tile >= z2gemm_tile_count(family)) return "?";
+  return family == 0 ? f64[tile] : family == 1 ? f32[tile] : spl[tile];
+}
+
+hipError_t launch_z2gemm_tile(const GemmArgs& args, int family, int pieces, int tile, long long mmax,
+                              long long nmax, hipStream_t stream) {
+  if (mmax <= 0 || nmax <= 0 || args.nsamples <= 0) return hipSuccess;
+  if (family == 0) return launch_tile_of<double>(args, tile, mmax, nmax, stream);
+  if (family == 1) return launch_tile_of<float>(args, tile, mmax, nmax, stream);
+  if (family == 2 && pieces == 2) return launch_split_tile<2>(args, tile, mmax, nmax, stream);
+  if (family == 2 && pieces == 3) return launch_split_tile<3>(args, tile, mmax, nmax, stream);
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_z2gemm(const GemmArgs& args, long long mmax, long long nmax, hipStream_t stream) {
+  const int family = sizeof(T) == 4 ? 1 : 0;
+  return launch_z2gemm_tile(args, family, 0, z2gemm_default_tile(family, mmax, nmax), mmax, nmax, stream);
 }
 
 template hipError_t launch_z2gemm<float>(const GemmArgs&, long long, long long, hipStream_t);
