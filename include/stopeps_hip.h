@@ -102,6 +102,22 @@ int spp_state_upload(spp_ctx* ctx, int Lx, int Ly, int kind, const int* dims, co
                      const double* const* data, double beta, double log_amplitude, spp_state** out);
 int spp_state_free(spp_state* state);
 
+/* `project_configuration(checkpoint, configuration)` (src/sampling/born.jl:27-36)
+ * of n configurations of an uploaded PEPO, on the device: state s gets site i
+ * = sum_a A_i[w,n,p,a,e,s] U[a][k] with k = labels[s][i] (0-based basis index;
+ * basis as in spp_sampling_cache_build, NULL = identity) and a one-index
+ * auxiliary leg of the label's charge, then `normalize_state!`
+ * (src/states/finite.jl:89-100): each site divided by its norm and the logs
+ * (normalizations [n][Lx*Ly], may be NULL) added to the log amplitude
+ * (log_amplitude [n], may be NULL). capacity[2] = (even, odd) bond space of
+ * every interior virtual leg of the new states (NULL: the PEPO's); the PEPO's
+ * live sectors are kept (the first live indices of each parity) and the rest
+ * of the capacity is zero. out[n] receives new state handles (free each with
+ * spp_state_free). Fails with "zero or nonfinite tensor at (x,y)" like the
+ * reference. */
+int spp_project_batch(spp_ctx* ctx, const spp_state* pepo, int n, const int* labels, const double* basis,
+                      const int* capacity, spp_state** out, double* log_amplitude, double* normalizations);
+
 /* Live sectors of every virtual bond (no reference counterpart: the device
  * keeps each leg at its uploaded capacity, and these counts are the bond space
  * the reference would hold after truncation). live: [Lx*Ly][4 legs W,N,E,S][2]

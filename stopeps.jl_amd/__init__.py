@@ -11,7 +11,8 @@ from .states import (Lattice, Site, FiniteState, FinitePEPO, FinitePEPS, identit
                      save_state_npz, load_state_npz,
                      LocalBasis, SymmetryBasis, XBasis, resolve_basis)
 from .sampling import (Context, DeviceState, SamplingCache, BornError, sample_configuration,
-                       sample_configurations, sample_uniforms, project_configuration, splitmix64)
+                       sample_configurations, sample_uniforms, project_configuration, project_configurations,
+                       splitmix64)
 from .measurement import (TFIM, spin_operators, operator_factors, EnergyError, measure_energies, measure_energy,
                           measure_energy_states)
 from .contraction import NormError, lognorm, lognorms, lognorm_gradient, lognorm_gradients, lognorm_rrule
@@ -21,12 +22,12 @@ from .particles import (Particle, ensemble_statistics, step_schedule, purify, st
 from .config import RunConfig, ResamplingOptions, validate_config, temperature_nodes, parameter_dict
 from .driver import ThermalCheckpoint, run_purification, run_stochastic
 from .results import save_result, load_result, save_checkpoint, load_checkpoint, RESULT_SCHEMA_VERSION
-from . import _lib, particles, repair
+from . import _lib, particles, repair, resample
 
 __all__ = ["synthetic_thermal_pepo", "random_peps", "Lattice", "Site", "FiniteState", "FinitePEPO", "FinitePEPS", "identity_pepo", "parity_sorted",
            "LocalBasis", "SymmetryBasis", "XBasis", "resolve_basis", "Context", "DeviceState",
            "SamplingCache", "BornError", "sample_configuration", "sample_configurations", "sample_uniforms",
-           "project_configuration", "splitmix64", "TFIM", "spin_operators", "operator_factors", "EnergyError",
+           "project_configuration", "project_configurations", "splitmix64", "TFIM", "spin_operators", "operator_factors", "EnergyError",
            "measure_energies", "measure_energy", "measure_energy_states", "NormError", "lognorm", "lognorms",
            "lognorm_gradient", "lognorm_gradients", "lognorm_rrule", "pad_bonds", "FullUpdate", "trotter_gate",
            "evolution_step", "evolution_steps", "Particle", "ensemble_statistics", "temperature_nodes",

@@ -56,6 +56,8 @@ _SIGNATURES = {
     "spp_selftest_fu_bond": (ctypes.c_int, [c_int_p, c_double_p, c_double_p, c_double_p, c_double_p,
                                             c_double_p, c_double_p]),
     "spp_state_set_live": (ctypes.c_int, [c_void_p, c_void_p, c_int_p]),
+    "spp_project_batch": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int, c_int_p, c_double_p, c_int_p,
+                                         ctypes.POINTER(c_void_p), c_double_p, c_double_p]),
     "spp_state_download": (ctypes.c_int, [c_void_p, c_void_p, ctypes.POINTER(c_double_p), c_double_p, c_double_p,
                                           c_int_p]),
     "spp_fu_evolution_step": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.POINTER(c_void_p), ctypes.c_int,

@@ -247,6 +247,40 @@ int spp_state_upload(spp_ctx* ctx, int Lx, int Ly, int kind, const int* dims, co
   return rc;
 }
 
+int spp_project_batch(spp_ctx* ctx, const spp_state* pepo, int n, const int* labels, const double* basis,
+                      const int* capacity, spp_state** out, double* log_amplitude, double* normalizations) {
+  return guarded(ctx, [&] {
+    if (!pepo || (n > 0 && (!labels || !out))) throw HipError("null argument");
+    if (n < 0) throw HipError("negative sample count");
+    if (pepo->s.kind != 0) throw HipError("projection needs a PEPO");
+    if (capacity && (capacity[0] < 0 || capacity[1] < 0 || capacity[0] + capacity[1] <= 0))
+      throw HipError("invalid bond capacity");
+    usable(pepo->s);
+    SPP_CHECK(hipSetDevice(ctx->c.device));
+    for (int i = 0; i < n; ++i) out[i] = nullptr;
+    std::vector<StateDev*> made;
+    project_batch(ctx->c, pepo->s, n, labels, basis, capacity, made, normalizations);
+    for (int i = 0; i < n; ++i) {
+      spp_state* h = new spp_state();
+      // move the device state into the handle (the block owns every site)
+      StateDev& d = h->s;
+      d.Lx = made[i]->Lx;
+      d.Ly = made[i]->Ly;
+      d.kind = made[i]->kind;
+      d.beta = made[i]->beta;
+      d.log_amplitude = made[i]->log_amplitude;
+      d.sites = made[i]->sites;
+      d.live = made[i]->live;
+      d.block = made[i]->block;
+      made[i]->block = nullptr;
+      made[i]->sites.clear();
+      delete made[i];
+      out[i] = h;
+      if (log_amplitude) log_amplitude[i] = d.log_amplitude;
+    }
+  });
+}
+
 int spp_state_set_live(spp_ctx* ctx, spp_state* state, const int* live) {
   return guarded(ctx, [&] {
     if (!state || !live) throw HipError("null argument");

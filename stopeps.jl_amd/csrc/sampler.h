@@ -35,6 +35,9 @@ struct StateDev {
   // nonzero: a full-update step failed on this state (its failure code); the
   // compute entry points refuse it
   int failed = 0;
+  // one device block holding every site (sites[i].d point into it; freed as a
+  // whole), or null when each site owns its own allocation
+  double* block = nullptr;
   ~StateDev();
 };
 
@@ -127,6 +130,16 @@ struct CacheDev {
   std::vector<int> reach;
   ~CacheDev();
 };
+
+// `project_configuration` (born.jl:27-36) of n configurations of a resident
+// PEPO on the device: site i of state s is sum_a A_i[w,n,p,a,e,s] U[a,k] with
+// k = labels[s][i] (a one-index auxiliary leg of the label's charge), its
+// interior bonds re-embedded in the capacity (even, odd) = cap (the PEPO's own
+// when cap is null: the first live indices of each parity kept, zero slots
+// appended), then `normalize_state!` (finite.jl:89-100): every site divided by
+// its norm, whose logs (normalizations [n][ns]) add to the log amplitude.
+void project_batch(Context& ctx, const StateDev& pepo, int n, const int* labels, const double* basis, const int* cap,
+                   std::vector<StateDev*>& out, double* normalizations);
 
 // kind 0: auxiliary kind of a PEPO; kind 1: physical kind of a PEPS.
 template <typename T>

@@ -14,12 +14,17 @@
 #include "boundary_ops.h"
 #include "tridiag.h"
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 
 namespace spp {
 
 StateDev::~StateDev() {
+  if (block) {
+    (void)hipFree(block);
+    return;
+  }
   for (auto& s : sites)
     if (s.d) (void)hipFree(s.d);
 }
@@ -75,6 +80,198 @@ __global__ void proj_lognorm_kernel(const double* a, int n_before, int naux, int
     out[0] = 0.5 * log(part[0][0]);
     out[1] = 0.5 * log(part[1][0]);
   }
+}
+
+// One (state, site) of project_batch: source PEPO site and destination PEPS
+// site (dense (W,N,P,A,E,S)), the dimensions of both, the basis column of the
+// site's label and, per virtual leg, the source index of every destination
+// index (-1: a zero slot of the larger capacity).
+struct ProjectJob {
+  const double* src;
+  double* dst;
+  int sd[6], dd[6];
+  int map_off[4];  // offset of each virtual leg's map in the job's map table
+  const int* maps;
+  double u[2];
+};
+
+// dst = the projected site; nrm2[job] = its squared norm (one workgroup per job).
+__global__ void project_site_kernel(const ProjectJob* jobs, double* nrm2) {
+  const ProjectJob& J = jobs[blockIdx.x];
+  const int dW = J.dd[0], dN = J.dd[1], dP = J.dd[2], dE = J.dd[4], dS = J.dd[5];
+  const long long total = (long long)dW * dN * dP * dE * dS;
+  const int sA = J.sd[3];
+  long long st[6];
+  st[5] = 1;
+  for (int k = 4; k >= 0; --k) st[k] = st[k + 1] * J.sd[k + 1];
+  double acc = 0.0;
+  for (long long t = threadIdx.x; t < total; t += blockDim.x) {
+    long long r = t;
+    const int is = (int)(r % dS); r /= dS;
+    const int ie = (int)(r % dE); r /= dE;
+    const int ip = (int)(r % dP); r /= dP;
+    const int in = (int)(r % dN); r /= dN;
+    const int iw = (int)r;
+    const int w = J.maps[J.map_off[0] + iw], n = J.maps[J.map_off[1] + in];
+    const int e = J.maps[J.map_off[2] + ie], sv = J.maps[J.map_off[3] + is];
+    double v = 0.0;
+    if (w >= 0 && n >= 0 && e >= 0 && sv >= 0) {
+      const long long base = w * st[0] + n * st[1] + ip * st[2] + e * st[4] + sv * st[5];
+      for (int a = 0; a < sA; ++a) v += J.src[base + a * st[3]] * J.u[a];
+    }
+    J.dst[t] = v;
+    acc += v * v;
+  }
+  __shared__ double part[256];
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) part[threadIdx.x] += part[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) nrm2[blockIdx.x] = part[0];
+}
+
+__global__ void project_scale_kernel(const ProjectJob* jobs, const double* nrm2) {
+  const ProjectJob& J = jobs[blockIdx.x];
+  const long long total = (long long)J.dd[0] * J.dd[1] * J.dd[2] * J.dd[4] * J.dd[5];
+  const double f = 1.0 / sqrt(nrm2[blockIdx.x]);
+  for (long long t = threadIdx.x; t < total; t += blockDim.x) J.dst[t] *= f;
+}
+
+void project_batch(Context& ctx, const StateDev& pepo, int n, const int* labels, const double* basis, const int* cap,
+                   std::vector<StateDev*>& out, double* normalizations) {
+  hipStream_t st = ctx.stream;
+  const int Lx = pepo.Lx, Ly = pepo.Ly, ns = Lx * Ly;
+  const double U[4] = {basis ? basis[0] : 1.0, basis ? basis[1] : 0.0, basis ? basis[2] : 0.0,
+                       basis ? basis[3] : 1.0};
+  bool aux_graded = false;
+  for (const SiteDev& s : pepo.sites) {
+    if (s.dim[3] != 2) throw HipError("projection needs a PEPO with a two-index auxiliary leg");
+    aux_graded = aux_graded || s.even[3] < s.dim[3];
+  }
+  const int legs[4] = {0, 1, 4, 5};
+  // per site: destination leg dims / evens and the index maps of its virtual legs
+  std::vector<std::array<int, 6>> ddim(ns), deven(ns);
+  std::vector<int> maps;
+  std::vector<std::array<int, 4>> moff(ns);
+  for (int i = 0; i < ns; ++i) {
+    const SiteDev& s = pepo.sites[i];
+    const int x = i % Lx, y = i / Lx;
+    for (int k = 0; k < 6; ++k) {
+      ddim[i][k] = s.dim[k];
+      deven[i][k] = s.even[k];
+    }
+    ddim[i][3] = 1;
+    for (int v = 0; v < 4; ++v) {
+      const int k = legs[v];
+      const bool edge = (k == 0 && x == 0) || (k == 1 && y == 0) || (k == 4 && x == Lx - 1) || (k == 5 && y == Ly - 1);
+      moff[i][v] = (int)maps.size();
+      if (edge || !cap) {
+        for (int j = 0; j < s.dim[k]; ++j) maps.push_back(j);
+        continue;
+      }
+      const int le = pepo.live[(size_t)i * 8 + 2 * v], lo = pepo.live[(size_t)i * 8 + 2 * v + 1];
+      if (le > cap[0] || lo > cap[1]) throw HipError("live bond sectors exceed the capacity");
+      ddim[i][k] = cap[0] + cap[1];
+      deven[i][k] = cap[0];
+      for (int j = 0; j < cap[0]; ++j) maps.push_back(j < le ? j : -1);
+      for (int j = 0; j < cap[1]; ++j) maps.push_back(j < lo ? s.even[k] + j : -1);
+    }
+  }
+  std::vector<long long> dsize(ns), doff(ns + 1, 0);
+  for (int i = 0; i < ns; ++i) {
+    dsize[i] = 1;
+    for (int k = 0; k < 6; ++k) dsize[i] *= ddim[i][k];
+    doff[i + 1] = doff[i] + dsize[i];
+  }
+  int* dmaps = nullptr;
+  ProjectJob* djobs = nullptr;
+  double* nrm2 = nullptr;
+  std::vector<StateDev*> made;
+  auto cleanup = [&]() {
+    for (void* p : {(void*)dmaps, (void*)djobs, (void*)nrm2})
+      if (p) (void)hipFree(p);
+  };
+  try {
+    SPP_CHECK(hipMalloc(&dmaps, sizeof(int) * std::max<size_t>(maps.size(), 1)));
+    upload(dmaps, maps.data(), sizeof(int) * maps.size(), st);
+    std::vector<ProjectJob> jobs((size_t)n * ns);
+    for (int s0 = 0; s0 < n; ++s0) {
+      StateDev* d = new StateDev();
+      made.push_back(d);
+      d->Lx = Lx;
+      d->Ly = Ly;
+      d->kind = 1;
+      d->beta = pepo.beta;
+      d->log_amplitude = pepo.log_amplitude;
+      d->sites.resize(ns);
+      SPP_CHECK(hipMalloc(&d->block, sizeof(double) * doff[ns]));
+      d->live.assign((size_t)ns * 8, 0);
+      for (int i = 0; i < ns; ++i) {
+        const int k = labels[(size_t)s0 * ns + i];
+        if (k != 0 && k != 1) throw HipError("labels must be 0 or 1");
+        SiteDev& sd = d->sites[i];
+        sd.d = d->block + doff[i];
+        for (int q = 0; q < 6; ++q) {
+          sd.dim[q] = ddim[i][q];
+          sd.even[q] = deven[i][q];
+        }
+        // the auxiliary index carries the label's charge (Z2), even otherwise
+        sd.even[3] = aux_graded && k == 1 ? 0 : 1;
+        // the live sectors stay those of the PEPO (its live indices come first)
+        for (int v = 0; v < 8; ++v) d->live[(size_t)i * 8 + v] = pepo.live[(size_t)i * 8 + v];
+        ProjectJob& J = jobs[(size_t)s0 * ns + i];
+        J.src = pepo.sites[i].d;
+        J.dst = sd.d;
+        for (int q = 0; q < 6; ++q) {
+          J.sd[q] = pepo.sites[i].dim[q];
+          J.dd[q] = ddim[i][q];
+        }
+        for (int v = 0; v < 4; ++v) J.map_off[v] = moff[i][v];
+        J.maps = dmaps;
+        J.u[0] = U[0 * 2 + k];
+        J.u[1] = U[1 * 2 + k];
+      }
+    }
+    SPP_CHECK(hipMalloc(&djobs, sizeof(ProjectJob) * std::max<size_t>(jobs.size(), 1)));
+    SPP_CHECK(hipMalloc(&nrm2, sizeof(double) * std::max<size_t>(jobs.size(), 1)));
+    upload(djobs, jobs.data(), sizeof(ProjectJob) * jobs.size(), st);
+    const int nj = (int)jobs.size();
+    for (int j0 = 0; j0 < nj; j0 += 65535) {
+      const int cnt = std::min(65535, nj - j0);
+      hipLaunchKernelGGL(project_site_kernel, dim3(cnt), dim3(256), 0, st, djobs + j0, nrm2 + j0);
+      SPP_CHECK(hipGetLastError());
+    }
+    std::vector<double> h(nj);
+    SPP_CHECK(hipMemcpyAsync(h.data(), nrm2, sizeof(double) * nj, hipMemcpyDeviceToHost, st));
+    SPP_CHECK(hipStreamSynchronize(st));
+    for (int j = 0; j < nj; ++j)
+      if (!(std::isfinite(h[j]) && h[j] > 0.0)) {
+        const int i = j % ns;
+        throw HipError("zero or nonfinite tensor at (" + std::to_string(i % Lx + 1) + "," +
+                       std::to_string(i / Lx + 1) + ")");
+      }
+    for (int j0 = 0; j0 < nj; j0 += 65535) {
+      const int cnt = std::min(65535, nj - j0);
+      hipLaunchKernelGGL(project_scale_kernel, dim3(cnt), dim3(256), 0, st, djobs + j0, nrm2 + j0);
+      SPP_CHECK(hipGetLastError());
+    }
+    for (int s0 = 0; s0 < n; ++s0)
+      for (int i = 0; i < ns; ++i) {
+        const double ln = 0.5 * std::log(h[(size_t)s0 * ns + i]);
+        made[s0]->log_amplitude += ln;
+        if (normalizations) normalizations[(size_t)s0 * ns + i] = ln;
+      }
+    SPP_CHECK(hipStreamSynchronize(st));
+  } catch (...) {
+    (void)hipStreamSynchronize(st);
+    for (StateDev* d : made) delete d;
+    cleanup();
+    throw;
+  }
+  cleanup();
+  out = made;
 }
 
 // Born conditional and categorical draw for one site, all samples.

@@ -18,7 +18,8 @@ import numpy as np
 from .contraction import lognorms
 from .evolution import FullUpdate, evolution_steps
 from .measurement import measure_energies, measure_energy_states
-from .sampling import DeviceState, SamplingCache, project_configuration, sample_configurations, sample_uniforms
+from .sampling import (DeviceState, SamplingCache, project_configurations, sample_configurations,
+                       sample_uniforms)
 from .states import normalize_state, pad_bonds, reembed_bonds, resolve_basis
 
 EPS = np.finfo(np.float64).eps
@@ -190,26 +191,17 @@ def project_particles(ctx, cache, configurations, ids, counts, log_p0, log_q, sa
                       chi_measurement, energy, bond_space=None, pepo_live=None, seed=0, generation=0,
                       record_entropy=False, measurement_space=None):
     """``_new_particle`` (particles.jl:42-53) for each given configuration:
-    project the PEPO of ``cache`` (host slice + ``normalize_state!``), upload
-    it as a resident PEPS (bonds embedded in ``bond_space`` = (even, odd)
-    capacities when given, live sectors ``pepo_live`` [Lx*Ly, 4, 2] of the
-    PEPO, the PEPO's zero slots dropped), observe the batch and set
+    project the resident PEPO of ``cache`` on the device
+    (``project_configurations``: aux projection and ``normalize_state!``,
+    bonds re-embedded in ``bond_space`` = (even, odd) capacities when given,
+    with the PEPO's live sectors), observe the batch and set
     relative_log_weight = log_norm - log_q. When ``energy`` is set the
     projected samples are measured through the sampling cache's shared layers
-    (the same values as measuring the uploaded states)."""
-    pepo = cache.state
-    states = []
-    for cfg in configurations:
-        st, _ = project_configuration(pepo, cfg, cache.basis)
-        live = pepo_live
-        if bond_space is not None:
-            if pepo_live is None:
-                st, live = pad_bonds(st, bond_space)
-            else:
-                st, live = reembed_bonds(st, pepo_live, bond_space)
-        states.append(DeviceState(ctx, st, live=live))
-    if not states:
+    (the same values as measuring the projected states). ``pepo_live`` is
+    accepted for compatibility; the resident PEPO's live sectors are used."""
+    if len(configurations) == 0:
         return []
+    states, _ = project_configurations(ctx, cache._dev, np.asarray(configurations), cache.basis, bond_space)
     t0 = time.perf_counter()
     if energy:
         m = measure_energies(cache, np.asarray(configurations), model, chi_measurement,
@@ -381,15 +373,16 @@ def stochastic_ensemble(ctx, pepo, model, beta0, beta_log, beta_measure, delta_t
     method = method or FullUpdate()
     live = None
     if isinstance(pepo, DeviceState):
+        # the cache reads the resident PEPO (its live sectors, advanced beta and log amplitude)
         live = pepo.live()
-        host = pepo.download()
+        host = pepo.state
     else:
         host = pepo
     if not same_beta(host.beta, beta0):
         raise ValueError("checkpoint beta0 mismatch")
     L = host.lattice
     ns = L.Lx * L.Ly
-    cache = SamplingCache(ctx, host, chi_sampling, basis=resolve_basis(host, basis), sector_space=sampling_space)
+    cache = SamplingCache(ctx, pepo, chi_sampling, basis=resolve_basis(host, basis), sector_space=sampling_space)
     if uniforms is None:
         uniforms = np.stack([sample_uniforms(seed, i, ns) for i in range(1, Ns + 1)])
     draws = sample_configurations(cache, uniforms=uniforms)

@@ -119,7 +119,20 @@ class Context:
 
 
 class DeviceState:
-    """A ``FiniteState`` uploaded to the device (``spp_state_upload``)."""
+    """A ``FiniteState`` uploaded to the device (``spp_state_upload``), or a
+    state the device created (``adopt``)."""
+
+    @classmethod
+    def adopt(cls, ctx, handle, shell):
+        """Wrap a device-created state handle; ``shell`` is a ``FiniteState``
+        with the state's lattice, kind, beta, log amplitude and parity-sorted
+        leg spaces (its arrays are zero-stride placeholders)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.state = shell
+        self._arrays = [shell[(x, y)].data for (x, y) in shell.lattice.site_order()]
+        self.handle = handle
+        return self
 
     def __init__(self, ctx, state, live=None):
         self.ctx = ctx
@@ -385,6 +398,52 @@ def sample_configuration(cache, uniforms=None, configuration=None):
     repairs = repair_records(cache, r["repairs"][0]) if "repairs" in r else []
     return {"configuration": r["configuration"][0], "log_probability": lp, "conditionals": cond,
             "max_environment_truncation": float(r["max_environment_truncation"][0]), "repairs": repairs}
+
+
+def project_configurations(ctx, pepo, configurations, basis=None, bond_space=None):
+    """``project_configuration`` (born.jl:27-36) of n configurations of a
+    resident PEPO (``DeviceState``) on the device (``spp_project_batch``):
+    returns the normalised projected PEPS as resident ``DeviceState`` objects
+    (parity-sorted legs, interior bonds in ``bond_space`` = (even, odd) when
+    given, the PEPO's live sectors kept) and the normalisation logs [n, Lx*Ly]."""
+    host = pepo.state
+    L = host.lattice
+    ns = L.Lx * L.Ly
+    basis = resolve_basis(host, basis)
+    cfg = np.asarray(configurations)
+    n = cfg.shape[0]
+    idx = np.vectorize(basis.index)(cfg) if n else np.zeros((0, L.Lx, L.Ly), dtype=np.int64)
+    labels = np.ascontiguousarray(np.transpose(idx, (0, 2, 1)).reshape(n, ns), dtype=np.int32)
+    U = np.ascontiguousarray(basis.U, dtype=np.float64)
+    cap = None if bond_space is None else np.ascontiguousarray([int(v) for v in bond_space], dtype=np.int32)
+    handles = (ctypes.c_void_p * max(n, 1))()
+    lamp = np.zeros(n)
+    norms = np.zeros((n, ns))
+    _lib.check(ctx.handle, ctx._lib.spp_project_batch(ctx.handle, pepo.handle, n, _lib.iptr(labels), _lib.dptr(U),
+                                                      _lib.iptr(cap), handles, _lib.dptr(lamp), _lib.dptr(norms)))
+    out = []
+    aux_graded = bool(np.any(host[(0, 0)].parity[3]))
+    for s_ in range(n):
+        sites = [[None] * L.Ly for _ in range(L.Lx)]
+        for (x, y) in L.site_order():
+            A = host[(x, y)]
+            par = []
+            for k in range(6):
+                e = int(np.sum(A.parity[k] == 0))
+                o = len(A.parity[k]) - e
+                edge = (k == 0 and x == 0) or (k == 1 and y == 0) or (k == 4 and x == L.Lx - 1) or \
+                    (k == 5 and y == L.Ly - 1)
+                if k in (0, 1, 4, 5) and cap is not None and not edge:
+                    e, o = int(cap[0]), int(cap[1])
+                if k == 3:
+                    q = basis.charges[labels[s_, x + y * L.Lx]] if aux_graded else 0
+                    e, o = (0, 1) if q else (1, 0)
+                par.append(np.array([0] * e + [1] * o))
+            shape = tuple(len(p) for p in par)
+            sites[x][y] = Site(np.broadcast_to(np.zeros(1), shape), par)
+        shell = FiniteState(L, sites, "peps", beta=host.beta, log_amplitude=float(lamp[s_]))
+        out.append(DeviceState.adopt(ctx, ctypes.c_void_p(handles[s_]), shell))
+    return out, norms
 
 
 def project_configuration(pepo, configuration, basis=None):
