@@ -77,7 +77,11 @@ int spp_set_precision(spp_ctx* ctx, int precision);
  * family, operand load modes, sample count and the dimension and fixed parity
  * split of every contracted and free leg — by timing every shape on the first
  * launch of the class; 0: the padded-area heuristic; -1: forget the tuned
- * shapes; every shape gives bit-identical results); "gemm_tile" (self-tests:
+ * shapes; every shape gives bit-identical results); "graphs" (1, the default: a
+ * sampling cache keeps its draw buffers between calls, and each row's
+ * environment-and-draw sweep is captured as a HIP graph the second time it runs
+ * with the same shape and replayed afterwards; 0: launched kernel by kernel;
+ * off while "profile" is on); "gemm_tile" (self-tests:
  * force one tile shape index of the kernel family on every launch, -1 = off). */
 int spp_set_option(spp_ctx* ctx, const char* name, double value);
 
@@ -91,7 +95,8 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value);
  * subspace iterations and their Chebyshev filter products; counters[15] = full-
  * update bonds repaired by the PSD projection of their hole; counters[16..18] = launch
  * classes the tile autotuner tuned, those whose fastest shape differs from the
- * heuristic's, and the milliseconds spent tuning. Returns the count written. */
+ * heuristic's, and the milliseconds spent tuning; counters[19..20] = draw row sweeps
+ * captured as HIP graphs and graph replays. Returns the count written. */
 int spp_get_counters(spp_ctx* ctx, double* counters, int n);
 int spp_reset_counters(spp_ctx* ctx);
 

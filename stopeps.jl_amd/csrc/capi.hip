@@ -166,6 +166,9 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value) {
         throw HipError("autotune must be 1 (on), 0 (off) or -1 (forget the tuned shapes)");
       if (value == -1) ctx->c.tile_table.clear();
       else ctx->c.autotune = (int)value;
+    } else if (n == "graphs") {
+      if (value != 0 && value != 1) throw HipError("graphs must be 0 or 1");
+      ctx->c.graphs = (int)value;
     } else if (n == "gemm_tile") {
       if (value < -1 || value >= kMaxTileShapes) throw HipError("gemm_tile must lie in [-1, 5]");
       ctx->c.force_tile = (int)value;
@@ -179,14 +182,14 @@ int spp_get_counters(spp_ctx* ctx, double* counters, int n) {
   if (!ctx || !counters) return -3;
   if (guarded(ctx, [&] { ctx->c.flush_timer(); }) != 0) return -1;
   const Counters& k = ctx->c.counters;
-  double v[19] = {k.gemm_flops,        k.gemm_launches,    (double)ctx->c.last_iterations,
+  double v[21] = {k.gemm_flops,        k.gemm_launches,    (double)ctx->c.last_iterations,
                   (double)ctx->c.arena.peak(), k.gemm_ms, k.gemm_flops_profiled,
                   k.prec_launches[0],  k.prec_ms[0],       k.prec_flops[0],
                   k.prec_launches[1],  k.prec_ms[1],       k.prec_flops[1],
                   k.eig_calls,         k.eig_iterations,   k.eig_filter_gemms,
                   k.fu_repairs,        k.tuned_classes,    k.tuned_changed,
-                  k.tune_ms};
-  int m = n < 19 ? n : 19;
+                  k.tune_ms,           k.graph_captures,   k.graph_replays};
+  int m = n < 21 ? n : 21;
   for (int i = 0; i < m; ++i) counters[i] = v[i];
   return m;
 }

@@ -47,6 +47,7 @@ class Arena {
   void release(size_t m) { top_ = m; }
   size_t capacity() const { return cap_; }
   size_t peak() const { return peak_; }
+  const char* base() const { return base_; }
   // Serve take() from caller-provided device memory [base, base + bytes)
   // until restore(); reserve() then only checks the size (no hipMalloc or
   // hipFree while borrowed). The owned block is kept and comes back on restore().
@@ -141,6 +142,8 @@ struct Counters {
   // contraction tile autotuner: launch classes tuned, classes whose fastest
   // shape is not the heuristic's, and the milliseconds the tuning launches took
   double tuned_classes = 0, tuned_changed = 0, tune_ms = 0;
+  // draw row sweeps captured as HIP graphs, and graph replays
+  double graph_captures = 0, graph_replays = 0;
 };
 
 // Event pairs recorded around contraction launches when profiling is on.
@@ -248,6 +251,11 @@ struct Context {
   int autotune = 1;
   int force_tile = -1;
   std::map<std::vector<long long>, int> tile_table;
+  // HIP graphs of the draw's row sweeps (1, the default): the second time a
+  // row sweep of one shape runs on a sampling cache it is captured, and later
+  // runs replay the graph; 0: every sweep is launched kernel by kernel.
+  int graphs = 1;
+  bool capturing = false;  // a row sweep is being captured: no tuning launches
   void flush_timer();       // accumulate finished event pairs into counters
 };
 

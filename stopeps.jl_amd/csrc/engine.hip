@@ -319,7 +319,7 @@ static int choose_tile(Context& ctx, const GemmArgs& g, long long mmax, long lon
     auto it = ctx.tile_table.find(cls);
     if (it != ctx.tile_table.end()) {
       tile = it->second;
-    } else if (!g.accumulate && !ctx.noalloc_dot) {
+    } else if (!g.accumulate && !ctx.noalloc_dot && !ctx.capturing) {
       tile = tune_tile(ctx, g, family, pieces, mmax, nmax);
       ctx.tile_table[cls] = tile;
     }

@@ -1,6 +1,8 @@
 // Perfect Born sampling of PEPO auxiliary indices, batched over samples.
 #pragma once
 #include "engine.h"
+#include <map>
+#include <memory>
 
 namespace spp {
 
@@ -107,6 +109,33 @@ __device__ inline int categorical_index(const double* p, int npos, double u) {
   return last;
 }
 
+// Device buffers of the batched draw, kept with the sampling cache between
+// calls (sized for nb samples per chunk), and the captured row sweeps: every
+// row's right-environment and draw sweep is a fixed sequence of launches over
+// these buffers, the context's workspace arena and its dot-product scratch, so
+// it is captured once per key (row, samples in the chunk, mode, bond
+// dimensions) and replayed while those addresses stay the same.
+template <typename T>
+struct DrawWork {
+  int nb = 0, ns = 0, lx = 0;
+  long long capmax = 0;
+  int *choice = nullptr, *labels = nullptr, *status = nullptr, *fail = nullptr, *done = nullptr, *prefix = nullptr;
+  int* dlab = nullptr;
+  double *logp = nullptr, *cond = nullptr, *raw0 = nullptr, *raw1 = nullptr, *nrm = nullptr, *fac = nullptr;
+  double *lnamp = nullptr, *mant = nullptr, *mtlive = nullptr, *drep = nullptr, *duni = nullptr;
+  void** ptrs = nullptr;
+  long long* dsst = nullptr;
+  T** dtens = nullptr;
+  Boundary<T> NB[2];
+  std::map<std::vector<long long>, hipGraphExec_t> graphs;  // null exec: capture failed, run eagerly
+  std::map<std::vector<long long>, int> seen;
+  const char* arena_base = nullptr;  // addresses the graphs were captured with
+  const double* dot_base = nullptr;
+  void allocate(int nb_, int ns_, int lx, long long capmax_, bool physical);
+  void drop_graphs();
+  ~DrawWork();
+};
+
 template <typename T>
 struct CacheDev {
   int Lx = 0, Ly = 0, chi = 0, precision = 0;
@@ -128,6 +157,7 @@ struct CacheDev {
   int psd_ld = 0;
   int target_charge = -1;
   std::vector<int> reach;
+  std::unique_ptr<DrawWork<T>> work;  // draw buffers and captured sweeps (first draw on)
   ~CacheDev();
 };
 

@@ -636,6 +636,58 @@ template void build_cache<float>(Context&, const StateDev&, int, const double*, 
 template void build_cache<double>(Context&, const StateDev&, int, const double*, CacheDev<double>&, int);
 
 template <typename T>
+void DrawWork<T>::allocate(int nb_, int ns_, int lx_, long long capmax_, bool physical) {
+  nb = nb_;
+  ns = ns_;
+  lx = lx_;
+  capmax = capmax_;
+  SPP_CHECK(hipMalloc(&choice, sizeof(int) * nb));
+  SPP_CHECK(hipMalloc(&labels, sizeof(int) * nb * ns));
+  SPP_CHECK(hipMalloc(&status, sizeof(int) * nb));
+  SPP_CHECK(hipMalloc(&fail, sizeof(int) * nb));
+  SPP_CHECK(hipMalloc(&done, sizeof(int) * nb));
+  SPP_CHECK(hipMalloc(&prefix, sizeof(int) * nb));
+  SPP_CHECK(hipMalloc(&dlab, sizeof(int) * nb * ns));
+  SPP_CHECK(hipMalloc(&logp, sizeof(double) * nb));
+  SPP_CHECK(hipMalloc(&cond, sizeof(double) * nb * ns));
+  SPP_CHECK(hipMalloc(&raw0, sizeof(double) * nb));
+  SPP_CHECK(hipMalloc(&raw1, sizeof(double) * nb));
+  SPP_CHECK(hipMalloc(&nrm, sizeof(double) * nb));
+  SPP_CHECK(hipMalloc(&fac, sizeof(double) * nb));
+  SPP_CHECK(hipMalloc(&lnamp, sizeof(double) * nb));
+  SPP_CHECK(hipMalloc(&mant, sizeof(double) * nb));
+  SPP_CHECK(hipMalloc(&mtlive, sizeof(double) * nb));
+  SPP_CHECK(hipMalloc(&duni, sizeof(double) * nb * ns));
+  if (physical) SPP_CHECK(hipMalloc(&drep, sizeof(double) * (size_t)nb * ns * 6));
+  SPP_CHECK(hipMalloc(&ptrs, sizeof(void*) * nb * lx));
+  SPP_CHECK(hipMalloc(&dsst, sizeof(long long) * lx));
+  SPP_CHECK(hipMalloc(&dtens, sizeof(T*) * lx));
+  prepare_boundary(NB[0], lx, nb, capmax);
+  prepare_boundary(NB[1], lx, nb, capmax);
+}
+
+template <typename T>
+void DrawWork<T>::drop_graphs() {
+  for (auto& g : graphs)
+    if (g.second) (void)hipGraphExecDestroy(g.second);
+  graphs.clear();
+  seen.clear();
+}
+
+template <typename T>
+DrawWork<T>::~DrawWork() {
+  drop_graphs();
+  NB[0].release();
+  NB[1].release();
+  for (void* p : {(void*)choice, (void*)labels, (void*)status, (void*)fail, (void*)done, (void*)prefix, (void*)dlab,
+                  (void*)logp, (void*)cond, (void*)raw0, (void*)raw1, (void*)nrm, (void*)fac, (void*)lnamp,
+                  (void*)mant, (void*)mtlive, (void*)drep, (void*)duni, (void*)ptrs, (void*)dsst, (void*)dtens})
+    if (p) (void)hipFree(p);
+}
+template struct DrawWork<float>;
+template struct DrawWork<double>;
+
+template <typename T>
 void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, const int* labels_in,
                 DrawOutputs& out) {
   hipStream_t st = ctx.stream;
@@ -653,43 +705,35 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
   chunk = std::min<long long>(chunk, 16384);
   ctx.arena.reserve(per * chunk);
   const int nb = (int)chunk;
-  // persistent per-chunk state
+  const bool phys = c.kind == 1;
+  // buffers kept with the cache (reallocated when a larger chunk comes)
   long long capmax = 1;
   for (auto& sp : c.spec) capmax = std::max<long long>(capmax, (long long)chi * sp.s.dim * chi);
-  Boundary<T> NB[2];
-  prepare_boundary(NB[0], Lx, nb, capmax);
-  prepare_boundary(NB[1], Lx, nb, capmax);
-  int *choice = nullptr, *labels = nullptr, *status = nullptr, *fail = nullptr, *done = nullptr;
-  double *logp = nullptr, *cond = nullptr, *raw0 = nullptr, *raw1 = nullptr, *nrm = nullptr, *fac = nullptr;
-  double *lnamp = nullptr, *mant = nullptr;
-  void** ptrs = nullptr;
-  int* dlabels_in = nullptr;
-  double* duni = nullptr;
-  long long* dsst = nullptr;
-  T** dtens = nullptr;
-  SPP_CHECK(hipMalloc(&choice, sizeof(int) * nb));
-  SPP_CHECK(hipMalloc(&labels, sizeof(int) * nb * ns));
-  SPP_CHECK(hipMalloc(&status, sizeof(int) * nb));
-  SPP_CHECK(hipMalloc(&fail, sizeof(int) * nb));
-  SPP_CHECK(hipMalloc(&done, sizeof(int) * nb));
-  SPP_CHECK(hipMalloc(&logp, sizeof(double) * nb));
-  SPP_CHECK(hipMalloc(&cond, sizeof(double) * nb * ns));
-  SPP_CHECK(hipMalloc(&raw0, sizeof(double) * nb));
-  SPP_CHECK(hipMalloc(&raw1, sizeof(double) * nb));
-  SPP_CHECK(hipMalloc(&nrm, sizeof(double) * nb));
-  SPP_CHECK(hipMalloc(&fac, sizeof(double) * nb));
-  SPP_CHECK(hipMalloc(&lnamp, sizeof(double) * nb));
-  SPP_CHECK(hipMalloc(&mant, sizeof(double) * nb));
-  SPP_CHECK(hipMalloc(&ptrs, sizeof(void*) * nb * Lx));
-  SPP_CHECK(hipMalloc(&dsst, sizeof(long long) * Lx));
-  SPP_CHECK(hipMalloc(&dtens, sizeof(T*) * Lx));
-  const bool phys = c.kind == 1;
-  int* prefix = nullptr;
-  double* mtlive = nullptr;
-  double* drep = nullptr;
-  SPP_CHECK(hipMalloc(&prefix, sizeof(int) * nb));
-  SPP_CHECK(hipMalloc(&mtlive, sizeof(double) * nb));
-  if (phys) SPP_CHECK(hipMalloc(&drep, sizeof(double) * (size_t)nb * ns * 6));
+  if (!c.work || c.work->nb < nb || c.work->capmax < capmax) {
+    c.work.reset();
+    c.work.reset(new DrawWork<T>());
+    c.work->allocate(nb, ns, Lx, capmax, phys);
+  }
+  DrawWork<T>& W = *c.work;
+  (void)dot_partials(ctx, nb);  // grown now, outside any captured sweep
+  if (W.arena_base != ctx.arena.base() || W.dot_base != ctx.dot_part.p) {
+    W.drop_graphs();
+    W.arena_base = ctx.arena.base();
+    W.dot_base = ctx.dot_part.p;
+  }
+  const bool use_graphs = ctx.graphs && !ctx.profile && !ctx.noalloc_dot;
+  Boundary<T>* NB = W.NB;
+  int *choice = W.choice, *labels = W.labels, *status = W.status, *fail = W.fail, *done = W.done;
+  double *logp = W.logp, *cond = W.cond, *raw0 = W.raw0, *raw1 = W.raw1, *nrm = W.nrm, *fac = W.fac;
+  double *lnamp = W.lnamp, *mant = W.mant;
+  void** ptrs = W.ptrs;
+  int* dlabels_in = labels_in ? W.dlab : nullptr;
+  double* duni = uniforms ? W.duni : nullptr;
+  long long* dsst = W.dsst;
+  T** dtens = W.dtens;
+  int* prefix = W.prefix;
+  double* mtlive = W.mtlive;
+  double* drep = W.drep;
   // evaluation mode of a charged physical cache: a configuration outside the
   // state's total charge has log p = -Inf before any contraction (born.jl:100-104)
   std::vector<int> h_rej(labels_in ? n : 0, 0);
@@ -701,12 +745,6 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
       h_rej[i] = q != c.target_charge;
       any_rej = any_rej || h_rej[i];
     }
-  if (labels_in) SPP_CHECK(hipMalloc(&dlabels_in, sizeof(int) * (long long)n * ns));
-  if (uniforms) SPP_CHECK(hipMalloc(&duni, sizeof(double) * (long long)n * ns));
-  if (labels_in)
-    upload(dlabels_in, labels_in, sizeof(int) * (long long)n * ns, st);
-  if (uniforms)
-    upload(duni, uniforms, sizeof(double) * (long long)n * ns, st);
   const int tb = 256, gb = (nb + tb - 1) / tb;
   std::vector<int> h_status(nb), h_fail(nb), h_labels((size_t)nb * ns);
   std::vector<double> h_logp(nb), h_cond((size_t)nb * ns), h_mt(nb), h_mant(nb), h_ls(nb), h_amp(nb);
@@ -720,6 +758,9 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
       const int cur = (int)std::min<long long>(nb, n - s0);
       NB[0].nb = cur;
       NB[1].nb = cur;
+      // this chunk's uniforms or labels (sample indices relative to the chunk)
+      if (labels_in) upload(dlabels_in, labels_in + s0 * ns, sizeof(int) * (size_t)cur * ns, st);
+      if (uniforms) upload(duni, uniforms + s0 * ns, sizeof(double) * (size_t)cur * ns, st);
       // per-chunk initialisation
       std::vector<Leg> phys0(Lx);
       for (int x = 0; x < Lx; ++x) phys0[x] = c.spec[x].n;
@@ -746,6 +787,25 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
         Boundary<T>& Nc = NB[which];
         const Boundary<T>& sb = c.south[y + 1];
         const size_t mark = ctx.arena.mark();
+        // the row sweep below is replayed from its graph when one was captured
+        std::vector<long long> gkey = {y, cur, labels_in ? 1 : 0, (long long)mark};
+        for (int b = 0; b <= Lx; ++b) gkey.push_back(Nc.bdim[b]);
+        bool replayed = false;
+        if (use_graphs) {
+          auto it = W.graphs.find(gkey);
+          if (it != W.graphs.end() && it->second) {
+            SPP_CHECK(hipGraphLaunch(it->second, st));
+            replayed = true;
+            ctx.counters.graph_replays += 1;
+          }
+        }
+        const bool capture = use_graphs && !replayed && W.seen.count(gkey) && !W.graphs.count(gkey);
+        if (capture) {
+          SPP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+          ctx.capturing = true;
+        }
+        try {
+        if (!replayed) {
         // ---------------- right environments ----------------
         std::vector<T*> R(Lx + 1);
         std::vector<std::vector<Leg>> Rl(Lx + 1);
@@ -820,7 +880,7 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
             w.norms = (double*)ctx.arena.take(sizeof(double) * cur * 4);
             w.res = (double*)ctx.arena.take(sizeof(double) * cur * 16);
             physical_conditionals(ctx, Hb, sizeof(T) == sizeof(float), hs, c.phys[site], c.basis, cur, w);
-            launch_physical_select(ctx, w, duni, dlabels_in, s0, ns, site, c.charges, c.target_charge,
+            launch_physical_select(ctx, w, duni, dlabels_in, 0, ns, site, c.charges, c.target_charge,
                                    c.reach[site + 1], prefix, choice, labels, logp, cond, status, fail, done, drep,
                                    cur);
             hipLaunchKernelGGL(make_ptrs_kernel, dim3(gb), dim3(tb), 0, st, choice, (void*)c.cand[0][site],
@@ -853,7 +913,7 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
             SPP_CHECK(launch_dot<T>(full_view(Or), full_view(dense_ref(R[x + 1], Rs[x + 1], "ced", Rl[x + 1])),
                                     rawk[k], cur, st, dot_partials(ctx, cur)));
           }
-          hipLaunchKernelGGL(born_select_kernel, dim3(gb), dim3(tb), 0, st, raw0, raw1, duni, dlabels_in, s0, ns,
+          hipLaunchKernelGGL(born_select_kernel, dim3(gb), dim3(tb), 0, st, raw0, raw1, duni, dlabels_in, 0, ns,
                              site, choice, labels, logp, cond, status, fail, done, cur);
           SPP_CHECK(hipGetLastError());
           hipLaunchKernelGGL(make_ptrs_kernel, dim3(gb), dim3(tb), 0, st, choice, (void*)c.cand[0][site],
@@ -873,6 +933,33 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
           }
           ctx.arena.release(xm);
         }
+        }  // !replayed
+        } catch (...) {
+          ctx.capturing = false;
+          if (capture) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            W.graphs[gkey] = nullptr;
+          }
+          throw;
+        }
+        ctx.capturing = false;
+        if (capture) {
+          hipGraph_t g = nullptr;
+          hipGraphExec_t ex = nullptr;
+          const hipError_t e1 = hipStreamEndCapture(st, &g);
+          const hipError_t e2 = (e1 == hipSuccess && g) ? hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) : e1;
+          if (g) (void)hipGraphDestroy(g);
+          if (e2 != hipSuccess || !ex) {
+            (void)hipGetLastError();
+            throw HipError(std::string("row sweep capture failed: ") + hipGetErrorString(e2));
+          }
+          W.graphs[gkey] = ex;
+          SPP_CHECK(hipGraphLaunch(ex, st));
+          ctx.counters.graph_captures += 1;
+        }
+        if (!replayed) W.seen[gkey] = 1;
         ctx.arena.release(mark);
         // ---------------- absorb the drawn row ----------------
         // (the physical kind has no use for the last one, born.jl:157-160)
@@ -935,23 +1022,11 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
       }
     }
   } catch (...) {
-    NB[0].release();
-    NB[1].release();
-    for (void* p : {(void*)choice, (void*)labels, (void*)status, (void*)fail, (void*)done, (void*)logp, (void*)cond,
-                    (void*)raw0, (void*)raw1, (void*)nrm, (void*)fac, (void*)lnamp, (void*)mant, (void*)ptrs,
-                    (void*)dlabels_in, (void*)duni, (void*)dsst, (void*)dtens, (void*)prefix, (void*)mtlive,
-                    (void*)drep})
-      if (p) (void)hipFree(p);
+    (void)hipStreamSynchronize(st);
     ctx.arena.release(0);
+    c.work.reset();  // buffers of a failed call are not reused
     throw;
   }
-  NB[0].release();
-  NB[1].release();
-  for (void* p : {(void*)choice, (void*)labels, (void*)status, (void*)fail, (void*)done, (void*)logp, (void*)cond,
-                  (void*)raw0, (void*)raw1, (void*)nrm, (void*)fac, (void*)lnamp, (void*)mant, (void*)ptrs,
-                  (void*)dlabels_in, (void*)duni, (void*)dsst, (void*)dtens, (void*)prefix, (void*)mtlive,
-                  (void*)drep})
-    if (p) SPP_CHECK(hipFree(p));
 }
 
 template void draw_batch<float>(Context&, CacheDev<float>&, int, const double*, const int*, DrawOutputs&);
