@@ -472,6 +472,7 @@ def main():
     pcache = spp.SamplingCache(pctx, pepo, args.chi)
     spp.sample_configurations(pcache, uniforms=uniforms(10 ** 6, B))  # warm-up (tunes the FP64 tiles)
     spp.sample_configurations(pcache, uniforms=uniforms(10 ** 6, 1))  # and those of the batch-1 path
+    spp.sample_configurations(pcache, uniforms=uniforms(10 ** 6 + 1, 1))  # which captures its row sweeps
     pctx.synchronize()
     t0 = time.perf_counter()
     spp.sample_configurations(pcache, uniforms=uniforms(index, B))
