@@ -263,6 +263,42 @@ def fu_timing(spp, batch=16, chi=32, J=1.0, h=2.0, delta=0.05):
             "contraction_tflops_over_step": cnt["gemm_flops"] / dt / 1e12}
 
 
+def physical_timing(spp, L=8, chi=32, batch=256, precision="fp64", D=(3, 3)):
+    """Physical-kind Born draws (the resampling proposals of one particle,
+    src/sampling/born.jl:116-134): a seeded random 8x8 Z2 PEPS with D=6 (3+3),
+    physical-kind cache at chi=32, one batch of 256 draws (the size of a
+    leave-one-out scoring batch: one member's cache over K configurations) after
+    a warm-up batch of the same size. Per site and sample the device forms the site metric, makes
+    it PSD (two 648x648 parity blocks: Householder tridiagonalisation and QL)
+    and draws."""
+    peps = spp.random_peps(spp.Lattice(L, L), D, seed=7)
+    uni = [np.stack([spp.sample_uniforms(11, k * batch + i, L * L) for i in range(1, batch + 1)]) for k in range(2)]
+    ctx = spp.Context(0, precision)
+    try:
+        t0 = time.perf_counter()
+        cache = spp.SamplingCache(ctx, peps, chi, kind="physical")
+        ctx.synchronize()
+        tc = time.perf_counter() - t0
+        spp.sample_configurations(cache, uniforms=uni[0])
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        r = spp.sample_configurations(cache, uniforms=uni[1])
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        rep = r["repairs"]
+        out = {"value": batch / dt, "unit": "physical-kind Born samples/s", "dtype": precision,
+               "workload": f"physical-kind draws (PSD site metric), random {L}x{L} Z2 PEPS D={sum(D)} ({D[0]}+{D[1]}), "
+                           f"chi={chi}, {batch} samples per call",
+               "batch": batch, "cache_build_s": tc, "s_per_call": dt,
+               "metric_blocks_per_call": 2 * batch * L * L,
+               "largest_negative_spectral_fraction": float(np.nanmax(rep[:, :, 3])),
+               "min_relative_eigenvalue": float(np.nanmin(rep[:, :, 0]))}
+        del cache
+    finally:
+        ctx.close()
+    return out
+
+
 def precision_sweep(spp, flops, L=6, chi=64, batch=8, D=(4, 4)):
     """Born draws at the bond and boundary dimensions of the 12x12 workload
     (D=8 as Z2Space(0=>4, 1=>4), chi=64) on a reduced LxL lattice, in every
@@ -330,6 +366,7 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true",
                     help="time only the log-norm gradient workload (for rocprofv3)")
     ap.add_argument("--fu-only", action="store_true", help="time only the full-update workload")
+    ap.add_argument("--physical-only", action="store_true", help="time only the physical-kind draws")
     ap.add_argument("--fu-batch", type=int, default=16, help="states per full-update step (0: skip)")
     ap.add_argument("--sweep-lattice", type=int, default=6,
                     help="lattice edge of the D=8 chi=64 precision sweep (12: the full lattice; 0: skip)")
@@ -353,6 +390,9 @@ def main():
         return
     if args.fu_only:
         print(json.dumps({"full_update": fu_timing(spp, batch=max(args.fu_batch, 1))}), flush=True)
+        return
+    if args.physical_only:
+        print(json.dumps({"physical_draws": physical_timing(spp)}), flush=True)
         return
     pepo, _ = spp.load_state_npz(PEPO_FIXTURE)
     L = pepo.lattice.Lx
@@ -523,6 +563,9 @@ def main():
         log("precision sweep ...")
         result["precision_sweep"] = precision_sweep(spp, flops, L=args.sweep_lattice, batch=args.sweep_batch)
     result["adjoint"] = adjoint_timing(spp)
+    log("physical-kind draws ...")
+    result["physical_draws"] = physical_timing(spp)
+    log(f"physical-kind draws: {result['physical_draws']['value']:.3f} samples/s")
     if args.fu_batch > 0:
         log("full-update step ...")
         result["full_update"] = fu_timing(spp, batch=args.fu_batch, J=args.J, h=args.h)

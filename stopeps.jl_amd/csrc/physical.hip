@@ -161,20 +161,52 @@ __global__ __launch_bounds__(kTriThreads) void psd_tridiag_kernel(double* M, lon
   }
 }
 
-// One thread per (sample, block): eigenvalues and projections by the implicit
-// QL iteration, then per block {min lambda, max |lambda|, negatives, sum of
-// negative squares, sum of squares, weight of label 0, weight of label 1,
-// failure flag}.
-__global__ void psd_ql_kernel(double* dv, double* ev, double* yv, int ld, int n0, int n1, int nprob, double* res) {
-  const int prob = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per (sample, block): the tridiagonal form and the label vectors are
+// staged in LDS, lane 0 runs the implicit QL iteration with its rotations
+// applied to the vectors that are nonzero in this block (in a Z2 metric each
+// label's vector lives in one parity block), then the wave sums per block
+// {min lambda, max |lambda|, negatives, sum of negative squares, sum of
+// squares, weight of label 0, weight of label 1, failure flag}.
+__global__ __launch_bounds__(64) void psd_ql_kernel(double* dv, const double* ev, double* yv, int ld, int n0, int n1,
+                                                    int nprob, double* res, int writeback) {
+  const int prob = blockIdx.x;
   if (prob >= nprob) return;
   const int n = (prob & 1) == 0 ? n0 : n1;
-  double* d = dv + (long long)prob * ld;
-  double* e = ev + (long long)prob * ld;
-  double* y = yv + (long long)prob * 2 * ld;
-  const int fail = ql_project(n, d, e, y, ld, 2, 60);
+  const int lane = threadIdx.x;
+  extern __shared__ double sh[];
+  double* d = sh;
+  double* e = d + ld;
+  double* y = e + ld;  // [2][ld]
+  __shared__ int nz[2];
+  __shared__ int fail;
+  double* dg = dv + (long long)prob * ld;
+  const double* eg = ev + (long long)prob * ld;
+  double* yg = yv + (long long)prob * 2 * ld;
+  bool any0 = false, any1 = false;
+  for (int j = lane; j < n; j += 64) {
+    d[j] = dg[j];
+    e[j] = eg[j];
+    y[j] = yg[j];
+    y[ld + j] = yg[ld + j];
+    any0 = any0 || yg[j] != 0.0;
+    any1 = any1 || yg[ld + j] != 0.0;
+  }
+  any0 = __any(any0);
+  any1 = __any(any1);
+  if (lane == 0) {
+    nz[0] = any0;
+    nz[1] = any1;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    // rotate only the nonzero vectors: both, the first, or the second alone
+    const int ny = nz[0] + nz[1];
+    double* yy = nz[0] ? y : y + ld;
+    fail = ql_project(n, d, e, yy, ld, ny, 60);
+  }
+  __syncthreads();
   double mn = 0.0, mx = 0.0, neg = 0.0, neg2 = 0.0, sum2 = 0.0, w0 = 0.0, w1 = 0.0;
-  for (int j = 0; j < n; ++j) {
+  for (int j = lane; j < n; j += 64) {
     const double l = d[j];
     mn = fmin(mn, l);
     mx = fmax(mx, fabs(l));
@@ -184,19 +216,36 @@ __global__ void psd_ql_kernel(double* dv, double* ev, double* yv, int ld, int n0
     }
     sum2 += l * l;
     if (l > 0.0) {
-      w0 += l * y[j] * y[j];
-      w1 += l * y[ld + j] * y[ld + j];
+      if (nz[0]) w0 += l * y[j] * y[j];
+      if (nz[1]) w1 += l * y[ld + j] * y[ld + j];
     }
   }
-  double* r = res + (long long)prob * 8;
-  r[0] = mn;
-  r[1] = mx;
-  r[2] = neg;
-  r[3] = neg2;
-  r[4] = sum2;
-  r[5] = w0;
-  r[6] = w1;
-  r[7] = fail;
+  for (int m = 32; m >= 1; m >>= 1) {
+    mn = fmin(mn, __shfl_xor(mn, m, 64));
+    mx = fmax(mx, __shfl_xor(mx, m, 64));
+    neg += __shfl_xor(neg, m, 64);
+    neg2 += __shfl_xor(neg2, m, 64);
+    sum2 += __shfl_xor(sum2, m, 64);
+    w0 += __shfl_xor(w0, m, 64);
+    w1 += __shfl_xor(w1, m, 64);
+  }
+  if (writeback)  // self-tests read the eigenvalues and projections back
+    for (int j = lane; j < n; j += 64) {
+      dg[j] = d[j];
+      yg[j] = y[j];
+      yg[ld + j] = y[ld + j];
+    }
+  if (lane == 0) {
+    double* r = res + (long long)prob * 8;
+    r[0] = mn;
+    r[1] = mx;
+    r[2] = neg;
+    r[3] = neg2;
+    r[4] = sum2;
+    r[5] = w0;
+    r[6] = w1;
+    r[7] = fail;
+  }
 }
 
 // Per sample: the site's report and its Born conditional from the block
@@ -321,8 +370,8 @@ void physical_conditionals(Context& ctx, const void* H, bool h_fp32, long long h
                      ps.inv, td, ps.a, ps.dim[2], U[0], U[1], U[2], U[3], w.taus, w.d, w.e, w.y, w.norms);
   SPP_CHECK(hipGetLastError());
   const int nprob = 2 * nb;
-  hipLaunchKernelGGL(psd_ql_kernel, dim3((nprob + 63) / 64), dim3(64), 0, st, w.d, w.e, w.y, ld, ps.nq[0], ps.nq[1],
-                     nprob, w.res);
+  hipLaunchKernelGGL(psd_ql_kernel, dim3(nprob), dim3(64), sizeof(double) * 4 * (size_t)ld, st, w.d, w.e, w.y, ld,
+                     ps.nq[0], ps.nq[1], nprob, w.res, 0);
   SPP_CHECK(hipGetLastError());
 }
 
@@ -376,7 +425,8 @@ void psd_selftest(Context& ctx, int nprob, int ld, const int* n_h, const double*
       hipLaunchKernelGGL(psd_tridiag_kernel, dim3(1), dim3(kTriThreads), lds, st, M, 2LL * ld * ld, ld, n, 0, inv, td,
                          site, 2, 1.0, 0.0, 0.0, 1.0, taus, d, e, y, nrm);
       SPP_CHECK(hipGetLastError());
-      hipLaunchKernelGGL(psd_ql_kernel, dim3(1), dim3(64), 0, st, d, e, y, ld, n, 0, 1, res);
+      hipLaunchKernelGGL(psd_ql_kernel, dim3(1), dim3(64), sizeof(double) * 4 * (size_t)ld, st, d, e, y, ld, n, 0, 1,
+                         res, 1);
       SPP_CHECK(hipGetLastError());
       SPP_CHECK(hipMemcpyAsync(hd.data(), d, sizeof(double) * ld, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipMemcpyAsync(hy.data(), y, sizeof(double) * 2 * ld, hipMemcpyDeviceToHost, st));

@@ -20,10 +20,12 @@ namespace spp {
 // y: ny vectors of length n (vector v at y + v * ystride), overwritten by the
 // projections. Returns 0, or 1 when an eigenvalue needed more than maxit
 // iterations (the values are then incomplete).
-__host__ __device__ inline int ql_project(int n, double* d, double* e, double* y, long long ystride, int ny,
-                                          int maxit) {
+__host__ __device__ inline int ql_project(int n, double* __restrict__ d, double* __restrict__ e,
+                                          double* __restrict__ y, long long ystride, int ny, int maxit) {
   if (n <= 0) return 0;
   e[n - 1] = 0.0;
+  double* __restrict__ y0 = y;
+  double* __restrict__ y1 = y + ystride;
   for (int l = 0; l < n; ++l) {
     int iter = 0;
     for (;;) {
@@ -41,10 +43,16 @@ __host__ __device__ inline int ql_project(int n, double* d, double* e, double* y
       g = d[m] - d[l] + e[l] / (g + copysign(r, g));
       double s = 1.0, c = 1.0, p = 0.0;
       bool split = false;
-      for (int i = m - 1; i >= l; --i) {
+      // the rotated entry i of each vector is carried to the next step in a
+      // register (it is that step's entry i + 1)
+      double c0 = ny > 0 ? y0[m] : 0.0, c1 = ny > 1 ? y1[m] : 0.0;
+      int i = m - 1;
+      for (; i >= l; --i) {
         const double f = s * e[i];
         const double b = c * e[i];
-        r = hypot(f, g);
+        // sqrt(f^2 + g^2), by hypot only where the squares could over- or underflow
+        const double big = fmax(fabs(f), fabs(g));
+        r = (big > 1e-150 && big < 1e150) ? sqrt(f * f + g * g) : hypot(f, g);
         e[i + 1] = r;
         if (r == 0.0) {
           // the coupling underflowed: the block splits at i + 1
@@ -53,20 +61,28 @@ __host__ __device__ inline int ql_project(int n, double* d, double* e, double* y
           split = true;
           break;
         }
-        s = f / r;
-        c = g / r;
+        const double rinv = 1.0 / r;
+        s = f * rinv;
+        c = g * rinv;
         g = d[i + 1] - p;
         r = (d[i] - g) * s + 2.0 * c * b;
         p = s * r;
         d[i + 1] = g + p;
         g = c * r - b;
-        for (int v = 0; v < ny; ++v) {
-          double* yv = y + v * ystride;
-          const double hi = yv[i + 1];
-          yv[i + 1] = s * yv[i] + c * hi;
-          yv[i] = c * yv[i] - s * hi;
+        if (ny > 0) {
+          const double lo = y0[i];
+          y0[i + 1] = s * lo + c * c0;
+          c0 = c * lo - s * c0;
+        }
+        if (ny > 1) {
+          const double lo = y1[i];
+          y1[i + 1] = s * lo + c * c1;
+          c1 = c * lo - s * c1;
         }
       }
+      // the last rotated entry (the step's entry l, or i + 1 where it split)
+      if (ny > 0) y0[i + 1] = c0;
+      if (ny > 1) y1[i + 1] = c1;
       if (split) continue;
       d[l] -= p;
       e[l] = g;

@@ -132,18 +132,41 @@ __device__ inline void tridiagonalize(double* A, int ld, int n, double* red, dou
     __syncthreads();
     const double tau2 = tri_reflector(A, ld, n, k + 1, v2, d, e, red, taus);
     // rows k+2.. : apply the step-k update (columns k+2..; column k+1 is not
-    // read again) and form the next product p = tau2 * S v2
-    for (int i = k + 2 + wave; i < n; i += kTriWaves) {
-      double* ri = A + (long long)i * ld;
-      const double vi = v[i], wi = w[i];
-      double acc = 0.0;
-      for (int j = k + 2 + lane; j < n; j += 64) {
-        const double a = ri[j] - (vi * w[j] + wi * v[j]);
-        ri[j] = a;
-        acc += a * v2[j];
+    // read again) and form the next product p = tau2 * S v2. Each wave takes
+    // four rows at a time so that four rows' loads are in flight together.
+    for (int i = k + 2 + wave; i < n; i += 4 * kTriWaves) {
+      int r[4];
+      bool h[4];
+      double* rr[4];
+      double vi[4], wi[4], acc[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        r[q] = i + q * kTriWaves;
+        h[q] = r[q] < n;
+        rr[q] = A + (long long)(h[q] ? r[q] : i) * ld;
+        vi[q] = h[q] ? v[r[q]] : 0.0;
+        wi[q] = h[q] ? w[r[q]] : 0.0;
+        acc[q] = 0.0;
       }
-      acc = tri_wave_sum(acc);
-      if (lane == 0) p[i] = tau2 * acc;
+      for (int j = k + 2 + lane; j < n; j += 64) {
+        const double wj = w[j], vj = v[j], v2j = v2[j];
+        double a[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] = h[q] ? rr[q][j] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (!h[q]) continue;
+          a[q] = a[q] - (vi[q] * wj + wi[q] * vj);
+          rr[q][j] = a[q];
+          acc[q] += a[q] * v2j;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = tri_wave_sum(acc[q]);
+      if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (h[q]) p[r[q]] = tau2 * acc[q];
     }
     __syncthreads();
     double* t = v;
