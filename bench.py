@@ -232,20 +232,24 @@ def entropy_timing(spp, n=16, chi=32, J=1.0, h=2.0):
             "max_boundary_entropy": float(np.max(out[True][1]["max_boundary_entropy"]))}
 
 
-def fu_timing(spp, batch=16, chi=32, J=1.0, h=2.0, delta=0.05):
+def fu_timing(spp, batch=16, chi=32, J=1.0, h=2.0, delta=0.05, warmup=False):
     """The full-update evolution workload: one second-order step (sweeps H->,
     V->, V<-, H<-; 224 bond updates) of a batch of 8x8 Z2 PEPS, D=6 (3+3),
     d=2, i.i.d. N(0,1) blocks scaled to unit norm (seeds 100, 101, ...), TFIM
     gates, boundary chi=32, FullUpdate with 40 ALS and 100 PCG iterations
     (the options of the reference's example configurations), FP64. One timed
-    step: a step is tens of seconds and allocates only its scratch."""
+    step (a step is tens of seconds); with ``warmup`` an untimed step first
+    tunes the tile shapes, otherwise the heuristic shapes are used."""
     lat = spp.Lattice(8, 8)
-    ctx = spp.Context(0, "fp64", autotune=0)  # timed on first calls: heuristic tile shapes
+    ctx = spp.Context(0, "fp64", autotune=1 if warmup else 0)
     dev = [spp.DeviceState(ctx, spp.random_peps(lat, (3, 3), seed=100 + k)) for k in range(batch)]
     model = spp.TFIM(J, h, symmetric=True)
     method = spp.FullUpdate(max_iterations=40, error_tolerance=1e-8, diagonal_shift=1e-10,
                             linear_max_iterations=100, linear_tolerance=1e-9)
-    ctx.synchronize()
+    if warmup:
+        spp.evolution_steps(ctx, dev, model, delta, chi, method)
+        ctx.synchronize()
+        log(f"full-update warm-up step done (batch {batch})")
     ctx.reset_counters()
     t0 = time.perf_counter()
     r = spp.evolution_steps(ctx, dev, model, delta, chi, method)
@@ -367,6 +371,7 @@ def main():
                     help="time only the log-norm gradient workload (for rocprofv3)")
     ap.add_argument("--fu-only", action="store_true", help="time only the full-update workload")
     ap.add_argument("--physical-only", action="store_true", help="time only the physical-kind draws")
+    ap.add_argument("--fu-warmup", action="store_true", help="an untimed full-update step first (tunes the tiles)")
     ap.add_argument("--fu-batch", type=int, default=16, help="states per full-update step (0: skip)")
     ap.add_argument("--sweep-lattice", type=int, default=6,
                     help="lattice edge of the D=8 chi=64 precision sweep (12: the full lattice; 0: skip)")
@@ -389,7 +394,8 @@ def main():
         print(json.dumps({"adjoint": adjoint_timing(spp)}), flush=True)
         return
     if args.fu_only:
-        print(json.dumps({"full_update": fu_timing(spp, batch=max(args.fu_batch, 1))}), flush=True)
+        print(json.dumps({"full_update": fu_timing(spp, batch=max(args.fu_batch, 1), warmup=args.fu_warmup)}),
+              flush=True)
         return
     if args.physical_only:
         print(json.dumps({"physical_draws": physical_timing(spp)}), flush=True)
@@ -568,7 +574,7 @@ def main():
     log(f"physical-kind draws: {result['physical_draws']['value']:.3f} samples/s")
     if args.fu_batch > 0:
         log("full-update step ...")
-        result["full_update"] = fu_timing(spp, batch=args.fu_batch, J=args.J, h=args.h)
+        result["full_update"] = fu_timing(spp, batch=args.fu_batch, J=args.J, h=args.h, warmup=True)
     if not args.no_cpu:
         log("CPU baseline ...")
         try:
@@ -597,8 +603,13 @@ def oracle_check(spp, caches, model, args):
         d = spp.sample_configurations(c, uniforms=uni)
         same = [np.array_equal(d["configuration"][k], ref["configuration"][k]) for k in range(len(idx))]
         dl = [abs(d["log_probability"][k] - ref["log_probability"][k]) for k in range(len(idx)) if same[k]]
+        # the discarded weight of the truncations: FP32 products put noise of
+        # about 1e-7 relative on it, so it is compared with the oracle's value
+        dmt = np.abs(d["max_environment_truncation"] - ref["max_environment_truncation"])
         out[name] = {"identical_configurations": int(sum(same)),
-                     "max_abs_dlog_probability": float(max(dl)) if dl else None}
+                     "max_abs_dlog_probability": float(max(dl)) if dl else None,
+                     "max_environment_truncation": float(np.max(d["max_environment_truncation"])),
+                     "max_abs_dmax_environment_truncation": float(np.max(dmt))}
     m = np.isfinite(ref["energy_per_site"])
     cfg = ref["configuration"][m].astype(np.int64)
     for name, c in caches:

@@ -96,7 +96,11 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value);
  * update bonds repaired by the PSD projection of their hole; counters[16..18] = launch
  * classes the tile autotuner tuned, those whose fastest shape differs from the
  * heuristic's, and the milliseconds spent tuning; counters[19..20] = draw row sweeps
- * captured as HIP graphs and graph replays. Returns the count written. */
+ * captured as HIP graphs and graph replays; counters[21] = boundary truncation
+ * eigenproblems (one per sample and bond parity) whose Ritz residuals were still
+ * above "subspace_tol" when "max_subspace_iter" ended the iteration (their kept
+ * vectors are those of the last iteration; the reference's boundary SVD has no
+ * such cap, boundary_mps.jl:33-43). Returns the count written. */
 int spp_get_counters(spp_ctx* ctx, double* counters, int n);
 int spp_reset_counters(spp_ctx* ctx);
 

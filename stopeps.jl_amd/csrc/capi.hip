@@ -182,14 +182,15 @@ int spp_get_counters(spp_ctx* ctx, double* counters, int n) {
   if (!ctx || !counters) return -3;
   if (guarded(ctx, [&] { ctx->c.flush_timer(); }) != 0) return -1;
   const Counters& k = ctx->c.counters;
-  double v[21] = {k.gemm_flops,        k.gemm_launches,    (double)ctx->c.last_iterations,
+  double v[22] = {k.gemm_flops,        k.gemm_launches,    (double)ctx->c.last_iterations,
                   (double)ctx->c.arena.peak(), k.gemm_ms, k.gemm_flops_profiled,
                   k.prec_launches[0],  k.prec_ms[0],       k.prec_flops[0],
                   k.prec_launches[1],  k.prec_ms[1],       k.prec_flops[1],
                   k.eig_calls,         k.eig_iterations,   k.eig_filter_gemms,
                   k.fu_repairs,        k.tuned_classes,    k.tuned_changed,
-                  k.tune_ms,           k.graph_captures,   k.graph_replays};
-  int m = n < 21 ? n : 21;
+                  k.tune_ms,           k.graph_captures,   k.graph_replays,
+                  k.eig_unconverged};
+  int m = n < 22 ? n : 22;
   for (int i = 0; i < m; ++i) counters[i] = v[i];
   return m;
 }

@@ -138,6 +138,9 @@ struct Counters {
   double prec_flops[2] = {0, 0};
   // truncation eigensolver: iterative calls, subspace iterations, filter products
   double eig_calls = 0, eig_iterations = 0, eig_filter_gemms = 0;
+  // truncation eigenproblems (sample, parity block) whose Ritz residuals were
+  // still above subspace_tol when max_subspace_iter ended the iteration
+  double eig_unconverged = 0;
   double fu_repairs = 0;  // bonds repaired by the PSD projection of the hole
   // contraction tile autotuner: launch classes tuned, classes whose fastest
   // shape is not the heuristic's, and the milliseconds the tuning launches took

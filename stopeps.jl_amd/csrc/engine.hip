@@ -1347,6 +1347,8 @@ static void eig_topk(Context& ctx, const double* rho, long long rho_pstride, int
   ctx.last_iterations = it + 1;
   ctx.counters.eig_calls += 1;
   ctx.counters.eig_iterations += it + 1;
+  if (it == ctx.max_subspace_iter)  // the cap ended the iteration: count the blocks short of the tolerance
+    for (int p = 0; p < nprob; ++p) ctx.counters.eig_unconverged += hconv[p] ? 0 : 1;
   res.w = sig;
   res.wstride = kk;
   res.v = Qb;

@@ -156,3 +156,28 @@ def test_z2_gemm_empty_k_block(spp, gpu_ctx, precision, layout):
     assert np.all(C[eM:, eN:] == 0.0)
     tol = 1e-12 if precision == "fp64" else 2e-5
     assert np.max(np.abs(C[neutral] - ref[neutral])) <= tol * np.max(np.abs(ref))
+
+
+def test_truncation_cap_is_counted(spp):
+    """A truncation eigensolve stopped by max_subspace_iter before its Ritz
+    residuals reach subspace_tol is counted (the reference's boundary SVD has
+    no cap); with the default cap the headline-like draw counts none."""
+    pepo = spp.synthetic_thermal_pepo(spp.Lattice(4, 4), D=(2, 2), eps=0.3, seed=2)
+    uni = np.stack([spp.sample_uniforms(1, i, 16) for i in range(1, 5)])
+    ctx = spp.Context(0, "fp64", ksub=8)
+    try:
+        cache = spp.SamplingCache(ctx, pepo, 6)
+        ctx.reset_counters()
+        spp.sample_configurations(cache, uniforms=uni)
+        assert ctx.counters()["eig_unconverged"] == 0
+        ctx.set_option("max_subspace_iter", 1)
+        ctx.set_option("subspace_tol", 1e-15)
+        ctx.reset_counters()
+        try:  # one subspace step leaves poor bases: the draw may stop with a Born error
+            spp.sample_configurations(cache, uniforms=uni)
+        except spp.BornError:
+            pass
+        c = ctx.counters()
+        assert c["eig_calls"] > 0 and c["eig_unconverged"] > 0
+    finally:
+        ctx.close()
