@@ -81,7 +81,12 @@ int spp_set_precision(spp_ctx* ctx, int precision);
  * sampling cache keeps its draw buffers between calls, and each row's
  * environment-and-draw sweep is captured as a HIP graph the second time it runs
  * with the same shape and replayed afterwards; 0: launched kernel by kernel;
- * off while "profile" is on); "gemm_tile" (self-tests:
+ * off while "profile" is on); "kron" (1, the default: the Gram transfers of
+ * the draw's FP32 absorbs apply each drawn layer through its ket and bra
+ * single-layer factors in one kernel, about 9x fewer multiply-adds than the
+ * dense double layer at D=6; 0: the dense double layer; results agree to FP32
+ * rounding); "kron_rows" (rows per block of that kernel, 0 = default);
+ * "gemm_tile" (self-tests:
  * force one tile shape index of the kernel family on every launch, -1 = off). */
 int spp_set_option(spp_ctx* ctx, const char* name, double value);
 
@@ -100,7 +105,9 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value);
  * eigenproblems (one per sample and bond parity) whose Ritz residuals were still
  * above "subspace_tol" when "max_subspace_iter" ended the iteration (their kept
  * vectors are those of the last iteration; the reference's boundary SVD has no
- * such cap, boundary_mps.jl:33-43). Returns the count written. */
+ * such cap, boundary_mps.jl:33-43); counters[22] = launches of the factorised
+ * Gram transfer kernel (its multiply-adds, register tile included, are added
+ * to counters[0]). Returns the count written. */
 int spp_get_counters(spp_ctx* ctx, double* counters, int n);
 int spp_reset_counters(spp_ctx* ctx);
 
@@ -457,6 +464,19 @@ int spp_selftest_psd(spp_ctx* ctx, int nprob, int ld, const int* n, const double
  * their projections on the eigenvectors. Returns 0, 1 when an eigenvalue
  * needed more than maxit iterations, or -3 on bad arguments. */
 int spp_selftest_ql(int n, double* d, double* e, double* vectors, int nvec, int maxit);
+
+/* Self-test of the factorised Gram transfer kernel (no reference
+ * counterpart: the reference contracts the double layer, boundary_mps.jl:136-151):
+ * single layer S[W][N][PHI][E][S] (d[4] / e[4]: dims and even counts of W, N,
+ * E, S; dphi / ephi of PHI; FP64, host, one parity), one sample, bonds l and R
+ * of dims nl / nr with even counts el / er. stage 0: x = z1[nl][nr][E][N][fold E]
+ * -> y = z2[nl][nr][E][W][fold S] (contracting N and E); stage 1: x = z2 ->
+ * y = z3[nl][nr][W][W][fold N] (contracting E and S); pair legs (ket, bra) in
+ * the parity-sorted order of the double layers, the last leg folded to the
+ * parity its neighbours fix. rows_per_block 0: the default. Returns 0, -1 with
+ * the error in spp_last_error, -2 when the layer does not fit the kernel. */
+int spp_selftest_kron(spp_ctx* ctx, const int* d, const int* e, int dphi, int ephi, const double* S, int stage,
+                      int nl, int el, int nr, int er, const float* x, float* y, int rows_per_block);
 
 /* Calibration copy (no reference counterpart): one launch of the library's
  * plain copy kernel moving n FP32 values between two device buffers it

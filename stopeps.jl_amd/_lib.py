@@ -9,6 +9,7 @@ _lib = None
 
 c_int_p = ctypes.POINTER(ctypes.c_int)
 c_double_p = ctypes.POINTER(ctypes.c_double)
+c_float_p = ctypes.POINTER(ctypes.c_float)
 c_void_p = ctypes.c_void_p
 
 _SIGNATURES = {
@@ -75,6 +76,9 @@ _SIGNATURES = {
     "spp_selftest_ql": (ctypes.c_int, [ctypes.c_int, c_double_p, c_double_p, c_double_p, ctypes.c_int,
                                        ctypes.c_int]),
     "spp_selftest_spectrum": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, c_int_p, c_double_p, c_double_p]),
+    "spp_selftest_kron": (ctypes.c_int, [c_void_p, c_int_p, c_int_p, ctypes.c_int, ctypes.c_int, c_double_p,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         c_float_p, c_float_p, ctypes.c_int]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -73,7 +73,9 @@ inline int gram_rchunk(long long unit, long long rdim, size_t elem, double limit
 // tensors).
 inline double absorb_bytes_per_sample(long long Lx, long long chi, long long D, size_t elem, int ksub) {
   const double ts = (double)elem;
-  const double half = chi * D / 2.0 + 1;
+  // rows of one compact rho block: chi * D / 2 when the double-layer leg splits
+  // evenly by parity, up to chi * D when it does not (e.g. D = 2 + 1)
+  const double half = (double)chi * D + 1;
   const long long unit = chi * D * D * D;
   double z = ts * 2.0 * (double)unit * gram_rchunk(unit, chi, elem);
   double g = ts * (Lx + 1) * (double)(chi * D) * (chi * D);

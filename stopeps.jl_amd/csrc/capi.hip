@@ -169,6 +169,15 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value) {
     } else if (n == "graphs") {
       if (value != 0 && value != 1) throw HipError("graphs must be 0 or 1");
       ctx->c.graphs = (int)value;
+    } else if (n == "kron") {
+      if (value != 0 && value != 1) throw HipError("kron must be 0 or 1");
+      ctx->c.kron = (int)value;
+    } else if (n == "kron_groups") {
+      if (value < 1 || value > 64) throw HipError("kron_groups must lie in [1, 64]");
+      ctx->c.kron_groups = (int)value;
+    } else if (n == "kron_rows") {
+      if (value < 0 || value > 32) throw HipError("kron_rows must lie in [0, 32]");
+      ctx->c.kron_rows = (int)value;
     } else if (n == "gemm_tile") {
       if (value < -1 || value >= kMaxTileShapes) throw HipError("gemm_tile must lie in [-1, 5]");
       ctx->c.force_tile = (int)value;
@@ -182,15 +191,15 @@ int spp_get_counters(spp_ctx* ctx, double* counters, int n) {
   if (!ctx || !counters) return -3;
   if (guarded(ctx, [&] { ctx->c.flush_timer(); }) != 0) return -1;
   const Counters& k = ctx->c.counters;
-  double v[22] = {k.gemm_flops,        k.gemm_launches,    (double)ctx->c.last_iterations,
+  double v[23] = {k.gemm_flops,        k.gemm_launches,    (double)ctx->c.last_iterations,
                   (double)ctx->c.arena.peak(), k.gemm_ms, k.gemm_flops_profiled,
                   k.prec_launches[0],  k.prec_ms[0],       k.prec_flops[0],
                   k.prec_launches[1],  k.prec_ms[1],       k.prec_flops[1],
                   k.eig_calls,         k.eig_iterations,   k.eig_filter_gemms,
                   k.fu_repairs,        k.tuned_classes,    k.tuned_changed,
                   k.tune_ms,           k.graph_captures,   k.graph_replays,
-                  k.eig_unconverged};
-  int m = n < 22 ? n : 22;
+                  k.eig_unconverged,   k.kron_launches};
+  int m = n < 23 ? n : 23;
   for (int i = 0; i < m; ++i) counters[i] = v[i];
   return m;
 }
@@ -738,6 +747,81 @@ int spp_selftest_psd(spp_ctx* ctx, int nprob, int ld, const int* n, const double
 int spp_selftest_ql(int n, double* d, double* e, double* vectors, int nvec, int maxit) {
   if (n < 0 || nvec < 0 || (n > 0 && (!d || !e || (nvec > 0 && !vectors)))) return -3;
   return ql_project(n, d, e, vectors, n, nvec, maxit);
+}
+
+int spp_selftest_kron(spp_ctx* ctx, const int* d, const int* e, int dphi, int ephi, const double* S, int stage,
+                      int nl, int el, int nr, int er, const float* x, float* y, int rows_per_block) {
+  if (!ctx || !d || !e || !S || !x || !y || stage < 0 || stage > 1 || nl < 1 || nr < 1) return -3;
+  KronSite ks;
+  if (!kron_site_layout(d, e, dphi, ephi, ks)) return -2;
+  const int q = kron_layer_parity(d, e, dphi, ephi, S);
+  if (q < 0) return -2;
+  return guarded(ctx, [&] {
+    hipStream_t st = ctx->c.stream;
+    std::vector<char> tabs;
+    kron_build_tables(ks, S, q, tabs);
+    kron_prepare();
+    long long pd[4], pf[4];
+    int pe[4];
+    for (int k = 0; k < 4; ++k) {
+      pd[k] = (long long)d[k] * d[k];
+      pe[k] = e[k] * e[k] + (d[k] - e[k]) * (d[k] - e[k]);
+      pf[k] = std::max<long long>(pe[k], pd[k] - pe[k]);
+    }
+    // legs W=0, N=1, E=2, S=3
+    KronRows kr;
+    kr.rc = nr;
+    kr.leven = el;
+    kr.reven = er;
+    long long nx, ny;
+    if (stage == 0) {
+      kr.nrows = (long long)nl * nr * pd[2];
+      kr.nf = (int)pd[2];
+      kr.ef = pe[2];
+      kr.sx_slab = pd[2] * pd[1] * pf[2];
+      kr.sxf = pd[1] * pf[2];
+      kr.sx1 = pf[2];
+      nx = (long long)nl * nr * pd[2] * pd[1] * pf[2];
+      ny = (long long)nl * nr * pd[2] * pd[0] * pf[3];
+    } else {
+      kr.nrows = (long long)nl * nr * pd[0];
+      kr.nf = (int)pd[0];
+      kr.ef = pe[0];
+      kr.sx_slab = pd[2] * pd[0] * pf[3];
+      kr.sxf = pf[3];
+      kr.sx1 = pd[0] * pf[3];
+      nx = (long long)nl * nr * pd[2] * pd[0] * pf[3];
+      ny = (long long)nl * nr * pd[0] * pd[0] * pf[1];
+    }
+    kr.sx2 = 1;
+    kr.xs = nx;
+    kr.ys = ny;
+    char* dt = nullptr;
+    float *dx = nullptr, *dy = nullptr;
+    void** dp = nullptr;
+    auto cleanup = [&] {
+      for (void* p : {(void*)dt, (void*)dx, (void*)dy, (void*)dp})
+        if (p) (void)hipFree(p);
+    };
+    try {
+      SPP_CHECK(hipMalloc(&dt, tabs.size()));
+      SPP_CHECK(hipMalloc(&dx, sizeof(float) * nx));
+      SPP_CHECK(hipMalloc(&dy, sizeof(float) * ny));
+      SPP_CHECK(hipMalloc(&dp, sizeof(void*)));
+      upload(dt, tabs.data(), tabs.size(), st);
+      upload(dx, x, sizeof(float) * nx, st);
+      void* tp = dt;
+      upload(dp, &tp, sizeof(void*), st);
+      SPP_CHECK(launch_fill<float>(dy, ny, 0.f, st));
+      SPP_CHECK(launch_kron_rows(ks, stage, dx, dy, kr, dp, 1, rows_per_block, st));
+      SPP_CHECK(hipMemcpyAsync(y, dy, sizeof(float) * ny, hipMemcpyDeviceToHost, st));
+      SPP_CHECK(hipStreamSynchronize(st));
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
+  });
 }
 
 int spp_selftest_copy(spp_ctx* ctx, long long n) {

@@ -14,6 +14,7 @@
 #include <stdexcept>
 #include "z2gemm.h"
 #include "kernels.h"
+#include "kron.h"
 
 namespace spp {
 
@@ -147,6 +148,7 @@ struct Counters {
   double tuned_classes = 0, tuned_changed = 0, tune_ms = 0;
   // draw row sweeps captured as HIP graphs, and graph replays
   double graph_captures = 0, graph_replays = 0;
+  double kron_launches = 0;  // factorised Gram transfer launches (their multiply-adds are in gemm_flops)
 };
 
 // Event pairs recorded around contraction launches when profiling is on.
@@ -259,6 +261,13 @@ struct Context {
   // runs replay the graph; 0: every sweep is launched kernel by kernel.
   int graphs = 1;
   bool capturing = false;  // a row sweep is being captured: no tuning launches
+  // Gram transfers of FP32 absorbs whose rows carry single-layer tables
+  // (kron.h): 1 (the default) applies each double layer through its ket and
+  // bra factors, 0 contracts the dense double layer; kron_rows: rows per block
+  // of that kernel (0: its default).
+  int kron = 1;
+  int kron_rows = 0;
+  int kron_groups = 1;      // row groups per block of that kernel
   void flush_timer();       // accumulate finished event pairs into counters
 };
 
@@ -336,6 +345,11 @@ struct RowLayers {
   std::vector<void* const*> ptrs;         // per site: per-sample pointer tables (device) or nullptr
   std::vector<T*> shared;                 // per site: shared layer when ptrs == nullptr
   bool rotated = false;                   // read as E_rot[w,n,e,s] = E[e,s,w,n]
+  // Single-layer factors of the layers (FP32 rows, unrotated; optional): per
+  // site the leg layout and the per-sample table pointers of kron.h. Empty:
+  // the Gram transfers contract the dense double layers.
+  std::vector<const KronSite*> kron_site;
+  std::vector<void* const*> kron_tabs;
   // Reference to site x with the given letters for (W,N,E,S) in the frame of this row.
   TRef ref(int x, const std::string& letters) const;
 };

@@ -1387,6 +1387,9 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
   T* Z1 = zmax ? (T*)ctx.arena.take(sizeof(T) * nb * zmax) : nullptr;
   T* Z2 = zmax ? (T*)ctx.arena.take(sizeof(T) * nb * zmax) : nullptr;
   int* rsplit = (int*)ctx.arena.take(sizeof(int) * nb);
+  bool use_kron = sizeof(T) == sizeof(float) && ctx.kron && !row.rotated && (int)row.kron_site.size() == L &&
+                  (int)row.kron_tabs.size() == L;
+  for (int x = 1; x < L && use_kron; ++x) use_kron = row.kron_site[x] && row.kron_tabs[x];
   for (int x = L - 1; x >= 1; --x) {
     const LayerSpec& sp = row.spec[x];
     Leg bl = env.bond(x), br = env.bond(x + 1);
@@ -1397,10 +1400,51 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
     TRef Gn = dense_ref(G[x + 1], (long long)Gl[x + 1][0].dim * Gl[x + 1][1].dim * Gl[x + 1][2].dim * Gl[x + 1][3].dim,
                         "reRF", Gl[x + 1]);
     TRef g = dense_ref(G[x], gs, "lwLW", Gl[x]);
+    const KronSite* ks = use_kron ? row.kron_site[x] : nullptr;
     for (int r0 = 0; r0 < br.dim; r0 += rchunk[x]) {
       const int c = std::min(rchunk[x], br.dim - r0);
       const bool whole = c == br.dim;
       const Leg rc = whole ? br : chunk_leg(ctx, br, r0, c, rsplit, nb);
+      if (ks) {
+        // rows (l, R, F) of z1 through the ket and bra factors of the layer
+        // into z2 [l][R][F][w][p'], then rows (l, R, w) of z2 into z3
+        TRef z1 = folded_ref(Z1, zmax, "lRFne", {bl, rc, sp.e, sp.n, sp.e});
+        contract<T>(ctx, z1, A, whole ? Gn : sub_leg(Gn, 'R', r0, rc), nb);
+        const long long fe = std::max(sp.e.even, sp.e.dim - sp.e.even);
+        const long long fs = std::max(sp.s.even, sp.s.dim - sp.s.even);
+        KronRows ka;
+        ka.nrows = (long long)bl.dim * rc.dim * sp.e.dim;
+        ka.nf = sp.e.dim;
+        ka.ef = sp.e.even;
+        ka.rc = rc.dim;
+        ka.lsplit = bl.split;
+        ka.leven = bl.even;
+        ka.rsplit = rc.split;
+        ka.reven = rc.even;
+        ka.xs = ka.ys = zmax;
+        ka.sx_slab = (long long)sp.e.dim * sp.n.dim * fe;
+        ka.sxf = (long long)sp.n.dim * fe;
+        ka.sx1 = fe;
+        ka.sx2 = 1;
+        SPP_CHECK(launch_kron_rows(*ks, 0, reinterpret_cast<const float*>(Z1), reinterpret_cast<float*>(Z2), ka,
+                                   row.kron_tabs[x], nb, ctx.kron_rows, st, ctx.kron_groups));
+        KronRows kb = ka;
+        kb.nrows = (long long)bl.dim * rc.dim * sp.w.dim;
+        kb.nf = sp.w.dim;
+        kb.ef = sp.w.even;
+        kb.sx_slab = (long long)sp.e.dim * sp.w.dim * fs;
+        kb.sxf = fs;
+        kb.sx1 = (long long)sp.w.dim * fs;
+        kb.sx2 = 1;
+        SPP_CHECK(launch_kron_rows(*ks, 1, reinterpret_cast<const float*>(Z2), reinterpret_cast<float*>(Z1), kb,
+                                   row.kron_tabs[x], nb, ctx.kron_rows, st, ctx.kron_groups));
+        ctx.counters.kron_launches += 2;
+        ctx.counters.gemm_flops += 2.0 * nb * (kron_row_macs(*ks, 0) * ka.nrows + kron_row_macs(*ks, 1) * kb.nrows);
+        TRef z3 = folded_ref(Z1, zmax, "lRwWN", {bl, rc, sp.w, sp.w, sp.n});
+        TRef Ar = env.ref(x, "LNR");
+        contract<T>(ctx, g, z3, whole ? Ar : sub_leg(Ar, 'R', r0, rc), nb, 1.0, r0 > 0);
+        continue;
+      }
       TRef z1 = folded_ref(Z1, zmax, "lneRF", {bl, sp.n, sp.e, rc, sp.e});
       contract<T>(ctx, z1, A, whole ? Gn : sub_leg(Gn, 'R', r0, rc), nb);
       TRef z2 = folded_ref(Z2, zmax, "lRFwp", {bl, rc, sp.e, sp.w, sp.s});

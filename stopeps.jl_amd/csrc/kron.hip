@@ -1,0 +1,577 @@
+// Row Kronecker apply of a double layer through its single-layer factors
+// (declarations and the operation in kron.h).
+//
+// Per row r of parity rho and for the single layer S' of parity q:
+//   stage 1, per bra tuple (c1b, c2b):
+//     T[c1b, c2b; phi, o1k, o2k] = sum_{c1k, c2k} X[r, (c1k, c1b), (c2k, c2b)] S'(c1k, c2k, phi, o1k, o2k)
+//   stage 2, per ket tuple (o1k, o2k):
+//     Y[r, (o1k, o1b), (o2k, o2b)] = sum_{c1b, c2b, phi} T[c1b, c2b; phi, o1k, o2k] S'(c1b, c2b, phi, o1b, o2b)
+// Z2 fixes which terms are nonzero: a stage-1 lane of class c = par(c1b, c2b) ^ rho sums over the
+// ket tuples (c1k, c2k) of parity c into the outputs (phi, o1k, o2k) of parity q ^ c; a stage-2
+// lane of class c2 = par(o1k, o2k) ^ rho sums over (c1b, c2b, phi) of parity q ^ c2 into the
+// bra tuples (o1b, o2b) of parity c2. Lanes are grouped by class so that every coefficient
+// a wave reads is the same for all its lanes (scalar loads); the coefficient blocks K1, K2
+// are tabulated per (site, label) on the host.
+//
+// One block holds `rows` rows: the X rows are staged in LDS, stage 1 writes T (rows x
+// tuples x no1) to LDS, stage 2 reads it, writes the Y rows over the X area, and the block
+// stores them contiguously.
+#include "kron.h"
+#include "engine.h"
+#include <algorithm>
+#include <cstring>
+
+namespace spp {
+
+namespace {
+
+constexpr int kKronLdsBytes = 160 * 1024;
+
+struct Inst {
+  int no1, no2;
+};
+// Register tiles (stage-1 outputs, stage-2 outputs per lane) with a kernel instantiation.
+constexpr Inst kInst[] = {{8, 8}, {16, 16}, {18, 18}, {32, 32}, {36, 18}, {64, 32}, {64, 64}};
+
+// Roles (c1, c2, o1, o2) of the layer legs W=0, N=1, E=2, S=3 per stage.
+constexpr int kRole[2][4] = {{1, 2, 0, 3}, {2, 3, 0, 1}};
+
+inline int parl(int i, int e) { return i >= e ? 1 : 0; }
+
+struct RoleLegs {
+  int d[4], e[4];
+};
+RoleLegs roles(const KronSite& ks, int stage) {
+  RoleLegs r;
+  for (int k = 0; k < 4; ++k) {
+    r.d[k] = ks.d[kRole[stage][k]];
+    r.e[k] = ks.e[kRole[stage][k]];
+  }
+  return r;
+}
+
+// tuples (a, b) of two single legs with par(a) + par(b) = pi
+int tuple_count(int da, int ea, int db, int eb, int pi) {
+  int n = 0;
+  for (int a = 0; a < da; ++a)
+    for (int b = 0; b < db; ++b)
+      if (((parl(a, ea) + parl(b, eb)) & 1) == pi) ++n;
+  return n;
+}
+
+// stage-1 outputs (phi, o1k, o2k) of parity pi
+int out1_count(const RoleLegs& r, int dphi, int ephi, int pi) {
+  int n = 0;
+  for (int f = 0; f < dphi; ++f)
+    for (int a = 0; a < r.d[2]; ++a)
+      for (int b = 0; b < r.d[3]; ++b)
+        if (((parl(f, ephi) + parl(a, r.e[2]) + parl(b, r.e[3])) & 1) == pi) ++n;
+  return n;
+}
+
+// stage-2 inner tuples (c1b, c2b, phi) of parity pi
+int inner2_count(const RoleLegs& r, int dphi, int ephi, int pi) {
+  int n = 0;
+  for (int a = 0; a < r.d[0]; ++a)
+    for (int b = 0; b < r.d[1]; ++b)
+      for (int f = 0; f < dphi; ++f)
+        if (((parl(a, r.e[0]) + parl(b, r.e[1]) + parl(f, ephi)) & 1) == pi) ++n;
+  return n;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+double layer_at(const KronSite& ks, const double* S, int w, int n, int f, int e, int s) {
+  return S[((((long long)w * ks.d[1] + n) * ks.dphi + f) * ks.d[2] + e) * ks.d[3] + s];
+}
+
+// S'(c1k, c2k, phi, o1k, o2k) of a stage
+double role_at(const KronSite& ks, const double* S, int stage, int c1, int c2, int f, int o1, int o2) {
+  int leg[4];
+  leg[kRole[stage][0]] = c1;
+  leg[kRole[stage][1]] = c2;
+  leg[kRole[stage][2]] = o1;
+  leg[kRole[stage][3]] = o2;
+  return layer_at(ks, S, leg[0], leg[1], f, leg[2], leg[3]);
+}
+
+}  // namespace
+
+bool kron_site_layout(const int d[4], const int e[4], int dphi, int ephi, KronSite& ks) {
+  for (int k = 0; k < 4; ++k) {
+    if (d[k] < 1 || d[k] > 8 || e[k] < 0 || e[k] > d[k]) return false;
+    ks.d[k] = d[k];
+    ks.e[k] = e[k];
+  }
+  if (dphi < 1 || dphi > 2 || ephi < 0 || ephi > dphi) return false;
+  ks.dphi = dphi;
+  ks.ephi = ephi;
+  int need1 = 0, need2 = 0, j1 = 0, j2 = 0;
+  for (int stage = 0; stage < 2; ++stage) {
+    const RoleLegs r = roles(ks, stage);
+    for (int pi = 0; pi < 2; ++pi) {
+      need1 = std::max(need1, out1_count(r, dphi, ephi, pi));
+      need2 = std::max(need2, tuple_count(r.d[2], r.e[2], r.d[3], r.e[3], pi));
+      j1 = std::max(j1, tuple_count(r.d[0], r.e[0], r.d[1], r.e[1], pi));
+      j2 = std::max(j2, inner2_count(r, dphi, ephi, pi));
+    }
+  }
+  int best = -1;
+  for (int i = 0; i < (int)(sizeof(kInst) / sizeof(kInst[0])); ++i)
+    if (kInst[i].no1 >= need1 && kInst[i].no2 >= need2 &&
+        (best < 0 || kInst[i].no1 + kInst[i].no2 < kInst[best].no1 + kInst[best].no2))
+      best = i;
+  if (best < 0 || j1 > kKronMaxTuple || j2 > kKronMaxJ2) return false;
+  ks.no1 = kInst[best].no1;
+  ks.no2 = kInst[best].no2;
+  ks.j1cap = j1;
+  ks.j2cap = j2;
+  ks.stage_bytes = align256(sizeof(KronTab) + sizeof(float) * (2 * (size_t)j1 * ks.no1 + 2 * (size_t)j2 * ks.no2));
+  return true;
+}
+
+double kron_row_macs(const KronSite& ks, int stage) {
+  const RoleLegs r = roles(ks, stage);
+  double macs = 0;
+  for (int rho = 0; rho < 2; ++rho)
+    for (int c = 0; c < 2; ++c) {
+      macs += (double)tuple_count(r.d[0], r.e[0], r.d[1], r.e[1], c ^ rho) *
+              tuple_count(r.d[0], r.e[0], r.d[1], r.e[1], c) * ks.no1;
+      macs += (double)tuple_count(r.d[2], r.e[2], r.d[3], r.e[3], c ^ rho) * inner2_count(r, ks.dphi, ks.ephi, c) *
+              ks.no2;
+    }
+  return 0.5 * macs;
+}
+
+int kron_layer_parity(const int d[4], const int e[4], int dphi, int ephi, const double* S) {
+  int seen[2] = {0, 0};
+  long long i = 0;
+  for (int w = 0; w < d[0]; ++w)
+    for (int n = 0; n < d[1]; ++n)
+      for (int f = 0; f < dphi; ++f)
+        for (int x = 0; x < d[2]; ++x)
+          for (int s = 0; s < d[3]; ++s, ++i)
+            if (S[i] != 0.0)
+              seen[(parl(w, e[0]) + parl(n, e[1]) + parl(f, ephi) + parl(x, e[2]) + parl(s, e[3])) & 1] = 1;
+  if (seen[0] && seen[1]) return -1;
+  return seen[1] ? 1 : 0;
+}
+
+void kron_build_tables(const KronSite& ks, const double* S, int q, std::vector<char>& out) {
+  out.assign(2 * ks.stage_bytes, 0);
+  for (int stage = 0; stage < 2; ++stage) {
+    const RoleLegs r = roles(ks, stage);
+    char* base = out.data() + stage * ks.stage_bytes;
+    KronTab* tab = reinterpret_cast<KronTab*>(base);
+    tab->q = q;
+    float* K1 = reinterpret_cast<float*>(base + sizeof(KronTab));
+    float* K2 = K1 + 2 * (size_t)ks.j1cap * ks.no1;
+    for (int pi = 0; pi < 2; ++pi) {
+      int n = 0;
+      for (int a = 0; a < r.d[0]; ++a)
+        for (int b = 0; b < r.d[1]; ++b)
+          if (((parl(a, r.e[0]) + parl(b, r.e[1])) & 1) == pi) tab->p12[pi][n++] = a | b << 8;
+      tab->n12[pi] = n;
+      n = 0;
+      for (int a = 0; a < r.d[2]; ++a)
+        for (int b = 0; b < r.d[3]; ++b)
+          if (((parl(a, r.e[2]) + parl(b, r.e[3])) & 1) == pi) tab->po[pi][n++] = a | b << 8;
+      tab->no[pi] = n;
+      // rank offsets of the stage-1 outputs of parity pi, lexicographic in (phi, o1k, o2k)
+      int cnt = 0;
+      for (int f = 0; f < ks.dphi; ++f)
+        for (int a = 0; a < r.d[2]; ++a) {
+          tab->pre[pi][f * r.d[2] + a] = cnt;
+          for (int b = 0; b < r.d[3]; ++b)
+            if (((parl(f, ks.ephi) + parl(a, r.e[2]) + parl(b, r.e[3])) & 1) == pi) ++cnt;
+        }
+    }
+    for (int c = 0; c < 2; ++c) {
+      // stage 1, class c: ket tuples of parity c, outputs of parity q ^ c
+      const int po = q ^ c;
+      for (int j = 0; j < tab->n12[c]; ++j) {
+        const int c1 = tab->p12[c][j] & 255, c2 = tab->p12[c][j] >> 8;
+        int o = 0;
+        for (int f = 0; f < ks.dphi; ++f)
+          for (int a = 0; a < r.d[2]; ++a)
+            for (int b = 0; b < r.d[3]; ++b)
+              if (((parl(f, ks.ephi) + parl(a, r.e[2]) + parl(b, r.e[3])) & 1) == po)
+                K1[((size_t)c * ks.j1cap + j) * ks.no1 + o++] = (float)role_at(ks, S, stage, c1, c2, f, a, b);
+      }
+      // stage 2, class c: inner tuples of parity q ^ c, bra outputs of parity c
+      int n = 0;
+      for (int a = 0; a < r.d[0]; ++a)
+        for (int b = 0; b < r.d[1]; ++b)
+          for (int f = 0; f < ks.dphi; ++f) {
+            if (((parl(a, r.e[0]) + parl(b, r.e[1]) + parl(f, ks.ephi)) & 1) != (q ^ c)) continue;
+            tab->j2[c][n] = (a * r.d[1] + b) | f << 16;
+            for (int o = 0; o < tab->no[c]; ++o) {
+              const int o1 = tab->po[c][o] & 255, o2 = tab->po[c][o] >> 8;
+              K2[((size_t)c * ks.j2cap + n) * ks.no2 + o] = (float)role_at(ks, S, stage, a, b, f, o1, o2);
+            }
+            ++n;
+          }
+      tab->nj2[c] = n;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+struct KronArgs {
+  int d[4], e[4];          // single legs of the roles c1, c2, o1, o2
+  int pe[4], pf[4], pd[4]; // their pair legs: even count, folded length, dimension
+  int dphi, ephi;
+  long long nrows;
+  int nf, ef, rc;
+  const int* lsplit;
+  int leven;
+  const int* rsplit;
+  int reven;
+  const float* x;
+  long long xs, sx_slab, sxf, sx1, sx2;
+  float* y;
+  long long ys, rowY;
+  void* const* tabs;
+  long long stage_off;
+  int rows, groups, rowmax, w1, w2, j1cap, j2cap, ntup1;
+  float inv_rowx, inv_pf1, inv_rowy;  // 1 / (pd[0] * pf[1]), 1 / pf[1], 1 / rowY
+  int sx1i, sx2i;                     // sx1, sx2 (a sample's X has fewer than 2^31 elements)
+  int xflat;                          // X rows are packed and consecutive in global memory
+};
+
+// n / m for 0 <= n < 2^22 and 1 <= m <= 4096 from inv = 1 / m: (n + 1/2) / m
+// stays at least 1/(2m) away from an integer, far beyond the FP32 error.
+__device__ __forceinline__ int small_div(int n, float inv) { return (int)(((float)n + 0.5f) * inv); }
+
+__device__ __forceinline__ int pair_base(int pi, int pj, int e, int o) {
+  // parity-sorted pair blocks: ee, oo, eo, oe
+  return pi == pj ? (pi ? e * e : 0) : (e * e + o * o + (pi ? e * o : 0));
+}
+
+constexpr int kKronPrefetch = 16;  // X loads in flight per thread
+
+template <int NO1, int NO2>
+__global__ void __launch_bounds__(1024) kron_rows_kernel(KronArgs a) {
+  constexpr int kNO1P = NO1 | 1;  // odd stride of a tuple's stage-1 outputs in LDS (no bank conflicts)
+  extern __shared__ float kron_lds[];
+  __shared__ int rho[32];
+  __shared__ int xrow0[32];
+  const int s = blockIdx.y;
+  const char* tb = reinterpret_cast<const char*>(a.tabs[s]) + a.stage_off;
+  const KronTab* tab = reinterpret_cast<const KronTab*>(tb);
+  const float* K1 = reinterpret_cast<const float*>(tb + sizeof(KronTab));
+  const float* K2 = K1 + 2 * a.j1cap * NO1;
+  const long long row0 = (long long)blockIdx.x * a.rows;
+  const int nr = (int)min((long long)a.rows, a.nrows - row0);
+  float* Xl = kron_lds;
+  float* Tl = kron_lds + a.rows * a.rowmax;
+  const float* xs = a.x + (long long)s * a.xs;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  if (tid < nr) {
+    const int el = a.lsplit ? a.lsplit[s] : a.leven;
+    const int er = a.rsplit ? a.rsplit[s] : a.reven;
+    const long long r = row0 + tid;
+    const long long slab = r / a.nf;
+    const int f = (int)(r - slab * a.nf);
+    const int l = (int)(slab / a.rc), R = (int)(slab - (long long)l * a.rc);
+    rho[tid] = ((l >= el) + (R >= er) + (f >= a.ef)) & 1;
+    xrow0[tid] = (int)(slab * a.sx_slab + f * a.sxf);
+  }
+  __syncthreads();
+  // X rows into LDS (packed, row stride rowX), kKronPrefetch independent loads in flight per thread
+  const int rowX = a.pd[0] * a.pf[1];
+  if (a.xflat) {
+    // consecutive rows are contiguous in global memory
+    const float* src = xs + xrow0[0];
+    const int total = nr * rowX;
+    for (int base = 0; base < total; base += kKronPrefetch * nthr) {
+      float v[kKronPrefetch];
+#pragma unroll
+      for (int u = 0; u < kKronPrefetch; ++u) {
+        const int i = base + u * nthr + tid;
+        if (i < total) v[u] = src[i];
+      }
+#pragma unroll
+      for (int u = 0; u < kKronPrefetch; ++u) {
+        const int i = base + u * nthr + tid;
+        if (i < total) Xl[i] = v[u];
+      }
+    }
+  } else {
+    const bool contig = a.sx2i == 1 && a.sx1i == a.pf[1];
+    const int total = nr * rowX;
+    for (int base = 0; base < total; base += kKronPrefetch * nthr) {
+      float v[kKronPrefetch];
+#pragma unroll
+      for (int u = 0; u < kKronPrefetch; ++u) {
+        const int i = base + u * nthr + tid;
+        if (i < total) {
+          const int rr = small_div(i, a.inv_rowx), k = i - rr * rowX;
+          int off = xrow0[rr];
+          if (contig) {
+            off += k;
+          } else {
+            const int c1 = small_div(k, a.inv_pf1);
+            off += c1 * a.sx1i + (k - c1 * a.pf[1]) * a.sx2i;
+          }
+          v[u] = xs[off];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kKronPrefetch; ++u) {
+        const int i = base + u * nthr + tid;
+        if (i < total) Xl[i] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  // ---------------- stage 1 ----------------
+  if (wave < 2 * a.w1) {
+    const int c = wave >= a.w1 ? 1 : 0;
+    int t = (wave - c * a.w1) * 64 + lane;
+    int r = 0, tup = 0;
+    bool active = false;
+    const int n12c0 = tab->n12[0], n12c1 = tab->n12[1];
+    for (int rr = 0; rr < nr; ++rr) {
+      const int pi = c ^ rho[rr];
+      const int n = pi ? n12c1 : n12c0;
+      if (t < n) {
+        r = rr;
+        tup = tab->p12[pi][t];
+        active = true;
+        break;
+      }
+      t -= n;
+    }
+    const int e1 = a.e[0], o1 = a.d[0] - a.e[0], e2 = a.e[1], o2 = a.d[1] - a.e[1];
+    const int c1b = tup & 255, c2b = tup >> 8;
+    const int p1b = c1b >= e1 ? 1 : 0, p2b = c2b >= e2 ? 1 : 0;
+    const int r1b = c1b - (p1b ? e1 : 0), r2b = c2b - (p2b ? e2 : 0);
+    const int mul1 = p1b ? o1 : e1, mul2 = p2b ? o2 : e2;
+    const int b10 = pair_base(0, p1b, e1, o1) + r1b, b11 = pair_base(1, p1b, e1, o1) + r1b;
+    const int b20 = pair_base(0, p2b, e2, o2) + r2b - (p2b ? a.pe[1] : 0);
+    const int b21 = pair_base(1, p2b, e2, o2) + r2b - (p2b ? 0 : a.pe[1]);
+    const float* xrow = Xl + r * rowX;
+    float acc[NO1];
+#pragma unroll
+    for (int o = 0; o < NO1; ++o) acc[o] = 0.f;
+    // ket tuples (c1k, c2k) of parity c in the tables' lexicographic order
+    const float* kj = K1 + c * a.j1cap * NO1;
+    for (int c1k = 0; c1k < a.d[0]; ++c1k) {
+      const int p1k = c1k >= e1 ? 1 : 0;
+      const int p2k = c ^ p1k;
+      const int n2 = p2k ? o2 : e2;
+      const float* xp = xrow + ((p1k ? b11 : b10) + (c1k - (p1k ? e1 : 0)) * mul1) * a.pf[1] + (p2k ? b21 : b20);
+#pragma unroll 2
+      for (int r2k = 0; r2k < n2; ++r2k) {
+        const float xv = active ? xp[r2k * mul2] : 0.f;
+#pragma unroll
+        for (int o = 0; o < NO1; ++o) acc[o] = fmaf(xv, kj[o], acc[o]);
+        kj += NO1;
+      }
+    }
+    if (active) {
+      float* tp = Tl + (r * a.ntup1 + c1b * a.d[1] + c2b) * kNO1P;
+#pragma unroll
+      for (int o = 0; o < NO1; ++o) tp[o] = acc[o];
+    }
+  }
+  __syncthreads();
+  // ---------------- stage 2 ----------------
+  if (wave < 2 * a.w2) {
+    const int c = wave >= a.w2 ? 1 : 0;
+    int t = (wave - c * a.w2) * 64 + lane;
+    int r = 0, tup = 0;
+    bool active = false;
+    const int noc0 = tab->no[0], noc1 = tab->no[1];
+    for (int rr = 0; rr < nr; ++rr) {
+      const int pi = c ^ rho[rr];
+      const int n = pi ? noc1 : noc0;
+      if (t < n) {
+        r = rr;
+        tup = tab->po[pi][t];
+        active = true;
+        break;
+      }
+      t -= n;
+    }
+    const int eo1 = a.e[2], oo1 = a.d[2] - a.e[2], eo2 = a.e[3], oo2 = a.d[3] - a.e[3];
+    const int o1k = tup & 255, o2k = tup >> 8;
+    const int p1k = o1k >= eo1 ? 1 : 0, p2k = o2k >= eo2 ? 1 : 0;
+    const int r1k = o1k - (p1k ? eo1 : 0), r2k = o2k - (p2k ? eo2 : 0);
+    const int pf0 = 0 >= a.ephi ? 1 : 0;
+    const int rk0 = tab->pre[pf0 ^ p1k ^ p2k][o1k] + r2k;
+    const int rk1 = a.dphi > 1 ? tab->pre[(1 >= a.ephi ? 1 : 0) ^ p1k ^ p2k][a.d[2] + o1k] + r2k : 0;
+    const float* trow = Tl + r * a.ntup1 * kNO1P;
+    float acc[NO2];
+#pragma unroll
+    for (int o = 0; o < NO2; ++o) acc[o] = 0.f;
+    // inner tuples (c1b, c2b, phi) of parity q ^ c in the tables' lexicographic order
+    const int want = tab->q ^ c;
+    const float* kj = K2 + c * a.j2cap * NO2;
+    const int e1 = a.e[0], e2 = a.e[1];
+    for (int c1b = 0; c1b < a.d[0]; ++c1b) {
+      const int p1b = c1b >= e1 ? 1 : 0;
+      const float* tp = trow + c1b * a.d[1] * kNO1P;
+      if (a.dphi == 1) {
+        const int p2b = want ^ p1b ^ pf0;
+        const int lo = p2b ? e2 : 0, hi = p2b ? a.d[1] : e2;
+#pragma unroll 2
+        for (int c2b = lo; c2b < hi; ++c2b) {
+          const float tv = active ? tp[c2b * kNO1P + rk0] : 0.f;
+#pragma unroll
+          for (int o = 0; o < NO2; ++o) acc[o] = fmaf(tv, kj[o], acc[o]);
+          kj += NO2;
+        }
+      } else {
+        for (int c2b = 0; c2b < a.d[1]; ++c2b) {
+          const int p2b = c2b >= e2 ? 1 : 0;
+          for (int f = 0; f < 2; ++f) {
+            if ((p1b ^ p2b ^ (f >= a.ephi ? 1 : 0)) != want) continue;
+            const float tv = active ? tp[c2b * kNO1P + (f ? rk1 : rk0)] : 0.f;
+#pragma unroll
+            for (int o = 0; o < NO2; ++o) acc[o] = fmaf(tv, kj[o], acc[o]);
+            kj += NO2;
+          }
+        }
+      }
+    }
+    if (active) {
+      // Y rows (packed, row stride rowY) overwrite the X rows (X was last read
+      // before the barrier above); the pair index of (o1k, o1b) is
+      // A1[par o1b] + rank of o1b, that of (o2k, o2b) folded A2[par o2b] + rank of o2b
+      float* yrow = Xl + r * (int)a.rowY;
+      const int A10 = pair_base(p1k, 0, eo1, oo1) + r1k * eo1, A11 = pair_base(p1k, 1, eo1, oo1) + r1k * oo1;
+      const int A20 = pair_base(p2k, 0, eo2, oo2) + r2k * eo2 - (p2k ? a.pe[3] : 0);
+      const int A21 = pair_base(p2k, 1, eo2, oo2) + r2k * oo2 - (p2k ? 0 : a.pe[3]);
+      const int no = tab->no[c];  // bra tuples (o1b, o2b) of parity c, wave-uniform
+#pragma unroll
+      for (int o = 0; o < NO2; ++o) {
+        if (o < no) {
+          const int ot = tab->po[c][o];
+          const int o1b = ot & 255, o2b = ot >> 8;
+          const int q1b = o1b >= eo1 ? 1 : 0, q2b = o2b >= eo2 ? 1 : 0;
+          const int po1 = (q1b ? A11 : A10) + (o1b - (q1b ? eo1 : 0));
+          const int po2 = (q2b ? A21 : A20) + (o2b - (q2b ? eo2 : 0));
+          yrow[po1 * a.pf[3] + po2] = acc[o];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  float* ys = a.y + (long long)s * a.ys + row0 * a.rowY;
+  const int totY = nr * (int)a.rowY;
+  for (int i = tid; i < totY; i += nthr) ys[i] = Xl[i];
+}
+
+template <int NO1, int NO2>
+static hipError_t launch_inst(const KronArgs& a, dim3 grid, dim3 block, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL((kron_rows_kernel<NO1, NO2>), grid, block, lds, st, a);
+  return hipGetLastError();
+}
+
+template <int NO1, int NO2>
+static void raise_lds() {
+  (void)hipFuncSetAttribute((const void*)kron_rows_kernel<NO1, NO2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            kKronLdsBytes - 1024);
+}
+
+void kron_prepare() {
+  static bool done = false;
+  if (done) return;
+  raise_lds<8, 8>();
+  raise_lds<16, 16>();
+  raise_lds<18, 18>();
+  raise_lds<32, 32>();
+  raise_lds<36, 18>();
+  raise_lds<64, 32>();
+  raise_lds<64, 64>();
+  done = true;
+}
+
+hipError_t launch_kron_rows(const KronSite& ks, int stage, const float* x, float* y, const KronRows& rw,
+                            void* const* tabs, int nb, int rows_per_block, hipStream_t st, int groups) {
+  if (nb <= 0 || rw.nrows <= 0) return hipSuccess;
+  KronArgs a{};
+  const RoleLegs r = roles(ks, stage);
+  for (int k = 0; k < 4; ++k) {
+    a.d[k] = r.d[k];
+    a.e[k] = r.e[k];
+    const int o = r.d[k] - r.e[k];
+    a.pd[k] = r.d[k] * r.d[k];
+    a.pe[k] = r.e[k] * r.e[k] + o * o;
+    a.pf[k] = std::max(a.pe[k], a.pd[k] - a.pe[k]);
+  }
+  a.dphi = ks.dphi;
+  a.ephi = ks.ephi;
+  a.nrows = rw.nrows;
+  a.nf = rw.nf;
+  a.ef = rw.ef;
+  a.rc = rw.rc;
+  a.lsplit = rw.lsplit;
+  a.leven = rw.leven;
+  a.rsplit = rw.rsplit;
+  a.reven = rw.reven;
+  a.x = x;
+  a.xs = rw.xs;
+  a.sx_slab = rw.sx_slab;
+  a.sxf = rw.sxf;
+  a.sx1 = rw.sx1;
+  a.sx2 = rw.sx2;
+  a.y = y;
+  a.ys = rw.ys;
+  a.rowY = (long long)a.pd[2] * a.pf[3];
+  a.tabs = tabs;
+  a.stage_off = (long long)stage * (long long)ks.stage_bytes;
+  a.j1cap = ks.j1cap;
+  a.j2cap = ks.j2cap;
+  a.ntup1 = r.d[0] * r.d[1];
+  a.rowmax = std::max(a.pd[0] * a.pf[1], a.pd[2] * a.pf[3]);
+  int n1 = 0, n2 = 0;
+  for (int pi = 0; pi < 2; ++pi) {
+    n1 = std::max(n1, tuple_count(r.d[0], r.e[0], r.d[1], r.e[1], pi));
+    n2 = std::max(n2, tuple_count(r.d[2], r.e[2], r.d[3], r.e[3], pi));
+  }
+  int rows = rows_per_block > 0 ? std::min(rows_per_block, 32) : 5;
+  auto fits = [&](int R) {
+    const int w1 = (R * n1 + 63) / 64, w2 = (R * n2 + 63) / 64;
+    const size_t lds = sizeof(float) * ((size_t)R * a.rowmax + (size_t)R * a.ntup1 * (ks.no1 | 1));
+    return 64 * 2 * std::max(w1, w2) <= 1024 && lds <= (size_t)kKronLdsBytes - 1024;
+  };
+  while (rows > 1 && !fits(rows)) --rows;
+  if (!fits(rows)) return hipErrorInvalidValue;
+  a.rows = rows;
+  a.inv_rowx = 1.0f / (float)(a.pd[0] * a.pf[1]);
+  a.inv_pf1 = 1.0f / (float)a.pf[1];
+  a.inv_rowy = 1.0f / (float)a.rowY;
+  // 32-bit element offsets within a sample's X
+  if (rw.sx_slab * ((rw.nrows + rw.nf - 1) / rw.nf) >= (1LL << 31) || rw.sx1 >= (1LL << 31) || rw.sx2 >= (1LL << 31))
+    return hipErrorInvalidValue;
+  a.sx1i = (int)rw.sx1;
+  a.sx2i = (int)rw.sx2;
+  a.xflat = rw.sx2 == 1 && rw.sx1 == a.pf[1] && rw.sxf == (long long)a.pd[0] * a.pf[1] &&
+            rw.sx_slab == (long long)rw.nf * rw.sxf;
+  if ((long long)rows * a.rowmax >= (1 << 22) || a.rowmax > 4096) return hipErrorInvalidValue;
+  a.groups = 1;
+  (void)groups;
+  a.w1 = (rows * n1 + 63) / 64;
+  a.w2 = (rows * n2 + 63) / 64;
+  const size_t lds = sizeof(float) * ((size_t)rows * a.rowmax + (size_t)rows * a.ntup1 * (ks.no1 | 1));
+  const long long nblk = (rw.nrows + (long long)rows * a.groups - 1) / ((long long)rows * a.groups);
+  if (nblk > 0x7fffffffLL || nb > 65535) return hipErrorInvalidValue;
+  dim3 grid((unsigned)nblk, (unsigned)nb), block((unsigned)(64 * 2 * std::max(a.w1, a.w2)));
+  switch (ks.no1 * 100 + ks.no2) {
+    case 808: return launch_inst<8, 8>(a, grid, block, lds, st);
+    case 1616: return launch_inst<16, 16>(a, grid, block, lds, st);
+    case 1818: return launch_inst<18, 18>(a, grid, block, lds, st);
+    case 3232: return launch_inst<32, 32>(a, grid, block, lds, st);
+    case 3618: return launch_inst<36, 18>(a, grid, block, lds, st);
+    case 6432: return launch_inst<64, 32>(a, grid, block, lds, st);
+    case 6464: return launch_inst<64, 64>(a, grid, block, lds, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace spp

@@ -158,6 +158,12 @@ struct CacheDev {
   int target_charge = -1;
   std::vector<int> reach;
   std::unique_ptr<DrawWork<T>> work;  // draw buffers and captured sweeps (first draw on)
+  // Single-layer tables of the candidates for the factorised Gram transfers
+  // of the draw's absorbs (FP32 caches whose every candidate fits, kron.h):
+  // per site the layout, per (site, label) the tables inside kron_buf.
+  std::vector<KronSite> kron_site;
+  std::vector<void*> kron_tab[2];
+  char* kron_buf = nullptr;
   ~CacheDev();
 };
 

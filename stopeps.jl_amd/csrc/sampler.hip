@@ -42,6 +42,7 @@ CacheDev<T>::~CacheDev() {
     if (ps.a) (void)hipFree(ps.a);
     if (ps.inv) (void)hipFree(ps.inv);
   }
+  if (kron_buf) (void)hipFree(kron_buf);
 }
 template struct CacheDev<float>;
 template struct CacheDev<double>;
@@ -343,10 +344,71 @@ __global__ void live_trunc_kernel(const double* mt, const int* done, double* out
   if (!done[s]) out[s] = mt[s];
 }
 
-__global__ void make_ptrs_kernel(const int* choice, void* c0, void* c1, void** ptrs, int nb) {
+// Per-sample pointers of the drawn candidate's layer and (k0, k1 non-null)
+// of its single-layer tables.
+__global__ void make_ptrs_kernel(const int* choice, void* c0, void* c1, void** ptrs, void* k0, void* k1,
+                                 void** kptrs, int nb) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nb) return;
   ptrs[s] = choice[s] == 0 ? c0 : c1;
+  if (k0) kptrs[s] = choice[s] == 0 ? k0 : k1;
+}
+
+// Single-layer tables of every (site, label) candidate of an FP32 cache: the
+// candidate's layer is sum_phi S_k (x) S_k with S_k the aux projection
+// sum_a U[a, k] A[w, n, phi, a, e, s] (auxiliary kind) or the physical one
+// sum_p u_k[p] A[w, n, p, 0, e, s] (physical kind, one inner index). Left
+// empty when a candidate carries both parities or a leg does not fit.
+template <typename T>
+static void build_kron(Context& ctx, const StateDev& st_, CacheDev<T>& c) {
+  if (sizeof(T) != sizeof(float)) return;
+  const int ns = (int)st_.sites.size();
+  std::vector<KronSite> ks(ns);
+  std::vector<std::vector<char>> tabs(2 * ns);
+  for (int i = 0; i < ns; ++i) {
+    const SiteDev& sd = st_.sites[i];
+    std::vector<double> a(sd.size());
+    SPP_CHECK(hipMemcpy(a.data(), sd.d, sizeof(double) * a.size(), hipMemcpyDeviceToHost));
+    const int d[4] = {sd.dim[0], sd.dim[1], sd.dim[4], sd.dim[5]};
+    const int e[4] = {sd.even[0], sd.even[1], sd.even[4], sd.even[5]};
+    const int dp = sd.dim[2], da = sd.dim[3];
+    const int dphi = c.kind == 0 ? dp : 1, ephi = c.kind == 0 ? sd.even[2] : 1;
+    if (!kron_site_layout(d, e, dphi, ephi, ks[i])) return;
+    const long long vol = (long long)d[0] * d[1] * d[2] * d[3];
+    for (int k = 0; k < 2; ++k) {
+      std::vector<double> S((size_t)vol * dphi, 0.0);
+      for (int w = 0; w < d[0]; ++w)
+        for (int n = 0; n < d[1]; ++n)
+          for (int x = 0; x < d[2]; ++x)
+            for (int y = 0; y < d[3]; ++y)
+              for (int p = 0; p < dp; ++p)
+                for (int q = 0; q < da; ++q) {
+                  const double v = a[((((((long long)w * d[1] + n) * dp + p) * da + q) * d[2] + x) * d[3] + y)];
+                  const int f = c.kind == 0 ? p : 0;
+                  const double u = c.kind == 0 ? c.basis[q * 2 + k] : c.basis[p * 2 + k];
+                  S[((((long long)w * d[1] + n) * dphi + f) * d[2] + x) * d[3] + y] += u * v;
+                }
+      const int q = kron_layer_parity(d, e, dphi, ephi, S.data());
+      if (q < 0) return;
+      kron_build_tables(ks[i], S.data(), q, tabs[2 * i + k]);
+    }
+  }
+  size_t total = 0;
+  for (auto& t : tabs) total += t.size();
+  SPP_CHECK(hipMalloc(&c.kron_buf, total));
+  c.kron_tab[0].assign(ns, nullptr);
+  c.kron_tab[1].assign(ns, nullptr);
+  size_t off = 0;
+  for (int i = 0; i < ns; ++i)
+    for (int k = 0; k < 2; ++k) {
+      const auto& t = tabs[2 * i + k];
+      SPP_CHECK(hipMemcpy(c.kron_buf + off, t.data(), t.size(), hipMemcpyHostToDevice));
+      c.kron_tab[k][i] = c.kron_buf + off;
+      off += t.size();
+    }
+  c.kron_site = ks;
+  kron_prepare();
+  (void)ctx;
 }
 
 // ---------------------------------------------------------------------------
@@ -575,6 +637,7 @@ void build_cache(Context& ctx, const StateDev& pepo, int chi, const double* basi
   SPP_CHECK(hipMalloc(&djobs, sizeof(LayerJob) * jobs.size()));
   upload(djobs, jobs.data(), sizeof(LayerJob) * jobs.size(), st);
   SPP_CHECK(launch_double_layers<T>(djobs, (int)jobs.size(), maxout, st));
+  build_kron<T>(ctx, pepo, c);
   SPP_CHECK(hipMalloc(&c.cand_lognorm, sizeof(double) * 2 * ns));
   if (kind == 1) SPP_CHECK(launch_fill<double>(c.cand_lognorm, 2 * ns, 0.0, st));
   for (int i = 0; i < ns && kind == 0; ++i) {
@@ -659,7 +722,7 @@ void DrawWork<T>::allocate(int nb_, int ns_, int lx_, long long capmax_, bool ph
   SPP_CHECK(hipMalloc(&mtlive, sizeof(double) * nb));
   SPP_CHECK(hipMalloc(&duni, sizeof(double) * nb * ns));
   if (physical) SPP_CHECK(hipMalloc(&drep, sizeof(double) * (size_t)nb * ns * 6));
-  SPP_CHECK(hipMalloc(&ptrs, sizeof(void*) * nb * lx));
+  SPP_CHECK(hipMalloc(&ptrs, sizeof(void*) * 2 * nb * lx));  // layers, then single-layer tables
   SPP_CHECK(hipMalloc(&dsst, sizeof(long long) * lx));
   SPP_CHECK(hipMalloc(&dtens, sizeof(T*) * lx));
   prepare_boundary(NB[0], lx, nb, capmax);
@@ -727,6 +790,10 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
   double *logp = W.logp, *cond = W.cond, *raw0 = W.raw0, *raw1 = W.raw1, *nrm = W.nrm, *fac = W.fac;
   double *lnamp = W.lnamp, *mant = W.mant;
   void** ptrs = W.ptrs;
+  void** kptrs = W.ptrs + (size_t)nb * Lx;
+  const bool kron = !c.kron_site.empty();
+  auto kron0 = [&](int site) -> void* { return kron ? c.kron_tab[0][site] : nullptr; };
+  auto kron1 = [&](int site) -> void* { return kron ? c.kron_tab[1][site] : nullptr; };
   int* dlabels_in = labels_in ? W.dlab : nullptr;
   double* duni = uniforms ? W.duni : nullptr;
   long long* dsst = W.dsst;
@@ -884,7 +951,8 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
                                    c.reach[site + 1], prefix, choice, labels, logp, cond, status, fail, done, drep,
                                    cur);
             hipLaunchKernelGGL(make_ptrs_kernel, dim3(gb), dim3(tb), 0, st, choice, (void*)c.cand[0][site],
-                               (void*)c.cand[1][site], ptrs + (long long)x * cur, cur);
+                               (void*)c.cand[1][site], ptrs + (long long)x * cur, kron0(site), kron1(site),
+                               kptrs + (long long)x * cur, cur);
             SPP_CHECK(hipGetLastError());
             if (x < Lx - 1) {
               // left <- safe_scale(left_step of the drawn candidate)
@@ -917,7 +985,8 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
                              site, choice, labels, logp, cond, status, fail, done, cur);
           SPP_CHECK(hipGetLastError());
           hipLaunchKernelGGL(make_ptrs_kernel, dim3(gb), dim3(tb), 0, st, choice, (void*)c.cand[0][site],
-                             (void*)c.cand[1][site], ptrs + (long long)x * cur, cur);
+                             (void*)c.cand[1][site], ptrs + (long long)x * cur, kron0(site), kron1(site),
+                             kptrs + (long long)x * cur, cur);
           SPP_CHECK(hipGetLastError());
           if (x < Lx - 1) {
             // left <- safe_scale(left_step of the drawn candidate)
@@ -969,6 +1038,11 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
         row.spec.assign(c.spec.begin() + y * Lx, c.spec.begin() + (y + 1) * Lx);
         row.ptrs.resize(Lx);
         for (int x = 0; x < Lx; ++x) row.ptrs[x] = ptrs + (long long)x * cur;
+        if (kron)
+          for (int x = 0; x < Lx; ++x) {
+            row.kron_site.push_back(&c.kron_site[y * Lx + x]);
+            row.kron_tabs.push_back(kptrs + (long long)x * cur);
+          }
         absorb_forward<T>(ctx, Nc, row, chi, NB[1 - which]);
         which = 1 - which;
         hipLaunchKernelGGL(live_trunc_kernel, dim3(gb), dim3(tb), 0, st, NB[which].max_trunc, done, mtlive, cur);

@@ -49,8 +49,8 @@ class Context:
         _lib.check(self.handle, self._lib.spp_set_option(self.handle, name.encode(), float(value)))
 
     def counters(self):
-        out = np.zeros(22)
-        rc = self._lib.spp_get_counters(self.handle, _lib.dptr(out), 22)
+        out = np.zeros(23)
+        rc = self._lib.spp_get_counters(self.handle, _lib.dptr(out), 23)
         if rc < 0:
             _lib.check(self.handle, rc)
         return {"gemm_flops": out[0], "gemm_launches": out[1], "subspace_iterations": out[2],
@@ -59,7 +59,8 @@ class Context:
                 "fp32": {"launches": out[9], "ms": out[10], "flops": out[11]},
                 "eig_calls": out[12], "eig_iterations": out[13], "eig_filter_gemms": out[14],
                 "fu_repairs": out[15], "tuned_classes": out[16], "tuned_changed": out[17], "tune_ms": out[18],
-                "graph_captures": out[19], "graph_replays": out[20], "eig_unconverged": out[21]}
+                "graph_captures": out[19], "graph_replays": out[20], "eig_unconverged": out[21],
+                "kron_launches": out[22]}
 
     def reset_counters(self):
         self._lib.spp_reset_counters(self.handle)
