@@ -96,7 +96,7 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value);
  * contraction kernel milliseconds and counters[5] = the flops of the timed
  * launches (profiling mode); counters[6..8] = launches, milliseconds and flops
  * of the timed FP64-kernel launches, counters[9..11] the same for the FP32
- * kernel; counters[12..14] = iterative truncation eigensolves, their summed
+ * kernels (the contraction kernel and the factorised Gram transfer kernel); counters[12..14] = iterative truncation eigensolves, their summed
  * subspace iterations and their Chebyshev filter products; counters[15] = full-
  * update bonds repaired by the PSD projection of their hole; counters[16..18] = launch
  * classes the tile autotuner tuned, those whose fastest shape differs from the

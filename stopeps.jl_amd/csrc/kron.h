@@ -19,20 +19,21 @@
 namespace spp {
 
 constexpr int kKronMaxTuple = 64;   // tuples of two single legs (D <= 8)
-constexpr int kKronMaxJ2 = 128;     // tuples of two single legs and the inner index
 
 // Lists of one stage (role assignment) of one (site, label), followed in
-// memory by the FP32 coefficients K1 [2][j1cap][no1] and K2 [2][j2cap][no2].
+// memory by the FP32 coefficients K1 [2][dphi][j1cap][no1] and
+// K2 [2][dphi][j2cap][no2]. The inner index phi is taken one value at a time:
+// pass phi of stage 1 yields the outputs (o1k, o2k) of that phi, pass phi of
+// stage 2 sums them.
 struct KronTab {
   int n12[2];                    // tuples (c1, c2) of parity pi
   int no[2];                     // tuples (o1, o2) of parity pi
-  int nj2[2];                    // stage-2 inner tuples (c1b, c2b, phi) of class c
+  int nj2[2][2];                 // stage-2 inner tuples (c1b, c2b) of class c, pass phi
   int q;                         // parity of the single layer
   int pad;
   int p12[2][kKronMaxTuple];     // c1 | c2 << 8
   int po[2][kKronMaxTuple];      // o1 | o2 << 8
-  int j2[2][kKronMaxJ2];         // (c1b * d_c2 + c2b) | phi << 16
-  int pre[2][16];                // rank of (phi, o1k, first o2k) among the stage-1 outputs of parity pi
+  int j2[2][2][kKronMaxTuple];   // c1b * d_c2 + c2b
 };
 
 // Single legs of one site's layer and the table layout of the site.

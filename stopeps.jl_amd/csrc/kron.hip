@@ -31,7 +31,7 @@ struct Inst {
   int no1, no2;
 };
 // Register tiles (stage-1 outputs, stage-2 outputs per lane) with a kernel instantiation.
-constexpr Inst kInst[] = {{8, 8}, {16, 16}, {18, 18}, {32, 32}, {36, 18}, {64, 32}, {64, 64}};
+constexpr Inst kInst[] = {{8, 8}, {16, 16}, {18, 18}, {32, 32}, {64, 64}};
 
 // Roles (c1, c2, o1, o2) of the layer legs W=0, N=1, E=2, S=3 per stage.
 constexpr int kRole[2][4] = {{1, 2, 0, 3}, {2, 3, 0, 1}};
@@ -106,27 +106,28 @@ bool kron_site_layout(const int d[4], const int e[4], int dphi, int ephi, KronSi
   if (dphi < 1 || dphi > 2 || ephi < 0 || ephi > dphi) return false;
   ks.dphi = dphi;
   ks.ephi = ephi;
-  int need1 = 0, need2 = 0, j1 = 0, j2 = 0;
+  // per pass the stage-1 outputs (o1k, o2k) and the stage-2 bra outputs
+  // (o1b, o2b) are tuples of one parity; the ket / bra inner lists too
+  int need = 0, j1 = 0;
   for (int stage = 0; stage < 2; ++stage) {
     const RoleLegs r = roles(ks, stage);
     for (int pi = 0; pi < 2; ++pi) {
-      need1 = std::max(need1, out1_count(r, dphi, ephi, pi));
-      need2 = std::max(need2, tuple_count(r.d[2], r.e[2], r.d[3], r.e[3], pi));
+      need = std::max(need, tuple_count(r.d[2], r.e[2], r.d[3], r.e[3], pi));
       j1 = std::max(j1, tuple_count(r.d[0], r.e[0], r.d[1], r.e[1], pi));
-      j2 = std::max(j2, inner2_count(r, dphi, ephi, pi));
     }
   }
   int best = -1;
   for (int i = 0; i < (int)(sizeof(kInst) / sizeof(kInst[0])); ++i)
-    if (kInst[i].no1 >= need1 && kInst[i].no2 >= need2 &&
+    if (kInst[i].no1 >= need && kInst[i].no2 >= need &&
         (best < 0 || kInst[i].no1 + kInst[i].no2 < kInst[best].no1 + kInst[best].no2))
       best = i;
-  if (best < 0 || j1 > kKronMaxTuple || j2 > kKronMaxJ2) return false;
+  if (best < 0 || j1 > kKronMaxTuple) return false;
   ks.no1 = kInst[best].no1;
   ks.no2 = kInst[best].no2;
   ks.j1cap = j1;
-  ks.j2cap = j2;
-  ks.stage_bytes = align256(sizeof(KronTab) + sizeof(float) * (2 * (size_t)j1 * ks.no1 + 2 * (size_t)j2 * ks.no2));
+  ks.j2cap = j1;
+  ks.stage_bytes = align256(sizeof(KronTab) + sizeof(float) * 2 * (size_t)dphi *
+                                                  ((size_t)j1 * ks.no1 + (size_t)j1 * ks.no2));
   return true;
 }
 
@@ -135,10 +136,13 @@ double kron_row_macs(const KronSite& ks, int stage) {
   double macs = 0;
   for (int rho = 0; rho < 2; ++rho)
     for (int c = 0; c < 2; ++c) {
+      // per pass phi: stage 1 over the ket tuples of parity c, stage 2 over
+      // the bra tuples of parity c ^ par(phi) (layer parity 0)
       macs += (double)tuple_count(r.d[0], r.e[0], r.d[1], r.e[1], c ^ rho) *
-              tuple_count(r.d[0], r.e[0], r.d[1], r.e[1], c) * ks.no1;
-      macs += (double)tuple_count(r.d[2], r.e[2], r.d[3], r.e[3], c ^ rho) * inner2_count(r, ks.dphi, ks.ephi, c) *
-              ks.no2;
+              tuple_count(r.d[0], r.e[0], r.d[1], r.e[1], c) * ks.no1 * ks.dphi;
+      for (int f = 0; f < ks.dphi; ++f)
+        macs += (double)tuple_count(r.d[2], r.e[2], r.d[3], r.e[3], c ^ rho) *
+                tuple_count(r.d[0], r.e[0], r.d[1], r.e[1], c ^ parl(f, ks.ephi)) * ks.no2;
     }
   return 0.5 * macs;
 }
@@ -165,7 +169,7 @@ void kron_build_tables(const KronSite& ks, const double* S, int q, std::vector<c
     KronTab* tab = reinterpret_cast<KronTab*>(base);
     tab->q = q;
     float* K1 = reinterpret_cast<float*>(base + sizeof(KronTab));
-    float* K2 = K1 + 2 * (size_t)ks.j1cap * ks.no1;
+    float* K2 = K1 + 2 * (size_t)ks.dphi * ks.j1cap * ks.no1;
     for (int pi = 0; pi < 2; ++pi) {
       int n = 0;
       for (int a = 0; a < r.d[0]; ++a)
@@ -177,42 +181,36 @@ void kron_build_tables(const KronSite& ks, const double* S, int q, std::vector<c
         for (int b = 0; b < r.d[3]; ++b)
           if (((parl(a, r.e[2]) + parl(b, r.e[3])) & 1) == pi) tab->po[pi][n++] = a | b << 8;
       tab->no[pi] = n;
-      // rank offsets of the stage-1 outputs of parity pi, lexicographic in (phi, o1k, o2k)
-      int cnt = 0;
-      for (int f = 0; f < ks.dphi; ++f)
-        for (int a = 0; a < r.d[2]; ++a) {
-          tab->pre[pi][f * r.d[2] + a] = cnt;
-          for (int b = 0; b < r.d[3]; ++b)
-            if (((parl(f, ks.ephi) + parl(a, r.e[2]) + parl(b, r.e[3])) & 1) == pi) ++cnt;
-        }
     }
-    for (int c = 0; c < 2; ++c) {
-      // stage 1, class c: ket tuples of parity c, outputs of parity q ^ c
-      const int po = q ^ c;
-      for (int j = 0; j < tab->n12[c]; ++j) {
-        const int c1 = tab->p12[c][j] & 255, c2 = tab->p12[c][j] >> 8;
-        int o = 0;
-        for (int f = 0; f < ks.dphi; ++f)
-          for (int a = 0; a < r.d[2]; ++a)
-            for (int b = 0; b < r.d[3]; ++b)
-              if (((parl(f, ks.ephi) + parl(a, r.e[2]) + parl(b, r.e[3])) & 1) == po)
-                K1[((size_t)c * ks.j1cap + j) * ks.no1 + o++] = (float)role_at(ks, S, stage, c1, c2, f, a, b);
-      }
-      // stage 2, class c: inner tuples of parity q ^ c, bra outputs of parity c
-      int n = 0;
-      for (int a = 0; a < r.d[0]; ++a)
-        for (int b = 0; b < r.d[1]; ++b)
-          for (int f = 0; f < ks.dphi; ++f) {
-            if (((parl(a, r.e[0]) + parl(b, r.e[1]) + parl(f, ks.ephi)) & 1) != (q ^ c)) continue;
-            tab->j2[c][n] = (a * r.d[1] + b) | f << 16;
-            for (int o = 0; o < tab->no[c]; ++o) {
-              const int o1 = tab->po[c][o] & 255, o2 = tab->po[c][o] >> 8;
-              K2[((size_t)c * ks.j2cap + n) * ks.no2 + o] = (float)role_at(ks, S, stage, a, b, f, o1, o2);
-            }
-            ++n;
+    for (int c = 0; c < 2; ++c)
+      for (int f = 0; f < ks.dphi; ++f) {
+        const int pf = parl(f, ks.ephi);
+        // stage 1, class c, pass f: ket tuples of parity c into the outputs
+        // (o1k, o2k) of parity q ^ c ^ par(f), lexicographic
+        const int po = q ^ c ^ pf;
+        for (int j = 0; j < tab->n12[c]; ++j) {
+          const int c1 = tab->p12[c][j] & 255, c2 = tab->p12[c][j] >> 8;
+          for (int o = 0; o < tab->no[po]; ++o) {
+            const int o1 = tab->po[po][o] & 255, o2 = tab->po[po][o] >> 8;
+            K1[(((size_t)c * ks.dphi + f) * ks.j1cap + j) * ks.no1 + o] =
+                (float)role_at(ks, S, stage, c1, c2, f, o1, o2);
           }
-      tab->nj2[c] = n;
-    }
+        }
+        // stage 2, class c, pass f: bra tuples (c1b, c2b) of parity q ^ c ^ par(f)
+        // into the bra outputs (o1b, o2b) of parity c
+        int n = 0;
+        for (int j = 0; j < tab->n12[po]; ++j) {
+          const int a = tab->p12[po][j] & 255, b = tab->p12[po][j] >> 8;
+          tab->j2[c][f][n] = a * r.d[1] + b;
+          for (int o = 0; o < tab->no[c]; ++o) {
+            const int o1 = tab->po[c][o] & 255, o2 = tab->po[c][o] >> 8;
+            K2[(((size_t)c * ks.dphi + f) * ks.j2cap + n) * ks.no2 + o] =
+                (float)role_at(ks, S, stage, a, b, f, o1, o2);
+          }
+          ++n;
+        }
+        tab->nj2[c][f] = n;
+      }
   }
 }
 
@@ -260,7 +258,7 @@ __global__ void __launch_bounds__(1024) kron_rows_kernel(KronArgs a) {
   const char* tb = reinterpret_cast<const char*>(a.tabs[s]) + a.stage_off;
   const KronTab* tab = reinterpret_cast<const KronTab*>(tb);
   const float* K1 = reinterpret_cast<const float*>(tb + sizeof(KronTab));
-  const float* K2 = K1 + 2 * a.j1cap * NO1;
+  const float* K2 = K1 + 2 * a.dphi * a.j1cap * NO1;
   const long long row0 = (long long)blockIdx.x * a.rows;
   const int nr = (int)min((long long)a.rows, a.nrows - row0);
   float* Xl = kron_lds;
@@ -327,136 +325,128 @@ __global__ void __launch_bounds__(1024) kron_rows_kernel(KronArgs a) {
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
-  // ---------------- stage 1 ----------------
-  if (wave < 2 * a.w1) {
-    const int c = wave >= a.w1 ? 1 : 0;
-    int t = (wave - c * a.w1) * 64 + lane;
-    int r = 0, tup = 0;
-    bool active = false;
-    const int n12c0 = tab->n12[0], n12c1 = tab->n12[1];
+  // stage-1 lane: a bra tuple (c1b, c2b) of a row, class c1 = par(c1b, c2b) ^ rho
+  const bool s1 = wave < 2 * a.w1;
+  const int c1 = wave >= a.w1 ? 1 : 0;
+  int r1 = 0, tup1 = 0;
+  bool act1 = false;
+  if (s1) {
+    int t = (wave - c1 * a.w1) * 64 + lane;
+    const int n0 = tab->n12[0], n1 = tab->n12[1];
     for (int rr = 0; rr < nr; ++rr) {
-      const int pi = c ^ rho[rr];
-      const int n = pi ? n12c1 : n12c0;
+      const int pi = c1 ^ rho[rr];
+      const int n = pi ? n1 : n0;
       if (t < n) {
-        r = rr;
-        tup = tab->p12[pi][t];
-        active = true;
+        r1 = rr;
+        tup1 = tab->p12[pi][t];
+        act1 = true;
         break;
       }
       t -= n;
     }
-    const int e1 = a.e[0], o1 = a.d[0] - a.e[0], e2 = a.e[1], o2 = a.d[1] - a.e[1];
-    const int c1b = tup & 255, c2b = tup >> 8;
-    const int p1b = c1b >= e1 ? 1 : 0, p2b = c2b >= e2 ? 1 : 0;
-    const int r1b = c1b - (p1b ? e1 : 0), r2b = c2b - (p2b ? e2 : 0);
-    const int mul1 = p1b ? o1 : e1, mul2 = p2b ? o2 : e2;
-    const int b10 = pair_base(0, p1b, e1, o1) + r1b, b11 = pair_base(1, p1b, e1, o1) + r1b;
-    const int b20 = pair_base(0, p2b, e2, o2) + r2b - (p2b ? a.pe[1] : 0);
-    const int b21 = pair_base(1, p2b, e2, o2) + r2b - (p2b ? 0 : a.pe[1]);
-    const float* xrow = Xl + r * rowX;
-    float acc[NO1];
+  }
+  // stage-2 lane: a ket tuple (o1k, o2k) of a row, class c2 = par(o1k, o2k) ^ rho;
+  // its index t2 among its row's tuples of that parity is its rank in the stored outputs
+  const bool s2 = wave < 2 * a.w2;
+  const int c2 = wave >= a.w2 ? 1 : 0;
+  int r2 = 0, tup2 = 0, t2 = 0;
+  bool act2 = false;
+  if (s2) {
+    int t = (wave - c2 * a.w2) * 64 + lane;
+    const int n0 = tab->no[0], n1 = tab->no[1];
+    for (int rr = 0; rr < nr; ++rr) {
+      const int pi = c2 ^ rho[rr];
+      const int n = pi ? n1 : n0;
+      if (t < n) {
+        r2 = rr;
+        tup2 = tab->po[pi][t];
+        t2 = t;
+        act2 = true;
+        break;
+      }
+      t -= n;
+    }
+  }
+  float acc2[NO2];
 #pragma unroll
-    for (int o = 0; o < NO1; ++o) acc[o] = 0.f;
-    // ket tuples (c1k, c2k) of parity c in the tables' lexicographic order
-    const float* kj = K1 + c * a.j1cap * NO1;
-    for (int c1k = 0; c1k < a.d[0]; ++c1k) {
-      const int p1k = c1k >= e1 ? 1 : 0;
-      const int p2k = c ^ p1k;
-      const int n2 = p2k ? o2 : e2;
-      const float* xp = xrow + ((p1k ? b11 : b10) + (c1k - (p1k ? e1 : 0)) * mul1) * a.pf[1] + (p2k ? b21 : b20);
-#pragma unroll 2
-      for (int r2k = 0; r2k < n2; ++r2k) {
-        const float xv = active ? xp[r2k * mul2] : 0.f;
+  for (int o = 0; o < NO2; ++o) acc2[o] = 0.f;
+  for (int f = 0; f < a.dphi; ++f) {
+    // ---------------- stage 1, pass f ----------------
+    if (s1) {
+      const int e1 = a.e[0], o1 = a.d[0] - a.e[0], e2 = a.e[1], o2 = a.d[1] - a.e[1];
+      const int c1b = tup1 & 255, c2b = tup1 >> 8;
+      const int p1b = c1b >= e1 ? 1 : 0, p2b = c2b >= e2 ? 1 : 0;
+      const int r1b = c1b - (p1b ? e1 : 0), r2b = c2b - (p2b ? e2 : 0);
+      const int mul1 = p1b ? o1 : e1, mul2 = p2b ? o2 : e2;
+      const int b10 = pair_base(0, p1b, e1, o1) + r1b, b11 = pair_base(1, p1b, e1, o1) + r1b;
+      const int b20 = pair_base(0, p2b, e2, o2) + r2b - (p2b ? a.pe[1] : 0);
+      const int b21 = pair_base(1, p2b, e2, o2) + r2b - (p2b ? 0 : a.pe[1]);
+      const float* xrow = Xl + r1 * rowX;
+      float acc[NO1];
+#pragma unroll
+      for (int o = 0; o < NO1; ++o) acc[o] = 0.f;
+      // ket tuples (c1k, c2k) of parity c1, one flat pass over the table's list
+      const float* kj = K1 + (c1 * a.dphi + f) * a.j1cap * NO1;
+      const int M1 = mul1 * a.pf[1];
+      const int B00 = b10 * a.pf[1] + b20, B01 = b10 * a.pf[1] + b21, B10 = b11 * a.pf[1] + b20,
+                B11 = b11 * a.pf[1] + b21;
+      const int nj = tab->n12[c1];
+      const int* jl = tab->p12[c1];
+      for (int j = 0; j < nj; ++j) {
+        const int jt = jl[j];
+        const int c1k = jt & 255, c2k = jt >> 8;
+        const int p1k = c1k >= e1 ? 1 : 0, p2k = c2k >= e2 ? 1 : 0;
+        const int base = p1k ? (p2k ? B11 : B10) : (p2k ? B01 : B00);
+        const float xv = act1 ? xrow[base + (c1k - (p1k ? e1 : 0)) * M1 + (c2k - (p2k ? e2 : 0)) * mul2] : 0.f;
 #pragma unroll
         for (int o = 0; o < NO1; ++o) acc[o] = fmaf(xv, kj[o], acc[o]);
         kj += NO1;
       }
-    }
-    if (active) {
-      float* tp = Tl + (r * a.ntup1 + c1b * a.d[1] + c2b) * kNO1P;
+      if (act1) {
+        float* tp = Tl + (r1 * a.ntup1 + c1b * a.d[1] + c2b) * kNO1P;
 #pragma unroll
-      for (int o = 0; o < NO1; ++o) tp[o] = acc[o];
-    }
-  }
-  __syncthreads();
-  // ---------------- stage 2 ----------------
-  if (wave < 2 * a.w2) {
-    const int c = wave >= a.w2 ? 1 : 0;
-    int t = (wave - c * a.w2) * 64 + lane;
-    int r = 0, tup = 0;
-    bool active = false;
-    const int noc0 = tab->no[0], noc1 = tab->no[1];
-    for (int rr = 0; rr < nr; ++rr) {
-      const int pi = c ^ rho[rr];
-      const int n = pi ? noc1 : noc0;
-      if (t < n) {
-        r = rr;
-        tup = tab->po[pi][t];
-        active = true;
-        break;
+        for (int o = 0; o < NO1; ++o) tp[o] = acc[o];
       }
-      t -= n;
     }
+    __syncthreads();
+    // ---------------- stage 2, pass f ----------------
+    if (s2) {
+      const float* trow = Tl + r2 * a.ntup1 * kNO1P + t2;
+      // bra tuples (c1b, c2b) of parity q ^ c2 ^ par(f), one flat pass over the table's list
+      const float* kj = K2 + (c2 * a.dphi + f) * a.j2cap * NO2;
+      const int nj = tab->nj2[c2][f];
+      const int* jl = tab->j2[c2][f];
+      for (int j = 0; j < nj; ++j) {
+        const float tv = act2 ? trow[jl[j] * kNO1P] : 0.f;
+#pragma unroll
+        for (int o = 0; o < NO2; ++o) acc2[o] = fmaf(tv, kj[o], acc2[o]);
+        kj += NO2;
+      }
+    }
+    if (f + 1 < a.dphi) __syncthreads();  // T of the next pass overwrites this one
+  }
+  if (act2) {
+    // Y rows (packed, row stride rowY) overwrite the X rows (X was last read
+    // before the barrier above); the pair index of (o1k, o1b) is
+    // A1[par o1b] + rank of o1b, that of (o2k, o2b) folded A2[par o2b] + rank of o2b
     const int eo1 = a.e[2], oo1 = a.d[2] - a.e[2], eo2 = a.e[3], oo2 = a.d[3] - a.e[3];
-    const int o1k = tup & 255, o2k = tup >> 8;
+    const int o1k = tup2 & 255, o2k = tup2 >> 8;
     const int p1k = o1k >= eo1 ? 1 : 0, p2k = o2k >= eo2 ? 1 : 0;
     const int r1k = o1k - (p1k ? eo1 : 0), r2k = o2k - (p2k ? eo2 : 0);
-    const int pf0 = 0 >= a.ephi ? 1 : 0;
-    const int rk0 = tab->pre[pf0 ^ p1k ^ p2k][o1k] + r2k;
-    const int rk1 = a.dphi > 1 ? tab->pre[(1 >= a.ephi ? 1 : 0) ^ p1k ^ p2k][a.d[2] + o1k] + r2k : 0;
-    const float* trow = Tl + r * a.ntup1 * kNO1P;
-    float acc[NO2];
+    float* yrow = Xl + r2 * (int)a.rowY;
+    const int A10 = pair_base(p1k, 0, eo1, oo1) + r1k * eo1, A11 = pair_base(p1k, 1, eo1, oo1) + r1k * oo1;
+    const int A20 = pair_base(p2k, 0, eo2, oo2) + r2k * eo2 - (p2k ? a.pe[3] : 0);
+    const int A21 = pair_base(p2k, 1, eo2, oo2) + r2k * oo2 - (p2k ? 0 : a.pe[3]);
+    const int no = tab->no[c2];  // bra tuples (o1b, o2b) of parity c2, wave-uniform
 #pragma unroll
-    for (int o = 0; o < NO2; ++o) acc[o] = 0.f;
-    // inner tuples (c1b, c2b, phi) of parity q ^ c in the tables' lexicographic order
-    const int want = tab->q ^ c;
-    const float* kj = K2 + c * a.j2cap * NO2;
-    const int e1 = a.e[0], e2 = a.e[1];
-    for (int c1b = 0; c1b < a.d[0]; ++c1b) {
-      const int p1b = c1b >= e1 ? 1 : 0;
-      const float* tp = trow + c1b * a.d[1] * kNO1P;
-      if (a.dphi == 1) {
-        const int p2b = want ^ p1b ^ pf0;
-        const int lo = p2b ? e2 : 0, hi = p2b ? a.d[1] : e2;
-#pragma unroll 2
-        for (int c2b = lo; c2b < hi; ++c2b) {
-          const float tv = active ? tp[c2b * kNO1P + rk0] : 0.f;
-#pragma unroll
-          for (int o = 0; o < NO2; ++o) acc[o] = fmaf(tv, kj[o], acc[o]);
-          kj += NO2;
-        }
-      } else {
-        for (int c2b = 0; c2b < a.d[1]; ++c2b) {
-          const int p2b = c2b >= e2 ? 1 : 0;
-          for (int f = 0; f < 2; ++f) {
-            if ((p1b ^ p2b ^ (f >= a.ephi ? 1 : 0)) != want) continue;
-            const float tv = active ? tp[c2b * kNO1P + (f ? rk1 : rk0)] : 0.f;
-#pragma unroll
-            for (int o = 0; o < NO2; ++o) acc[o] = fmaf(tv, kj[o], acc[o]);
-            kj += NO2;
-          }
-        }
-      }
-    }
-    if (active) {
-      // Y rows (packed, row stride rowY) overwrite the X rows (X was last read
-      // before the barrier above); the pair index of (o1k, o1b) is
-      // A1[par o1b] + rank of o1b, that of (o2k, o2b) folded A2[par o2b] + rank of o2b
-      float* yrow = Xl + r * (int)a.rowY;
-      const int A10 = pair_base(p1k, 0, eo1, oo1) + r1k * eo1, A11 = pair_base(p1k, 1, eo1, oo1) + r1k * oo1;
-      const int A20 = pair_base(p2k, 0, eo2, oo2) + r2k * eo2 - (p2k ? a.pe[3] : 0);
-      const int A21 = pair_base(p2k, 1, eo2, oo2) + r2k * oo2 - (p2k ? 0 : a.pe[3]);
-      const int no = tab->no[c];  // bra tuples (o1b, o2b) of parity c, wave-uniform
-#pragma unroll
-      for (int o = 0; o < NO2; ++o) {
-        if (o < no) {
-          const int ot = tab->po[c][o];
-          const int o1b = ot & 255, o2b = ot >> 8;
-          const int q1b = o1b >= eo1 ? 1 : 0, q2b = o2b >= eo2 ? 1 : 0;
-          const int po1 = (q1b ? A11 : A10) + (o1b - (q1b ? eo1 : 0));
-          const int po2 = (q2b ? A21 : A20) + (o2b - (q2b ? eo2 : 0));
-          yrow[po1 * a.pf[3] + po2] = acc[o];
-        }
+    for (int o = 0; o < NO2; ++o) {
+      if (o < no) {
+        const int ot = tab->po[c2][o];
+        const int o1b = ot & 255, o2b = ot >> 8;
+        const int q1b = o1b >= eo1 ? 1 : 0, q2b = o2b >= eo2 ? 1 : 0;
+        const int po1 = (q1b ? A11 : A10) + (o1b - (q1b ? eo1 : 0));
+        const int po2 = (q2b ? A21 : A20) + (o2b - (q2b ? eo2 : 0));
+        yrow[po1 * a.pf[3] + po2] = acc2[o];
       }
     }
   }
@@ -485,8 +475,6 @@ void kron_prepare() {
   raise_lds<16, 16>();
   raise_lds<18, 18>();
   raise_lds<32, 32>();
-  raise_lds<36, 18>();
-  raise_lds<64, 32>();
   raise_lds<64, 64>();
   done = true;
 }
@@ -534,7 +522,7 @@ hipError_t launch_kron_rows(const KronSite& ks, int stage, const float* x, float
     n1 = std::max(n1, tuple_count(r.d[0], r.e[0], r.d[1], r.e[1], pi));
     n2 = std::max(n2, tuple_count(r.d[2], r.e[2], r.d[3], r.e[3], pi));
   }
-  int rows = rows_per_block > 0 ? std::min(rows_per_block, 32) : 5;
+  int rows = rows_per_block > 0 ? std::min(rows_per_block, 32) : 7;
   auto fits = [&](int R) {
     const int w1 = (R * n1 + 63) / 64, w2 = (R * n2 + 63) / 64;
     const size_t lds = sizeof(float) * ((size_t)R * a.rowmax + (size_t)R * a.ntup1 * (ks.no1 | 1));
@@ -567,8 +555,6 @@ hipError_t launch_kron_rows(const KronSite& ks, int stage, const float* x, float
     case 1616: return launch_inst<16, 16>(a, grid, block, lds, st);
     case 1818: return launch_inst<18, 18>(a, grid, block, lds, st);
     case 3232: return launch_inst<32, 32>(a, grid, block, lds, st);
-    case 3618: return launch_inst<36, 18>(a, grid, block, lds, st);
-    case 6432: return launch_inst<64, 32>(a, grid, block, lds, st);
     case 6464: return launch_inst<64, 64>(a, grid, block, lds, st);
     default: return hipErrorInvalidValue;
   }
