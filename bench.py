@@ -161,17 +161,18 @@ def cpu_baseline(chi=32, seed=5, procs=None):
             "row_seconds": per_row, "host": host_info()}
 
 
-def measured_traffic(precision, batch, L, chi):
-    """HBM bytes per launch of the dominant kernel for this workload, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent or for
-    another workload)."""
-    path = os.path.join(ROOT, "profiles", f"z2gemm_{precision}_traffic.json")
+def measured_traffic(precision, batch, L, chi, kron):
+    """HBM bytes per launch of the contraction kernels for this workload, from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent or
+    for another workload)."""
+    path = os.path.join(ROOT, "profiles", f"contraction_{precision}_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None, None
-    if (t.get("precision"), t.get("batch"), t.get("L"), t.get("chi")) != (precision, batch, L, chi):
+    if (t.get("precision"), t.get("batch"), t.get("L"), t.get("chi"), t.get("kron")) != (precision, batch, L, chi,
+                                                                                       kron):
         return None, None
     return t["traffic_bytes_per_launch"], t.get("source")
 
@@ -361,6 +362,8 @@ def main():
     ap.add_argument("--batch", type=int, default=128, help="Born samples per step")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "bf16x6", "bf16x3"])
     ap.add_argument("--chi", type=int, default=32)
+    ap.add_argument("--kron", type=int, default=1, choices=[0, 1],
+                    help="1: factorised Gram transfers in the FP32 draws (default); 0: dense double layers")
     ap.add_argument("--plain-draws", type=int, default=3, help="batch-1 FP64 draws for the plain path")
     ap.add_argument("--check-samples", type=int, default=8,
                     help="drawn samples measured in the energy-throughput figure")
@@ -409,7 +412,7 @@ def main():
     B = args.batch
     w_draw = flops.draw_flops(pepo, args.chi)  # algorithmic flops of one draw
 
-    ctx = spp.Context(0, args.precision)
+    ctx = spp.Context(0, args.precision, kron=args.kron)
     t0 = time.perf_counter()
     cache = spp.SamplingCache(ctx, pepo, args.chi)
     ctx.synchronize()
@@ -439,7 +442,7 @@ def main():
     ndraws = B * args.steps
     value = ndraws / elapsed
     log(f"timed {args.steps} x {B} draws in {elapsed:.2f} s -> {value:.3f} samples/s")
-    traffic, traffic_src = measured_traffic(args.precision, B, L, args.chi)
+    traffic, traffic_src = measured_traffic(args.precision, B, L, args.chi, args.kron)
     kc = cnt["fp64" if args.precision == "fp64" else "fp32"]  # launches of this precision's contraction kernel
     avg_ms = kc["ms"] / max(kc["launches"], 1)
     draws_per_launch = ndraws / max(kc["launches"], 1)
@@ -453,6 +456,7 @@ def main():
         "config": {"workload": f"batched Born draws, {L}x{L} TFIM J=1 h=2.0 Z2 PEPO at beta0=0.5, D=6 (3+3), d=2, "
                                f"chi={args.chi}, {B} samples per step", "lattice": [L, L], "D": 6, "chi": args.chi,
                    "samples_per_step": B, "cache_build_s": t_cache,
+                   "gram_transfers": "factorised (ket and bra single layers)" if args.kron else "dense double layers",
                    "input": "tests/golden/cfg2_pepo_beta05.npz: the beta0=0.5 PEPO of full-update purification "
                             "(5 steps of dtau=0.05, FP64), frozen; uniforms of sample indices 1.. of seed 1"},
         "roofline": {"bound": "mfma", "kernel": KERNEL[args.precision],
