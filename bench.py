@@ -317,8 +317,9 @@ def precision_sweep(spp, flops, L=6, chi=64, batch=8, D=(4, 4)):
     unit-norm sites; no purified beta0=0.4 PEPO at these sizes is committed)."""
     pepo = spp.synthetic_thermal_pepo(spp.Lattice(L, L), D=D, eps=0.3, seed=4)
     uni = np.stack([spp.sample_uniforms(1, i, L * L) for i in range(1, batch + 1)])
+    scope = "the 12x12 D=8 chi=64 workload" if L == 12 else "reduced lattice of the 12x12 D=8 chi=64 workload"
     out = {"workload": f"batched Born draws, {L}x{L} synthetic Z2 PEPO, D={sum(D)} ({D[0]}+{D[1]}), d=2, chi={chi}, "
-                       f"{batch} samples per step (reduced lattice of the 12x12 D=8 chi=64 workload)",
+                       f"{batch} samples per step ({scope})",
            "algorithmic_flops_per_draw": flops.draw_flops(pepo, chi), "samples": batch}
     ctx = spp.Context(0, "fp64", autotune=0)  # timed on first calls: heuristic tile shapes
     ref = None
@@ -382,6 +383,7 @@ def main():
     ap.add_argument("--sweep-lattice", type=int, default=6,
                     help="lattice edge of the D=8 chi=64 precision sweep (12: the full lattice; 0: skip)")
     ap.add_argument("--sweep-batch", type=int, default=8, help="samples per precision in the sweep")
+    ap.add_argument("--sweep-only", action="store_true", help="run only the precision sweep")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed loop, copy 256 MiB device to device (HBM counter calibration)")
     args = ap.parse_args()
@@ -402,6 +404,10 @@ def main():
     if args.fu_only:
         print(json.dumps({"full_update": fu_timing(spp, batch=max(args.fu_batch, 1), warmup=args.fu_warmup)}),
               flush=True)
+        return
+    if args.sweep_only:
+        print(json.dumps({"precision_sweep": precision_sweep(spp, flops, L=args.sweep_lattice,
+                                                             batch=args.sweep_batch)}), flush=True)
         return
     if args.physical_only:
         print(json.dumps({"physical_draws": physical_timing(spp)}), flush=True)
