@@ -235,6 +235,7 @@ struct KronArgs {
   float inv_rowx, inv_pf1, inv_rowy;  // 1 / (pd[0] * pf[1]), 1 / pf[1], 1 / rowY
   int sx1i, sx2i;                     // sx1, sx2 (a sample's X has fewer than 2^31 elements)
   int xflat;                          // X rows are packed and consecutive in global memory
+  float inv_pd0;                      // 1 / pd[0]
 };
 
 // n / m for 0 <= n < 2^22 and 1 <= m <= 4096 from inv = 1 / m: (n + 1/2) / m
@@ -248,8 +249,12 @@ __device__ __forceinline__ int pair_base(int pi, int pj, int e, int o) {
 
 constexpr int kKronPrefetch = 16;  // X loads in flight per thread
 
+// Threads per block: the wide register tiles keep 512 threads so that their
+// accumulators (both stages live across the passes) fit without spilling.
+constexpr int kron_max_threads(int no1) { return no1 > 24 ? 512 : 1024; }
+
 template <int NO1, int NO2>
-__global__ void __launch_bounds__(1024) kron_rows_kernel(KronArgs a) {
+__global__ void __launch_bounds__(kron_max_threads(NO1)) kron_rows_kernel(KronArgs a) {
   constexpr int kNO1P = NO1 | 1;  // odd stride of a tuple's stage-1 outputs in LDS (no bank conflicts)
   extern __shared__ float kron_lds[];
   __shared__ int rho[32];
@@ -294,6 +299,16 @@ __global__ void __launch_bounds__(1024) kron_rows_kernel(KronArgs a) {
         const int i = base + u * nthr + tid;
         if (i < total) Xl[i] = v[u];
       }
+    }
+  } else if (a.sx2i == 1 && (a.pf[1] & 1) == 0 && (a.sx1i & 1) == 0 && (a.sxf & 1) == 0 && (a.sx_slab & 1) == 0) {
+    // rows of pd[0] chunks of pf[1] contiguous values (even lengths and offsets): one
+    // chunk per thread, copied as 8-byte pairs
+    const int nch = nr * a.pd[0], half = a.pf[1] >> 1;
+    for (int ch = tid; ch < nch; ch += nthr) {
+      const int rr = small_div(ch, a.inv_pd0), c1 = ch - rr * a.pd[0];
+      const float2* src = reinterpret_cast<const float2*>(xs + xrow0[rr] + c1 * a.sx1i);
+      float2* dst = reinterpret_cast<float2*>(Xl + rr * rowX + c1 * a.pf[1]);
+      for (int u = 0; u < half; ++u) dst[u] = src[u];
     }
   } else {
     const bool contig = a.sx2i == 1 && a.sx1i == a.pf[1];
@@ -385,22 +400,24 @@ __global__ void __launch_bounds__(1024) kron_rows_kernel(KronArgs a) {
       float acc[NO1];
 #pragma unroll
       for (int o = 0; o < NO1; ++o) acc[o] = 0.f;
-      // ket tuples (c1k, c2k) of parity c1, one flat pass over the table's list
+      // ket tuples (c1k, c2k) of parity c1 in the tables' order: the even c1k with the c2k of
+      // parity c1, then the odd c1k with the c2k of parity c1 ^ 1; in each group the X offset
+      // advances by mul1 * pf per c1k and by mul2 per c2k
       const float* kj = K1 + (c1 * a.dphi + f) * a.j1cap * NO1;
       const int M1 = mul1 * a.pf[1];
-      const int B00 = b10 * a.pf[1] + b20, B01 = b10 * a.pf[1] + b21, B10 = b11 * a.pf[1] + b20,
-                B11 = b11 * a.pf[1] + b21;
-      const int nj = tab->n12[c1];
-      const int* jl = tab->p12[c1];
-      for (int j = 0; j < nj; ++j) {
-        const int jt = jl[j];
-        const int c1k = jt & 255, c2k = jt >> 8;
-        const int p1k = c1k >= e1 ? 1 : 0, p2k = c2k >= e2 ? 1 : 0;
-        const int base = p1k ? (p2k ? B11 : B10) : (p2k ? B01 : B00);
-        const float xv = act1 ? xrow[base + (c1k - (p1k ? e1 : 0)) * M1 + (c2k - (p2k ? e2 : 0)) * mul2] : 0.f;
+      for (int g = 0; g < 2; ++g) {
+        const int p2k = c1 ^ g;
+        const int n1 = g ? o1 : e1, n2 = p2k ? o2 : e2;
+        const float* xg = xrow + (g ? b11 : b10) * a.pf[1] + (p2k ? b21 : b20);
+        for (int i1 = 0; i1 < n1; ++i1) {
+          const float* xp = xg + i1 * M1;
+          for (int i2 = 0; i2 < n2; ++i2) {
+            const float xv = act1 ? xp[i2 * mul2] : 0.f;
 #pragma unroll
-        for (int o = 0; o < NO1; ++o) acc[o] = fmaf(xv, kj[o], acc[o]);
-        kj += NO1;
+            for (int o = 0; o < NO1; ++o) acc[o] = fmaf(xv, kj[o], acc[o]);
+            kj += NO1;
+          }
+        }
       }
       if (act1) {
         float* tp = Tl + (r1 * a.ntup1 + c1b * a.d[1] + c2b) * kNO1P;
@@ -526,7 +543,7 @@ hipError_t launch_kron_rows(const KronSite& ks, int stage, const float* x, float
   auto fits = [&](int R) {
     const int w1 = (R * n1 + 63) / 64, w2 = (R * n2 + 63) / 64;
     const size_t lds = sizeof(float) * ((size_t)R * a.rowmax + (size_t)R * a.ntup1 * (ks.no1 | 1));
-    return 64 * 2 * std::max(w1, w2) <= 1024 && lds <= (size_t)kKronLdsBytes - 1024;
+    return 64 * 2 * std::max(w1, w2) <= kron_max_threads(ks.no1) && lds <= (size_t)kKronLdsBytes - 1024;
   };
   while (rows > 1 && !fits(rows)) --rows;
   if (!fits(rows)) return hipErrorInvalidValue;
@@ -539,6 +556,7 @@ hipError_t launch_kron_rows(const KronSite& ks, int stage, const float* x, float
     return hipErrorInvalidValue;
   a.sx1i = (int)rw.sx1;
   a.sx2i = (int)rw.sx2;
+  a.inv_pd0 = 1.0f / (float)a.pd[0];
   a.xflat = rw.sx2 == 1 && rw.sx1 == a.pf[1] && rw.sxf == (long long)a.pd[0] * a.pf[1] &&
             rw.sx_slab == (long long)rw.nf * rw.sxf;
   if ((long long)rows * a.rowmax >= (1 << 22) || a.rowmax > 4096) return hipErrorInvalidValue;
