@@ -1054,20 +1054,39 @@ __global__ void degree_mask_kernel(const int* active, const int* deg, int j, int
   if (p < n) out[p] = (active[p] && deg[p] > j) ? 1 : 0;
 }
 
-// rho <- (rho + rho^T) / 2 on each compact n x n block (rho built in FP32 is
-// symmetric only to rounding; the eigensolver needs an exactly symmetric matrix).
-__global__ void symmetrize_kernel(double* rho, long long pstride, int ld, const int* n_of) {
+
+// rho = (A + A^T) / 2 in FP64 from the FP32 compact blocks A (leading n x n
+// of each problem, leading dimension ld), through 32 x 32 LDS tiles: block x
+// of problem y takes the tile pair (ti, tj), ti <= tj, of the upper triangle,
+// reads both tiles and writes both. The same arithmetic as converting to FP64
+// first and averaging there.
+__global__ void symmetrize_convert_kernel(const float* a, double* rho, long long pstride, int ld, const int* n_of) {
+  __shared__ float sa[32][33], sb[32][33];
   const int prob = blockIdx.y;
   const int n = n_of[prob];
-  double* b = rho + (long long)prob * pstride;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n * n;
-       i += (long long)gridDim.x * blockDim.x) {
-    int r = (int)(i / n), c = (int)(i % n);
-    if (r < c) {
-      double v = 0.5 * (b[(long long)r * ld + c] + b[(long long)c * ld + r]);
-      b[(long long)r * ld + c] = v;
-      b[(long long)c * ld + r] = v;
-    }
+  const int T = (ld + 31) / 32;
+  int k = blockIdx.x, ti = 0;
+  while (ti < T && k >= T - ti) {
+    k -= T - ti;
+    ++ti;
+  }
+  const int tj = ti + k;
+  if (ti >= T || ti * 32 >= n || tj * 32 >= n) return;
+  const float* ab = a + (long long)prob * pstride;
+  double* rb = rho + (long long)prob * pstride;
+  const int c = threadIdx.x & 31, r0 = threadIdx.x >> 5;
+  for (int r = r0; r < 32; r += 8) {
+    const int i = ti * 32 + r, j = tj * 32 + c;
+    sa[r][c] = (i < n && j < n) ? ab[(long long)i * ld + j] : 0.f;
+    const int i2 = tj * 32 + r, j2 = ti * 32 + c;
+    sb[r][c] = (i2 < n && j2 < n) ? ab[(long long)i2 * ld + j2] : 0.f;
+  }
+  __syncthreads();
+  for (int r = r0; r < 32; r += 8) {
+    const int i = ti * 32 + r, j = tj * 32 + c;
+    if (i < n && j < n) rb[(long long)i * ld + j] = 0.5 * ((double)sa[r][c] + (double)sb[c][r]);
+    const int i2 = tj * 32 + r, j2 = ti * 32 + c;
+    if (ti != tj && i2 < n && j2 < n) rb[(long long)i2 * ld + j2] = 0.5 * ((double)sb[r][c] + (double)sa[c][r]);
   }
 }
 
@@ -1572,8 +1591,9 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
       T* rho_t = (T*)ctx.arena.take(sizeof(T) * 2 * nb * rps);
       contract<T>(ctx, compact_ref(rho_t, 2 * rps, rps, nmax, "kpKP", 2, {kb, sp.s, kb, sp.s}), Wr,
                   with_letters(Yr, "KPRF"), nb);
-      SPP_CHECK((launch_convert<double, T>(rho, rho_t, 2 * nb * rps, st)));
-      hipLaunchKernelGGL(symmetrize_kernel, dim3(64, 2 * nb), dim3(256), 0, st, rho, rps, (int)nmax, ncount);
+      const int tiles = (int)((nmax + 31) / 32);
+      hipLaunchKernelGGL(symmetrize_convert_kernel, dim3(tiles * (tiles + 1) / 2, 2 * nb), dim3(256), 0, st,
+                         reinterpret_cast<const float*>(rho_t), rho, rps, (int)nmax, ncount);
       SPP_CHECK(hipGetLastError());
     }
     hipLaunchKernelGGL(trace_kernel, dim3(2 * nb), dim3(256), 0, st, rho, rps, (int)nmax, ncount, tr);
