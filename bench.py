@@ -177,6 +177,57 @@ def measured_traffic(precision, batch, L, chi, kron):
     return t["traffic_bytes_per_launch"], t.get("source")
 
 
+def uneven_timing(spp, L=16, chi=32, batch=64, spread=(2, 3, 4), seed=11):
+    """Uneven batch at the 16x16 lattice: batch random Z2 PEPS whose bond
+    dimensions spread over D in `spread` (seeded; D split as (ceil(D/2),
+    floor(D/2))), embedded in the capacity of the largest D and packed into one
+    batched log-norm call (the per-sample parity splits drive the ragged
+    contraction kernel, tile shapes tuned per launch class), FP32 contractions.
+    Against the same states in FP64 (accuracy) and against a batch whose states
+    all have the largest D (what padding to one capacity would cost)."""
+    rng = np.random.default_rng(seed)
+    lat = spp.Lattice(L, L)
+    dmax = max(spread)
+    cap = ((dmax + 1) // 2, dmax // 2)
+    Ds = rng.choice(spread, size=batch)
+    states, lives = [], []
+    for k, D in enumerate(Ds):
+        st = spp.random_peps(lat, ((D + 1) // 2, D // 2), seed=1000 + k)
+        padded, live = spp.pad_bonds(st, cap)
+        states.append(padded)
+        lives.append(live)
+    full = [spp.random_peps(lat, cap, seed=2000 + k) for k in range(batch)]
+    out = {"workload": f"log-norm of {batch} random {L}x{L} Z2 PEPS, bond D spread over {list(spread)} "
+                       f"(counts {[int(np.sum(Ds == d)) for d in spread]}), capacity D={dmax}, chi={chi}",
+           "batch": batch}
+    for prec in ("fp32", "fp64"):
+        ctx = spp.Context(0, prec)
+        try:
+            dev = [spp.DeviceState(ctx, s, live=lv) for s, lv in zip(states, lives)]
+            spp.lognorms(ctx, dev, chi)  # warm-up: tunes the launch classes
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            r = spp.lognorms(ctx, dev, chi)
+            ctx.synchronize()
+            dt = time.perf_counter() - t0
+            out[prec] = {"value": batch / dt, "unit": "states/s", "log_norm": r["log_norm"]}
+            if prec == "fp32":
+                devf = [spp.DeviceState(ctx, s) for s in full]
+                spp.lognorms(ctx, devf, chi)
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                spp.lognorms(ctx, devf, chi)
+                ctx.synchronize()
+                out["fp32_all_at_capacity"] = {"value": batch / (time.perf_counter() - t0), "unit": "states/s"}
+            log(f"uneven {prec}: {out[prec]['value']:.3f} states/s")
+        finally:
+            ctx.close()
+    a, b = out["fp32"].pop("log_norm"), out["fp64"].pop("log_norm")
+    out["fp32"]["max_rel_dlog_norm_vs_fp64"] = float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b))))
+    out["uneven_gain_over_capacity"] = out["fp32"]["value"] / out["fp32_all_at_capacity"]["value"]
+    return out
+
+
 def adjoint_timing(spp, reps=2, batch=8, chi=36):
     """The log-norm gradient workload: 8x8 Z2 PEPS, D=6 (3+3), d=2, i.i.d.
     N(0,1) blocks scaled to unit norm (seed 7), chi=36, FP64 contractions: all
@@ -384,6 +435,8 @@ def main():
                     help="lattice edge of the D=8 chi=64 precision sweep (12: the full lattice; 0: skip)")
     ap.add_argument("--sweep-batch", type=int, default=8, help="samples per precision in the sweep")
     ap.add_argument("--sweep-only", action="store_true", help="run only the precision sweep")
+    ap.add_argument("--uneven-only", action="store_true",
+                    help="time only the uneven-bond batch at the 16x16 lattice (log-norms)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed loop, copy 256 MiB device to device (HBM counter calibration)")
     args = ap.parse_args()
@@ -404,6 +457,9 @@ def main():
     if args.fu_only:
         print(json.dumps({"full_update": fu_timing(spp, batch=max(args.fu_batch, 1), warmup=args.fu_warmup)}),
               flush=True)
+        return
+    if args.uneven_only:
+        print(json.dumps({"uneven_batch": uneven_timing(spp)}), flush=True)
         return
     if args.sweep_only:
         print(json.dumps({"precision_sweep": precision_sweep(spp, flops, L=args.sweep_lattice,

@@ -57,7 +57,10 @@ int spp_synchronize(spp_ctx* ctx);
 int spp_set_precision(spp_ctx* ctx, int precision);
 
 /* Tuning: "ksub" (subspace size of the truncation eigensolver, <= 64),
- * "max_subspace_iter", "subspace_tol", "cheb_degree" (largest Chebyshev
+ * "max_subspace_iter", "subspace_tol" (relative Ritz residual at which a
+ * truncation eigensolve stops, default 1e-12; where rho is formed by FP32
+ * products the stop is at max(subspace_tol, 1e-6), the rounding level of that
+ * rho), "cheb_degree" (largest Chebyshev
  * filter degree of a subspace step, 1 = plain power steps; default 8),
  * "chunk" (samples per device batch,
  * 0 = sized from free memory), "profile" (1: time each contraction launch

@@ -1192,6 +1192,9 @@ __global__ void norm_finish_kernel(const double* nrm2, double* log_scale, double
 }
 
 // --------------------------- truncation driver ------------------------------
+// Relative Ritz residual at which a truncation eigensolve of an FP32-formed rho stops.
+constexpr double kFp32RhoTol = 1e-6;
+
 struct EigResult {
   double* w;        // [nprob][wld]
   long long wstride;
@@ -1202,7 +1205,7 @@ struct EigResult {
 };
 
 static void eig_topk(Context& ctx, const double* rho, long long rho_pstride, int rho_ld, const Leg& lk,
-                     const Leg& lp, const int* n_of, int nmax, int nb, int want, EigResult& res) {
+                     const Leg& lp, const int* n_of, int nmax, int nb, int want, EigResult& res, double tol) {
   const int nprob = 2 * nb;
   hipStream_t st = ctx.stream;
   if (nmax <= ctx.ksub) {
@@ -1323,7 +1326,7 @@ static void eig_topk(Context& ctx, const double* rho, long long rho_pstride, int
     rmul(Qb, Qa, Vm);
     rmul(Zb, Za, Vm);
     hipLaunchKernelGGL(residual_kernel, dim3(nprob), dim3(256), 0, st, Qb, Zb, bs, kk, sig, (long long)kk, n_of,
-                       t, ctx.subspace_tol, conv, worst, active, kk, cholqr ? ctx.cheb_degree : 1, degm);
+                       t, tol, conv, worst, active, kk, cholqr ? ctx.cheb_degree : 1, degm);
     SPP_CHECK(hipGetLastError());
     SPP_CHECK(hipMemcpyAsync(hconv.data(), conv, sizeof(int) * nprob, hipMemcpyDeviceToHost, st));
     SPP_CHECK(hipMemcpyAsync(hdeg.data(), degm, sizeof(int) * nprob, hipMemcpyDeviceToHost, st));
@@ -1617,7 +1620,11 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
     int bcap = (int)std::min<long long>(cap, capq[0] + capq[1]);
     if (bcap < 1) bcap = 1;
     EigResult er{};
-    eig_topk(ctx, rho, rps, (int)nmax, kb, sp.s, ncount, (int)nmax, nb, cap, er);
+    // rho formed by FP32 products carries rounding of about 1e-6 of its largest
+    // eigenvalue: Ritz residuals below that are not resolved, so the iteration
+    // stops there instead of running to its cap
+    const double eig_tol = sizeof(T) == sizeof(float) ? std::max(ctx.subspace_tol, kFp32RhoTol) : ctx.subspace_tol;
+    eig_topk(ctx, rho, rps, (int)nmax, kb, sp.s, ncount, (int)nmax, nb, cap, er, eig_tol);
     // U[k,p,c] into the output boundary storage
     long long us = (long long)kb.dim * sp.s.dim * bcap;
     T* U = out.t[x];
