@@ -1260,6 +1260,9 @@ static void eig_topk(Context& ctx, const double* rho, long long rho_pstride, int
     if (kk <= 64 && nmax <= 8 * 72) {
       hipLaunchKernelGGL((householder_q_reg_kernel<8, 72>), dim3(nprob), dim3(512), 0, st, Z, Q, bs, kk, kk, n_of,
                          active);
+    } else if (kk <= 64 && nmax <= 16 * 41) {
+      hipLaunchKernelGGL((householder_q_reg_kernel<16, 41>), dim3(nprob), dim3(1024), 0, st, Z, Q, bs, kk, kk, n_of,
+                         active);
     } else {
       hipLaunchKernelGGL(householder_q_kernel, dim3(nprob), dim3(256), 0, st, Z, Q, bs, kk, kk, n_of, tau, active);
     }
@@ -1301,6 +1304,9 @@ static void eig_topk(Context& ctx, const double* rho, long long rho_pstride, int
     SPP_CHECK(hipGetLastError());
     if (kk <= 64 && nmax <= 8 * 72) {
       hipLaunchKernelGGL((householder_q_reg_kernel<8, 72>), dim3(nprob), dim3(512), 0, st, Z, Q, bs, kk, kk, n_of,
+                         okm);
+    } else if (kk <= 64 && nmax <= 16 * 41) {
+      hipLaunchKernelGGL((householder_q_reg_kernel<16, 41>), dim3(nprob), dim3(1024), 0, st, Z, Q, bs, kk, kk, n_of,
                          okm);
     } else {
       hipLaunchKernelGGL(householder_q_kernel, dim3(nprob), dim3(256), 0, st, Z, Q, bs, kk, kk, n_of, tau, okm);
