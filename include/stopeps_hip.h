@@ -474,7 +474,9 @@ int spp_selftest_ql(int n, double* d, double* e, double* vectors, int nvec, int 
  * E, S; dphi / ephi of PHI; FP64, host, one parity), one sample, bonds l and R
  * of dims nl / nr with even counts el / er. stage 0: x = z1[nl][nr][E][N][fold E]
  * -> y = z2[nl][nr][E][W][fold S] (contracting N and E); stage 1: x = z2 ->
- * y = z3[nl][nr][W][W][fold N] (contracting E and S); pair legs (ket, bra) in
+ * y = z3[nl][nr][W][W][fold N] (contracting E and S); stage 2: x =
+ * X[W][nl][nr][fold N] -> y = Y[nl][nr][E][fold S] (contracting W and N, the
+ * draw's candidate left steps); pair legs (ket, bra) in
  * the parity-sorted order of the double layers, the last leg folded to the
  * parity its neighbours fix. rows_per_block 0: the default. Returns 0, -1 with
  * the error in spp_last_error, -2 when the layer does not fit the kernel. */

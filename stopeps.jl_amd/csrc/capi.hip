@@ -751,7 +751,7 @@ int spp_selftest_ql(int n, double* d, double* e, double* vectors, int nvec, int 
 
 int spp_selftest_kron(spp_ctx* ctx, const int* d, const int* e, int dphi, int ephi, const double* S, int stage,
                       int nl, int el, int nr, int er, const float* x, float* y, int rows_per_block) {
-  if (!ctx || !d || !e || !S || !x || !y || stage < 0 || stage > 1 || nl < 1 || nr < 1) return -3;
+  if (!ctx || !d || !e || !S || !x || !y || stage < 0 || stage >= kKronStages || nl < 1 || nr < 1) return -3;
   KronSite ks;
   if (!kron_site_layout(d, e, dphi, ephi, ks)) return -2;
   const int q = kron_layer_parity(d, e, dphi, ephi, S);
@@ -783,7 +783,7 @@ int spp_selftest_kron(spp_ctx* ctx, const int* d, const int* e, int dphi, int ep
       kr.sx1 = pf[2];
       nx = (long long)nl * nr * pd[2] * pd[1] * pf[2];
       ny = (long long)nl * nr * pd[2] * pd[0] * pf[3];
-    } else {
+    } else if (stage == 1) {
       kr.nrows = (long long)nl * nr * pd[0];
       kr.nf = (int)pd[0];
       kr.ef = pe[0];
@@ -792,6 +792,16 @@ int spp_selftest_kron(spp_ctx* ctx, const int* d, const int* e, int dphi, int ep
       kr.sx1 = pd[0] * pf[3];
       nx = (long long)nl * nr * pd[2] * pd[0] * pf[3];
       ny = (long long)nl * nr * pd[0] * pd[0] * pf[1];
+    } else {
+      // x = X[W][nl][nr][fold N], rows (l, R); y = Y[nl][nr][E][fold S]
+      kr.nrows = (long long)nl * nr;
+      kr.nf = 1;
+      kr.ef = 1;
+      kr.sx_slab = pf[1];
+      kr.sxf = 0;
+      kr.sx1 = (long long)nl * nr * pf[1];
+      nx = pd[0] * nl * nr * pf[1];
+      ny = (long long)nl * nr * pd[2] * pf[3];
     }
     kr.sx2 = 1;
     kr.xs = nx;

@@ -300,6 +300,26 @@ void contract(Context& ctx, const TRef& C, const TRef& A, const TRef& B, int nsa
 
 FullView full_view(const TRef& t);
 
+// A factorised double-layer launch (FP32, kron.h), timed like the contraction
+// launches when profiling is on.
+template <typename F>
+void timed_kron(Context& ctx, double flops, F&& launch) {
+  if (!ctx.profile) {
+    SPP_CHECK(launch());
+    return;
+  }
+  hipEvent_t a, b;
+  SPP_CHECK(hipEventCreate(&a));
+  SPP_CHECK(hipEventCreate(&b));
+  SPP_CHECK(hipEventRecord(a, ctx.stream));
+  SPP_CHECK(launch());
+  SPP_CHECK(hipEventRecord(b, ctx.stream));
+  ctx.timer.pending.emplace_back(a, b);
+  ctx.timer.pending_flops.push_back(flops);
+  ctx.timer.pending_prec.push_back(1);
+  if (ctx.timer.pending.size() >= 4096) ctx.flush_timer();
+}
+
 // ------------------------------ boundary MPS -------------------------------
 template <typename T>
 struct Boundary {

@@ -1386,26 +1386,6 @@ static void eig_topk(Context& ctx, const double* rho, long long rho_pstride, int
 }
 
 // ------------------------------- absorb -------------------------------------
-// A factorised Gram transfer launch (FP32), timed like the contraction
-// launches when profiling is on.
-template <typename F>
-static void timed_launch(Context& ctx, double flops, F&& launch) {
-  if (!ctx.profile) {
-    SPP_CHECK(launch());
-    return;
-  }
-  hipEvent_t a, b;
-  SPP_CHECK(hipEventCreate(&a));
-  SPP_CHECK(hipEventCreate(&b));
-  SPP_CHECK(hipEventRecord(a, ctx.stream));
-  SPP_CHECK(launch());
-  SPP_CHECK(hipEventRecord(b, ctx.stream));
-  ctx.timer.pending.emplace_back(a, b);
-  ctx.timer.pending_flops.push_back(flops);
-  ctx.timer.pending_prec.push_back(1);
-  if (ctx.timer.pending.size() >= 4096) ctx.flush_timer();
-}
-
 template <typename T>
 void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& row, int chi, Boundary<T>& out) {
   const int L = env.L;
@@ -1475,7 +1455,7 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
         ka.sx1 = fe;
         ka.sx2 = 1;
         const double fa = 2.0 * nb * kron_row_macs(*ks, 0) * ka.nrows;
-        timed_launch(ctx, fa, [&] {
+        timed_kron(ctx, fa, [&] {
           return launch_kron_rows(*ks, 0, reinterpret_cast<const float*>(Z1), reinterpret_cast<float*>(Z2), ka,
                                   row.kron_tabs[x], nb, ctx.kron_rows, st, ctx.kron_groups);
         });
@@ -1488,7 +1468,7 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
         kb.sx1 = (long long)sp.w.dim * fs;
         kb.sx2 = 1;
         const double fb = 2.0 * nb * kron_row_macs(*ks, 1) * kb.nrows;
-        timed_launch(ctx, fb, [&] {
+        timed_kron(ctx, fb, [&] {
           return launch_kron_rows(*ks, 1, reinterpret_cast<const float*>(Z2), reinterpret_cast<float*>(Z1), kb,
                                   row.kron_tabs[x], nb, ctx.kron_rows, st, ctx.kron_groups);
         });

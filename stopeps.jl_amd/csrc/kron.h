@@ -19,6 +19,10 @@
 namespace spp {
 
 constexpr int kKronMaxTuple = 64;   // tuples of two single legs (D <= 8)
+// Role assignments (stages) tabulated per (site, label): 0 contracts the
+// layer's N and E legs into (W, S), 1 contracts E and S into (W, N), 2
+// contracts W and N into (E, S).
+constexpr int kKronStages = 3;
 
 // Lists of one stage (role assignment) of one (site, label), followed in
 // memory by the FP32 coefficients K1 [2][dphi][j1cap][no1] and
@@ -42,14 +46,14 @@ struct KronSite {
   int dphi = 1, ephi = 1;                         // inner index summed between ket and bra
   int no1 = 0, no2 = 0;                           // register tile of the kernel instantiation
   int j1cap = 0, j2cap = 0;
-  size_t stage_bytes = 0;                         // one stage's tables; stage 1 follows stage 0
+  size_t stage_bytes = 0;                         // one stage's tables; stage k at k * stage_bytes
 };
 
 // Chooses the instantiation and table capacities of a layer with these single
 // legs; false when no instantiation fits (the dense route is used then).
 bool kron_site_layout(const int d[4], const int e[4], int dphi, int ephi, KronSite& ks);
 // Tables of both stages for the single layer S (W, N, PHI, E, S; row-major,
-// FP64, host) of parity q, into out (2 * ks.stage_bytes bytes).
+// FP64, host) of parity q, into out (kKronStages * ks.stage_bytes bytes).
 void kron_build_tables(const KronSite& ks, const double* S, int q, std::vector<char>& out);
 // Parity of a single layer: 0 or 1 when every nonzero entry has that parity,
 // -1 when both occur (the factorised route needs one).
@@ -71,11 +75,12 @@ struct KronRows {
   long long ys = 0;
 };
 
-// Stage 0 contracts the layer's N and E legs and yields (W, S); stage 1
-// contracts E and S and yields (W, N). tabs[s] points at the (site, label)
-// tables of sample s. rows_per_block <= 0: the default.
+// One stage (kKronStages) over the rows. tabs[s] points at the (site, label)
+// tables of sample s; with tabs null every sample uses tab_shared.
+// rows_per_block <= 0: the default.
 hipError_t launch_kron_rows(const KronSite& ks, int stage, const float* x, float* y, const KronRows& rows,
-                            void* const* tabs, int nb, int rows_per_block, hipStream_t st, int groups = 1);
+                            void* const* tabs, int nb, int rows_per_block, hipStream_t st, int groups = 1,
+                            const void* tab_shared = nullptr);
 // Multiply-adds the kernel executes per row of a stage (register tile
 // included), averaged over the two row parities.
 double kron_row_macs(const KronSite& ks, int stage);

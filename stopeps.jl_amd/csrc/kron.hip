@@ -34,7 +34,7 @@ struct Inst {
 constexpr Inst kInst[] = {{8, 8}, {16, 16}, {18, 18}, {32, 32}, {64, 64}};
 
 // Roles (c1, c2, o1, o2) of the layer legs W=0, N=1, E=2, S=3 per stage.
-constexpr int kRole[2][4] = {{1, 2, 0, 3}, {2, 3, 0, 1}};
+constexpr int kRole[kKronStages][4] = {{1, 2, 0, 3}, {2, 3, 0, 1}, {0, 1, 2, 3}};
 
 inline int parl(int i, int e) { return i >= e ? 1 : 0; }
 
@@ -109,7 +109,7 @@ bool kron_site_layout(const int d[4], const int e[4], int dphi, int ephi, KronSi
   // per pass the stage-1 outputs (o1k, o2k) and the stage-2 bra outputs
   // (o1b, o2b) are tuples of one parity; the ket / bra inner lists too
   int need = 0, j1 = 0;
-  for (int stage = 0; stage < 2; ++stage) {
+  for (int stage = 0; stage < kKronStages; ++stage) {
     const RoleLegs r = roles(ks, stage);
     for (int pi = 0; pi < 2; ++pi) {
       need = std::max(need, tuple_count(r.d[2], r.e[2], r.d[3], r.e[3], pi));
@@ -162,8 +162,8 @@ int kron_layer_parity(const int d[4], const int e[4], int dphi, int ephi, const 
 }
 
 void kron_build_tables(const KronSite& ks, const double* S, int q, std::vector<char>& out) {
-  out.assign(2 * ks.stage_bytes, 0);
-  for (int stage = 0; stage < 2; ++stage) {
+  out.assign(kKronStages * ks.stage_bytes, 0);
+  for (int stage = 0; stage < kKronStages; ++stage) {
     const RoleLegs r = roles(ks, stage);
     char* base = out.data() + stage * ks.stage_bytes;
     KronTab* tab = reinterpret_cast<KronTab*>(base);
@@ -230,6 +230,7 @@ struct KronArgs {
   float* y;
   long long ys, rowY;
   void* const* tabs;
+  const void* tab_shared;  // tables of every sample when tabs is null
   long long stage_off;
   int rows, groups, rowmax, w1, w2, j1cap, j2cap, ntup1;
   float inv_rowx, inv_pf1, inv_rowy;  // 1 / (pd[0] * pf[1]), 1 / pf[1], 1 / rowY
@@ -260,7 +261,7 @@ __global__ void __launch_bounds__(kron_max_threads(NO1)) kron_rows_kernel(KronAr
   __shared__ int rho[32];
   __shared__ int xrow0[32];
   const int s = blockIdx.y;
-  const char* tb = reinterpret_cast<const char*>(a.tabs[s]) + a.stage_off;
+  const char* tb = reinterpret_cast<const char*>(a.tabs ? a.tabs[s] : a.tab_shared) + a.stage_off;
   const KronTab* tab = reinterpret_cast<const KronTab*>(tb);
   const float* K1 = reinterpret_cast<const float*>(tb + sizeof(KronTab));
   const float* K2 = K1 + 2 * a.dphi * a.j1cap * NO1;
@@ -497,7 +498,8 @@ void kron_prepare() {
 }
 
 hipError_t launch_kron_rows(const KronSite& ks, int stage, const float* x, float* y, const KronRows& rw,
-                            void* const* tabs, int nb, int rows_per_block, hipStream_t st, int groups) {
+                            void* const* tabs, int nb, int rows_per_block, hipStream_t st, int groups,
+                            const void* tab_shared) {
   if (nb <= 0 || rw.nrows <= 0) return hipSuccess;
   KronArgs a{};
   const RoleLegs r = roles(ks, stage);
@@ -529,6 +531,8 @@ hipError_t launch_kron_rows(const KronSite& ks, int stage, const float* x, float
   a.ys = rw.ys;
   a.rowY = (long long)a.pd[2] * a.pf[3];
   a.tabs = tabs;
+  a.tab_shared = tab_shared;
+  if (!tabs && !tab_shared) return hipErrorInvalidValue;
   a.stage_off = (long long)stage * (long long)ks.stage_bytes;
   a.j1cap = ks.j1cap;
   a.j2cap = ks.j2cap;

@@ -750,6 +750,43 @@ DrawWork<T>::~DrawWork() {
 template struct DrawWork<float>;
 template struct DrawWork<double>;
 
+// A left step of the draw, Y[b,c,e,s] = sum_{w,n} X[w,b,c,n] E[w,n,e,s], with
+// the layer E applied through its single-layer tables (kron.h, stage 2): per
+// sample (tabs) or one candidate for all samples (shared). X is folded on n
+// ([w][b][c][n']), Y on s ([b][c][e][s']). FP32 caches only.
+template <typename T>
+static void left_step_kron(Context& ctx, const KronSite& ks, const T* X, T* Y, long long xs, long long ys,
+                           const Leg& sbl, const Leg& nb1, const LayerSpec& sp, void* const* tabs, const void* shared,
+                           int cur) {
+  if constexpr (sizeof(T) != sizeof(float)) {
+    throw HipError("factorised left steps need an FP32 cache");
+  } else {
+    const long long fn = std::max(sp.n.even, sp.n.dim - sp.n.even);
+    KronRows kr;
+    kr.nrows = (long long)sbl.dim * nb1.dim;
+    kr.nf = 1;
+    kr.ef = 1;
+    kr.rc = nb1.dim;
+    kr.lsplit = sbl.split;
+    kr.leven = sbl.even;
+    kr.rsplit = nb1.split;
+    kr.reven = nb1.even;
+    kr.xs = xs;
+    kr.ys = ys;
+    kr.sx_slab = fn;
+    kr.sxf = 0;
+    kr.sx1 = (long long)sbl.dim * nb1.dim * fn;
+    kr.sx2 = 1;
+    const double fl = 2.0 * cur * kron_row_macs(ks, 2) * kr.nrows;
+    timed_kron(ctx, fl, [&] {
+      return launch_kron_rows(ks, 2, reinterpret_cast<const float*>(X), reinterpret_cast<float*>(Y), kr, tabs, cur,
+                              ctx.kron_rows, ctx.stream, ctx.kron_groups, shared);
+    });
+    ctx.counters.kron_launches += 1;
+    ctx.counters.gemm_flops += fl;
+  }
+}
+
 template <typename T>
 void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, const int* labels_in,
                 DrawOutputs& out) {
@@ -792,6 +829,7 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
   void** ptrs = W.ptrs;
   void** kptrs = W.ptrs + (size_t)nb * Lx;
   const bool kron = !c.kron_site.empty();
+  const bool kron_steps = kron && ctx.kron;  // candidate left steps through the single layers
   auto kron0 = [&](int site) -> void* { return kron ? c.kron_tab[0][site] : nullptr; };
   auto kron1 = [&](int site) -> void* { return kron ? c.kron_tab[1][site] : nullptr; };
   int* dlabels_in = labels_in ? W.dlab : nullptr;
@@ -959,7 +997,11 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
               TRef lay = dense_ref(nullptr, 0, "wnes", {sp.w, sp.n, sp.e, sp.s});
               lay.ptrs = (void* const*)(ptrs + (long long)x * cur);
               TRef Yr = folded_ref(Yk, ys, "bces", {sbl, Nc.bond(x + 1), sp.e, sp.s});
-              contract<T>(ctx, Yr, Xr, lay, cur);
+              if (kron_steps)
+                left_step_kron(ctx, c.kron_site[site], X, Yk, xs, ys, sbl, Nc.bond(x + 1), sp,
+                               (void* const*)(kptrs + (long long)x * cur), nullptr, cur);
+              else
+                contract<T>(ctx, Yr, Xr, lay, cur);
               TRef Lnew = dense_ref(Lb, os, "ced", Ol);
               contract<T>(ctx, Lnew, Yr, bottom_ref(sb, Lx, x, "bsd"), cur);
               FullView fv = full_view(Lnew);
@@ -975,7 +1017,11 @@ void draw_batch(Context& ctx, CacheDev<T>& c, int n, const double* uniforms, con
           double* rawk[2] = {raw0, raw1};
           for (int k = 0; k < 2; ++k) {
             TRef Yr = folded_ref(Yk, ys, "bces", {sbl, Nc.bond(x + 1), sp.e, sp.s});
-            contract<T>(ctx, Yr, Xr, dense_ref(c.cand[k][site], 0, "wnes", {sp.w, sp.n, sp.e, sp.s}), cur);
+            if (kron_steps)
+              left_step_kron(ctx, c.kron_site[site], X, Yk, xs, ys, sbl, Nc.bond(x + 1), sp, nullptr,
+                             c.kron_tab[k][site], cur);
+            else
+              contract<T>(ctx, Yr, Xr, dense_ref(c.cand[k][site], 0, "wnes", {sp.w, sp.n, sp.e, sp.s}), cur);
             TRef Or = dense_ref(O[k], os, "ced", Ol);
             contract<T>(ctx, Or, Yr, bottom_ref(sb, Lx, x, "bsd"), cur);
             SPP_CHECK(launch_dot<T>(full_view(Or), full_view(dense_ref(R[x + 1], Rs[x + 1], "ced", Rl[x + 1])),

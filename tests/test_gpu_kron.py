@@ -1,12 +1,13 @@
-"""Factorised Gram transfers (csrc/kron.hip) on the GPU.
+"""Factorised double-layer products (csrc/kron.hip) on the GPU.
 
 The kernel applies a double layer E = sum_phi S (x) S through its single
 layer S. Its reference here is the dense contraction with the double layer
 built from S in numpy (FP64), in the parity-sorted pair order and folded
 layouts of the absorb's intermediates: stage 0 is
-z2[l,R,F,w,p] = sum_{n,e} z1[l,R,F,n,e] E[w,n,e,p] and stage 1 is
+z2[l,R,F,w,p] = sum_{n,e} z1[l,R,F,n,e] E[w,n,e,p], stage 1 is
 z3[l,R,w,W,N] = sum_{F,p} z2[l,R,F,w,p] E[W,N,F,p] (boundary_mps.jl:136-151
-contracts the same double layer). The draws through the factorised route are
+contracts the same double layer) and stage 2, the draw's candidate left step,
+Y[b,c,e,s] = sum_{w,n} X[w,b,c,n] E[w,n,e,s] (boundary_mps.jl:233-239). The draws through the factorised route are
 then compared with the dense route and with the FP64 oracle."""
 import ctypes
 
@@ -102,8 +103,10 @@ def run_kron(spp, ctx, S, d, e, dphi, ephi, stage, nl, el, nr, er, x, rows=0):
     fold = lambda leg: max(leg[1], leg[0] - leg[1])
     if stage == 0:
         ny = nl * nr * Ep[0] * W[0] * fold(Sp)
-    else:
+    elif stage == 1:
         ny = nl * nr * W[0] * W[0] * fold(N)
+    else:
+        ny = nl * nr * Ep[0] * fold(Sp)
     xf = np.ascontiguousarray(x, dtype=np.float32).ravel()
     y = np.zeros(ny, dtype=np.float32)
     Sd = np.ascontiguousarray(S, dtype=np.float64)
@@ -130,7 +133,7 @@ CASES = [
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"d{c[0]}e{c[1]}phi{c[2]}q{c[4]}")
-@pytest.mark.parametrize("stage", [0, 1])
+@pytest.mark.parametrize("stage", [0, 1, 2])
 def test_kron_kernel_matches_dense_double_layer(spp, gpu_ctx, case, stage):
     d, e, dphi, ephi, q = case
     rng = np.random.default_rng(7 + stage + 3 * sum(d))
@@ -144,11 +147,16 @@ def test_kron_kernel_matches_dense_double_layer(spp, gpu_ctx, case, stage):
         z = neutral_random(rng, legs_in)
         ref = np.tensordot(z, E, axes=([3, 4], [1, 2]))          # [l, R, F, w, p]
         legs_out = [lleg, rleg, Ep, W, Sp]
-    else:
+    elif stage == 1:
         legs_in = [lleg, rleg, Ep, W, Sp]          # z2[l, R, F, w, p]
         z = neutral_random(rng, legs_in)
         ref = np.tensordot(z, E, axes=([2, 4], [2, 3]))          # [l, R, w, W, N]
         legs_out = [lleg, rleg, W, W, N]
+    else:
+        legs_in = [W, lleg, rleg, N]               # X[w, b, c, n] of a candidate left step
+        z = neutral_random(rng, legs_in)
+        ref = np.tensordot(z, E, axes=([0, 3], [0, 1]))          # [b, c, e, s]
+        legs_out = [lleg, rleg, Ep, Sp]
     x, _ = fold_last(z, legs_in)
     want, valid = fold_last(ref, legs_out)
     scale = max(np.abs(want).max(), 1e-30)
