@@ -414,8 +414,9 @@ def main():
     ap.add_argument("--batch", type=int, default=128, help="Born samples per step")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "bf16x6", "bf16x3"])
     ap.add_argument("--chi", type=int, default=32)
-    ap.add_argument("--kron", type=int, default=1, choices=[0, 1],
-                    help="1: factorised Gram transfers in the FP32 draws (default); 0: dense double layers")
+    ap.add_argument("--kron", type=int, default=1, choices=[0, 1, 2],
+                    help="1: factorised Gram transfers in the FP32 draws on the vector ALUs (default); "
+                         "2: the same on the matrix cores; 0: dense double layers")
     ap.add_argument("--plain-draws", type=int, default=3, help="batch-1 FP64 draws for the plain path")
     ap.add_argument("--check-samples", type=int, default=8,
                     help="drawn samples measured in the energy-throughput figure")
@@ -518,7 +519,8 @@ def main():
         "config": {"workload": f"batched Born draws, {L}x{L} TFIM J=1 h=2.0 Z2 PEPO at beta0=0.5, D=6 (3+3), d=2, "
                                f"chi={args.chi}, {B} samples per step", "lattice": [L, L], "D": 6, "chi": args.chi,
                    "samples_per_step": B, "cache_build_s": t_cache,
-                   "gram_transfers": "factorised (ket and bra single layers)" if args.kron else "dense double layers",
+                   "gram_transfers": {0: "dense double layers", 1: "factorised (ket and bra single layers)",
+                                      2: "factorised, matrix-core kernel"}[args.kron],
                    "input": "tests/golden/cfg2_pepo_beta05.npz: the beta0=0.5 PEPO of full-update purification "
                             "(5 steps of dtau=0.05, FP64), frozen; uniforms of sample indices 1.. of seed 1"},
         "roofline": {"bound": "mfma", "kernel": KERNEL[args.precision],

@@ -86,9 +86,12 @@ int spp_set_precision(spp_ctx* ctx, int precision);
  * with the same shape and replayed afterwards; 0: launched kernel by kernel;
  * off while "profile" is on); "kron" (1, the default: the Gram transfers of
  * the draw's FP32 absorbs apply each drawn layer through its ket and bra
- * single-layer factors in one kernel, about 9x fewer multiply-adds than the
- * dense double layer at D=6; 0: the dense double layer; results agree to FP32
- * rounding); "kron_rows" (rows per block of that kernel, 0 = default);
+ * single-layer factors in one kernel on the vector ALUs, about 9x fewer
+ * multiply-adds than the dense double layer at D=6; 2: the same products on
+ * the matrix cores (16x16x4 FP32 tiles; slower than 1 at D=6); 0: the dense
+ * double layer; results agree to FP32 rounding); "kron_rows" (rows per block
+ * of that kernel, 0 = default); "kron_groups" (row groups per block of the
+ * matrix-core kernel, 1..64, default 4);
  * "gemm_tile" (self-tests:
  * force one tile shape index of the kernel family on every launch, -1 = off). */
 int spp_set_option(spp_ctx* ctx, const char* name, double value);

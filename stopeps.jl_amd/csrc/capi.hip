@@ -170,7 +170,7 @@ int spp_set_option(spp_ctx* ctx, const char* name, double value) {
       if (value != 0 && value != 1) throw HipError("graphs must be 0 or 1");
       ctx->c.graphs = (int)value;
     } else if (n == "kron") {
-      if (value != 0 && value != 1) throw HipError("kron must be 0 or 1");
+      if (value != 0 && value != 1 && value != 2) throw HipError("kron must be 0, 1 or 2");
       ctx->c.kron = (int)value;
     } else if (n == "kron_groups") {
       if (value < 1 || value > 64) throw HipError("kron_groups must lie in [1, 64]");
@@ -823,7 +823,8 @@ int spp_selftest_kron(spp_ctx* ctx, const int* d, const int* e, int dphi, int ep
       void* tp = dt;
       upload(dp, &tp, sizeof(void*), st);
       SPP_CHECK(launch_fill<float>(dy, ny, 0.f, st));
-      SPP_CHECK(launch_kron_rows(ks, stage, dx, dy, kr, dp, 1, rows_per_block, st));
+      SPP_CHECK(launch_kron_rows(ks, stage, dx, dy, kr, dp, 1, rows_per_block, st, ctx->c.kron_groups, nullptr,
+                                 ctx->c.kron == 2));
       SPP_CHECK(hipMemcpyAsync(y, dy, sizeof(float) * ny, hipMemcpyDeviceToHost, st));
       SPP_CHECK(hipStreamSynchronize(st));
     } catch (...) {

@@ -263,11 +263,13 @@ struct Context {
   bool capturing = false;  // a row sweep is being captured: no tuning launches
   // Gram transfers of FP32 absorbs whose rows carry single-layer tables
   // (kron.h): 1 (the default) applies each double layer through its ket and
-  // bra factors, 0 contracts the dense double layer; kron_rows: rows per block
-  // of that kernel (0: its default).
+  // bra factors on the vector ALUs, 2 the same on the matrix cores (the vector
+  // kernel where no matrix-core instantiation covers the layer), 0 contracts
+  // the dense double layer; kron_rows: rows per block of that kernel (0: its
+  // default).
   int kron = 1;
   int kron_rows = 0;
-  int kron_groups = 1;      // row groups per block of that kernel
+  int kron_groups = 4;      // row groups per block of the matrix-core kernel
   void flush_timer();       // accumulate finished event pairs into counters
 };
 

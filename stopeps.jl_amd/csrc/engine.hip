@@ -1454,10 +1454,12 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
         ka.sxf = (long long)sp.n.dim * fe;
         ka.sx1 = fe;
         ka.sx2 = 1;
-        const double fa = 2.0 * nb * kron_row_macs(*ks, 0) * ka.nrows;
+        const double fa =
+            2.0 * nb * (ctx.kron == 2 ? kron_mfma_row_macs(*ks, 0) : kron_row_macs(*ks, 0)) * ka.nrows;
         timed_kron(ctx, fa, [&] {
           return launch_kron_rows(*ks, 0, reinterpret_cast<const float*>(Z1), reinterpret_cast<float*>(Z2), ka,
-                                  row.kron_tabs[x], nb, ctx.kron_rows, st, ctx.kron_groups);
+                                  row.kron_tabs[x], nb, ctx.kron_rows, st, ctx.kron_groups, nullptr,
+                                  ctx.kron == 2);
         });
         KronRows kb = ka;
         kb.nrows = (long long)bl.dim * rc.dim * sp.w.dim;
@@ -1467,10 +1469,12 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
         kb.sxf = fs;
         kb.sx1 = (long long)sp.w.dim * fs;
         kb.sx2 = 1;
-        const double fb = 2.0 * nb * kron_row_macs(*ks, 1) * kb.nrows;
+        const double fb =
+            2.0 * nb * (ctx.kron == 2 ? kron_mfma_row_macs(*ks, 1) : kron_row_macs(*ks, 1)) * kb.nrows;
         timed_kron(ctx, fb, [&] {
           return launch_kron_rows(*ks, 1, reinterpret_cast<const float*>(Z2), reinterpret_cast<float*>(Z1), kb,
-                                  row.kron_tabs[x], nb, ctx.kron_rows, st, ctx.kron_groups);
+                                  row.kron_tabs[x], nb, ctx.kron_rows, st, ctx.kron_groups, nullptr,
+                                  ctx.kron == 2);
         });
         ctx.counters.kron_launches += 2;
         ctx.counters.gemm_flops += fa + fb;

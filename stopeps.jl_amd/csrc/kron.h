@@ -77,13 +77,17 @@ struct KronRows {
 
 // One stage (kKronStages) over the rows. tabs[s] points at the (site, label)
 // tables of sample s; with tabs null every sample uses tab_shared.
-// rows_per_block <= 0: the default.
+// rows_per_block <= 0: the default. mfma: the matrix-core kernel where one of
+// its instantiations covers the layer (the vector kernel otherwise); groups:
+// row groups per block of the matrix-core kernel.
 hipError_t launch_kron_rows(const KronSite& ks, int stage, const float* x, float* y, const KronRows& rows,
                             void* const* tabs, int nb, int rows_per_block, hipStream_t st, int groups = 1,
-                            const void* tab_shared = nullptr);
-// Multiply-adds the kernel executes per row of a stage (register tile
-// included), averaged over the two row parities.
+                            const void* tab_shared = nullptr, bool mfma = false);
+// Multiply-adds the vector kernel executes per row of a stage (register tile
+// included), averaged over the two row parities; kron_mfma_row_macs: the same
+// for the matrix-core kernel (its tile padding included).
 double kron_row_macs(const KronSite& ks, int stage);
+double kron_mfma_row_macs(const KronSite& ks, int stage);
 // Raise the LDS limit of every instantiation (once per process, before any
 // capture).
 void kron_prepare();

@@ -777,10 +777,11 @@ static void left_step_kron(Context& ctx, const KronSite& ks, const T* X, T* Y, l
     kr.sxf = 0;
     kr.sx1 = (long long)sbl.dim * nb1.dim * fn;
     kr.sx2 = 1;
-    const double fl = 2.0 * cur * kron_row_macs(ks, 2) * kr.nrows;
+    const double fl =
+        2.0 * cur * (ctx.kron == 2 ? kron_mfma_row_macs(ks, 2) : kron_row_macs(ks, 2)) * kr.nrows;
     timed_kron(ctx, fl, [&] {
       return launch_kron_rows(ks, 2, reinterpret_cast<const float*>(X), reinterpret_cast<float*>(Y), kr, tabs, cur,
-                              ctx.kron_rows, ctx.stream, ctx.kron_groups, shared);
+                              ctx.kron_rows, ctx.stream, ctx.kron_groups, shared, ctx.kron == 2);
     });
     ctx.counters.kron_launches += 1;
     ctx.counters.gemm_flops += fl;

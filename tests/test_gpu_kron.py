@@ -7,8 +7,9 @@ layouts of the absorb's intermediates: stage 0 is
 z2[l,R,F,w,p] = sum_{n,e} z1[l,R,F,n,e] E[w,n,e,p], stage 1 is
 z3[l,R,w,W,N] = sum_{F,p} z2[l,R,F,w,p] E[W,N,F,p] (boundary_mps.jl:136-151
 contracts the same double layer) and stage 2, the draw's candidate left step,
-Y[b,c,e,s] = sum_{w,n} X[w,b,c,n] E[w,n,e,s] (boundary_mps.jl:233-239). The draws through the factorised route are
-then compared with the dense route and with the FP64 oracle."""
+Y[b,c,e,s] = sum_{w,n} X[w,b,c,n] E[w,n,e,s] (boundary_mps.jl:233-239). Both kernels run every case: the vector one (the default) and the matrix-core
+one (16x16x4 FP32 MFMA tiles). The draws through the factorised route are then compared with
+the dense route and with the FP64 oracle."""
 import ctypes
 
 import numpy as np
@@ -134,7 +135,16 @@ CASES = [
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"d{c[0]}e{c[1]}phi{c[2]}q{c[4]}")
 @pytest.mark.parametrize("stage", [0, 1, 2])
-def test_kron_kernel_matches_dense_double_layer(spp, gpu_ctx, case, stage):
+@pytest.mark.parametrize("kernel", [1, 2], ids=["vector", "matrix_cores"])
+def test_kron_kernel_matches_dense_double_layer(spp, gpu_ctx, case, stage, kernel):
+    gpu_ctx.set_option("kron", kernel)
+    try:
+        check_kron_case(spp, gpu_ctx, case, stage)
+    finally:
+        gpu_ctx.set_option("kron", 1)
+
+
+def check_kron_case(spp, gpu_ctx, case, stage):
     d, e, dphi, ephi, q = case
     rng = np.random.default_rng(7 + stage + 3 * sum(d))
     S = random_layer(rng, d, e, dphi, ephi, q)
@@ -168,13 +178,14 @@ def test_kron_kernel_matches_dense_double_layer(spp, gpu_ctx, case, stage):
 
 @pytest.mark.parametrize("D,chi", [(2, 6), (3, 8)])
 def test_kron_draws_match_dense_route(spp, D, chi):
-    """FP32 draws through the factorised Gram transfers against the dense
-    double-layer route on the same uniforms: the same configurations (up to a
-    uniform within FP32 rounding of a boundary) and log p to FP32 rounding."""
+    """FP32 draws through the factorised Gram transfers (vector and matrix-core
+    kernels) against the dense double-layer route on the same uniforms: the
+    same configurations (up to a uniform within FP32 rounding of a boundary)
+    and log p to FP32 rounding."""
     pepo = spp.synthetic_thermal_pepo(spp.Lattice(4, 4), D=(D - D // 2, D // 2), eps=0.3, seed=11)
     uni = np.stack([spp.sample_uniforms(3, i, 16) for i in range(1, 17)])
     out = {}
-    for kron in (1, 0):
+    for kron in (1, 2, 0):
         ctx = spp.Context(0, "fp32", kron=kron, graphs=0)
         try:
             cache = spp.SamplingCache(ctx, pepo, chi)
@@ -182,14 +193,16 @@ def test_kron_draws_match_dense_route(spp, D, chi):
             out[kron] = (spp.sample_configurations(cache, uniforms=uni), ctx.counters()["kron_launches"])
         finally:
             ctx.close()
-    assert out[1][1] > 0 and out[0][1] == 0
-    a, b = out[1][0], out[0][0]
-    same = [np.array_equal(a["configuration"][i], b["configuration"][i]) for i in range(len(uni))]
-    assert sum(same) >= len(uni) - 1
-    for i in range(len(uni)):
-        if same[i]:
-            assert abs(a["log_probability"][i] - b["log_probability"][i]) <= 1e-4 * max(1.0, abs(b["log_probability"][i]))
-            np.testing.assert_allclose(a["conditionals"][i], b["conditionals"][i], atol=1e-4)
+    assert out[1][1] > 0 and out[2][1] > 0 and out[0][1] == 0
+    b = out[0][0]
+    for kernel in (1, 2):
+        a = out[kernel][0]
+        same = [np.array_equal(a["configuration"][i], b["configuration"][i]) for i in range(len(uni))]
+        assert sum(same) >= len(uni) - 1
+        for i in range(len(uni)):
+            if same[i]:
+                assert abs(a["log_probability"][i] - b["log_probability"][i]) <= 1e-4 * max(1.0, abs(b["log_probability"][i]))
+                np.testing.assert_allclose(a["conditionals"][i], b["conditionals"][i], atol=1e-4)
 
 
 def test_kron_draws_match_oracle(spp):
