@@ -761,20 +761,24 @@ int spp_selftest_kron(spp_ctx* ctx, const int* d, const int* e, int dphi, int ep
     std::vector<char> tabs;
     kron_build_tables(ks, S, q, tabs);
     kron_prepare();
+    // pair legs W=0, N=1, E=2, S=3 of the absorb's frame: the layer's own legs, or for
+    // stage 3 (the first transfer of a rotated row) the rotated frame (E, S, W, N)
+    static const int kFrame[2][4] = {{0, 1, 2, 3}, {2, 3, 0, 1}};
+    const int* fr = kFrame[stage == 3 ? 1 : 0];
     long long pd[4], pf[4];
     int pe[4];
     for (int k = 0; k < 4; ++k) {
-      pd[k] = (long long)d[k] * d[k];
-      pe[k] = e[k] * e[k] + (d[k] - e[k]) * (d[k] - e[k]);
+      const int dk = d[fr[k]], ek = e[fr[k]];
+      pd[k] = (long long)dk * dk;
+      pe[k] = ek * ek + (dk - ek) * (dk - ek);
       pf[k] = std::max<long long>(pe[k], pd[k] - pe[k]);
     }
-    // legs W=0, N=1, E=2, S=3
     KronRows kr;
     kr.rc = nr;
     kr.leven = el;
     kr.reven = er;
     long long nx, ny;
-    if (stage == 0) {
+    if (stage == 0 || stage == 3) {
       kr.nrows = (long long)nl * nr * pd[2];
       kr.nf = (int)pd[2];
       kr.ef = pe[2];

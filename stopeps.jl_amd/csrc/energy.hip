@@ -529,6 +529,28 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
     dtens = (T**)dev(sizeof(T*) * Lx);
     upload(dbase, base.data(), sizeof(void*) * base.size(), st);
     upload(dlabels, labels_h, sizeof(int) * (size_t)n * ns, st);
+    // The sweeps' absorbs apply each projected layer through its single-layer
+    // tables (FP32 caches that carry them, kron.h): per chunk, the (site, label)
+    // table pointer of every sample, dktab[site * nb + s].
+    const bool kron = sizeof(T) == sizeof(float) && ctx.kron && !c.kron_site.empty();
+    std::vector<void*> kbase;
+    void** dkbase = nullptr;
+    void** dktab = nullptr;
+    if (kron) {
+      kbase.resize((size_t)ns * 2);
+      for (int i = 0; i < ns; ++i)
+        for (int k = 0; k < 2; ++k) kbase[(size_t)i * 2 + k] = c.kron_tab[k][i];
+      dkbase = (void**)dev(sizeof(void*) * kbase.size());
+      dktab = (void**)dev(sizeof(void*) * (size_t)ns * nb);
+      upload(dkbase, kbase.data(), sizeof(void*) * kbase.size(), st);
+    }
+    auto kron_row = [&](RowLayers<T>& r, int slot, int site) {
+      if (!kron) return;
+      r.kron_site.resize(Lx, nullptr);
+      r.kron_tabs.resize(Lx, nullptr);
+      r.kron_site[slot] = &c.kron_site[site];
+      r.kron_tabs[slot] = dktab + (size_t)site * nb;
+    };
     std::vector<int> h_status(nb), h_fail(nb);
     std::vector<double> h_x(nb), h_zz(nb), h_mant(nb), h_ls(nb), h_mtn(nb), h_mts(nb), h_amp(nb);
     for (long long s0 = 0; s0 < n; s0 += nb) {
@@ -539,6 +561,11 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
       hipLaunchKernelGGL(layer_tables_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, lab, ns,
                          (void* const*)dbase, (int)kKinds, run.tables, cur, nb);
       SPP_CHECK(hipGetLastError());
+      if (kron) {
+        hipLaunchKernelGGL(layer_tables_kernel, dim3((unsigned)(((long long)ns * cur + 255) / 256)), dim3(256), 0,
+                           st, lab, ns, (void* const*)dkbase, 1, dktab, cur, nb);
+        SPP_CHECK(hipGetLastError());
+      }
       SPP_CHECK(launch_fill<int>(run.status, nb, 0, st));
       SPP_CHECK(launch_fill<int>(run.fail, nb, -1, st));
       SPP_CHECK(launch_fill<double>(run.xsum, nb, 0.0, st));
@@ -560,6 +587,7 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
           const LayerSpec& sp = c.spec[x + y * Lx];
           rr.spec[xr] = LayerSpec{sp.e, sp.s, sp.w, sp.n};
           rr.ptrs[xr] = run.table(kPlain, x + y * Lx);
+          kron_row(rr, xr, x + y * Lx);
         }
         eobs.attach(south[y], 0, y);
         absorb_forward<T>(ctx, south[y + 1], rr, chi, south[y]);
@@ -579,7 +607,10 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
         row.L = Lx;
         row.spec.assign(c.spec.begin() + y * Lx, c.spec.begin() + (y + 1) * Lx);
         row.ptrs.resize(Lx);
-        for (int x = 0; x < Lx; ++x) row.ptrs[x] = run.table(kPlain, x + y * Lx);
+        for (int x = 0; x < Lx; ++x) {
+          row.ptrs[x] = run.table(kPlain, x + y * Lx);
+          kron_row(row, x, x + y * Lx);
+        }
         eobs.attach(NB[1 - which], 1, y);
         absorb_forward<T>(ctx, Nc, row, chi, NB[1 - which]);
         which = 1 - which;

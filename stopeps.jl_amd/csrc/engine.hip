@@ -1415,8 +1415,10 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
   T* Z1 = zmax ? (T*)ctx.arena.take(sizeof(T) * nb * zmax) : nullptr;
   T* Z2 = zmax ? (T*)ctx.arena.take(sizeof(T) * nb * zmax) : nullptr;
   int* rsplit = (int*)ctx.arena.take(sizeof(int) * nb);
-  bool use_kron = sizeof(T) == sizeof(float) && ctx.kron && !row.rotated && (int)row.kron_site.size() == L &&
+  bool use_kron = sizeof(T) == sizeof(float) && ctx.kron && (int)row.kron_site.size() == L &&
                   (int)row.kron_tabs.size() == L;
+  // role assignments of the two transfers in the row's frame (kron.h)
+  const int kst0 = row.rotated ? 3 : 0, kst1 = row.rotated ? 2 : 1;
   for (int x = 1; x < L && use_kron; ++x) use_kron = row.kron_site[x] && row.kron_tabs[x];
   for (int x = L - 1; x >= 1; --x) {
     const LayerSpec& sp = row.spec[x];
@@ -1455,9 +1457,9 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
         ka.sx1 = fe;
         ka.sx2 = 1;
         const double fa =
-            2.0 * nb * (ctx.kron == 2 ? kron_mfma_row_macs(*ks, 0) : kron_row_macs(*ks, 0)) * ka.nrows;
+            2.0 * nb * (ctx.kron == 2 ? kron_mfma_row_macs(*ks, kst0) : kron_row_macs(*ks, kst0)) * ka.nrows;
         timed_kron(ctx, fa, [&] {
-          return launch_kron_rows(*ks, 0, reinterpret_cast<const float*>(Z1), reinterpret_cast<float*>(Z2), ka,
+          return launch_kron_rows(*ks, kst0, reinterpret_cast<const float*>(Z1), reinterpret_cast<float*>(Z2), ka,
                                   row.kron_tabs[x], nb, ctx.kron_rows, st, ctx.kron_groups, nullptr,
                                   ctx.kron == 2);
         });
@@ -1470,9 +1472,9 @@ void absorb_forward(Context& ctx, const Boundary<T>& env, const RowLayers<T>& ro
         kb.sx1 = (long long)sp.w.dim * fs;
         kb.sx2 = 1;
         const double fb =
-            2.0 * nb * (ctx.kron == 2 ? kron_mfma_row_macs(*ks, 1) : kron_row_macs(*ks, 1)) * kb.nrows;
+            2.0 * nb * (ctx.kron == 2 ? kron_mfma_row_macs(*ks, kst1) : kron_row_macs(*ks, kst1)) * kb.nrows;
         timed_kron(ctx, fb, [&] {
-          return launch_kron_rows(*ks, 1, reinterpret_cast<const float*>(Z2), reinterpret_cast<float*>(Z1), kb,
+          return launch_kron_rows(*ks, kst1, reinterpret_cast<const float*>(Z2), reinterpret_cast<float*>(Z1), kb,
                                   row.kron_tabs[x], nb, ctx.kron_rows, st, ctx.kron_groups, nullptr,
                                   ctx.kron == 2);
         });

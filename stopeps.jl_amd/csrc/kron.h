@@ -21,8 +21,10 @@ namespace spp {
 constexpr int kKronMaxTuple = 64;   // tuples of two single legs (D <= 8)
 // Role assignments (stages) tabulated per (site, label): 0 contracts the
 // layer's N and E legs into (W, S), 1 contracts E and S into (W, N), 2
-// contracts W and N into (E, S).
-constexpr int kKronStages = 3;
+// contracts W and N into (E, S), 3 contracts S and W into (E, N). An absorb
+// runs 0 then 1; an absorb in the rotated frame (W', N', E', S') = (E, S, W, N)
+// runs 3 then 2, which are its 0 and 1 in that frame.
+constexpr int kKronStages = 4;
 
 // Lists of one stage (role assignment) of one (site, label), followed in
 // memory by the FP32 coefficients K1 [2][dphi][j1cap][no1] and
@@ -52,7 +54,7 @@ struct KronSite {
 // Chooses the instantiation and table capacities of a layer with these single
 // legs; false when no instantiation fits (the dense route is used then).
 bool kron_site_layout(const int d[4], const int e[4], int dphi, int ephi, KronSite& ks);
-// Tables of both stages for the single layer S (W, N, PHI, E, S; row-major,
+// Tables of every stage for the single layer S (W, N, PHI, E, S; row-major,
 // FP64, host) of parity q, into out (kKronStages * ks.stage_bytes bytes).
 void kron_build_tables(const KronSite& ks, const double* S, int q, std::vector<char>& out);
 // Parity of a single layer: 0 or 1 when every nonzero entry has that parity,

@@ -34,7 +34,7 @@ struct Inst {
 constexpr Inst kInst[] = {{8, 8}, {16, 16}, {18, 18}, {32, 32}, {64, 64}};
 
 // Roles (c1, c2, o1, o2) of the layer legs W=0, N=1, E=2, S=3 per stage.
-constexpr int kRole[kKronStages][4] = {{1, 2, 0, 3}, {2, 3, 0, 1}, {0, 1, 2, 3}};
+constexpr int kRole[kKronStages][4] = {{1, 2, 0, 3}, {2, 3, 0, 1}, {0, 1, 2, 3}, {3, 0, 2, 1}};
 
 inline int parl(int i, int e) { return i >= e ? 1 : 0; }
 

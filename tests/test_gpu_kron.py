@@ -6,8 +6,11 @@ built from S in numpy (FP64), in the parity-sorted pair order and folded
 layouts of the absorb's intermediates: stage 0 is
 z2[l,R,F,w,p] = sum_{n,e} z1[l,R,F,n,e] E[w,n,e,p], stage 1 is
 z3[l,R,w,W,N] = sum_{F,p} z2[l,R,F,w,p] E[W,N,F,p] (boundary_mps.jl:136-151
-contracts the same double layer) and stage 2, the draw's candidate left step,
-Y[b,c,e,s] = sum_{w,n} X[w,b,c,n] E[w,n,e,s] (boundary_mps.jl:233-239). Both kernels run every case: the vector one (the default) and the matrix-core
+contracts the same double layer), stage 2, the draw's candidate left step,
+Y[b,c,e,s] = sum_{w,n} X[w,b,c,n] E[w,n,e,s] (boundary_mps.jl:233-239), and
+stage 3, the first transfer of an absorb in the rotated frame (the energy's
+south sweep, boundary_mps.jl:193-209), z2[l,R,F,e,n] = sum_{s,w} z1[l,R,F,s,w]
+E[w,n,e,s]; the rotated frame's second transfer is stage 2. Both kernels run every case: the vector one (the default) and the matrix-core
 one (16x16x4 FP32 MFMA tiles). The draws through the factorised route are then compared with
 the dense route and with the FP64 oracle."""
 import ctypes
@@ -104,6 +107,8 @@ def run_kron(spp, ctx, S, d, e, dphi, ephi, stage, nl, el, nr, er, x, rows=0):
     fold = lambda leg: max(leg[1], leg[0] - leg[1])
     if stage == 0:
         ny = nl * nr * Ep[0] * W[0] * fold(Sp)
+    elif stage == 3:
+        ny = nl * nr * W[0] * Ep[0] * fold(N)
     elif stage == 1:
         ny = nl * nr * W[0] * W[0] * fold(N)
     else:
@@ -134,7 +139,7 @@ CASES = [
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"d{c[0]}e{c[1]}phi{c[2]}q{c[4]}")
-@pytest.mark.parametrize("stage", [0, 1, 2])
+@pytest.mark.parametrize("stage", [0, 1, 2, 3])
 @pytest.mark.parametrize("kernel", [1, 2], ids=["vector", "matrix_cores"])
 def test_kron_kernel_matches_dense_double_layer(spp, gpu_ctx, case, stage, kernel):
     gpu_ctx.set_option("kron", kernel)
@@ -162,11 +167,18 @@ def check_kron_case(spp, gpu_ctx, case, stage):
         z = neutral_random(rng, legs_in)
         ref = np.tensordot(z, E, axes=([2, 4], [2, 3]))          # [l, R, w, W, N]
         legs_out = [lleg, rleg, W, W, N]
-    else:
+    elif stage == 2:
         legs_in = [W, lleg, rleg, N]               # X[w, b, c, n] of a candidate left step
         z = neutral_random(rng, legs_in)
         ref = np.tensordot(z, E, axes=([0, 3], [0, 1]))          # [b, c, e, s]
         legs_out = [lleg, rleg, Ep, Sp]
+    else:
+        # the rotated frame's first transfer, frame legs (W', N', E', S') = (E, S, W, N):
+        # z2[l, R, F, w', p'] = sum_{n', e'} z1[l, R, F, n', e'] E[w=e', n=p', e=w', s=n']
+        legs_in = [lleg, rleg, W, Sp, W]           # z1[l, R, F, n', e']
+        z = neutral_random(rng, legs_in)
+        ref = np.tensordot(z, E, axes=([3, 4], [3, 0])).swapaxes(3, 4)   # [l, R, F, w', p']
+        legs_out = [lleg, rleg, W, Ep, N]
     x, _ = fold_last(z, legs_in)
     want, valid = fold_last(ref, legs_out)
     scale = max(np.abs(want).max(), 1e-30)
@@ -228,3 +240,32 @@ def test_kron_draws_match_oracle(spp):
             agree += 1
             assert abs(ref["log_probability"] - got["log_probability"][i]) <= 1e-4 * max(1.0, abs(ref["log_probability"]))
     assert agree >= len(uni) - 1
+
+
+@pytest.mark.parametrize("D,chi", [(3, 8)])
+def test_kron_energies_match_dense_route(spp, D, chi):
+    """FP32 energies of projected samples with the sweeps' absorbs (north, and
+    south in the rotated frame) through the single-layer tables against the
+    dense double layers: energies, log norms and the recorded boundary
+    entropies to FP32 rounding."""
+    pepo = spp.synthetic_thermal_pepo(spp.Lattice(4, 4), D=(D - D // 2, D // 2), eps=0.3, seed=13)
+    rng = np.random.default_rng(5)
+    out = {}
+    for kron in (1, 0):
+        ctx = spp.Context(0, "fp32", kron=kron, graphs=0)
+        try:
+            cache = spp.SamplingCache(ctx, pepo, chi)
+            cfgs = np.asarray(cache.basis.labels)[rng.integers(0, 2, size=(6, 4, 4))] if kron else out[1][2]
+            ctx.reset_counters()
+            got = spp.measure_energies(cache, cfgs, spp.TFIM(1.0, 2.0, symmetric=True), chi=chi, record_entropy=True)
+            out[kron] = (got, ctx.counters()["kron_launches"], cfgs)
+        finally:
+            ctx.close()
+    assert out[1][1] > 0 and out[0][1] == 0
+    a, b = out[1][0], out[0][0]
+    assert np.all(a["status"] == 0) and np.all(b["status"] == 0)
+    for k in ("energy_total", "energy_per_site", "log_norm", "tensor_log_norm"):
+        np.testing.assert_allclose(a[k], b[k], rtol=2e-5, atol=2e-5, err_msg=k)
+    for i in range(len(a["Se"])):
+        for side in ("north", "south"):
+            np.testing.assert_allclose(a["Se"][i][side], b["Se"][i][side], atol=2e-4)
