@@ -287,6 +287,40 @@ def entropy_timing(spp, n=16, chi=32, J=1.0, h=2.0):
             "max_boundary_entropy": float(np.max(out[True][1]["max_boundary_entropy"]))}
 
 
+def energy_timing(spp, n=32, chi=32, J=1.0, h=2.0, krons=(1, 0)):
+    """FP32 energy measurement of n projected samples of the frozen beta0 PEPO
+    (random labels, seed 0) at chi, one call after a 4-sample warm-up call
+    (which tunes the tile shapes), per value of the ``kron`` option: the sweeps'
+    absorbs through the single-layer tables (1) or the dense double layers (0)."""
+    pepo, _ = spp.load_state_npz(PEPO_FIXTURE)
+    model = spp.TFIM(J, h, symmetric=True)
+    out, res = {}, {}
+    for kron in krons:
+        ctx = spp.Context(0, "fp32", kron=kron)
+        try:
+            cache = spp.SamplingCache(ctx, pepo, chi)
+            labels = np.asarray(cache.basis.labels)
+            cfg = labels[np.random.default_rng(0).integers(0, 2, size=(n, 8, 8))]
+            spp.measure_energies(cache, cfg[:4], model)
+            ctx.synchronize()
+            ctx.reset_counters()
+            t0 = time.perf_counter()
+            r = spp.measure_energies(cache, cfg, model)
+            ctx.synchronize()
+            t = time.perf_counter() - t0
+            res[kron] = r
+            out[f"kron{kron}"] = {"value": n / t, "unit": "energy measurements/s", "s": t,
+                                  "kron_launches": ctx.counters()["kron_launches"]}
+            log(f"energy kron={kron}: {n / t:.3f} measurements/s")
+        finally:
+            ctx.close()
+    if 1 in res and 0 in res:
+        out["max_abs_dE_per_site_kron_vs_dense"] = float(np.max(np.abs(res[1]["energy_per_site"] -
+                                                                      res[0]["energy_per_site"])))
+    out["workload"] = f"FP32 energy of {n} projected samples (random labels), cfg2 beta0 PEPO 8x8 D=6, chi={chi}"
+    return out
+
+
 def fu_timing(spp, batch=16, chi=32, J=1.0, h=2.0, delta=0.05, warmup=False):
     """The full-update evolution workload: one second-order step (sweeps H->,
     V->, V<-, H<-; 224 bond updates) of a batch of 8x8 Z2 PEPS, D=6 (3+3),
@@ -429,6 +463,8 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true",
                     help="time only the log-norm gradient workload (for rocprofv3)")
     ap.add_argument("--fu-only", action="store_true", help="time only the full-update workload")
+    ap.add_argument("--energy-only", type=str, default=None, metavar="KRONS",
+                    help="time only the FP32 energy measurement for these comma-separated kron values (e.g. 1,0)")
     ap.add_argument("--physical-only", action="store_true", help="time only the physical-kind draws")
     ap.add_argument("--fu-warmup", action="store_true", help="an untimed full-update step first (tunes the tiles)")
     ap.add_argument("--fu-batch", type=int, default=16, help="states per full-update step (0: skip)")
@@ -454,6 +490,10 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_timing(spp)}), flush=True)
+        return
+    if args.energy_only:
+        krons = tuple(int(v) for v in args.energy_only.split(","))
+        print(json.dumps({"energy_measurement": energy_timing(spp, krons=krons)}), flush=True)
         return
     if args.fu_only:
         print(json.dumps({"full_update": fu_timing(spp, batch=max(args.fu_batch, 1), warmup=args.fu_warmup)}),
