@@ -507,7 +507,8 @@ def main():
                                                              batch=args.sweep_batch)}), flush=True)
         return
     if args.physical_only:
-        print(json.dumps({"physical_draws": physical_timing(spp)}), flush=True)
+        print(json.dumps({"physical_draws": {p: physical_timing(spp, precision=p) for p in ("fp64", "fp32")}}),
+              flush=True)
         return
     pepo, _ = spp.load_state_npz(PEPO_FIXTURE)
     L = pepo.lattice.Lx
@@ -682,7 +683,10 @@ def main():
     result["adjoint"] = adjoint_timing(spp)
     log("physical-kind draws ...")
     result["physical_draws"] = physical_timing(spp)
-    log(f"physical-kind draws: {result['physical_draws']['value']:.3f} samples/s")
+    p32 = physical_timing(spp, precision="fp32")
+    result["physical_draws"]["fp32"] = {k: p32[k] for k in ("value", "unit", "dtype", "s_per_call",
+                                                           "largest_negative_spectral_fraction")}
+    log(f"physical-kind draws: {result['physical_draws']['value']:.3f} samples/s (fp32: {p32['value']:.3f})")
     if args.fu_batch > 0:
         log("full-update step ...")
         result["full_update"] = fu_timing(spp, batch=args.fu_batch, J=args.J, h=args.h, warmup=True)

@@ -24,6 +24,15 @@ __host__ __device__ inline int ql_project(int n, double* __restrict__ d, double*
                                           double* __restrict__ y, long long ystride, int ny, int maxit) {
   if (n <= 0) return 0;
   e[n - 1] = 0.0;
+  // A coupling is negligible below eps times either its two diagonal entries or
+  // the matrix's norm (a row-sum bound, invariant up to rounding under the
+  // rotations): the second keeps rank-deficient matrices, whose null space ends
+  // up as blocks of entries at the rounding level of the norm, from iterating on
+  // couplings that no local test can call negligible (absolute accuracy
+  // eps * |T|, that of a backward-stable eigensolver).
+  double anorm = 0.0;
+  for (int i = 0; i < n; ++i) anorm = fmax(anorm, fabs(d[i]) + fabs(e[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0));
+  const double tol = DBL_EPSILON * anorm;
   double* __restrict__ y0 = y;
   double* __restrict__ y1 = y + ystride;
   for (int l = 0; l < n; ++l) {
@@ -33,7 +42,7 @@ __host__ __device__ inline int ql_project(int n, double* __restrict__ d, double*
       int m = l;
       for (; m < n - 1; ++m) {
         const double dd = fabs(d[m]) + fabs(d[m + 1]);
-        if (fabs(e[m]) <= DBL_EPSILON * dd) break;
+        if (fabs(e[m]) <= DBL_EPSILON * dd || fabs(e[m]) <= tol) break;
       }
       if (m == l) break;
       if (iter++ >= maxit) return 1;

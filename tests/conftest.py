@@ -71,3 +71,18 @@ def to_oracle_state(pstate):
     L = pstate.lattice
     sites = [[Z2(pstate[(x, y)].data, pstate[(x, y)].parity) for y in range(L.Ly)] for x in range(L.Lx)]
     return State(Lattice(L.Lx, L.Ly), sites, pstate.kind, beta=pstate.beta, log_amplitude=pstate.log_amplitude)
+
+
+def boundary_distance(basis, ref, cfg_ref, cfg_got, u, L):
+    """Distance of the uniform from the oracle's cumulative-probability
+    boundary at the first site (row-major) where two draws differ: with two
+    outcomes, ``_categorical`` (src/resample/selection.jl:68-80) picks the
+    first when u < p_first, so a draw can flip only if u lies within the
+    conditional's error of p_first."""
+    for j in range(L.Lx * L.Ly):
+        x, y = j % L.Lx, j // L.Lx
+        if cfg_ref[x][y] != cfg_got[x][y]:
+            c = ref["conditionals"][j]
+            p_first = c if basis.index(cfg_ref[x][y]) == 0 else 1.0 - c
+            return abs(u[j] - p_first)
+    raise AssertionError("draws do not differ")

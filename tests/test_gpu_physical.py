@@ -18,7 +18,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import to_product_state, oracle_cfg, charged_peps
+from conftest import to_product_state, oracle_cfg, charged_peps, boundary_distance
 from oracle.model import Lattice
 from oracle import born as ob
 
@@ -176,3 +176,42 @@ def test_physical_fp32_draws(spp):
             ctx.close()
     assert np.array_equal(res["fp64"]["configuration"], res["fp32"]["configuration"])
     assert np.max(np.abs(res["fp64"]["log_probability"] - res["fp32"]["log_probability"])) < 1e-5
+
+
+@pytest.mark.parametrize("symmetric,D,chi", [(True, (2, 1), 3), (True, (2, 2), 16), (False, 3, 3)])
+def test_physical_draws_fp32_match_oracle(spp, symmetric, D, chi):
+    """Physical-kind draws with FP32 contractions (the factorised Gram transfers
+    and left steps where the layers allow them) against the FP64 oracle on the
+    same uniforms: configurations identical, except a draw whose first differing
+    site has its uniform within 1e-4 of a cumulative-probability boundary of the
+    oracle's conditional there; log p of the agreeing draws within 1e-4.
+
+    The truncating cases use a chi that does not cut a degenerate pair: a
+    projected PEPS row is a ket row times its own bra, so a north boundary's
+    Gram spectrum holds the products s_i s_j of the ket boundary's values, each
+    pair i != j twice. A chi that keeps one of a pair leaves the kept direction
+    to rounding, in any implementation, and with the large truncation of these
+    tiny lattices the draws' log p then differ at the 1e-2 level in FP64 as
+    well (the same uniforms, from the first site below that boundary on)."""
+    L = Lattice(3, 3)
+    aux = [1, 0, 0, 1, 1, 0, 1, 0, 1]
+    st = charged_peps(L, D, 23, aux, symmetric)
+    basis = ob.resolve_basis(symmetric)
+    ocache = ob.SamplingCache(st, chi, basis=basis, kind="physical")
+    ctx = spp.Context(0, "fp32")
+    try:
+        cache = spp.SamplingCache(ctx, to_product_state(spp, st), chi, kind="physical")
+        uni = np.stack([ob.counter_uniforms(9, i, 9) for i in range(1, 9)])
+        say("physical draws fp32", symmetric, D, chi)
+        got = spp.sample_configurations(cache, uniforms=uni)
+    finally:
+        ctx.close()
+    near = []
+    for i in range(len(uni)):
+        ref = ob.sample_configuration(ocache, uniforms=uni[i])
+        cfg = oracle_cfg(ref, L)
+        if np.array_equal(cfg, got["configuration"][i]):
+            assert abs(got["log_probability"][i] - ref["log_probability"]) <= 1e-4 * max(1.0, abs(ref["log_probability"]))
+            continue
+        near.append(boundary_distance(basis, ref, cfg, got["configuration"][i], uni[i], L))
+    assert len(near) <= 1 and all(d < 1e-4 for d in near), near

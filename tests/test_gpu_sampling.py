@@ -6,7 +6,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import to_product_state, oracle_cfg
+from conftest import to_product_state, oracle_cfg, boundary_distance
 from oracle.model import Lattice, embed_identity_pepo, random_state
 from oracle import born as ob
 from oracle import boundary as obm
@@ -34,21 +34,6 @@ def test_gemm_selftest(spp, gpu_ctx, precision, tol, shape, layout):
     ref = A @ B
     err = np.max(np.abs(C - ref)) / np.max(np.abs(ref))
     assert err < tol, err
-
-
-def boundary_distance(basis, ref, cfg_ref, cfg_got, u, L):
-    """Distance of the uniform from the oracle's cumulative-probability
-    boundary at the first site (row-major) where two draws differ: with two
-    outcomes, ``_categorical`` (src/resample/selection.jl:68-80) picks the
-    first when u < p_first, so a draw can flip only if u lies within the
-    conditional's error of p_first."""
-    for j in range(L.Lx * L.Ly):
-        x, y = j % L.Lx, j // L.Lx
-        if cfg_ref[x][y] != cfg_got[x][y]:
-            c = ref["conditionals"][j]
-            p_first = c if basis.index(cfg_ref[x][y]) == 0 else 1.0 - c
-            return abs(u[j] - p_first)
-    raise AssertionError("draws do not differ")
 
 
 def _compare_draws(spp, ctx, pepo, chi, nsamp, seed, tol_logp, tol_cond, precision="fp64", check_norm=True):
