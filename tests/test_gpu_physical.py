@@ -98,13 +98,15 @@ def _compare_repairs(spp, cache, got, ref_results, exact):
 
 
 @pytest.mark.parametrize("symmetric,D,chi", [(True, (1, 1), 16), (True, (1, 1), 2), (True, (2, 1), 3),
-                                             (False, 2, 16), (False, 3, 3)])
+                                             (False, 2, 16), (False, 3, 3), (True, (2, 2), 16)])
 def test_physical_draws_match_oracle(spp, gpu_ctx, symmetric, D, chi):
     """Physical-kind draws of a 3x3 PEPS with mixed auxiliary charges (a
     projected particle's legs) at exact and truncating chi: configurations,
     log p, conditionals, truncation and the per-site PSD reports against the
     oracle on the same uniforms; the truncating cases have indefinite site
-    metrics, so the clipping changes the conditionals."""
+    metrics, so the clipping changes the conditionals. D=(2, 2) at exact chi:
+    site metrics of large null space (the QL iteration's norm-relative
+    deflation)."""
     L = Lattice(3, 3)
     aux = [1, 0, 0, 1, 1, 0, 1, 0, 1]
     st = charged_peps(L, D, 21, aux, symmetric)
