@@ -22,6 +22,7 @@
 // reference's num/den up to rounding.
 #include "energy.h"
 #include "boundary_ops.h"
+#include "kron.h"
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -207,6 +208,50 @@ struct EnergyRun {
 
   EnergyRun(Context& cx, int lx, int ly, int chi_) : ctx(cx), Lx(lx), Ly(ly), ns(lx * ly), chi(chi_) {}
 
+  // Single-layer tables of the plain (projected) layers, FP32 caches that carry
+  // them (kron.h): the layout per site and the table pointer of every sample,
+  // ktab[site * nb + s]. The plain-layer products of the row and slab terms run
+  // through them; impurity layers (an operator on the ket) keep the GEMM.
+  const KronSite* ksite = nullptr;
+  void** ktab = nullptr;
+  bool kron_plain(int kind) const { return kind == kPlain && ksite != nullptr && ktab != nullptr; }
+  static long long fold(const Leg& l) { return std::max(l.even, l.dim - l.even); }
+  // rows of (l, R, f) with the contracted pair leg c1 at stride sx1 and the folded c2 contiguous
+  static KronRows rows3(const Leg& l, const Leg& R, const Leg* f, long long sx1) {
+    KronRows kr;
+    const int nf = f ? f->dim : 1;
+    kr.nrows = (long long)l.dim * R.dim * nf;
+    kr.nf = nf;
+    kr.ef = f ? f->even : 1;
+    kr.fsplit = f ? f->split : nullptr;
+    kr.rc = R.dim;
+    kr.lsplit = l.split;
+    kr.leven = l.even;
+    kr.rsplit = R.split;
+    kr.reven = R.even;
+    kr.sx1 = sx1;
+    kr.sx2 = 1;
+    return kr;
+  }
+  void kron_apply(int stage, int site, const TRef& X, const TRef& Y, KronRows kr) {
+    if constexpr (sizeof(T) != sizeof(float)) {
+      throw HipError("factorised layer products need an FP32 cache");
+    } else {
+      kr.xs = X.sstride;
+      kr.ys = Y.sstride;
+      const KronSite& ks = ksite[site];
+      const double fl =
+          2.0 * cur * (ctx.kron == 2 ? kron_mfma_row_macs(ks, stage) : kron_row_macs(ks, stage)) * kr.nrows;
+      timed_kron(ctx, fl, [&] {
+        return launch_kron_rows(ks, stage, reinterpret_cast<const float*>(X.base) + X.offset,
+                                reinterpret_cast<float*>(Y.base) + Y.offset, kr, ktab + (size_t)site * nb, cur,
+                                ctx.kron_rows, ctx.stream, ctx.kron_groups, nullptr, ctx.kron == 2);
+      });
+      ctx.counters.kron_launches += 1;
+      ctx.counters.gemm_flops += fl;
+    }
+  }
+
   void* const* table(int kind, int site) const { return tables + ((long long)kind * ns + site) * nb; }
   TRef layer(int kind, int site, const std::string& letters) const {
     const LayerSpec& sp = spec[kind][site];
@@ -261,7 +306,15 @@ struct EnergyRun {
       contract<T>(ctx, Xr, N.ref(x, "anc"), Rref(x + 1), cur);
       std::vector<Leg> yl = {N.bond(x), Rl[x + 1][2], sp.w, sp.s};
       TRef Yr = folded_ref(BY, count(yl), "adws", yl);
-      contract<T>(ctx, Yr, Xr, layer(kPlain, site, "wnes"), cur);
+      if (kron_plain(kPlain)) {
+        // rows (a, d), contract (n, e) into (w, s): role assignment 0
+        KronRows kr = rows3(xl[0], xl[1], nullptr, fold(xl[3]));
+        kr.sx_slab = (long long)xl[2].dim * fold(xl[3]);
+        kr.sxf = 0;
+        kron_apply(0, site, Xr, Yr, kr);
+      } else {
+        contract<T>(ctx, Yr, Xr, layer(kPlain, site, "wnes"), cur);
+      }
       TRef Rr = dense_ref(R[x], count(Rl[x]), "awb", Rl[x]);
       contract<T>(ctx, Rr, Yr, bottom_ref(S, Lx, x, "bsd", true), cur);
       normalize(Rr, site);
@@ -285,7 +338,16 @@ struct EnergyRun {
       const LayerSpec& q = spec[kind][x + y * Lx];
       std::vector<Leg> yl = {Xr.leg[1], N.bond(x + 1), q.e, q.s};
       TRef Yr = folded_ref(BY, count(yl), "bces", yl);
-      contract<T>(ctx, Yr, Xr, layer(kind, x + y * Lx, "wnes"), cur);
+      if (kron_plain(kind)) {
+        // X[w][b][c][n]: rows (b, c), contract (w, n) into (e, s): role assignment 2
+        const long long fn = fold(Xr.leg[3]);
+        KronRows kr = rows3(Xr.leg[1], Xr.leg[2], nullptr, (long long)Xr.leg[1].dim * Xr.leg[2].dim * fn);
+        kr.sx_slab = fn;
+        kr.sxf = 0;
+        kron_apply(2, x + y * Lx, Xr, Yr, kr);
+      } else {
+        contract<T>(ctx, Yr, Xr, layer(kind, x + y * Lx, "wnes"), cur);
+      }
       std::vector<Leg> ol = {N.bond(x + 1), q.e, south_bond(S, Lx, x + 1)};
       TRef Or = dense_ref(outbuf, count(ol), "ced", ol);
       contract<T>(ctx, Or, Yr, bottom_ref(S, Lx, x, "bsd", true), cur);
@@ -347,7 +409,16 @@ struct EnergyRun {
       contract<T>(ctx, Yr, Xr, layer(kPlain, s1(x), "wnek"), cur);
       std::vector<Leg> zl = {N.bond(x), u.w, d, l.w, l.s};
       TRef Zr = folded_ref(BX, count(zl), "awdvs", zl);
-      contract<T>(ctx, Zr, Yr, layer(kPlain, s2(x), "vkfs"), cur);
+      if (kron_plain(kPlain)) {
+        // Y[a][w][d][k][f]: rows (a, w, d), contract (k, f) into (v, s): role assignment 0
+        const long long ff = fold(yl[4]);
+        KronRows kr = rows3(yl[0], yl[1], &yl[2], ff);
+        kr.sxf = (long long)yl[3].dim * ff;
+        kr.sx_slab = (long long)yl[2].dim * kr.sxf;
+        kron_apply(0, s2(x), Yr, Zr, kr);
+      } else {
+        contract<T>(ctx, Zr, Yr, layer(kPlain, s2(x), "vkfs"), cur);
+      }
       TRef Rr = dense_ref(R[x], count(Rl[x]), "awvb", Rl[x]);
       contract<T>(ctx, Rr, Zr, bottom_ref(S, Lx, x, "bsd", true), cur);
       normalize(Rr, s1(x));
@@ -367,10 +438,28 @@ struct EnergyRun {
       contract<T>(ctx, Xr, L, N.ref(x, "anc"), cur);
       std::vector<Leg> yl = {L.leg[2], b, c, u.e, u.s};
       TRef Yr = folded_ref(BY, count(yl), "vbcek", yl);
-      contract<T>(ctx, Yr, Xr, layer(ku, s1(x), "wnek"), cur);
+      if (kron_plain(ku)) {
+        // X[w][v][b][c][n]: rows (v, b, c), contract (w, n) into (e, k): role assignment 2
+        const long long fn = fold(xl[4]);
+        KronRows kr = rows3(xl[1], xl[2], &xl[3], (long long)xl[1].dim * xl[2].dim * xl[3].dim * fn);
+        kr.sxf = fn;
+        kr.sx_slab = (long long)xl[3].dim * fn;
+        kron_apply(2, s1(x), Xr, Yr, kr);
+      } else {
+        contract<T>(ctx, Yr, Xr, layer(ku, s1(x), "wnek"), cur);
+      }
       std::vector<Leg> zl = {b, c, u.e, l.e, l.s};
       TRef Zr = folded_ref(BX, count(zl), "bcefs", zl);
-      contract<T>(ctx, Zr, Yr, layer(kl, s2(x), "vkfs"), cur);
+      if (kron_plain(kl)) {
+        // Y[v][b][c][e][k]: rows (b, c, e), contract (v, k) into (f, s): role assignment 2
+        const long long fk = fold(yl[4]);
+        KronRows kr = rows3(yl[1], yl[2], &yl[3], (long long)yl[1].dim * yl[2].dim * yl[3].dim * fk);
+        kr.sxf = fk;
+        kr.sx_slab = (long long)yl[3].dim * fk;
+        kron_apply(2, s2(x), Yr, Zr, kr);
+      } else {
+        contract<T>(ctx, Zr, Yr, layer(kl, s2(x), "vkfs"), cur);
+      }
       std::vector<Leg> ol = {c, u.e, l.e, south_bond(S, Lx, x + 1)};
       TRef Or = dense_ref(outbuf, count(ol), "cefd", ol);
       contract<T>(ctx, Or, Zr, bottom_ref(S, Lx, x, "bsd", true), cur);
@@ -543,6 +632,8 @@ void measure_energy_batch(Context& ctx, const StateDev& pepo, const CacheDev<T>&
       dkbase = (void**)dev(sizeof(void*) * kbase.size());
       dktab = (void**)dev(sizeof(void*) * (size_t)ns * nb);
       upload(dkbase, kbase.data(), sizeof(void*) * kbase.size(), st);
+      run.ksite = c.kron_site.data();
+      run.ktab = dktab;
     }
     auto kron_row = [&](RowLayers<T>& r, int slot, int site) {
       if (!kron) return;

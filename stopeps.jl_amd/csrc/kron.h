@@ -66,6 +66,7 @@ int kron_layer_parity(const int d[4], const int e[4], int dphi, int ephi, const 
 struct KronRows {
   long long nrows = 0;
   int nf = 1, ef = 1, rc = 1;
+  const int* fsplit = nullptr;  // even count of f per sample (else ef)
   const int* lsplit = nullptr;
   int leven = 1;
   const int* rsplit = nullptr;

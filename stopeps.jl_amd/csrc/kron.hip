@@ -221,6 +221,7 @@ struct KronArgs {
   int dphi, ephi;
   long long nrows;
   int nf, ef, rc;
+  const int* fsplit;
   const int* lsplit;
   int leven;
   const int* rsplit;
@@ -367,7 +368,7 @@ __global__ void __launch_bounds__(kron_max_threads(NO1)) kron_rows_kernel(KronAr
     const long long slab = r / a.nf;
     const int f = (int)(r - slab * a.nf);
     const int l = (int)(slab / a.rc), R = (int)(slab - (long long)l * a.rc);
-    rho[tid] = ((l >= el) + (R >= er) + (f >= a.ef)) & 1;
+    rho[tid] = ((l >= el) + (R >= er) + (f >= (a.fsplit ? a.fsplit[s] : a.ef))) & 1;
     xrow0[tid] = (int)(slab * a.sx_slab + f * a.sxf);
   }
   __syncthreads();
@@ -629,7 +630,7 @@ __global__ void __launch_bounds__(kMfmaMaxThreads) kron_mfma_kernel(KronArgs a) 
       const long long slab = r / a.nf;
       const int f = (int)(r - slab * a.nf);
       const int l = (int)(slab / a.rc), Rr = (int)(slab - (long long)l * a.rc);
-      rho[tid] = ((l >= el) + (Rr >= er) + (f >= a.ef)) & 1;
+      rho[tid] = ((l >= el) + (Rr >= er) + (f >= (a.fsplit ? a.fsplit[s] : a.ef))) & 1;
       xrow0[tid] = (int)(slab * a.sx_slab + f * a.sxf);
     }
     __syncthreads();
@@ -875,6 +876,7 @@ hipError_t launch_kron_rows(const KronSite& ks, int stage, const float* x, float
   a.nf = rw.nf;
   a.ef = rw.ef;
   a.rc = rw.rc;
+  a.fsplit = rw.fsplit;
   a.lsplit = rw.lsplit;
   a.leven = rw.leven;
   a.rsplit = rw.rsplit;
