@@ -245,27 +245,31 @@ def test_kron_draws_match_oracle(spp):
 @pytest.mark.parametrize("D,chi", [(3, 8)])
 def test_kron_energies_match_dense_route(spp, D, chi):
     """FP32 energies of projected samples with the sweeps' absorbs (north, and
-    south in the rotated frame) through the single-layer tables against the
-    dense double layers: energies, log norms and the recorded boundary
-    entropies to FP32 rounding."""
+    south in the rotated frame) and the row and slab terms' plain-layer
+    products through the single-layer tables (vector and matrix-core kernels)
+    against the dense double layers: energies, log norms and the recorded
+    boundary entropies to FP32 rounding."""
     pepo = spp.synthetic_thermal_pepo(spp.Lattice(4, 4), D=(D - D // 2, D // 2), eps=0.3, seed=13)
     rng = np.random.default_rng(5)
     out = {}
-    for kron in (1, 0):
+    for kron in (1, 2, 0):
         ctx = spp.Context(0, "fp32", kron=kron, graphs=0)
         try:
             cache = spp.SamplingCache(ctx, pepo, chi)
-            cfgs = np.asarray(cache.basis.labels)[rng.integers(0, 2, size=(6, 4, 4))] if kron else out[1][2]
+            cfgs = np.asarray(cache.basis.labels)[rng.integers(0, 2, size=(6, 4, 4))] if kron == 1 else out[1][2]
             ctx.reset_counters()
             got = spp.measure_energies(cache, cfgs, spp.TFIM(1.0, 2.0, symmetric=True), chi=chi, record_entropy=True)
             out[kron] = (got, ctx.counters()["kron_launches"], cfgs)
         finally:
             ctx.close()
-    assert out[1][1] > 0 and out[0][1] == 0
-    a, b = out[1][0], out[0][0]
-    assert np.all(a["status"] == 0) and np.all(b["status"] == 0)
-    for k in ("energy_total", "energy_per_site", "log_norm", "tensor_log_norm"):
-        np.testing.assert_allclose(a[k], b[k], rtol=2e-5, atol=2e-5, err_msg=k)
-    for i in range(len(a["Se"])):
-        for side in ("north", "south"):
-            np.testing.assert_allclose(a["Se"][i][side], b["Se"][i][side], atol=2e-4)
+    assert out[1][1] > 0 and out[2][1] > 0 and out[0][1] == 0
+    b = out[0][0]
+    assert np.all(b["status"] == 0)
+    for kernel in (1, 2):
+        a = out[kernel][0]
+        assert np.all(a["status"] == 0)
+        for k in ("energy_total", "energy_per_site", "log_norm", "tensor_log_norm"):
+            np.testing.assert_allclose(a[k], b[k], rtol=2e-5, atol=2e-5, err_msg=k)
+        for i in range(len(a["Se"])):
+            for side in ("north", "south"):
+                np.testing.assert_allclose(a["Se"][i][side], b["Se"][i][side], atol=2e-4)
