@@ -234,7 +234,7 @@ struct KronArgs {
   const void* tab_shared;  // tables of every sample when tabs is null
   long long stage_off;
   int rows, groups, rowmax, w1, w2, j1cap, j2cap, ntup1;
-  float inv_rowx, inv_pf1, inv_rowy;  // 1 / (pd[0] * pf[1]), 1 / pf[1], 1 / rowY
+  float inv_rowx, inv_pf1;  // 1 / (pd[0] * pf[1]), 1 / pf[1]
   int sx1i, sx2i;                     // sx1, sx2 (a sample's X has fewer than 2^31 elements)
   int xflat;                          // X rows are packed and consecutive in global memory
   float inv_pd0;                      // 1 / pd[0]
@@ -242,7 +242,6 @@ struct KronArgs {
   int tno1, tno2;           // row widths of the K1 / K2 tables
   int nofmax, ncol1, ts;    // outputs per pass, stage-1 columns (dphi * nofmax), T row stride (odd)
   int wc;                   // waves per class
-  int kt1, nt1, kt2, nt2;   // K steps and N tiles of the two products
   int n1max, n2max, m1cap, m2cap;
 };
 
@@ -911,7 +910,6 @@ hipError_t launch_kron_rows(const KronSite& ks, int stage, const float* x, float
   };
   a.inv_rowx = 1.0f / (float)(a.pd[0] * a.pf[1]);
   a.inv_pf1 = 1.0f / (float)a.pf[1];
-  a.inv_rowy = 1.0f / (float)a.rowY;
   // 32-bit element offsets within a sample's X
   if (rw.sx_slab * ((rw.nrows + rw.nf - 1) / rw.nf) >= (1LL << 31) || rw.sx1 >= (1LL << 31) || rw.sx2 >= (1LL << 31))
     return hipErrorInvalidValue;
@@ -930,10 +928,6 @@ hipError_t launch_kron_rows(const KronSite& ks, int stage, const float* x, float
       a.nofmax = m.n2max;
       a.ncol1 = ks.dphi * m.n2max;
       a.ts = a.ncol1 | 1;
-      a.kt1 = m.kt1;
-      a.nt1 = m.nt1;
-      a.kt2 = m.kt2;
-      a.nt2 = m.nt2;
       a.n1max = m.n1max;
       a.n2max = m.n2max;
       int R = rows_per_block > 0 ? std::min(rows_per_block, 32) : 8;
