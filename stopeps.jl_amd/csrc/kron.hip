@@ -384,16 +384,26 @@ __global__ void __launch_bounds__(kron_max_threads(NO1)) kron_rows_kernel(KronAr
   if (s1) {
     int t = (wave - c1 * a.w1) * 64 + lane;
     const int n0 = tab->n12[0], n1 = tab->n12[1];
-    for (int rr = 0; rr < nr; ++rr) {
-      const int pi = c1 ^ rho[rr];
-      const int n = pi ? n1 : n0;
-      if (t < n) {
+    if (n0 == n1) {
+      // every row holds n0 tuples of the class: the lane's row is a quotient
+      const int rr = n0 > 0 ? t / n0 : nr;
+      if (rr < nr) {
         r1 = rr;
-        tup1 = tab->p12[pi][t];
+        tup1 = tab->p12[c1 ^ rho[rr]][t - rr * n0];
         act1 = true;
-        break;
       }
-      t -= n;
+    } else {
+      for (int rr = 0; rr < nr; ++rr) {
+        const int pi = c1 ^ rho[rr];
+        const int n = pi ? n1 : n0;
+        if (t < n) {
+          r1 = rr;
+          tup1 = tab->p12[pi][t];
+          act1 = true;
+          break;
+        }
+        t -= n;
+      }
     }
   }
   // stage-2 lane: a ket tuple (o1k, o2k) of a row, class c2 = par(o1k, o2k) ^ rho;
@@ -405,17 +415,27 @@ __global__ void __launch_bounds__(kron_max_threads(NO1)) kron_rows_kernel(KronAr
   if (s2) {
     int t = (wave - c2 * a.w2) * 64 + lane;
     const int n0 = tab->no[0], n1 = tab->no[1];
-    for (int rr = 0; rr < nr; ++rr) {
-      const int pi = c2 ^ rho[rr];
-      const int n = pi ? n1 : n0;
-      if (t < n) {
+    if (n0 == n1) {
+      const int rr = n0 > 0 ? t / n0 : nr;
+      if (rr < nr) {
         r2 = rr;
-        tup2 = tab->po[pi][t];
-        t2 = t;
+        t2 = t - rr * n0;
+        tup2 = tab->po[c2 ^ rho[rr]][t2];
         act2 = true;
-        break;
       }
-      t -= n;
+    } else {
+      for (int rr = 0; rr < nr; ++rr) {
+        const int pi = c2 ^ rho[rr];
+        const int n = pi ? n1 : n0;
+        if (t < n) {
+          r2 = rr;
+          tup2 = tab->po[pi][t];
+          t2 = t;
+          act2 = true;
+          break;
+        }
+        t -= n;
+      }
     }
   }
   float acc2[NO2];
