@@ -19,6 +19,7 @@
 #include "kron.h"
 #include "engine.h"
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 
 namespace spp {
@@ -285,9 +286,26 @@ __device__ __forceinline__ void kron_stage_x(const StageX a, const float* xs, in
   const int* xrow0 = kron_xrow0;
   const int rowX = a.pd0 * a.pf1;
   if (a.xflat) {
-    // consecutive rows are contiguous in global memory
+    // consecutive rows are contiguous in global memory: 16-byte copies where aligned
     const float* src = xs + xrow0[0];
     const int total = nr * rowX;
+    if ((reinterpret_cast<uintptr_t>(src) & 15) == 0 && (total & 3) == 0) {
+      constexpr int PF4 = PF >= 4 ? PF / 4 : 1;
+      const int t4 = total >> 2;
+      const float4* s4 = reinterpret_cast<const float4*>(src);
+      float4* d4 = reinterpret_cast<float4*>(Xl);
+      for (int base = 0; base < t4; base += PF4 * nthr) {
+        float4 v[PF4];
+#pragma unroll
+        for (int u = 0; u < PF4; ++u) v[u] = s4[min(base + u * nthr + tid, t4 - 1)];  // clamped: stored only in range
+#pragma unroll
+        for (int u = 0; u < PF4; ++u) {
+          const int i = base + u * nthr + tid;
+          if (i < t4) d4[i] = v[u];
+        }
+      }
+      return;
+    }
     for (int base = 0; base < total; base += PF * nthr) {
       float v[PF];
 #pragma unroll
@@ -338,6 +356,18 @@ __device__ __forceinline__ void kron_stage_x(const StageX a, const float* xs, in
       }
     }
   }
+}
+
+// The block's Y rows from LDS to global memory, 16 bytes at a time where the
+// destination is aligned.
+__device__ __forceinline__ void kron_copy_out(float* ys, const float* Xl, int totY, int tid, int nthr) {
+  if ((reinterpret_cast<uintptr_t>(ys) & 15) == 0 && (totY & 3) == 0) {
+    const float4* s4 = reinterpret_cast<const float4*>(Xl);
+    float4* d4 = reinterpret_cast<float4*>(ys);
+    for (int i = tid; i < (totY >> 2); i += nthr) d4[i] = s4[i];
+    return;
+  }
+  for (int i = tid; i < totY; i += nthr) ys[i] = Xl[i];
 }
 
 // Threads per block: the wide register tiles keep 512 threads so that their
@@ -526,7 +556,7 @@ __global__ void __launch_bounds__(kron_max_threads(NO1)) kron_rows_kernel(KronAr
   __syncthreads();
   float* ys = a.y + (long long)s * a.ys + row0 * a.rowY;
   const int totY = nr * (int)a.rowY;
-  for (int i = tid; i < totY; i += nthr) ys[i] = Xl[i];
+  kron_copy_out(ys, Xl, totY, tid, nthr);
 }
 
 // ---------------------------------------------------------------------------
@@ -773,7 +803,7 @@ __global__ void __launch_bounds__(kMfmaMaxThreads) kron_mfma_kernel(KronArgs a) 
     __syncthreads();
     float* ys = a.y + (long long)s * a.ys + row0 * a.rowY;
     const int totY = nr * rowY;
-    for (int i = tid; i < totY; i += nthr) ys[i] = Xl[i];
+    kron_copy_out(ys, Xl, totY, tid, nthr);
     __syncthreads();
   }
 }
