@@ -498,7 +498,7 @@ __global__ void __launch_bounds__(kron_max_threads(NO1)) kron_rows_kernel(KronAr
         for (int i1 = 0; i1 < n1; ++i1) {
           const float* xp = xg + i1 * M1;
           for (int i2 = 0; i2 < n2; ++i2) {
-            const float xv = act1 ? xp[i2 * mul2] : 0.f;
+            const float xv = xp[i2 * mul2];  // lanes without a tuple read row 0, never stored
 #pragma unroll
             for (int o = 0; o < NO1; ++o) acc[o] = fmaf(xv, kj[o], acc[o]);
             kj += NO1;
@@ -520,7 +520,7 @@ __global__ void __launch_bounds__(kron_max_threads(NO1)) kron_rows_kernel(KronAr
       const int nj = tab->nj2[c2][f];
       const int* jl = tab->j2[c2][f];
       for (int j = 0; j < nj; ++j) {
-        const float tv = act2 ? trow[jl[j] * kNO1P] : 0.f;
+        const float tv = trow[jl[j] * kNO1P];  // lanes without a tuple read row 0, never stored
 #pragma unroll
         for (int o = 0; o < NO2; ++o) acc2[o] = fmaf(tv, kj[o], acc2[o]);
         kj += NO2;
