@@ -497,6 +497,7 @@ __global__ void __launch_bounds__(kron_max_threads(NO1)) kron_rows_kernel(KronAr
         const float* xg = xrow + (g ? b11 : b10) * a.pf[1] + (p2k ? b21 : b20);
         for (int i1 = 0; i1 < n1; ++i1) {
           const float* xp = xg + i1 * M1;
+#pragma unroll 2  // two tuples' coefficient loads per wait
           for (int i2 = 0; i2 < n2; ++i2) {
             const float xv = xp[i2 * mul2];  // lanes without a tuple read row 0, never stored
 #pragma unroll
